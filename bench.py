@@ -1,0 +1,148 @@
+"""Headline benchmark: oblivious bitonic sort of 2^28 (u64 key, u32 payload)
+records on one MI355X, through the library's C ABI.
+
+A step sorts one fresh batch: the device generator writes 2^28 new keys
+(SplitMix64 stream of seed = step) and row-index payloads, then the
+oblivious sort runs.  Both are inside the timed region; inputs never cross
+PCIe.  Prints one JSON line (rank 0) with the roofline of the dominant
+kernel family (HIP events around every launch, on the library's stream) and
+a CPU baseline (the oracle's single-thread bitonic network on a bounded
+sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "oblivious sort Mrec/s at 2^28 (%HBM peak); served QPS + p50/p99 vs plain"
+REC_BYTES = 12  # u64 key + u32 payload, structure of arrays
+
+
+def cpu_baseline(budget_s=12.0):
+    """Oracle bitonic network (reference host path semantics, one core) on
+    2^20-record samples of the same workload, repeated for ~budget_s."""
+    import numpy as np
+
+    import oracle
+    n = 1 << 20
+    k, p = oracle.gen_sort_input(1, n)
+    oracle.bitonic_sort_kp(k, p, n, n)  # warm-up
+    times = []
+    t_end = time.time() + budget_s
+    while time.time() < t_end or len(times) < 3:
+        t0 = time.perf_counter()
+        oracle.bitonic_sort_kp(k, p, n, n)
+        times.append(time.perf_counter() - t0)
+        if len(times) >= 50:
+            break
+    med = float(np.median(times))
+    rate = n / med / 1e6
+    return {"value": round(rate, 3), "unit": "Mrec/s", "cores": 1, "kind": "port",
+            "sample": f"oracle single-thread bitonic network (reference (k,j,i) schedule, branch-free "
+                      f"cmov) on 2^20 records, median of {len(times)} runs; per-record work grows with "
+                      f"log2N(log2N+1)/2, so at 2^28 the same code would run at ~{rate * 210 / 406:.2f} Mrec/s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tuning", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the oblivious sort is not sharded across GPUs in this build "
+              "(see DESIGN.md, 'How the path would shard'); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import velodb_amd
+    from velodb_amd import _abi
+
+    n = 1 << args.log2n
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    keys = torch.empty(n, dtype=torch.int64, device=dev)
+    pay = torch.empty(n, dtype=torch.int32, device=dev)
+    ctx = velodb_amd.Context(stream=stream.cuda_stream, flags=_abi.CFG_KERNEL_TIMING, tuning=args.tuning)
+
+    def step(i):
+        ctx.gen_sort_input(1000 + i, keys, pay, n)
+        ctx.sort_kp_async(keys, pay, n, n)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    ctx.kernel_stats_reset()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(args.warmup + i)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+
+    total_s = t1 - t0
+    ms_per_step = total_s * 1e3 / args.steps
+    value = n * args.steps / total_s / 1e6
+
+    # per-family kernel time inside the timed region
+    fams = {}
+    for f in ["sort_local", "sort_global", "sort_merge", "gen_sort"]:
+        cnt, ms = ctx.kernel_stats(f)
+        if cnt:
+            fams[f] = (cnt, ms)
+    sort_fams = {f: v for f, v in fams.items() if f.startswith("sort_")}
+    dom = max(sort_fams, key=lambda f: sort_fams[f][1])
+    cnt, ms = sort_fams[dom]
+    avg_ms = ms / cnt
+    alg_bytes = 2 * REC_BYTES * n  # every record read and written once per pass
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    passes = ctx.sort_plan(n)
+    sort_ms = sum(v[1] for v in sort_fams.values()) / args.steps
+    per_pass = {f: round(alg_bytes / (v[1] / v[0] * 1e-3) / 1e9, 1) for f, v in sort_fams.items()}
+
+    # sanity: the last step's output is sorted (cheap device-side check)
+    k_last = keys.view(torch.int64)
+    ok = bool(torch.all((k_last[1:] ^ (-(2 ** 63))) >= (k_last[:-1] ^ (-(2 ** 63)))).item())
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrec/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (SplitMix64 keys from seed = step, payload = row index, generated on device)",
+        "config": {"workload": f"oblivious bitonic sort of 2^{args.log2n} records (u64 key, u32 payload), "
+                               f"ascending by key, ties by payload; device key generation inside each step",
+                   "n": n, "record_bytes": REC_BYTES, "layout": "SoA", "hbm_passes": passes,
+                   "sort_ms_per_step": round(sort_ms, 3), "output_sorted": ok},
+        "pct_hbm_peak": round(100 * achieved / HBM_PEAK_GBS, 1),
+        "per_pass_GBps": per_pass,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": alg_bytes},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,375 @@
+/*
+ * vo_api.h -- C ABI of the MI355X (gfx950) oblivious-operator library.
+ *
+ * This is the drop-in boundary for VelODB's device runtime: the
+ * TaskManager / Command interface that the reference's operators call
+ * (include/cuda/task_manager.hpp:12-30, include/cuda/commands.hpp:16-201).
+ * Everything crossing it is plain C: pointers, sizes, POD structs.
+ *
+ * Ownership (as in the reference, src/operator/sort_operator.cpp:54-56):
+ * callers own every device buffer they pass.  Buffers named "n_pad" long
+ * must hold at least n_pad elements; the library writes the pad region
+ * [n, n_pad) exactly as include/cuda/sort.cuh:79-100 does.  Temporary device
+ * memory (scan partials, union arrays, trace slots) comes from a scratch
+ * arena owned by the context and grown on demand.
+ *
+ * Errors: every call returns a vo_status (0 = VO_OK).  vo_last_error()
+ * gives the message of the context's last failure.  Device faults surface
+ * through the stream status, as src/cuda/commands.cpp:78-82 does, and waits
+ * carry a timeout (the reference waits forever).
+ *
+ * Threading: vo_submit is safe to call from several host threads on one
+ * context (the ring is guarded; the reference ring is single-producer,
+ * src/cuda/commands.cpp:47-67).  Commands complete in submission order on one
+ * context; separate contexts run concurrently on separate HIP streams.
+ */
+#ifndef VO_API_H
+#define VO_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(VO_BUILD)
+#define VO_EXPORT __attribute__((visibility("default")))
+#else
+#define VO_EXPORT
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VO_API_VERSION 1
+#define VO_MAX_SORT_COLUMNS 8 /* include/common/constants.hpp.in:17 */
+
+typedef enum vo_status {
+    VO_OK = 0,
+    VO_ERR_INVALID = 1,     /* bad argument (size, alignment, type id) */
+    VO_ERR_HIP = 2,         /* HIP runtime error or device fault */
+    VO_ERR_TIMEOUT = 3,     /* vo_wait / vo_queue_stop timed out */
+    VO_ERR_NOMEM = 4,       /* device or pinned allocation failed */
+    VO_ERR_UNSUPPORTED = 5, /* opcode or type the library does not implement */
+    VO_ERR_QUEUE = 6,       /* command queue not running / stopped / full */
+    VO_ERR_ADMISSION = 7    /* serving loop: request exceeds the HBM budget */
+} vo_status;
+
+/* Column type ids: the reference's DataTypeId values
+ * (include/data/data_type.hpp:12-27, device types include/data/type_traits.hpp:10-19),
+ * plus VO_T_UBIGINT for the unsigned 64-bit keys the reference lacks. */
+typedef enum vo_type {
+    VO_T_BOOLEAN = 1,  /* uint8_t  */
+    VO_T_TINYINT = 2,  /* int8_t   */
+    VO_T_SMALLINT = 3, /* int16_t  */
+    VO_T_INTEGER = 4,  /* int32_t  */
+    VO_T_BIGINT = 5,   /* int64_t  */
+    VO_T_FLOAT = 6,    /* float    */
+    VO_T_DOUBLE = 7,   /* double   */
+    VO_T_VARCHAR = 8,  /* size_t ordinal */
+    VO_T_DATE = 10,    /* uint32_t */
+    VO_T_UBIGINT = 64  /* uint64_t (new) */
+} vo_type;
+
+/* Opcodes.  Values 0..12 and 0xFFFFFFFF are the reference's OpCode
+ * (include/cuda/commands.hpp:16-31); 0x100+ are the oblivious operators. */
+typedef enum vo_opcode {
+    VO_OP_NOP = 0,
+    VO_OP_SCATTER = 1,                  /* compaction positions (stable prefix count) */
+    VO_OP_GATHER = 2,                   /* out[pos[i]] = in[i] if mask (data-dependent writes) */
+    VO_OP_SORT = 3,                     /* multi-column bitonic sort, reference semantics */
+    VO_OP_PERMUTE = 4,                  /* out[i] = in[idx[i]] (data-dependent reads) */
+    VO_OP_SORT_MERGE_JOIN_COUNT = 5,    /* unsupported: replaced by VO_OP_PKFK_JOIN_GROUPSUM */
+    VO_OP_SORT_MERGE_JOIN_WRITE = 6,    /* unsupported */
+    VO_OP_HASH_JOIN_BUILD = 7,          /* unsupported: non-oblivious on the device */
+    VO_OP_HASH_JOIN_COUNT = 8,          /* unsupported */
+    VO_OP_HASH_JOIN_WRITE = 9,          /* unsupported */
+    VO_OP_COMPARE_EQ = 10,              /* out_mask[i] = a[i] == b[i] */
+    VO_OP_COMPOSE_POSITION_MAP = 11,    /* new_map[i] = old_map[sigma[i]] */
+    VO_OP_ENRICH_POSITIONS = 12,        /* positions[i] = map[record_ids[i]] */
+    VO_OP_SORT_KP = 0x100,              /* (u64 key, u32 payload) oblivious sort */
+    VO_OP_COMPACT = 0x101,              /* ORCompact routing-network compaction */
+    VO_OP_SHUFFLE = 0x102,              /* Philox-tag oblivious shuffle */
+    VO_OP_FILTER_SUM = 0x103,           /* predicate + compaction + int128 SUM */
+    VO_OP_PKFK_JOIN_GROUPSUM = 0x104,   /* PK/FK join + GROUP BY SUM */
+    VO_OP_SORT_BATCH = 0x105,           /* many independent (u64,u32) sorts */
+    VO_OP_TERMINATE = 0xFFFFFFFFu
+} vo_opcode;
+
+/* ------------------------------------------------------------------ args */
+
+/* OP_SORT: mirror of CommandArgs::SortArgs (include/cuda/commands.hpp:64-72).
+ * Sorts the columns in place over n_padded_rows rows (pad rows filled with
+ * max for ASC / lowest for DESC), and writes indices[i] = original row. */
+typedef struct vo_sort_args {
+    void *sort_cols[VO_MAX_SORT_COLUMNS];
+    int64_t *indices;
+    uint64_t n_sort_columns;
+    uint64_t n_rows;
+    uint64_t n_padded_rows;
+    uint32_t col_types[VO_MAX_SORT_COLUMNS];
+    uint8_t ascending_flags[VO_MAX_SORT_COLUMNS];
+} vo_sort_args;
+
+/* OP_SORT_KP: records (keys[i], payload[i]); ascending by key, ties by
+ * payload.  Rows [n, n_pad) are set to (UINT64_MAX, i) first. */
+typedef struct vo_sort_kp_args {
+    uint64_t *keys;
+    uint32_t *payload;
+    uint64_t n;
+    uint64_t n_pad;
+} vo_sort_kp_args;
+
+/* OP_PERMUTE: mirror of CommandArgs::PermuteArgs (commands.hpp:74-80). */
+typedef struct vo_permute_args {
+    void *out_data;
+    const void *in_data;
+    const int64_t *in_indices;
+    uint64_t n;
+    uint32_t type_id;
+} vo_permute_args;
+
+/* OP_SCATTER: mirror of CommandArgs::ScatterArgs (commands.hpp:48-53).
+ * out_indices[i] = number of selected rows before i (for selected i);
+ * *out_count = total.  Stable, unlike the reference's per-tile atomicAdd. */
+typedef struct vo_scatter_args {
+    int32_t *out_indices;
+    uint64_t *out_count;
+    const uint8_t *in_mask;
+    uint64_t n;
+} vo_scatter_args;
+
+/* OP_GATHER: mirror of CommandArgs::GatherArgs (commands.hpp:55-62). */
+typedef struct vo_gather_args {
+    void *out_data;
+    const void *in_data;
+    const int32_t *in_indices;
+    const uint8_t *in_mask;
+    uint64_t n;
+    uint32_t type_id;
+} vo_gather_args;
+
+/* OP_COMPARE_EQ: mirror of CommandArgs::CompareEqArgs (commands.hpp:129-135). */
+typedef struct vo_compare_eq_args {
+    const void *left_data;
+    const void *right_data;
+    uint8_t *out_mask;
+    uint64_t n;
+    uint32_t type_id;
+} vo_compare_eq_args;
+
+/* OP_COMPOSE_POSITION_MAP / OP_ENRICH_POSITIONS (commands.hpp:137-149). */
+typedef struct vo_compose_args {
+    const uint32_t *old_map;
+    const uint32_t *sigma;
+    uint32_t *new_map;
+    uint64_t n;
+} vo_compose_args;
+
+typedef struct vo_enrich_args {
+    const uint32_t *record_ids;
+    const uint32_t *position_map;
+    uint32_t *positions;
+    uint64_t n;
+} vo_enrich_args;
+
+/* OP_COMPACT: order-preserving oblivious compaction of up to 8 columns of
+ * n_pad rows in place (rows with mask[i] != 0 move to the front in order;
+ * rows >= n count as unselected).  Every row is read and written at every
+ * level of the routing network regardless of the mask.  *out_count (device
+ * pointer, may be NULL) receives the number of selected rows. */
+typedef struct vo_compact_args {
+    const uint8_t *mask;
+    void *cols[VO_MAX_SORT_COLUMNS];
+    uint32_t col_bytes[VO_MAX_SORT_COLUMNS]; /* 1, 2, 4 or 8 */
+    uint32_t n_cols;
+    uint64_t n;
+    uint64_t n_pad;
+    uint64_t *out_count;
+} vo_compact_args;
+
+/* OP_SHUFFLE: oblivious shuffle of n_pad rows: tag_i = Philox4x32-10(i, seed),
+ * sort by (tag, i) and apply the permutation to the payload column.
+ * perm_out (optional) receives the source row of each output row. */
+typedef struct vo_shuffle_args {
+    uint32_t *payload; /* n_pad rows, shuffled in place */
+    uint32_t *perm_out;
+    uint64_t n_pad;
+    uint64_t seed;
+} vo_shuffle_args;
+
+/* OP_FILTER_SUM: Q6-style predicate
+ *   day_lo <= day < day_hi && disc_lo <= disc <= disc_hi && qty < qty_lt
+ * over n rows padded to n_pad, oblivious compaction of (price, disc) into
+ * out_price / out_disc (n_pad each, may be NULL to skip writing them), and
+ * exact int128 SUM(price * disc) over the selected rows.
+ * out (device pointer, 32 bytes): { u64 count, u64 pad, u64 sum_lo, i64 sum_hi }. */
+typedef struct vo_filter_args {
+    const int32_t *day;
+    const int64_t *qty;
+    const int64_t *price;
+    const int64_t *disc;
+    int64_t *out_price;
+    int64_t *out_disc;
+    uint64_t n;
+    uint64_t n_pad;
+    int32_t day_lo, day_hi;
+    int64_t disc_lo, disc_hi;
+    int64_t qty_lt;
+    uint64_t *out;
+} vo_filter_args;
+
+/* OP_PKFK_JOIN_GROUPSUM: dimension (dim_key unique, dim_group in
+ * [0, n_groups)) joined with fact (fact_fk, fact_val) on key, then
+ * SUM(fact_val) GROUP BY dim_group.  Union of n_dim + n_fact rows padded to
+ * n_pad.  out_group / out_sum (device, max_groups each) receive the groups
+ * ascending; *out_n_groups (device) the group count; sorted_keys /
+ * sorted_vals (device, n_pad each, optional) the sorted union. */
+typedef struct vo_join_args {
+    const int32_t *dim_key;
+    const int32_t *dim_group;
+    uint64_t n_dim;
+    const int32_t *fact_fk;
+    const int64_t *fact_val;
+    uint64_t n_fact;
+    uint64_t n_pad;
+    int32_t *out_group;
+    int64_t *out_sum;
+    uint64_t *out_n_groups;
+    uint64_t max_groups;
+    uint64_t *sorted_keys;
+    uint64_t *sorted_vals;
+} vo_join_args;
+
+/* OP_SORT_BATCH: independent sorts in one schedule.  Item i sorts
+ * (keys + offset, payload + offset) over n rows padded to n_pad (a power of
+ * two; offset must be a multiple of n_pad).  Items are bucketed by n_pad. */
+typedef struct vo_batch_item {
+    uint64_t offset;
+    uint64_t n;
+    uint64_t n_pad;
+} vo_batch_item;
+
+typedef struct vo_sort_batch_args {
+    uint64_t *keys;
+    uint32_t *payload;
+    const vo_batch_item *items; /* host array */
+    uint64_t n_items;
+} vo_sort_batch_args;
+
+typedef union vo_command_args {
+    vo_sort_args sort;
+    vo_sort_kp_args sort_kp;
+    vo_permute_args permute;
+    vo_scatter_args scatter;
+    vo_gather_args gather;
+    vo_compare_eq_args compare_eq;
+    vo_compose_args compose_position_map;
+    vo_enrich_args enrich_positions;
+    vo_compact_args compact;
+    vo_shuffle_args shuffle;
+    vo_filter_args filter;
+    vo_join_args join;
+    vo_sort_batch_args sort_batch;
+} vo_command_args;
+
+/* Mirror of Command (commands.hpp:152-156). */
+typedef struct vo_command {
+    uint64_t sequence_id; /* set by vo_submit */
+    uint32_t opcode;      /* vo_opcode */
+    uint32_t flags;       /* reserved, 0 */
+    vo_command_args args;
+} vo_command;
+
+/* ------------------------------------------------------------- context */
+
+enum {
+    VO_CFG_PERSISTENT = 1u << 0,    /* run commands in the resident queue kernel */
+    VO_CFG_KERNEL_TIMING = 1u << 1, /* HIP events around every launch (vo_kernel_stats) */
+};
+
+typedef struct vo_config {
+    uint32_t queue_capacity; /* ring entries, power of two; 0 -> 1024 (task_manager.hpp:14) */
+    uint32_t flags;          /* VO_CFG_* */
+    void *stream;            /* hipStream_t to run on; NULL -> the context creates one */
+    int32_t device;          /* HIP device ordinal; -1 -> current */
+    uint32_t wait_timeout_ms;/* default timeout of vo_wait when 0 is passed; 0 -> 60000 */
+    uint32_t tuning;         /* 0 -> autotuned table / built-in defaults */
+} vo_config;
+
+typedef struct vo_ctx vo_ctx;
+
+VO_EXPORT int vo_ctx_create(const vo_config *cfg, vo_ctx **out);
+VO_EXPORT int vo_ctx_destroy(vo_ctx *ctx);
+VO_EXPORT const char *vo_last_error(const vo_ctx *ctx);
+VO_EXPORT const char *vo_status_string(int status);
+VO_EXPORT int vo_version(void);
+
+/* TaskManager::start / stop (src/cuda/task_manager.cu:26-124).  start()
+ * launches the resident queue kernel when VO_CFG_PERSISTENT is set; without
+ * it commands are launched directly on the context stream. */
+VO_EXPORT int vo_queue_start(vo_ctx *ctx);
+VO_EXPORT int vo_queue_stop(vo_ctx *ctx, uint32_t timeout_ms);
+VO_EXPORT int vo_queue_running(const vo_ctx *ctx);
+
+/* TaskManager::submitCommand / waitCommand.  vo_submit returns the sequence
+ * id (> 0) and stores it in cmd->sequence_id, or 0 on error (see
+ * vo_last_error).  vo_wait returns VO_ERR_TIMEOUT after timeout_ms
+ * (0 -> the context default). */
+VO_EXPORT uint64_t vo_submit(vo_ctx *ctx, vo_command *cmd);
+VO_EXPORT int vo_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms);
+VO_EXPORT int vo_poll(vo_ctx *ctx, uint64_t seq); /* 1 done, 0 pending, <0 -error */
+VO_EXPORT int vo_sync(vo_ctx *ctx);               /* wait for everything submitted */
+
+/* Synchronous conveniences: submit + wait. */
+VO_EXPORT int vo_sort_kp(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad);
+VO_EXPORT int vo_sort(vo_ctx *ctx, const vo_sort_args *args);
+VO_EXPORT int vo_permute(vo_ctx *ctx, const vo_permute_args *args);
+VO_EXPORT int vo_compact(vo_ctx *ctx, const vo_compact_args *args);
+VO_EXPORT int vo_shuffle(vo_ctx *ctx, const vo_shuffle_args *args);
+VO_EXPORT int vo_filter_sum(vo_ctx *ctx, const vo_filter_args *args);
+VO_EXPORT int vo_pkfk_join_groupsum(vo_ctx *ctx, const vo_join_args *args);
+VO_EXPORT int vo_sort_batch(vo_ctx *ctx, const vo_sort_batch_args *args);
+
+/* Asynchronous forms: enqueue on the context stream and return. */
+VO_EXPORT int vo_sort_kp_async(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad);
+
+/* --------------------------------------------------------- diagnostics */
+
+/* Obliviousness trace (modelled on TraceMem,
+ * obliviator/join_kks_INT_INT/IncludeL3/trace_mem.h:23-116).  Between begin
+ * and end every launch is recorded (kernel id, grid, block, LDS bytes and its
+ * non-pointer arguments) and runs an instrumented build of the kernel that
+ * folds every global and LDS address it touches into a per-thread hash. */
+typedef struct vo_trace {
+    uint64_t n_launches;
+    uint64_t launch_digest;  /* hash of the launch sequence and shapes */
+    uint64_t address_digest; /* hash of all per-launch address digests */
+    uint64_t n_addresses;    /* number of addresses folded */
+} vo_trace;
+
+VO_EXPORT int vo_trace_begin(vo_ctx *ctx);
+VO_EXPORT int vo_trace_end(vo_ctx *ctx, vo_trace *out);
+/* Per-launch record i of the last trace: kernel id string, grid/block dims,
+ * that launch's address digest. */
+VO_EXPORT int vo_trace_launch(vo_ctx *ctx, uint64_t i, char *name, uint32_t name_cap,
+                    uint32_t grid[3], uint32_t block[3], uint64_t *addr_digest);
+
+/* Kernel timing (VO_CFG_KERNEL_TIMING): launches and summed milliseconds
+ * per kernel family since the last reset. */
+VO_EXPORT int vo_kernel_stats(vo_ctx *ctx, const char *family, uint64_t *launches, double *total_ms);
+VO_EXPORT int vo_kernel_stats_reset(vo_ctx *ctx);
+
+/* Sort schedule for n_pad rows: number of HBM passes (launches) and the
+ * per-pass record of (kernel family, level, stages), for planning and docs. */
+VO_EXPORT int vo_sort_plan(vo_ctx *ctx, uint64_t n_pad, uint32_t *n_passes);
+
+/* ------------------------------------------------------ data generators */
+
+/* Device generators of the synthetic workloads (same streams as the
+ * host generator; key[i] = SplitMix64 stream 0 of seed, payload[i] = i). */
+VO_EXPORT int vo_gen_sort_input(vo_ctx *ctx, uint64_t seed, uint64_t *keys, uint32_t *payload, uint64_t n);
+VO_EXPORT int vo_gen_fact(vo_ctx *ctx, uint64_t seed, int32_t *day, int64_t *qty, int64_t *price, int64_t *disc, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,213 @@
+"""Python access to the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker / CPU baseline.  The product
+library never imports or calls this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+REF_KAT = os.path.join(_HERE, "_ref", "kks_test_sorts")
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        u64, u32, i64, i32, sz = C.c_uint64, C.c_uint32, C.c_int64, C.c_int32, C.c_size_t
+        L.orc_mix64.restype = u64
+        L.orc_mix64.argtypes = [u64]
+        L.orc_rand64.restype = u64
+        L.orc_rand64.argtypes = [u64, u32, u64]
+        L.orc_rand_range.restype = i64
+        L.orc_rand_range.argtypes = [u64, u32, u64, i64, u64]
+        L.orc_philox4x32_10.restype = None
+        L.orc_philox4x32_10.argtypes = [P, P, P]
+        L.orc_philox_tag.restype = u64
+        L.orc_philox_tag.argtypes = [u64, u64]
+        L.orc_gen_sort_input.restype = None
+        L.orc_gen_sort_input.argtypes = [u64, sz, P, P]
+        L.orc_bitonic_sort_kp.restype = None
+        L.orc_bitonic_sort_kp.argtypes = [P, P, sz, sz, P]
+        L.orc_bitonic_sort_kv.restype = None
+        L.orc_bitonic_sort_kv.argtypes = [P, P, sz]
+        L.orc_bitonic_sort_multi.restype = C.c_int
+        L.orc_bitonic_sort_multi.argtypes = [P, P, P, sz, P, sz, sz]
+        L.orc_permute.restype = None
+        L.orc_permute.argtypes = [P, P, P, sz, sz]
+        L.orc_compact.restype = None
+        L.orc_compact.argtypes = [P, sz, P, sz]
+        L.orc_compact_levels.restype = None
+        L.orc_compact_levels.argtypes = [P, sz, P, sz]
+        L.orc_gen_fact.restype = None
+        L.orc_gen_fact.argtypes = [u64, sz, P, P, P, P]
+        L.orc_filter_mask.restype = None
+        L.orc_filter_mask.argtypes = [P, P, P, sz, sz, P, P]
+        L.orc_filter_sum.restype = sz
+        L.orc_filter_sum.argtypes = [P, P, P, P, sz, sz, P, P, P, P, P]
+        L.orc_gen_join.restype = None
+        L.orc_gen_join.argtypes = [u64, sz, sz, u32, i64, i64, P, P, P, P]
+        L.orc_union_key.restype = u64
+        L.orc_union_key.argtypes = [i32, C.c_int, u32]
+        L.orc_join_groupsum.restype = sz
+        L.orc_join_groupsum.argtypes = [P, P, sz, P, P, sz, sz, P, P, P, P, sz]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Trace(C.Structure):
+    _fields_ = [("hash", C.c_uint64), ("n_ops", C.c_uint64), ("n_stages", C.c_uint64)]
+
+
+class FilterSpec(C.Structure):
+    _fields_ = [("day_lo", C.c_int32), ("day_hi", C.c_int32), ("disc_lo", C.c_int64), ("disc_hi", C.c_int64),
+                ("qty_lt", C.c_int64)]
+
+
+Q6 = dict(day_lo=8766, day_hi=9131, disc_lo=5, disc_hi=7, qty_lt=24)
+
+
+def gen_sort_input(seed, n):
+    k = np.empty(n, np.uint64)
+    p = np.empty(n, np.uint32)
+    lib().orc_gen_sort_input(seed, n, _p(k), _p(p))
+    return k, p
+
+
+def bitonic_sort_kp(keys, payload, n, npad, trace=False):
+    """Returns (keys, payload[, Trace]) sorted over npad rows (copies)."""
+    k = np.empty(npad, np.uint64)
+    p = np.empty(npad, np.uint32)
+    k[:n] = keys[:n]
+    p[:n] = payload[:n]
+    t = Trace() if trace else None
+    lib().orc_bitonic_sort_kp(_p(k), _p(p), n, npad, C.byref(t) if trace else None)
+    return (k, p, t) if trace else (k, p)
+
+
+def bitonic_sort_kv(keys, vals):
+    k = np.ascontiguousarray(keys, np.uint64).copy()
+    v = np.ascontiguousarray(vals, np.uint64).copy()
+    lib().orc_bitonic_sort_kv(_p(k), _p(v), len(k))
+    return k, v
+
+
+def bitonic_sort_multi(cols, types, asc, n, npad):
+    """cols: list of numpy arrays of length >= npad (copied).  Returns (cols, idx)."""
+    cs = [np.ascontiguousarray(c).copy() for c in cols]
+    ptrs = (C.c_void_p * 8)(*[c.ctypes.data for c in cs])
+    ty = (C.c_uint32 * 8)(*types)
+    fl = (C.c_uint8 * 8)(*[1 if a else 0 for a in asc])
+    idx = np.empty(npad, np.int64)
+    rc = lib().orc_bitonic_sort_multi(ptrs, ty, fl, len(cs), _p(idx), n, npad)
+    if rc != 0:
+        raise ValueError("orc_bitonic_sort_multi rejected its arguments")
+    return cs, idx
+
+
+def compact(buf, selected):
+    """buf: 2-D array (rows x bytes-per-row view) compacted in place copy."""
+    b = np.ascontiguousarray(buf).copy()
+    s = np.ascontiguousarray(selected, np.uint8)
+    lib().orc_compact(_p(b), b.itemsize * (b.shape[1] if b.ndim == 2 else 1), _p(s), len(s))
+    return b
+
+
+def compact_levels(buf, selected):
+    b = np.ascontiguousarray(buf).copy()
+    s = np.ascontiguousarray(selected, np.uint8)
+    lib().orc_compact_levels(_p(b), b.itemsize * (b.shape[1] if b.ndim == 2 else 1), _p(s), len(s))
+    return b
+
+
+def gen_fact(seed, n):
+    day = np.empty(n, np.int32)
+    qty = np.empty(n, np.int64)
+    price = np.empty(n, np.int64)
+    disc = np.empty(n, np.int64)
+    lib().orc_gen_fact(seed, n, _p(day), _p(qty), _p(price), _p(disc))
+    return day, qty, price, disc
+
+
+def filter_sum(day, qty, price, disc, n, npad, spec=Q6, want_rows=False):
+    """Returns (count, int128 sum as Python int[, out_price, out_disc])."""
+    def padded(a, dt):
+        out = np.zeros(npad, dt)
+        out[:n] = a[:n]
+        return out
+    d, q, pr, di = padded(day, np.int32), padded(qty, np.int64), padded(price, np.int64), padded(disc, np.int64)
+    op = np.empty(npad, np.int64) if want_rows else None
+    od = np.empty(npad, np.int64) if want_rows else None
+    lo, hi = C.c_uint64(), C.c_int64()
+    fs = FilterSpec(**spec)
+    m = lib().orc_filter_sum(_p(d), _p(q), _p(pr), _p(di), n, npad, C.byref(fs), _p(op), _p(od),
+                             C.byref(lo), C.byref(hi))
+    s = (hi.value << 64) | lo.value
+    return (m, s, op, od) if want_rows else (m, s)
+
+
+def filter_mask(day, qty, disc, n, npad, spec=Q6):
+    m = np.empty(npad, np.uint8)
+    fs = FilterSpec(**spec)
+
+    def padded(a, dt):
+        out = np.zeros(npad, dt)
+        out[:n] = a[:n]
+        return out
+    lib().orc_filter_mask(_p(padded(day, np.int32)), _p(padded(qty, np.int64)), _p(padded(disc, np.int64)),
+                          n, npad, C.byref(fs), _p(m))
+    return m
+
+
+def gen_join(seed, n_dim, n_fact, n_groups=25, vlo=100, vhi=10_000_000):
+    dk = np.empty(n_dim, np.int32)
+    dg = np.empty(n_dim, np.int32)
+    fk = np.empty(n_fact, np.int32)
+    fv = np.empty(n_fact, np.int64)
+    lib().orc_gen_join(seed, n_dim, n_fact, n_groups, vlo, vhi, _p(dk), _p(dg), _p(fk), _p(fv))
+    return dk, dg, fk, fv
+
+
+def join_groupsum(dk, dg, fk, fv, npad, max_groups=64, want_sorted=False):
+    sk = np.empty(npad, np.uint64) if want_sorted else None
+    sv = np.empty(npad, np.uint64) if want_sorted else None
+    og = np.empty(max_groups, np.int32)
+    os_ = np.empty(max_groups, np.int64)
+    G = lib().orc_join_groupsum(_p(dk), _p(dg), len(dk), _p(fk), _p(fv), len(fk), npad, _p(sk), _p(sv),
+                                _p(og), _p(os_), max_groups)
+    out = (og[:G].copy(), os_[:G].copy())
+    return out + (sk, sv) if want_sorted else out
+
+
+def rand64(seed, stream, i):
+    return lib().orc_rand64(seed, stream, i)
+
+
+def philox4x32_10(ctr, key):
+    c = np.array(ctr, np.uint32)
+    k = np.array(key, np.uint32)
+    o = np.empty(4, np.uint32)
+    lib().orc_philox4x32_10(_p(c), _p(k), _p(o))
+    return [int(x) for x in o]
+
+
+def philox_tag(seed, i):
+    return lib().orc_philox_tag(seed, i)

@@ -1,0 +1,538 @@
+/*
+ * oracle.c -- CPU restatement of the reference's oblivious-operator
+ * algorithms.  TEST INFRASTRUCTURE ONLY (see oracle.h): the checker for the
+ * HIP path, never linked into or called by the product library.
+ */
+#include "oracle.h"
+
+#include <limits.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------- gen --- */
+
+#define GOLDEN64 0x9E3779B97F4A7C15ull
+
+uint64_t orc_mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+uint64_t orc_rand64(uint64_t seed, uint32_t stream, uint64_t i)
+{
+    uint64_t s0 = orc_mix64(seed ^ (GOLDEN64 * ((uint64_t)stream + 1)));
+    return orc_mix64(s0 + (i + 1) * GOLDEN64);
+}
+
+int64_t orc_rand_range(uint64_t seed, uint32_t stream, uint64_t i, int64_t lo, uint64_t range)
+{
+    uint64_t r32 = orc_rand64(seed, stream, i) >> 32;
+    return lo + (int64_t)((r32 * range) >> 32);
+}
+
+static inline uint32_t mulhilo32(uint32_t a, uint32_t b, uint32_t *hi)
+{
+    uint64_t p = (uint64_t)a * b;
+    *hi = (uint32_t)(p >> 32);
+    return (uint32_t)p;
+}
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    /* Round multipliers and Weyl key increments from the SC'11 paper. */
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0, hi1;
+        uint32_t lo0 = mulhilo32(M0, c0, &hi0);
+        uint32_t lo1 = mulhilo32(M1, c2, &hi1);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n1 = lo1;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        uint32_t n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += W0; k1 += W1;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+uint64_t orc_philox_tag(uint64_t seed, uint64_t i)
+{
+    uint32_t ctr[4] = {(uint32_t)i, (uint32_t)(i >> 32), 0u, 0u};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t out[4];
+    orc_philox4x32_10(ctr, key, out);
+    return ((uint64_t)out[1] << 32) | out[0];
+}
+
+void orc_gen_sort_input(uint64_t seed, size_t n, uint64_t *keys, uint32_t *payload)
+{
+    for (size_t i = 0; i < n; ++i) {
+        keys[i] = orc_rand64(seed, 0, i);
+        payload[i] = (uint32_t)i;
+    }
+}
+
+/* ------------------------------------------------------------ bitonic --- */
+
+static inline void trace_fold(orc_trace *t, int64_t v)
+{
+    /* FNV-1a over the 8 bytes of v */
+    uint64_t h = t->hash;
+    for (int b = 0; b < 8; ++b) {
+        h ^= (uint64_t)((v >> (8 * b)) & 0xff);
+        h *= 0x100000001B3ull;
+    }
+    t->hash = h;
+    t->n_ops++;
+}
+
+void orc_bitonic_sort_kp(uint64_t *keys, uint32_t *payload, size_t n, size_t npad, orc_trace *trace)
+{
+    if (trace) {
+        trace->hash = 0xCBF29CE484222325ull;
+        trace->n_ops = 0;
+        trace->n_stages = 0;
+    }
+    for (size_t i = n; i < npad; ++i) {
+        keys[i] = UINT64_MAX;
+        payload[i] = (uint32_t)i;
+        if (trace) trace_fold(trace, -1 - (int64_t)i);
+    }
+    for (size_t k = 2; k <= npad; k <<= 1) {
+        for (size_t j = k >> 1; j > 0; j >>= 1) {
+            if (trace) trace->n_stages++;
+            for (size_t i = 0; i < npad; ++i) {
+                size_t ixj = i ^ j;
+                if (i < ixj) {
+                    int ascending = ((i & k) == 0);
+                    uint64_t ka = keys[i], kb = keys[ixj];
+                    uint32_t pa = payload[i], pb = payload[ixj];
+                    int gt = (ka > kb) || (ka == kb && pa > pb);
+                    int lt = (ka < kb) || (ka == kb && pa < pb);
+                    int sw = ascending ? gt : lt;
+                    /* branch-free select, both slots always written */
+                    uint64_t m64 = (uint64_t)0 - (uint64_t)sw;
+                    uint32_t m32 = (uint32_t)0 - (uint32_t)sw;
+                    keys[i] = (ka & ~m64) | (kb & m64);
+                    keys[ixj] = (kb & ~m64) | (ka & m64);
+                    payload[i] = (pa & ~m32) | (pb & m32);
+                    payload[ixj] = (pb & ~m32) | (pa & m32);
+                    if (trace) {
+                        trace_fold(trace, (int64_t)i);
+                        trace_fold(trace, (int64_t)ixj);
+                        trace_fold(trace, -1 - (int64_t)i);
+                        trace_fold(trace, -1 - (int64_t)ixj);
+                    }
+                }
+            }
+        }
+    }
+}
+
+void orc_bitonic_sort_kv(uint64_t *keys, uint64_t *vals, size_t npad)
+{
+    for (size_t k = 2; k <= npad; k <<= 1) {
+        for (size_t j = k >> 1; j > 0; j >>= 1) {
+            for (size_t i = 0; i < npad; ++i) {
+                size_t ixj = i ^ j;
+                if (i < ixj) {
+                    int ascending = ((i & k) == 0);
+                    uint64_t ka = keys[i], kb = keys[ixj];
+                    uint64_t va = vals[i], vb = vals[ixj];
+                    int gt = (ka > kb) || (ka == kb && va > vb);
+                    int lt = (ka < kb) || (ka == kb && va < vb);
+                    if (ascending ? gt : lt) {
+                        keys[i] = kb; keys[ixj] = ka;
+                        vals[i] = vb; vals[ixj] = va;
+                    }
+                }
+            }
+        }
+    }
+}
+
+static size_t type_size(uint32_t t)
+{
+    switch (t) {
+    case ORC_T_BOOLEAN: case ORC_T_TINYINT: return 1;
+    case ORC_T_SMALLINT: return 2;
+    case ORC_T_INTEGER: case ORC_T_FLOAT: case ORC_T_DATE: return 4;
+    case ORC_T_BIGINT: case ORC_T_DOUBLE: case ORC_T_VARCHAR: case ORC_T_UBIGINT: return 8;
+    default: return 0;
+    }
+}
+
+/* compareSingle (sort.cuh:11-29): 0 on equality, -1 if a < b, else 1 */
+static int compare_single(const void *col, uint32_t t, size_t a, size_t b)
+{
+#define CMP(T)                                                   \
+    do {                                                         \
+        T x = ((const T *)col)[a], y = ((const T *)col)[b];      \
+        if (x == y) return 0;                                    \
+        return (x < y) ? -1 : 1;                                 \
+    } while (0)
+    switch (t) {
+    case ORC_T_BOOLEAN: CMP(uint8_t);
+    case ORC_T_TINYINT: CMP(int8_t);
+    case ORC_T_SMALLINT: CMP(int16_t);
+    case ORC_T_INTEGER: CMP(int32_t);
+    case ORC_T_BIGINT: CMP(int64_t);
+    case ORC_T_FLOAT: CMP(float);
+    case ORC_T_DOUBLE: CMP(double);
+    case ORC_T_VARCHAR: CMP(uint64_t);
+    case ORC_T_DATE: CMP(uint32_t);
+    case ORC_T_UBIGINT: CMP(uint64_t);
+    default: return 0;
+    }
+#undef CMP
+}
+
+static void fill_pad(void *col, uint32_t t, int asc, size_t i)
+{
+    switch (t) {
+    case ORC_T_BOOLEAN: ((uint8_t *)col)[i] = asc ? UINT8_MAX : 0; break;
+    case ORC_T_TINYINT: ((int8_t *)col)[i] = asc ? INT8_MAX : INT8_MIN; break;
+    case ORC_T_SMALLINT: ((int16_t *)col)[i] = asc ? INT16_MAX : INT16_MIN; break;
+    case ORC_T_INTEGER: ((int32_t *)col)[i] = asc ? INT32_MAX : INT32_MIN; break;
+    case ORC_T_BIGINT: ((int64_t *)col)[i] = asc ? INT64_MAX : INT64_MIN; break;
+    case ORC_T_FLOAT: {
+        /* numeric_limits<float>::max() / lowest() */
+        float v = asc ? 3.40282347e+38f : -3.40282347e+38f;
+        ((float *)col)[i] = v;
+        break;
+    }
+    case ORC_T_DOUBLE: ((double *)col)[i] = asc ? 1.7976931348623157e+308 : -1.7976931348623157e+308; break;
+    case ORC_T_VARCHAR: ((uint64_t *)col)[i] = asc ? UINT64_MAX : 0; break;
+    case ORC_T_DATE: ((uint32_t *)col)[i] = asc ? UINT32_MAX : 0; break;
+    case ORC_T_UBIGINT: ((uint64_t *)col)[i] = asc ? UINT64_MAX : 0; break;
+    default: break;
+    }
+}
+
+static void swap_bytes(void *col, size_t sz, size_t a, size_t b)
+{
+    unsigned char tmp[16];
+    unsigned char *p = (unsigned char *)col;
+    memcpy(tmp, p + a * sz, sz);
+    memcpy(p + a * sz, p + b * sz, sz);
+    memcpy(p + b * sz, tmp, sz);
+}
+
+int orc_bitonic_sort_multi(void *const *cols, const uint32_t *types, const uint8_t *asc,
+                           size_t ncols, int64_t *indices, size_t n, size_t npad)
+{
+    if (ncols == 0 || ncols > 8 || n > npad || (npad & (npad - 1)) != 0) return -1;
+    for (size_t c = 0; c < ncols; ++c)
+        if (type_size(types[c]) == 0) return -1;
+    for (size_t i = 0; i < npad; ++i) {
+        indices[i] = (int64_t)i;
+        if (i >= n)
+            for (size_t c = 0; c < ncols; ++c) fill_pad(cols[c], types[c], asc[c], i);
+    }
+    for (size_t k = 2; k <= npad; k <<= 1) {
+        for (size_t j = k >> 1; j > 0; j >>= 1) {
+            for (size_t i = 0; i < npad; ++i) {
+                size_t ixj = i ^ j;
+                if (i < ixj) {
+                    int ascending = ((i & k) == 0);
+                    int need = 0;
+                    for (size_t c = 0; c < ncols; ++c) {
+                        int cmp = compare_single(cols[c], types[c], i, ixj);
+                        if (cmp != 0) {
+                            need = (ascending ^ (asc[c] != 0)) ? (cmp < 0) : (cmp > 0);
+                            break;
+                        }
+                    }
+                    if (need) {
+                        int64_t t = indices[i];
+                        indices[i] = indices[ixj];
+                        indices[ixj] = t;
+                        for (size_t c = 0; c < ncols; ++c) swap_bytes(cols[c], type_size(types[c]), i, ixj);
+                    }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+void orc_permute(void *out, const void *in, const int64_t *idx, size_t n, size_t elem_size)
+{
+    unsigned char *o = (unsigned char *)out;
+    const unsigned char *s = (const unsigned char *)in;
+    for (size_t i = 0; i < n; ++i) memcpy(o + i * elem_size, s + (size_t)idx[i] * elem_size, elem_size);
+}
+
+/* ---------------------------------------------------------- ORCompact --- */
+
+/* o_memswap(a, b, size, cond): conditional swap of whole elements */
+static void cswap(unsigned char *a, unsigned char *b, size_t sz, int cond)
+{
+    unsigned char m = (unsigned char)(0 - (cond != 0));
+    for (size_t i = 0; i < sz; ++i) {
+        unsigned char x = (unsigned char)((a[i] ^ b[i]) & m);
+        a[i] ^= x;
+        b[i] ^= x;
+    }
+}
+
+/* largest power of two strictly below N for N > 1 (common/util.c:58-65) */
+static size_t pow2_lt(size_t N)
+{
+    size_t N1 = 1;
+    while (N1 < N) N1 <<= 1;
+    return N1 >> 1;
+}
+
+/* oblivious_compact_2power_inner_elem (oblivious_compact.c:59-106) */
+static void compact_2power(unsigned char *buf, size_t N, size_t sz, size_t offset,
+                           const uint8_t *sel, const size_t *cnt)
+{
+    if (N == 1) return;
+    if (N == 2) {
+        int swap = (((!sel[0]) & sel[1]) != 0) ^ (int)(offset & 1);
+        cswap(buf, buf + sz, sz, swap);
+        return;
+    }
+    size_t half = N / 2;
+    size_t m1 = cnt[half] - cnt[0];
+    size_t offset_mod = offset & (half - 1);
+    size_t offset_m1_mod = (offset + m1) & (half - 1);
+    int offset_right = offset >= half;
+    int left_wrapped = (offset_mod + m1) >= half;
+    compact_2power(buf, half, sz, offset_mod, sel, cnt);
+    compact_2power(buf + half * sz, half, sz, offset_m1_mod, sel + half, cnt + half);
+    int swap_flag = left_wrapped ^ offset_right;
+    for (size_t i = 0; i < half; ++i) {
+        swap_flag = swap_flag ^ (i == offset_m1_mod);
+        cswap(buf + i * sz, buf + (half + i) * sz, sz, swap_flag);
+    }
+}
+
+/* oblivious_compact_inner_elem (oblivious_compact.c:108-153) */
+static void compact_inner(unsigned char *buf, size_t N, size_t sz, const uint8_t *sel, const size_t *cnt)
+{
+    if (N <= 1) return;
+    if (N == 2) {
+        cswap(buf, buf + sz, sz, ((!sel[0]) & sel[1]) != 0);
+        return;
+    }
+    size_t gt_pow2 = pow2_lt(N);
+    size_t split = N - gt_pow2;
+    size_t mL = cnt[split] - cnt[0];
+    compact_inner(buf, split, sz, sel, cnt);
+    compact_2power(buf + split * sz, gt_pow2, sz, (gt_pow2 - split + mL) % gt_pow2, sel + split, cnt + split);
+    unsigned char *L = buf, *R = buf + gt_pow2 * sz;
+    for (size_t i = 0; i < split; ++i) cswap(L + i * sz, R + i * sz, sz, i >= mL);
+}
+
+void orc_compact(void *buf, size_t elem_size, const uint8_t *selected, size_t n)
+{
+    size_t *cnt = (size_t *)calloc(n + 1, sizeof(size_t));
+    for (size_t i = 0; i < n; ++i) cnt[i + 1] = cnt[i] + (selected[i] != 0);
+    uint8_t *sel = (uint8_t *)malloc(n ? n : 1);
+    for (size_t i = 0; i < n; ++i) sel[i] = selected[i] != 0;
+    compact_inner((unsigned char *)buf, n, elem_size, sel, cnt);
+    free(sel);
+    free(cnt);
+}
+
+void orc_compact_levels(void *buf, size_t elem_size, const uint8_t *selected, size_t n)
+{
+    size_t *C = (size_t *)calloc(n + 1, sizeof(size_t));
+    for (size_t i = 0; i < n; ++i) C[i + 1] = C[i] + (selected[i] != 0);
+    unsigned char *p = (unsigned char *)buf;
+    for (size_t s = 2; s <= n; s <<= 1) {
+        size_t half = s / 2;
+        for (size_t b = 0; b < n; b += s) {
+            size_t o = C[b] & (s - 1);
+            if (s == 2) {
+                int sw = ((selected[b] == 0) & (selected[b + 1] != 0)) ^ (int)(o & 1);
+                cswap(p + b * elem_size, p + (b + 1) * elem_size, elem_size, sw);
+                continue;
+            }
+            size_t m1 = C[b + half] - C[b];
+            size_t om = o & (half - 1);
+            size_t om1 = (o + m1) & (half - 1);
+            int base = ((om + m1) >= half) ^ (o >= half);
+            for (size_t i = 0; i < half; ++i)
+                cswap(p + (b + i) * elem_size, p + (b + half + i) * elem_size, elem_size, base ^ (i >= om1));
+        }
+    }
+    free(C);
+}
+
+/* ------------------------------------------------------ filter + sum --- */
+
+void orc_gen_fact(uint64_t seed, size_t n, int32_t *day, int64_t *qty, int64_t *price, int64_t *disc)
+{
+    for (size_t i = 0; i < n; ++i) {
+        day[i] = (int32_t)orc_rand_range(seed, 1, i, 8036, 10561 - 8036 + 1);
+        qty[i] = orc_rand_range(seed, 2, i, 1, 50);
+        price[i] = orc_rand_range(seed, 3, i, 90000, 10495000 - 90000 + 1);
+        disc[i] = orc_rand_range(seed, 4, i, 0, 11);
+    }
+}
+
+void orc_filter_mask(const int32_t *day, const int64_t *qty, const int64_t *disc,
+                     size_t n, size_t npad, const orc_filter_spec *spec, uint8_t *mask)
+{
+    for (size_t i = 0; i < npad; ++i) {
+        int live = i < n;
+        size_t r = live ? i : 0;
+        uint8_t m = (uint8_t)((day[r] >= spec->day_lo) & (day[r] < spec->day_hi) &
+                              (disc[r] >= spec->disc_lo) & (disc[r] <= spec->disc_hi) &
+                              (qty[r] < spec->qty_lt) & live);
+        mask[i] = m;
+    }
+}
+
+size_t orc_filter_sum(const int32_t *day, const int64_t *qty, const int64_t *price, const int64_t *disc,
+                      size_t n, size_t npad, const orc_filter_spec *spec,
+                      int64_t *out_price, int64_t *out_disc, uint64_t *sum_lo, int64_t *sum_hi)
+{
+    uint8_t *mask = (uint8_t *)malloc(npad);
+    int64_t *pairs = (int64_t *)malloc(npad * 2 * sizeof(int64_t));
+    orc_filter_mask(day, qty, disc, n, npad, spec, mask);
+    size_t m = 0;
+    for (size_t i = 0; i < npad; ++i) {
+        m += mask[i];
+        pairs[2 * i] = i < n ? price[i] : 0;
+        pairs[2 * i + 1] = i < n ? disc[i] : 0;
+    }
+    orc_compact(pairs, 2 * sizeof(int64_t), mask, npad);
+    __int128 acc = 0;
+    for (size_t i = 0; i < npad; ++i) {
+        __int128 prod = (__int128)pairs[2 * i] * pairs[2 * i + 1];
+        __int128 keep = (__int128)(i < m);
+        acc += prod * keep;
+        if (out_price) out_price[i] = pairs[2 * i];
+        if (out_disc) out_disc[i] = pairs[2 * i + 1];
+    }
+    *sum_lo = (uint64_t)acc;
+    *sum_hi = (int64_t)(acc >> 64);
+    free(mask);
+    free(pairs);
+    return m;
+}
+
+/* ------------------------------------------------------ join + group --- */
+
+typedef struct { uint64_t tag; uint64_t idx; } tag_pair;
+
+static int cmp_tag_pair(const void *a, const void *b)
+{
+    const tag_pair *x = (const tag_pair *)a, *y = (const tag_pair *)b;
+    if (x->tag != y->tag) return x->tag < y->tag ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+void orc_gen_join(uint64_t seed, size_t n_dim, size_t n_fact, uint32_t n_groups,
+                  int64_t vlo, int64_t vhi,
+                  int32_t *dim_key, int32_t *dim_group, int32_t *fact_fk, int64_t *fact_val)
+{
+    tag_pair *tp = (tag_pair *)malloc((n_dim ? n_dim : 1) * sizeof(tag_pair));
+    for (size_t i = 0; i < n_dim; ++i) {
+        tp[i].tag = orc_rand64(seed, 5, i);
+        tp[i].idx = i;
+    }
+    qsort(tp, n_dim, sizeof(tag_pair), cmp_tag_pair);
+    for (size_t i = 0; i < n_dim; ++i) {
+        dim_key[i] = (int32_t)(tp[i].idx + 1);
+        dim_group[i] = (int32_t)orc_rand_range(seed, 6, i, 0, n_groups);
+    }
+    for (size_t i = 0; i < n_fact; ++i) {
+        fact_fk[i] = (int32_t)orc_rand_range(seed, 7, i, 1, n_dim);
+        fact_val[i] = orc_rand_range(seed, 8, i, vlo, (uint64_t)(vhi - vlo + 1));
+    }
+    free(tp);
+}
+
+uint64_t orc_union_key(int32_t key, int is_fact, uint32_t row)
+{
+    return ((uint64_t)(uint32_t)key << 32) | ((uint64_t)(is_fact != 0) << 31) | (row & 0x7FFFFFFFu);
+}
+
+size_t orc_join_groupsum(const int32_t *dim_key, const int32_t *dim_group, size_t n_dim,
+                         const int32_t *fact_fk, const int64_t *fact_val, size_t n_fact,
+                         size_t npad,
+                         uint64_t *sorted_keys, uint64_t *sorted_vals,
+                         int32_t *out_group, int64_t *out_sum, size_t max_groups)
+{
+    uint64_t *k = (uint64_t *)malloc(npad * sizeof(uint64_t));
+    uint64_t *v = (uint64_t *)malloc(npad * sizeof(uint64_t));
+    /* 1. union: dimension rows then fact rows, pads with key = max */
+    for (size_t i = 0; i < npad; ++i) {
+        if (i < n_dim) {
+            k[i] = orc_union_key(dim_key[i], 0, (uint32_t)i);
+            v[i] = (uint64_t)(uint32_t)dim_group[i];
+        } else if (i < n_dim + n_fact) {
+            size_t f = i - n_dim;
+            k[i] = orc_union_key(fact_fk[f], 1, (uint32_t)f);
+            v[i] = (uint64_t)fact_val[f];
+        } else {
+            k[i] = UINT64_MAX;
+            v[i] = 0;
+        }
+    }
+    /* 2. oblivious sort by union key */
+    orc_bitonic_sort_kv(k, v, npad);
+    if (sorted_keys) memcpy(sorted_keys, k, npad * sizeof(uint64_t));
+    if (sorted_vals) memcpy(sorted_vals, v, npad * sizeof(uint64_t));
+    /* 3. forward fill of the last dimension row (branch-free selects) */
+    uint8_t *sel = (uint8_t *)malloc(npad);
+    int64_t *rows = (int64_t *)malloc(npad * 2 * sizeof(int64_t));
+    uint64_t cur_key = UINT64_MAX, cur_group = 0;
+    for (size_t i = 0; i < npad; ++i) {
+        uint64_t key32 = k[i] >> 32;
+        int is_pad = k[i] == UINT64_MAX;
+        int is_fact = (int)((k[i] >> 31) & 1) & !is_pad;
+        int is_dim = !is_fact & !is_pad;
+        cur_key = is_dim ? key32 : cur_key;
+        cur_group = is_dim ? v[i] : cur_group;
+        int matched = is_fact & (cur_key == key32);
+        sel[i] = (uint8_t)matched;
+        rows[2 * i] = (int64_t)cur_group;
+        rows[2 * i + 1] = (int64_t)v[i];
+    }
+    /* 4. compaction of matched fact rows */
+    size_t m = 0;
+    for (size_t i = 0; i < npad; ++i) m += sel[i];
+    orc_compact(rows, 2 * sizeof(int64_t), sel, npad);
+    /* 5. sort join rows by (group, position); rows >= m get group 0xFFFFFFFF */
+    for (size_t i = 0; i < npad; ++i) {
+        uint64_t g = i < m ? (uint64_t)rows[2 * i] : 0xFFFFFFFFull;
+        k[i] = (g << 32) | (uint64_t)i;
+        v[i] = (uint64_t)rows[2 * i + 1];
+    }
+    orc_bitonic_sort_kv(k, v, npad);
+    /* 6. segmented inclusive sum, last row of each group selected */
+    int64_t run = 0;
+    for (size_t i = 0; i < npad; ++i) {
+        uint64_t g = k[i] >> 32;
+        int same = i > 0 && (k[i - 1] >> 32) == g;
+        run = (same ? run : 0) + (int64_t)v[i];
+        int last = (i + 1 == npad) || ((k[i + 1] >> 32) != g);
+        sel[i] = (uint8_t)(last & (g != 0xFFFFFFFFull));
+        rows[2 * i] = (int64_t)g;
+        rows[2 * i + 1] = run;
+    }
+    /* 7. compaction -> one row per group */
+    size_t G = 0;
+    for (size_t i = 0; i < npad; ++i) G += sel[i];
+    orc_compact(rows, 2 * sizeof(int64_t), sel, npad);
+    for (size_t i = 0; i < G && i < max_groups; ++i) {
+        out_group[i] = (int32_t)rows[2 * i];
+        out_sum[i] = rows[2 * i + 1];
+    }
+    free(k);
+    free(v);
+    free(sel);
+    free(rows);
+    return G;
+}

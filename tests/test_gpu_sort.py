@@ -1,0 +1,160 @@
+"""GPU parity of the oblivious bitonic sort (through the C ABI) against the
+CPU oracle: bit-exact outputs, value-independent launch/address traces."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.gpu_util import empty_dev, from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+M64 = np.uint64((1 << 64) - 1)
+
+
+def _run_sort(vctx, k, p, n, npad):
+    dk = to_dev(k, npad)
+    dp = to_dev(p, npad)
+    vctx.sort_kp(dk, dp, n, npad)
+    return from_dev(dk, np.uint64), from_dev(dp, np.uint32)
+
+
+@pytest.mark.parametrize("n,npad", [(0, 1), (1, 1), (2, 2), (3, 4), (7, 8), (16, 16), (17, 32), (1000, 1024),
+                                    (4095, 4096), (8192, 8192), (8193, 16384), (50000, 65536),
+                                    (1 << 17, 1 << 17), (300000, 1 << 19)])
+def test_sort_kp_bit_exact(vctx, n, npad):
+    print(f"sort_kp n={n} npad={npad}", flush=True)
+    k, p = oracle.gen_sort_input(1000 + n, n)
+    gk, gp = _run_sort(vctx, k, p, n, npad)
+    ek, ep = oracle.bitonic_sort_kp(k, p, n, npad)
+    assert np.array_equal(gk, ek)
+    assert np.array_equal(gp, ep)
+
+
+def test_sort_config1_2p20_two_seeds(vctx):
+    """Config: 2^20 records, u64 uniform keys, payload = row index, seeds 1 and 2."""
+    n = 1 << 20
+    for seed in (1, 2):
+        print(f"config1 seed={seed}", flush=True)
+        k, p = oracle.gen_sort_input(seed, n)
+        gk, gp = _run_sort(vctx, k, p, n, n)
+        ek, ep = oracle.bitonic_sort_kp(k, p, n, n)
+        assert np.array_equal(gk, ek) and np.array_equal(gp, ep)
+
+
+def test_sort_duplicate_keys_ties_by_payload(vctx):
+    rng = np.random.default_rng(2)
+    n = 1 << 15
+    k = rng.integers(0, 5, n).astype(np.uint64)
+    p = rng.permutation(n).astype(np.uint32)
+    gk, gp = _run_sort(vctx, k, p, n, n)
+    order = np.lexsort((p, k))
+    assert np.array_equal(gk, k[order]) and np.array_equal(gp, p[order])
+
+
+def test_sort_extreme_keys(vctx):
+    # keys at 0 and 2^64-1 next to pad rows; payload = row index keeps them ahead of pads
+    n, npad = 1000, 1024
+    k = np.where(np.arange(n) % 3 == 0, M64, np.uint64(0)).astype(np.uint64)
+    p = np.arange(n, dtype=np.uint32)
+    gk, gp = _run_sort(vctx, k, p, n, npad)
+    ek, ep = oracle.bitonic_sort_kp(k, p, n, npad)
+    assert np.array_equal(gk, ek) and np.array_equal(gp, ep)
+    assert gp[:n].max() < n
+
+
+def test_device_generator_matches_oracle(vctx):
+    n = 100003
+    dk = empty_dev(n, np.uint64)
+    dp = empty_dev(n, np.uint32)
+    vctx.gen_sort_input(7, dk, dp, n)
+    k, p = oracle.gen_sort_input(7, n)
+    assert np.array_equal(from_dev(dk, np.uint64), k) and np.array_equal(from_dev(dp, np.uint32), p)
+
+
+def _trace(vctx, seed, n, npad):
+    k, p = oracle.gen_sort_input(seed, n)
+    dk = to_dev(k, npad)
+    dp = to_dev(p, npad)
+    vctx.trace_begin()
+    vctx.sort_kp(dk, dp, n, npad)
+    t = vctx.trace_end()
+    return t, vctx.trace_launches(t.n_launches)
+
+
+@pytest.mark.parametrize("n,npad", [(1 << 20, 1 << 20), (5000, 8192), (100, 128)])
+def test_sort_trace_value_independent(vctx, n, npad):
+    """Launch sequence, grid shapes and per-launch address digests are equal
+    for two inputs of equal padded size and different values."""
+    print(f"trace n={n}", flush=True)
+    t1, l1 = _trace(vctx, 1, n, npad)
+    t2, l2 = _trace(vctx, 2, n, npad)
+    assert t1.n_launches == t2.n_launches > 0
+    assert l1 == l2
+    assert t1.launch_digest == t2.launch_digest
+    assert t1.address_digest == t2.address_digest
+    # every launch folded addresses: each record is read and written once per pass
+    assert t1.n_addresses >= 2 * npad * t1.n_launches
+
+
+def test_sort_trace_sees_a_different_shape(vctx):
+    # sanity: the digest is sensitive (a different padded size changes it)
+    t1, _ = _trace(vctx, 1, 4000, 4096)
+    t2, _ = _trace(vctx, 1, 4000, 8192)
+    assert t1.launch_digest != t2.launch_digest
+
+
+def test_sort_large_properties(vctx):
+    """2^24 records: size-independent properties (sorted, payload a permutation)."""
+    import torch
+    n = 1 << 24
+    dk = empty_dev(n, np.uint64)
+    dp = empty_dev(n, np.uint32)
+    vctx.gen_sort_input(3, dk, dp, n)
+    vctx.sort_kp(dk, dp, n, n)
+    k = from_dev(dk, np.uint64)
+    p = from_dev(dp, np.uint32)
+    assert np.all(k[1:] >= k[:-1])
+    assert np.array_equal(np.sort(p), np.arange(n, dtype=np.uint32))
+    # keys travelled with their payloads
+    kk, _ = oracle.gen_sort_input(3, n)
+    assert np.array_equal(kk[p], k)
+    del torch
+
+
+# ----------------------------------------------------- multi-column OP_SORT
+
+TYPES = {1: np.uint8, 2: np.int8, 3: np.int16, 4: np.int32, 5: np.int64, 6: np.float32, 7: np.float64,
+         8: np.uint64, 10: np.uint32, 64: np.uint64}
+
+
+def _rand_col(rng, t, n, distinct):
+    dt = TYPES[t]
+    if t in (6, 7):
+        return (rng.integers(-distinct, distinct, n) * 0.5).astype(dt)
+    info = np.iinfo(dt)
+    lo, hi = max(int(info.min), -distinct), min(int(info.max), distinct)
+    return rng.integers(lo, hi, n, endpoint=True).astype(dt)
+
+
+@pytest.mark.parametrize("types,asc,n,npad,distinct", [
+    ([4], [True], 1000, 1024, 50),
+    ([5, 4], [True, False], 3000, 4096, 10),
+    ([7, 8], [False, True], 5000, 8192, 20),
+    ([1, 3, 2], [True, True, False], 20000, 32768, 3),
+    ([6, 10, 5, 4], [False, True, True, False], 9000, 16384, 4),
+    ([4, 4, 4, 4, 4, 4, 4, 4], [True, False] * 4, 2000, 2048, 2),
+    ([64], [True], 70000, 1 << 17, 1 << 30),
+])
+def test_sort_multi_bit_exact_with_ties(vctx, types, asc, n, npad, distinct):
+    """Reference OP_SORT semantics incl. ties (decided by the network) and the
+    max/lowest pad rows; indices and every column bit-exact."""
+    print(f"sort_multi types={types} n={n}", flush=True)
+    rng = np.random.default_rng(len(types) * 1000 + n)
+    cols = [_rand_col(rng, t, npad, distinct) for t in types]
+    ecols, eidx = oracle.bitonic_sort_multi(cols, types, asc, n, npad)
+    dcols = [to_dev(c) for c in cols]
+    didx = empty_dev(npad, np.int64)
+    vctx.sort(dcols, types, asc, didx, n, npad)
+    assert np.array_equal(from_dev(didx, np.int64), eidx)
+    for dc, ec, t in zip(dcols, ecols, types):
+        assert np.array_equal(from_dev(dc, TYPES[t]).view(np.uint8), ec.view(np.uint8))

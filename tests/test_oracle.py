@@ -1,0 +1,265 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+
+Each check is independent of the oracle's own formula: published known-answer
+vectors, the reference's own known-answer programs and test expectations,
+closed forms (numpy sorts, stable partitions, brute-force sums) and an
+independent formulation of the same network.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+
+M64 = (1 << 64) - 1
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _mix64_py(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+# ------------------------------------------------------------- generators
+
+def test_splitmix64_published_sequence():
+    # SplitMix64 from state 0 (Vigna's splitmix64.c): first three outputs.
+    lib = oracle.lib()
+    g = 0x9E3779B97F4A7C15
+    want = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    assert [lib.orc_mix64((g * (i + 1)) & M64) for i in range(3)] == want
+
+
+def test_rand64_stream_definition():
+    # counter-based stream = SplitMix64 started at mix64(seed ^ g*(stream+1))
+    g = 0x9E3779B97F4A7C15
+    for seed, stream in [(1, 0), (2, 0), (123456789, 3), (M64, 8)]:
+        s0 = _mix64_py(seed ^ ((g * (stream + 1)) & M64))
+        for i in [0, 1, 2, 1000, 2**31 + 5]:
+            assert oracle.rand64(seed, stream, i) == _mix64_py((s0 + (i + 1) * g) & M64)
+
+
+def test_philox4x32_10_random123_kat():
+    # Random123 kat_vectors, philox4x32 with 10 rounds.
+    assert oracle.philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert oracle.philox4x32_10([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6,
+                                                                       0x6D5451FD]
+    assert oracle.philox4x32_10([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_fact_generator_ranges():
+    day, qty, price, disc = oracle.gen_fact(7, 200000)
+    assert day.min() >= 8036 and day.max() <= 10561
+    assert qty.min() >= 1 and qty.max() <= 50
+    assert price.min() >= 90000 and price.max() <= 10495000
+    assert disc.min() >= 0 and disc.max() <= 10
+    # every value of the small ranges appears
+    assert len(np.unique(disc)) == 11 and len(np.unique(qty)) == 50
+
+
+# ---------------------------------------------------------------- bitonic
+
+def _lex_sorted(keys, pay):
+    order = np.lexsort((pay, keys))
+    return keys[order], pay[order]
+
+
+@pytest.mark.parametrize("n,npad", [(0, 1), (1, 1), (1, 2), (3, 4), (5, 8), (8, 8), (11, 16), (1000, 1024),
+                                    (1024, 1024), (3000, 4096), (65536, 65536)])
+def test_bitonic_kp_equals_unique_sorted_order(n, npad):
+    k, p = oracle.gen_sort_input(11 + n, n)
+    sk, sp = oracle.bitonic_sort_kp(k, p, n, npad)
+    ek = np.concatenate([k, np.full(npad - n, M64, np.uint64)])
+    ep = np.concatenate([p, np.arange(n, npad, dtype=np.uint32)])
+    ek, ep = _lex_sorted(ek, ep)
+    assert np.array_equal(sk, ek) and np.array_equal(sp, ep)
+
+
+def test_bitonic_kp_with_duplicate_keys_ties_by_payload():
+    rng = np.random.default_rng(5)
+    k = rng.integers(0, 7, 4096).astype(np.uint64)
+    p = rng.permutation(4096).astype(np.uint32)
+    sk, sp = oracle.bitonic_sort_kp(k, p, 4096, 4096)
+    ek, ep = _lex_sorted(k, p)
+    assert np.array_equal(sk, ek) and np.array_equal(sp, ep)
+
+
+def test_bitonic_kks_known_answers():
+    # obliviator/join_kks_INT_INT/test_sort/test_sorts.cpp:12-44
+    a = np.array([3, 7, 4, 6, 0, 2, 1, 5], np.uint64)
+    b = np.array([7, 10, 8, 1, 3, 6, 9, 2, 4, 5, 0], np.uint64)
+    sa, _ = oracle.bitonic_sort_kp(a, np.arange(8, dtype=np.uint32), 8, 8)
+    sb, _ = oracle.bitonic_sort_kp(b, np.arange(11, dtype=np.uint32), 11, 16)
+    assert sa.tolist() == list(range(8))
+    assert sb[:11].tolist() == list(range(11))
+
+
+def test_bitonic_matches_reference_built_kat_program():
+    """oracle/_ref/kks_test_sorts is compiled from the reference's own sources
+    (make -C oracle ref); its printed outputs must equal the oracle's."""
+    if not os.path.exists(oracle.REF_KAT):
+        if os.path.isdir("/root/reference"):
+            subprocess.check_call(["make", "-s", "-C", os.path.dirname(oracle.REF_KAT), "ref"])
+        else:
+            pytest.skip("reference KAT program not built and /root/reference absent")
+    out = subprocess.check_output([oracle.REF_KAT], timeout=30).decode().split("\n")
+    rows = [list(map(int, out[i + 1].split())) for i, line in enumerate(out) if line.endswith("sorted:")]
+    a = np.array([3, 7, 4, 6, 0, 2, 1, 5], np.uint64)
+    b = np.array([7, 10, 8, 1, 3, 6, 9, 2, 4, 5, 0], np.uint64)
+    sa, _ = oracle.bitonic_sort_kp(a, np.arange(8, dtype=np.uint32), 8, 8)
+    sb, _ = oracle.bitonic_sort_kp(b, np.arange(11, dtype=np.uint32), 11, 16)
+    assert rows == [sa.tolist(), sb[:11].tolist()]
+
+
+def test_bitonic_trace_is_value_independent_and_counts_stages():
+    for npad in [16, 1024]:
+        t = []
+        for seed in (1, 2):
+            k, p = oracle.gen_sort_input(seed, npad - 3)
+            _, _, tr = oracle.bitonic_sort_kp(k, p, npad - 3, npad, trace=True)
+            t.append((tr.hash, tr.n_ops, tr.n_stages))
+        lg = int(np.log2(npad))
+        assert t[0] == t[1]
+        assert t[0][2] == lg * (lg + 1) // 2
+        # every stage touches every element: 2 reads + 2 writes per pair
+        assert t[0][1] == 3 + t[0][2] * npad * 2
+
+
+def test_multi_column_sort_reference_order_by_expectations():
+    g = json.load(open(os.path.join(GOLDEN, "order_by_employees.json")))
+    cols = g["columns"]
+    types = {"id": 4, "name": 8, "salary": 7, "department": 8, "age": 4, "active": 1}
+    dt = {4: np.int32, 8: np.uint64, 7: np.float64, 1: np.uint8}
+    n, npad = 8, 8
+    for case in g["cases"]:
+        names = [c for c, _ in case["order_by"]]
+        arrs = [np.array(cols[c], dt[types[c]]) for c in names]
+        out, idx = oracle.bitonic_sort_multi(arrs, [types[c] for c in names], [a for _, a in case["order_by"]],
+                                             n, npad)
+        for col, want in case["expect_prefix"].items():
+            got = out[names.index(col)] if col in names else np.array(cols[col])[idx]
+            assert got[:len(want)].tolist() == want
+
+
+def test_multi_column_sort_unique_keys_equal_lexsort():
+    rng = np.random.default_rng(3)
+    n, npad = 1000, 1024
+    a = rng.integers(-5, 5, npad).astype(np.int32)
+    b = rng.permutation(npad).astype(np.int64)  # unique second key
+    out, idx = oracle.bitonic_sort_multi([a, b], [4, 5], [True, False], n, npad)
+    # expected: (a asc, b desc) over real rows, then pad rows (max, lowest)
+    order = np.lexsort((-b[:n], a[:n]))
+    assert np.array_equal(idx[:n], order)
+    assert np.all(out[0][n:] == np.iinfo(np.int32).max) and np.all(out[1][n:] == np.iinfo(np.int64).min)
+
+
+def test_multi_column_sort_permute_consistency():
+    # sorted columns equal the input columns permuted by the index output (permute.cuh:10-22)
+    rng = np.random.default_rng(9)
+    npad = 512
+    a = rng.integers(0, 4, npad).astype(np.int16)
+    f = rng.standard_normal(npad).astype(np.float32)
+    out, idx = oracle.bitonic_sort_multi([a, f], [3, 6], [False, True], npad, npad)
+    assert np.array_equal(out[0], a[idx]) and np.array_equal(out[1], f[idx])
+    assert np.all(np.diff(out[0].astype(np.int32)) <= 0)
+
+
+# ------------------------------------------------------------- ORCompact
+
+def _stable(buf, sel):
+    return np.concatenate([buf[sel != 0], buf[sel == 0]])
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8])
+def test_orcompact_exhaustive_small(n):
+    buf = np.arange(n, dtype=np.int64) + 100
+    for mask in range(1 << n):
+        sel = np.array([(mask >> i) & 1 for i in range(n)], np.uint8)
+        out = oracle.compact(buf, sel)
+        m = int(sel.sum())
+        assert np.array_equal(out[:m], buf[sel != 0])
+        assert sorted(out[m:].tolist()) == sorted(buf[sel == 0].tolist())
+        if n & (n - 1) == 0:
+            assert np.array_equal(out, oracle.compact_levels(buf, sel))
+
+
+@pytest.mark.parametrize("n,p", [(1 << 12, 0.02), (1 << 13, 0.5), (1 << 14, 0.97), (3000, 0.3), (12345, 0.1)])
+def test_orcompact_stable_prefix_and_closed_form(n, p):
+    rng = np.random.default_rng(n)
+    buf = np.stack([np.arange(n, dtype=np.int64), rng.integers(0, 1 << 40, n)], axis=1)
+    sel = (rng.random(n) < p).astype(np.uint8)
+    out = oracle.compact(buf, sel)
+    m = int(sel.sum())
+    assert np.array_equal(out[:m], buf[sel != 0])
+    assert sorted(out[m:, 0].tolist()) == sorted(buf[sel == 0, 0].tolist())
+    if n & (n - 1) == 0:
+        assert np.array_equal(out, oracle.compact_levels(buf, sel))
+
+
+# ------------------------------------------------------- filter + int128
+
+@pytest.mark.parametrize("n,npad", [(0, 1), (1000, 1024), (60000, 65536), (1 << 16, 1 << 16)])
+def test_filter_sum_brute_force(n, npad):
+    day, qty, price, disc = oracle.gen_fact(3, n)
+    m, s, op, od = oracle.filter_sum(day, qty, price, disc, n, npad, want_rows=True)
+    sel = (day >= 8766) & (day < 9131) & (disc >= 5) & (disc <= 7) & (qty < 24)
+    assert m == int(sel.sum())
+    assert s == sum(int(a) * int(b) for a, b in zip(price[sel], disc[sel]))
+    assert np.array_equal(op[:m], price[sel]) and np.array_equal(od[:m], disc[sel])
+
+
+def test_filter_selectivity_near_expected():
+    day, qty, price, disc = oracle.gen_fact(1, 400000)
+    sel = oracle.filter_mask(day, qty, disc, 400000, 400000)
+    frac = sel.mean()
+    expect = 365 / 2526 * 3 / 11 * 23 / 50
+    assert abs(frac - expect) < 0.002
+
+
+def test_filter_int128_exceeds_int64():
+    # the int128 path must hold sums beyond 2^63: scale a copy of the prices
+    n = 1024
+    day = np.full(n, 8800, np.int32)
+    qty = np.ones(n, np.int64)
+    price = np.full(n, (1 << 62) - 1, np.int64)
+    disc = np.full(n, 6, np.int64)
+    m, s = oracle.filter_sum(day, qty, price, disc, n, n)
+    assert m == n and s == n * ((1 << 62) - 1) * 6
+
+
+# --------------------------------------------------------- join + group
+
+@pytest.mark.parametrize("n_dim,n_fact,npad", [(100, 400, 512), (1500, 6000, 8192), (5000, 3000, 8192)])
+def test_join_groupsum_brute_force(n_dim, n_fact, npad):
+    dk, dg, fk, fv = oracle.gen_join(4, n_dim, n_fact)
+    assert sorted(dk.tolist()) == list(range(1, n_dim + 1))
+    groups, sums, sk, sv = oracle.join_groupsum(dk, dg, fk, fv, npad, want_sorted=True)
+    g_of_key = np.zeros(n_dim + 1, np.int64)
+    g_of_key[dk] = dg
+    want = {}
+    for k, v in zip(fk.tolist(), fv.tolist()):
+        g = int(g_of_key[k])
+        want[g] = want.get(g, 0) + v
+    assert groups.tolist() == sorted(want)
+    assert sums.tolist() == [want[g] for g in sorted(want)]
+    # sorted union equals numpy's sort of the union keys
+    lib = oracle.lib()
+    keys = [lib.orc_union_key(int(k), 0, i) for i, k in enumerate(dk)] + \
+           [lib.orc_union_key(int(k), 1, i) for i, k in enumerate(fk)]
+    keys = np.array(keys + [M64] * (npad - n_dim - n_fact), np.uint64)
+    assert np.array_equal(sk, np.sort(keys))
+
+
+def test_join_with_unmatched_fact_rows():
+    # fact keys outside the dimension never join
+    dk = np.array([1, 2, 3], np.int32)
+    dg = np.array([0, 1, 0], np.int32)
+    fk = np.array([1, 4, 3, 3, 9, 2], np.int32)
+    fv = np.array([10, 1000, 5, 7, 1000, 3], np.int64)
+    groups, sums = oracle.join_groupsum(dk, dg, fk, fv, 16)
+    assert groups.tolist() == [0, 1] and sums.tolist() == [22, 3]

@@ -1,0 +1,197 @@
+"""velodb_amd -- MI355X (gfx950) oblivious-operator library, Python host side.
+
+The compute path is libvelodb_amd.so (hand-written HIP kernels behind the C
+ABI of include/vo_api.h).  This module only loads it through ctypes and wraps
+the calls; there is no CPU fallback: if the library is missing, loading fails
+with an error.
+"""
+import ctypes as C
+import os
+
+from . import _abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvelodb_amd.so")
+
+_lib = None
+
+
+class VoError(RuntimeError):
+    """A non-zero vo_status from the library, with vo_last_error's text."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"velodb_amd status {status}: {msg}")
+        self.status = status
+
+
+def load_library(path=None):
+    """Load and bind libvelodb_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"{p} is missing: build it with __graft_entry__.build() "
+                          f"(make -C velodb_amd/csrc)")
+    lib = _abi.bind(C.CDLL(p))
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(x):
+    """Device pointer of a torch tensor (or an int address / None)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+class Context:
+    """One vo_ctx: a HIP stream plus submission bookkeeping.
+
+    stream: a hipStream_t address (e.g. torch.cuda.current_stream().cuda_stream)
+    so that library work is ordered with torch's copies; None lets the library
+    create its own stream.
+    """
+
+    def __init__(self, stream=None, flags=0, device=-1, queue_capacity=0, wait_timeout_ms=0, tuning=0):
+        self.lib = load_library()
+        cfg = _abi.Config(queue_capacity, flags, stream, device, wait_timeout_ms, tuning)
+        h = C.c_void_p()
+        st = self.lib.vo_ctx_create(C.byref(cfg), C.byref(h))
+        if st != 0:
+            raise VoError(st, self.lib.vo_status_string(st).decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.vo_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st):
+        if st != 0:
+            raise VoError(st, self.lib.vo_last_error(self.h).decode())
+        return st
+
+    # ---- ops -------------------------------------------------------------
+    def sort_kp(self, keys, payload, n, n_pad):
+        return self.check(self.lib.vo_sort_kp(self.h, _ptr(keys), _ptr(payload), n, n_pad))
+
+    def sort_kp_async(self, keys, payload, n, n_pad):
+        return self.check(self.lib.vo_sort_kp_async(self.h, _ptr(keys), _ptr(payload), n, n_pad))
+
+    def sort(self, cols, types, ascending, indices, n_rows, n_padded_rows):
+        a = _abi.SortArgs()
+        for i, c in enumerate(cols):
+            a.sort_cols[i] = _ptr(c)
+            a.col_types[i] = types[i]
+            a.ascending_flags[i] = 1 if ascending[i] else 0
+        a.indices = _ptr(indices)
+        a.n_sort_columns = len(cols)
+        a.n_rows = n_rows
+        a.n_padded_rows = n_padded_rows
+        return self.check(self.lib.vo_sort(self.h, C.byref(a)))
+
+    def permute(self, out, inp, idx, n, type_id):
+        a = _abi.PermuteArgs(_ptr(out), _ptr(inp), _ptr(idx), n, type_id)
+        return self.check(self.lib.vo_permute(self.h, C.byref(a)))
+
+    def compact(self, mask, cols, col_bytes, n, n_pad, out_count=None):
+        a = _abi.CompactArgs()
+        a.mask = _ptr(mask)
+        for i, c in enumerate(cols):
+            a.cols[i] = _ptr(c)
+            a.col_bytes[i] = col_bytes[i]
+        a.n_cols = len(cols)
+        a.n, a.n_pad = n, n_pad
+        a.out_count = _ptr(out_count)
+        return self.check(self.lib.vo_compact(self.h, C.byref(a)))
+
+    def shuffle(self, payload, n_pad, seed, perm_out=None):
+        a = _abi.ShuffleArgs(_ptr(payload), _ptr(perm_out), n_pad, seed)
+        return self.check(self.lib.vo_shuffle(self.h, C.byref(a)))
+
+    def filter_sum(self, day, qty, price, disc, n, n_pad, out, out_price=None, out_disc=None,
+                   day_lo=8766, day_hi=9131, disc_lo=5, disc_hi=7, qty_lt=24):
+        a = _abi.FilterArgs(_ptr(day), _ptr(qty), _ptr(price), _ptr(disc), _ptr(out_price), _ptr(out_disc),
+                            n, n_pad, day_lo, day_hi, disc_lo, disc_hi, qty_lt, _ptr(out))
+        return self.check(self.lib.vo_filter_sum(self.h, C.byref(a)))
+
+    def join_groupsum(self, dim_key, dim_group, n_dim, fact_fk, fact_val, n_fact, n_pad,
+                      out_group, out_sum, out_n_groups, max_groups, sorted_keys=None, sorted_vals=None):
+        a = _abi.JoinArgs(_ptr(dim_key), _ptr(dim_group), n_dim, _ptr(fact_fk), _ptr(fact_val), n_fact, n_pad,
+                          _ptr(out_group), _ptr(out_sum), _ptr(out_n_groups), max_groups,
+                          _ptr(sorted_keys), _ptr(sorted_vals))
+        return self.check(self.lib.vo_pkfk_join_groupsum(self.h, C.byref(a)))
+
+    def sort_batch(self, keys, payload, items):
+        arr = (_abi.BatchItem * len(items))(*[_abi.BatchItem(o, n, p) for (o, n, p) in items])
+        a = _abi.SortBatchArgs(_ptr(keys), _ptr(payload), C.cast(arr, C.c_void_p), len(items))
+        return self.check(self.lib.vo_sort_batch(self.h, C.byref(a)))
+
+    def submit(self, cmd):
+        s = self.lib.vo_submit(self.h, C.byref(cmd))
+        if s == 0:
+            raise VoError(-1, self.lib.vo_last_error(self.h).decode())
+        return s
+
+    def wait(self, seq, timeout_ms=0):
+        return self.check(self.lib.vo_wait(self.h, seq, timeout_ms))
+
+    def sync(self):
+        return self.check(self.lib.vo_sync(self.h))
+
+    def gen_sort_input(self, seed, keys, payload, n):
+        return self.check(self.lib.vo_gen_sort_input(self.h, seed, _ptr(keys), _ptr(payload), n))
+
+    def gen_fact(self, seed, day, qty, price, disc, n):
+        return self.check(self.lib.vo_gen_fact(self.h, seed, _ptr(day), _ptr(qty), _ptr(price), _ptr(disc), n))
+
+    # ---- diagnostics -----------------------------------------------------
+    def trace_begin(self):
+        return self.check(self.lib.vo_trace_begin(self.h))
+
+    def trace_end(self):
+        t = _abi.Trace()
+        self.check(self.lib.vo_trace_end(self.h, C.byref(t)))
+        return t
+
+    def trace_launches(self, n):
+        out = []
+        name = C.create_string_buffer(64)
+        g = (C.c_uint32 * 3)()
+        b = (C.c_uint32 * 3)()
+        d = C.c_uint64()
+        for i in range(n):
+            self.check(self.lib.vo_trace_launch(self.h, i, name, 64, g, b, C.byref(d)))
+            out.append((name.value.decode(), tuple(g), tuple(b), d.value))
+        return out
+
+    def kernel_stats(self, family=None):
+        n = C.c_uint64()
+        ms = C.c_double()
+        fam = family.encode() if family else None
+        self.check(self.lib.vo_kernel_stats(self.h, fam, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def kernel_stats_reset(self):
+        return self.check(self.lib.vo_kernel_stats_reset(self.h))
+
+    def sort_plan(self, n_pad):
+        n = C.c_uint32()
+        self.check(self.lib.vo_sort_plan(self.h, n_pad, C.byref(n)))
+        return n.value

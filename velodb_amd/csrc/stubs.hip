@@ -1,0 +1,24 @@
+// stubs.hip -- operators not built yet report VO_ERR_UNSUPPORTED.
+#include "vo_internal.hpp"
+
+namespace vo {
+
+void op_scatter(vo_ctx *, const vo_scatter_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_SCATTER is not supported yet"}; }
+void op_compact(vo_ctx *, const vo_compact_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_COMPACT is not supported yet"}; }
+void op_shuffle(vo_ctx *, const vo_shuffle_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_SHUFFLE is not supported yet"}; }
+void op_filter_sum(vo_ctx *, const vo_filter_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_FILTER_SUM is not supported yet"}; }
+void op_join_groupsum(vo_ctx *, const vo_join_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_PKFK_JOIN_GROUPSUM is not supported yet"}; }
+void op_sort_batch(vo_ctx *, const vo_sort_batch_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_SORT_BATCH is not supported yet"}; }
+
+uint64_t queue_submit(vo_ctx *ctx, vo_command *) { ctx->last_error = "queue not built"; ctx->last_status = VO_ERR_QUEUE; return 0; }
+int queue_wait(vo_ctx *, uint64_t, uint32_t) { return VO_ERR_QUEUE; }
+int queue_poll(vo_ctx *, uint64_t) { return -VO_ERR_QUEUE; }
+int queue_sync(vo_ctx *) { return VO_ERR_QUEUE; }
+
+} // namespace vo
+
+extern "C" {
+int vo_queue_start(vo_ctx *) { return VO_ERR_UNSUPPORTED; }
+int vo_queue_stop(vo_ctx *, uint32_t) { return VO_OK; }
+int vo_queue_running(const vo_ctx *) { return 0; }
+}

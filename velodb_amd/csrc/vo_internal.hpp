@@ -1,0 +1,246 @@
+// vo_internal.hpp -- context, launch bookkeeping and device helpers shared by
+// every translation unit of libvelodb_amd.so.  gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vo_api.h"
+
+namespace vo {
+
+// ------------------------------------------------------------------ errors
+
+struct Error {
+    int status;
+    std::string msg;
+};
+
+#define VO_HIP(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw ::vo::Error{VO_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)};     \
+    } while (0)
+
+#define VO_CHECK(cond, status, msg)                                                               \
+    do {                                                                                          \
+        if (!(cond)) throw ::vo::Error{(status), (msg)};                                          \
+    } while (0)
+
+// ------------------------------------------------------------------ tracing
+
+// Per-launch record kept while a trace is open.
+struct LaunchRecord {
+    std::string name;
+    uint32_t grid[3];
+    uint32_t block[3];
+    uint32_t lds;
+    uint64_t scalar_hash; // hash of the launch's non-pointer arguments
+    uint64_t addr_digest; // filled at trace end from the device slot
+    uint64_t n_addr;
+};
+
+// Per-launch device slot written by instrumented kernels: {sum of mixed
+// per-thread address hashes, number of addresses}.
+struct TraceSlot {
+    unsigned long long digest;
+    unsigned long long count;
+};
+
+struct TimingRecord {
+    std::string family;
+    hipEvent_t start, stop;
+};
+
+// ------------------------------------------------------------------ tuning
+
+// Public-size-only schedule parameters (chosen by the autotuner or defaults).
+struct SortTuning {
+    int max_fused_stages = 4; // stages per global HBM pass (1..5)
+    int use_hybrid = 1;       // LDS-staged multi-stage global passes
+    int hybrid_col_log = 6;   // contiguous run (log2 elements) per hybrid column
+};
+
+} // namespace vo
+
+// ------------------------------------------------------------------ context
+
+struct vo_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    uint32_t flags = 0;
+    uint32_t queue_capacity = 1024;
+    uint32_t wait_timeout_ms = 60000;
+    int num_cus = 0;
+    std::string last_error;
+    int last_status = 0;
+    std::mutex mu; // guards submission and bookkeeping
+
+    // completion tracking (launch mode)
+    uint64_t next_seq = 1;
+    uint64_t done_seq = 0;
+    std::vector<std::pair<uint64_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> event_pool;
+
+    // scratch arena
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+
+    // tracing
+    bool tracing = false;
+    std::vector<vo::LaunchRecord> trace_launches;
+    vo::TraceSlot *trace_slots = nullptr; // device
+    size_t trace_capacity = 0;
+    std::vector<vo::LaunchRecord> last_trace;
+
+    // timing
+    std::vector<vo::TimingRecord> timings;
+
+    vo::SortTuning tuning;
+
+    // persistent queue (queue.hip)
+    void *queue = nullptr;
+};
+
+namespace vo {
+
+// Scratch arena: returns at least `bytes` of device memory, valid until the
+// next call on this context.  Synchronises the stream before growing.
+void *scratch(vo_ctx *ctx, size_t bytes);
+
+// Trace slot for the next launch (nullptr when not tracing).
+TraceSlot *trace_slot(vo_ctx *ctx);
+
+// Record a launch (trace + timing bookkeeping).  `pre` is called before the
+// kernel launch (after the timing start event); returns nothing.
+struct LaunchInfo {
+    const char *family;
+    dim3 grid, block;
+    uint32_t lds;
+    uint64_t scalar_hash;
+};
+
+void begin_launch(vo_ctx *ctx, const LaunchInfo &li, hipEvent_t *ev_stop);
+void end_launch(vo_ctx *ctx, hipEvent_t ev_stop);
+
+template <typename K, typename... Args>
+inline void launch(vo_ctx *ctx, const LaunchInfo &li, K kernel, Args... args)
+{
+    hipEvent_t ev = nullptr;
+    begin_launch(ctx, li, &ev);
+    hipLaunchKernelGGL(kernel, li.grid, li.block, li.lds, ctx->stream, args...);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        throw Error{VO_ERR_HIP, std::string("launch ") + li.family + ": " + hipGetErrorString(e)};
+    end_launch(ctx, ev);
+}
+
+inline uint64_t hash_mix(uint64_t h, uint64_t v)
+{
+    h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    h ^= h >> 29;
+    h *= 0xBF58476D1CE4E5B9ull;
+    return h ^ (h >> 32);
+}
+
+inline int ilog2(uint64_t v)
+{
+    int r = -1;
+    while (v) {
+        v >>= 1;
+        ++r;
+    }
+    return r;
+}
+
+inline bool is_pow2(uint64_t v) { return v && !(v & (v - 1)); }
+
+// ---- op entry points implemented in the .hip files (enqueue only)
+void op_sort_kp(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad);
+void op_sort_multi(vo_ctx *ctx, const vo_sort_args &a);
+void op_sort_batch(vo_ctx *ctx, const vo_sort_batch_args &a);
+void op_permute(vo_ctx *ctx, const vo_permute_args &a);
+void op_scatter(vo_ctx *ctx, const vo_scatter_args &a);
+void op_gather(vo_ctx *ctx, const vo_gather_args &a);
+void op_compare_eq(vo_ctx *ctx, const vo_compare_eq_args &a);
+void op_compose(vo_ctx *ctx, const vo_compose_args &a);
+void op_enrich(vo_ctx *ctx, const vo_enrich_args &a);
+void op_compact(vo_ctx *ctx, const vo_compact_args &a);
+void op_shuffle(vo_ctx *ctx, const vo_shuffle_args &a);
+void op_filter_sum(vo_ctx *ctx, const vo_filter_args &a);
+void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a);
+void op_gen_sort_input(vo_ctx *ctx, uint64_t seed, uint64_t *keys, uint32_t *payload, uint64_t n);
+void op_gen_fact(vo_ctx *ctx, uint64_t seed, int32_t *day, int64_t *qty, int64_t *price, int64_t *disc,
+                 uint64_t n);
+int sort_plan_passes(vo_ctx *ctx, uint64_t n_pad);
+
+// ---- resident command queue (queue.hip); used when ctx->queue is set
+uint64_t queue_submit(vo_ctx *ctx, vo_command *cmd);
+int queue_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms);
+int queue_poll(vo_ctx *ctx, uint64_t seq);
+int queue_sync(vo_ctx *ctx);
+
+// ---- shared device helpers (sort.hip) used by other ops
+// Sort (u64 key, u64 value) records by (key, value) ascending, n_pad = 2^k.
+void sort_kv(vo_ctx *ctx, uint64_t *keys, uint64_t *vals, uint64_t n_pad);
+// Sort (u64 key, u32 payload) without padding (all n_pad rows are live).
+void sort_kp_full(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n_pad);
+
+} // namespace vo
+
+// ------------------------------------------------------------ device side
+
+namespace vo {
+namespace dev {
+
+// Address-trace folding used by the instrumented kernel builds.
+__device__ __forceinline__ uint64_t tfold(uint64_t h, uint64_t tag, uint64_t idx)
+{
+    uint64_t v = (idx << 4) ^ tag;
+    h ^= v * 0x9E3779B97F4A7C15ull;
+    h = (h << 27) | (h >> 37);
+    return h * 0xBF58476D1CE4E5B9ull + 0x94D049BB133111EBull;
+}
+
+// Fold a thread's final hash into the launch slot (order independent sum).
+__device__ __forceinline__ void tcommit(TraceSlot *slot, uint64_t gtid, uint64_t h, uint64_t n)
+{
+    uint64_t z = h ^ (gtid * 0xD1B54A32D192ED03ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    atomicAdd(&slot->digest, (unsigned long long)z);
+    atomicAdd(&slot->count, (unsigned long long)n);
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t rand64(uint64_t seed, uint32_t stream, uint64_t i)
+{
+    const uint64_t g = 0x9E3779B97F4A7C15ull;
+    uint64_t s0 = mix64(seed ^ (g * ((uint64_t)stream + 1)));
+    return mix64(s0 + (i + 1) * g);
+}
+
+__device__ __forceinline__ int64_t rand_range(uint64_t seed, uint32_t stream, uint64_t i, int64_t lo,
+                                              uint64_t range)
+{
+    uint64_t r32 = rand64(seed, stream, i) >> 32;
+    return lo + (int64_t)((r32 * range) >> 32);
+}
+
+} // namespace dev
+} // namespace vo
