@@ -56,8 +56,50 @@ __device__ __forceinline__ bool rec_gt(const R &a, const R &b)
 // Compare-exchange: afterwards a <= b when asc, a >= b otherwise.  Ties
 // (equal over every compared field) are left in place, as needSwap does
 // (sort.cuh:31-40).
+// The comparison runs on wavefront lane masks: each field costs one v_cmp
+// for > and one for <, the lexicographic combination and the direction are
+// scalar mask operations, and the swap is v_cndmask on the resulting mask
+// (2 per 32-bit word).  `ascm` holds the lanes whose pair sorts ascending.
+template <class R>
+__device__ __forceinline__ void cmpx_m(R &a, R &b, uint64_t ascm)
+{
+    uint64_t gt = 0, lt = 0, eq = ~0ull;
+#pragma unroll
+    for (int w = 0; w < R::KW; ++w) {
+        const uint64_t g = __builtin_amdgcn_ballot_w64(a.k[w] > b.k[w]);
+        const uint64_t l = __builtin_amdgcn_ballot_w64(a.k[w] < b.k[w]);
+        gt |= eq & g;
+        lt |= eq & l;
+        eq &= ~(g | l);
+    }
+    if constexpr (R::TIE) {
+        const uint64_t g = __builtin_amdgcn_ballot_w64(a.p > b.p);
+        const uint64_t l = __builtin_amdgcn_ballot_w64(a.p < b.p);
+        gt |= eq & g;
+        lt |= eq & l;
+    }
+    const bool sw = __builtin_amdgcn_inverse_ballot_w64((ascm & gt) | (~ascm & lt));
+#pragma unroll
+    for (int w = 0; w < R::KW; ++w) {
+        const uint64_t x = a.k[w], y = b.k[w];
+        a.k[w] = sw ? y : x;
+        b.k[w] = sw ? x : y;
+    }
+    const typename R::P x = a.p, y = b.p;
+    a.p = sw ? y : x;
+    b.p = sw ? x : y;
+}
+
 template <class R>
 __device__ __forceinline__ void cmpx(R &a, R &b, bool asc)
+{
+    cmpx_m(a, b, __builtin_amdgcn_ballot_w64(asc));
+}
+
+// Scalar reference form of the same exchange (used by the one-thread kernel,
+// where lane masks would not describe the per-iteration pairs).
+template <class R>
+__device__ __forceinline__ void cmpx_scalar(R &a, R &b, bool asc)
 {
     const bool sw = asc ? rec_gt(a, b) : rec_gt(b, a);
 #pragma unroll
@@ -246,6 +288,186 @@ __device__ __forceinline__ void tile_level(R (&r)[1 << R_LOG], const Lds<R> &s, 
         q = a - 1;
     }
 }
+
+// ------------------------------------------------- compile-time tile schedule
+
+// Same schedule as tile_level, with the tile size, every layout and every
+// level fixed at compile time, so that all LDS offsets are immediates and the
+// stage loops fully unroll.  Used for full 2^T_LOG tiles.
+template <class R, int T_LOG, int R_LOG, bool TRACE>
+struct TileCT {
+    static constexpr int RR = 1 << R_LOG;
+    static constexpr uint32_t T = 1u << T_LOG;
+    static constexpr uint32_t TPAD = T + (T >> 4);
+    using P = typename R::P;
+
+    // layout reached after stages (q_start .. 0) starting from layout a
+    static constexpr int end_layout(int q, int a)
+    {
+        while (q >= 0) {
+            if (!(a <= q && q < a + R_LOG)) a = q - R_LOG + 1 > 0 ? q - R_LOG + 1 : 0;
+            q = a - 1;
+        }
+        return a;
+    }
+
+    template <int A>
+    static __device__ __forceinline__ void put(const R (&r)[RR], char *lds, uint32_t tau, uint64_t &h, uint32_t &n)
+    {
+        uint64_t *sk = reinterpret_cast<uint64_t *>(lds);
+        P *sp = reinterpret_cast<P *>(lds + sizeof(uint64_t) * R::KW * TPAD);
+        const uint32_t lb = lpos(lay_base<R_LOG>(tau, A));
+#pragma unroll
+        for (int m = 0; m < RR; ++m) {
+            const uint32_t l = lb + lpos((uint32_t)m << A);
+#pragma unroll
+            for (int w = 0; w < R::KW; ++w) sk[w * TPAD + l] = r[m].k[w];
+            sp[l] = r[m].p;
+            if constexpr (TRACE) {
+                h = tfold(h, 3, l);
+                ++n;
+            }
+        }
+    }
+
+    template <int A>
+    static __device__ __forceinline__ void get(R (&r)[RR], char *lds, uint32_t tau, uint64_t &h, uint32_t &n)
+    {
+        const uint64_t *sk = reinterpret_cast<const uint64_t *>(lds);
+        const P *sp = reinterpret_cast<const P *>(lds + sizeof(uint64_t) * R::KW * TPAD);
+        const uint32_t lb = lpos(lay_base<R_LOG>(tau, A));
+#pragma unroll
+        for (int m = 0; m < RR; ++m) {
+            const uint32_t l = lb + lpos((uint32_t)m << A);
+#pragma unroll
+            for (int w = 0; w < R::KW; ++w) r[m].k[w] = sk[w * TPAD + l];
+            r[m].p = sp[l];
+            if constexpr (TRACE) {
+                h = tfold(h, 4, l);
+                ++n;
+            }
+        }
+    }
+
+    // LDS bytes of one block: a single column buffer (the widest column)
+    // that every column passes through in turn.  Keeping it to 8 B per slot
+    // lets two 2^13-record tiles share a CU's 160 KB.
+    static constexpr size_t lds_bytes() { return (size_t)TPAD * sizeof(uint64_t); }
+
+    template <int A, typename C>
+    static __device__ __forceinline__ void put_col(const C (&v)[RR], C *buf, uint32_t tau, uint64_t &h, uint32_t &n)
+    {
+        const uint32_t lb = lpos(lay_base<R_LOG>(tau, A));
+#pragma unroll
+        for (int m = 0; m < RR; ++m) {
+            const uint32_t l = lb + lpos((uint32_t)m << A);
+            buf[l] = v[m];
+            if constexpr (TRACE) {
+                h = tfold(h, 3, l);
+                ++n;
+            }
+        }
+    }
+
+    template <int A, typename C>
+    static __device__ __forceinline__ void get_col(C (&v)[RR], const C *buf, uint32_t tau, uint64_t &h, uint32_t &n)
+    {
+        const uint32_t lb = lpos(lay_base<R_LOG>(tau, A));
+#pragma unroll
+        for (int m = 0; m < RR; ++m) {
+            const uint32_t l = lb + lpos((uint32_t)m << A);
+            v[m] = buf[l];
+            if constexpr (TRACE) {
+                h = tfold(h, 4, l);
+                ++n;
+            }
+        }
+    }
+
+    template <int A_OLD, int A_NEW>
+    static __device__ __forceinline__ void relayout(R (&r)[RR], char *lds, uint32_t tau, uint64_t &h, uint32_t &n)
+    {
+        if constexpr (A_OLD != A_NEW) {
+#pragma unroll
+            for (int w = 0; w < R::KW; ++w) {
+                uint64_t v[RR];
+#pragma unroll
+                for (int m = 0; m < RR; ++m) v[m] = r[m].k[w];
+                __syncthreads();
+                put_col<A_OLD>(v, reinterpret_cast<uint64_t *>(lds), tau, h, n);
+                __syncthreads();
+                get_col<A_NEW>(v, reinterpret_cast<const uint64_t *>(lds), tau, h, n);
+#pragma unroll
+                for (int m = 0; m < RR; ++m) r[m].k[w] = v[m];
+            }
+            P v[RR];
+#pragma unroll
+            for (int m = 0; m < RR; ++m) v[m] = r[m].p;
+            __syncthreads();
+            put_col<A_OLD>(v, reinterpret_cast<P *>(lds), tau, h, n);
+            __syncthreads();
+            get_col<A_NEW>(v, reinterpret_cast<const P *>(lds), tau, h, n);
+#pragma unroll
+            for (int m = 0; m < RR; ++m) r[m].p = v[m];
+        }
+    }
+
+    // Stages q = QHI .. QLO of level P (P >= 0 compile-time; P < 0 means a
+    // level above the tile whose direction `asc_hi` is uniform per block).
+    template <int P, int A, int QHI, int QLO>
+    static __device__ __forceinline__ void stages(R (&r)[RR], uint32_t tau, bool asc_hi, int p_top)
+    {
+        const uint32_t base = lay_base<R_LOG>(tau, A);
+#pragma unroll
+        for (int q = QHI; q >= QLO; --q) {
+            const int qq = q - A;
+#pragma unroll
+            for (int m = 0; m < RR; ++m) {
+                if (m & (1 << qq)) continue;
+                bool asc;
+                if constexpr (P < 0) {
+                    asc = asc_hi;
+                } else {
+                    const uint32_t l = base | ((uint32_t)m << A);
+                    asc = (P >= p_top) | (P >= T_LOG ? asc_hi : (((l >> P) & 1u) == 0));
+                }
+                cmpx(r[m], r[m | (1 << qq)], asc);
+            }
+        }
+    }
+
+    // Stages (P, Q .. 0) from layout A, changing layout as needed.
+    template <int P, int Q, int A>
+    static __device__ __forceinline__ void level(R (&r)[RR], char *lds, uint32_t tau, bool asc_hi, int p_top,
+                                                 uint64_t &h, uint32_t &n)
+    {
+        if constexpr (Q >= 0) {
+            constexpr bool covered = (A <= Q && Q < A + R_LOG);
+            constexpr int AN = covered ? A : (Q - R_LOG + 1 > 0 ? Q - R_LOG + 1 : 0);
+            relayout<A, AN>(r, lds, tau, h, n);
+            stages<P, AN, Q, AN>(r, tau, asc_hi, p_top);
+            level<P, AN - 1, AN>(r, lds, tau, asc_hi, p_top, h, n);
+        }
+    }
+
+    // Levels P .. T_LOG of a full local sort from layout A; returns through
+    // the template chain and leaves the tile in layout end_all(P, A).
+    static constexpr int end_all(int p, int a)
+    {
+        for (; p <= T_LOG; ++p) a = end_layout(p - 1, a);
+        return a;
+    }
+
+    template <int P, int A>
+    static __device__ __forceinline__ void levels(R (&r)[RR], char *lds, uint32_t tau, bool asc_hi, int p_top,
+                                                  uint64_t &h, uint32_t &n)
+    {
+        if constexpr (P <= T_LOG) {
+            level<P, P - 1, A>(r, lds, tau, asc_hi, p_top, h, n);
+            levels<P + 1, end_layout(P - 1, A)>(r, lds, tau, asc_hi, p_top, h, n);
+        }
+    }
+};
 
 } // namespace dev
 } // namespace vo

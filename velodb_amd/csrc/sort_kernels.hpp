@@ -1,0 +1,384 @@
+// sort_kernels.hpp -- oblivious bitonic sort kernels and their pass schedule.
+//
+// Reference: executeBitonicSort (include/cuda/sort.cuh:70-118) runs one
+// grid-wide pass per (k, j) stage, S = log2N(log2N+1)/2 passes.  Here the
+// same compare-exchange network is grouped into far fewer HBM passes:
+//   pass 0       k_sort_local_ct : every tile of 2^T_LOG records fully
+//                                  sorted (levels p <= T_LOG), registers+LDS;
+//   per level p > T_LOG:
+//                k_sort_global   : stages q >= T_LOG, up to S_MAX per pass;
+//                k_sort_merge    : stages q < T_LOG in one tile pass.
+// The schedule is a function of n_pad only.
+//
+// Addressing: every block's records sit at one uniform (scalar) base plus a
+// per-thread offset below 2^13, so loads and stores use the scalar-base +
+// 32-bit vector-offset form and the address costs one VGPR.  Batched sorts
+// map a block's uniform virtual base through a per-instance offset table
+// (instances are never smaller than a tile on these kernels).
+#pragma once
+
+#include "sort_core.hpp"
+
+#include <algorithm>
+
+namespace vo {
+namespace dev {
+
+// Uniform virtual index -> physical index (identity without a table).
+__device__ __forceinline__ uint64_t map_uniform(const IMap &m, uint64_t v)
+{
+    if (m.offsets == nullptr) return v;
+    return m.offsets[v >> m.c] + (v & ((1ull << m.c) - 1));
+}
+
+// Raw buffer access: the uniform base goes into a buffer resource (scalar
+// registers), the per-thread part is a 32-bit byte offset.  Dword 3 of the
+// resource selects the 32-bit data format (0x00020000) used for untyped
+// loads on gfx9-family parts.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *ubase)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(ubase), (short)0, 0x7FFFFFFF, 0x00020000);
+}
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ T ld_u(const T *ubase, uint32_t vi)
+{
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(ubase);
+    if constexpr (sizeof(T) == 8) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vi * 8u, 0, 0));
+    } else {
+        static_assert(sizeof(T) == 4, "4- or 8-byte columns");
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vi * 4u, 0, 0));
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void st_u(T *ubase, uint32_t vi, T v)
+{
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(ubase);
+    if constexpr (sizeof(T) == 8) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vi * 8u, 0, 0);
+    } else {
+        static_assert(sizeof(T) == 4, "4- or 8-byte columns");
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vi * 4u, 0, 0);
+    }
+}
+
+// Record at uniform index u plus per-thread offset vi.
+template <class R, bool TRACE>
+__device__ __forceinline__ void gload_u(R &r, const Bufs<R> &b, uint64_t u, uint32_t vi, uint64_t &h, uint32_t &n)
+{
+#pragma unroll
+    for (int w = 0; w < R::KW; ++w) r.k[w] = ld_u(b.k[w] + u, vi);
+    r.p = ld_u(b.p + u, vi);
+    if constexpr (TRACE) {
+        h = tfold(h, 1, u + vi);
+        ++n;
+    }
+}
+
+template <class R, bool TRACE>
+__device__ __forceinline__ void gstore_u(const R &r, const Bufs<R> &b, uint64_t u, uint32_t vi, uint64_t &h,
+                                         uint32_t &n)
+{
+#pragma unroll
+    for (int w = 0; w < R::KW; ++w) st_u(b.k[w] + u, vi, r.k[w]);
+    st_u(b.p + u, vi, r.p);
+    if constexpr (TRACE) {
+        h = tfold(h, 2, u + vi);
+        ++n;
+    }
+}
+
+// ----------------------------------------------------------- tiny arrays
+
+// n_pad <= 8: one thread runs the network directly.
+template <class R, bool TRACE>
+__global__ void k_sort_tiny(Bufs<R> b, int n_log, uint64_t n_valid, int fill, TraceSlot *slot)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t h = 0;
+    uint32_t nops = 0;
+    const uint32_t N = 1u << n_log;
+    R r[8];
+    for (uint32_t i = 0; i < 8; ++i) {
+        if (i < N) {
+            gload<R, TRACE>(r[i], b, i, h, nops);
+            if (fill) pad_select(r[i], i, n_valid);
+        }
+    }
+    for (int p = 1; p <= n_log; ++p)
+        for (int q = p - 1; q >= 0; --q)
+            for (uint32_t i = 0; i < N; ++i) {
+                const uint32_t ixj = i ^ (1u << q);
+                if (i < ixj) cmpx_scalar(r[i], r[ixj], dir_asc(i, p, n_log));
+            }
+    for (uint32_t i = 0; i < 8; ++i)
+        if (i < N) gstore<R, TRACE>(r[i], b, i, h, nops);
+    if constexpr (TRACE) tcommit(slot, 0, h, nops);
+}
+
+// ---------------------------------------------------------- tile kernels
+
+// Tiles of 2^t records for t < T_LOG (small arrays): levels 1 .. p_max with
+// runtime layouts.  Loads and stores use the coalesced layout a = t - R_LOG.
+template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
+__global__ __launch_bounds__(BLOCK) void k_sort_local(Bufs<R> b, int t, int p_max, int p_top, uint64_t n_valid,
+                                                      int fill, TraceSlot *slot)
+{
+    extern __shared__ __align__(16) char lds_raw[];
+    const uint32_t T = 1u << t;
+    const Lds<R> s = lds_view<R>(lds_raw, T + (T >> 4));
+    const uint32_t tau = threadIdx.x;
+    const bool act = tau < (T >> R_LOG);
+    const uint64_t vbase = (uint64_t)blockIdx.x << t;
+    uint64_t h = 0;
+    uint32_t nops = 0;
+    R r[1 << R_LOG];
+    int a = t - R_LOG;
+    if (act) {
+        const uint32_t base = lay_base<R_LOG>(tau, a);
+#pragma unroll
+        for (int m = 0; m < (1 << R_LOG); ++m) {
+            const uint64_t v = vbase + (base | ((uint32_t)m << a));
+            gload<R, TRACE>(r[m], b, v, h, nops);
+            if (fill) pad_select(r[m], v, n_valid);
+        }
+    }
+    for (int p = 1; p <= p_max; ++p) tile_level<R, R_LOG, TRACE>(r, s, tau, act, a, p, p - 1, p_top, vbase, h, nops);
+    const int a_out = t - R_LOG;
+    if (a != a_out) {
+        relayout<R, R_LOG, TRACE>(r, s, tau, act, a, a_out, h, nops);
+        a = a_out;
+    }
+    if (act) {
+        const uint32_t base = lay_base<R_LOG>(tau, a);
+#pragma unroll
+        for (int m = 0; m < (1 << R_LOG); ++m) gstore<R, TRACE>(r[m], b, vbase + (base | ((uint32_t)m << a)), h, nops);
+    }
+    if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
+}
+
+// Full 2^T_LOG tiles: all levels p = 1 .. T_LOG with a compile-time schedule.
+template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
+__global__ __launch_bounds__(BLOCK) void k_sort_local_ct(Bufs<R> b, IMap map, int p_top, uint64_t n_valid, int fill,
+                                                         TraceSlot *slot)
+{
+    using TL = TileCT<R, T_LOG, R_LOG, TRACE>;
+    constexpr int A0 = T_LOG - R_LOG;
+    extern __shared__ __align__(16) char lds[];
+    const uint32_t tau = threadIdx.x;
+    const uint64_t vbase = (uint64_t)blockIdx.x << T_LOG;
+    const uint64_t pbase = map_uniform(map, vbase);
+    uint64_t h = 0;
+    uint32_t nops = 0;
+    R r[1 << R_LOG];
+    const uint32_t base = lay_base<R_LOG>(tau, A0);
+#pragma unroll
+    for (int m = 0; m < (1 << R_LOG); ++m) {
+        gload_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << A0), base, h, nops);
+        if (fill) pad_select(r[m], vbase + (base | ((uint32_t)m << A0)), n_valid);
+    }
+    const bool asc_hi = dir_asc(vbase, T_LOG, p_top);
+    TL::template levels<1, A0>(r, lds, tau, asc_hi, p_top, h, nops);
+    TL::template relayout<TL::end_all(1, A0), A0>(r, lds, tau, h, nops);
+#pragma unroll
+    for (int m = 0; m < (1 << R_LOG); ++m) gstore_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << A0), base, h, nops);
+    if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
+}
+
+// Stages (p, T_LOG-1 .. 0) of one level p > T_LOG on tiles of 2^T_LOG.
+template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
+__global__ __launch_bounds__(BLOCK) void k_sort_merge(Bufs<R> b, IMap map, int p, int p_top, TraceSlot *slot)
+{
+    using TL = TileCT<R, T_LOG, R_LOG, TRACE>;
+    constexpr int A0 = T_LOG - R_LOG;
+    extern __shared__ __align__(16) char lds[];
+    const uint32_t tau = threadIdx.x;
+    const uint64_t vbase = (uint64_t)blockIdx.x << T_LOG;
+    const uint64_t pbase = map_uniform(map, vbase);
+    uint64_t h = 0;
+    uint32_t nops = 0;
+    R r[1 << R_LOG];
+    const uint32_t base = lay_base<R_LOG>(tau, A0);
+#pragma unroll
+    for (int m = 0; m < (1 << R_LOG); ++m) gload_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << A0), base, h, nops);
+    const bool asc = dir_asc(vbase, p, p_top);
+    TL::template level<-1, T_LOG - 1, A0>(r, lds, tau, asc, p_top, h, nops);
+    TL::template relayout<TL::end_layout(T_LOG - 1, A0), A0>(r, lds, tau, h, nops);
+#pragma unroll
+    for (int m = 0; m < (1 << R_LOG); ++m) gstore_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << A0), base, h, nops);
+    if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
+}
+
+// ------------------------------------------------------------ global pass
+
+// S fused stages (p, q_lo+S-1 .. q_lo): thread g owns the 2^S records
+// base(g) + m * 2^q_lo, base(g) = (g mod 2^q_lo) + (g / 2^q_lo) * 2^(q_lo+S).
+// With q_lo >= 8 a block's 256 threads share every bit of base(g) above bit 7,
+// so base(g) = ubase + tid and consecutive lanes touch consecutive records:
+// each wavefront load or store is one contiguous 512-byte (keys) or 256-byte
+// (payload) run.
+template <class R, int S, bool TRACE>
+__global__ __launch_bounds__(256) void k_sort_global(Bufs<R> b, IMap map, int p, int q_lo, int p_top,
+                                                     TraceSlot *slot)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint64_t g0 = (uint64_t)blockIdx.x * 256;
+    const uint64_t ubase = (g0 & ((1ull << q_lo) - 1)) | ((g0 >> q_lo) << (q_lo + S));
+    const uint64_t pbase = map_uniform(map, ubase);
+    uint64_t h = 0;
+    uint32_t nops = 0;
+    R r[1 << S];
+#pragma unroll
+    for (int m = 0; m < (1 << S); ++m) gload_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << q_lo), tid, h, nops);
+    const bool asc = dir_asc(ubase, p, p_top);
+#pragma unroll
+    for (int qq = S - 1; qq >= 0; --qq) {
+#pragma unroll
+        for (int m = 0; m < (1 << S); ++m) {
+            if (m & (1 << qq)) continue;
+            cmpx(r[m], r[m | (1 << qq)], asc);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < (1 << S); ++m) gstore_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << q_lo), tid, h, nops);
+    if constexpr (TRACE) tcommit(slot, g0 + tid, h, nops);
+}
+
+} // namespace dev
+
+// ================================================================== host
+
+namespace sortimpl {
+
+using dev::Bufs;
+using dev::IMap;
+
+struct Pass {
+    int kind; // 0 tiny, 1 local, 2 global, 3 merge
+    int p, q_lo, s;
+};
+
+// Pass schedule for n_log = log2(n_pad) with tiles of 2^t_log and at most
+// s_max fused global stages.  Depends on public sizes only.
+inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log)
+{
+    std::vector<Pass> v;
+    if (n_log < r_log) {
+        v.push_back({0, n_log, 0, 0});
+        return v;
+    }
+    const int t = std::min(n_log, t_log);
+    v.push_back({1, t, 0, t});
+    for (int p = t_log + 1; p <= n_log; ++p) {
+        int q = p - 1;
+        while (q >= t_log) {
+            int s = std::min(s_max, q - t_log + 1);
+            v.push_back({2, p, q - s + 1, s});
+            q -= s;
+        }
+        v.push_back({3, p, 0, 0});
+    }
+    return v;
+}
+
+template <class R, int T_LOG, int R_LOG, int BLOCK, int S_MAX>
+struct SortImpl {
+    static_assert(T_LOG >= 8, "global passes need q_lo >= 8 (256-thread blocks)");
+
+    template <bool TRACE, int S>
+    static void launch_global(vo_ctx *ctx, const LaunchInfo &li, Bufs<R> b, IMap map, int p, int q_lo, int p_top)
+    {
+        if constexpr (S <= S_MAX) launch(ctx, li, dev::k_sort_global<R, S, TRACE>, b, map, p, q_lo, p_top, trace_slot(ctx));
+    }
+
+    template <bool TRACE>
+    static void global_pass(vo_ctx *ctx, Bufs<R> b, IMap map, int p, int q_lo, int s, int p_top, uint64_t n_virtual)
+    {
+        const uint64_t threads = n_virtual >> s;
+        VO_CHECK(threads >= 256 && q_lo >= 8, VO_ERR_INVALID, "global pass below one block");
+        LaunchInfo li{"sort_global", dim3((uint32_t)(threads / 256)), dim3(256), 0,
+                      hash_mix(hash_mix(p, q_lo), hash_mix(s, p_top))};
+        switch (s) {
+        case 1: launch_global<TRACE, 1>(ctx, li, b, map, p, q_lo, p_top); break;
+        case 2: launch_global<TRACE, 2>(ctx, li, b, map, p, q_lo, p_top); break;
+        case 3: launch_global<TRACE, 3>(ctx, li, b, map, p, q_lo, p_top); break;
+        case 4: launch_global<TRACE, 4>(ctx, li, b, map, p, q_lo, p_top); break;
+        case 5: launch_global<TRACE, 5>(ctx, li, b, map, p, q_lo, p_top); break;
+        default: throw Error{VO_ERR_INVALID, "bad fused stage count"};
+        }
+    }
+
+    template <bool TRACE>
+    static void run_t(vo_ctx *ctx, Bufs<R> b, int n_log, uint64_t n_valid, int fill, IMap map, int p_top,
+                      uint64_t n_virtual)
+    {
+        const int s_max = std::max(1, std::min(S_MAX, ctx->tuning.max_fused_stages));
+        for (const Pass &ps : plan_passes(n_log, T_LOG, s_max, R_LOG)) {
+            switch (ps.kind) {
+            case 0: {
+                LaunchInfo li{"sort_tiny", dim3(1), dim3(64), 0, hash_mix(n_log, fill)};
+                launch(ctx, li, dev::k_sort_tiny<R, TRACE>, b, n_log, n_valid, fill, trace_slot(ctx));
+                break;
+            }
+            case 1: {
+                const int t = ps.p;
+                const bool ct = (t == T_LOG && p_top >= T_LOG);
+                const size_t lds = ct ? dev::TileCT<R, T_LOG, R_LOG, TRACE>::lds_bytes() : dev::lds_bytes<R>(t);
+                LaunchInfo li{"sort_local", dim3((uint32_t)(n_virtual >> t)), dim3(BLOCK), (uint32_t)lds,
+                              hash_mix(hash_mix(t, p_top), fill)};
+                if (ct) {
+                    auto k = dev::k_sort_local_ct<R, T_LOG, R_LOG, BLOCK, TRACE>;
+                    VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    launch(ctx, li, k, b, map, p_top, n_valid, fill, trace_slot(ctx));
+                } else {
+                    VO_CHECK(map.offsets == nullptr, VO_ERR_INVALID, "mapped sort below one tile");
+                    auto k = dev::k_sort_local<R, T_LOG, R_LOG, BLOCK, TRACE>;
+                    VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    launch(ctx, li, k, b, t, std::min(t, p_top), p_top, n_valid, fill, trace_slot(ctx));
+                }
+                break;
+            }
+            case 2: global_pass<TRACE>(ctx, b, map, ps.p, ps.q_lo, ps.s, p_top, n_virtual); break;
+            case 3: {
+                const size_t lds = dev::TileCT<R, T_LOG, R_LOG, TRACE>::lds_bytes();
+                LaunchInfo li{"sort_merge", dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BLOCK), (uint32_t)lds,
+                              hash_mix(ps.p, p_top)};
+                auto k = dev::k_sort_merge<R, T_LOG, R_LOG, BLOCK, TRACE>;
+                VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                launch(ctx, li, k, b, map, ps.p, p_top, trace_slot(ctx));
+                break;
+            }
+            }
+        }
+    }
+
+    static void run(vo_ctx *ctx, Bufs<R> b, int n_log, uint64_t n_valid, int fill)
+    {
+        IMap map{nullptr, 0};
+        if (ctx->tracing)
+            run_t<true>(ctx, b, n_log, n_valid, fill, map, n_log, 1ull << n_log);
+        else
+            run_t<false>(ctx, b, n_log, n_valid, fill, map, n_log, 1ull << n_log);
+    }
+
+    static int passes(vo_ctx *ctx, int n_log)
+    {
+        const int s_max = std::max(1, std::min(S_MAX, ctx->tuning.max_fused_stages));
+        return (int)plan_passes(n_log, T_LOG, s_max, R_LOG).size();
+    }
+};
+
+// Record kinds and their tile shapes (LDS = 2^T_LOG * 17/16 * record bytes).
+using RecKP = dev::Rec<1, uint32_t, true>; // 12 B: u64 key + u32 payload
+using RecKV = dev::Rec<1, uint64_t, true>; // 16 B: u64 key + u64 value
+using RecM1 = dev::Rec<1, int64_t, false>; // multi-column: packed key words + index
+using RecM2 = dev::Rec<2, int64_t, false>;
+using RecM4 = dev::Rec<4, int64_t, false>;
+using RecM8 = dev::Rec<8, int64_t, false>;
+
+} // namespace sortimpl
+} // namespace vo
