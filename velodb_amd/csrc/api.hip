@@ -242,7 +242,9 @@ int vo_ctx_create(const vo_config *cfg, vo_ctx **out)
         hipDeviceProp_t prop;
         VO_HIP(hipGetDeviceProperties(&prop, dev));
         ctx->num_cus = prop.multiProcessorCount;
-        if (c.tuning) ctx->tuning.max_fused_stages = (int)(c.tuning & 0xF);
+        // tuning word: bits 0-3 register-pass stages, bits 4-7 hybrid-pass stages (+1, so 0 keeps the default)
+        if (c.tuning & 0xF) ctx->tuning.max_fused_stages = (int)(c.tuning & 0xF);
+        if (c.tuning & 0xF0) ctx->tuning.max_hybrid_stages = (int)((c.tuning >> 4) & 0xF) - 1;
     } catch (const Error &e) {
         delete ctx;
         return e.status;

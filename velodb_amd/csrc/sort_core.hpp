@@ -302,11 +302,11 @@ struct TileCT {
     using P = typename R::P;
 
     // layout reached after stages (q_start .. 0) starting from layout a
-    static constexpr int end_layout(int q, int a)
+    static constexpr int end_layout(int q, int a, int qlo = 0)
     {
-        while (q >= 0) {
+        while (q >= qlo) {
             if (!(a <= q && q < a + R_LOG)) a = q - R_LOG + 1 > 0 ? q - R_LOG + 1 : 0;
-            q = a - 1;
+            q = (a > qlo ? a : qlo) - 1;
         }
         return a;
     }
@@ -436,17 +436,18 @@ struct TileCT {
         }
     }
 
-    // Stages (P, Q .. 0) from layout A, changing layout as needed.
-    template <int P, int Q, int A>
+    // Stages (P, Q .. QLO) from layout A, changing layout as needed.
+    template <int P, int Q, int A, int QLO = 0>
     static __device__ __forceinline__ void level(R (&r)[RR], char *lds, uint32_t tau, bool asc_hi, int p_top,
                                                  uint64_t &h, uint32_t &n)
     {
-        if constexpr (Q >= 0) {
+        if constexpr (Q >= QLO) {
             constexpr bool covered = (A <= Q && Q < A + R_LOG);
             constexpr int AN = covered ? A : (Q - R_LOG + 1 > 0 ? Q - R_LOG + 1 : 0);
+            constexpr int LO = AN > QLO ? AN : QLO;
             relayout<A, AN>(r, lds, tau, h, n);
-            stages<P, AN, Q, AN>(r, tau, asc_hi, p_top);
-            level<P, AN - 1, AN>(r, lds, tau, asc_hi, p_top, h, n);
+            stages<P, AN, Q, LO>(r, tau, asc_hi, p_top);
+            level<P, LO - 1, AN, QLO>(r, lds, tau, asc_hi, p_top, h, n);
         }
     }
 

@@ -213,6 +213,48 @@ __global__ __launch_bounds__(BLOCK) void k_sort_merge(Bufs<R> b, IMap map, int p
     if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
 }
 
+// ------------------------------------------------------------ hybrid pass
+
+// S fused stages (p, q_lo+S-1 .. q_lo) with S up to 7, staged through LDS.
+// A block holds 2^T_LOG records: 2^S runs of C = T_LOG - S contiguous
+// records, run r starting at element ubase + r * 2^q_lo.  Tile-local index l
+// maps to element (l mod 2^C) + (l >> C) * 2^q_lo (plus the block's uniform
+// bits), so tile bits [C, T_LOG) are the stage bits and the stages run as a
+// tile schedule on them.  With C >= 6 a wavefront's 64 lanes always read
+// and write 64 consecutive records.
+template <class R, int T_LOG, int R_LOG, int BLOCK, int S, bool TRACE>
+__global__ __launch_bounds__(BLOCK) void k_sort_hybrid(Bufs<R> b, IMap map, int p, int q_lo, int p_top,
+                                                       TraceSlot *slot)
+{
+    using TL = TileCT<R, T_LOG, R_LOG, TRACE>;
+    constexpr int C = T_LOG - S;
+    constexpr int A0 = T_LOG - R_LOG;
+    static_assert(C >= 6 && A0 >= 6, "runs and lanes must cover 64 contiguous records");
+    extern __shared__ __align__(16) char lds[];
+    const uint32_t tau = threadIdx.x;
+    const uint32_t lane = tau & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tau >> 6);
+    const uint64_t blk = blockIdx.x;
+    const uint64_t b_lo = blk & ((1ull << (q_lo - C)) - 1);
+    const uint64_t b_hi = blk >> (q_lo - C);
+    const uint64_t ublk = (b_lo << C) | (b_hi << (q_lo + S));
+    // element offset of tile-local index l (a bit permutation, additive over disjoint bit fields)
+    auto eoff = [&](uint64_t l) { return (l & ((1ull << C) - 1)) | ((l >> C) << q_lo); };
+    const uint64_t vw = ublk + eoff((uint64_t)wave << 6);
+    const uint64_t pw = map_uniform(map, vw);
+    uint64_t h = 0;
+    uint32_t nops = 0;
+    R r[1 << R_LOG];
+#pragma unroll
+    for (int m = 0; m < (1 << R_LOG); ++m) gload_u<R, TRACE>(r[m], b, pw + eoff((uint64_t)m << A0), lane, h, nops);
+    const bool asc = dir_asc(ublk, p, p_top);
+    TL::template level<-1, T_LOG - 1, A0, C>(r, lds, tau, asc, p_top, h, nops);
+    TL::template relayout<TL::end_layout(T_LOG - 1, A0, C), A0>(r, lds, tau, h, nops);
+#pragma unroll
+    for (int m = 0; m < (1 << R_LOG); ++m) gstore_u<R, TRACE>(r[m], b, pw + eoff((uint64_t)m << A0), lane, h, nops);
+    if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
+}
+
 // ------------------------------------------------------------ global pass
 
 // S fused stages (p, q_lo+S-1 .. q_lo): thread g owns the 2^S records
@@ -264,7 +306,10 @@ struct Pass {
 
 // Pass schedule for n_log = log2(n_pad) with tiles of 2^t_log and at most
 // s_max fused global stages.  Depends on public sizes only.
-inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log)
+// With h_max > s_max, groups larger than s_max run as LDS-staged hybrid
+// passes (kind 4) of up to h_max stages; a level's stages above the tile are
+// split into the fewest passes, as evenly as possible.
+inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log, int h_max = 0)
 {
     std::vector<Pass> v;
     if (n_log < r_log) {
@@ -273,11 +318,14 @@ inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log)
     }
     const int t = std::min(n_log, t_log);
     v.push_back({1, t, 0, t});
+    const int g_max = std::max(s_max, h_max);
     for (int p = t_log + 1; p <= n_log; ++p) {
+        const int count = p - t_log;
+        const int groups = (count + g_max - 1) / g_max;
         int q = p - 1;
-        while (q >= t_log) {
-            int s = std::min(s_max, q - t_log + 1);
-            v.push_back({2, p, q - s + 1, s});
+        for (int gi = 0; gi < groups; ++gi) {
+            const int s = count / groups + (gi < count % groups ? 1 : 0);
+            v.push_back({s <= s_max ? 2 : 4, p, q - s + 1, s});
             q -= s;
         }
         v.push_back({3, p, 0, 0});
@@ -285,9 +333,29 @@ inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log)
     return v;
 }
 
-template <class R, int T_LOG, int R_LOG, int BLOCK, int S_MAX>
+template <class R, int T_LOG, int R_LOG, int BLOCK, int S_MAX, int H_MAX = 0>
 struct SortImpl {
     static_assert(T_LOG >= 8, "global passes need q_lo >= 8 (256-thread blocks)");
+    static_assert(H_MAX == 0 || (T_LOG - H_MAX >= 6 && BLOCK << R_LOG == 1 << T_LOG), "hybrid tile shape");
+
+    static int smax(vo_ctx *ctx) { return std::max(1, std::min(S_MAX, ctx->tuning.max_fused_stages)); }
+    static int hmax(vo_ctx *ctx) { return std::min(H_MAX, ctx->tuning.max_hybrid_stages); }
+
+    template <bool TRACE, int S>
+    static void launch_hybrid(vo_ctx *ctx, Bufs<R> b, IMap map, int p, int q_lo, int p_top, uint64_t n_virtual)
+    {
+        if constexpr (S <= H_MAX && T_LOG - S >= 6) {
+            using TL = dev::TileCT<R, T_LOG, R_LOG, TRACE>;
+            const size_t lds = TL::lds_bytes();
+            LaunchInfo li{"sort_global", dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BLOCK), (uint32_t)lds,
+                          hash_mix(hash_mix(p, q_lo), hash_mix(S + 100, p_top))};
+            auto k = dev::k_sort_hybrid<R, T_LOG, R_LOG, BLOCK, S, TRACE>;
+            VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            launch(ctx, li, k, b, map, p, q_lo, p_top, trace_slot(ctx));
+        } else {
+            throw Error{VO_ERR_INVALID, "hybrid pass width not built"};
+        }
+    }
 
     template <bool TRACE, int S>
     static void launch_global(vo_ctx *ctx, const LaunchInfo &li, Bufs<R> b, IMap map, int p, int q_lo, int p_top)
@@ -316,8 +384,7 @@ struct SortImpl {
     static void run_t(vo_ctx *ctx, Bufs<R> b, int n_log, uint64_t n_valid, int fill, IMap map, int p_top,
                       uint64_t n_virtual)
     {
-        const int s_max = std::max(1, std::min(S_MAX, ctx->tuning.max_fused_stages));
-        for (const Pass &ps : plan_passes(n_log, T_LOG, s_max, R_LOG)) {
+        for (const Pass &ps : plan_passes(n_log, T_LOG, smax(ctx), R_LOG, hmax(ctx))) {
             switch (ps.kind) {
             case 0: {
                 LaunchInfo li{"sort_tiny", dim3(1), dim3(64), 0, hash_mix(n_log, fill)};
@@ -343,6 +410,13 @@ struct SortImpl {
                 break;
             }
             case 2: global_pass<TRACE>(ctx, b, map, ps.p, ps.q_lo, ps.s, p_top, n_virtual); break;
+            case 4:
+                switch (ps.s) {
+                case 6: launch_hybrid<TRACE, 6>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
+                case 7: launch_hybrid<TRACE, 7>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
+                default: throw Error{VO_ERR_INVALID, "bad hybrid stage count"};
+                }
+                break;
             case 3: {
                 const size_t lds = dev::TileCT<R, T_LOG, R_LOG, TRACE>::lds_bytes();
                 LaunchInfo li{"sort_merge", dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BLOCK), (uint32_t)lds,
@@ -365,11 +439,7 @@ struct SortImpl {
             run_t<false>(ctx, b, n_log, n_valid, fill, map, n_log, 1ull << n_log);
     }
 
-    static int passes(vo_ctx *ctx, int n_log)
-    {
-        const int s_max = std::max(1, std::min(S_MAX, ctx->tuning.max_fused_stages));
-        return (int)plan_passes(n_log, T_LOG, s_max, R_LOG).size();
-    }
+    static int passes(vo_ctx *ctx, int n_log) { return (int)plan_passes(n_log, T_LOG, smax(ctx), R_LOG, hmax(ctx)).size(); }
 };
 
 // Record kinds and their tile shapes (LDS = 2^T_LOG * 17/16 * record bytes).

@@ -7,7 +7,7 @@ namespace vo {
 using namespace sortimpl;
 
 namespace {
-using SortKP = SortImpl<RecKP, 13, 4, 512, 5>;
+using SortKP = SortImpl<RecKP, 13, 4, 512, 5, 7>;
 using SortKV = SortImpl<RecKV, 12, 4, 256, 4>;
 } // namespace
 

@@ -63,9 +63,8 @@ struct TimingRecord {
 
 // Public-size-only schedule parameters (chosen by the autotuner or defaults).
 struct SortTuning {
-    int max_fused_stages = 5; // stages per global HBM pass (1..5)
-    int use_hybrid = 1;       // LDS-staged multi-stage global passes
-    int hybrid_col_log = 6;   // contiguous run (log2 elements) per hybrid column
+    int max_fused_stages = 5;  // stages per register-only global pass (1..5)
+    int max_hybrid_stages = 7; // stages per LDS-staged global pass (0 = off, 6..7)
 };
 
 } // namespace vo
