@@ -245,6 +245,7 @@ int vo_ctx_create(const vo_config *cfg, vo_ctx **out)
         // tuning word: bits 0-3 register-pass stages, bits 4-7 hybrid-pass stages (+1, so 0 keeps the default)
         if (c.tuning & 0xF) ctx->tuning.max_fused_stages = (int)(c.tuning & 0xF);
         if (c.tuning & 0xF0) ctx->tuning.max_hybrid_stages = (int)((c.tuning >> 4) & 0xF) - 1;
+        if (c.tuning & 0x100) ctx->tuning.pipe_merge = !ctx->tuning.pipe_merge;
     } catch (const Error &e) {
         delete ctx;
         return e.status;

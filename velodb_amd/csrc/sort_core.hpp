@@ -60,9 +60,59 @@ __device__ __forceinline__ bool rec_gt(const R &a, const R &b)
 // for > and one for <, the lexicographic combination and the direction are
 // scalar mask operations, and the swap is v_cndmask on the resulting mask
 // (2 per 32-bit word).  `ascm` holds the lanes whose pair sorts ascending.
+// Lanes where (a2:a1:a0) < (b2:b1:b0) as unsigned 96-bit numbers: the borrow
+// out of a subtract-with-borrow chain (three VALU ops, no scalar logic).
+__device__ __forceinline__ uint64_t lt_mask96(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint32_t a2,
+                                              uint32_t b2)
+{
+    uint64_t m;
+    uint32_t t0, t1, t2;
+    asm("v_sub_co_u32_e64 %0, %3, %4, %5\n\t"
+        "v_subb_co_u32_e64 %1, %3, %6, %7, %3\n\t"
+        "v_subb_co_u32_e64 %2, %3, %8, %9, %3"
+        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&s"(m)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1), "v"(a2), "v"(b2));
+    return m;
+}
+
+__device__ __forceinline__ uint64_t lt_mask128(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint32_t a2,
+                                               uint32_t b2, uint32_t a3, uint32_t b3)
+{
+    uint64_t m;
+    uint32_t t0, t1, t2, t3;
+    asm("v_sub_co_u32_e64 %0, %4, %5, %6\n\t"
+        "v_subb_co_u32_e64 %1, %4, %7, %8, %4\n\t"
+        "v_subb_co_u32_e64 %2, %4, %9, %10, %4\n\t"
+        "v_subb_co_u32_e64 %3, %4, %11, %12, %4"
+        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3), "=&s"(m)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1), "v"(a2), "v"(b2), "v"(a3), "v"(b3));
+    return m;
+}
+
 template <class R>
 __device__ __forceinline__ void cmpx_m(R &a, R &b, uint64_t ascm)
 {
+    if constexpr (R::TIE && R::KW == 1) {
+        // (key, payload) records compare as one integer, key high.  Records
+        // equal in every field are identical, so exchanging them on a tie
+        // changes nothing and the swap is simply lt(a, b) xor ascending.
+        uint64_t lt;
+        if constexpr (sizeof(typename R::P) == 4)
+            lt = lt_mask96((uint32_t)a.p, (uint32_t)b.p, (uint32_t)a.k[0], (uint32_t)b.k[0],
+                           (uint32_t)(a.k[0] >> 32), (uint32_t)(b.k[0] >> 32));
+        else
+            lt = lt_mask128((uint32_t)a.p, (uint32_t)b.p, (uint32_t)((uint64_t)a.p >> 32),
+                            (uint32_t)((uint64_t)b.p >> 32), (uint32_t)a.k[0], (uint32_t)b.k[0],
+                            (uint32_t)(a.k[0] >> 32), (uint32_t)(b.k[0] >> 32));
+        const bool sw = __builtin_amdgcn_inverse_ballot_w64(lt ^ ascm);
+        const uint64_t x = a.k[0], y = b.k[0];
+        a.k[0] = sw ? y : x;
+        b.k[0] = sw ? x : y;
+        const typename R::P u = a.p, v = b.p;
+        a.p = sw ? v : u;
+        b.p = sw ? u : v;
+        return;
+    }
     uint64_t gt = 0, lt = 0, eq = ~0ull;
 #pragma unroll
     for (int w = 0; w < R::KW; ++w) {

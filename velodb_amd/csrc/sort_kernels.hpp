@@ -37,7 +37,7 @@ __device__ __forceinline__ uint64_t map_uniform(const IMap &m, uint64_t v)
 // loads on gfx9-family parts.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *ubase)
 {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(ubase), (short)0, 0x7FFFFFFF, 0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(ubase), (short)0, (int)0xFFFFFFFFu, 0x00020000);
 }
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -88,6 +88,54 @@ __device__ __forceinline__ void gstore_u(const R &r, const Bufs<R> &b, uint64_t 
     st_u(b.p + u, vi, r.p);
     if constexpr (TRACE) {
         h = tfold(h, 2, u + vi);
+        ++n;
+    }
+}
+
+
+// Tile form: one buffer resource per column for the tile's uniform base u,
+// the per-register offset so (elements, uniform, < 2^16) in the scalar
+// offset field and the per-thread offset vi in the vector offset.
+template <typename T>
+__device__ __forceinline__ T ld_t(__amdgpu_buffer_rsrc_t r, uint32_t vi, uint32_t so)
+{
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vi * 8u, so * 8u, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vi * 4u, so * 4u, 0));
+}
+
+template <typename T>
+__device__ __forceinline__ void st_t(__amdgpu_buffer_rsrc_t r, uint32_t vi, uint32_t so, T v)
+{
+    if constexpr (sizeof(T) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vi * 8u, so * 8u, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vi * 4u, so * 4u, 0);
+}
+
+template <class R, bool TRACE>
+__device__ __forceinline__ void gload_t(R &r, const Bufs<R> &b, uint64_t u, uint32_t so, uint32_t vi, uint64_t &h,
+                                        uint32_t &n)
+{
+#pragma unroll
+    for (int w = 0; w < R::KW; ++w) r.k[w] = ld_t<uint64_t>(buf_rsrc(b.k[w] + u), vi, so);
+    r.p = ld_t<typename R::P>(buf_rsrc(b.p + u), vi, so);
+    if constexpr (TRACE) {
+        h = tfold(h, 1, u + so + vi);
+        ++n;
+    }
+}
+
+template <class R, bool TRACE>
+__device__ __forceinline__ void gstore_t(const R &r, const Bufs<R> &b, uint64_t u, uint32_t so, uint32_t vi,
+                                         uint64_t &h, uint32_t &n)
+{
+#pragma unroll
+    for (int w = 0; w < R::KW; ++w) st_t<uint64_t>(buf_rsrc(b.k[w] + u), vi, so, r.k[w]);
+    st_t<typename R::P>(buf_rsrc(b.p + u), vi, so, r.p);
+    if constexpr (TRACE) {
+        h = tfold(h, 2, u + so + vi);
         ++n;
     }
 }
@@ -178,14 +226,14 @@ __global__ __launch_bounds__(BLOCK) void k_sort_local_ct(Bufs<R> b, IMap map, in
     const uint32_t base = lay_base<R_LOG>(tau, A0);
 #pragma unroll
     for (int m = 0; m < (1 << R_LOG); ++m) {
-        gload_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << A0), base, h, nops);
+        gload_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
         if (fill) pad_select(r[m], vbase + (base | ((uint32_t)m << A0)), n_valid);
     }
     const bool asc_hi = dir_asc(vbase, T_LOG, p_top);
     TL::template levels<1, A0>(r, lds, tau, asc_hi, p_top, h, nops);
     TL::template relayout<TL::end_all(1, A0), A0>(r, lds, tau, h, nops);
 #pragma unroll
-    for (int m = 0; m < (1 << R_LOG); ++m) gstore_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << A0), base, h, nops);
+    for (int m = 0; m < (1 << R_LOG); ++m) gstore_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
     if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
 }
 
@@ -204,12 +252,55 @@ __global__ __launch_bounds__(BLOCK) void k_sort_merge(Bufs<R> b, IMap map, int p
     R r[1 << R_LOG];
     const uint32_t base = lay_base<R_LOG>(tau, A0);
 #pragma unroll
-    for (int m = 0; m < (1 << R_LOG); ++m) gload_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << A0), base, h, nops);
+    for (int m = 0; m < (1 << R_LOG); ++m) gload_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
     const bool asc = dir_asc(vbase, p, p_top);
     TL::template level<-1, T_LOG - 1, A0>(r, lds, tau, asc, p_top, h, nops);
     TL::template relayout<TL::end_layout(T_LOG - 1, A0), A0>(r, lds, tau, h, nops);
 #pragma unroll
-    for (int m = 0; m < (1 << R_LOG); ++m) gstore_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << A0), base, h, nops);
+    for (int m = 0; m < (1 << R_LOG); ++m) gstore_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
+    if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
+}
+
+// The same merge with resident blocks: each block walks tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... and issues the HBM loads of its
+// next tile before working on the current one, so the loads overlap the LDS
+// and compare-exchange phase of the tile in hand.
+template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
+__global__ __launch_bounds__(BLOCK) void k_sort_merge_pipe(Bufs<R> b, IMap map, int p, int p_top, uint32_t n_tiles,
+                                                           TraceSlot *slot)
+{
+    using TL = TileCT<R, T_LOG, R_LOG, TRACE>;
+    constexpr int A0 = T_LOG - R_LOG;
+    constexpr int RR = 1 << R_LOG;
+    extern __shared__ __align__(16) char lds[];
+    const uint32_t tau = threadIdx.x;
+    const uint32_t base = lay_base<R_LOG>(tau, A0);
+    uint64_t h = 0;
+    uint32_t nops = 0;
+    R cur[RR], nxt[RR];
+    uint32_t t = blockIdx.x;
+    if (t < n_tiles) {
+        const uint64_t pb = map_uniform(map, (uint64_t)t << T_LOG);
+#pragma unroll
+        for (int m = 0; m < RR; ++m) gload_t<R, TRACE>(cur[m], b, pb, (uint32_t)m << A0, base, h, nops);
+    }
+    for (; t < n_tiles; t += gridDim.x) {
+        const uint32_t tn = t + gridDim.x;
+        if (tn < n_tiles) {
+            const uint64_t pn = map_uniform(map, (uint64_t)tn << T_LOG);
+#pragma unroll
+            for (int m = 0; m < RR; ++m) gload_t<R, TRACE>(nxt[m], b, pn, (uint32_t)m << A0, base, h, nops);
+        }
+        const uint64_t vbase = (uint64_t)t << T_LOG;
+        const uint64_t pbase = map_uniform(map, vbase);
+        const bool asc = dir_asc(vbase, p, p_top);
+        TL::template level<-1, T_LOG - 1, A0>(cur, lds, tau, asc, p_top, h, nops);
+        TL::template relayout<TL::end_layout(T_LOG - 1, A0), A0>(cur, lds, tau, h, nops);
+#pragma unroll
+        for (int m = 0; m < RR; ++m) gstore_t<R, TRACE>(cur[m], b, pbase, (uint32_t)m << A0, base, h, nops);
+#pragma unroll
+        for (int m = 0; m < RR; ++m) cur[m] = nxt[m];
+    }
     if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
 }
 
@@ -275,7 +366,7 @@ __global__ __launch_bounds__(256) void k_sort_global(Bufs<R> b, IMap map, int p,
     uint32_t nops = 0;
     R r[1 << S];
 #pragma unroll
-    for (int m = 0; m < (1 << S); ++m) gload_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << q_lo), tid, h, nops);
+    for (int m = 0; m < (1 << S); ++m) gload_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << q_lo, tid, h, nops);
     const bool asc = dir_asc(ubase, p, p_top);
 #pragma unroll
     for (int qq = S - 1; qq >= 0; --qq) {
@@ -286,7 +377,7 @@ __global__ __launch_bounds__(256) void k_sort_global(Bufs<R> b, IMap map, int p,
         }
     }
 #pragma unroll
-    for (int m = 0; m < (1 << S); ++m) gstore_u<R, TRACE>(r[m], b, pbase + ((uint64_t)m << q_lo), tid, h, nops);
+    for (int m = 0; m < (1 << S); ++m) gstore_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << q_lo, tid, h, nops);
     if constexpr (TRACE) tcommit(slot, g0 + tid, h, nops);
 }
 
@@ -298,6 +389,10 @@ namespace sortimpl {
 
 using dev::Bufs;
 using dev::IMap;
+
+// Largest padded row count: 8-byte columns of 2^29 rows keep every byte
+// offset of the buffer-resource addressing below 2^32.
+constexpr uint64_t kMaxSortRows = 1ull << 29;
 
 struct Pass {
     int kind; // 0 tiny, 1 local, 2 global, 3 merge
@@ -319,9 +414,19 @@ inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log,
     const int t = std::min(n_log, t_log);
     v.push_back({1, t, 0, t});
     const int g_max = std::max(s_max, h_max);
+    // a group width is buildable when it is a register pass (<= s_max) or a
+    // hybrid pass (6 .. h_max)
+    auto ok = [&](int count, int groups) {
+        for (int gi = 0; gi < groups; ++gi) {
+            const int s = count / groups + (gi < count % groups ? 1 : 0);
+            if (!(s <= s_max || (s >= 6 && s <= h_max))) return false;
+        }
+        return true;
+    };
     for (int p = t_log + 1; p <= n_log; ++p) {
         const int count = p - t_log;
-        const int groups = (count + g_max - 1) / g_max;
+        int groups = (count + g_max - 1) / g_max;
+        while (!ok(count, groups)) ++groups;
         int q = p - 1;
         for (int gi = 0; gi < groups; ++gi) {
             const int s = count / groups + (gi < count % groups ? 1 : 0);
@@ -419,11 +524,21 @@ struct SortImpl {
                 break;
             case 3: {
                 const size_t lds = dev::TileCT<R, T_LOG, R_LOG, TRACE>::lds_bytes();
-                LaunchInfo li{"sort_merge", dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BLOCK), (uint32_t)lds,
-                              hash_mix(ps.p, p_top)};
-                auto k = dev::k_sort_merge<R, T_LOG, R_LOG, BLOCK, TRACE>;
-                VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                launch(ctx, li, k, b, map, ps.p, p_top, trace_slot(ctx));
+                const uint32_t n_tiles = (uint32_t)(n_virtual >> T_LOG);
+                if (ctx->tuning.pipe_merge) {
+                    auto k = dev::k_sort_merge_pipe<R, T_LOG, R_LOG, BLOCK, TRACE>;
+                    VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    int per_cu = 0;
+                    VO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, BLOCK, lds));
+                    const uint32_t grid = std::min<uint32_t>(n_tiles, (uint32_t)std::max(1, per_cu) * ctx->num_cus);
+                    LaunchInfo li{"sort_merge", dim3(grid), dim3(BLOCK), (uint32_t)lds, hash_mix(ps.p, p_top)};
+                    launch(ctx, li, k, b, map, ps.p, p_top, n_tiles, trace_slot(ctx));
+                } else {
+                    LaunchInfo li{"sort_merge", dim3(n_tiles), dim3(BLOCK), (uint32_t)lds, hash_mix(ps.p, p_top)};
+                    auto k = dev::k_sort_merge<R, T_LOG, R_LOG, BLOCK, TRACE>;
+                    VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    launch(ctx, li, k, b, map, ps.p, p_top, trace_slot(ctx));
+                }
                 break;
             }
             }
@@ -432,6 +547,8 @@ struct SortImpl {
 
     static void run(vo_ctx *ctx, Bufs<R> b, int n_log, uint64_t n_valid, int fill)
     {
+        VO_CHECK((1ull << n_log) <= kMaxSortRows, VO_ERR_INVALID,
+                 "n_pad above 2^29 (32-bit buffer offsets of 8-byte columns)");
         IMap map{nullptr, 0};
         if (ctx->tracing)
             run_t<true>(ctx, b, n_log, n_valid, fill, map, n_log, 1ull << n_log);

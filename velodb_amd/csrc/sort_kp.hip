@@ -14,7 +14,7 @@ using SortKV = SortImpl<RecKV, 12, 4, 256, 4>;
 void op_sort_kp(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad)
 {
     VO_CHECK(is_pow2(n_pad) && n <= n_pad, VO_ERR_INVALID, "n_pad must be a power of two >= n");
-    VO_CHECK(n_pad <= (1ull << 32), VO_ERR_INVALID, "n_pad above 2^32 (u32 payload pad index)");
+    VO_CHECK(n_pad <= kMaxSortRows, VO_ERR_INVALID, "n_pad above 2^29 (32-bit buffer offsets of 8-byte columns)");
     VO_CHECK(keys && payload, VO_ERR_INVALID, "null buffer");
     Bufs<RecKP> b;
     b.k[0] = keys;
