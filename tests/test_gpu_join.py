@@ -1,0 +1,72 @@
+"""GPU parity of the oblivious PK/FK join + GROUP BY SUM against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.gpu_util import empty_dev, from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(vctx, dk, dg, fk, fv, n_pad, max_groups=64):
+    d = [to_dev(x) for x in (dk, dg, fk, fv)]
+    og = empty_dev(max_groups, np.int32)
+    os_ = empty_dev(max_groups, np.int64)
+    on = empty_dev(1, np.uint64)
+    sk = empty_dev(n_pad, np.uint64)
+    sv = empty_dev(n_pad, np.uint64)
+    vctx.join_groupsum(d[0], d[1], len(dk), d[2], d[3], len(fk), n_pad, og, os_, on, max_groups, sk, sv)
+    G = int(from_dev(on, np.uint64)[0])
+    g = from_dev(og, np.int32)
+    s = from_dev(os_, np.int64)
+    assert np.all(g[G:] == -1) and np.all(s[G:] == 0)
+    return g[:G], s[:G], from_dev(sk, np.uint64), from_dev(sv, np.uint64)
+
+
+@pytest.mark.parametrize("n_dim,n_fact,n_pad", [(100, 400, 4096), (1500, 6000, 8192), (20000, 50000, 1 << 17),
+                                                (150000, 600000, 1 << 20)])
+def test_join_groupsum_bit_exact(vctx, n_dim, n_fact, n_pad):
+    print(f"join n_dim={n_dim} n_fact={n_fact}", flush=True)
+    dk, dg, fk, fv = oracle.gen_join(4, n_dim, n_fact)
+    g, s, sk, sv = _run(vctx, dk, dg, fk, fv, n_pad)
+    eg, es, esk, esv = oracle.join_groupsum(dk, dg, fk, fv, n_pad, want_sorted=True)
+    assert np.array_equal(sk, esk) and np.array_equal(sv, esv)
+    assert np.array_equal(g, eg) and np.array_equal(s, es)
+
+
+def test_join_config3_full_size(vctx):
+    """Config: 1.5M dimension rows (unique keys 1..1.5M, 25 groups), 6M fact
+    rows, union padded to 2^23; 25 exact int64 sums."""
+    n_dim, n_fact, n_pad = 1_500_000, 6_000_000, 1 << 23
+    dk, dg, fk, fv = oracle.gen_join(1, n_dim, n_fact)
+    g, s, _, _ = _run(vctx, dk, dg, fk, fv, n_pad)
+    # closed form: every fact row matches exactly one dimension row
+    g_of_key = np.zeros(n_dim + 1, np.int64)
+    g_of_key[dk] = dg
+    want = np.bincount(g_of_key[fk], weights=None, minlength=25)
+    sums = np.zeros(25, np.int64)
+    np.add.at(sums, g_of_key[fk], fv)
+    assert g.tolist() == list(range(25))
+    assert s.tolist() == sums.tolist()
+    assert want.sum() == n_fact
+
+
+def test_join_unmatched_and_empty_groups(vctx):
+    dk = np.array([1, 2, 3, 10], np.int32)
+    dg = np.array([0, 1, 0, 7], np.int32)
+    fk = np.array([1, 4, 3, 3, 9, 2, 11, 11], np.int32)
+    fv = np.array([10, 1000, 5, 7, 1000, 3, 99, 99], np.int64)
+    g, s, _, _ = _run(vctx, dk, dg, fk, fv, 4096)
+    assert g.tolist() == [0, 1] and s.tolist() == [22, 3]
+
+
+def test_join_single_group_spans_tiles(vctx):
+    # one group: the segmented sum must carry across every tile
+    n_dim, n_fact = 1000, 70000
+    dk = np.arange(1, n_dim + 1, dtype=np.int32)
+    dg = np.zeros(n_dim, np.int32)
+    rng = np.random.default_rng(1)
+    fk = rng.integers(1, n_dim + 1, n_fact).astype(np.int32)
+    fv = rng.integers(100, 10_000_000, n_fact).astype(np.int64)
+    g, s, _, _ = _run(vctx, dk, dg, fk, fv, 1 << 17)
+    assert g.tolist() == [0] and s.tolist() == [int(fv.sum())]

@@ -1,0 +1,551 @@
+// compact.hip -- oblivious order-preserving compaction (ORCompact routing
+// network), the Q6-style filter + exact int128 SUM, and OP_SCATTER.
+//
+// Reference method: oblivious_compact_elem / _inner_elem / _2power_inner_elem
+// (obliviator/join/enclave/oblivious_compact.c:59-153, 327-338).  For a
+// power-of-two n the recursion unrolls into log2(n) levels: at level q
+// (block size s = 2^(q+1), half = 2^q) every aligned block [b, b+s) with
+// offset o = C[b] mod s (C = exclusive prefix count of the selected rows)
+// conditionally swaps rows b+i and b+half+i for all i < half, with
+//   s == 2 : swap = (!sel[b] & sel[b+1]) ^ o
+//   s  > 2 : m1 = C[b+half] - C[b], om = o mod half, om1 = (o+m1) mod half,
+//            swap = ((om + m1 >= half) ^ (o >= half)) ^ (i >= om1)
+// (the offset form of :88-105; the chain of :121-153 reduces to o = C[b]
+// mod s for the blocks it creates).  Every row is read and written at every
+// level; the swap decision is a select, so addresses depend on n_pad only.
+//
+// Passes: count (selected rows per tile) -> tile scan -> local levels
+// q < T_LOG inside each tile (LDS) -> fused global levels, S per HBM pass,
+// rows at stride 2^q_lo as in the sort's global passes.
+#include "vo_internal.hpp"
+
+#include <algorithm>
+
+namespace vo {
+namespace {
+
+constexpr int CT_LOG = 12; // tile: 4096 rows
+constexpr int CT = 1 << CT_LOG;
+constexpr int CBLOCK = 256;
+
+// A column of the compaction: pointer and element width (1, 2, 4, 8 bytes).
+struct Col {
+    void *p;
+    uint32_t w;
+};
+
+template <int NW>
+struct Cols {
+    Col c[NW];
+};
+
+__device__ __forceinline__ uint64_t ld_col(const Col &c, uint64_t i)
+{
+    switch (c.w) {
+    case 8: return ((const uint64_t *)c.p)[i];
+    case 4: return ((const uint32_t *)c.p)[i];
+    case 2: return ((const uint16_t *)c.p)[i];
+    default: return ((const uint8_t *)c.p)[i];
+    }
+}
+
+__device__ __forceinline__ void st_col(const Col &c, uint64_t i, uint64_t v)
+{
+    switch (c.w) {
+    case 8: ((uint64_t *)c.p)[i] = v; break;
+    case 4: ((uint32_t *)c.p)[i] = (uint32_t)v; break;
+    case 2: ((uint16_t *)c.p)[i] = (uint16_t)v; break;
+    default: ((uint8_t *)c.p)[i] = (uint8_t)v; break;
+    }
+}
+
+// swap flag of the pair at offset i in the block starting at prefix count
+// cb, with cbh = prefix count at the block's middle; s2 = half size
+__device__ __forceinline__ bool level_flag(uint64_t cb, uint64_t cbh, uint64_t half, uint64_t i)
+{
+    const uint64_t s = half * 2;
+    const uint64_t o = cb & (s - 1);
+    const uint64_t m1 = cbh - cb;
+    const uint64_t om = o & (half - 1);
+    const uint64_t om1 = (o + m1) & (half - 1);
+    const bool base = ((om + m1) >= half) ^ (o >= half);
+    return base ^ (i >= om1);
+}
+
+// ---------------------------------------------------------------- counting
+
+// Selected rows per tile: sel[i] = mask[i] != 0 && i < n.
+__global__ __launch_bounds__(CBLOCK) void k_count_mask(const uint8_t *__restrict__ mask, uint64_t n,
+                                                       uint32_t *__restrict__ tile_cnt)
+{
+    __shared__ uint32_t red[CBLOCK / 64];
+    const uint64_t t0 = (uint64_t)blockIdx.x * CT;
+    uint32_t c = 0;
+#pragma unroll 4
+    for (int k = 0; k < CT / CBLOCK; ++k) {
+        const uint64_t i = t0 + k * CBLOCK + threadIdx.x;
+        c += (mask[i] != 0) & (i < n);
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int w = 0; w < CBLOCK / 64; ++w) s += red[w];
+        tile_cnt[blockIdx.x] = s;
+    }
+}
+
+// Q6-style predicate: writes mask[i] and per-tile counts in one pass.
+struct Pred {
+    int32_t day_lo, day_hi;
+    int64_t disc_lo, disc_hi, qty_lt;
+};
+
+__global__ __launch_bounds__(CBLOCK) void k_filter_count(const int32_t *__restrict__ day,
+                                                         const int64_t *__restrict__ qty,
+                                                         const int64_t *__restrict__ disc, uint64_t n, Pred pr,
+                                                         uint8_t *__restrict__ mask, uint32_t *__restrict__ tile_cnt)
+{
+    __shared__ uint32_t red[CBLOCK / 64];
+    const uint64_t t0 = (uint64_t)blockIdx.x * CT;
+    uint32_t c = 0;
+#pragma unroll 4
+    for (int k = 0; k < CT / CBLOCK; ++k) {
+        const uint64_t i = t0 + k * CBLOCK + threadIdx.x;
+        const int32_t d = day[i];
+        const int64_t q = qty[i], ds = disc[i];
+        const uint32_t m = (d >= pr.day_lo) & (d < pr.day_hi) & (ds >= pr.disc_lo) & (ds <= pr.disc_hi) &
+                           (q < pr.qty_lt) & (i < n);
+        mask[i] = (uint8_t)m;
+        c += m;
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int w = 0; w < CBLOCK / 64; ++w) s += red[w];
+        tile_cnt[blockIdx.x] = s;
+    }
+}
+
+// Exclusive scan of the tile counts (one block): pre[0..nt], pre[nt] = total.
+__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict__ cnt, uint32_t nt,
+                                                     uint64_t *__restrict__ pre, uint64_t *__restrict__ out_count)
+{
+    __shared__ uint64_t part[1024];
+    const uint32_t per = (nt + 1023) / 1024;
+    const uint32_t lo = threadIdx.x * per;
+    uint64_t s = 0;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t i = lo + k;
+        s += (i < nt) ? cnt[i] : 0;
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint64_t v = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint64_t run = part[threadIdx.x] - s; // exclusive prefix of this thread's chunk
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t i = lo + k;
+        if (i < nt) {
+            pre[i] = run;
+            run += cnt[i];
+        }
+    }
+    if (threadIdx.x == 1023) {
+        pre[nt] = part[1023];
+        if (out_count) *out_count = part[1023];
+    }
+}
+
+// ---------------------------------------------------------- local levels
+
+// Levels q = 0 .. CT_LOG-1 inside each 4096-row tile, rows staged in LDS as
+// NW 64-bit words; cpre (LDS) holds the tile's exclusive prefix counts.
+template <int NW>
+__global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> cols, const uint8_t *__restrict__ mask, uint64_t n,
+                                                          const uint64_t *__restrict__ tile_pre)
+{
+    extern __shared__ __align__(16) char lds_raw[];
+    uint64_t *rows = reinterpret_cast<uint64_t *>(lds_raw);              // NW * CT words
+    uint16_t *cpre = reinterpret_cast<uint16_t *>(rows + (size_t)NW * CT); // CT + 1 counts <= 4096
+    __shared__ uint32_t wsum[CBLOCK / 64];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * CT;
+    const uint64_t tbase = tile_pre[blockIdx.x];
+    constexpr int PER = CT / CBLOCK; // 16 consecutive rows per thread for the scan
+
+    // exclusive prefix of sel inside the tile: thread tid owns rows [16 tid, 16 tid + 16)
+    uint32_t sel[PER];
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint64_t i = t0 + tid * PER + k;
+        sel[k] = (mask[i] != 0) & (i < n);
+        run += sel[k];
+    }
+    // block exclusive scan of the per-thread totals
+    uint32_t incl = run;
+    const uint32_t lane = tid & 63;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off);
+        incl += lane >= (uint32_t)off ? v : 0;
+    }
+    if (lane == 63) wsum[tid >> 6] = incl;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (uint32_t w = 0; w < (tid >> 6); ++w) wpre += wsum[w];
+    uint32_t c = wpre + incl - run;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        cpre[tid * PER + k] = (uint16_t)c;
+        c += sel[k];
+    }
+    if (tid == CBLOCK - 1) cpre[CT] = (uint16_t)c;
+    // rows into LDS (coalesced)
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+#pragma unroll 4
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t l = k * CBLOCK + tid;
+            rows[w * CT + l] = ld_col(cols.c[w], t0 + l);
+        }
+    __syncthreads();
+
+    for (int q = 0; q < CT_LOG; ++q) {
+        const uint32_t half = 1u << q;
+#pragma unroll 4
+        for (int k = 0; k < CT / 2 / CBLOCK; ++k) {
+            const uint32_t pi = k * CBLOCK + tid;
+            const uint32_t e0 = ((pi >> q) << (q + 1)) | (pi & (half - 1));
+            const uint32_t e1 = e0 | half;
+            const uint32_t b = e0 & ~(2 * half - 1);
+            const uint64_t cb = tbase + cpre[b];
+            bool sw;
+            if (q == 0) {
+                const bool s0 = cpre[e0 + 1] != cpre[e0];
+                const bool s1 = cpre[e1 + 1] != cpre[e1];
+                sw = ((!s0) & s1) ^ (bool)(cb & 1);
+            } else {
+                sw = level_flag(cb, tbase + cpre[b + half], half, pi & (half - 1));
+            }
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint64_t x = rows[w * CT + e0], y = rows[w * CT + e1];
+                rows[w * CT + e0] = sw ? y : x;
+                rows[w * CT + e1] = sw ? x : y;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+#pragma unroll 4
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t l = k * CBLOCK + tid;
+            st_col(cols.c[w], t0 + l, rows[w * CT + l]);
+        }
+}
+
+// ---------------------------------------------------------- global levels
+
+// Levels q_lo .. q_lo+S-1 (ascending, q_lo >= CT_LOG): thread g owns the 2^S
+// rows base + m * 2^q_lo.  The block start of any pair has every bit below
+// q_lo + 1 clear, so its prefix counts are the same for all lanes of a
+// wavefront and come from the tile prefix array.
+template <int NW, int S>
+__global__ __launch_bounds__(256) void k_compact_global(Cols<NW> cols, const uint64_t *__restrict__ tile_pre, int q_lo)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t base = (g & ((1ull << q_lo) - 1)) | ((g >> q_lo) << (q_lo + S));
+    uint64_t r[1 << S][NW];
+#pragma unroll
+    for (int m = 0; m < (1 << S); ++m)
+#pragma unroll
+        for (int w = 0; w < NW; ++w) r[m][w] = ld_col(cols.c[w], base + ((uint64_t)m << q_lo));
+    // uniform part of base (bits >= 8 are shared by the wavefront)
+    const uint64_t ubase = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
+                           __builtin_amdgcn_readfirstlane((uint32_t)base & ~0xFFu);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const int q = q_lo + k;
+        const uint64_t half = 1ull << q;
+#pragma unroll
+        for (int m = 0; m < (1 << S); ++m) {
+            if (m & (1 << k)) continue;
+            const uint64_t e = base + ((uint64_t)m << q_lo);
+            const uint64_t ue = ubase + ((uint64_t)m << q_lo);
+            const uint64_t b = ue & ~(2 * half - 1);
+            const uint64_t cb = tile_pre[b >> CT_LOG];
+            const uint64_t cbh = tile_pre[(b + half) >> CT_LOG];
+            const bool sw = level_flag(cb, cbh, half, e & (half - 1));
+            const int m2 = m | (1 << k);
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint64_t x = r[m][w], y = r[m2][w];
+                r[m][w] = sw ? y : x;
+                r[m2][w] = sw ? x : y;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < (1 << S); ++m)
+#pragma unroll
+        for (int w = 0; w < NW; ++w) st_col(cols.c[w], base + ((uint64_t)m << q_lo), r[m][w]);
+}
+
+// ------------------------------------------------------- int128 SUM pass
+
+// Exact SUM(a[i] * b[i]) over rows i < count (count read on the device);
+// every row is read, rows >= count contribute a multiplied-out zero.
+__global__ __launch_bounds__(256) void k_sum_prod_i128(const int64_t *__restrict__ a, const int64_t *__restrict__ b,
+                                                       uint64_t n_pad, const uint64_t *__restrict__ count,
+                                                       __int128 *__restrict__ partial)
+{
+    __shared__ __int128 red[256];
+    const uint64_t m = *count;
+    __int128 acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * 256) {
+        const __int128 prod = (__int128)a[i] * (__int128)b[i];
+        acc += prod * (__int128)(i < m);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (uint32_t)off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_final(const __int128 *__restrict__ partial, uint32_t np, const uint64_t *__restrict__ count,
+                            uint64_t *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __int128 s = 0;
+    for (uint32_t i = 0; i < np; ++i) s += partial[i];
+    out[0] = *count;
+    out[1] = 0;
+    out[2] = (uint64_t)s;
+    out[3] = (uint64_t)(s >> 64);
+}
+
+// ------------------------------------------------------------ OP_SCATTER
+
+__global__ __launch_bounds__(CBLOCK) void k_scatter_positions(const uint8_t *__restrict__ mask, uint64_t n,
+                                                              const uint64_t *__restrict__ tile_pre,
+                                                              int32_t *__restrict__ out)
+{
+    __shared__ uint32_t wsum[CBLOCK / 64];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * CT;
+    constexpr int PER = CT / CBLOCK;
+    uint32_t sel[PER], run = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint64_t i = t0 + tid * PER + k;
+        sel[k] = (i < n) ? (mask[i] != 0) : 0;
+        run += sel[k];
+    }
+    uint32_t incl = run;
+    const uint32_t lane = tid & 63;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off);
+        incl += lane >= (uint32_t)off ? v : 0;
+    }
+    if (lane == 63) wsum[tid >> 6] = incl;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (uint32_t w = 0; w < (tid >> 6); ++w) wpre += wsum[w];
+    uint64_t c = tile_pre[blockIdx.x] + wpre + incl - run;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint64_t i = t0 + tid * PER + k;
+        if (sel[k]) out[i] = (int32_t)c; // reference semantics: only selected rows are written
+        c += sel[k];
+    }
+}
+
+// --------------------------------------------------------------- drivers
+
+struct Workspace {
+    uint32_t *tile_cnt;
+    uint64_t *tile_pre;
+    uint64_t *count;
+    uint8_t *mask;
+    __int128 *partial;
+};
+
+constexpr uint32_t kSumBlocks = 2048;
+
+Workspace workspace(vo_ctx *ctx, uint64_t n_pad, bool want_mask)
+{
+    const uint64_t nt = n_pad / CT;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_cnt = take(nt * 4), o_pre = take((nt + 1) * 8), o_count = take(8),
+                 o_part = take(kSumBlocks * sizeof(__int128)), o_mask = take(want_mask ? n_pad : 0);
+    char *base = (char *)scratch(ctx, off);
+    return Workspace{(uint32_t *)(base + o_cnt), (uint64_t *)(base + o_pre), (uint64_t *)(base + o_count),
+                     want_mask ? (uint8_t *)(base + o_mask) : nullptr, (__int128 *)(base + o_part)};
+}
+
+template <int NW>
+void run_network(vo_ctx *ctx, Cols<NW> cols, const uint8_t *mask, uint64_t n, uint64_t n_pad, const uint64_t *tile_pre)
+{
+    const int n_log = ilog2(n_pad);
+    const uint32_t nt = (uint32_t)(n_pad / CT);
+    const size_t lds = (size_t)NW * CT * 8 + (CT + 1) * 2;
+    auto kl = k_compact_local<NW>;
+    VO_HIP(hipFuncSetAttribute((const void *)kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    launch(ctx, LaunchInfo{"compact_local", dim3(nt), dim3(CBLOCK), (uint32_t)lds, hash_mix(NW, n_log)}, kl, cols,
+           mask, n, tile_pre);
+    int q = CT_LOG;
+    while (q < n_log) {
+        const int s = std::min(n_log - q, NW == 1 ? 5 : 4);
+        const uint32_t blocks = (uint32_t)((n_pad >> s) / 256);
+        LaunchInfo li{"compact_global", dim3(blocks), dim3(256), 0, hash_mix(hash_mix(NW, q), s)};
+        switch (s) {
+        case 1: launch(ctx, li, k_compact_global<NW, 1>, cols, tile_pre, q); break;
+        case 2: launch(ctx, li, k_compact_global<NW, 2>, cols, tile_pre, q); break;
+        case 3: launch(ctx, li, k_compact_global<NW, 3>, cols, tile_pre, q); break;
+        case 4: launch(ctx, li, k_compact_global<NW, 4>, cols, tile_pre, q); break;
+        default:
+            if constexpr (NW == 1) {
+                launch(ctx, li, k_compact_global<NW, 5>, cols, tile_pre, q);
+                break;
+            }
+            throw Error{VO_ERR_INVALID, "compaction pass width"};
+        }
+        q += s;
+    }
+}
+
+// Count + scan for a mask already in device memory.
+void count_and_scan(vo_ctx *ctx, const uint8_t *mask, uint64_t n, uint64_t n_pad, const Workspace &ws,
+                    uint64_t *out_count)
+{
+    const uint32_t nt = (uint32_t)(n_pad / CT);
+    launch(ctx, LaunchInfo{"compact_count", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_count_mask, mask, n, ws.tile_cnt);
+    launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)ws.tile_cnt,
+           nt, ws.tile_pre, out_count ? out_count : ws.count);
+}
+
+void check_pad(uint64_t n, uint64_t n_pad)
+{
+    VO_CHECK(is_pow2(n_pad) && n <= n_pad, VO_ERR_INVALID, "n_pad must be a power of two >= n");
+    VO_CHECK(n_pad >= (uint64_t)CT, VO_ERR_INVALID, "compaction needs n_pad >= 4096 (pad further)");
+    VO_CHECK(n_pad <= (1ull << 32), VO_ERR_INVALID, "compaction n_pad above 2^32");
+}
+
+} // namespace
+
+// Compaction of two 8-byte columns by a device mask of n_pad rows (all rows
+// live); used by the join and group-by.  Leaves the count in *count_dev.
+void compact_rows2(vo_ctx *ctx, uint64_t *c0, uint64_t *c1, const uint8_t *mask, uint64_t n_pad, uint64_t *count_dev)
+{
+    check_pad(n_pad, n_pad);
+    // the workspace shares the context arena with the caller's buffers, so
+    // the caller passes its own regions; here only tile arrays are needed
+    const uint32_t nt = (uint32_t)(n_pad / CT);
+    uint32_t *tile_cnt = (uint32_t *)(count_dev + 1);
+    uint64_t *tile_pre = (uint64_t *)(((uintptr_t)(tile_cnt + nt) + 255) & ~(uintptr_t)255);
+    launch(ctx, LaunchInfo{"compact_count", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_count_mask, mask, n_pad, tile_cnt);
+    launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)tile_cnt, nt,
+           tile_pre, count_dev);
+    Cols<2> cs{{{c0, 8}, {c1, 8}}};
+    run_network<2>(ctx, cs, mask, n_pad, n_pad, tile_pre);
+}
+
+// Bytes of device memory compact_rows2 needs after count_dev.
+size_t compact_rows2_scratch(uint64_t n_pad)
+{
+    const uint64_t nt = n_pad / CT;
+    return 8 + nt * 4 + 256 + (nt + 1) * 8 + 256;
+}
+
+void op_compact(vo_ctx *ctx, const vo_compact_args &a)
+{
+    check_pad(a.n, a.n_pad);
+    VO_CHECK(a.mask && a.n_cols >= 1 && a.n_cols <= VO_MAX_SORT_COLUMNS, VO_ERR_INVALID, "bad compaction columns");
+    Workspace ws = workspace(ctx, a.n_pad, false);
+    count_and_scan(ctx, a.mask, a.n, a.n_pad, ws, a.out_count);
+    // columns in pairs of two words per network run; the network depends on
+    // the mask only, so each run applies the same routing
+    uint32_t c = 0;
+    while (c < a.n_cols) {
+        const uint32_t take = std::min<uint32_t>(2, a.n_cols - c);
+        for (uint32_t k = 0; k < take; ++k) {
+            const uint32_t w = a.col_bytes[c + k];
+            VO_CHECK(a.cols[c + k] && (w == 1 || w == 2 || w == 4 || w == 8), VO_ERR_INVALID, "bad column");
+        }
+        if (take == 2) {
+            Cols<2> cs{{{a.cols[c], a.col_bytes[c]}, {a.cols[c + 1], a.col_bytes[c + 1]}}};
+            run_network<2>(ctx, cs, a.mask, a.n, a.n_pad, ws.tile_pre);
+        } else {
+            Cols<1> cs{{{a.cols[c], a.col_bytes[c]}}};
+            run_network<1>(ctx, cs, a.mask, a.n, a.n_pad, ws.tile_pre);
+        }
+        c += take;
+    }
+}
+
+void op_filter_sum(vo_ctx *ctx, const vo_filter_args &a)
+{
+    check_pad(a.n, a.n_pad);
+    VO_CHECK(a.day && a.qty && a.price && a.disc && a.out, VO_ERR_INVALID, "null buffer");
+    VO_CHECK((a.out_price == nullptr) == (a.out_disc == nullptr), VO_ERR_INVALID,
+             "out_price and out_disc are given together");
+    const bool in_place_copy = a.out_price != nullptr;
+    Workspace ws = workspace(ctx, a.n_pad, true);
+    const uint32_t nt = (uint32_t)(a.n_pad / CT);
+    Pred pr{a.day_lo, a.day_hi, a.disc_lo, a.disc_hi, a.qty_lt};
+    launch(ctx, LaunchInfo{"filter_count", dim3(nt), dim3(CBLOCK), 0, a.n_pad}, k_filter_count, a.day, a.qty, a.disc,
+           a.n, pr, ws.mask, ws.tile_cnt);
+    launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)ws.tile_cnt, nt,
+           ws.tile_pre, ws.count);
+    // the network runs on the output columns (a copy of price/disc) or on
+    // the caller's price/disc columns in place when no outputs are given
+    int64_t *op = in_place_copy ? a.out_price : const_cast<int64_t *>(a.price);
+    int64_t *od = in_place_copy ? a.out_disc : const_cast<int64_t *>(a.disc);
+    if (in_place_copy) {
+        VO_HIP(hipMemcpyAsync(op, a.price, a.n_pad * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        VO_HIP(hipMemcpyAsync(od, a.disc, a.n_pad * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    Cols<2> cs{{{op, 8}, {od, 8}}};
+    run_network<2>(ctx, cs, ws.mask, a.n, a.n_pad, ws.tile_pre);
+    launch(ctx, LaunchInfo{"filter_sum", dim3(kSumBlocks), dim3(256), 0, a.n_pad}, k_sum_prod_i128, (const int64_t *)op,
+           (const int64_t *)od, a.n_pad, (const uint64_t *)ws.count, ws.partial);
+    launch(ctx, LaunchInfo{"filter_sum_final", dim3(1), dim3(64), 0, 0}, k_sum_final, (const __int128 *)ws.partial,
+           kSumBlocks, (const uint64_t *)ws.count, a.out);
+}
+
+void op_scatter(vo_ctx *ctx, const vo_scatter_args &a)
+{
+    VO_CHECK(a.in_mask && a.out_indices, VO_ERR_INVALID, "null buffer");
+    if (a.n == 0) {
+        if (a.out_count) VO_HIP(hipMemsetAsync(a.out_count, 0, 8, ctx->stream));
+        return;
+    }
+    // pad the count/scan grid to whole tiles; rows >= n are never read
+    const uint64_t n_pad = ((a.n + CT - 1) / CT) * CT;
+    Workspace ws = workspace(ctx, n_pad, true);
+    const uint32_t nt = (uint32_t)(n_pad / CT);
+    VO_HIP(hipMemsetAsync(ws.mask, 0, n_pad, ctx->stream));
+    VO_HIP(hipMemcpyAsync(ws.mask, a.in_mask, a.n, hipMemcpyDeviceToDevice, ctx->stream));
+    count_and_scan(ctx, ws.mask, a.n, n_pad, ws, a.out_count);
+    launch(ctx, LaunchInfo{"scatter_positions", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_scatter_positions,
+           (const uint8_t *)ws.mask, a.n, (const uint64_t *)ws.tile_pre, a.out_indices);
+}
+
+} // namespace vo

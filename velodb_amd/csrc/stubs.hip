@@ -3,11 +3,10 @@
 
 namespace vo {
 
-void op_scatter(vo_ctx *, const vo_scatter_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_SCATTER is not supported yet"}; }
-void op_compact(vo_ctx *, const vo_compact_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_COMPACT is not supported yet"}; }
+
+
 void op_shuffle(vo_ctx *, const vo_shuffle_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_SHUFFLE is not supported yet"}; }
-void op_filter_sum(vo_ctx *, const vo_filter_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_FILTER_SUM is not supported yet"}; }
-void op_join_groupsum(vo_ctx *, const vo_join_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_PKFK_JOIN_GROUPSUM is not supported yet"}; }
+
 void op_sort_batch(vo_ctx *, const vo_sort_batch_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_SORT_BATCH is not supported yet"}; }
 
 uint64_t queue_submit(vo_ctx *ctx, vo_command *) { ctx->last_error = "queue not built"; ctx->last_status = VO_ERR_QUEUE; return 0; }

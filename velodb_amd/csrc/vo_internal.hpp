@@ -194,6 +194,13 @@ void sort_kv(vo_ctx *ctx, uint64_t *keys, uint64_t *vals, uint64_t n_pad);
 // Sort (u64 key, u32 payload) without padding (all n_pad rows are live).
 void sort_kp_full(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n_pad);
 
+// ---- compaction helpers (compact.hip) used by the join / group-by
+// ORCompact of two 8-byte columns by a device mask over n_pad live rows;
+// the selected count lands in *count_dev, followed by
+// compact_rows2_scratch(n_pad) - 8 bytes of tile arrays the call owns.
+void compact_rows2(vo_ctx *ctx, uint64_t *c0, uint64_t *c1, const uint8_t *mask, uint64_t n_pad, uint64_t *count_dev);
+size_t compact_rows2_scratch(uint64_t n_pad);
+
 } // namespace vo
 
 // ------------------------------------------------------------ device side
