@@ -5,7 +5,6 @@ namespace vo {
 
 
 
-void op_shuffle(vo_ctx *, const vo_shuffle_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_SHUFFLE is not supported yet"}; }
 
 void op_sort_batch(vo_ctx *, const vo_sort_batch_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_SORT_BATCH is not supported yet"}; }
 
