@@ -48,6 +48,40 @@ def cpu_baseline(budget_s=12.0):
                       f"log2N(log2N+1)/2, so at 2^28 the same code would run at ~{rate * 210 / 406:.2f} Mrec/s"}
 
 
+def measured_traffic(family, n):
+    """HBM bytes per launch of `family` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "sort_2p28_traffic.md")
+    if n != 1 << 28 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == family:
+            return int(parts[2])
+    return None
+
+
+def calibrate_copy(steps):
+    """Plain HBM copy of 3 GiB through the library (rocprof counter calibration)."""
+    import torch
+
+    import velodb_amd
+    nbytes = 3 << 30
+    src = torch.empty(nbytes // 8, dtype=torch.int64, device="cuda:0")
+    dst = torch.empty_like(src)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream)
+    ctx.hbm_copy(dst, src, nbytes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.hbm_copy(dst, src, nbytes)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    print(json.dumps({"calibration": "hbm_copy", "bytes_each_way": nbytes, "ms": round(dt * 1e3, 3),
+                      "GBps_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,7 +90,12 @@ def main():
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run the plain 3 GiB HBM copy kernel instead (traffic-counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(max(1, args.steps))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the oblivious sort is not sharded across GPUs in this build "
@@ -97,7 +136,9 @@ def main():
 
     # per-family kernel time inside the timed region
     fams = {}
-    for f in ["sort_local", "sort_global", "sort_merge", "gen_sort"]:
+    # one family per kernel instantiation, so each maps to one rocprof kernel name
+    for f in (["sort_local", "sort_merge", "gen_sort"] + [f"sort_global_s{s}" for s in range(1, 6)] +
+              ["sort_hybrid_s6", "sort_hybrid_s7"]):
         cnt, ms = ctx.kernel_stats(f)
         if cnt:
             fams[f] = (cnt, ms)
@@ -135,7 +176,7 @@ def main():
         "pct_hbm_peak": round(100 * achieved / HBM_PEAK_GBS, 1),
         "per_pass_GBps": per_pass,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, n), "kernel": dom,
                      "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": alg_bytes},
     }
     if not args.no_cpu_baseline:

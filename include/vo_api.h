@@ -362,6 +362,11 @@ VO_EXPORT int vo_kernel_stats_reset(vo_ctx *ctx);
  * per-pass record of (kernel family, level, stages), for planning and docs. */
 VO_EXPORT int vo_sort_plan(vo_ctx *ctx, uint64_t n_pad, uint32_t *n_passes);
 
+/* Plain device-to-device copy of `bytes` (a multiple of 16) with a
+ * streaming kernel: the reference point for HBM traffic counters and the
+ * practical copy bandwidth.  No counterpart in the reference. */
+VO_EXPORT int vo_hbm_copy(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
 /* ------------------------------------------------------ data generators */
 
 /* Device generators of the synthetic workloads (same streams as the

@@ -155,6 +155,9 @@ class Context:
     def sync(self):
         return self.check(self.lib.vo_sync(self.h))
 
+    def hbm_copy(self, dst, src, nbytes):
+        return self.check(self.lib.vo_hbm_copy(self.h, _ptr(dst), _ptr(src), nbytes))
+
     def gen_sort_input(self, seed, keys, payload, n):
         return self.check(self.lib.vo_gen_sort_input(self.h, seed, _ptr(keys), _ptr(payload), n))
 

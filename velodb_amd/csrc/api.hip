@@ -441,7 +441,12 @@ int vo_kernel_stats(vo_ctx *ctx, const char *family, uint64_t *launches, double 
         uint64_t cnt = 0;
         double ms = 0;
         for (auto &t : ctx->timings) {
-            if (family && t.family != family) continue;
+            if (family) {
+                // "name*" matches every family that starts with "name"
+                const size_t fl = std::strlen(family);
+                const bool prefix = fl > 0 && family[fl - 1] == '*';
+                if (prefix ? t.family.compare(0, fl - 1, family, fl - 1) != 0 : t.family != family) continue;
+            }
             float f = 0;
             VO_HIP(hipEventElapsedTime(&f, t.start, t.stop));
             ms += f;
@@ -470,6 +475,15 @@ int vo_sort_plan(vo_ctx *ctx, uint64_t n_pad, uint32_t *n_passes)
     return guarded(ctx, [&] {
         VO_CHECK(is_pow2(n_pad), VO_ERR_INVALID, "n_pad must be a power of two");
         if (n_passes) *n_passes = (uint32_t)sort_plan_passes(ctx, n_pad);
+    });
+}
+
+int vo_hbm_copy(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        op_hbm_copy(ctx, dst, src, bytes);
+        VO_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
 

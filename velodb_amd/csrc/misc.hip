@@ -89,7 +89,26 @@ __global__ void k_gen_fact(uint64_t seed, int32_t *__restrict__ day, int64_t *__
     }
 }
 
-#define VO_TYPE_SWITCH(type_id, BODY)                                                                  \
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// 16 bytes per lane, 4 per thread, non-temporal (streaming) loads and stores.
+__global__ __launch_bounds__(256) void k_hbm_copy(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst, uint64_t n16)
+{
+    const uint64_t base = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
+    u32x4 v[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const uint64_t i = base + m * 256;
+        v[m] = i < n16 ? __builtin_nontemporal_load(src + i) : u32x4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const uint64_t i = base + m * 256;
+        if (i < n16) __builtin_nontemporal_store(v[m], dst + i);
+    }
+}
+
+#define VO_TYPE_SWITCH(type_id, BODY)                                                                \
     switch (type_bytes(type_id)) {                                                                     \
     case 1: { using T = uint8_t; BODY; break; }                                                        \
     case 2: { using T = uint16_t; BODY; break; }                                                       \
@@ -149,6 +168,15 @@ void op_enrich(vo_ctx *ctx, const vo_enrich_args &a)
     VO_CHECK(a.record_ids && a.position_map && a.positions, VO_ERR_INVALID, "null buffer");
     LaunchInfo li{"enrich_positions", grid_for(a.n), dim3(256), 0, a.n};
     launch(ctx, li, k_enrich, a.record_ids, a.position_map, a.positions, a.n);
+}
+
+void op_hbm_copy(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    VO_CHECK(dst && src && bytes % 16 == 0, VO_ERR_INVALID, "copy needs non-null buffers and a multiple of 16 bytes");
+    const uint64_t n16 = bytes / 16;
+    if (n16 == 0) return;
+    LaunchInfo li{"hbm_copy", dim3((uint32_t)((n16 + 1023) / 1024)), dim3(256), 0, bytes};
+    launch(ctx, li, k_hbm_copy, (const u32x4 *)src, (u32x4 *)dst, n16);
 }
 
 void op_gen_sort_input(vo_ctx *ctx, uint64_t seed, uint64_t *keys, uint32_t *payload, uint64_t n)

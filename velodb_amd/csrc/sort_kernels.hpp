@@ -452,7 +452,8 @@ struct SortImpl {
         if constexpr (S <= H_MAX && T_LOG - S >= 6) {
             using TL = dev::TileCT<R, T_LOG, R_LOG, TRACE>;
             const size_t lds = TL::lds_bytes();
-            LaunchInfo li{"sort_global", dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BLOCK), (uint32_t)lds,
+            static const char *const names[] = {"sort_hybrid_s6", "sort_hybrid_s7"};
+            LaunchInfo li{names[S - 6], dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BLOCK), (uint32_t)lds,
                           hash_mix(hash_mix(p, q_lo), hash_mix(S + 100, p_top))};
             auto k = dev::k_sort_hybrid<R, T_LOG, R_LOG, BLOCK, S, TRACE>;
             VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -473,7 +474,10 @@ struct SortImpl {
     {
         const uint64_t threads = n_virtual >> s;
         VO_CHECK(threads >= 256 && q_lo >= 8, VO_ERR_INVALID, "global pass below one block");
-        LaunchInfo li{"sort_global", dim3((uint32_t)(threads / 256)), dim3(256), 0,
+        static const char *const names[] = {"sort_global_s0", "sort_global_s1", "sort_global_s2", "sort_global_s3",
+                                            "sort_global_s4", "sort_global_s5"};
+        VO_CHECK(s >= 1 && s <= 5, VO_ERR_INVALID, "bad fused stage count");
+        LaunchInfo li{names[s], dim3((uint32_t)(threads / 256)), dim3(256), 0,
                       hash_mix(hash_mix(p, q_lo), hash_mix(s, p_top))};
         switch (s) {
         case 1: launch_global<TRACE, 1>(ctx, li, b, map, p, q_lo, p_top); break;

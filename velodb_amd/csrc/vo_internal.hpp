@@ -181,6 +181,7 @@ void op_gen_sort_input(vo_ctx *ctx, uint64_t seed, uint64_t *keys, uint32_t *pay
 void op_gen_fact(vo_ctx *ctx, uint64_t seed, int32_t *day, int64_t *qty, int64_t *price, int64_t *disc,
                  uint64_t n);
 int sort_plan_passes(vo_ctx *ctx, uint64_t n_pad);
+void op_hbm_copy(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
 // ---- resident command queue (queue.hip); used when ctx->queue is set
 uint64_t queue_submit(vo_ctx *ctx, vo_command *cmd);
