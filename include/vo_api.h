@@ -362,6 +362,19 @@ VO_EXPORT int vo_kernel_stats_reset(vo_ctx *ctx);
  * per-pass record of (kernel family, level, stages), for planning and docs. */
 VO_EXPORT int vo_sort_plan(vo_ctx *ctx, uint64_t n_pad, uint32_t *n_passes);
 
+/* Autotuner (no counterpart in the reference, which hard-codes its launch
+ * shape, src/cuda/task_manager.cu:32-55).  For each size class
+ * 2^log_min .. 2^log_max it times the schedule variants (stages per
+ * register pass, stages per LDS-staged pass, resident merge blocks) on a
+ * synthetic batch of about rows_per_class rows and keeps the fastest for
+ * that class in the context; vo_sort_batch then uses it.  The choice
+ * depends only on public sizes.  chosen (optional, log_max-log_min+1
+ * entries) receives the tuning words (vo_config.tuning format). */
+VO_EXPORT int vo_autotune_sort(vo_ctx *ctx, uint32_t log_min, uint32_t log_max, uint64_t rows_per_class,
+                               uint32_t *chosen);
+VO_EXPORT int vo_set_class_tuning(vo_ctx *ctx, uint32_t log_n, uint32_t tuning_word);
+VO_EXPORT int vo_get_class_tuning(vo_ctx *ctx, uint32_t log_n, uint32_t *tuning_word);
+
 /* Plain device-to-device copy of `bytes` (a multiple of 16) with a
  * streaming kernel: the reference point for HBM traffic counters and the
  * practical copy bandwidth.  No counterpart in the reference. */

@@ -155,6 +155,19 @@ class Context:
     def sync(self):
         return self.check(self.lib.vo_sync(self.h))
 
+    def autotune_sort(self, log_min, log_max, rows_per_class=1 << 24):
+        out = (C.c_uint32 * (log_max - log_min + 1))()
+        self.check(self.lib.vo_autotune_sort(self.h, log_min, log_max, rows_per_class, out))
+        return {log_min + i: int(w) for i, w in enumerate(out)}
+
+    def set_class_tuning(self, log_n, word):
+        return self.check(self.lib.vo_set_class_tuning(self.h, log_n, word))
+
+    def get_class_tuning(self, log_n):
+        w = C.c_uint32()
+        self.check(self.lib.vo_get_class_tuning(self.h, log_n, C.byref(w)))
+        return w.value
+
     def hbm_copy(self, dst, src, nbytes):
         return self.check(self.lib.vo_hbm_copy(self.h, _ptr(dst), _ptr(src), nbytes))
 

@@ -160,6 +160,9 @@ PROTOTYPES = [
     ("vo_kernel_stats", C.c_int, [P, C.c_char_p, C.POINTER(u64), C.POINTER(C.c_double)]),
     ("vo_kernel_stats_reset", C.c_int, [P]),
     ("vo_sort_plan", C.c_int, [P, u64, C.POINTER(u32)]),
+    ("vo_autotune_sort", C.c_int, [P, u32, u32, u64, C.POINTER(u32)]),
+    ("vo_set_class_tuning", C.c_int, [P, u32, u32]),
+    ("vo_get_class_tuning", C.c_int, [P, u32, C.POINTER(u32)]),
     ("vo_hbm_copy", C.c_int, [P, P, P, u64]),
     ("vo_gen_sort_input", C.c_int, [P, u64, P, P, u64]),
     ("vo_gen_fact", C.c_int, [P, u64, P, P, P, P, u64]),

@@ -242,10 +242,7 @@ int vo_ctx_create(const vo_config *cfg, vo_ctx **out)
         hipDeviceProp_t prop;
         VO_HIP(hipGetDeviceProperties(&prop, dev));
         ctx->num_cus = prop.multiProcessorCount;
-        // tuning word: bits 0-3 register-pass stages, bits 4-7 hybrid-pass stages (+1, so 0 keeps the default)
-        if (c.tuning & 0xF) ctx->tuning.max_fused_stages = (int)(c.tuning & 0xF);
-        if (c.tuning & 0xF0) ctx->tuning.max_hybrid_stages = (int)((c.tuning >> 4) & 0xF) - 1;
-        if (c.tuning & 0x100) ctx->tuning.pipe_merge = !ctx->tuning.pipe_merge;
+        ctx->tuning = apply_tuning_word(ctx->tuning, c.tuning);
     } catch (const Error &e) {
         delete ctx;
         return e.status;
@@ -475,6 +472,31 @@ int vo_sort_plan(vo_ctx *ctx, uint64_t n_pad, uint32_t *n_passes)
     return guarded(ctx, [&] {
         VO_CHECK(is_pow2(n_pad), VO_ERR_INVALID, "n_pad must be a power of two");
         if (n_passes) *n_passes = (uint32_t)sort_plan_passes(ctx, n_pad);
+    });
+}
+
+int vo_autotune_sort(vo_ctx *ctx, uint32_t log_min, uint32_t log_max, uint64_t rows_per_class, uint32_t *chosen)
+{
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        VO_CHECK(!ctx->queue && !ctx->tracing, VO_ERR_INVALID, "autotune runs in launch mode without a trace");
+        autotune_sort(ctx, (int)log_min, (int)log_max, rows_per_class, chosen);
+    });
+}
+
+int vo_set_class_tuning(vo_ctx *ctx, uint32_t log_n, uint32_t tuning_word)
+{
+    return guarded(ctx, [&] {
+        VO_CHECK(log_n <= 32, VO_ERR_INVALID, "log_n above 32");
+        ctx->class_tuning[log_n] = tuning_word;
+    });
+}
+
+int vo_get_class_tuning(vo_ctx *ctx, uint32_t log_n, uint32_t *tuning_word)
+{
+    return guarded(ctx, [&] {
+        VO_CHECK(log_n <= 32 && tuning_word, VO_ERR_INVALID, "bad arguments");
+        *tuning_word = ctx->class_tuning[log_n];
     });
 }
 

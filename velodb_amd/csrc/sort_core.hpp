@@ -175,12 +175,25 @@ __device__ __forceinline__ bool dir_asc(uint64_t g, int p, int p_top)
 struct IMap {
     const uint64_t *offsets; // nullptr -> identity
     int c;                   // log2 instance size when offsets != nullptr
+    const uint64_t *ns;      // live rows per instance (pad fill), or nullptr
 };
 
 __device__ __forceinline__ uint64_t phys(const IMap &m, uint64_t v)
 {
     if (m.offsets == nullptr) return v;
     return m.offsets[v >> m.c] + (v & ((1ull << m.c) - 1));
+}
+
+// Pad fill for virtual row v: its row index inside its instance and the
+// instance's live row count.
+__device__ __forceinline__ uint64_t local_row(const IMap &m, uint64_t v)
+{
+    return m.offsets == nullptr ? v : (v & ((1ull << m.c) - 1));
+}
+
+__device__ __forceinline__ uint64_t live_rows(const IMap &m, uint64_t v, uint64_t n_valid)
+{
+    return m.ns == nullptr ? n_valid : m.ns[v >> m.c];
 }
 
 template <class R, bool TRACE>

@@ -173,7 +173,7 @@ __global__ void k_sort_tiny(Bufs<R> b, int n_log, uint64_t n_valid, int fill, Tr
 // Tiles of 2^t records for t < T_LOG (small arrays): levels 1 .. p_max with
 // runtime layouts.  Loads and stores use the coalesced layout a = t - R_LOG.
 template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
-__global__ __launch_bounds__(BLOCK) void k_sort_local(Bufs<R> b, int t, int p_max, int p_top, uint64_t n_valid,
+__global__ __launch_bounds__(BLOCK) void k_sort_local(Bufs<R> b, IMap map, int t, int p_max, int p_top, uint64_t n_valid,
                                                       int fill, TraceSlot *slot)
 {
     extern __shared__ __align__(16) char lds_raw[];
@@ -191,8 +191,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_local(Bufs<R> b, int t, int p_ma
 #pragma unroll
         for (int m = 0; m < (1 << R_LOG); ++m) {
             const uint64_t v = vbase + (base | ((uint32_t)m << a));
-            gload<R, TRACE>(r[m], b, v, h, nops);
-            if (fill) pad_select(r[m], v, n_valid);
+            gload<R, TRACE>(r[m], b, phys(map, v), h, nops);
+            if (fill) pad_select(r[m], local_row(map, v), live_rows(map, v, n_valid));
         }
     }
     for (int p = 1; p <= p_max; ++p) tile_level<R, R_LOG, TRACE>(r, s, tau, act, a, p, p - 1, p_top, vbase, h, nops);
@@ -204,7 +204,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_local(Bufs<R> b, int t, int p_ma
     if (act) {
         const uint32_t base = lay_base<R_LOG>(tau, a);
 #pragma unroll
-        for (int m = 0; m < (1 << R_LOG); ++m) gstore<R, TRACE>(r[m], b, vbase + (base | ((uint32_t)m << a)), h, nops);
+        for (int m = 0; m < (1 << R_LOG); ++m)
+            gstore<R, TRACE>(r[m], b, phys(map, vbase + (base | ((uint32_t)m << a))), h, nops);
     }
     if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
 }
@@ -227,7 +228,10 @@ __global__ __launch_bounds__(BLOCK) void k_sort_local_ct(Bufs<R> b, IMap map, in
 #pragma unroll
     for (int m = 0; m < (1 << R_LOG); ++m) {
         gload_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
-        if (fill) pad_select(r[m], vbase + (base | ((uint32_t)m << A0)), n_valid);
+        if (fill) {
+            const uint64_t v = vbase + (base | ((uint32_t)m << A0));
+            pad_select(r[m], local_row(map, v), live_rows(map, vbase, n_valid));
+        }
     }
     const bool asc_hi = dir_asc(vbase, T_LOG, p_top);
     TL::template levels<1, A0>(r, lds, tau, asc_hi, p_top, h, nops);
@@ -511,10 +515,9 @@ struct SortImpl {
                     VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                     launch(ctx, li, k, b, map, p_top, n_valid, fill, trace_slot(ctx));
                 } else {
-                    VO_CHECK(map.offsets == nullptr, VO_ERR_INVALID, "mapped sort below one tile");
                     auto k = dev::k_sort_local<R, T_LOG, R_LOG, BLOCK, TRACE>;
                     VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                    launch(ctx, li, k, b, t, std::min(t, p_top), p_top, n_valid, fill, trace_slot(ctx));
+                    launch(ctx, li, k, b, map, t, std::min(t, p_top), p_top, n_valid, fill, trace_slot(ctx));
                 }
                 break;
             }
@@ -553,11 +556,24 @@ struct SortImpl {
     {
         VO_CHECK((1ull << n_log) <= kMaxSortRows, VO_ERR_INVALID,
                  "n_pad above 2^29 (32-bit buffer offsets of 8-byte columns)");
-        IMap map{nullptr, 0};
+        IMap map{nullptr, 0, nullptr};
         if (ctx->tracing)
             run_t<true>(ctx, b, n_log, n_valid, fill, map, n_log, 1ull << n_log);
         else
             run_t<false>(ctx, b, n_log, n_valid, fill, map, n_log, 1ull << n_log);
+    }
+
+    // `count` independent instances of 2^c rows, instance i at element
+    // offsets[i] with ns[i] live rows (both device arrays): one schedule for
+    // all of them, every launch covering every instance.
+    static void run_mapped(vo_ctx *ctx, Bufs<R> b, int c, uint64_t count, const uint64_t *offsets, const uint64_t *ns)
+    {
+        VO_CHECK(c >= R_LOG && (1ull << c) <= kMaxSortRows, VO_ERR_INVALID, "batched instance size out of range");
+        IMap map{offsets, c, ns};
+        if (ctx->tracing)
+            run_t<true>(ctx, b, c, 0, 1, map, c, count << c);
+        else
+            run_t<false>(ctx, b, c, 0, 1, map, c, count << c);
     }
 
     static int passes(vo_ctx *ctx, int n_log) { return (int)plan_passes(n_log, T_LOG, smax(ctx), R_LOG, hmax(ctx)).size(); }

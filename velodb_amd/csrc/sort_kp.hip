@@ -40,4 +40,138 @@ void sort_kv(vo_ctx *ctx, uint64_t *keys, uint64_t *vals, uint64_t n_pad)
 
 int sort_plan_passes(vo_ctx *ctx, uint64_t n_pad) { return SortKP::passes(ctx, ilog2(n_pad)); }
 
+// ------------------------------------------------------------ batched sorts
+
+namespace {
+
+// Runs one size class with the class's tuning word (autotuned or default).
+void run_class(vo_ctx *ctx, Bufs<RecKP> b, int c, uint64_t count, const uint64_t *offsets, const uint64_t *ns)
+{
+    const SortTuning saved = ctx->tuning;
+    ctx->tuning = apply_tuning_word(saved, ctx->class_tuning[c]);
+    try {
+        SortKP::run_mapped(ctx, b, c, count, offsets, ns);
+    } catch (...) {
+        ctx->tuning = saved;
+        throw;
+    }
+    ctx->tuning = saved;
+}
+
+} // namespace
+
+// Independent sorts of uneven public sizes, bucketed by power-of-two size
+// class: every class is one schedule (one launch per pass for all of its
+// instances), instead of one schedule per instance.
+void op_sort_batch(vo_ctx *ctx, const vo_sort_batch_args &a)
+{
+    VO_CHECK(a.keys && a.payload && (a.items || a.n_items == 0), VO_ERR_INVALID, "null buffer");
+    std::vector<std::vector<uint64_t>> off(33), ns(33);
+    for (uint64_t i = 0; i < a.n_items; ++i) {
+        const vo_batch_item &it = a.items[i];
+        VO_CHECK(is_pow2(it.n_pad) && it.n <= it.n_pad && it.offset % it.n_pad == 0, VO_ERR_INVALID,
+                 "batch item: n_pad must be a power of two >= n dividing offset");
+        VO_CHECK(it.n_pad <= kMaxSortRows, VO_ERR_INVALID, "batch item above 2^29 rows");
+        const int c = ilog2(it.n_pad);
+        off[c].push_back(it.offset);
+        ns[c].push_back(it.n);
+    }
+    // tables for every class in one upload (offsets then live counts)
+    std::vector<uint64_t> host;
+    std::vector<size_t> at(33, 0);
+    for (int c = 0; c <= 32; ++c) {
+        at[c] = host.size();
+        host.insert(host.end(), off[c].begin(), off[c].end());
+        host.insert(host.end(), ns[c].begin(), ns[c].end());
+    }
+    if (host.empty()) return;
+    uint64_t *dtab = (uint64_t *)scratch(ctx, host.size() * 8);
+    VO_HIP(hipMemcpyAsync(dtab, host.data(), host.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    Bufs<RecKP> b;
+    b.k[0] = a.keys;
+    b.p = a.payload;
+    for (int c = 32; c >= 0; --c) {
+        const uint64_t cnt = off[c].size();
+        if (cnt == 0) continue;
+        if (c < 4) {
+            // instances of at most 8 rows: one small launch each
+            for (uint64_t i = 0; i < cnt; ++i) op_sort_kp(ctx, a.keys + off[c][i], a.payload + off[c][i], ns[c][i], 1ull << c);
+            continue;
+        }
+        run_class(ctx, b, c, cnt, dtab + at[c], dtab + at[c] + cnt);
+    }
+    // the host table must outlive its upload
+    VO_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// Autotuner: for each size class in [log_min, log_max], times every
+// schedule variant on a synthetic batch of that class (public sizes only)
+// and keeps the fastest tuning word in ctx->class_tuning.
+void autotune_sort(vo_ctx *ctx, int log_min, int log_max, uint64_t rows_per_class, uint32_t *chosen)
+{
+    VO_CHECK(log_min >= 4 && log_max <= 29 && log_min <= log_max, VO_ERR_INVALID, "autotune class range 4..29");
+    static const uint32_t candidates[] = {
+        0x5 | (8u << 4),         0x5 | (8u << 4) | 0x100, 0x5 | (7u << 4), 0x5 | (1u << 4),
+        0x4 | (8u << 4),         0x4 | (1u << 4),         0x3 | (8u << 4), 0x5 | (1u << 4) | 0x100,
+    };
+    for (int c = log_min; c <= log_max; ++c) {
+        const uint64_t inst = 1ull << c;
+        const uint64_t count = std::max<uint64_t>(1, rows_per_class / inst);
+        const uint64_t total = count * inst;
+        uint64_t *keys = nullptr;
+        uint32_t *pay = nullptr;
+        VO_HIP(hipMalloc(&keys, total * 8));
+        hipError_t e = hipMalloc(&pay, total * 4);
+        if (e != hipSuccess) {
+            (void)hipFree(keys);
+            VO_HIP(e);
+        }
+        std::vector<uint64_t> host(2 * count);
+        for (uint64_t i = 0; i < count; ++i) {
+            host[i] = i * inst;
+            host[count + i] = inst;
+        }
+        uint64_t *dtab = nullptr;
+        VO_HIP(hipMalloc(&dtab, host.size() * 8));
+        VO_HIP(hipMemcpy(dtab, host.data(), host.size() * 8, hipMemcpyHostToDevice));
+        Bufs<RecKP> b;
+        b.k[0] = keys;
+        b.p = pay;
+        hipEvent_t e0, e1;
+        VO_HIP(hipEventCreate(&e0));
+        VO_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        uint32_t best_w = 0;
+        const uint32_t saved_word = ctx->class_tuning[c];
+        for (uint32_t w : candidates) {
+            ctx->class_tuning[c] = w;
+            op_gen_sort_input(ctx, 7, keys, pay, total);
+            run_class(ctx, b, c, count, dtab, dtab + count); // warm-up
+            float ms_total = 0;
+            for (int rep = 0; rep < 3; ++rep) {
+                op_gen_sort_input(ctx, 8 + rep, keys, pay, total);
+                VO_HIP(hipEventRecord(e0, ctx->stream));
+                run_class(ctx, b, c, count, dtab, dtab + count);
+                VO_HIP(hipEventRecord(e1, ctx->stream));
+                VO_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                VO_HIP(hipEventElapsedTime(&ms, e0, e1));
+                ms_total += ms;
+            }
+            if (ms_total < best) {
+                best = ms_total;
+                best_w = w;
+            }
+        }
+        (void)saved_word;
+        ctx->class_tuning[c] = best_w;
+        if (chosen) chosen[c - log_min] = best_w;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(dtab);
+        (void)hipFree(pay);
+        (void)hipFree(keys);
+    }
+}
+
 } // namespace vo

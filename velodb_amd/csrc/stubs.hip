@@ -6,7 +6,6 @@ namespace vo {
 
 
 
-void op_sort_batch(vo_ctx *, const vo_sort_batch_args &) { throw Error{VO_ERR_UNSUPPORTED, "OP_SORT_BATCH is not supported yet"}; }
 
 uint64_t queue_submit(vo_ctx *ctx, vo_command *) { ctx->last_error = "queue not built"; ctx->last_status = VO_ERR_QUEUE; return 0; }
 int queue_wait(vo_ctx *, uint64_t, uint32_t) { return VO_ERR_QUEUE; }

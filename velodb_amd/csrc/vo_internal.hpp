@@ -105,6 +105,7 @@ struct vo_ctx {
     std::vector<vo::TimingRecord> timings;
 
     vo::SortTuning tuning;
+    uint32_t class_tuning[33] = {}; // per log2(n_pad) tuning word from the autotuner (0 = context default)
 
     // persistent queue (queue.hip)
     void *queue = nullptr;
@@ -163,6 +164,17 @@ inline int ilog2(uint64_t v)
 
 inline bool is_pow2(uint64_t v) { return v && !(v & (v - 1)); }
 
+// Tuning word (vo_config.tuning): bits 0-3 register-pass stages, bits 4-7
+// hybrid-pass stages + 1, bit 8 toggles resident merge blocks.  0 keeps the
+// defaults.
+inline SortTuning apply_tuning_word(SortTuning t, uint32_t w)
+{
+    if (w & 0xF) t.max_fused_stages = (int)(w & 0xF);
+    if (w & 0xF0) t.max_hybrid_stages = (int)((w >> 4) & 0xF) - 1;
+    if (w & 0x100) t.pipe_merge = !t.pipe_merge;
+    return t;
+}
+
 // ---- op entry points implemented in the .hip files (enqueue only)
 void op_sort_kp(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad);
 void op_sort_multi(vo_ctx *ctx, const vo_sort_args &a);
@@ -182,6 +194,7 @@ void op_gen_fact(vo_ctx *ctx, uint64_t seed, int32_t *day, int64_t *qty, int64_t
                  uint64_t n);
 int sort_plan_passes(vo_ctx *ctx, uint64_t n_pad);
 void op_hbm_copy(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+void autotune_sort(vo_ctx *ctx, int log_min, int log_max, uint64_t rows_per_class, uint32_t *chosen);
 
 // ---- resident command queue (queue.hip); used when ctx->queue is set
 uint64_t queue_submit(vo_ctx *ctx, vo_command *cmd);
