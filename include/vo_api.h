@@ -380,6 +380,59 @@ VO_EXPORT int vo_get_class_tuning(vo_ctx *ctx, uint32_t log_n, uint32_t *tuning_
  * practical copy bandwidth.  No counterpart in the reference. */
 VO_EXPORT int vo_hbm_copy(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
+/* ----------------------------------------------------------- serving loop */
+
+/* A continuous-batching query server (no counterpart in the reference,
+ * whose ExecutionEngine runs one query at a time,
+ * src/execution/execution_engine.cpp:67-142).  Tables live on the device
+ * for the server's lifetime.  A run replays a list of queries with arrival
+ * times: a query is admitted when it has arrived, a slot is free and its
+ * padded HBM footprint fits in the remaining budget; admitted queries run
+ * concurrently, one HIP stream per slot.  The schedule depends only on
+ * public sizes and arrival times. */
+enum { VO_Q_FILTER_SUM = 1, VO_Q_JOIN_GROUPSUM = 2, VO_Q_SORT = 3 };
+
+typedef struct vo_server_config {
+    uint32_t n_slots;          /* concurrent queries (1 = the plain loop) */
+    uint32_t device;
+    uint64_t hbm_budget_bytes; /* admission budget for query intermediates */
+} vo_server_config;
+
+typedef struct vo_query {
+    uint32_t kind;  /* VO_Q_* */
+    uint32_t table; /* table id of that kind */
+    double arrival_s;
+} vo_query;
+
+typedef struct vo_query_result {
+    double admit_s, end_s, latency_s; /* relative to the run start; latency = end - arrival */
+    uint64_t footprint_bytes;
+    uint64_t r[4]; /* filter: count, sum_lo, sum_hi; join: groups, sum-weighted checksum, total;
+                      sort: rows, key checksum, payload checksum */
+    int32_t status;
+    uint32_t slot;
+} vo_query_result;
+
+typedef struct vo_run_stats {
+    double makespan_s, qps, p50_s, p99_s, mean_s;
+    uint64_t peak_bytes_in_flight;
+    uint32_t max_concurrent;
+} vo_run_stats;
+
+typedef struct vo_server vo_server;
+
+VO_EXPORT int vo_server_create(const vo_server_config *cfg, vo_server **out);
+VO_EXPORT int vo_server_destroy(vo_server *s);
+VO_EXPORT const char *vo_server_last_error(const vo_server *s);
+/* Device-resident tables; each returns its id through *id. */
+VO_EXPORT int vo_server_add_fact_table(vo_server *s, uint64_t seed, uint64_t n_rows, uint32_t *id);
+VO_EXPORT int vo_server_add_join_tables(vo_server *s, uint64_t seed, uint64_t n_dim, uint64_t n_fact, uint32_t *id);
+VO_EXPORT int vo_server_add_sort_table(vo_server *s, uint64_t seed, uint64_t n_rows, uint32_t *id);
+/* Replays n queries; n_slots_override (0 = the server's) picks the
+ * concurrency, 1 gives the one-query-at-a-time loop. */
+VO_EXPORT int vo_server_run(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_slots_override,
+                            vo_query_result *out, vo_run_stats *stats);
+
 /* ------------------------------------------------------ data generators */
 
 /* Device generators of the synthetic workloads (same streams as the

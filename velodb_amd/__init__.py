@@ -48,6 +48,60 @@ def _ptr(x):
     return x.data_ptr()
 
 
+class Server:
+    """Continuous-batching query server over device-resident tables
+    (vo_server_* in include/vo_api.h)."""
+
+    FILTER_SUM, JOIN_GROUPSUM, SORT = _abi.Q_FILTER_SUM, _abi.Q_JOIN_GROUPSUM, _abi.Q_SORT
+
+    def __init__(self, n_slots=16, hbm_budget_bytes=200 << 30, device=0):
+        self.lib = load_library()
+        cfg = _abi.ServerConfig(n_slots, device, hbm_budget_bytes)
+        h = C.c_void_p()
+        st = self.lib.vo_server_create(C.byref(cfg), C.byref(h))
+        if st != 0:
+            raise VoError(st, "vo_server_create failed")
+        self.h = h
+
+    def _check(self, st):
+        if st != 0:
+            raise VoError(st, self.lib.vo_server_last_error(self.h).decode())
+
+    def add_fact_table(self, seed, n_rows):
+        i = C.c_uint32()
+        self._check(self.lib.vo_server_add_fact_table(self.h, seed, n_rows, C.byref(i)))
+        return i.value
+
+    def add_join_tables(self, seed, n_dim, n_fact):
+        i = C.c_uint32()
+        self._check(self.lib.vo_server_add_join_tables(self.h, seed, n_dim, n_fact, C.byref(i)))
+        return i.value
+
+    def add_sort_table(self, seed, n_rows):
+        i = C.c_uint32()
+        self._check(self.lib.vo_server_add_sort_table(self.h, seed, n_rows, C.byref(i)))
+        return i.value
+
+    def run(self, queries, n_slots=0):
+        """queries: list of (kind, table, arrival_s).  Returns (results, stats)."""
+        arr = (_abi.Query * len(queries))(*[_abi.Query(k, t, a) for (k, t, a) in queries])
+        res = (_abi.QueryResult * len(queries))()
+        st = _abi.RunStats()
+        self._check(self.lib.vo_server_run(self.h, arr, len(queries), n_slots, res, C.byref(st)))
+        return list(res), st
+
+    def close(self):
+        if self.h:
+            self.lib.vo_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 class Context:
     """One vo_ctx: a HIP stream plus submission bookkeeping.
 

@@ -127,6 +127,27 @@ class Config(C.Structure):
                 ("wait_timeout_ms", u32), ("tuning", u32)]
 
 
+Q_FILTER_SUM, Q_JOIN_GROUPSUM, Q_SORT = 1, 2, 3
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("n_slots", u32), ("device", u32), ("hbm_budget_bytes", u64)]
+
+
+class Query(C.Structure):
+    _fields_ = [("kind", u32), ("table", u32), ("arrival_s", C.c_double)]
+
+
+class QueryResult(C.Structure):
+    _fields_ = [("admit_s", C.c_double), ("end_s", C.c_double), ("latency_s", C.c_double),
+                ("footprint_bytes", u64), ("r", u64 * 4), ("status", i32), ("slot", u32)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("makespan_s", C.c_double), ("qps", C.c_double), ("p50_s", C.c_double), ("p99_s", C.c_double),
+                ("mean_s", C.c_double), ("peak_bytes_in_flight", u64), ("max_concurrent", u32)]
+
+
 class Trace(C.Structure):
     _fields_ = [("n_launches", u64), ("launch_digest", u64), ("address_digest", u64), ("n_addresses", u64)]
 
@@ -164,6 +185,13 @@ PROTOTYPES = [
     ("vo_set_class_tuning", C.c_int, [P, u32, u32]),
     ("vo_get_class_tuning", C.c_int, [P, u32, C.POINTER(u32)]),
     ("vo_hbm_copy", C.c_int, [P, P, P, u64]),
+    ("vo_server_create", C.c_int, [C.POINTER(ServerConfig), C.POINTER(P)]),
+    ("vo_server_destroy", C.c_int, [P]),
+    ("vo_server_last_error", C.c_char_p, [P]),
+    ("vo_server_add_fact_table", C.c_int, [P, u64, u64, C.POINTER(u32)]),
+    ("vo_server_add_join_tables", C.c_int, [P, u64, u64, u64, C.POINTER(u32)]),
+    ("vo_server_add_sort_table", C.c_int, [P, u64, u64, C.POINTER(u32)]),
+    ("vo_server_run", C.c_int, [P, C.POINTER(Query), u64, u32, C.POINTER(QueryResult), C.POINTER(RunStats)]),
     ("vo_gen_sort_input", C.c_int, [P, u64, P, P, u64]),
     ("vo_gen_fact", C.c_int, [P, u64, P, P, P, P, u64]),
 ]

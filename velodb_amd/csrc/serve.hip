@@ -1,0 +1,532 @@
+// serve.hip -- continuous-batching query server over device-resident tables.
+//
+// The reference executes one query at a time through its pull-based engine
+// (src/execution/execution_engine.cpp:67-142) and re-transfers tables per
+// query (include/data/value_vector.hpp:248-272).  Here tables stay in HBM
+// for the server's lifetime and a run replays a list of queries with
+// arrival times through n_slots worker slots, each a library context with
+// its own HIP stream:
+//   * admission: a free slot takes the earliest query (in arrival order) that
+//     has arrived, when its padded HBM footprint fits in the budget left by
+//     the queries in flight (FIFO, no reordering);
+//   * the admitted query's operators run on that slot's stream, so queries in
+//     flight overlap on the device;
+//   * completion frees the slot and the footprint; latency = end - arrival.
+// n_slots = 1 is the plain one-query-at-a-time loop.  Everything the
+// schedule looks at (arrival times, table sizes) is public.
+#include "vo_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <thread>
+
+struct vo_server;
+
+namespace vo {
+namespace {
+
+struct FactTable {
+    uint64_t n, n_pad;
+    int32_t *day;
+    int64_t *qty, *price, *disc;
+};
+
+struct JoinTable {
+    uint64_t n_dim, n_fact, n_pad;
+    int32_t *dk, *dg, *fk;
+    int64_t *fv;
+};
+
+struct SortTable {
+    uint64_t n, n_pad;
+    uint64_t *keys;
+    uint32_t *pay;
+};
+
+uint64_t pow2_at_least(uint64_t n, uint64_t floor_)
+{
+    uint64_t p = floor_;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+__global__ void k_gen_join_cols(uint64_t seed, const uint32_t *__restrict__ perm, uint64_t n_dim, uint64_t n_fact,
+                                int32_t *__restrict__ dk, int32_t *__restrict__ dg, int32_t *__restrict__ fk,
+                                int64_t *__restrict__ fv)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_dim; i += stride) {
+        dk[i] = (int32_t)perm[i];
+        dg[i] = (int32_t)dev::rand_range(seed, 6, i, 0, 25);
+    }
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_fact; i += stride) {
+        // foreign keys uniform over the dimension keys (generation reads perm at a random row)
+        fk[i] = (int32_t)perm[dev::rand_range(seed, 7, i, 0, n_dim)];
+        fv[i] = dev::rand_range(seed, 8, i, 100, 10000000 - 100 + 1);
+    }
+}
+
+__global__ void k_iota1(uint32_t *__restrict__ p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)(i + 1);
+}
+
+// Order-sensitive checksums of a sorted table: sum key[i]*(2i+1), sum pay[i]*(i+1) mod 2^64.
+__global__ __launch_bounds__(256) void k_sort_checksum(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ pay,
+                                                       uint64_t n, uint64_t *__restrict__ partial)
+{
+    __shared__ uint64_t a[256], b[256];
+    uint64_t x = 0, y = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        x += keys[i] * (2 * i + 1);
+        y += (uint64_t)pay[i] * (i + 1);
+    }
+    a[threadIdx.x] = x;
+    b[threadIdx.x] = y;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (uint32_t)off) {
+            a[threadIdx.x] += a[threadIdx.x + off];
+            b[threadIdx.x] += b[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = a[0];
+        partial[2 * blockIdx.x + 1] = b[0];
+    }
+}
+
+constexpr uint32_t kChecksumBlocks = 256;
+constexpr uint64_t kMaxGroups = 64;
+
+// Query intermediates that live outside the slot context's scratch arena.
+uint64_t query_buffer_bytes(uint32_t kind, uint64_t n_pad)
+{
+    switch (kind) {
+    case VO_Q_FILTER_SUM: return 16 * n_pad + 64;
+    case VO_Q_SORT: return 12 * n_pad + 2 * kChecksumBlocks * 8;
+    default: return kMaxGroups * 12 + 64;
+    }
+}
+
+// Scratch-arena bytes the query's operators take (upper bounds of compact.hip / join.hip).
+uint64_t query_scratch_bytes(uint32_t kind, uint64_t n_pad)
+{
+    switch (kind) {
+    case VO_Q_FILTER_SUM: return 2 * n_pad + (1u << 20);
+    case VO_Q_JOIN_GROUPSUM: return 34 * n_pad + (4u << 20);
+    default: return 1u << 20;
+    }
+}
+
+} // namespace
+} // namespace vo
+
+using namespace vo;
+
+struct vo_server {
+    int device = 0;
+    uint32_t n_slots = 1;
+    uint64_t budget = 0;
+    std::string last_error;
+    std::vector<vo_ctx *> slots;
+    std::vector<void *> slot_buf;
+    std::vector<uint64_t> slot_buf_bytes;
+    std::vector<FactTable> facts;
+    std::vector<JoinTable> joins;
+    std::vector<SortTable> sorts;
+    std::vector<void *> allocations;
+};
+
+namespace {
+
+template <typename F>
+int sguard(vo_server *s, F &&f)
+{
+    if (!s) return VO_ERR_INVALID;
+    try {
+        VO_HIP(hipSetDevice(s->device));
+        f();
+        return VO_OK;
+    } catch (const Error &e) {
+        s->last_error = e.msg;
+        return e.status;
+    } catch (const std::exception &e) {
+        s->last_error = e.what();
+        return VO_ERR_INVALID;
+    }
+}
+
+void *dalloc(vo_server *s, size_t bytes)
+{
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{VO_ERR_NOMEM, "server table allocation failed"};
+    }
+    s->allocations.push_back(p);
+    return p;
+}
+
+vo_ctx *slot_ctx(vo_server *s, uint32_t i)
+{
+    while (s->slots.size() <= i) {
+        vo_config c{};
+        c.device = s->device;
+        vo_ctx *ctx = nullptr;
+        int st = vo_ctx_create(&c, &ctx);
+        if (st != VO_OK) throw Error{st, "server slot context creation failed"};
+        s->slots.push_back(ctx);
+        s->slot_buf.push_back(nullptr);
+        s->slot_buf_bytes.push_back(0);
+    }
+    return s->slots[i];
+}
+
+uint64_t table_n_pad(vo_server *s, const vo_query &q)
+{
+    switch (q.kind) {
+    case VO_Q_FILTER_SUM:
+        VO_CHECK(q.table < s->facts.size(), VO_ERR_INVALID, "unknown fact table");
+        return s->facts[q.table].n_pad;
+    case VO_Q_JOIN_GROUPSUM:
+        VO_CHECK(q.table < s->joins.size(), VO_ERR_INVALID, "unknown join table");
+        return s->joins[q.table].n_pad;
+    case VO_Q_SORT:
+        VO_CHECK(q.table < s->sorts.size(), VO_ERR_INVALID, "unknown sort table");
+        return s->sorts[q.table].n_pad;
+    default: throw Error{VO_ERR_INVALID, "unknown query kind"};
+    }
+}
+
+// Runs one query on a slot and fills its result words (blocking).
+void run_query(vo_server *s, vo_ctx *ctx, void *buf, const vo_query &q, uint64_t r[4])
+{
+    char *b = (char *)buf;
+    switch (q.kind) {
+    case VO_Q_FILTER_SUM: {
+        const FactTable &t = s->facts[q.table];
+        vo_filter_args a{};
+        a.day = t.day;
+        a.qty = t.qty;
+        a.price = t.price;
+        a.disc = t.disc;
+        a.out_price = (int64_t *)b;
+        a.out_disc = (int64_t *)(b + 8 * t.n_pad);
+        a.n = t.n;
+        a.n_pad = t.n_pad;
+        a.day_lo = 8766;
+        a.day_hi = 9131;
+        a.disc_lo = 5;
+        a.disc_hi = 7;
+        a.qty_lt = 24;
+        a.out = (uint64_t *)(b + 16 * t.n_pad);
+        op_filter_sum(ctx, a);
+        uint64_t h[4];
+        VO_HIP(hipMemcpyAsync(h, a.out, 32, hipMemcpyDeviceToHost, ctx->stream));
+        VO_HIP(hipStreamSynchronize(ctx->stream));
+        r[0] = h[0];
+        r[1] = h[2];
+        r[2] = h[3];
+        r[3] = 0;
+        break;
+    }
+    case VO_Q_JOIN_GROUPSUM: {
+        const JoinTable &t = s->joins[q.table];
+        vo_join_args a{};
+        a.dim_key = t.dk;
+        a.dim_group = t.dg;
+        a.n_dim = t.n_dim;
+        a.fact_fk = t.fk;
+        a.fact_val = t.fv;
+        a.n_fact = t.n_fact;
+        a.n_pad = t.n_pad;
+        a.out_group = (int32_t *)b;
+        a.out_sum = (int64_t *)(b + kMaxGroups * 4);
+        a.out_n_groups = (uint64_t *)(b + kMaxGroups * 12);
+        a.max_groups = kMaxGroups;
+        op_join_groupsum(ctx, a);
+        std::vector<char> h(kMaxGroups * 12 + 8);
+        VO_HIP(hipMemcpyAsync(h.data(), b, h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        VO_HIP(hipStreamSynchronize(ctx->stream));
+        const int32_t *g = (const int32_t *)h.data();
+        const int64_t *sm = (const int64_t *)(h.data() + kMaxGroups * 4);
+        const uint64_t G = *(const uint64_t *)(h.data() + kMaxGroups * 12);
+        uint64_t w = 0, tot = 0;
+        for (uint64_t i = 0; i < std::min<uint64_t>(G, kMaxGroups); ++i) {
+            w += (uint64_t)(g[i] + 1) * (uint64_t)sm[i];
+            tot += (uint64_t)sm[i];
+        }
+        r[0] = G;
+        r[1] = w;
+        r[2] = tot;
+        r[3] = 0;
+        break;
+    }
+    case VO_Q_SORT: {
+        const SortTable &t = s->sorts[q.table];
+        uint64_t *k = (uint64_t *)b;
+        uint32_t *p = (uint32_t *)(b + 8 * t.n_pad);
+        uint64_t *part = (uint64_t *)(b + 12 * t.n_pad);
+        // the resident table stays unsorted: the query sorts its own copy
+        VO_HIP(hipMemcpyAsync(k, t.keys, 8 * t.n_pad, hipMemcpyDeviceToDevice, ctx->stream));
+        VO_HIP(hipMemcpyAsync(p, t.pay, 4 * t.n_pad, hipMemcpyDeviceToDevice, ctx->stream));
+        op_sort_kp(ctx, k, p, t.n, t.n_pad);
+        launch(ctx, LaunchInfo{"sort_checksum", dim3(kChecksumBlocks), dim3(256), 0, t.n_pad}, k_sort_checksum,
+               (const uint64_t *)k, (const uint32_t *)p, t.n_pad, part);
+        std::vector<uint64_t> h(2 * kChecksumBlocks);
+        VO_HIP(hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+        VO_HIP(hipStreamSynchronize(ctx->stream));
+        uint64_t x = 0, y = 0;
+        for (uint32_t i = 0; i < kChecksumBlocks; ++i) {
+            x += h[2 * i];
+            y += h[2 * i + 1];
+        }
+        r[0] = t.n;
+        r[1] = x;
+        r[2] = y;
+        r[3] = 0;
+        break;
+    }
+    default: throw Error{VO_ERR_INVALID, "unknown query kind"};
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int vo_server_create(const vo_server_config *cfg, vo_server **out)
+{
+    if (!cfg || !out) return VO_ERR_INVALID;
+    *out = nullptr;
+    vo_server *s = new vo_server();
+    s->device = (int)cfg->device;
+    s->n_slots = std::max<uint32_t>(1, cfg->n_slots);
+    s->budget = cfg->hbm_budget_bytes ? cfg->hbm_budget_bytes : (200ull << 30);
+    int st = sguard(s, [&] {
+        for (uint32_t i = 0; i < s->n_slots; ++i) slot_ctx(s, i);
+    });
+    if (st != VO_OK) {
+        vo_server_destroy(s);
+        return st;
+    }
+    *out = s;
+    return VO_OK;
+}
+
+int vo_server_destroy(vo_server *s)
+{
+    if (!s) return VO_ERR_INVALID;
+    (void)hipSetDevice(s->device);
+    for (vo_ctx *c : s->slots) vo_ctx_destroy(c);
+    for (void *p : s->slot_buf)
+        if (p) (void)hipFree(p);
+    for (void *p : s->allocations) (void)hipFree(p);
+    delete s;
+    return VO_OK;
+}
+
+const char *vo_server_last_error(const vo_server *s) { return s ? s->last_error.c_str() : "null server"; }
+
+int vo_server_add_fact_table(vo_server *s, uint64_t seed, uint64_t n_rows, uint32_t *id)
+{
+    return sguard(s, [&] {
+        VO_CHECK(n_rows > 0 && n_rows <= (1ull << 29), VO_ERR_INVALID, "fact rows out of range");
+        FactTable t;
+        t.n = n_rows;
+        t.n_pad = pow2_at_least(n_rows, 4096);
+        t.day = (int32_t *)dalloc(s, 4 * t.n_pad);
+        t.qty = (int64_t *)dalloc(s, 8 * t.n_pad);
+        t.price = (int64_t *)dalloc(s, 8 * t.n_pad);
+        t.disc = (int64_t *)dalloc(s, 8 * t.n_pad);
+        vo_ctx *ctx = slot_ctx(s, 0);
+        op_gen_fact(ctx, seed, t.day, t.qty, t.price, t.disc, t.n);
+        VO_HIP(hipStreamSynchronize(ctx->stream));
+        s->facts.push_back(t);
+        if (id) *id = (uint32_t)(s->facts.size() - 1);
+    });
+}
+
+int vo_server_add_join_tables(vo_server *s, uint64_t seed, uint64_t n_dim, uint64_t n_fact, uint32_t *id)
+{
+    return sguard(s, [&] {
+        VO_CHECK(n_dim > 0 && n_fact > 0 && n_dim + n_fact <= (1ull << 29), VO_ERR_INVALID, "join rows out of range");
+        JoinTable t;
+        t.n_dim = n_dim;
+        t.n_fact = n_fact;
+        t.n_pad = pow2_at_least(n_dim + n_fact, 4096);
+        const uint64_t dpad = pow2_at_least(n_dim, 1);
+        t.dk = (int32_t *)dalloc(s, 4 * n_dim);
+        t.dg = (int32_t *)dalloc(s, 4 * n_dim);
+        t.fk = (int32_t *)dalloc(s, 4 * n_fact);
+        t.fv = (int64_t *)dalloc(s, 8 * n_fact);
+        uint32_t *perm = (uint32_t *)dalloc(s, 4 * dpad);
+        vo_ctx *ctx = slot_ctx(s, 0);
+        // unique dimension keys: a Philox-tag shuffle of 1..dpad, first n_dim of them
+        const dim3 g((uint32_t)std::min<uint64_t>(65536, (dpad + 255) / 256));
+        launch(ctx, LaunchInfo{"server_iota", g, dim3(256), 0, dpad}, k_iota1, perm, dpad);
+        vo_shuffle_args sh{perm, nullptr, dpad, seed};
+        op_shuffle(ctx, sh);
+        const dim3 g2((uint32_t)std::min<uint64_t>(65536, (std::max(n_dim, n_fact) + 255) / 256));
+        launch(ctx, LaunchInfo{"server_gen_join", g2, dim3(256), 0, n_dim}, k_gen_join_cols, seed,
+               (const uint32_t *)perm, n_dim, n_fact, t.dk, t.dg, t.fk, t.fv);
+        VO_HIP(hipStreamSynchronize(ctx->stream));
+        s->joins.push_back(t);
+        if (id) *id = (uint32_t)(s->joins.size() - 1);
+    });
+}
+
+int vo_server_add_sort_table(vo_server *s, uint64_t seed, uint64_t n_rows, uint32_t *id)
+{
+    return sguard(s, [&] {
+        VO_CHECK(n_rows > 0 && n_rows <= (1ull << 29), VO_ERR_INVALID, "sort rows out of range");
+        SortTable t;
+        t.n = n_rows;
+        t.n_pad = pow2_at_least(n_rows, 1);
+        t.keys = (uint64_t *)dalloc(s, 8 * t.n_pad);
+        t.pay = (uint32_t *)dalloc(s, 4 * t.n_pad);
+        vo_ctx *ctx = slot_ctx(s, 0);
+        op_gen_sort_input(ctx, seed, t.keys, t.pay, t.n);
+        VO_HIP(hipStreamSynchronize(ctx->stream));
+        s->sorts.push_back(t);
+        if (id) *id = (uint32_t)(s->sorts.size() - 1);
+    });
+}
+
+int vo_server_run(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_slots_override, vo_query_result *out,
+                  vo_run_stats *stats)
+{
+    return sguard(s, [&] {
+        VO_CHECK(q && out, VO_ERR_INVALID, "null query list");
+        const uint32_t slots = n_slots_override ? n_slots_override : s->n_slots;
+        // size every slot for the largest query it may take (no allocation while timing)
+        std::vector<uint64_t> fp(n);
+        uint64_t max_buf = 64, max_scr = 1 << 20;
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t np = table_n_pad(s, q[i]);
+            const uint64_t b = vo::query_buffer_bytes(q[i].kind, np), c = vo::query_scratch_bytes(q[i].kind, np);
+            fp[i] = b + c;
+            max_buf = std::max(max_buf, b);
+            max_scr = std::max(max_scr, c);
+        }
+        for (uint32_t i = 0; i < slots; ++i) {
+            vo_ctx *ctx = slot_ctx(s, i);
+            if (s->slot_buf_bytes[i] < max_buf) {
+                if (s->slot_buf[i]) VO_HIP(hipFree(s->slot_buf[i]));
+                s->slot_buf[i] = nullptr;
+                VO_HIP(hipMalloc(&s->slot_buf[i], max_buf));
+                s->slot_buf_bytes[i] = max_buf;
+            }
+            scratch(ctx, max_scr);
+        }
+        VO_HIP(hipDeviceSynchronize());
+
+        std::mutex mu;
+        std::condition_variable cv;
+        uint64_t next = 0, in_flight = 0, peak = 0;
+        uint32_t running = 0, max_running = 0;
+        int first_error = VO_OK;
+        std::string err_msg;
+        using clk = std::chrono::steady_clock;
+        const auto t0 = clk::now();
+        auto now_s = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
+
+        auto worker = [&](uint32_t slot) {
+            (void)hipSetDevice(s->device);
+            vo_ctx *ctx = s->slots[slot];
+            for (;;) {
+                uint64_t i;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    for (;;) {
+                        if (next >= n || first_error != VO_OK) return;
+                        const vo_query &qq = q[next];
+                        const double wait = qq.arrival_s - now_s();
+                        if (wait > 0) {
+                            cv.wait_for(lk, std::chrono::duration<double>(wait));
+                            continue;
+                        }
+                        if (fp[next] > s->budget) {
+                            // can never fit: reject instead of blocking the queue
+                            out[next] = vo_query_result{};
+                            out[next].status = VO_ERR_ADMISSION;
+                            out[next].footprint_bytes = fp[next];
+                            ++next;
+                            continue;
+                        }
+                        if (in_flight + fp[next] > s->budget) {
+                            cv.wait(lk);
+                            continue;
+                        }
+                        break;
+                    }
+                    i = next++;
+                    in_flight += fp[i];
+                    peak = std::max(peak, in_flight);
+                    max_running = std::max(max_running, ++running);
+                }
+                vo_query_result r{};
+                r.admit_s = now_s();
+                r.footprint_bytes = fp[i];
+                r.slot = slot;
+                try {
+                    run_query(s, ctx, s->slot_buf[slot], q[i], r.r);
+                    r.status = VO_OK;
+                } catch (const Error &e) {
+                    r.status = e.status;
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_error == VO_OK) {
+                        first_error = e.status;
+                        err_msg = e.msg;
+                    }
+                }
+                r.end_s = now_s();
+                r.latency_s = r.end_s - q[i].arrival_s;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    out[i] = r;
+                    in_flight -= fp[i];
+                    --running;
+                }
+                cv.notify_all();
+            }
+        };
+        std::vector<std::thread> th;
+        for (uint32_t i = 0; i < slots; ++i) th.emplace_back(worker, i);
+        for (auto &t : th) t.join();
+        if (first_error != VO_OK) throw Error{first_error, err_msg};
+        if (stats) {
+            std::vector<double> lat;
+            double first_arrival = 1e300, last_end = 0;
+            for (uint64_t i = 0; i < n; ++i) {
+                if (out[i].status != VO_OK) continue;
+                lat.push_back(out[i].latency_s);
+                first_arrival = std::min(first_arrival, q[i].arrival_s);
+                last_end = std::max(last_end, out[i].end_s);
+            }
+            std::sort(lat.begin(), lat.end());
+            vo_run_stats st{};
+            if (!lat.empty()) {
+                auto pct = [&](double p) { return lat[std::min(lat.size() - 1, (size_t)(p * (lat.size() - 1) + 0.5))]; };
+                double sum = 0;
+                for (double x : lat) sum += x;
+                st.makespan_s = last_end - first_arrival;
+                st.qps = st.makespan_s > 0 ? lat.size() / st.makespan_s : 0;
+                st.p50_s = pct(0.50);
+                st.p99_s = pct(0.99);
+                st.mean_s = sum / lat.size();
+            }
+            st.peak_bytes_in_flight = peak;
+            st.max_concurrent = max_running;
+            *stats = st;
+        }
+    });
+}
+
+} // extern "C"
