@@ -82,6 +82,302 @@ def calibrate_copy(steps):
     ctx.close()
 
 
+# --------------------------------------------------------------------------
+# Secondary workloads (one JSON line each; the headline line stays the sort).
+
+def _family_stats(ctx, fams):
+    out = {}
+    for f in fams:
+        cnt, ms = ctx.kernel_stats(f)
+        if cnt:
+            out[f] = (cnt, ms)
+    return out
+
+
+def _roofline(stats, bytes_per_row, n_pad):
+    """Roofline of the family with the largest total time; bytes_per_row maps
+    a family to its algorithmic bytes per padded row and launch."""
+    cand = {f: v for f, v in stats.items() if f in bytes_per_row}
+    dom = max(cand, key=lambda f: cand[f][1])
+    cnt, ms = cand[dom]
+    alg = bytes_per_row[dom] * n_pad
+    ach = alg / (ms / cnt * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "avg_launch_ms": round(ms / cnt, 4), "alg_bytes_per_launch": alg}
+
+
+def _line(metric, value, unit, steps, warmup, ms, dtype, config, **extra):
+    d = {"metric": metric, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+         "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+         "dtype": dtype, "data": "synthetic", "config": config}
+    d.update(extra)
+    print(json.dumps(d), flush=True)
+
+
+def bench_filter(args):
+    """Q6-style filter + ORCompact of (price, disc) + exact int128 SUM over a
+    fact table resident in HBM (6e6 rows / 2^23, or 3e8 rows / 2^29)."""
+    import numpy as np
+    import torch
+
+    import velodb_amd
+    from velodb_amd import _abi
+    n_pad = 1 << (args.log2n if args.log2n != 28 else 23)
+    n = 6_000_000 if n_pad == 1 << 23 else (300_000_000 if n_pad == 1 << 29 else n_pad - n_pad // 4)
+    dev = torch.device("cuda:0")
+    day = torch.empty(n_pad, dtype=torch.int32, device=dev)
+    qty, price, disc, op, od = (torch.empty(n_pad, dtype=torch.int64, device=dev) for _ in range(5))
+    out = torch.empty(4, dtype=torch.int64, device=dev)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+    ctx.gen_fact(1, day, qty, price, disc, n)
+
+    def step():
+        ctx.filter_sum(day, qty, price, disc, n, n_pad, out, out_price=op, out_disc=od)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.kernel_stats_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    o = out.cpu().numpy().view(np.uint64)
+    stats = _family_stats(ctx, ["filter_count", "compact_scan", "compact_local", "compact_global", "filter_sum"])
+    # algorithmic bytes per padded row and launch: predicate reads day+qty+disc
+    # and writes the mask; the local network reads the mask and both columns
+    # and writes the columns; a global pass reads and writes both columns
+    bpr = {"filter_count": 4 + 8 + 8 + 1, "compact_local": 1 + 2 * 16, "compact_global": 2 * 16,
+           "filter_sum": 16}
+    _line("oblivious filter + compaction + int128 SUM (Mrows/s)", n / dt / 1e6, "Mrows/s", args.steps,
+          args.warmup, dt * 1e3, "int64/int128",
+          {"workload": f"Q6-style filter over {n} rows padded to 2^{ilog2(n_pad)}, ORCompact of (price, disc), "
+                       f"exact int128 SUM(price*disc)", "n": n, "n_pad": n_pad,
+           "selected": int(o[0]), "sum": str(int(o[2]) | (int(o[3]) << 64))},
+          roofline=_roofline(stats, bpr, n_pad),
+          kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()})
+    ctx.close()
+
+
+def ilog2(v):
+    return v.bit_length() - 1
+
+
+def bench_join(args):
+    """PK/FK join + GROUP BY SUM: 1.5M dimension rows (unique keys 1..1.5M,
+    group 0..24), 6M fact rows (FK uniform over dimension keys, value
+    100..1e7); union padded to 2^23."""
+    import numpy as np
+    import torch
+
+    import velodb_amd
+    from velodb_amd import _abi
+    n_dim, n_fact, n_pad = 1_500_000, 6_000_000, 1 << 23
+    rng = np.random.default_rng(1)
+    dk = (rng.permutation(n_dim) + 1).astype(np.int32)
+    dg = rng.integers(0, 25, n_dim, dtype=np.int32)
+    fk = dk[rng.integers(0, n_dim, n_fact)]
+    fv = rng.integers(100, 10_000_001, n_fact, dtype=np.int64)
+    dev = torch.device("cuda:0")
+    d_dk, d_dg, d_fk, d_fv = (torch.from_numpy(a).to(dev) for a in (dk, dg, fk, fv))
+    og = torch.empty(64, dtype=torch.int32, device=dev)
+    osum = torch.empty(64, dtype=torch.int64, device=dev)
+    on = torch.empty(1, dtype=torch.int64, device=dev)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+
+    def step():
+        ctx.join_groupsum(d_dk, d_dg, n_dim, d_fk, d_fv, n_fact, n_pad, og, osum, on, 64)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.kernel_stats_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # the host check is a plain numpy group-by over the same arrays
+    grp = np.zeros(n_dim + 2, np.int64)
+    grp[dk] = dg
+    exp = np.bincount(grp[fk], minlength=25)
+    sums = np.zeros(25, np.int64)
+    np.add.at(sums, grp[fk], fv)
+    ng = int(on.item())
+    got_g = og[:ng].cpu().numpy()
+    got_s = osum[:ng].cpu().numpy()
+    ok = ng == int((exp > 0).sum()) and np.array_equal(got_s, sums[got_g])
+    fams = ["join_build", "join_fill_tiles", "join_fill_rows", "group_keys", "group_seg_tiles", "group_seg_rows",
+            "group_out", "compact_count", "compact_local", "compact_global", "sort_local", "sort_merge",
+            "sort_global_s1", "sort_global_s2", "sort_global_s3", "sort_global_s4", "sort_hybrid_s6"]
+    stats = _family_stats(ctx, fams)
+    bpr = {f: 32 for f in fams if f.startswith("sort_") or f == "compact_global"}
+    _line("oblivious PK/FK join + GROUP BY SUM (Mrows/s)", (n_dim + n_fact) / dt / 1e6, "Mrows/s", args.steps,
+          args.warmup, dt * 1e3, "int64",
+          {"workload": "PK/FK join of 1.5M dimension x 6M fact rows (union padded to 2^23) + exact int64 "
+                       "SUM per group (25 groups)", "n_dim": n_dim, "n_fact": n_fact, "n_pad": n_pad,
+           "groups": ng, "matches_numpy_groupby": bool(ok)},
+          roofline=_roofline(stats, bpr, n_pad),
+          kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()})
+    ctx.close()
+
+
+def bench_batch(args):
+    """4096 independent sorts, sizes log-uniform in [2^10, 2^20]
+    (numpy default_rng(0)), each padded to its power of two; one batched call
+    vs one call per instance."""
+    import numpy as np
+    import torch
+
+    import velodb_amd
+    rng = np.random.default_rng(0)
+    sizes = np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 20), 4096)).astype(np.int64)
+    items, off = [], 0
+    for n in sizes:
+        p = 1 << int(np.ceil(np.log2(n)))
+        off = (off + p - 1) // p * p
+        items.append((off, int(n), p))
+        off += p
+    total = off
+    padded = sum(p for _, _, p in items)
+    dev = torch.device("cuda:0")
+    keys = torch.empty(total, dtype=torch.int64, device=dev)
+    pay = torch.empty(total, dtype=torch.int32, device=dev)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream)
+    if args.autotune:
+        ctx.autotune_sort(10, 20, rows_per_class=1 << 24)
+
+    def run(fn, reps):
+        ts = []
+        for _ in range(reps):
+            ctx.gen_sort_input(7, keys, pay, total)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    def batched():
+        ctx.sort_batch(keys, pay, items)
+
+    def per_instance():
+        for (o, n, p) in items:
+            ctx.sort_kp_async(keys[o:], pay[o:], n, p)
+
+    run(batched, max(1, args.warmup))
+    tb = run(batched, args.steps)
+    kb, pb = keys.clone(), pay.clone()
+    tp = run(per_instance, max(1, min(args.steps, 3)))
+    same = bool(torch.equal(kb, keys) and torch.equal(pb, pay))
+    _line("batched oblivious sorts (Mrec/s over padded records)", padded / tb / 1e6, "Mrec/s", args.steps,
+          args.warmup, tb * 1e3, "u64",
+          {"workload": "4096 sorts, sizes log-uniform in [2^10, 2^20] (numpy default_rng(0)), u64 keys + u32 "
+                       "payload, padded per instance, one call bucketed by size class",
+           "instances": len(items), "raw_records": int(sizes.sum()), "padded_records": padded,
+           "autotuned": bool(args.autotune)},
+          per_instance_ms=round(tp * 1e3, 2), speedup_vs_per_instance=round(tp / tb, 2),
+          batched_equals_per_instance=same)
+    ctx.close()
+
+
+def bench_queue(args):
+    """Command round trip (submit -> wait returns) through the resident queue
+    kernel vs launch mode: NOP commands and 4096-record sorts."""
+    import numpy as np
+    import torch
+
+    import velodb_amd
+    from velodb_amd import _abi
+    dev = torch.device("cuda:0")
+    n = 4096
+    keys = torch.empty(n, dtype=torch.int64, device=dev)
+    pay = torch.empty(n, dtype=torch.int32, device=dev)
+    reps = max(50, args.steps * 40)
+    res = {}
+    for mode, flags in (("launch", 0), ("persistent", _abi.CFG_PERSISTENT)):
+        ctx = velodb_amd.Context(flags=flags)
+        ctx.gen_sort_input(3, keys, pay, n)
+        ctx.sync()
+        ctx.queue_start()
+        for kind in ("nop", "sort4096"):
+            cmd = _abi.Command()
+            if kind == "nop":
+                cmd.opcode = _abi.OP_NOP
+            else:
+                cmd.opcode = _abi.OP_SORT_KP
+                cmd.args.sort_kp = _abi.SortKpArgs(keys.data_ptr(), pay.data_ptr(), n, n)
+            for _ in range(10):
+                ctx.wait(ctx.submit(cmd))
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                ctx.wait(ctx.submit(cmd))
+                ts.append(time.perf_counter() - t0)
+            res[f"{mode}_{kind}_us"] = {"p50": round(float(np.median(ts)) * 1e6, 1),
+                                        "p99": round(float(np.percentile(ts, 99)) * 1e6, 1)}
+        ctx.queue_stop()
+        ctx.close()
+    p50 = res["persistent_sort4096_us"]["p50"]
+    d = {"metric": "command round-trip latency, 4096-record sort (us, p50)", "value": p50, "unit": "us",
+         "n_gpus": 1, "steps": reps, "warmup": 10, "ms_per_step": round(p50 / 1e3, 4), "higher_is_better": False,
+         "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+         "config": {"workload": "submit + wait of one command through the resident queue kernel vs launch mode"},
+         "latency": res}
+    print(json.dumps(d), flush=True)
+
+
+def bench_serve(args):
+    """Served mix (50% filter+SUM, 30% join+group-by, 20% sort) over 12 tables
+    of each kind with row counts log-uniform in [1e4, 1e7]; all queries
+    arriving at once, then Poisson arrivals at 50/200/800 QPS; 16 slots vs the
+    plain one-query-at-a-time loop, 200 GB admission budget."""
+    import numpy as np
+
+    import velodb_amd
+    S = velodb_amd.Server
+    srv = S(n_slots=16, hbm_budget_bytes=200 << 30)
+    rng = np.random.default_rng(0)
+    tables = {"f": [], "j": [], "s": []}
+    for i in range(12):
+        n = int(np.exp(rng.uniform(np.log(1e4), np.log(1e7))))
+        tables["f"].append(srv.add_fact_table(100 + i, n))
+        tables["j"].append(srv.add_join_tables(200 + i, max(1, n // 5), n - n // 5))
+        tables["s"].append(srv.add_sort_table(300 + i, n))
+
+    def mix(count, rate, seed):
+        r = np.random.default_rng(seed)
+        t, qs = 0.0, []
+        for _ in range(count):
+            t += r.exponential(1.0 / rate) if rate else 0.0
+            u = r.random()
+            kind, key = ((S.FILTER_SUM, "f") if u < 0.5 else (S.JOIN_GROUPSUM, "j") if u < 0.8 else (S.SORT, "s"))
+            qs.append((kind, int(r.choice(tables[key])), t))
+        return qs
+
+    def summ(st):
+        return {"qps": round(st.qps, 1), "p50_ms": round(st.p50_s * 1e3, 2), "p99_ms": round(st.p99_s * 1e3, 2),
+                "max_concurrent": st.max_concurrent, "peak_GB": round(st.peak_bytes_in_flight / 1e9, 2)}
+
+    srv.run(mix(64, 0, 99))  # warm-up
+    for rate in [0, 50, 200, 800]:
+        count = 600 if rate == 0 else max(200, rate * 2)
+        qs = mix(count, rate, 7 + rate)
+        rp, sp = srv.run(qs, n_slots=1)
+        rb, sb = srv.run(qs)
+        same = [list(a.r) for a in rp] == [list(b.r) for b in rb]
+        _line("served queries/s (continuous batching, 16 slots)", sb.qps, "QPS", count, 64, 1e3 / max(sb.qps, 1e-9),
+              "int64/int128",
+              {"workload": "served mix 50% filter+SUM / 30% join+group-by / 20% sort, tables log-uniform "
+                           "in [1e4, 1e7] rows, 200 GB admission budget",
+               "offered_qps": rate if rate else "all at t=0", "queries": count},
+              batched=summ(sb), plain=summ(sp), throughput_ratio=round(sb.qps / max(sp.qps, 1e-9), 2),
+              results_equal=same)
+    srv.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +386,9 @@ def main():
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", type=int, default=0)
+    ap.add_argument("--workload", choices=["sort", "filter", "join", "batch", "serve", "queue"], default="sort",
+                    help="sort = the headline line; the others print their own line")
+    ap.add_argument("--autotune", action="store_true", help="batch: run the autotuner first")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run the plain 3 GiB HBM copy kernel instead (traffic-counter calibration)")
     args = ap.parse_args()
@@ -98,9 +397,13 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the oblivious sort is not sharded across GPUs in this build "
-              "(see DESIGN.md, 'How the path would shard'); run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the oblivious sort is not sharded across GPUs "
+              "(no cross-GPU exchange stages are built); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "sort":
+        {"filter": bench_filter, "join": bench_join, "batch": bench_batch, "serve": bench_serve,
+         "queue": bench_queue}[args.workload](args)
+        return
 
     import torch
 

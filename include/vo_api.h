@@ -284,6 +284,8 @@ typedef struct vo_command {
 enum {
     VO_CFG_PERSISTENT = 1u << 0,    /* run commands in the resident queue kernel */
     VO_CFG_KERNEL_TIMING = 1u << 1, /* HIP events around every launch (vo_kernel_stats) */
+    VO_CFG_QUEUE_SHORT_IDLE = 1u << 2, /* resident kernel leaves after 50 ms without commands
+                                          (default 20 s); the next vo_submit starts it again */
 };
 
 typedef struct vo_config {

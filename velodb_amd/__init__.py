@@ -197,6 +197,21 @@ class Context:
         a = _abi.SortBatchArgs(_ptr(keys), _ptr(payload), C.cast(arr, C.c_void_p), len(items))
         return self.check(self.lib.vo_sort_batch(self.h, C.byref(a)))
 
+    def queue_start(self):
+        return self.check(self.lib.vo_queue_start(self.h))
+
+    def queue_stop(self, timeout_ms=5000):
+        return self.check(self.lib.vo_queue_stop(self.h, timeout_ms))
+
+    def queue_running(self):
+        return bool(self.lib.vo_queue_running(self.h))
+
+    def poll(self, seq):
+        r = self.lib.vo_poll(self.h, seq)
+        if r < 0:
+            raise VoError(-r, self.lib.vo_last_error(self.h).decode())
+        return bool(r)
+
     def submit(self, cmd):
         s = self.lib.vo_submit(self.h, C.byref(cmd))
         if s == 0:

@@ -42,6 +42,7 @@ OP_TERMINATE = 0xFFFFFFFF
 
 CFG_PERSISTENT = 1 << 0
 CFG_KERNEL_TIMING = 1 << 1
+CFG_QUEUE_SHORT_IDLE = 1 << 2
 
 MAX_SORT_COLUMNS = 8
 

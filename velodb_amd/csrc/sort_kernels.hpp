@@ -143,10 +143,15 @@ __device__ __forceinline__ void gstore_t(const R &r, const Bufs<R> &b, uint64_t 
 // ----------------------------------------------------------- tiny arrays
 
 // n_pad <= 8: one thread runs the network directly.
+// Each kernel is a device body taking its block index explicitly plus a thin
+// __global__ wrapper; the resident queue kernel (queue.hip) runs the same
+// bodies over its own block loop.
+
 template <class R, bool TRACE>
-__global__ void k_sort_tiny(Bufs<R> b, int n_log, uint64_t n_valid, int fill, TraceSlot *slot)
+__device__ __forceinline__ void body_tiny(Bufs<R> b, int n_log, uint64_t n_valid, int fill, TraceSlot *slot,
+                                          uint32_t blk)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (threadIdx.x != 0 || blk != 0) return;
     uint64_t h = 0;
     uint32_t nops = 0;
     const uint32_t N = 1u << n_log;
@@ -168,20 +173,25 @@ __global__ void k_sort_tiny(Bufs<R> b, int n_log, uint64_t n_valid, int fill, Tr
     if constexpr (TRACE) tcommit(slot, 0, h, nops);
 }
 
+template <class R, bool TRACE>
+__global__ void k_sort_tiny(Bufs<R> b, int n_log, uint64_t n_valid, int fill, TraceSlot *slot)
+{
+    body_tiny<R, TRACE>(b, n_log, n_valid, fill, slot, blockIdx.x);
+}
+
 // ---------------------------------------------------------- tile kernels
 
 // Tiles of 2^t records for t < T_LOG (small arrays): levels 1 .. p_max with
 // runtime layouts.  Loads and stores use the coalesced layout a = t - R_LOG.
 template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
-__global__ __launch_bounds__(BLOCK) void k_sort_local(Bufs<R> b, IMap map, int t, int p_max, int p_top, uint64_t n_valid,
-                                                      int fill, TraceSlot *slot)
+__device__ __forceinline__ void body_local(Bufs<R> b, IMap map, int t, int p_max, int p_top, uint64_t n_valid,
+                                           int fill, TraceSlot *slot, char *lds_raw, uint32_t blk)
 {
-    extern __shared__ __align__(16) char lds_raw[];
     const uint32_t T = 1u << t;
     const Lds<R> s = lds_view<R>(lds_raw, T + (T >> 4));
     const uint32_t tau = threadIdx.x;
     const bool act = tau < (T >> R_LOG);
-    const uint64_t vbase = (uint64_t)blockIdx.x << t;
+    const uint64_t vbase = (uint64_t)blk << t;
     uint64_t h = 0;
     uint32_t nops = 0;
     R r[1 << R_LOG];
@@ -207,19 +217,26 @@ __global__ __launch_bounds__(BLOCK) void k_sort_local(Bufs<R> b, IMap map, int t
         for (int m = 0; m < (1 << R_LOG); ++m)
             gstore<R, TRACE>(r[m], b, phys(map, vbase + (base | ((uint32_t)m << a))), h, nops);
     }
-    if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
+    if constexpr (TRACE) tcommit(slot, (uint64_t)blk * BLOCK + tau, h, nops);
+}
+
+template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
+__global__ __launch_bounds__(BLOCK) void k_sort_local(Bufs<R> b, IMap map, int t, int p_max, int p_top, uint64_t n_valid,
+                                                      int fill, TraceSlot *slot)
+{
+    extern __shared__ __align__(16) char lds_raw[];
+    body_local<R, T_LOG, R_LOG, BLOCK, TRACE>(b, map, t, p_max, p_top, n_valid, fill, slot, lds_raw, blockIdx.x);
 }
 
 // Full 2^T_LOG tiles: all levels p = 1 .. T_LOG with a compile-time schedule.
 template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
-__global__ __launch_bounds__(BLOCK) void k_sort_local_ct(Bufs<R> b, IMap map, int p_top, uint64_t n_valid, int fill,
-                                                         TraceSlot *slot)
+__device__ __forceinline__ void body_local_ct(Bufs<R> b, IMap map, int p_top, uint64_t n_valid, int fill,
+                                              TraceSlot *slot, char *lds, uint32_t blk)
 {
     using TL = TileCT<R, T_LOG, R_LOG, TRACE>;
     constexpr int A0 = T_LOG - R_LOG;
-    extern __shared__ __align__(16) char lds[];
     const uint32_t tau = threadIdx.x;
-    const uint64_t vbase = (uint64_t)blockIdx.x << T_LOG;
+    const uint64_t vbase = (uint64_t)blk << T_LOG;
     const uint64_t pbase = map_uniform(map, vbase);
     uint64_t h = 0;
     uint32_t nops = 0;
@@ -238,18 +255,26 @@ __global__ __launch_bounds__(BLOCK) void k_sort_local_ct(Bufs<R> b, IMap map, in
     TL::template relayout<TL::end_all(1, A0), A0>(r, lds, tau, h, nops);
 #pragma unroll
     for (int m = 0; m < (1 << R_LOG); ++m) gstore_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
-    if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
+    if constexpr (TRACE) tcommit(slot, (uint64_t)blk * BLOCK + tau, h, nops);
+}
+
+template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
+__global__ __launch_bounds__(BLOCK) void k_sort_local_ct(Bufs<R> b, IMap map, int p_top, uint64_t n_valid, int fill,
+                                                         TraceSlot *slot)
+{
+    extern __shared__ __align__(16) char lds[];
+    body_local_ct<R, T_LOG, R_LOG, BLOCK, TRACE>(b, map, p_top, n_valid, fill, slot, lds, blockIdx.x);
 }
 
 // Stages (p, T_LOG-1 .. 0) of one level p > T_LOG on tiles of 2^T_LOG.
 template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
-__global__ __launch_bounds__(BLOCK) void k_sort_merge(Bufs<R> b, IMap map, int p, int p_top, TraceSlot *slot)
+__device__ __forceinline__ void body_merge(Bufs<R> b, IMap map, int p, int p_top, TraceSlot *slot, char *lds,
+                                           uint32_t blk)
 {
     using TL = TileCT<R, T_LOG, R_LOG, TRACE>;
     constexpr int A0 = T_LOG - R_LOG;
-    extern __shared__ __align__(16) char lds[];
     const uint32_t tau = threadIdx.x;
-    const uint64_t vbase = (uint64_t)blockIdx.x << T_LOG;
+    const uint64_t vbase = (uint64_t)blk << T_LOG;
     const uint64_t pbase = map_uniform(map, vbase);
     uint64_t h = 0;
     uint32_t nops = 0;
@@ -262,7 +287,14 @@ __global__ __launch_bounds__(BLOCK) void k_sort_merge(Bufs<R> b, IMap map, int p
     TL::template relayout<TL::end_layout(T_LOG - 1, A0), A0>(r, lds, tau, h, nops);
 #pragma unroll
     for (int m = 0; m < (1 << R_LOG); ++m) gstore_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
-    if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
+    if constexpr (TRACE) tcommit(slot, (uint64_t)blk * BLOCK + tau, h, nops);
+}
+
+template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
+__global__ __launch_bounds__(BLOCK) void k_sort_merge(Bufs<R> b, IMap map, int p, int p_top, TraceSlot *slot)
+{
+    extern __shared__ __align__(16) char lds[];
+    body_merge<R, T_LOG, R_LOG, BLOCK, TRACE>(b, map, p, p_top, slot, lds, blockIdx.x);
 }
 
 // The same merge with resident blocks: each block walks tiles
@@ -318,18 +350,17 @@ __global__ __launch_bounds__(BLOCK) void k_sort_merge_pipe(Bufs<R> b, IMap map, 
 // tile schedule on them.  With C >= 6 a wavefront's 64 lanes always read
 // and write 64 consecutive records.
 template <class R, int T_LOG, int R_LOG, int BLOCK, int S, bool TRACE>
-__global__ __launch_bounds__(BLOCK) void k_sort_hybrid(Bufs<R> b, IMap map, int p, int q_lo, int p_top,
-                                                       TraceSlot *slot)
+__device__ __forceinline__ void body_hybrid(Bufs<R> b, IMap map, int p, int q_lo, int p_top, TraceSlot *slot,
+                                            char *lds, uint32_t blk32)
 {
     using TL = TileCT<R, T_LOG, R_LOG, TRACE>;
     constexpr int C = T_LOG - S;
     constexpr int A0 = T_LOG - R_LOG;
     static_assert(C >= 6 && A0 >= 6, "runs and lanes must cover 64 contiguous records");
-    extern __shared__ __align__(16) char lds[];
     const uint32_t tau = threadIdx.x;
     const uint32_t lane = tau & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tau >> 6);
-    const uint64_t blk = blockIdx.x;
+    const uint64_t blk = blk32;
     const uint64_t b_lo = blk & ((1ull << (q_lo - C)) - 1);
     const uint64_t b_hi = blk >> (q_lo - C);
     const uint64_t ublk = (b_lo << C) | (b_hi << (q_lo + S));
@@ -347,7 +378,15 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hybrid(Bufs<R> b, IMap map, int 
     TL::template relayout<TL::end_layout(T_LOG - 1, A0, C), A0>(r, lds, tau, h, nops);
 #pragma unroll
     for (int m = 0; m < (1 << R_LOG); ++m) gstore_u<R, TRACE>(r[m], b, pw + eoff((uint64_t)m << A0), lane, h, nops);
-    if constexpr (TRACE) tcommit(slot, (uint64_t)blockIdx.x * BLOCK + tau, h, nops);
+    if constexpr (TRACE) tcommit(slot, blk * BLOCK + tau, h, nops);
+}
+
+template <class R, int T_LOG, int R_LOG, int BLOCK, int S, bool TRACE>
+__global__ __launch_bounds__(BLOCK) void k_sort_hybrid(Bufs<R> b, IMap map, int p, int q_lo, int p_top,
+                                                       TraceSlot *slot)
+{
+    extern __shared__ __align__(16) char lds[];
+    body_hybrid<R, T_LOG, R_LOG, BLOCK, S, TRACE>(b, map, p, q_lo, p_top, slot, lds, blockIdx.x);
 }
 
 // ------------------------------------------------------------ global pass
@@ -359,11 +398,10 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hybrid(Bufs<R> b, IMap map, int 
 // each wavefront load or store is one contiguous 512-byte (keys) or 256-byte
 // (payload) run.
 template <class R, int S, bool TRACE>
-__global__ __launch_bounds__(256) void k_sort_global(Bufs<R> b, IMap map, int p, int q_lo, int p_top,
-                                                     TraceSlot *slot)
+__device__ __forceinline__ void body_global(Bufs<R> b, IMap map, int p, int q_lo, int p_top, TraceSlot *slot,
+                                            uint32_t blk, uint32_t tid)
 {
-    const uint32_t tid = threadIdx.x;
-    const uint64_t g0 = (uint64_t)blockIdx.x * 256;
+    const uint64_t g0 = (uint64_t)blk * 256;
     const uint64_t ubase = (g0 & ((1ull << q_lo) - 1)) | ((g0 >> q_lo) << (q_lo + S));
     const uint64_t pbase = map_uniform(map, ubase);
     uint64_t h = 0;
@@ -383,6 +421,13 @@ __global__ __launch_bounds__(256) void k_sort_global(Bufs<R> b, IMap map, int p,
 #pragma unroll
     for (int m = 0; m < (1 << S); ++m) gstore_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << q_lo, tid, h, nops);
     if constexpr (TRACE) tcommit(slot, g0 + tid, h, nops);
+}
+
+template <class R, int S, bool TRACE>
+__global__ __launch_bounds__(256) void k_sort_global(Bufs<R> b, IMap map, int p, int q_lo, int p_top,
+                                                     TraceSlot *slot)
+{
+    body_global<R, S, TRACE>(b, map, p, q_lo, p_top, slot, blockIdx.x, threadIdx.x);
 }
 
 } // namespace dev
@@ -408,6 +453,27 @@ struct Pass {
 // With h_max > s_max, groups larger than s_max run as LDS-staged hybrid
 // passes (kind 4) of up to h_max stages; a level's stages above the tile are
 // split into the fewest passes, as evenly as possible.
+// Width of group gi when `count` stages are split into `groups` passes.
+__host__ __device__ constexpr int plan_width(int count, int groups, int gi)
+{
+    return count / groups + (gi < count % groups ? 1 : 0);
+}
+
+// Fewest passes for `count` stages above the tile such that every width is
+// buildable: a register pass (<= s_max) or a hybrid pass (6 .. h_max).
+__host__ __device__ inline int plan_groups(int count, int s_max, int h_max)
+{
+    const int g_max = s_max > h_max ? s_max : h_max;
+    for (int groups = (count + g_max - 1) / g_max;; ++groups) {
+        bool ok = true;
+        for (int gi = 0; gi < groups; ++gi) {
+            const int s = plan_width(count, groups, gi);
+            if (!(s <= s_max || (s >= 6 && s <= h_max))) ok = false;
+        }
+        if (ok) return groups;
+    }
+}
+
 inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log, int h_max = 0)
 {
     std::vector<Pass> v;
@@ -417,23 +483,12 @@ inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log,
     }
     const int t = std::min(n_log, t_log);
     v.push_back({1, t, 0, t});
-    const int g_max = std::max(s_max, h_max);
-    // a group width is buildable when it is a register pass (<= s_max) or a
-    // hybrid pass (6 .. h_max)
-    auto ok = [&](int count, int groups) {
-        for (int gi = 0; gi < groups; ++gi) {
-            const int s = count / groups + (gi < count % groups ? 1 : 0);
-            if (!(s <= s_max || (s >= 6 && s <= h_max))) return false;
-        }
-        return true;
-    };
     for (int p = t_log + 1; p <= n_log; ++p) {
         const int count = p - t_log;
-        int groups = (count + g_max - 1) / g_max;
-        while (!ok(count, groups)) ++groups;
+        const int groups = plan_groups(count, s_max, h_max);
         int q = p - 1;
         for (int gi = 0; gi < groups; ++gi) {
-            const int s = count / groups + (gi < count % groups ? 1 : 0);
+            const int s = plan_width(count, groups, gi);
             v.push_back({s <= s_max ? 2 : 4, p, q - s + 1, s});
             q -= s;
         }
