@@ -145,12 +145,13 @@ def bench_filter(args):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     o = out.cpu().numpy().view(np.uint64)
-    stats = _family_stats(ctx, ["filter_count", "compact_scan", "compact_local", "compact_global", "filter_sum"])
+    stats = _family_stats(ctx, ["filter_count", "compact_scan", "compact_local", "compact_hybrid", "compact_global",
+                                "filter_sum"])
     # algorithmic bytes per padded row and launch: predicate reads day+qty+disc
     # and writes the mask; the local network reads the mask and both columns
-    # and writes the columns; a global pass reads and writes both columns
-    bpr = {"filter_count": 4 + 8 + 8 + 1, "compact_local": 1 + 2 * 16, "compact_global": 2 * 16,
-           "filter_sum": 16}
+    # and writes the columns; a global or LDS-staged pass reads and writes both
+    bpr = {"filter_count": 4 + 8 + 8 + 1, "compact_local": 1 + 2 * 16, "compact_hybrid": 2 * 16,
+           "compact_global": 2 * 16, "filter_sum": 16}
     _line("oblivious filter + compaction + int128 SUM (Mrows/s)", n / dt / 1e6, "Mrows/s", args.steps,
           args.warmup, dt * 1e3, "int64/int128",
           {"workload": f"Q6-style filter over {n} rows padded to 2^{ilog2(n_pad)}, ORCompact of (price, disc), "
@@ -187,18 +188,27 @@ def bench_join(args):
     on = torch.empty(1, dtype=torch.int64, device=dev)
     ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
 
-    def step():
-        ctx.join_groupsum(d_dk, d_dg, n_dim, d_fk, d_fv, n_fact, n_pad, og, osum, on, 64)
+    def step(dense):
+        # the group attribute's public domain is 0..24: dense GROUP BY over 25 ids
+        ctx.join_groupsum(d_dk, d_dg, n_dim, d_fk, d_fv, n_fact, n_pad, og, osum, on, 25 if dense else 64,
+                          dense_groups=dense)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ctx.kernel_stats_reset()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+    def timed(dense):
+        for _ in range(args.warmup):
+            step(dense)
+        torch.cuda.synchronize()
+        ctx.kernel_stats_reset()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(dense)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps
+
+    dt_sort = timed(False)
+    ok_sort = None
+    ng_sort = int(on.item())
+    sums_sort = osum[:ng_sort].cpu().numpy().copy()
+    dt = timed(True)
     # the host check is a plain numpy group-by over the same arrays
     grp = np.zeros(n_dim + 2, np.int64)
     grp[dk] = dg
@@ -209,16 +219,19 @@ def bench_join(args):
     got_g = og[:ng].cpu().numpy()
     got_s = osum[:ng].cpu().numpy()
     ok = ng == int((exp > 0).sum()) and np.array_equal(got_s, sums[got_g])
-    fams = ["join_build", "join_fill_tiles", "join_fill_rows", "group_keys", "group_seg_tiles", "group_seg_rows",
-            "group_out", "compact_count", "compact_local", "compact_global", "sort_local", "sort_merge",
+    ok_sort = ng_sort == ng and np.array_equal(sums_sort, got_s)
+    fams = ["join_build", "join_fill_tiles", "join_fill_rows", "group_dense_agg", "group_dense_out",
+            "compact_count", "compact_local", "compact_hybrid", "compact_global", "sort_local", "sort_merge",
             "sort_global_s1", "sort_global_s2", "sort_global_s3", "sort_global_s4", "sort_hybrid_s6"]
     stats = _family_stats(ctx, fams)
-    bpr = {f: 32 for f in fams if f.startswith("sort_") or f == "compact_global"}
+    bpr = {f: 32 for f in fams if f.startswith("sort_") or f in ("compact_global", "compact_hybrid")}
     _line("oblivious PK/FK join + GROUP BY SUM (Mrows/s)", (n_dim + n_fact) / dt / 1e6, "Mrows/s", args.steps,
           args.warmup, dt * 1e3, "int64",
           {"workload": "PK/FK join of 1.5M dimension x 6M fact rows (union padded to 2^23) + exact int64 "
                        "SUM per group (25 groups)", "n_dim": n_dim, "n_fact": n_fact, "n_pad": n_pad,
-           "groups": ng, "matches_numpy_groupby": bool(ok)},
+           "groups": ng, "group_by": "dense oblivious aggregation over the 25-id group domain",
+           "matches_numpy_groupby": bool(ok)},
+          sort_groupby_ms=round(dt_sort * 1e3, 3), sort_groupby_equal=bool(ok_sort),
           roofline=_roofline(stats, bpr, n_pad),
           kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()})
     ctx.close()
@@ -362,18 +375,25 @@ def bench_serve(args):
                 "max_concurrent": st.max_concurrent, "peak_GB": round(st.peak_bytes_in_flight / 1e9, 2)}
 
     srv.run(mix(64, 0, 99))  # warm-up
+    srv.run(mix(16, 0, 98), n_slots=1, transfer=True)
     for rate in [0, 50, 200, 800]:
         count = 600 if rate == 0 else max(200, rate * 2)
         qs = mix(count, rate, 7 + rate)
-        rp, sp = srv.run(qs, n_slots=1)
+        # plain: one query at a time, inputs shipped host->device per query
+        # (the reference engine's path); plain_resident: one at a time over the
+        # resident tables; served: 16 slots over the resident tables
+        rp, sp = srv.run(qs, n_slots=1, transfer=True)
+        rr, sr = srv.run(qs, n_slots=1)
         rb, sb = srv.run(qs)
-        same = [list(a.r) for a in rp] == [list(b.r) for b in rb]
+        same = [list(a.r) for a in rp] == [list(b.r) for b in rb] == [list(c.r) for c in rr]
         _line("served queries/s (continuous batching, 16 slots)", sb.qps, "QPS", count, 64, 1e3 / max(sb.qps, 1e-9),
               "int64/int128",
               {"workload": "served mix 50% filter+SUM / 30% join+group-by / 20% sort, tables log-uniform "
                            "in [1e4, 1e7] rows, 200 GB admission budget",
                "offered_qps": rate if rate else "all at t=0", "queries": count},
-              batched=summ(sb), plain=summ(sp), throughput_ratio=round(sb.qps / max(sp.qps, 1e-9), 2),
+              served=summ(sb), plain=summ(sp), plain_resident=summ(sr),
+              throughput_ratio=round(sb.qps / max(sp.qps, 1e-9), 2),
+              throughput_ratio_vs_plain_resident=round(sb.qps / max(sr.qps, 1e-9), 2),
               results_equal=same)
     srv.close()
 

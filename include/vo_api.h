@@ -237,6 +237,10 @@ typedef struct vo_join_args {
     uint64_t max_groups;
     uint64_t *sorted_keys;
     uint64_t *sorted_vals;
+    uint32_t dense_groups; /* 1: group ids lie in [0, max_groups) (a public bound, max_groups <= 64);
+                              the GROUP BY is an oblivious dense aggregation over the compacted
+                              matched rows instead of sort + segmented scan + compaction.  Rows
+                              whose group is outside the bound are not counted. */
 } vo_join_args;
 
 /* OP_SORT_BATCH: independent sorts in one schedule.  Item i sorts
@@ -284,8 +288,6 @@ typedef struct vo_command {
 enum {
     VO_CFG_PERSISTENT = 1u << 0,    /* run commands in the resident queue kernel */
     VO_CFG_KERNEL_TIMING = 1u << 1, /* HIP events around every launch (vo_kernel_stats) */
-    VO_CFG_QUEUE_SHORT_IDLE = 1u << 2, /* resident kernel leaves after 50 ms without commands
-                                          (default 20 s); the next vo_submit starts it again */
 };
 
 typedef struct vo_config {
@@ -295,6 +297,8 @@ typedef struct vo_config {
     int32_t device;          /* HIP device ordinal; -1 -> current */
     uint32_t wait_timeout_ms;/* default timeout of vo_wait when 0 is passed; 0 -> 60000 */
     uint32_t tuning;         /* 0 -> autotuned table / built-in defaults */
+    uint32_t queue_idle_us;  /* the resident queue kernel leaves after this long without a
+                                command (0 -> 20000); the next vo_submit starts it again */
 } vo_config;
 
 typedef struct vo_ctx vo_ctx;
@@ -307,7 +311,12 @@ VO_EXPORT int vo_version(void);
 
 /* TaskManager::start / stop (src/cuda/task_manager.cu:26-124).  start()
  * launches the resident queue kernel when VO_CFG_PERSISTENT is set; without
- * it commands are launched directly on the context stream. */
+ * it commands are launched directly on the context stream.
+ * While the resident kernel runs it holds registers and LDS on every CU, so
+ * kernels of other streams and libraries may wait until it leaves: it
+ * leaves after queue_idle_us without commands, on vo_queue_stop, and when a
+ * synchronous call of this context needs an operator the resident kernel
+ * does not run (the kernel first finishes every submitted command). */
 VO_EXPORT int vo_queue_start(vo_ctx *ctx);
 VO_EXPORT int vo_queue_stop(vo_ctx *ctx, uint32_t timeout_ms);
 VO_EXPORT int vo_queue_running(const vo_ctx *ctx);
@@ -431,9 +440,21 @@ VO_EXPORT int vo_server_add_fact_table(vo_server *s, uint64_t seed, uint64_t n_r
 VO_EXPORT int vo_server_add_join_tables(vo_server *s, uint64_t seed, uint64_t n_dim, uint64_t n_fact, uint32_t *id);
 VO_EXPORT int vo_server_add_sort_table(vo_server *s, uint64_t seed, uint64_t n_rows, uint32_t *id);
 /* Replays n queries; n_slots_override (0 = the server's) picks the
- * concurrency, 1 gives the one-query-at-a-time loop. */
+ * concurrency, 1 gives the one-query-at-a-time loop.  Tables are read in
+ * place in HBM (VO_RUN_RESIDENT). */
 VO_EXPORT int vo_server_run(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_slots_override,
                             vo_query_result *out, vo_run_stats *stats);
+
+/* How a run reaches the tables. */
+enum {
+    VO_RUN_RESIDENT = 0, /* queries read the device-resident tables in place */
+    VO_RUN_TRANSFER = 1  /* the reference engine's per-query path: each query copies its input
+                            columns from pinned host memory into device buffers
+                            (ValueVector::to(CUDA), include/data/value_vector.hpp:248-272), and a
+                            sort copies its sorted columns back to host memory */
+};
+VO_EXPORT int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_slots_override,
+                                 uint32_t mode, vo_query_result *out, vo_run_stats *stats);
 
 /* ------------------------------------------------------ data generators */
 

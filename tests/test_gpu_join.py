@@ -8,14 +8,15 @@ from tests.gpu_util import empty_dev, from_dev, to_dev
 pytestmark = pytest.mark.gpu
 
 
-def _run(vctx, dk, dg, fk, fv, n_pad, max_groups=64):
+def _run(vctx, dk, dg, fk, fv, n_pad, max_groups=64, dense=False):
     d = [to_dev(x) for x in (dk, dg, fk, fv)]
     og = empty_dev(max_groups, np.int32)
     os_ = empty_dev(max_groups, np.int64)
     on = empty_dev(1, np.uint64)
     sk = empty_dev(n_pad, np.uint64)
     sv = empty_dev(n_pad, np.uint64)
-    vctx.join_groupsum(d[0], d[1], len(dk), d[2], d[3], len(fk), n_pad, og, os_, on, max_groups, sk, sv)
+    vctx.join_groupsum(d[0], d[1], len(dk), d[2], d[3], len(fk), n_pad, og, os_, on, max_groups, sk, sv,
+                       dense_groups=dense)
     G = int(from_dev(on, np.uint64)[0])
     g = from_dev(og, np.int32)
     s = from_dev(os_, np.int64)
@@ -23,23 +24,25 @@ def _run(vctx, dk, dg, fk, fv, n_pad, max_groups=64):
     return g[:G], s[:G], from_dev(sk, np.uint64), from_dev(sv, np.uint64)
 
 
+@pytest.mark.parametrize("dense", [False, True])
 @pytest.mark.parametrize("n_dim,n_fact,n_pad", [(100, 400, 4096), (1500, 6000, 8192), (20000, 50000, 1 << 17),
                                                 (150000, 600000, 1 << 20)])
-def test_join_groupsum_bit_exact(vctx, n_dim, n_fact, n_pad):
-    print(f"join n_dim={n_dim} n_fact={n_fact}", flush=True)
+def test_join_groupsum_bit_exact(vctx, n_dim, n_fact, n_pad, dense):
+    print(f"join n_dim={n_dim} n_fact={n_fact} dense={dense}", flush=True)
     dk, dg, fk, fv = oracle.gen_join(4, n_dim, n_fact)
-    g, s, sk, sv = _run(vctx, dk, dg, fk, fv, n_pad)
+    g, s, sk, sv = _run(vctx, dk, dg, fk, fv, n_pad, dense=dense)
     eg, es, esk, esv = oracle.join_groupsum(dk, dg, fk, fv, n_pad, want_sorted=True)
     assert np.array_equal(sk, esk) and np.array_equal(sv, esv)
     assert np.array_equal(g, eg) and np.array_equal(s, es)
 
 
-def test_join_config3_full_size(vctx):
+@pytest.mark.parametrize("dense", [False, True])
+def test_join_config3_full_size(vctx, dense):
     """Config: 1.5M dimension rows (unique keys 1..1.5M, 25 groups), 6M fact
     rows, union padded to 2^23; 25 exact int64 sums."""
     n_dim, n_fact, n_pad = 1_500_000, 6_000_000, 1 << 23
     dk, dg, fk, fv = oracle.gen_join(1, n_dim, n_fact)
-    g, s, _, _ = _run(vctx, dk, dg, fk, fv, n_pad)
+    g, s, _, _ = _run(vctx, dk, dg, fk, fv, n_pad, max_groups=25 if dense else 64, dense=dense)
     # closed form: every fact row matches exactly one dimension row
     g_of_key = np.zeros(n_dim + 1, np.int64)
     g_of_key[dk] = dg
@@ -51,13 +54,30 @@ def test_join_config3_full_size(vctx):
     assert want.sum() == n_fact
 
 
-def test_join_unmatched_and_empty_groups(vctx):
+@pytest.mark.parametrize("dense", [False, True])
+def test_join_unmatched_and_empty_groups(vctx, dense):
     dk = np.array([1, 2, 3, 10], np.int32)
     dg = np.array([0, 1, 0, 7], np.int32)
     fk = np.array([1, 4, 3, 3, 9, 2, 11, 11], np.int32)
     fv = np.array([10, 1000, 5, 7, 1000, 3, 99, 99], np.int64)
-    g, s, _, _ = _run(vctx, dk, dg, fk, fv, 4096)
+    g, s, _, _ = _run(vctx, dk, dg, fk, fv, 4096, max_groups=8, dense=dense)
     assert g.tolist() == [0, 1] and s.tolist() == [22, 3]
+
+
+def test_join_dense_groups_all_64_present(vctx):
+    """Every group id 0..63 present: the dense path fills all 64 output slots
+    in ascending order, equal to the sort-based path."""
+    rng = np.random.default_rng(9)
+    n_dim, n_fact = 6400, 40000
+    dk = (rng.permutation(n_dim) + 1).astype(np.int32)
+    dg = (np.arange(n_dim) % 64).astype(np.int32)
+    fk = dk[rng.integers(0, n_dim, n_fact)]
+    fv = rng.integers(100, 10_000_000, n_fact).astype(np.int64)
+    gd, sd, _, _ = _run(vctx, dk, dg, fk, fv, 1 << 16, max_groups=64, dense=True)
+    gs, ss, _, _ = _run(vctx, dk, dg, fk, fv, 1 << 16, max_groups=64, dense=False)
+    eg, es = oracle.join_groupsum(dk, dg, fk, fv, 1 << 16)
+    assert gd.tolist() == list(range(64)) and np.array_equal(gd, eg) and np.array_equal(sd, es)
+    assert np.array_equal(gs, eg) and np.array_equal(ss, es)
 
 
 def test_join_single_group_spans_tiles(vctx):

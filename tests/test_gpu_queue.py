@@ -168,7 +168,7 @@ def test_queue_rejects_launch_only_opcode(qctx):
 def test_queue_restarts_after_idle_exit(torch_dev):
     import velodb_amd
     from velodb_amd import _abi
-    ctx = velodb_amd.Context(flags=_abi.CFG_PERSISTENT | _abi.CFG_QUEUE_SHORT_IDLE)
+    ctx = velodb_amd.Context(flags=_abi.CFG_PERSISTENT, queue_idle_us=2000)
     try:
         ctx.queue_start()
         for rnd in range(3):
@@ -182,10 +182,34 @@ def test_queue_restarts_after_idle_exit(torch_dev):
             t0 = time.time()
             while ctx.queue_running() and time.time() - t0 < 5:
                 time.sleep(0.01)
-            assert not ctx.queue_running()  # left after 50 ms without commands
+            assert not ctx.queue_running()  # left after 2 ms without commands
         ctx.queue_stop()
     finally:
         ctx.close()
+
+
+def test_queue_parks_for_launch_only_call(qctx):
+    """A synchronous launch-only operator in persistent mode first finishes
+    the submitted commands and lets the resident kernel leave; the next
+    submit starts it again."""
+    n = 6000
+    k, p = oracle.gen_sort_input(77, n)
+    dk, dp = to_dev(k, 8192), to_dev(p, 8192)
+    pay = to_dev(np.arange(1 << 12, dtype=np.uint32))
+    _sync()
+    s1 = qctx.submit(_sort_cmd(dk, dp, n, 8192))
+    qctx.shuffle(pay, 1 << 12, 5)  # launch-only: parks the queue first
+    assert qctx.poll(s1)
+    assert not qctx.queue_running()
+    assert sorted(_host(pay, np.uint32).tolist()) == list(range(1 << 12))
+    ek, ep = oracle.bitonic_sort_kp(k, p, n, 8192)
+    assert np.array_equal(_host(dk, np.uint64), ek) and np.array_equal(_host(dp, np.uint32), ep)
+    k2, p2 = oracle.gen_sort_input(78, n)
+    dk2, dp2 = to_dev(k2, 8192), to_dev(p2, 8192)
+    _sync()
+    qctx.wait(qctx.submit(_sort_cmd(dk2, dp2, n, 8192)), 20000)
+    ek2, ep2 = oracle.bitonic_sort_kp(k2, p2, n, 8192)
+    assert np.array_equal(_host(dk2, np.uint64), ek2) and np.array_equal(_host(dp2, np.uint32), ep2)
 
 
 def test_queue_start_without_persistent_flag_is_launch_mode(vctx):

@@ -48,6 +48,20 @@ def test_results_identical_alone_and_in_the_mix():
         s.close()
 
 
+def test_transfer_mode_matches_resident():
+    """The reference-style path (inputs shipped from pinned host memory per
+    query) computes the same results as the resident tables."""
+    s, tables = _server(n_slots=4)
+    try:
+        qs = _mix(tables, 30, 0, 4)
+        res, _ = s.run(qs)
+        tr, _ = s.run(qs, n_slots=1, transfer=True)
+        assert all(r.status == 0 for r in tr)
+        assert [list(r.r) for r in res] == [list(r.r) for r in tr]
+    finally:
+        s.close()
+
+
 def test_admission_respects_budget():
     s, tables = _server(n_slots=8, budget=300 << 20)
     try:

@@ -33,6 +33,14 @@ def load_library(path=None):
     if not os.path.exists(p):
         raise ImportError(f"{p} is missing: build it with __graft_entry__.build() "
                           f"(make -C velodb_amd/csrc)")
+    # One HIP runtime per process: when torch (which ships its own HIP and HSA
+    # runtimes) is installed, load it first so that the library binds to the
+    # runtime torch uses.  Loaded the other way round, the process would hold
+    # two runtimes and torch could no longer open the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = _abi.bind(C.CDLL(p))
     if path is None:
         _lib = lib
@@ -82,12 +90,16 @@ class Server:
         self._check(self.lib.vo_server_add_sort_table(self.h, seed, n_rows, C.byref(i)))
         return i.value
 
-    def run(self, queries, n_slots=0):
-        """queries: list of (kind, table, arrival_s).  Returns (results, stats)."""
+    def run(self, queries, n_slots=0, transfer=False):
+        """queries: list of (kind, table, arrival_s).  Returns (results, stats).
+        transfer=True ships each query's inputs from pinned host memory (the
+        reference engine's per-query path) instead of reading the resident
+        tables."""
         arr = (_abi.Query * len(queries))(*[_abi.Query(k, t, a) for (k, t, a) in queries])
         res = (_abi.QueryResult * len(queries))()
         st = _abi.RunStats()
-        self._check(self.lib.vo_server_run(self.h, arr, len(queries), n_slots, res, C.byref(st)))
+        mode = _abi.RUN_TRANSFER if transfer else _abi.RUN_RESIDENT
+        self._check(self.lib.vo_server_run_mode(self.h, arr, len(queries), n_slots, mode, res, C.byref(st)))
         return list(res), st
 
     def close(self):
@@ -110,9 +122,10 @@ class Context:
     create its own stream.
     """
 
-    def __init__(self, stream=None, flags=0, device=-1, queue_capacity=0, wait_timeout_ms=0, tuning=0):
+    def __init__(self, stream=None, flags=0, device=-1, queue_capacity=0, wait_timeout_ms=0, tuning=0,
+                 queue_idle_us=0):
         self.lib = load_library()
-        cfg = _abi.Config(queue_capacity, flags, stream, device, wait_timeout_ms, tuning)
+        cfg = _abi.Config(queue_capacity, flags, stream, device, wait_timeout_ms, tuning, queue_idle_us)
         h = C.c_void_p()
         st = self.lib.vo_ctx_create(C.byref(cfg), C.byref(h))
         if st != 0:
@@ -186,10 +199,11 @@ class Context:
         return self.check(self.lib.vo_filter_sum(self.h, C.byref(a)))
 
     def join_groupsum(self, dim_key, dim_group, n_dim, fact_fk, fact_val, n_fact, n_pad,
-                      out_group, out_sum, out_n_groups, max_groups, sorted_keys=None, sorted_vals=None):
+                      out_group, out_sum, out_n_groups, max_groups, sorted_keys=None, sorted_vals=None,
+                      dense_groups=False):
         a = _abi.JoinArgs(_ptr(dim_key), _ptr(dim_group), n_dim, _ptr(fact_fk), _ptr(fact_val), n_fact, n_pad,
                           _ptr(out_group), _ptr(out_sum), _ptr(out_n_groups), max_groups,
-                          _ptr(sorted_keys), _ptr(sorted_vals))
+                          _ptr(sorted_keys), _ptr(sorted_vals), 1 if dense_groups else 0)
         return self.check(self.lib.vo_pkfk_join_groupsum(self.h, C.byref(a)))
 
     def sort_batch(self, keys, payload, items):

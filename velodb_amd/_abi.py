@@ -42,7 +42,6 @@ OP_TERMINATE = 0xFFFFFFFF
 
 CFG_PERSISTENT = 1 << 0
 CFG_KERNEL_TIMING = 1 << 1
-CFG_QUEUE_SHORT_IDLE = 1 << 2
 
 MAX_SORT_COLUMNS = 8
 
@@ -101,7 +100,7 @@ class FilterArgs(C.Structure):
 class JoinArgs(C.Structure):
     _fields_ = [("dim_key", P), ("dim_group", P), ("n_dim", u64), ("fact_fk", P), ("fact_val", P),
                 ("n_fact", u64), ("n_pad", u64), ("out_group", P), ("out_sum", P), ("out_n_groups", P),
-                ("max_groups", u64), ("sorted_keys", P), ("sorted_vals", P)]
+                ("max_groups", u64), ("sorted_keys", P), ("sorted_vals", P), ("dense_groups", u32)]
 
 
 class BatchItem(C.Structure):
@@ -125,10 +124,11 @@ class Command(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("queue_capacity", u32), ("flags", u32), ("stream", P), ("device", i32),
-                ("wait_timeout_ms", u32), ("tuning", u32)]
+                ("wait_timeout_ms", u32), ("tuning", u32), ("queue_idle_us", u32)]
 
 
 Q_FILTER_SUM, Q_JOIN_GROUPSUM, Q_SORT = 1, 2, 3
+RUN_RESIDENT, RUN_TRANSFER = 0, 1
 
 
 class ServerConfig(C.Structure):
@@ -193,6 +193,8 @@ PROTOTYPES = [
     ("vo_server_add_join_tables", C.c_int, [P, u64, u64, u64, C.POINTER(u32)]),
     ("vo_server_add_sort_table", C.c_int, [P, u64, u64, C.POINTER(u32)]),
     ("vo_server_run", C.c_int, [P, C.POINTER(Query), u64, u32, C.POINTER(QueryResult), C.POINTER(RunStats)]),
+    ("vo_server_run_mode", C.c_int, [P, C.POINTER(Query), u64, u32, u32, C.POINTER(QueryResult),
+                                     C.POINTER(RunStats)]),
     ("vo_gen_sort_input", C.c_int, [P, u64, P, P, u64]),
     ("vo_gen_fact", C.c_int, [P, u64, P, P, P, P, u64]),
 ]

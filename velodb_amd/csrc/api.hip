@@ -233,6 +233,7 @@ int vo_ctx_create(const vo_config *cfg, vo_ctx **out)
         ctx->queue_capacity = c.queue_capacity ? c.queue_capacity : 1024;
         VO_CHECK(is_pow2(ctx->queue_capacity), VO_ERR_INVALID, "queue_capacity must be a power of two");
         ctx->wait_timeout_ms = c.wait_timeout_ms ? c.wait_timeout_ms : 60000;
+        ctx->queue_idle_us = c.queue_idle_us ? c.queue_idle_us : 20000;
         if (c.stream) {
             ctx->stream = (hipStream_t)c.stream;
         } else {
@@ -329,6 +330,19 @@ int vo_sync(vo_ctx *ctx)
 static int run_sync(vo_ctx *ctx, vo_command &cmd)
 {
     if (!ctx) return VO_ERR_INVALID;
+    if (ctx->queue && !queue_supports(cmd.opcode)) {
+        // persistent mode, operator the resident kernel does not run: finish
+        // the submitted commands, let the kernel leave (its registers and LDS
+        // would keep these kernels from starting) and launch on the context
+        // stream; the next vo_submit starts the resident kernel again
+        const int st = queue_park(ctx);
+        if (st != VO_OK) return st;
+        return guarded(ctx, [&] {
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            dispatch(ctx, cmd);
+            VO_HIP(hipStreamSynchronize(ctx->stream));
+        });
+    }
     uint64_t s = vo_submit(ctx, &cmd);
     if (s == 0) return ctx->last_status ? ctx->last_status : VO_ERR_INVALID;
     return vo_wait(ctx, s, 0);
@@ -344,6 +358,10 @@ int vo_sort_kp(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint6
 
 int vo_sort_kp_async(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad)
 {
+    if (ctx && ctx->queue) {
+        const int st = queue_park(ctx);
+        if (st != VO_OK) return st;
+    }
     return guarded(ctx, [&] {
         std::lock_guard<std::mutex> lk(ctx->mu);
         op_sort_kp(ctx, keys, payload, n, n_pad);

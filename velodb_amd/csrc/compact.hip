@@ -72,6 +72,21 @@ __device__ __forceinline__ bool level_flag(uint64_t cb, uint64_t cbh, uint64_t h
     return base ^ (i >= om1);
 }
 
+// The same flag in 32-bit arithmetic, for n_pad <= 2^32 (q <= 31): only the
+// low 32 bits of the prefix counts matter (o is their low q+1 bits, and the
+// count difference m1 <= half is exact modulo 2^32).
+__device__ __forceinline__ bool level_flag32(uint32_t cb, uint32_t cbh, int q, uint32_t i)
+{
+    const uint32_t half = 1u << q;
+    const uint32_t smask = half | (half - 1);
+    const uint32_t o = cb & smask;
+    const uint32_t m1 = cbh - cb;
+    const uint32_t om = o & (half - 1);
+    const uint32_t om1 = (o + m1) & (half - 1);
+    const bool base = ((om + m1) >= half) ^ (o >= half);
+    return base ^ (i >= om1);
+}
+
 // ---------------------------------------------------------------- counting
 
 // Selected rows per tile: sel[i] = mask[i] != 0 && i < n.
@@ -166,20 +181,90 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict_
 
 // ---------------------------------------------------------- local levels
 
-// Levels q = 0 .. CT_LOG-1 inside each 4096-row tile, rows staged in LDS as
-// NW 64-bit words; cpre (LDS) holds the tile's exclusive prefix counts.
+// Levels q = 0 .. CT_LOG-1 inside each 4096-row tile.  Thread tau holds 16
+// rows of each column in registers under layout a: row base(tau, a) | m << a
+// for m < 16, base(tau, a) = the 8 bits of tau around the 4-bit hole at a.
+// Levels a .. a+3 pair rows inside a thread; layouts a = 0, 4, 8 cover the 12
+// levels, with two LDS transposes between them (and one after the coalesced
+// load, layout 8).  Rows are read from `src` and written to `dst` (which may
+// be the same columns).  cpre (LDS) holds the tile's exclusive prefix counts.
+constexpr uint32_t CTPAD = CT + CT / 16; // transpose buffer: row l at l + l/16
+
+__device__ __forceinline__ uint32_t cpos(uint32_t l) { return l + (l >> 4); }
+
+__device__ __forceinline__ uint32_t clay(uint32_t tau, int a) { return (tau & ((1u << a) - 1)) | ((tau >> a) << (a + 4)); }
+
 template <int NW>
-__global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> cols, const uint8_t *__restrict__ mask, uint64_t n,
-                                                          const uint64_t *__restrict__ tile_pre)
+__device__ __forceinline__ void ctranspose(uint64_t (&r)[16][NW], uint64_t *buf, uint32_t tau, int a_from, int a_to)
 {
+    const uint32_t bf = clay(tau, a_from), bt = clay(tau, a_to);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m) buf[cpos(bf | ((uint32_t)m << a_from))] = r[m][w];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 16; ++m) r[m][w] = buf[cpos(bt | ((uint32_t)m << a_to))];
+        __syncthreads();
+    }
+}
+
+template <int NW, int A>
+__device__ __forceinline__ void clevels(uint64_t (&r)[16][NW], const uint16_t *cpre, uint32_t tbase, uint32_t tau)
+{
+    const uint32_t base = clay(tau, A);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int q = A + k;
+        const uint32_t half = 1u << q;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            if (m & (1 << k)) continue;
+            const int m2 = m | (1 << k);
+            const uint32_t e0 = base | ((uint32_t)m << A);
+            const uint32_t b = e0 & ~(2 * half - 1);
+            const uint32_t cb = tbase + cpre[b];
+            bool sw;
+            if (q == 0) {
+                const uint32_t e1 = e0 | 1;
+                const bool s0 = cpre[e0 + 1] != cpre[e0];
+                const bool s1 = cpre[e1 + 1] != cpre[e1];
+                sw = ((!s0) & s1) ^ (bool)(cb & 1);
+            } else {
+                sw = level_flag32(cb, tbase + cpre[b + half], q, e0 & (half - 1));
+            }
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint64_t x = r[m][w], y = r[m2][w];
+                r[m][w] = sw ? y : x;
+                r[m2][w] = sw ? x : y;
+            }
+        }
+    }
+}
+
+// W8: every column is 8 bytes wide (no per-load width dispatch)
+template <int NW, bool W8>
+__global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> src, Cols<NW> dst, const uint8_t *__restrict__ mask,
+                                                          uint64_t n, const uint64_t *__restrict__ tile_pre)
+{
+    static_assert(CT == 16 * CBLOCK, "16 rows per thread");
     extern __shared__ __align__(16) char lds_raw[];
-    uint64_t *rows = reinterpret_cast<uint64_t *>(lds_raw);              // NW * CT words
-    uint16_t *cpre = reinterpret_cast<uint16_t *>(rows + (size_t)NW * CT); // CT + 1 counts <= 4096
+    uint64_t *buf = reinterpret_cast<uint64_t *>(lds_raw);          // one column, CTPAD words
+    uint16_t *cpre = reinterpret_cast<uint16_t *>(buf + CTPAD);      // CT + 1 counts <= 4096
     __shared__ uint32_t wsum[CBLOCK / 64];
     const uint32_t tid = threadIdx.x;
     const uint64_t t0 = (uint64_t)blockIdx.x * CT;
     const uint64_t tbase = tile_pre[blockIdx.x];
     constexpr int PER = CT / CBLOCK; // 16 consecutive rows per thread for the scan
+
+    // rows, coalesced (layout 8: row tid + 256 m); issued before the scan
+    uint64_t r[16][NW];
+#pragma unroll
+    for (int m = 0; m < 16; ++m)
+#pragma unroll
+        for (int w = 0; w < NW; ++w)
+            r[m][w] = W8 ? ((const uint64_t *)src.c[w].p)[t0 + tid + 256u * m] : ld_col(src.c[w], t0 + tid + 256u * m);
 
     // exclusive prefix of sel inside the tile: thread tid owns rows [16 tid, 16 tid + 16)
     uint32_t sel[PER];
@@ -190,7 +275,6 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> cols, const u
         sel[k] = (mask[i] != 0) & (i < n);
         run += sel[k];
     }
-    // block exclusive scan of the per-thread totals
     uint32_t incl = run;
     const uint32_t lane = tid & 63;
     for (int off = 1; off < 64; off <<= 1) {
@@ -208,48 +292,23 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> cols, const u
         c += sel[k];
     }
     if (tid == CBLOCK - 1) cpre[CT] = (uint16_t)c;
-    // rows into LDS (coalesced)
+    // (the first transpose's barriers also publish cpre)
+    ctranspose<NW>(r, buf, tid, 8, 0);
+    clevels<NW, 0>(r, cpre, (uint32_t)tbase, tid);
+    ctranspose<NW>(r, buf, tid, 0, 4);
+    clevels<NW, 4>(r, cpre, (uint32_t)tbase, tid);
+    ctranspose<NW>(r, buf, tid, 4, 8);
+    clevels<NW, 8>(r, cpre, (uint32_t)tbase, tid);
+    uint32_t tid_s = tid; // store addresses recomputed here (register pressure)
+    asm volatile("" : "+v"(tid_s));
 #pragma unroll
-    for (int w = 0; w < NW; ++w)
-#pragma unroll 4
-        for (int k = 0; k < PER; ++k) {
-            const uint32_t l = k * CBLOCK + tid;
-            rows[w * CT + l] = ld_col(cols.c[w], t0 + l);
-        }
-    __syncthreads();
-
-    for (int q = 0; q < CT_LOG; ++q) {
-        const uint32_t half = 1u << q;
-#pragma unroll 4
-        for (int k = 0; k < CT / 2 / CBLOCK; ++k) {
-            const uint32_t pi = k * CBLOCK + tid;
-            const uint32_t e0 = ((pi >> q) << (q + 1)) | (pi & (half - 1));
-            const uint32_t e1 = e0 | half;
-            const uint32_t b = e0 & ~(2 * half - 1);
-            const uint64_t cb = tbase + cpre[b];
-            bool sw;
-            if (q == 0) {
-                const bool s0 = cpre[e0 + 1] != cpre[e0];
-                const bool s1 = cpre[e1 + 1] != cpre[e1];
-                sw = ((!s0) & s1) ^ (bool)(cb & 1);
-            } else {
-                sw = level_flag(cb, tbase + cpre[b + half], half, pi & (half - 1));
-            }
+    for (int m = 0; m < 16; ++m)
 #pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                const uint64_t x = rows[w * CT + e0], y = rows[w * CT + e1];
-                rows[w * CT + e0] = sw ? y : x;
-                rows[w * CT + e1] = sw ? x : y;
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int w = 0; w < NW; ++w)
-#pragma unroll 4
-        for (int k = 0; k < PER; ++k) {
-            const uint32_t l = k * CBLOCK + tid;
-            st_col(cols.c[w], t0 + l, rows[w * CT + l]);
+        for (int w = 0; w < NW; ++w) {
+            if (W8)
+                ((uint64_t *)dst.c[w].p)[t0 + tid_s + 256u * m] = r[m][w];
+            else
+                st_col(dst.c[w], t0 + tid_s + 256u * m, r[m][w]);
         }
 }
 
@@ -298,6 +357,88 @@ __global__ __launch_bounds__(256) void k_compact_global(Cols<NW> cols, const uin
     for (int m = 0; m < (1 << S); ++m)
 #pragma unroll
         for (int w = 0; w < NW; ++w) st_col(cols.c[w], base + ((uint64_t)m << q_lo), r[m][w]);
+}
+
+// ------------------------------------------------- LDS-staged global levels
+
+// S = 5..8 levels q_lo .. q_lo+S-1 (q_lo >= CT_LOG) in one pass.  A block
+// takes the 4096 rows hibase + m * 2^q_lo + j0 + c (m < 2^S, c < 2^(12-S)):
+// local row l = m << (12-S) | c, so level q_lo+k pairs local bit 12-S+k.
+// Registers + LDS transposes as in the local pass (layout 4 for local bits
+// 12-S .. 7, layout 8 for bits 8 .. 11); the prefix counts at every block
+// boundary the levels need, tile_pre at hibase + m * 2^q_lo, go to LDS.
+template <int NW, int A, int S>
+__device__ __forceinline__ void hlevels(uint64_t (&r)[16][NW], const uint32_t *P, uint32_t tau, int q_lo, uint32_t j0)
+{
+    constexpr int CB = CT_LOG - S;
+    const uint32_t base = clay(tau, A);
+#pragma unroll
+    for (int bit = A; bit < A + 4; ++bit) {
+        if (bit < CB) continue;
+        const int k = bit - CB; // level q_lo + k
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            if (m & (1 << (bit - A))) continue;
+            const int m2 = m | (1 << (bit - A));
+            const uint32_t l0 = base | ((uint32_t)m << A);
+            const uint32_t ml = l0 >> CB;
+            const uint32_t mb = ml & ~((2u << k) - 1);
+            const uint32_t i = ((ml & ((1u << k) - 1)) << q_lo) + j0 + (l0 & ((1u << CB) - 1));
+            const bool sw = level_flag32(P[mb], P[mb + (1u << k)], q_lo + k, i);
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint64_t x = r[m][w], y = r[m2][w];
+                r[m][w] = sw ? y : x;
+                r[m2][w] = sw ? x : y;
+            }
+        }
+    }
+}
+
+template <int NW, int S, bool W8>
+__global__ __launch_bounds__(CBLOCK) void k_compact_hybrid(Cols<NW> cols, const uint64_t *__restrict__ tile_pre,
+                                                           int q_lo)
+{
+    static_assert(S >= 5 && S <= 8, "hybrid widths 5..8");
+    constexpr int CB = CT_LOG - S;
+    extern __shared__ __align__(16) char lds_raw[];
+    uint64_t *buf = reinterpret_cast<uint64_t *>(lds_raw);     // CTPAD words
+    uint32_t *P = reinterpret_cast<uint32_t *>(buf + CTPAD); // low words of 2^S + 1 prefix counts
+    const uint32_t tid = threadIdx.x;
+    // row indices fit 32 bits (n_pad <= 2^32)
+    const uint32_t runs = (1u << q_lo) >> CB; // blocks per group of 2^(q_lo+S) rows
+    const uint32_t hibase = (blockIdx.x / runs) << (q_lo + S);
+    const uint32_t j0 = (blockIdx.x % runs) << CB;
+    for (uint32_t k = tid; k <= (1u << S); k += CBLOCK)
+        P[k] = (uint32_t)tile_pre[((uint64_t)hibase + ((uint64_t)k << q_lo)) >> CT_LOG];
+    uint64_t r[16][NW];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const uint32_t l = tid | ((uint32_t)m << 8);
+        const uint32_t g = hibase + ((l >> CB) << q_lo) + j0 + (l & ((1u << CB) - 1));
+#pragma unroll
+        for (int w = 0; w < NW; ++w) r[m][w] = W8 ? ((const uint64_t *)cols.c[w].p)[g] : ld_col(cols.c[w], g);
+    }
+    ctranspose<NW>(r, buf, tid, 8, 4); // its barriers also publish P
+    hlevels<NW, 4, S>(r, P, tid, q_lo, j0);
+    ctranspose<NW>(r, buf, tid, 4, 8);
+    hlevels<NW, 8, S>(r, P, tid, q_lo, j0);
+    // recompute the store addresses here rather than keep the load addresses
+    // live across the levels (register pressure)
+    uint32_t tid_s = tid;
+    asm volatile("" : "+v"(tid_s));
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const uint32_t l = tid_s | ((uint32_t)m << 8);
+        const uint32_t g = hibase + ((l >> CB) << q_lo) + j0 + (l & ((1u << CB) - 1));
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            if (W8)
+                ((uint64_t *)cols.c[w].p)[g] = r[m][w];
+            else
+                st_col(cols.c[w], g, r[m][w]);
+        }
+    }
 }
 
 // ------------------------------------------------------- int128 SUM pass
@@ -401,18 +542,42 @@ Workspace workspace(vo_ctx *ctx, uint64_t n_pad, bool want_mask)
 }
 
 template <int NW>
-void run_network(vo_ctx *ctx, Cols<NW> cols, const uint8_t *mask, uint64_t n, uint64_t n_pad, const uint64_t *tile_pre)
+void run_network(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, uint64_t n, uint64_t n_pad,
+                 const uint64_t *tile_pre)
 {
     const int n_log = ilog2(n_pad);
     const uint32_t nt = (uint32_t)(n_pad / CT);
-    const size_t lds = (size_t)NW * CT * 8 + (CT + 1) * 2;
-    auto kl = k_compact_local<NW>;
+    const size_t lds = (size_t)CTPAD * 8 + (CT + 1) * 2;
+    bool w8 = true;
+    for (int w = 0; w < NW; ++w) w8 = w8 && src.c[w].w == 8 && cols.c[w].w == 8;
+    auto kl = w8 ? k_compact_local<NW, true> : k_compact_local<NW, false>;
     VO_HIP(hipFuncSetAttribute((const void *)kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    launch(ctx, LaunchInfo{"compact_local", dim3(nt), dim3(CBLOCK), (uint32_t)lds, hash_mix(NW, n_log)}, kl, cols,
-           mask, n, tile_pre);
+    launch(ctx, LaunchInfo{"compact_local", dim3(nt), dim3(CBLOCK), (uint32_t)lds, hash_mix(NW, n_log)}, kl, src,
+           cols, mask, n, tile_pre);
+    // levels CT_LOG .. n_log-1: passes of at most 8 levels, as even as
+    // possible; 5..8 levels run LDS-staged, fewer in registers
+    const int count = n_log - CT_LOG;
+    const int groups = (count + 7) / 8;
     int q = CT_LOG;
-    while (q < n_log) {
-        const int s = std::min(n_log - q, NW == 1 ? 5 : 4);
+    for (int gi = 0; gi < groups; ++gi) {
+        const int s = count / groups + (gi < count % groups ? 1 : 0);
+        if (s >= 5) {
+            const size_t hl = (size_t)CTPAD * 8 + 257 * 8;
+            LaunchInfo li{"compact_hybrid", dim3(nt), dim3(CBLOCK), (uint32_t)hl, hash_mix(hash_mix(NW, q), s + 100)};
+            auto pick = [&](auto k5, auto k6, auto k7, auto k8) {
+                auto k = s == 5 ? k5 : s == 6 ? k6 : s == 7 ? k7 : k8;
+                VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hl));
+                launch(ctx, li, k, cols, tile_pre, q);
+            };
+            if (w8)
+                pick(k_compact_hybrid<NW, 5, true>, k_compact_hybrid<NW, 6, true>, k_compact_hybrid<NW, 7, true>,
+                     k_compact_hybrid<NW, 8, true>);
+            else
+                pick(k_compact_hybrid<NW, 5, false>, k_compact_hybrid<NW, 6, false>, k_compact_hybrid<NW, 7, false>,
+                     k_compact_hybrid<NW, 8, false>);
+            q += s;
+            continue;
+        }
         const uint32_t blocks = (uint32_t)((n_pad >> s) / 256);
         LaunchInfo li{"compact_global", dim3(blocks), dim3(256), 0, hash_mix(hash_mix(NW, q), s)};
         switch (s) {
@@ -464,7 +629,7 @@ void compact_rows2(vo_ctx *ctx, uint64_t *c0, uint64_t *c1, const uint8_t *mask,
     launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)tile_cnt, nt,
            tile_pre, count_dev);
     Cols<2> cs{{{c0, 8}, {c1, 8}}};
-    run_network<2>(ctx, cs, mask, n_pad, n_pad, tile_pre);
+    run_network<2>(ctx, cs, cs, mask, n_pad, n_pad, tile_pre);
 }
 
 // Bytes of device memory compact_rows2 needs after count_dev.
@@ -491,10 +656,10 @@ void op_compact(vo_ctx *ctx, const vo_compact_args &a)
         }
         if (take == 2) {
             Cols<2> cs{{{a.cols[c], a.col_bytes[c]}, {a.cols[c + 1], a.col_bytes[c + 1]}}};
-            run_network<2>(ctx, cs, a.mask, a.n, a.n_pad, ws.tile_pre);
+            run_network<2>(ctx, cs, cs, a.mask, a.n, a.n_pad, ws.tile_pre);
         } else {
             Cols<1> cs{{{a.cols[c], a.col_bytes[c]}}};
-            run_network<1>(ctx, cs, a.mask, a.n, a.n_pad, ws.tile_pre);
+            run_network<1>(ctx, cs, cs, a.mask, a.n, a.n_pad, ws.tile_pre);
         }
         c += take;
     }
@@ -514,16 +679,13 @@ void op_filter_sum(vo_ctx *ctx, const vo_filter_args &a)
            a.n, pr, ws.mask, ws.tile_cnt);
     launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)ws.tile_cnt, nt,
            ws.tile_pre, ws.count);
-    // the network runs on the output columns (a copy of price/disc) or on
-    // the caller's price/disc columns in place when no outputs are given
+    // the network's first pass reads the caller's price/disc and writes the
+    // output columns (or the inputs in place when no outputs are given)
     int64_t *op = in_place_copy ? a.out_price : const_cast<int64_t *>(a.price);
     int64_t *od = in_place_copy ? a.out_disc : const_cast<int64_t *>(a.disc);
-    if (in_place_copy) {
-        VO_HIP(hipMemcpyAsync(op, a.price, a.n_pad * 8, hipMemcpyDeviceToDevice, ctx->stream));
-        VO_HIP(hipMemcpyAsync(od, a.disc, a.n_pad * 8, hipMemcpyDeviceToDevice, ctx->stream));
-    }
+    Cols<2> in{{{const_cast<int64_t *>(a.price), 8}, {const_cast<int64_t *>(a.disc), 8}}};
     Cols<2> cs{{{op, 8}, {od, 8}}};
-    run_network<2>(ctx, cs, ws.mask, a.n, a.n_pad, ws.tile_pre);
+    run_network<2>(ctx, in, cs, ws.mask, a.n, a.n_pad, ws.tile_pre);
     launch(ctx, LaunchInfo{"filter_sum", dim3(kSumBlocks), dim3(256), 0, a.n_pad}, k_sum_prod_i128, (const int64_t *)op,
            (const int64_t *)od, a.n_pad, (const uint64_t *)ws.count, ws.partial);
     launch(ctx, LaunchInfo{"filter_sum_final", dim3(1), dim3(64), 0, 0}, k_sum_final, (const __int128 *)ws.partial,

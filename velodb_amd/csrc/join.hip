@@ -19,6 +19,9 @@
 //      group 0xFFFFFFFF
 //   6. segmented inclusive sum by group; the last row of each group selected
 //   7. ORCompact -> the first G rows are (group, sum), ascending by group.
+// With dense_groups (group ids known to lie in [0, max_groups), max_groups
+// <= 64), steps 5-7 become one oblivious dense aggregation over the
+// compacted rows: every row updates all max_groups accumulators by selects.
 // Every pass reads and writes every row; only the group count G and the
 // match count leave the device.
 #include "vo_internal.hpp"
@@ -89,18 +92,50 @@ __global__ __launch_bounds__(JB) void k_fill_tiles(const uint64_t *__restrict__ 
     }
 }
 
-// Exclusive "last dimension row" scan over tiles (one thread, nt entries).
-__global__ void k_fill_carry(const Fill *__restrict__ tile_last, uint32_t nt, Fill *__restrict__ carry)
+// Exclusive scan over the nt tile summaries with one 1024-thread block:
+// each thread folds a contiguous chunk, a Hillis-Steele scan combines the
+// chunk summaries in LDS, then each thread writes its chunk's exclusive
+// prefixes.  op(a, b) is associative with identity `id`.
+constexpr uint32_t kScanThreads = 1024;
+
+template <class T, class Op>
+__device__ void block_exclusive_scan(const T *__restrict__ in, uint32_t nt, T *__restrict__ out, T id, Op op, T *sh)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    Fill c{PAD_KEY, 0};
-    for (uint32_t t = 0; t < nt; ++t) {
-        carry[t] = c;
-        const Fill f = tile_last[t];
-        const bool has = f.key != PAD_KEY;
-        c.key = has ? f.key : c.key;
-        c.group = has ? f.group : c.group;
+    const uint32_t per = (nt + kScanThreads - 1) / kScanThreads;
+    const uint32_t lo = threadIdx.x * per;
+    T s = id;
+    for (uint32_t k = 0; k < per; ++k)
+        if (lo + k < nt) s = op(s, in[lo + k]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
+        const T v = threadIdx.x >= off ? sh[threadIdx.x - off] : id;
+        __syncthreads();
+        if (threadIdx.x >= off) sh[threadIdx.x] = op(v, sh[threadIdx.x]);
+        __syncthreads();
     }
+    T run = threadIdx.x ? sh[threadIdx.x - 1] : id; // exclusive prefix of this chunk
+    for (uint32_t k = 0; k < per; ++k)
+        if (lo + k < nt) {
+            out[lo + k] = run;
+            run = op(run, in[lo + k]);
+        }
+}
+
+struct FillOp {
+    __device__ Fill operator()(const Fill &a, const Fill &b) const
+    {
+        const bool has = b.key != PAD_KEY;
+        return Fill{has ? b.key : a.key, has ? b.group : a.group};
+    }
+};
+
+// Exclusive "last dimension row" scan over tiles.
+__global__ __launch_bounds__(kScanThreads) void k_fill_carry(const Fill *__restrict__ tile_last, uint32_t nt,
+                                                             Fill *__restrict__ carry)
+{
+    __shared__ Fill sh[kScanThreads];
+    block_exclusive_scan(tile_last, nt, carry, Fill{PAD_KEY, 0}, FillOp{}, sh);
 }
 
 // Per row: (group, value) columns and the matched mask.
@@ -108,7 +143,7 @@ __global__ __launch_bounds__(JB) void k_fill_rows(const uint64_t *__restrict__ k
                                                   const Fill *__restrict__ carry, uint64_t *__restrict__ col_g,
                                                   uint64_t *__restrict__ col_v, uint8_t *__restrict__ mask)
 {
-    __shared__ uint64_t wk[JB], wg[JB];
+    __shared__ uint64_t wtk[JB / 64], wtg[JB / 64];
     const uint64_t t0 = (uint64_t)blockIdx.x * JT + (uint64_t)threadIdx.x * JPER;
     uint64_t lk = PAD_KEY, lg = 0;
 #pragma unroll
@@ -118,15 +153,34 @@ __global__ __launch_bounds__(JB) void k_fill_rows(const uint64_t *__restrict__ k
         lk = d ? (key >> 32) : lk;
         lg = d ? v : lg;
     }
-    wk[threadIdx.x] = lk;
-    wg[threadIdx.x] = lg;
+    // carry into this thread (the last dimension row before its 16 rows):
+    // wavefront inclusive scan of "the later dimension row wins", then the
+    // wavefront totals through LDS
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t ik = lk, ig = lg;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t pk = __shfl_up(ik, off, 64), pg = __shfl_up(ig, off, 64);
+        const bool take = (lane >= (uint32_t)off) & (ik == PAD_KEY);
+        ik = take ? pk : ik;
+        ig = take ? pg : ig;
+    }
+    const uint64_t ek = __shfl_up(ik, 1, 64), eg = __shfl_up(ig, 1, 64);
+    if (lane == 63) {
+        wtk[wv] = ik;
+        wtg[wv] = ig;
+    }
     __syncthreads();
-    // carry into this thread: the last dimension row before its 16 rows
     Fill c = carry[blockIdx.x];
-    for (uint32_t t = 0; t < threadIdx.x; ++t) {
-        const bool has = wk[t] != PAD_KEY;
-        c.key = has ? wk[t] : c.key;
-        c.group = has ? wg[t] : c.group;
+    for (uint32_t w = 0; w < wv; ++w) {
+        const bool has = wtk[w] != PAD_KEY;
+        c.key = has ? wtk[w] : c.key;
+        c.group = has ? wtg[w] : c.group;
+    }
+    {
+        const bool has = (lane > 0) & (ek != PAD_KEY);
+        c.key = has ? ek : c.key;
+        c.group = has ? eg : c.group;
     }
 #pragma unroll
     for (int k = 0; k < JPER; ++k) {
@@ -210,20 +264,45 @@ __global__ __launch_bounds__(JB) void k_seg_tiles(const uint64_t *__restrict__ k
     }
 }
 
-// Carry into each tile: the sum of the run of the tile's first group that
-// ends just before the tile (0 when the previous tile ends another group).
-__global__ void k_seg_carry(const Seg *__restrict__ seg, const uint64_t *__restrict__ keys, uint32_t nt,
-                            int64_t *__restrict__ carry)
+// Summary of a range of tiles for the segmented carry: group of its first
+// and last row, sum of its trailing run of last_g, whether it is one group.
+struct SegSum {
+    uint64_t first_g, last_g;
+    int64_t tail;
+    uint32_t all_one, empty;
+};
+
+struct SegOp {
+    __device__ SegSum operator()(const SegSum &a, const SegSum &b) const
+    {
+        if (a.empty) return b;
+        if (b.empty) return a;
+        const bool joins = b.all_one & (a.last_g == b.first_g);
+        return SegSum{a.first_g, b.last_g, joins ? a.tail + b.tail : b.tail,
+                      (uint32_t)(a.all_one & joins), 0u};
+    }
+};
+
+__global__ void k_seg_summ(const Seg *__restrict__ seg, const uint64_t *__restrict__ keys, uint32_t nt,
+                           SegSum *__restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int64_t run = 0;  // running sum of the trailing group up to the end of tile t-1
-    uint64_t rg = ~0ull;
-    for (uint32_t t = 0; t < nt; ++t) {
-        const uint64_t first = grp(keys[(uint64_t)t * JT]);
-        carry[t] = (first == rg) ? run : 0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
         const Seg s = seg[t];
-        run = (s.all_one & (s.last_g == rg)) ? run + s.tail_sum : s.tail_sum;
-        rg = s.last_g;
+        out[t] = SegSum{grp(keys[(uint64_t)t * JT]), s.last_g, s.tail_sum, s.all_one, 0u};
+    }
+}
+
+// Carry into each tile: the sum of the run of the tile's first group that
+// ends just before the tile (0 when the tiles before end in another group).
+__global__ __launch_bounds__(kScanThreads) void k_seg_carry(const SegSum *__restrict__ summ, uint32_t nt,
+                                                            SegSum *__restrict__ pre, int64_t *__restrict__ carry)
+{
+    __shared__ SegSum sh[kScanThreads];
+    block_exclusive_scan(summ, nt, pre, SegSum{0, 0, 0, 0, 1}, SegOp{}, sh);
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < nt; t += kScanThreads) {
+        const SegSum p = pre[t];
+        carry[t] = (!p.empty && p.last_g == summ[t].first_g) ? p.tail : 0;
     }
 }
 
@@ -291,6 +370,120 @@ __global__ void k_group_out(const uint64_t *__restrict__ col_g, const uint64_t *
     if (blockIdx.x == 0 && threadIdx.x == 0 && out_n) *out_n = G;
 }
 
+// ------------------------------- dense GROUP BY (group ids in [0, G))
+
+// Oblivious dense aggregation over the compacted matched rows: every row
+// (all n_pad of them; rows >= count contribute zero) updates all G per-thread
+// accumulators through selects, so no address depends on a group id.
+constexpr uint32_t kAggBlocks = 1024;
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_dense_agg(const uint64_t *__restrict__ col_g, const uint64_t *__restrict__ col_v,
+                                                   const uint64_t *__restrict__ count, uint64_t n_pad,
+                                                   int64_t *__restrict__ part_sum, int64_t *__restrict__ part_cnt)
+{
+    __shared__ int64_t rs[4][G], rc[4][G];
+    const uint64_t m = *count;
+    int64_t acc[G];
+    uint32_t cnt[G]; // rows per thread stay far below 2^32
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        acc[g] = 0;
+        cnt[g] = 0;
+    }
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * 256) {
+        const uint64_t grp_i = col_g[i] & 0xFFFFFFFFull;
+        const int64_t v = (int64_t)col_v[i];
+        const bool live = i < m;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const bool hit = live & (grp_i == (uint64_t)g);
+            acc[g] += hit ? v : 0;
+            cnt[g] += hit ? 1 : 0;
+        }
+    }
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int64_t s = wave_sum64(acc[g]), c = wave_sum64((int64_t)cnt[g]);
+        if (lane == 0) {
+            rs[w][g] = s;
+            rc[w][g] = c;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < G) {
+        const uint32_t g = threadIdx.x;
+        part_sum[(uint64_t)blockIdx.x * G + g] = rs[0][g] + rs[1][g] + rs[2][g] + rs[3][g];
+        part_cnt[(uint64_t)blockIdx.x * G + g] = rc[0][g] + rc[1][g] + rc[2][g] + rc[3][g];
+    }
+}
+
+// One block: totals per group (fixed summation order), then output slot t
+// takes the t-th present group, chosen by selects over all groups.
+template <int G>
+__global__ __launch_bounds__(256) void k_dense_out(const int64_t *__restrict__ part_sum,
+                                                   const int64_t *__restrict__ part_cnt, uint32_t nparts,
+                                                   uint64_t max_groups, int32_t *__restrict__ out_g,
+                                                   int64_t *__restrict__ out_s, uint64_t *__restrict__ out_n)
+{
+    __shared__ int64_t ts[G], tc[G];
+    __shared__ int64_t ws[4][G], wc[4][G];
+    const uint32_t t = threadIdx.x;
+    // thread t sums partial rows t, t+256, ...; then a block reduction per group
+    // (a thread's column of partials is contiguous: coalesced across lanes)
+    for (int g0 = 0; g0 < G; g0 += 16) {
+        int64_t s[16], c[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s[k] = c[k] = 0;
+        for (uint32_t p = t; p < nparts; p += 256) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                s[k] += part_sum[(uint64_t)p * G + g0 + k];
+                c[k] += part_cnt[(uint64_t)p * G + g0 + k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int64_t rs_ = wave_sum64(s[k]), rc_ = wave_sum64(c[k]);
+            if ((t & 63) == 0) {
+                ws[t >> 6][g0 + k] = rs_;
+                wc[t >> 6][g0 + k] = rc_;
+            }
+        }
+    }
+    __syncthreads();
+    if (t < G) {
+        ts[t] = ws[0][t] + ws[1][t] + ws[2][t] + ws[3][t];
+        const int64_t c = wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+        tc[t] = (uint64_t)t < max_groups ? c : 0;
+    }
+    __syncthreads();
+    uint64_t present = 0;
+    for (int g = 0; g < G; ++g) present += tc[g] > 0;
+    for (uint64_t slot = t; slot < max_groups; slot += blockDim.x) {
+        int32_t og = -1;
+        int64_t os = 0;
+        uint64_t rank = 0;
+        for (int g = 0; g < G; ++g) {
+            const bool here = (tc[g] > 0) & (rank == slot);
+            og = here ? g : og;
+            os = here ? ts[g] : os;
+            rank += tc[g] > 0;
+        }
+        out_g[slot] = og;
+        out_s[slot] = os;
+    }
+    if (t == 0 && out_n) *out_n = present;
+}
+
 dim3 grid_for(uint64_t n) { return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 65536))); }
 
 } // namespace
@@ -303,6 +496,8 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
     VO_CHECK(a.n_dim < (1ull << 31) && a.n_fact < (1ull << 31), VO_ERR_INVALID, "row counts must be below 2^31");
     VO_CHECK(a.dim_key && a.dim_group && a.fact_fk && a.fact_val && a.out_group && a.out_sum, VO_ERR_INVALID,
              "null buffer");
+    VO_CHECK(!a.dense_groups || (a.max_groups >= 1 && a.max_groups <= 64), VO_ERR_INVALID,
+             "dense GROUP BY takes 1 <= max_groups <= 64");
     const uint32_t nt = (uint32_t)(n_pad / JT);
     // arena: keys, vals, col_a, col_b (8 B each), mask (1 B), tile arrays, compaction scratch
     size_t off = 0;
@@ -313,8 +508,9 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
     };
     const size_t o_k = take(n_pad * 8), o_v = take(n_pad * 8), o_a = take(n_pad * 8), o_b = take(n_pad * 8),
                  o_m = take(n_pad), o_fl = take(nt * sizeof(Fill)), o_fc = take(nt * sizeof(Fill)),
-                 o_sg = take(nt * sizeof(Seg)), o_sc = take(nt * 8), o_c1 = take(compact_rows2_scratch(n_pad)),
-                 o_c2 = take(compact_rows2_scratch(n_pad));
+                 o_sg = take(nt * sizeof(Seg)), o_sc = take(nt * 8), o_ss = take(nt * sizeof(SegSum)),
+                 o_sp = take(nt * sizeof(SegSum)), o_c1 = take(compact_rows2_scratch(n_pad)),
+                 o_c2 = take(compact_rows2_scratch(n_pad)), o_part = take(kAggBlocks * 64 * 16);
     char *base = (char *)scratch(ctx, off);
     uint64_t *keys = (uint64_t *)(base + o_k), *vals = (uint64_t *)(base + o_v);
     uint64_t *col_a = (uint64_t *)(base + o_a), *col_b = (uint64_t *)(base + o_b);
@@ -322,6 +518,7 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
     Fill *tile_last = (Fill *)(base + o_fl), *fcarry = (Fill *)(base + o_fc);
     Seg *seg = (Seg *)(base + o_sg);
     int64_t *scarry = (int64_t *)(base + o_sc);
+    SegSum *ssum = (SegSum *)(base + o_ss), *spre = (SegSum *)(base + o_sp);
     uint64_t *cnt1 = (uint64_t *)(base + o_c1), *cnt2 = (uint64_t *)(base + o_c2);
 
     // 1-2. union and sort
@@ -333,12 +530,35 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
     // 3. forward fill
     launch(ctx, LaunchInfo{"join_fill_tiles", dim3(nt), dim3(JB), 0, n_pad}, k_fill_tiles, (const uint64_t *)keys,
            (const uint64_t *)vals, tile_last);
-    launch(ctx, LaunchInfo{"join_fill_carry", dim3(1), dim3(64), 0, nt}, k_fill_carry, (const Fill *)tile_last, nt,
-           fcarry);
+    launch(ctx, LaunchInfo{"join_fill_carry", dim3(1), dim3(kScanThreads), 0, nt}, k_fill_carry,
+           (const Fill *)tile_last, nt, fcarry);
     launch(ctx, LaunchInfo{"join_fill_rows", dim3(nt), dim3(JB), 0, n_pad}, k_fill_rows, (const uint64_t *)keys,
            (const uint64_t *)vals, (const Fill *)fcarry, col_a, col_b, mask);
     // 4. compaction of matched rows
     compact_rows2(ctx, col_a, col_b, mask, n_pad, cnt1);
+    if (a.dense_groups) {
+        // 5'. dense GROUP BY over the compacted rows
+        int64_t *part = (int64_t *)(base + o_part);
+        // 8 rows per thread at least; at most kAggBlocks partial rows
+        const uint32_t nb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kAggBlocks, n_pad / 2048));
+        const uint32_t G = a.max_groups <= 32 ? 32 : 64;
+        if (G == 32) {
+            launch(ctx, LaunchInfo{"group_dense_agg", dim3(nb), dim3(256), 0, hash_mix(n_pad, 32)},
+                   k_dense_agg<32>, (const uint64_t *)col_a, (const uint64_t *)col_b, (const uint64_t *)cnt1, n_pad,
+                   part, part + (uint64_t)nb * 32);
+            launch(ctx, LaunchInfo{"group_dense_out", dim3(1), dim3(256), 0, a.max_groups}, k_dense_out<32>,
+                   (const int64_t *)part, (const int64_t *)(part + (uint64_t)nb * 32), nb, a.max_groups,
+                   a.out_group, a.out_sum, a.out_n_groups);
+        } else {
+            launch(ctx, LaunchInfo{"group_dense_agg", dim3(nb), dim3(256), 0, hash_mix(n_pad, 64)},
+                   k_dense_agg<64>, (const uint64_t *)col_a, (const uint64_t *)col_b, (const uint64_t *)cnt1, n_pad,
+                   part, part + (uint64_t)nb * 64);
+            launch(ctx, LaunchInfo{"group_dense_out", dim3(1), dim3(256), 0, a.max_groups}, k_dense_out<64>,
+                   (const int64_t *)part, (const int64_t *)(part + (uint64_t)nb * 64), nb, a.max_groups,
+                   a.out_group, a.out_sum, a.out_n_groups);
+        }
+        return;
+    }
     // 5. group-by sort
     launch(ctx, LaunchInfo{"group_keys", grid_for(n_pad), dim3(256), 0, n_pad}, k_group_keys, (const uint64_t *)col_a,
            (const uint64_t *)col_b, (const uint64_t *)cnt1, n_pad, keys, vals);
@@ -346,8 +566,10 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
     // 6. segmented sum
     launch(ctx, LaunchInfo{"group_seg_tiles", dim3(nt), dim3(JB), 0, n_pad}, k_seg_tiles, (const uint64_t *)keys,
            (const uint64_t *)vals, seg);
-    launch(ctx, LaunchInfo{"group_seg_carry", dim3(1), dim3(64), 0, nt}, k_seg_carry, (const Seg *)seg,
-           (const uint64_t *)keys, nt, scarry);
+    launch(ctx, LaunchInfo{"group_seg_summ", grid_for(nt), dim3(256), 0, nt}, k_seg_summ, (const Seg *)seg,
+           (const uint64_t *)keys, nt, ssum);
+    launch(ctx, LaunchInfo{"group_seg_carry", dim3(1), dim3(kScanThreads), 0, nt}, k_seg_carry, (const SegSum *)ssum,
+           nt, spre, scarry);
     launch(ctx, LaunchInfo{"group_seg_rows", dim3(nt), dim3(JB), 0, n_pad}, k_seg_rows, (const uint64_t *)keys,
            (const uint64_t *)vals, (const int64_t *)scarry, n_pad, col_a, col_b, mask);
     // 7. one row per group

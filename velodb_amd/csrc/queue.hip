@@ -52,13 +52,22 @@ struct QCtrl {
 static_assert(sizeof(vo_command) % 8 == 0, "commands are copied in 8-byte words");
 
 // Grid coordination state (device memory).
+// Each block announces its arrival at a barrier in its own 64-byte line of
+// `arrive` (no two blocks contend for one address); block 0 collects the
+// arrivals with one wavefront of parallel loads and releases the grid with
+// one store to `release`.
+constexpr uint32_t kMaxQueueBlocks = 2048;
+constexpr uint32_t kFlagStride = 16; // u32 words per flag: one 64-byte line
+
 struct QDev {
-    uint32_t bar_count;
-    uint32_t bar_gen;
+    uint32_t release;
     uint32_t abort;
     uint32_t exit;
+    uint32_t epoch; // barrier generation when the kernel last left
     uint64_t consumed;
+    uint64_t pad_[5];
     vo_command cmd;
+    uint32_t arrive[kMaxQueueBlocks * kFlagStride];
 };
 
 __device__ __forceinline__ uint64_t ld_sys(const uint64_t *p)
@@ -81,40 +90,82 @@ __device__ __forceinline__ void st_sys32(uint32_t *p, uint32_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Grid barrier with a time limit; returns false (and flags an abort) if the
-// other blocks do not arrive in time.
-__device__ bool grid_barrier(QDev *qd, uint32_t nb, uint64_t limit_ticks)
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+__device__ __forceinline__ bool timed_out(QDev *qd, uint64_t t0, uint64_t limit)
+{
+    if (__hip_atomic_load(&qd->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return true;
+    if (wall_clock64() - t0 > limit) {
+        __hip_atomic_store(&qd->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
+    return false;
+}
+
+// Grid barrier number ++epoch.  Block 0 waits until every other block has
+// arrived, runs hook() with all its threads (the hook sees every block's
+// writes), then releases the grid.  Every wait is bounded: a barrier that
+// does not complete within limit_ticks sets `abort` and returns false in
+// every block, so the grid drains.
+template <class Hook>
+__device__ bool qbarrier(QDev *qd, uint32_t nb, uint32_t &epoch, uint64_t limit_ticks, Hook hook)
 {
     __shared__ int ok;
+    const uint32_t e = ++epoch;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int good = 1;
-        __threadfence();
-        const uint32_t gen = __hip_atomic_load(&qd->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t arrived = __hip_atomic_fetch_add(&qd->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nb - 1) {
-            __hip_atomic_store(&qd->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&qd->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+    if (blockIdx.x != 0) {
+        if (threadIdx.x == 0) {
+            __threadfence();
+            __hip_atomic_store(&qd->arrive[blockIdx.x * kFlagStride], e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int good = 1;
             const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(&qd->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                if (__hip_atomic_load(&qd->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    good = 0;
-                    break;
-                }
-                if (wall_clock64() - t0 > limit_ticks) {
-                    __hip_atomic_store(&qd->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while ((int32_t)(__hip_atomic_load(&qd->release, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - e) < 0) {
+                if (timed_out(qd, t0, limit_ticks)) {
                     good = 0;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            __threadfence();
+            ok = good;
         }
-        __threadfence();
-        ok = good;
+        __syncthreads();
+        return ok != 0;
+    }
+    if (threadIdx.x < 64) {
+        int good = 1;
+        const uint64_t t0 = wall_clock64();
+        for (;;) {
+            int all = 1;
+            for (uint32_t blk = 1 + threadIdx.x; blk < nb; blk += 64)
+                all &= (int32_t)(__hip_atomic_load(&qd->arrive[blk * kFlagStride], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT) - e) >= 0;
+            if (__all(all)) break;
+            if (timed_out(qd, t0, limit_ticks)) { // uniform: scalar clock, same flag for every lane
+                good = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (threadIdx.x == 0) ok = good;
     }
     __syncthreads();
-    return ok != 0;
+    if (!ok) return false;
+    hook();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_store(&qd->release, e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool qbarrier(QDev *qd, uint32_t nb, uint32_t &epoch, uint64_t limit_ticks)
+{
+    return qbarrier(qd, nb, epoch, limit_ticks, NoHook{});
 }
 
 // ------------------------------------------------------------ op bodies
@@ -134,7 +185,8 @@ __device__ __forceinline__ void sort_global_items(const dev::Bufs<RecKP> &b, con
 
 // SORT_KP: the launch-mode pass schedule (plan_passes with register passes
 // of at most 4 stages and hybrid passes of 6-7), one grid barrier per pass.
-__device__ bool exec_sort_kp(const vo_sort_kp_args &a, char *lds, QDev *qd, uint32_t nb, uint64_t bar)
+__device__ bool exec_sort_kp(const vo_sort_kp_args &a, char *lds, QDev *qd, uint32_t nb, uint32_t &epoch,
+                             uint64_t bar)
 {
     dev::Bufs<RecKP> b;
     b.k[0] = a.keys;
@@ -144,17 +196,17 @@ __device__ bool exec_sort_kp(const vo_sort_kp_args &a, char *lds, QDev *qd, uint
     while ((1ull << n_log) < a.n_pad) ++n_log;
     if (n_log < QR_LOG) {
         dev::body_tiny<RecKP, false>(b, n_log, a.n, 1, nullptr, blockIdx.x);
-        return grid_barrier(qd, nb, bar);
+        return qbarrier(qd, nb, epoch, bar);
     }
     if (n_log < QT_LOG) {
         if (blockIdx.x == 0)
             dev::body_local<RecKP, QT_LOG, QR_LOG, QBLOCK, false>(b, map, n_log, n_log, n_log, a.n, 1, nullptr, lds, 0);
-        return grid_barrier(qd, nb, bar);
+        return qbarrier(qd, nb, epoch, bar);
     }
     const uint32_t tiles = (uint32_t)(a.n_pad >> QT_LOG);
     for (uint32_t w = blockIdx.x; w < tiles; w += nb)
         dev::body_local_ct<RecKP, QT_LOG, QR_LOG, QBLOCK, false>(b, map, n_log, a.n, 1, nullptr, lds, w);
-    if (!grid_barrier(qd, nb, bar)) return false;
+    if (!qbarrier(qd, nb, epoch, bar)) return false;
     for (int p = QT_LOG + 1; p <= n_log; ++p) {
         const int count = p - QT_LOG;
         const int groups = plan_groups(count, kQueueSMax, kQueueHMax);
@@ -177,11 +229,11 @@ __device__ bool exec_sort_kp(const vo_sort_kp_args &a, char *lds, QDev *qd, uint
                     dev::body_hybrid<RecKP, QT_LOG, QR_LOG, QBLOCK, 7, false>(b, map, p, q_lo, n_log, nullptr, lds, w);
                 break;
             }
-            if (!grid_barrier(qd, nb, bar)) return false;
+            if (!qbarrier(qd, nb, epoch, bar)) return false;
         }
         for (uint32_t w = blockIdx.x; w < tiles; w += nb)
             dev::body_merge<RecKP, QT_LOG, QR_LOG, QBLOCK, false>(b, map, p, n_log, nullptr, lds, w);
-        if (!grid_barrier(qd, nb, bar)) return false;
+        if (!qbarrier(qd, nb, epoch, bar)) return false;
     }
     return true;
 }
@@ -265,6 +317,48 @@ __device__ void exec_elementwise(const vo_command &c, uint32_t nb)
     }
 }
 
+// Block 0's work at the barrier that ends one command and starts the next:
+// publish completion of the command every block just finished, then wait
+// (bounded by the idle time) for the host to publish the next one and copy it
+// into device memory with one wavefront (one PCIe round trip per 64 words).
+struct FetchHook {
+    const vo_command *ring;
+    QCtrl *ctrl;
+    QDev *qd;
+    uint32_t cap;
+    uint64_t consumed; // commands completed so far
+    bool publish;      // a command finished since the last fetch
+    uint64_t idle_ticks;
+
+    __device__ void operator()() const
+    {
+        if (threadIdx.x >= 64) return;
+        const uint32_t lane = threadIdx.x;
+        if (publish && lane == 0) st_sys(&ctrl->done, consumed);
+        const uint64_t t0 = wall_clock64();
+        uint32_t ex = 0;
+        for (;;) {
+            if (ld_sys(&ctrl->head) > consumed) break; // same address in every lane: uniform
+            if (ld_sys32(&ctrl->stop)) {
+                ex = 1;
+                break;
+            }
+            if (wall_clock64() - t0 > idle_ticks) {
+                ex = 2;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (ex == 0) {
+            const uint64_t *src = reinterpret_cast<const uint64_t *>(&ring[consumed % cap]);
+            uint64_t *dst = reinterpret_cast<uint64_t *>(&qd->cmd);
+            for (uint32_t k = lane; k < sizeof(vo_command) / 8; k += 64)
+                dst[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        if (lane == 0) qd->exit = ex;
+    }
+};
+
 __global__ __launch_bounds__(QBLOCK) void k_queue(const vo_command *ring, QCtrl *ctrl, QDev *qd, uint32_t cap,
                                                    uint64_t idle_ticks, uint64_t bar_ticks)
 {
@@ -273,34 +367,13 @@ __global__ __launch_bounds__(QBLOCK) void k_queue(const vo_command *ring, QCtrl 
     const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
     if (leader) st_sys32(&ctrl->alive, 1);
     uint64_t consumed = qd->consumed;
+    uint32_t epoch = qd->epoch;
     uint32_t reason = 0;
+    bool publish = false;
     for (;;) {
-        if (leader) {
-            const uint64_t t0 = wall_clock64();
-            uint32_t ex = 0;
-            for (;;) {
-                if (ld_sys(&ctrl->head) > consumed) {
-                    // copy the command into device memory for every block
-                    const uint64_t *src = reinterpret_cast<const uint64_t *>(&ring[consumed % cap]);
-                    uint64_t *dst = reinterpret_cast<uint64_t *>(&qd->cmd);
-                    for (uint32_t k = 0; k < sizeof(vo_command) / 8; ++k)
-                        dst[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                    break;
-                }
-                if (ld_sys32(&ctrl->stop)) {
-                    ex = 1;
-                    break;
-                }
-                if (wall_clock64() - t0 > idle_ticks) {
-                    ex = 2;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-            qd->exit = ex;
-        }
-        // the other blocks wait here while the leader polls: allow the idle time
-        if (!grid_barrier(qd, nb, idle_ticks + bar_ticks)) {
+        // the other blocks wait here while block 0 polls the ring: allow the idle time
+        const FetchHook hook{ring, ctrl, qd, cap, consumed, publish, idle_ticks};
+        if (!qbarrier(qd, nb, epoch, idle_ticks + bar_ticks, hook)) {
             reason = 4;
             break;
         }
@@ -314,29 +387,30 @@ __global__ __launch_bounds__(QBLOCK) void k_queue(const vo_command *ring, QCtrl 
         if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
         __syncthreads();
         const vo_command cmd = qd->cmd;
-        bool ok = true;
-        if (cmd.opcode == VO_OP_SORT_KP)
-            ok = exec_sort_kp(cmd.args.sort_kp, lds, qd, nb, bar_ticks);
-        else
-            exec_elementwise(cmd, nb);
-        // results out of every XCD's L2 before the host is told
-        __syncthreads();
-        if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-        if (!ok || !grid_barrier(qd, nb, bar_ticks)) {
-            reason = 4;
-            break;
-        }
-        ++consumed;
-        if (leader) {
-            qd->consumed = consumed;
-            st_sys(&ctrl->done, consumed);
-        }
         if (cmd.opcode == VO_OP_TERMINATE) {
+            ++consumed;
+            if (leader) st_sys(&ctrl->done, consumed);
             reason = 3;
             break;
         }
+        bool ok = true;
+        if (cmd.opcode == VO_OP_SORT_KP)
+            ok = exec_sort_kp(cmd.args.sort_kp, lds, qd, nb, epoch, bar_ticks);
+        else
+            exec_elementwise(cmd, nb);
+        if (!ok) {
+            reason = 4;
+            break;
+        }
+        // results out of every XCD's L2 before block 0 tells the host
+        __syncthreads();
+        if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        ++consumed;
+        publish = true;
     }
     if (leader) {
+        qd->consumed = consumed;
+        qd->epoch = epoch;
         if (reason == 4) st_sys32(&ctrl->error, 1);
         st_sys32(&ctrl->exit_reason, reason);
         st_sys32(&ctrl->alive, 0);
@@ -357,8 +431,6 @@ struct QueueState {
     uint64_t idle_ticks = 0, bar_ticks = 0;
 };
 
-constexpr double kIdleSeconds = 20.0;
-constexpr double kShortIdleSeconds = 0.05;
 constexpr double kBarrierSeconds = 10.0;
 
 volatile QCtrl *vctrl(QueueState *q) { return q->ctrl; }
@@ -537,6 +609,35 @@ int queue_poll(vo_ctx *ctx, uint64_t seq)
     }
 }
 
+bool queue_supports(uint32_t opcode) { return supported(opcode); }
+
+int queue_park(vo_ctx *ctx)
+{
+    QueueState *q = (QueueState *)ctx->queue;
+    try {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        reap_error(ctx, q);
+        // the kernel takes every published command before it looks at `stop`
+        vctrl(q)->stop = 1;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (;;) {
+            const hipError_t e = hipStreamQuery(q->stream);
+            if (e == hipSuccess) break;
+            if (e != hipErrorNotReady) throw Error{VO_ERR_HIP, std::string("queue kernel: ") + hipGetErrorString(e)};
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(ctx->wait_timeout_ms))
+                throw Error{VO_ERR_TIMEOUT, "queue kernel did not leave"};
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+        vctrl(q)->stop = 0;
+        reap_error(ctx, q);
+        VO_CHECK(__atomic_load_n(&q->ctrl->done, __ATOMIC_ACQUIRE) == q->submitted, VO_ERR_QUEUE,
+                 "queue kernel left with commands outstanding");
+        return VO_OK;
+    } catch (const Error &e) {
+        return fail_q(ctx, e);
+    }
+}
+
 int queue_sync(vo_ctx *ctx)
 {
     QueueState *q = (QueueState *)ctx->queue;
@@ -571,11 +672,11 @@ int vo_queue_start(vo_ctx *ctx)
         int per_cu = 0;
         VO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_queue, QBLOCK, q->lds));
         VO_CHECK(per_cu >= 1, VO_ERR_UNSUPPORTED, "queue kernel does not fit on a CU");
-        q->grid = (uint32_t)per_cu * (uint32_t)ctx->num_cus;
+        q->grid = std::min<uint32_t>((uint32_t)per_cu * (uint32_t)ctx->num_cus, kMaxQueueBlocks);
         int rate_khz = 0;
         VO_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
         const double hz = rate_khz > 0 ? rate_khz * 1e3 : 1e8;
-        q->idle_ticks = (uint64_t)(((ctx->flags & VO_CFG_QUEUE_SHORT_IDLE) ? kShortIdleSeconds : kIdleSeconds) * hz);
+        q->idle_ticks = (uint64_t)(ctx->queue_idle_us * 1e-6 * hz);
         q->bar_ticks = (uint64_t)(kBarrierSeconds * hz);
         VO_HIP(hipStreamSynchronize(ctx->stream)); // earlier launch-mode work first
         launch_queue_kernel(ctx, q);

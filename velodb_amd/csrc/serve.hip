@@ -27,23 +27,33 @@ struct vo_server;
 namespace vo {
 namespace {
 
+// Every table also keeps a pinned host copy: the transfer-mode run ships it
+// to the device per query, as the reference's engine does.
 struct FactTable {
     uint64_t n, n_pad;
     int32_t *day;
     int64_t *qty, *price, *disc;
+    int32_t *h_day;
+    int64_t *h_qty, *h_price, *h_disc;
 };
 
 struct JoinTable {
     uint64_t n_dim, n_fact, n_pad;
     int32_t *dk, *dg, *fk;
     int64_t *fv;
+    int32_t *h_dk, *h_dg, *h_fk;
+    int64_t *h_fv;
 };
 
 struct SortTable {
     uint64_t n, n_pad;
     uint64_t *keys;
     uint32_t *pay;
+    uint64_t *h_keys;
+    uint32_t *h_pay;
 };
+
+uint64_t align256(uint64_t b) { return (b + 255) & ~255ull; }
 
 uint64_t pow2_at_least(uint64_t n, uint64_t floor_)
 {
@@ -102,14 +112,16 @@ __global__ __launch_bounds__(256) void k_sort_checksum(const uint64_t *__restric
 
 constexpr uint32_t kChecksumBlocks = 256;
 constexpr uint64_t kMaxGroups = 64;
+constexpr uint64_t kJoinGroups = 32; // public bound of the generated group attribute (0..24)
 
 // Query intermediates that live outside the slot context's scratch arena.
-uint64_t query_buffer_bytes(uint32_t kind, uint64_t n_pad)
+// In transfer mode the buffer also holds the query's input columns.
+uint64_t query_buffer_bytes(uint32_t kind, uint64_t n_pad, bool transfer)
 {
     switch (kind) {
-    case VO_Q_FILTER_SUM: return 16 * n_pad + 64;
+    case VO_Q_FILTER_SUM: return (transfer ? 28 : 16) * n_pad + 64;
     case VO_Q_SORT: return 12 * n_pad + 2 * kChecksumBlocks * 8;
-    default: return kMaxGroups * 12 + 64;
+    default: return kMaxGroups * 12 + 64 + (transfer ? 4 * 256 + 20 * n_pad : 0);
     }
 }
 
@@ -140,6 +152,9 @@ struct vo_server {
     std::vector<JoinTable> joins;
     std::vector<SortTable> sorts;
     std::vector<void *> allocations;
+    std::vector<void *> host_allocations;
+    std::vector<void *> slot_hbuf; // pinned host staging per slot (transfer mode)
+    std::vector<uint64_t> slot_hbuf_bytes;
 };
 
 namespace {
@@ -173,6 +188,18 @@ void *dalloc(vo_server *s, size_t bytes)
     return p;
 }
 
+void *halloc(vo_server *s, size_t bytes)
+{
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, std::max<size_t>(bytes, 256), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{VO_ERR_NOMEM, "server host table allocation failed"};
+    }
+    s->host_allocations.push_back(p);
+    return p;
+}
+
 vo_ctx *slot_ctx(vo_server *s, uint32_t i)
 {
     while (s->slots.size() <= i) {
@@ -184,6 +211,8 @@ vo_ctx *slot_ctx(vo_server *s, uint32_t i)
         s->slots.push_back(ctx);
         s->slot_buf.push_back(nullptr);
         s->slot_buf_bytes.push_back(0);
+        s->slot_hbuf.push_back(nullptr);
+        s->slot_hbuf_bytes.push_back(0);
     }
     return s->slots[i];
 }
@@ -204,20 +233,42 @@ uint64_t table_n_pad(vo_server *s, const vo_query &q)
     }
 }
 
-// Runs one query on a slot and fills its result words (blocking).
-void run_query(vo_server *s, vo_ctx *ctx, void *buf, const vo_query &q, uint64_t r[4])
+void h2d(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    if (bytes) VO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+}
+
+// Runs one query on a slot and fills its result words (blocking).  With
+// `transfer`, the input columns are first copied from the table's pinned host
+// copy into the slot buffer and the query runs on those copies; a sort also
+// copies its sorted columns back to host memory (hbuf).
+void run_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_query &q, bool transfer, uint64_t r[4])
 {
     char *b = (char *)buf;
     switch (q.kind) {
     case VO_Q_FILTER_SUM: {
         const FactTable &t = s->facts[q.table];
         vo_filter_args a{};
-        a.day = t.day;
-        a.qty = t.qty;
-        a.price = t.price;
-        a.disc = t.disc;
-        a.out_price = (int64_t *)b;
-        a.out_disc = (int64_t *)(b + 8 * t.n_pad);
+        if (transfer) {
+            // private copies: the network compacts price/disc in place
+            a.day = (int32_t *)b;
+            a.qty = (int64_t *)(b + 4 * t.n_pad);
+            a.price = (int64_t *)(b + 12 * t.n_pad);
+            a.disc = (int64_t *)(b + 20 * t.n_pad);
+            h2d(ctx, (void *)a.day, t.h_day, 4 * t.n);
+            h2d(ctx, (void *)a.qty, t.h_qty, 8 * t.n);
+            h2d(ctx, (void *)a.price, t.h_price, 8 * t.n);
+            h2d(ctx, (void *)a.disc, t.h_disc, 8 * t.n);
+            a.out = (uint64_t *)(b + 28 * t.n_pad);
+        } else {
+            a.day = t.day;
+            a.qty = t.qty;
+            a.price = t.price;
+            a.disc = t.disc;
+            a.out_price = (int64_t *)b;
+            a.out_disc = (int64_t *)(b + 8 * t.n_pad);
+            a.out = (uint64_t *)(b + 16 * t.n_pad);
+        }
         a.n = t.n;
         a.n_pad = t.n_pad;
         a.day_lo = 8766;
@@ -225,7 +276,6 @@ void run_query(vo_server *s, vo_ctx *ctx, void *buf, const vo_query &q, uint64_t
         a.disc_lo = 5;
         a.disc_hi = 7;
         a.qty_lt = 24;
-        a.out = (uint64_t *)(b + 16 * t.n_pad);
         op_filter_sum(ctx, a);
         uint64_t h[4];
         VO_HIP(hipMemcpyAsync(h, a.out, 32, hipMemcpyDeviceToHost, ctx->stream));
@@ -239,20 +289,38 @@ void run_query(vo_server *s, vo_ctx *ctx, void *buf, const vo_query &q, uint64_t
     case VO_Q_JOIN_GROUPSUM: {
         const JoinTable &t = s->joins[q.table];
         vo_join_args a{};
-        a.dim_key = t.dk;
-        a.dim_group = t.dg;
+        char *o = b;
+        if (transfer) {
+            char *in = b + align256(kMaxGroups * 12 + 64);
+            a.dim_key = (const int32_t *)in;
+            in += align256(4 * t.n_dim);
+            a.dim_group = (const int32_t *)in;
+            in += align256(4 * t.n_dim);
+            a.fact_fk = (const int32_t *)in;
+            in += align256(4 * t.n_fact);
+            a.fact_val = (const int64_t *)in;
+            h2d(ctx, (void *)a.dim_key, t.h_dk, 4 * t.n_dim);
+            h2d(ctx, (void *)a.dim_group, t.h_dg, 4 * t.n_dim);
+            h2d(ctx, (void *)a.fact_fk, t.h_fk, 4 * t.n_fact);
+            h2d(ctx, (void *)a.fact_val, t.h_fv, 8 * t.n_fact);
+        } else {
+            a.dim_key = t.dk;
+            a.dim_group = t.dg;
+            a.fact_fk = t.fk;
+            a.fact_val = t.fv;
+        }
         a.n_dim = t.n_dim;
-        a.fact_fk = t.fk;
-        a.fact_val = t.fv;
         a.n_fact = t.n_fact;
         a.n_pad = t.n_pad;
-        a.out_group = (int32_t *)b;
-        a.out_sum = (int64_t *)(b + kMaxGroups * 4);
-        a.out_n_groups = (uint64_t *)(b + kMaxGroups * 12);
-        a.max_groups = kMaxGroups;
+        a.out_group = (int32_t *)o;
+        a.out_sum = (int64_t *)(o + kMaxGroups * 4);
+        a.out_n_groups = (uint64_t *)(o + kMaxGroups * 12);
+        // the tables' group attribute lies in [0, 25): dense GROUP BY over 32 ids
+        a.max_groups = kJoinGroups;
+        a.dense_groups = 1;
         op_join_groupsum(ctx, a);
         std::vector<char> h(kMaxGroups * 12 + 8);
-        VO_HIP(hipMemcpyAsync(h.data(), b, h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        VO_HIP(hipMemcpyAsync(h.data(), o, h.size(), hipMemcpyDeviceToHost, ctx->stream));
         VO_HIP(hipStreamSynchronize(ctx->stream));
         const int32_t *g = (const int32_t *)h.data();
         const int64_t *sm = (const int64_t *)(h.data() + kMaxGroups * 4);
@@ -273,12 +341,22 @@ void run_query(vo_server *s, vo_ctx *ctx, void *buf, const vo_query &q, uint64_t
         uint64_t *k = (uint64_t *)b;
         uint32_t *p = (uint32_t *)(b + 8 * t.n_pad);
         uint64_t *part = (uint64_t *)(b + 12 * t.n_pad);
-        // the resident table stays unsorted: the query sorts its own copy
-        VO_HIP(hipMemcpyAsync(k, t.keys, 8 * t.n_pad, hipMemcpyDeviceToDevice, ctx->stream));
-        VO_HIP(hipMemcpyAsync(p, t.pay, 4 * t.n_pad, hipMemcpyDeviceToDevice, ctx->stream));
+        if (transfer) {
+            h2d(ctx, k, t.h_keys, 8 * t.n);
+            h2d(ctx, p, t.h_pay, 4 * t.n);
+        } else {
+            // the resident table stays unsorted: the query sorts its own copy
+            VO_HIP(hipMemcpyAsync(k, t.keys, 8 * t.n_pad, hipMemcpyDeviceToDevice, ctx->stream));
+            VO_HIP(hipMemcpyAsync(p, t.pay, 4 * t.n_pad, hipMemcpyDeviceToDevice, ctx->stream));
+        }
         op_sort_kp(ctx, k, p, t.n, t.n_pad);
         launch(ctx, LaunchInfo{"sort_checksum", dim3(kChecksumBlocks), dim3(256), 0, t.n_pad}, k_sort_checksum,
                (const uint64_t *)k, (const uint32_t *)p, t.n_pad, part);
+        if (transfer) { // the sorted relation goes back to the host
+            char *hb = (char *)hbuf;
+            VO_HIP(hipMemcpyAsync(hb, k, 8 * t.n, hipMemcpyDeviceToHost, ctx->stream));
+            VO_HIP(hipMemcpyAsync(hb + 8 * t.n, p, 4 * t.n, hipMemcpyDeviceToHost, ctx->stream));
+        }
         std::vector<uint64_t> h(2 * kChecksumBlocks);
         VO_HIP(hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
         VO_HIP(hipStreamSynchronize(ctx->stream));
@@ -328,6 +406,9 @@ int vo_server_destroy(vo_server *s)
     for (void *p : s->slot_buf)
         if (p) (void)hipFree(p);
     for (void *p : s->allocations) (void)hipFree(p);
+    for (void *p : s->slot_hbuf)
+        if (p) (void)hipHostFree(p);
+    for (void *p : s->host_allocations) (void)hipHostFree(p);
     delete s;
     return VO_OK;
 }
@@ -347,6 +428,14 @@ int vo_server_add_fact_table(vo_server *s, uint64_t seed, uint64_t n_rows, uint3
         t.disc = (int64_t *)dalloc(s, 8 * t.n_pad);
         vo_ctx *ctx = slot_ctx(s, 0);
         op_gen_fact(ctx, seed, t.day, t.qty, t.price, t.disc, t.n);
+        t.h_day = (int32_t *)halloc(s, 4 * t.n);
+        t.h_qty = (int64_t *)halloc(s, 8 * t.n);
+        t.h_price = (int64_t *)halloc(s, 8 * t.n);
+        t.h_disc = (int64_t *)halloc(s, 8 * t.n);
+        VO_HIP(hipMemcpyAsync(t.h_day, t.day, 4 * t.n, hipMemcpyDeviceToHost, ctx->stream));
+        VO_HIP(hipMemcpyAsync(t.h_qty, t.qty, 8 * t.n, hipMemcpyDeviceToHost, ctx->stream));
+        VO_HIP(hipMemcpyAsync(t.h_price, t.price, 8 * t.n, hipMemcpyDeviceToHost, ctx->stream));
+        VO_HIP(hipMemcpyAsync(t.h_disc, t.disc, 8 * t.n, hipMemcpyDeviceToHost, ctx->stream));
         VO_HIP(hipStreamSynchronize(ctx->stream));
         s->facts.push_back(t);
         if (id) *id = (uint32_t)(s->facts.size() - 1);
@@ -376,6 +465,14 @@ int vo_server_add_join_tables(vo_server *s, uint64_t seed, uint64_t n_dim, uint6
         const dim3 g2((uint32_t)std::min<uint64_t>(65536, (std::max(n_dim, n_fact) + 255) / 256));
         launch(ctx, LaunchInfo{"server_gen_join", g2, dim3(256), 0, n_dim}, k_gen_join_cols, seed,
                (const uint32_t *)perm, n_dim, n_fact, t.dk, t.dg, t.fk, t.fv);
+        t.h_dk = (int32_t *)halloc(s, 4 * n_dim);
+        t.h_dg = (int32_t *)halloc(s, 4 * n_dim);
+        t.h_fk = (int32_t *)halloc(s, 4 * n_fact);
+        t.h_fv = (int64_t *)halloc(s, 8 * n_fact);
+        VO_HIP(hipMemcpyAsync(t.h_dk, t.dk, 4 * n_dim, hipMemcpyDeviceToHost, ctx->stream));
+        VO_HIP(hipMemcpyAsync(t.h_dg, t.dg, 4 * n_dim, hipMemcpyDeviceToHost, ctx->stream));
+        VO_HIP(hipMemcpyAsync(t.h_fk, t.fk, 4 * n_fact, hipMemcpyDeviceToHost, ctx->stream));
+        VO_HIP(hipMemcpyAsync(t.h_fv, t.fv, 8 * n_fact, hipMemcpyDeviceToHost, ctx->stream));
         VO_HIP(hipStreamSynchronize(ctx->stream));
         s->joins.push_back(t);
         if (id) *id = (uint32_t)(s->joins.size() - 1);
@@ -393,6 +490,10 @@ int vo_server_add_sort_table(vo_server *s, uint64_t seed, uint64_t n_rows, uint3
         t.pay = (uint32_t *)dalloc(s, 4 * t.n_pad);
         vo_ctx *ctx = slot_ctx(s, 0);
         op_gen_sort_input(ctx, seed, t.keys, t.pay, t.n);
+        t.h_keys = (uint64_t *)halloc(s, 8 * t.n);
+        t.h_pay = (uint32_t *)halloc(s, 4 * t.n);
+        VO_HIP(hipMemcpyAsync(t.h_keys, t.keys, 8 * t.n, hipMemcpyDeviceToHost, ctx->stream));
+        VO_HIP(hipMemcpyAsync(t.h_pay, t.pay, 4 * t.n, hipMemcpyDeviceToHost, ctx->stream));
         VO_HIP(hipStreamSynchronize(ctx->stream));
         s->sorts.push_back(t);
         if (id) *id = (uint32_t)(s->sorts.size() - 1);
@@ -402,18 +503,28 @@ int vo_server_add_sort_table(vo_server *s, uint64_t seed, uint64_t n_rows, uint3
 int vo_server_run(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_slots_override, vo_query_result *out,
                   vo_run_stats *stats)
 {
+    return vo_server_run_mode(s, q, n, n_slots_override, VO_RUN_RESIDENT, out, stats);
+}
+
+int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_slots_override, uint32_t mode,
+                       vo_query_result *out, vo_run_stats *stats)
+{
     return sguard(s, [&] {
         VO_CHECK(q && out, VO_ERR_INVALID, "null query list");
+        VO_CHECK(mode == VO_RUN_RESIDENT || mode == VO_RUN_TRANSFER, VO_ERR_INVALID, "unknown run mode");
+        const bool transfer = mode == VO_RUN_TRANSFER;
         const uint32_t slots = n_slots_override ? n_slots_override : s->n_slots;
         // size every slot for the largest query it may take (no allocation while timing)
         std::vector<uint64_t> fp(n);
-        uint64_t max_buf = 64, max_scr = 1 << 20;
+        uint64_t max_buf = 64, max_scr = 1 << 20, max_hbuf = 0;
         for (uint64_t i = 0; i < n; ++i) {
             const uint64_t np = table_n_pad(s, q[i]);
-            const uint64_t b = vo::query_buffer_bytes(q[i].kind, np), c = vo::query_scratch_bytes(q[i].kind, np);
+            const uint64_t b = vo::query_buffer_bytes(q[i].kind, np, transfer);
+            const uint64_t c = vo::query_scratch_bytes(q[i].kind, np);
             fp[i] = b + c;
             max_buf = std::max(max_buf, b);
             max_scr = std::max(max_scr, c);
+            if (transfer && q[i].kind == VO_Q_SORT) max_hbuf = std::max(max_hbuf, 12 * s->sorts[q[i].table].n);
         }
         for (uint32_t i = 0; i < slots; ++i) {
             vo_ctx *ctx = slot_ctx(s, i);
@@ -424,6 +535,12 @@ int vo_server_run(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_slots_
                 s->slot_buf_bytes[i] = max_buf;
             }
             scratch(ctx, max_scr);
+            if (s->slot_hbuf_bytes[i] < max_hbuf) {
+                if (s->slot_hbuf[i]) VO_HIP(hipHostFree(s->slot_hbuf[i]));
+                s->slot_hbuf[i] = nullptr;
+                VO_HIP(hipHostMalloc(&s->slot_hbuf[i], max_hbuf, hipHostMallocDefault));
+                s->slot_hbuf_bytes[i] = max_hbuf;
+            }
         }
         VO_HIP(hipDeviceSynchronize());
 
@@ -476,7 +593,7 @@ int vo_server_run(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_slots_
                 r.footprint_bytes = fp[i];
                 r.slot = slot;
                 try {
-                    run_query(s, ctx, s->slot_buf[slot], q[i], r.r);
+                    run_query(s, ctx, s->slot_buf[slot], s->slot_hbuf[slot], q[i], transfer, r.r);
                     r.status = VO_OK;
                 } catch (const Error &e) {
                     r.status = e.status;

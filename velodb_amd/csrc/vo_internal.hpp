@@ -79,6 +79,7 @@ struct vo_ctx {
     uint32_t flags = 0;
     uint32_t queue_capacity = 1024;
     uint32_t wait_timeout_ms = 60000;
+    uint32_t queue_idle_us = 20000;
     int num_cus = 0;
     std::string last_error;
     int last_status = 0;
@@ -201,6 +202,8 @@ uint64_t queue_submit(vo_ctx *ctx, vo_command *cmd);
 int queue_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms);
 int queue_poll(vo_ctx *ctx, uint64_t seq);
 int queue_sync(vo_ctx *ctx);
+bool queue_supports(uint32_t opcode); // runs in the resident kernel
+int queue_park(vo_ctx *ctx);          // finish submitted commands, let the resident kernel leave
 
 // ---- shared device helpers (sort.hip) used by other ops
 // Sort (u64 key, u64 value) records by (key, value) ascending, n_pad = 2^k.
