@@ -351,7 +351,7 @@ def bench_serve(args):
 
     import velodb_amd
     S = velodb_amd.Server
-    srv = S(n_slots=16, hbm_budget_bytes=200 << 30)
+    srv = S(n_slots=args.slots, hbm_budget_bytes=200 << 30)
     rng = np.random.default_rng(0)
     tables = {"f": [], "j": [], "s": []}
     for i in range(12):
@@ -370,13 +370,22 @@ def bench_serve(args):
             qs.append((kind, int(r.choice(tables[key])), t))
         return qs
 
-    def summ(st):
-        return {"qps": round(st.qps, 1), "p50_ms": round(st.p50_s * 1e3, 2), "p99_ms": round(st.p99_s * 1e3, 2),
-                "max_concurrent": st.max_concurrent, "peak_GB": round(st.peak_bytes_in_flight / 1e9, 2)}
+    names = {S.FILTER_SUM: "filter_sum", S.JOIN_GROUPSUM: "join_groupsum", S.SORT: "sort"}
+
+    def summ(st, res=None, qs=None):
+        d = {"qps": round(st.qps, 1), "p50_ms": round(st.p50_s * 1e3, 2), "p99_ms": round(st.p99_s * 1e3, 2),
+             "max_concurrent": st.max_concurrent, "peak_GB": round(st.peak_bytes_in_flight / 1e9, 2)}
+        if res is not None:
+            # mean time from admission to completion, per query kind
+            per = {}
+            for r, q in zip(res, qs):
+                per.setdefault(names[q[0]], []).append(r.end_s - r.admit_s)
+            d["service_ms_by_kind"] = {k: round(1e3 * sum(v) / len(v), 3) for k, v in per.items()}
+        return d
 
     srv.run(mix(64, 0, 99))  # warm-up
     srv.run(mix(16, 0, 98), n_slots=1, transfer=True)
-    for rate in [0, 50, 200, 800]:
+    for rate in [int(x) for x in args.rates.split(",")]:
         count = 600 if rate == 0 else max(200, rate * 2)
         qs = mix(count, rate, 7 + rate)
         # plain: one query at a time, inputs shipped host->device per query
@@ -386,12 +395,13 @@ def bench_serve(args):
         rr, sr = srv.run(qs, n_slots=1)
         rb, sb = srv.run(qs)
         same = [list(a.r) for a in rp] == [list(b.r) for b in rb] == [list(c.r) for c in rr]
-        _line("served queries/s (continuous batching, 16 slots)", sb.qps, "QPS", count, 64, 1e3 / max(sb.qps, 1e-9),
+        _line(f"served queries/s (continuous batching, {args.slots} slots)", sb.qps, "QPS", count, 64,
+              1e3 / max(sb.qps, 1e-9),
               "int64/int128",
               {"workload": "served mix 50% filter+SUM / 30% join+group-by / 20% sort, tables log-uniform "
                            "in [1e4, 1e7] rows, 200 GB admission budget",
                "offered_qps": rate if rate else "all at t=0", "queries": count},
-              served=summ(sb), plain=summ(sp), plain_resident=summ(sr),
+              served=summ(sb, rb, qs), plain=summ(sp, rp, qs), plain_resident=summ(sr, rr, qs),
               throughput_ratio=round(sb.qps / max(sp.qps, 1e-9), 2),
               throughput_ratio_vs_plain_resident=round(sb.qps / max(sr.qps, 1e-9), 2),
               results_equal=same)
@@ -409,6 +419,9 @@ def main():
     ap.add_argument("--workload", choices=["sort", "filter", "join", "batch", "serve", "queue"], default="sort",
                     help="sort = the headline line; the others print their own line")
     ap.add_argument("--autotune", action="store_true", help="batch: run the autotuner first")
+    ap.add_argument("--slots", type=int, default=16, help="serve: concurrent query slots")
+    ap.add_argument("--rates", type=str, default="0,50,200,800",
+                    help="serve: offered loads in QPS (0 = every query arrives at once)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run the plain 3 GiB HBM copy kernel instead (traffic-counter calibration)")
     args = ap.parse_args()

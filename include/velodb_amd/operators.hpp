@@ -1,0 +1,552 @@
+/*
+ * velodb_amd/operators.hpp -- C++ host side of the oblivious operator path,
+ * header-only, over the C ABI of vo_api.h.
+ *
+ * It mirrors the reference's operator surface for this path so a caller of
+ * VelODB's operators finds the same names, argument meaning and error
+ * behaviour:
+ *   TaskManager             include/cuda/task_manager.hpp:12-30
+ *   ExecutionContext        include/catalog/execution_context.hpp:24 (getTaskManager)
+ *   Result<T>               include/common/result.hpp (success / failure / value / error)
+ *   AbstractOperator::next  include/operator/abstract_operator.hpp:18-36
+ *   SortOperator            include/operator/sort_operator.hpp:15-19, src/operator/sort_operator.cpp:21-134
+ *   FilterCompactionOperator include/operator/filter_compaction_operator.hpp:14-17,
+ *                           src/operator/filter_compaction_operator.cpp:65-184
+ * and adds PKFKJoinGroupSumOperator for the oblivious join + GROUP BY.
+ *
+ * Columns live in device memory with a power-of-two capacity (at least 4096
+ * rows), as the reference's device ValueVector does (value_vector.hpp:43,
+ * 248-272), so the sort and the compaction may use the pad rows in place.
+ * A column may carry a null bitmap of one byte per row (non-zero = valid),
+ * moved with its rows.
+ *
+ * Differences from the reference, by design of the oblivious path:
+ *   - FilterCompactionOperator moves the selected rows with the ORCompact
+ *     routing network (every row read and written regardless of the mask)
+ *     instead of scatter + gather, and takes the mask column's index instead
+ *     of looking up "$_mask" by name (there is no schema here);
+ *   - SortOperator returns the whole sorted batch from its first next()
+ *     rather than slices of MAX_BATCH_SIZE;
+ *   - failures come back as Result::failure with the library's message;
+ *     TaskManager methods throw ExecutionError, as CHECKED_CALL_THROW does
+ *     (include/cuda/helper.hpp:42-48).
+ */
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <variant>
+#include <vector>
+
+#include "vo_api.h"
+
+namespace velodb_amd {
+
+class ExecutionError : public std::runtime_error {
+public:
+    explicit ExecutionError(const std::string &m) : std::runtime_error(m) {}
+};
+
+inline void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw ExecutionError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ----------------------------------------------------------------- Result
+
+template <typename T, typename E = std::string>
+class Result {
+public:
+    static Result success(T &&v) { return Result(std::in_place_index<0>, std::move(v)); }
+    static Result failure(E e) { return Result(std::in_place_index<1>, std::move(e)); }
+    explicit operator bool() const noexcept { return v_.index() == 0; }
+    T &value() & { return std::get<0>(v_); }
+    T &&value() && { return std::get<0>(std::move(v_)); }
+    const E &error() const & { return std::get<1>(v_); }
+
+private:
+    template <size_t I, class A>
+    Result(std::in_place_index_t<I> i, A &&a) : v_(i, std::forward<A>(a))
+    {
+    }
+    std::variant<T, E> v_;
+};
+
+// ------------------------------------------------------------ TaskManager
+
+class TaskManager {
+public:
+    explicit TaskManager(size_t queue_capacity = 1024, bool persistent = false, hipStream_t stream = nullptr)
+    {
+        vo_config cfg{};
+        cfg.queue_capacity = static_cast<uint32_t>(queue_capacity);
+        cfg.flags = persistent ? VO_CFG_PERSISTENT : 0;
+        cfg.stream = stream;
+        cfg.device = -1;
+        const int st = vo_ctx_create(&cfg, &ctx_);
+        if (st != VO_OK) throw ExecutionError(std::string("vo_ctx_create: ") + vo_status_string(st));
+    }
+    TaskManager(const TaskManager &) = delete;
+    TaskManager &operator=(const TaskManager &) = delete;
+    ~TaskManager()
+    {
+        if (ctx_) vo_ctx_destroy(ctx_);
+    }
+
+    bool start() { return is_running_ = (vo_queue_start(ctx_) == VO_OK); }
+    bool stop(uint32_t timeout_ms = 5000)
+    {
+        const int st = vo_queue_stop(ctx_, timeout_ms);
+        is_running_ = false;
+        return st == VO_OK;
+    }
+    bool isRunning() const { return is_running_; }
+
+    uint64_t submitCommand(vo_command &cmd)
+    {
+        const uint64_t seq = vo_submit(ctx_, &cmd);
+        if (seq == 0) throw ExecutionError(vo_last_error(ctx_));
+        return seq;
+    }
+    void waitCommand(uint64_t wait_id)
+    {
+        if (vo_wait(ctx_, wait_id, 0) != VO_OK) throw ExecutionError(vo_last_error(ctx_));
+    }
+    // throws ExecutionError on a non-zero status of a synchronous vo_* call
+    void check(int status) const
+    {
+        if (status != VO_OK) throw ExecutionError(vo_last_error(ctx_));
+    }
+    vo_ctx *native() const { return ctx_; }
+
+private:
+    vo_ctx *ctx_ = nullptr;
+    bool is_running_ = false;
+};
+
+class ExecutionContext {
+public:
+    explicit ExecutionContext(TaskManager &tm) : tm_(tm) {}
+    TaskManager &getTaskManager() { return tm_; }
+
+private:
+    TaskManager &tm_;
+};
+
+// ----------------------------------------------------------------- Column
+
+inline size_t type_width(uint32_t type_id)
+{
+    switch (type_id) {
+    case VO_T_BOOLEAN: case VO_T_TINYINT: return 1;
+    case VO_T_SMALLINT: return 2;
+    case VO_T_INTEGER: case VO_T_FLOAT: case VO_T_DATE: return 4;
+    case VO_T_BIGINT: case VO_T_DOUBLE: case VO_T_VARCHAR: case VO_T_UBIGINT: return 8;
+    default: return 0;
+    }
+}
+
+inline size_t next_pow2(size_t n, size_t floor_ = 4096)
+{
+    size_t p = floor_;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+// A device column owning its buffers (data: capacity * width bytes; bitmap:
+// capacity bytes or none).
+class Column {
+public:
+    Column() = default;
+    Column(uint32_t type_id, size_t capacity, bool nullable) : type_(type_id), capacity_(capacity)
+    {
+        if (type_width(type_id) == 0) throw ExecutionError("unsupported column type");
+        hip_check(hipMalloc(&data_, capacity * type_width(type_id)), "hipMalloc");
+        if (nullable) hip_check(hipMalloc(reinterpret_cast<void **>(&bitmap_), capacity), "hipMalloc");
+    }
+    // n values from host memory; valid (optional, n bytes) becomes the bitmap
+    static Column fromHost(uint32_t type_id, const void *values, size_t n, const uint8_t *valid = nullptr)
+    {
+        Column c(type_id, next_pow2(n), valid != nullptr);
+        hip_check(hipMemcpy(c.data_, values, n * type_width(type_id), hipMemcpyHostToDevice), "hipMemcpy");
+        if (valid) hip_check(hipMemcpy(c.bitmap_, valid, n, hipMemcpyHostToDevice), "hipMemcpy");
+        return c;
+    }
+    template <typename T>
+    std::vector<T> toHost(size_t n) const
+    {
+        if (sizeof(T) != type_width(type_)) throw ExecutionError("toHost: element size does not match the column type");
+        std::vector<T> v(n);
+        if (n) hip_check(hipMemcpy(v.data(), data_, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy");
+        return v;
+    }
+    std::vector<uint8_t> bitmapToHost(size_t n) const
+    {
+        std::vector<uint8_t> v(n, 1);
+        if (bitmap_ && n) hip_check(hipMemcpy(v.data(), bitmap_, n, hipMemcpyDeviceToHost), "hipMemcpy");
+        return v;
+    }
+
+    Column(Column &&o) noexcept { *this = std::move(o); }
+    Column &operator=(Column &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            type_ = o.type_;
+            capacity_ = o.capacity_;
+            data_ = std::exchange(o.data_, nullptr);
+            bitmap_ = std::exchange(o.bitmap_, nullptr);
+        }
+        return *this;
+    }
+    Column(const Column &) = delete;
+    Column &operator=(const Column &) = delete;
+    ~Column() { release(); }
+
+    uint32_t getTypeId() const { return type_; }
+    size_t capacity() const { return capacity_; }
+    void *rawData() { return data_; }
+    const void *rawData() const { return data_; }
+    uint8_t *rawBitmapData() { return bitmap_; }
+    // new device buffers for a permuted copy (Column::getDeviceBuffer analogue)
+    void *getDeviceBuffer() const
+    {
+        void *p = nullptr;
+        hip_check(hipMalloc(&p, capacity_ * type_width(type_)), "hipMalloc");
+        return p;
+    }
+    uint8_t *getDeviceBitmapBuffer() const
+    {
+        if (!bitmap_) return nullptr;
+        void *p = nullptr;
+        hip_check(hipMalloc(&p, capacity_), "hipMalloc");
+        return static_cast<uint8_t *>(p);
+    }
+    // take ownership of new buffers, freeing the old ones (setFromDeviceBuffers)
+    void setFromDeviceBuffers(void *data, uint8_t *bitmap)
+    {
+        if (data) {
+            (void)hipFree(data_);
+            data_ = data;
+        }
+        if (bitmap) {
+            (void)hipFree(bitmap_);
+            bitmap_ = bitmap;
+        }
+    }
+
+private:
+    void release()
+    {
+        if (data_) (void)hipFree(data_);
+        if (bitmap_) (void)hipFree(bitmap_);
+        data_ = nullptr;
+        bitmap_ = nullptr;
+    }
+    uint32_t type_ = 0;
+    size_t capacity_ = 0;
+    void *data_ = nullptr;
+    uint8_t *bitmap_ = nullptr;
+};
+
+class RowBatch {
+public:
+    RowBatch() = default;
+    RowBatch(std::vector<Column> cols, size_t rows) : cols_(std::move(cols)), rows_(rows) {}
+    size_t getRowCount() const { return rows_; }
+    size_t getColumnCount() const { return cols_.size(); }
+    Column &getColumn(size_t i) { return cols_.at(i); }
+    const Column &getColumn(size_t i) const { return cols_.at(i); }
+    void setRowCount(size_t n) { rows_ = n; }
+
+private:
+    std::vector<Column> cols_;
+    size_t rows_ = 0;
+};
+
+// --------------------------------------------------------------- operators
+
+class AbstractOperator {
+public:
+    explicit AbstractOperator(ExecutionContext &context) : context_(context) {}
+    virtual ~AbstractOperator() = default;
+    AbstractOperator(const AbstractOperator &) = delete;
+    AbstractOperator &operator=(const AbstractOperator &) = delete;
+    // an empty batch (no rows) ends the stream
+    virtual Result<RowBatch> next() = 0;
+
+protected:
+    ExecutionContext &context_;
+};
+
+// Source of one prepared batch (stands in for a scan of device columns).
+class BatchSourceOperator : public AbstractOperator {
+public:
+    BatchSourceOperator(ExecutionContext &context, RowBatch batch) : AbstractOperator(context), batch_(std::move(batch))
+    {
+    }
+    Result<RowBatch> next() override
+    {
+        if (done_) return Result<RowBatch>::success(RowBatch());
+        done_ = true;
+        return Result<RowBatch>::success(std::move(batch_));
+    }
+
+private:
+    RowBatch batch_;
+    bool done_ = false;
+};
+
+// Oblivious multi-column sort: OP_SORT over the key columns (pads written in
+// place, int64 row indices produced), then OP_PERMUTE of every other column
+// and of every null bitmap (sort_operator.cpp:44-123).  One input batch.
+class SortOperator : public AbstractOperator {
+public:
+    SortOperator(ExecutionContext &context, std::unique_ptr<AbstractOperator> child, std::vector<size_t> order_indices,
+                 std::vector<bool> ascending_flags)
+        : AbstractOperator(context), child_(std::move(child)), order_(std::move(order_indices)),
+          asc_(std::move(ascending_flags))
+    {
+    }
+
+    Result<RowBatch> next() override
+    {
+        if (sorted_) return Result<RowBatch>::success(RowBatch());
+        sorted_ = true;
+        auto in = child_->next();
+        if (!in) return in;
+        RowBatch batch = std::move(in).value();
+        try {
+            run(batch);
+        } catch (const std::exception &e) {
+            return Result<RowBatch>::failure(e.what());
+        }
+        return Result<RowBatch>::success(std::move(batch));
+    }
+
+private:
+    void run(RowBatch &batch)
+    {
+        const size_t n = batch.getRowCount();
+        if (n == 0) return;
+        if (order_.empty() || order_.size() > VO_MAX_SORT_COLUMNS || asc_.size() != order_.size())
+            throw ExecutionError("SortOperator: 1..8 sort columns, one ascending flag each");
+        const size_t n_pad = batch.getColumn(order_[0]).capacity();
+        TaskManager &tm = context_.getTaskManager();
+        int64_t *d_indices = nullptr;
+        hip_check(hipMalloc(reinterpret_cast<void **>(&d_indices), n_pad * sizeof(int64_t)), "hipMalloc");
+        std::unique_ptr<int64_t, void (*)(int64_t *)> guard(d_indices, [](int64_t *p) { (void)hipFree(p); });
+        std::vector<bool> is_key(batch.getColumnCount(), false);
+        vo_command sort_cmd{};
+        sort_cmd.opcode = VO_OP_SORT;
+        sort_cmd.args.sort.indices = d_indices;
+        sort_cmd.args.sort.n_sort_columns = order_.size();
+        sort_cmd.args.sort.n_rows = n;
+        sort_cmd.args.sort.n_padded_rows = n_pad;
+        for (size_t i = 0; i < order_.size(); ++i) {
+            Column &c = batch.getColumn(order_[i]);
+            if (c.capacity() != n_pad) throw ExecutionError("SortOperator: key columns differ in capacity");
+            is_key[order_[i]] = true;
+            sort_cmd.args.sort.sort_cols[i] = c.rawData();
+            sort_cmd.args.sort.col_types[i] = c.getTypeId();
+            sort_cmd.args.sort.ascending_flags[i] = asc_[i] ? 1 : 0;
+        }
+        tm.waitCommand(tm.submitCommand(sort_cmd));
+        for (size_t i = 0; i < batch.getColumnCount(); ++i) {
+            Column &c = batch.getColumn(i);
+            void *data_buf = nullptr;
+            uint64_t last = 0;
+            if (!is_key[i]) {
+                data_buf = c.getDeviceBuffer();
+                vo_command p{};
+                p.opcode = VO_OP_PERMUTE;
+                p.args.permute = {data_buf, c.rawData(), d_indices, n, c.getTypeId()};
+                last = tm.submitCommand(p);
+            }
+            uint8_t *bits_buf = c.getDeviceBitmapBuffer();
+            if (bits_buf) {
+                vo_command p{};
+                p.opcode = VO_OP_PERMUTE;
+                p.args.permute = {bits_buf, c.rawBitmapData(), d_indices, n, VO_T_BOOLEAN};
+                last = tm.submitCommand(p);
+            }
+            if (last) tm.waitCommand(last);
+            c.setFromDeviceBuffers(data_buf, bits_buf);
+        }
+    }
+
+    std::unique_ptr<AbstractOperator> child_;
+    std::vector<size_t> order_;
+    std::vector<bool> asc_;
+    bool sorted_ = false;
+};
+
+// Oblivious filter compaction: the rows whose mask byte (column mask_index,
+// BOOLEAN) is non-zero move to the front, in order, in every column marked in
+// compact_columns (data and null bitmap); the batch's row count becomes the
+// selected count.  Columns not marked keep their rows, as the reference
+// leaves them (filter_compaction_operator.cpp:112-143).
+class FilterCompactionOperator : public AbstractOperator {
+public:
+    FilterCompactionOperator(ExecutionContext &context, std::unique_ptr<AbstractOperator> child, size_t mask_index,
+                             std::vector<bool> compact_columns = {})
+        : AbstractOperator(context), child_(std::move(child)), mask_index_(mask_index),
+          compact_(std::move(compact_columns))
+    {
+    }
+
+    Result<RowBatch> next() override
+    {
+        auto in = child_->next();
+        if (!in) return in;
+        RowBatch batch = std::move(in).value();
+        if (batch.getRowCount() == 0) return Result<RowBatch>::success(std::move(batch));
+        try {
+            run(batch);
+        } catch (const std::exception &e) {
+            return Result<RowBatch>::failure(e.what());
+        }
+        return Result<RowBatch>::success(std::move(batch));
+    }
+
+private:
+    void run(RowBatch &batch)
+    {
+        const size_t n = batch.getRowCount();
+        Column &mask = batch.getColumn(mask_index_);
+        if (type_width(mask.getTypeId()) != 1) throw ExecutionError("FilterCompactionOperator: mask must be 1 byte/row");
+        const size_t n_pad = mask.capacity();
+        std::vector<std::pair<void *, uint32_t>> cols; // every buffer moved by the network
+        for (size_t i = 0; i < batch.getColumnCount(); ++i) {
+            if (i == mask_index_) continue;
+            if (!compact_.empty() && (i >= compact_.size() || !compact_[i])) continue;
+            Column &c = batch.getColumn(i);
+            if (c.capacity() != n_pad) throw ExecutionError("FilterCompactionOperator: columns differ in capacity");
+            cols.emplace_back(c.rawData(), (uint32_t)type_width(c.getTypeId()));
+            if (c.rawBitmapData()) cols.emplace_back(c.rawBitmapData(), 1u);
+        }
+        uint64_t *d_count = nullptr;
+        hip_check(hipMalloc(reinterpret_cast<void **>(&d_count), sizeof(uint64_t)), "hipMalloc");
+        std::unique_ptr<uint64_t, void (*)(uint64_t *)> guard(d_count, [](uint64_t *p) { (void)hipFree(p); });
+        TaskManager &tm = context_.getTaskManager();
+        // the routing depends on the mask only: each group of up to 8
+        // buffers is moved by the same network
+        for (size_t at = 0; at < std::max<size_t>(cols.size(), 1); at += VO_MAX_SORT_COLUMNS) {
+            vo_compact_args a{};
+            a.mask = static_cast<const uint8_t *>(mask.rawData());
+            const size_t k = std::min<size_t>(VO_MAX_SORT_COLUMNS, cols.size() - std::min(at, cols.size()));
+            if (k == 0) break;
+            for (size_t j = 0; j < k; ++j) {
+                a.cols[j] = cols[at + j].first;
+                a.col_bytes[j] = cols[at + j].second;
+            }
+            a.n_cols = (uint32_t)k;
+            a.n = n;
+            a.n_pad = n_pad;
+            a.out_count = d_count;
+            tm.check(vo_compact(tm.native(), &a));
+        }
+        uint64_t count = 0;
+        if (cols.empty()) {
+            // nothing to move: the count alone (a compaction of the mask copy)
+            void *tmp = mask.getDeviceBuffer();
+            std::unique_ptr<void, void (*)(void *)> g2(tmp, [](void *p) { (void)hipFree(p); });
+            hip_check(hipMemcpy(tmp, mask.rawData(), n_pad, hipMemcpyDeviceToDevice), "hipMemcpy");
+            vo_compact_args a{};
+            a.mask = static_cast<const uint8_t *>(mask.rawData());
+            a.cols[0] = tmp;
+            a.col_bytes[0] = 1;
+            a.n_cols = 1;
+            a.n = n;
+            a.n_pad = n_pad;
+            a.out_count = d_count;
+            tm.check(vo_compact(tm.native(), &a));
+        }
+        hip_check(hipMemcpy(&count, d_count, sizeof(count), hipMemcpyDeviceToHost), "hipMemcpy");
+        batch.setRowCount(count);
+    }
+
+    std::unique_ptr<AbstractOperator> child_;
+    size_t mask_index_;
+    std::vector<bool> compact_;
+};
+
+// Oblivious PK/FK join + SUM(value) GROUP BY group: dimension batch (key
+// INTEGER, group INTEGER), fact batch (foreign key INTEGER, value BIGINT).
+// Output: one batch of G rows, columns (group INTEGER, sum BIGINT), groups
+// ascending.  dense_groups: group ids lie in [0, max_groups) (max_groups <=
+// 64), a public bound of the schema.
+class PKFKJoinGroupSumOperator : public AbstractOperator {
+public:
+    PKFKJoinGroupSumOperator(ExecutionContext &context, std::unique_ptr<AbstractOperator> dim,
+                             std::unique_ptr<AbstractOperator> fact, size_t dim_key, size_t dim_group, size_t fact_fk,
+                             size_t fact_val, size_t max_groups, bool dense_groups)
+        : AbstractOperator(context), dim_(std::move(dim)), fact_(std::move(fact)), dk_(dim_key), dg_(dim_group),
+          fk_(fact_fk), fv_(fact_val), max_groups_(max_groups), dense_(dense_groups)
+    {
+    }
+
+    Result<RowBatch> next() override
+    {
+        if (done_) return Result<RowBatch>::success(RowBatch());
+        done_ = true;
+        auto d = dim_->next();
+        if (!d) return d;
+        auto f = fact_->next();
+        if (!f) return f;
+        RowBatch db = std::move(d).value(), fb = std::move(f).value();
+        try {
+            return Result<RowBatch>::success(run(db, fb));
+        } catch (const std::exception &e) {
+            return Result<RowBatch>::failure(e.what());
+        }
+    }
+
+private:
+    RowBatch run(RowBatch &db, RowBatch &fb)
+    {
+        Column &dk = db.getColumn(dk_), &dg = db.getColumn(dg_), &fk = fb.getColumn(fk_), &fv = fb.getColumn(fv_);
+        if (dk.getTypeId() != VO_T_INTEGER || dg.getTypeId() != VO_T_INTEGER || fk.getTypeId() != VO_T_INTEGER ||
+            fv.getTypeId() != VO_T_BIGINT)
+            throw ExecutionError("PKFKJoinGroupSumOperator: keys and groups INTEGER, values BIGINT");
+        Column og(VO_T_INTEGER, next_pow2(max_groups_, 1), false), os(VO_T_BIGINT, next_pow2(max_groups_, 1), false);
+        uint64_t *d_n = nullptr;
+        hip_check(hipMalloc(reinterpret_cast<void **>(&d_n), sizeof(uint64_t)), "hipMalloc");
+        std::unique_ptr<uint64_t, void (*)(uint64_t *)> guard(d_n, [](uint64_t *p) { (void)hipFree(p); });
+        vo_join_args a{};
+        a.dim_key = static_cast<const int32_t *>(dk.rawData());
+        a.dim_group = static_cast<const int32_t *>(dg.rawData());
+        a.n_dim = db.getRowCount();
+        a.fact_fk = static_cast<const int32_t *>(fk.rawData());
+        a.fact_val = static_cast<const int64_t *>(fv.rawData());
+        a.n_fact = fb.getRowCount();
+        a.n_pad = next_pow2(a.n_dim + a.n_fact);
+        a.out_group = static_cast<int32_t *>(og.rawData());
+        a.out_sum = static_cast<int64_t *>(os.rawData());
+        a.out_n_groups = d_n;
+        a.max_groups = max_groups_;
+        a.dense_groups = dense_ ? 1 : 0;
+        TaskManager &tm = context_.getTaskManager();
+        tm.check(vo_pkfk_join_groupsum(tm.native(), &a));
+        uint64_t g = 0;
+        hip_check(hipMemcpy(&g, d_n, sizeof(g), hipMemcpyDeviceToHost), "hipMemcpy");
+        std::vector<Column> cols;
+        cols.push_back(std::move(og));
+        cols.push_back(std::move(os));
+        return RowBatch(std::move(cols), std::min<uint64_t>(g, max_groups_));
+    }
+
+    std::unique_ptr<AbstractOperator> dim_, fact_;
+    size_t dk_, dg_, fk_, fv_, max_groups_;
+    bool dense_;
+    bool done_ = false;
+};
+
+} // namespace velodb_amd

@@ -85,3 +85,14 @@ def test_status_strings_without_device():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(ImportError):
         velodb_amd.load_library(str(tmp_path / "absent.so"))
+
+
+def test_cpp_operator_header_compiles():
+    """The header-only C++ operator layer builds against the C ABI (syntax
+    and templates; nothing runs without a GPU)."""
+    import subprocess
+    src = os.path.join(ROOT, "include", "velodb_amd", "operators.hpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+                        "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"), "-x", "c++", src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -1,0 +1,21 @@
+"""C++ operator layer (include/velodb_amd/operators.hpp: TaskManager,
+SortOperator, FilterCompactionOperator, PKFKJoinGroupSumOperator) on the
+GPU: runs tests/cpp/test_operators, the ORDER BY cases of the reference's
+tests/query/test_order_by.cpp plus join and queue checks."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_operators")
+
+
+def test_cpp_operator_suite():
+    assert os.path.exists(BIN), "tests/cpp/test_operators is not built (__graft_entry__.build())"
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=110)
+    print(r.stdout)
+    print(r.stderr)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "OK: 0 failed checks" in r.stdout

@@ -465,16 +465,27 @@ __global__ __launch_bounds__(256) void k_sum_prod_i128(const int64_t *__restrict
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ void k_sum_final(const __int128 *__restrict__ partial, uint32_t np, const uint64_t *__restrict__ count,
-                            uint64_t *__restrict__ out)
+// One 256-thread block: strided partial sums, then a tree in LDS (exact:
+// int128 addition is associative).
+__global__ __launch_bounds__(256) void k_sum_final(const __int128 *__restrict__ partial, uint32_t np,
+                                                   const uint64_t *__restrict__ count, uint64_t *__restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ __int128 red[256];
     __int128 s = 0;
-    for (uint32_t i = 0; i < np; ++i) s += partial[i];
-    out[0] = *count;
-    out[1] = 0;
-    out[2] = (uint64_t)s;
-    out[3] = (uint64_t)(s >> 64);
+    for (uint32_t i = threadIdx.x; i < np; i += 256) s += partial[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (uint32_t)off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const __int128 t = red[0];
+        out[0] = *count;
+        out[1] = 0;
+        out[2] = (uint64_t)t;
+        out[3] = (uint64_t)(t >> 64);
+    }
 }
 
 // ------------------------------------------------------------ OP_SCATTER
@@ -686,10 +697,12 @@ void op_filter_sum(vo_ctx *ctx, const vo_filter_args &a)
     Cols<2> in{{{const_cast<int64_t *>(a.price), 8}, {const_cast<int64_t *>(a.disc), 8}}};
     Cols<2> cs{{{op, 8}, {od, 8}}};
     run_network<2>(ctx, in, cs, ws.mask, a.n, a.n_pad, ws.tile_pre);
-    launch(ctx, LaunchInfo{"filter_sum", dim3(kSumBlocks), dim3(256), 0, a.n_pad}, k_sum_prod_i128, (const int64_t *)op,
+    // at least 8 rows per thread, at most kSumBlocks partial sums
+    const uint32_t sb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kSumBlocks, a.n_pad / 2048));
+    launch(ctx, LaunchInfo{"filter_sum", dim3(sb), dim3(256), 0, a.n_pad}, k_sum_prod_i128, (const int64_t *)op,
            (const int64_t *)od, a.n_pad, (const uint64_t *)ws.count, ws.partial);
-    launch(ctx, LaunchInfo{"filter_sum_final", dim3(1), dim3(64), 0, 0}, k_sum_final, (const __int128 *)ws.partial,
-           kSumBlocks, (const uint64_t *)ws.count, a.out);
+    launch(ctx, LaunchInfo{"filter_sum_final", dim3(1), dim3(256), 0, 0}, k_sum_final, (const __int128 *)ws.partial,
+           sb, (const uint64_t *)ws.count, a.out);
 }
 
 void op_scatter(vo_ctx *ctx, const vo_scatter_args &a)
