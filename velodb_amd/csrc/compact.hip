@@ -91,15 +91,17 @@ __device__ __forceinline__ bool level_flag32(uint32_t cb, uint32_t cbh, int q, u
 
 // Selected rows per tile: sel[i] = mask[i] != 0 && i < n.
 __global__ __launch_bounds__(CBLOCK) void k_count_mask(const uint8_t *__restrict__ mask, uint64_t n,
-                                                       uint32_t *__restrict__ tile_cnt)
+                                                       uint32_t *__restrict__ tile_cnt, TraceSlot *slot)
 {
     __shared__ uint32_t red[CBLOCK / 64];
+    dev::Atr tr(slot);
     const uint64_t t0 = (uint64_t)blockIdx.x * CT;
     uint32_t c = 0;
 #pragma unroll 4
     for (int k = 0; k < CT / CBLOCK; ++k) {
         const uint64_t i = t0 + k * CBLOCK + threadIdx.x;
         c += (mask[i] != 0) & (i < n);
+        tr.at(1, i);
     }
     for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
@@ -108,7 +110,9 @@ __global__ __launch_bounds__(CBLOCK) void k_count_mask(const uint8_t *__restrict
         uint32_t s = 0;
         for (int w = 0; w < CBLOCK / 64; ++w) s += red[w];
         tile_cnt[blockIdx.x] = s;
+        tr.at(2, blockIdx.x);
     }
+    tr.done(dev::gtid());
 }
 
 // Q6-style predicate: writes mask[i] and per-tile counts in one pass.
@@ -120,9 +124,11 @@ struct Pred {
 __global__ __launch_bounds__(CBLOCK) void k_filter_count(const int32_t *__restrict__ day,
                                                          const int64_t *__restrict__ qty,
                                                          const int64_t *__restrict__ disc, uint64_t n, Pred pr,
-                                                         uint8_t *__restrict__ mask, uint32_t *__restrict__ tile_cnt)
+                                                         uint8_t *__restrict__ mask, uint32_t *__restrict__ tile_cnt,
+                                                         TraceSlot *slot)
 {
     __shared__ uint32_t red[CBLOCK / 64];
+    dev::Atr tr(slot);
     const uint64_t t0 = (uint64_t)blockIdx.x * CT;
     uint32_t c = 0;
 #pragma unroll 4
@@ -134,6 +140,7 @@ __global__ __launch_bounds__(CBLOCK) void k_filter_count(const int32_t *__restri
                            (q < pr.qty_lt) & (i < n);
         mask[i] = (uint8_t)m;
         c += m;
+        tr.at(3, i); // day, qty, disc read and mask written at row i
     }
     for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
@@ -142,20 +149,27 @@ __global__ __launch_bounds__(CBLOCK) void k_filter_count(const int32_t *__restri
         uint32_t s = 0;
         for (int w = 0; w < CBLOCK / 64; ++w) s += red[w];
         tile_cnt[blockIdx.x] = s;
+        tr.at(2, blockIdx.x);
     }
+    tr.done(dev::gtid());
 }
 
 // Exclusive scan of the tile counts (one block): pre[0..nt], pre[nt] = total.
 __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict__ cnt, uint32_t nt,
-                                                     uint64_t *__restrict__ pre, uint64_t *__restrict__ out_count)
+                                                     uint64_t *__restrict__ pre, uint64_t *__restrict__ out_count,
+                                                     TraceSlot *slot)
 {
     __shared__ uint64_t part[1024];
+    dev::Atr tr(slot);
     const uint32_t per = (nt + 1023) / 1024;
     const uint32_t lo = threadIdx.x * per;
     uint64_t s = 0;
     for (uint32_t k = 0; k < per; ++k) {
         const uint32_t i = lo + k;
-        s += (i < nt) ? cnt[i] : 0;
+        if (i < nt) {
+            s += cnt[i];
+            tr.at(4, i);
+        }
     }
     part[threadIdx.x] = s;
     __syncthreads();
@@ -171,12 +185,15 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict_
         if (i < nt) {
             pre[i] = run;
             run += cnt[i];
+            tr.at(5, i);
         }
     }
     if (threadIdx.x == 1023) {
         pre[nt] = part[1023];
         if (out_count) *out_count = part[1023];
+        tr.at(6, nt);
     }
+    tr.done(dev::gtid());
 }
 
 // ---------------------------------------------------------- local levels
@@ -244,9 +261,10 @@ __device__ __forceinline__ void clevels(uint64_t (&r)[16][NW], const uint16_t *c
 }
 
 // W8: every column is 8 bytes wide (no per-load width dispatch)
-template <int NW, bool W8>
+template <int NW, bool W8, bool TRACE>
 __global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> src, Cols<NW> dst, const uint8_t *__restrict__ mask,
-                                                          uint64_t n, const uint64_t *__restrict__ tile_pre)
+                                                          uint64_t n, const uint64_t *__restrict__ tile_pre,
+                                                          TraceSlot *slot)
 {
     static_assert(CT == 16 * CBLOCK, "16 rows per thread");
     extern __shared__ __align__(16) char lds_raw[];
@@ -257,14 +275,18 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> src, Cols<NW>
     const uint64_t t0 = (uint64_t)blockIdx.x * CT;
     const uint64_t tbase = tile_pre[blockIdx.x];
     constexpr int PER = CT / CBLOCK; // 16 consecutive rows per thread for the scan
+    dev::AtrT<TRACE> tr(slot);
+    tr.at(7, blockIdx.x);
 
     // rows, coalesced (layout 8: row tid + 256 m); issued before the scan
     uint64_t r[16][NW];
 #pragma unroll
     for (int m = 0; m < 16; ++m)
 #pragma unroll
-        for (int w = 0; w < NW; ++w)
+        for (int w = 0; w < NW; ++w) {
             r[m][w] = W8 ? ((const uint64_t *)src.c[w].p)[t0 + tid + 256u * m] : ld_col(src.c[w], t0 + tid + 256u * m);
+            tr.at(8 + w, t0 + tid + 256u * m);
+        }
 
     // exclusive prefix of sel inside the tile: thread tid owns rows [16 tid, 16 tid + 16)
     uint32_t sel[PER];
@@ -274,6 +296,7 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> src, Cols<NW>
         const uint64_t i = t0 + tid * PER + k;
         sel[k] = (mask[i] != 0) & (i < n);
         run += sel[k];
+        tr.at(1, i);
     }
     uint32_t incl = run;
     const uint32_t lane = tid & 63;
@@ -309,7 +332,9 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> src, Cols<NW>
                 ((uint64_t *)dst.c[w].p)[t0 + tid_s + 256u * m] = r[m][w];
             else
                 st_col(dst.c[w], t0 + tid_s + 256u * m, r[m][w]);
+            tr.at(16 + w, t0 + tid_s + 256u * m);
         }
+    tr.done(dev::gtid());
 }
 
 // ---------------------------------------------------------- global levels
@@ -318,16 +343,21 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> src, Cols<NW>
 // rows base + m * 2^q_lo.  The block start of any pair has every bit below
 // q_lo + 1 clear, so its prefix counts are the same for all lanes of a
 // wavefront and come from the tile prefix array.
-template <int NW, int S>
-__global__ __launch_bounds__(256) void k_compact_global(Cols<NW> cols, const uint64_t *__restrict__ tile_pre, int q_lo)
+template <int NW, int S, bool TRACE>
+__global__ __launch_bounds__(256) void k_compact_global(Cols<NW> cols, const uint64_t *__restrict__ tile_pre, int q_lo,
+                                                        TraceSlot *slot)
 {
+    dev::AtrT<TRACE> tr(slot);
     const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     const uint64_t base = (g & ((1ull << q_lo) - 1)) | ((g >> q_lo) << (q_lo + S));
     uint64_t r[1 << S][NW];
 #pragma unroll
     for (int m = 0; m < (1 << S); ++m)
 #pragma unroll
-        for (int w = 0; w < NW; ++w) r[m][w] = ld_col(cols.c[w], base + ((uint64_t)m << q_lo));
+        for (int w = 0; w < NW; ++w) {
+            r[m][w] = ld_col(cols.c[w], base + ((uint64_t)m << q_lo));
+            tr.at(8 + w, base + ((uint64_t)m << q_lo));
+        }
     // uniform part of base (bits >= 8 are shared by the wavefront)
     const uint64_t ubase = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
                            __builtin_amdgcn_readfirstlane((uint32_t)base & ~0xFFu);
@@ -343,6 +373,8 @@ __global__ __launch_bounds__(256) void k_compact_global(Cols<NW> cols, const uin
             const uint64_t b = ue & ~(2 * half - 1);
             const uint64_t cb = tile_pre[b >> CT_LOG];
             const uint64_t cbh = tile_pre[(b + half) >> CT_LOG];
+            tr.at(7, b >> CT_LOG);
+            tr.at(7, (b + half) >> CT_LOG);
             const bool sw = level_flag(cb, cbh, half, e & (half - 1));
             const int m2 = m | (1 << k);
 #pragma unroll
@@ -356,7 +388,11 @@ __global__ __launch_bounds__(256) void k_compact_global(Cols<NW> cols, const uin
 #pragma unroll
     for (int m = 0; m < (1 << S); ++m)
 #pragma unroll
-        for (int w = 0; w < NW; ++w) st_col(cols.c[w], base + ((uint64_t)m << q_lo), r[m][w]);
+        for (int w = 0; w < NW; ++w) {
+            st_col(cols.c[w], base + ((uint64_t)m << q_lo), r[m][w]);
+            tr.at(16 + w, base + ((uint64_t)m << q_lo));
+        }
+    tr.done(dev::gtid());
 }
 
 // ------------------------------------------------- LDS-staged global levels
@@ -395,10 +431,11 @@ __device__ __forceinline__ void hlevels(uint64_t (&r)[16][NW], const uint32_t *P
     }
 }
 
-template <int NW, int S, bool W8>
+template <int NW, int S, bool W8, bool TRACE>
 __global__ __launch_bounds__(CBLOCK) void k_compact_hybrid(Cols<NW> cols, const uint64_t *__restrict__ tile_pre,
-                                                           int q_lo)
+                                                           int q_lo, TraceSlot *slot)
 {
+    dev::AtrT<TRACE> tr(slot);
     static_assert(S >= 5 && S <= 8, "hybrid widths 5..8");
     constexpr int CB = CT_LOG - S;
     extern __shared__ __align__(16) char lds_raw[];
@@ -409,15 +446,20 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_hybrid(Cols<NW> cols, const 
     const uint32_t runs = (1u << q_lo) >> CB; // blocks per group of 2^(q_lo+S) rows
     const uint32_t hibase = (blockIdx.x / runs) << (q_lo + S);
     const uint32_t j0 = (blockIdx.x % runs) << CB;
-    for (uint32_t k = tid; k <= (1u << S); k += CBLOCK)
+    for (uint32_t k = tid; k <= (1u << S); k += CBLOCK) {
         P[k] = (uint32_t)tile_pre[((uint64_t)hibase + ((uint64_t)k << q_lo)) >> CT_LOG];
+        tr.at(7, ((uint64_t)hibase + ((uint64_t)k << q_lo)) >> CT_LOG);
+    }
     uint64_t r[16][NW];
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const uint32_t l = tid | ((uint32_t)m << 8);
         const uint32_t g = hibase + ((l >> CB) << q_lo) + j0 + (l & ((1u << CB) - 1));
 #pragma unroll
-        for (int w = 0; w < NW; ++w) r[m][w] = W8 ? ((const uint64_t *)cols.c[w].p)[g] : ld_col(cols.c[w], g);
+        for (int w = 0; w < NW; ++w) {
+            r[m][w] = W8 ? ((const uint64_t *)cols.c[w].p)[g] : ld_col(cols.c[w], g);
+            tr.at(8 + w, g);
+        }
     }
     ctranspose<NW>(r, buf, tid, 8, 4); // its barriers also publish P
     hlevels<NW, 4, S>(r, P, tid, q_lo, j0);
@@ -437,8 +479,10 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_hybrid(Cols<NW> cols, const 
                 ((uint64_t *)cols.c[w].p)[g] = r[m][w];
             else
                 st_col(cols.c[w], g, r[m][w]);
+            tr.at(16 + w, g);
         }
     }
+    tr.done(dev::gtid());
 }
 
 // ------------------------------------------------------- int128 SUM pass
@@ -447,14 +491,16 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_hybrid(Cols<NW> cols, const 
 // every row is read, rows >= count contribute a multiplied-out zero.
 __global__ __launch_bounds__(256) void k_sum_prod_i128(const int64_t *__restrict__ a, const int64_t *__restrict__ b,
                                                        uint64_t n_pad, const uint64_t *__restrict__ count,
-                                                       __int128 *__restrict__ partial)
+                                                       __int128 *__restrict__ partial, TraceSlot *slot)
 {
     __shared__ __int128 red[256];
+    dev::Atr tr(slot);
     const uint64_t m = *count;
     __int128 acc = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * 256) {
         const __int128 prod = (__int128)a[i] * (__int128)b[i];
         acc += prod * (__int128)(i < m);
+        tr.at(8, i);
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -462,17 +508,26 @@ __global__ __launch_bounds__(256) void k_sum_prod_i128(const int64_t *__restrict
         if (threadIdx.x < (uint32_t)off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = red[0];
+        tr.at(16, blockIdx.x);
+    }
+    tr.done(dev::gtid());
 }
 
 // One 256-thread block: strided partial sums, then a tree in LDS (exact:
 // int128 addition is associative).
 __global__ __launch_bounds__(256) void k_sum_final(const __int128 *__restrict__ partial, uint32_t np,
-                                                   const uint64_t *__restrict__ count, uint64_t *__restrict__ out)
+                                                   const uint64_t *__restrict__ count, uint64_t *__restrict__ out,
+                                                   TraceSlot *slot)
 {
     __shared__ __int128 red[256];
+    dev::Atr tr(slot);
     __int128 s = 0;
-    for (uint32_t i = threadIdx.x; i < np; i += 256) s += partial[i];
+    for (uint32_t i = threadIdx.x; i < np; i += 256) {
+        s += partial[i];
+        tr.at(8, i);
+    }
     red[threadIdx.x] = s;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
@@ -485,16 +540,19 @@ __global__ __launch_bounds__(256) void k_sum_final(const __int128 *__restrict__ 
         out[1] = 0;
         out[2] = (uint64_t)t;
         out[3] = (uint64_t)(t >> 64);
+        tr.at(16, 0);
     }
+    tr.done(dev::gtid());
 }
 
 // ------------------------------------------------------------ OP_SCATTER
 
 __global__ __launch_bounds__(CBLOCK) void k_scatter_positions(const uint8_t *__restrict__ mask, uint64_t n,
                                                               const uint64_t *__restrict__ tile_pre,
-                                                              int32_t *__restrict__ out)
+                                                              int32_t *__restrict__ out, TraceSlot *slot)
 {
     __shared__ uint32_t wsum[CBLOCK / 64];
+    dev::Atr tr(slot);
     const uint32_t tid = threadIdx.x;
     const uint64_t t0 = (uint64_t)blockIdx.x * CT;
     constexpr int PER = CT / CBLOCK;
@@ -502,7 +560,12 @@ __global__ __launch_bounds__(CBLOCK) void k_scatter_positions(const uint8_t *__r
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const uint64_t i = t0 + tid * PER + k;
-        sel[k] = (i < n) ? (mask[i] != 0) : 0;
+        if (i < n) {
+            sel[k] = mask[i] != 0;
+            tr.at(1, i);
+        } else {
+            sel[k] = 0;
+        }
         run += sel[k];
     }
     uint32_t incl = run;
@@ -519,9 +582,13 @@ __global__ __launch_bounds__(CBLOCK) void k_scatter_positions(const uint8_t *__r
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const uint64_t i = t0 + tid * PER + k;
-        if (sel[k]) out[i] = (int32_t)c; // reference semantics: only selected rows are written
+        if (sel[k]) { // reference semantics: only selected rows are written (not oblivious)
+            out[i] = (int32_t)c;
+            tr.at(16, i);
+        }
         c += sel[k];
     }
+    tr.done(dev::gtid());
 }
 
 // --------------------------------------------------------------- drivers
@@ -561,10 +628,12 @@ void run_network(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, 
     const size_t lds = (size_t)CTPAD * 8 + (CT + 1) * 2;
     bool w8 = true;
     for (int w = 0; w < NW; ++w) w8 = w8 && src.c[w].w == 8 && cols.c[w].w == 8;
-    auto kl = w8 ? k_compact_local<NW, true> : k_compact_local<NW, false>;
+    const bool tr = ctx->tracing;
+    auto kl = tr ? (w8 ? k_compact_local<NW, true, true> : k_compact_local<NW, false, true>)
+                 : (w8 ? k_compact_local<NW, true, false> : k_compact_local<NW, false, false>);
     VO_HIP(hipFuncSetAttribute((const void *)kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     launch(ctx, LaunchInfo{"compact_local", dim3(nt), dim3(CBLOCK), (uint32_t)lds, hash_mix(NW, n_log)}, kl, src,
-           cols, mask, n, tile_pre);
+           cols, mask, n, tile_pre, trace_slot(ctx));
     // levels CT_LOG .. n_log-1: passes of at most 8 levels, as even as
     // possible; 5..8 levels run LDS-staged, fewer in registers
     const int count = n_log - CT_LOG;
@@ -578,27 +647,41 @@ void run_network(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, 
             auto pick = [&](auto k5, auto k6, auto k7, auto k8) {
                 auto k = s == 5 ? k5 : s == 6 ? k6 : s == 7 ? k7 : k8;
                 VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hl));
-                launch(ctx, li, k, cols, tile_pre, q);
+                launch(ctx, li, k, cols, tile_pre, q, trace_slot(ctx));
             };
-            if (w8)
-                pick(k_compact_hybrid<NW, 5, true>, k_compact_hybrid<NW, 6, true>, k_compact_hybrid<NW, 7, true>,
-                     k_compact_hybrid<NW, 8, true>);
-            else
-                pick(k_compact_hybrid<NW, 5, false>, k_compact_hybrid<NW, 6, false>, k_compact_hybrid<NW, 7, false>,
-                     k_compact_hybrid<NW, 8, false>);
+            if (tr) {
+                if (w8)
+                    pick(k_compact_hybrid<NW, 5, true, true>, k_compact_hybrid<NW, 6, true, true>,
+                         k_compact_hybrid<NW, 7, true, true>, k_compact_hybrid<NW, 8, true, true>);
+                else
+                    pick(k_compact_hybrid<NW, 5, false, true>, k_compact_hybrid<NW, 6, false, true>,
+                         k_compact_hybrid<NW, 7, false, true>, k_compact_hybrid<NW, 8, false, true>);
+            } else {
+                if (w8)
+                    pick(k_compact_hybrid<NW, 5, true, false>, k_compact_hybrid<NW, 6, true, false>,
+                         k_compact_hybrid<NW, 7, true, false>, k_compact_hybrid<NW, 8, true, false>);
+                else
+                    pick(k_compact_hybrid<NW, 5, false, false>, k_compact_hybrid<NW, 6, false, false>,
+                         k_compact_hybrid<NW, 7, false, false>, k_compact_hybrid<NW, 8, false, false>);
+            }
             q += s;
             continue;
         }
         const uint32_t blocks = (uint32_t)((n_pad >> s) / 256);
         LaunchInfo li{"compact_global", dim3(blocks), dim3(256), 0, hash_mix(hash_mix(NW, q), s)};
         switch (s) {
-        case 1: launch(ctx, li, k_compact_global<NW, 1>, cols, tile_pre, q); break;
-        case 2: launch(ctx, li, k_compact_global<NW, 2>, cols, tile_pre, q); break;
-        case 3: launch(ctx, li, k_compact_global<NW, 3>, cols, tile_pre, q); break;
-        case 4: launch(ctx, li, k_compact_global<NW, 4>, cols, tile_pre, q); break;
+        case 1: launch(ctx, li, tr ? k_compact_global<NW, 1, true> : k_compact_global<NW, 1, false>, cols, tile_pre, q,
+                          trace_slot(ctx)); break;
+        case 2: launch(ctx, li, tr ? k_compact_global<NW, 2, true> : k_compact_global<NW, 2, false>, cols, tile_pre, q,
+                          trace_slot(ctx)); break;
+        case 3: launch(ctx, li, tr ? k_compact_global<NW, 3, true> : k_compact_global<NW, 3, false>, cols, tile_pre, q,
+                          trace_slot(ctx)); break;
+        case 4: launch(ctx, li, tr ? k_compact_global<NW, 4, true> : k_compact_global<NW, 4, false>, cols, tile_pre, q,
+                          trace_slot(ctx)); break;
         default:
             if constexpr (NW == 1) {
-                launch(ctx, li, k_compact_global<NW, 5>, cols, tile_pre, q);
+                launch(ctx, li, tr ? k_compact_global<NW, 5, true> : k_compact_global<NW, 5, false>, cols, tile_pre, q,
+                       trace_slot(ctx));
                 break;
             }
             throw Error{VO_ERR_INVALID, "compaction pass width"};
@@ -612,9 +695,10 @@ void count_and_scan(vo_ctx *ctx, const uint8_t *mask, uint64_t n, uint64_t n_pad
                     uint64_t *out_count)
 {
     const uint32_t nt = (uint32_t)(n_pad / CT);
-    launch(ctx, LaunchInfo{"compact_count", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_count_mask, mask, n, ws.tile_cnt);
+    launch(ctx, LaunchInfo{"compact_count", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_count_mask, mask, n, ws.tile_cnt,
+           trace_slot(ctx));
     launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)ws.tile_cnt,
-           nt, ws.tile_pre, out_count ? out_count : ws.count);
+           nt, ws.tile_pre, out_count ? out_count : ws.count, trace_slot(ctx));
 }
 
 void check_pad(uint64_t n, uint64_t n_pad)
@@ -636,9 +720,10 @@ void compact_rows2(vo_ctx *ctx, uint64_t *c0, uint64_t *c1, const uint8_t *mask,
     const uint32_t nt = (uint32_t)(n_pad / CT);
     uint32_t *tile_cnt = (uint32_t *)(count_dev + 1);
     uint64_t *tile_pre = (uint64_t *)(((uintptr_t)(tile_cnt + nt) + 255) & ~(uintptr_t)255);
-    launch(ctx, LaunchInfo{"compact_count", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_count_mask, mask, n_pad, tile_cnt);
+    launch(ctx, LaunchInfo{"compact_count", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_count_mask, mask, n_pad, tile_cnt,
+           trace_slot(ctx));
     launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)tile_cnt, nt,
-           tile_pre, count_dev);
+           tile_pre, count_dev, trace_slot(ctx));
     Cols<2> cs{{{c0, 8}, {c1, 8}}};
     run_network<2>(ctx, cs, cs, mask, n_pad, n_pad, tile_pre);
 }
@@ -687,9 +772,9 @@ void op_filter_sum(vo_ctx *ctx, const vo_filter_args &a)
     const uint32_t nt = (uint32_t)(a.n_pad / CT);
     Pred pr{a.day_lo, a.day_hi, a.disc_lo, a.disc_hi, a.qty_lt};
     launch(ctx, LaunchInfo{"filter_count", dim3(nt), dim3(CBLOCK), 0, a.n_pad}, k_filter_count, a.day, a.qty, a.disc,
-           a.n, pr, ws.mask, ws.tile_cnt);
+           a.n, pr, ws.mask, ws.tile_cnt, trace_slot(ctx));
     launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)ws.tile_cnt, nt,
-           ws.tile_pre, ws.count);
+           ws.tile_pre, ws.count, trace_slot(ctx));
     // the network's first pass reads the caller's price/disc and writes the
     // output columns (or the inputs in place when no outputs are given)
     int64_t *op = in_place_copy ? a.out_price : const_cast<int64_t *>(a.price);
@@ -700,9 +785,9 @@ void op_filter_sum(vo_ctx *ctx, const vo_filter_args &a)
     // at least 8 rows per thread, at most kSumBlocks partial sums
     const uint32_t sb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kSumBlocks, a.n_pad / 2048));
     launch(ctx, LaunchInfo{"filter_sum", dim3(sb), dim3(256), 0, a.n_pad}, k_sum_prod_i128, (const int64_t *)op,
-           (const int64_t *)od, a.n_pad, (const uint64_t *)ws.count, ws.partial);
+           (const int64_t *)od, a.n_pad, (const uint64_t *)ws.count, ws.partial, trace_slot(ctx));
     launch(ctx, LaunchInfo{"filter_sum_final", dim3(1), dim3(256), 0, 0}, k_sum_final, (const __int128 *)ws.partial,
-           sb, (const uint64_t *)ws.count, a.out);
+           sb, (const uint64_t *)ws.count, a.out, trace_slot(ctx));
 }
 
 void op_scatter(vo_ctx *ctx, const vo_scatter_args &a)
@@ -720,7 +805,7 @@ void op_scatter(vo_ctx *ctx, const vo_scatter_args &a)
     VO_HIP(hipMemcpyAsync(ws.mask, a.in_mask, a.n, hipMemcpyDeviceToDevice, ctx->stream));
     count_and_scan(ctx, ws.mask, a.n, n_pad, ws, a.out_count);
     launch(ctx, LaunchInfo{"scatter_positions", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_scatter_positions,
-           (const uint8_t *)ws.mask, a.n, (const uint64_t *)ws.tile_pre, a.out_indices);
+           (const uint8_t *)ws.mask, a.n, (const uint64_t *)ws.tile_pre, a.out_indices, trace_slot(ctx));
 }
 
 } // namespace vo

@@ -36,23 +36,30 @@ constexpr int JB = 256;
 constexpr int JPER = JT / JB;
 constexpr uint64_t PAD_KEY = ~0ull;
 
+template <bool TRACE>
 __global__ void k_join_build(const int32_t *__restrict__ dk, const int32_t *__restrict__ dg, uint64_t n_dim,
                              const int32_t *__restrict__ fk, const int64_t *__restrict__ fv, uint64_t n_fact,
-                             uint64_t n_pad, uint64_t *__restrict__ keys, uint64_t *__restrict__ vals)
+                             uint64_t n_pad, uint64_t *__restrict__ keys, uint64_t *__restrict__ vals,
+                             TraceSlot *slot)
 {
+    dev::AtrT<TRACE> tr(slot);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t k = PAD_KEY, v = 0;
-        if (i < n_dim) {
+        if (i < n_dim) { // public sizes only
             k = ((uint64_t)(uint32_t)dk[i] << 32) | (i & 0x7FFFFFFFu);
             v = (uint64_t)(uint32_t)dg[i];
+            tr.at(1, i);
         } else if (i < n_dim + n_fact) {
             const uint64_t f = i - n_dim;
             k = ((uint64_t)(uint32_t)fk[f] << 32) | (1ull << 31) | (f & 0x7FFFFFFFu);
             v = (uint64_t)fv[f];
+            tr.at(2, f);
         }
         keys[i] = k;
         vals[i] = v;
+        tr.at(3, i);
     }
+    tr.done(dev::gtid());
 }
 
 // ------------------------------------------------- forward fill (step 3)
@@ -65,15 +72,18 @@ struct Fill {
 __device__ __forceinline__ bool is_dim(uint64_t k) { return k != PAD_KEY && ((k >> 31) & 1) == 0; }
 
 // Per tile: the last dimension row (key, group) or none.
+template <bool TRACE>
 __global__ __launch_bounds__(JB) void k_fill_tiles(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
-                                                   Fill *__restrict__ tile_last)
+                                                   Fill *__restrict__ tile_last, TraceSlot *slot)
 {
     __shared__ uint64_t wk[JB], wg[JB];
+    dev::AtrT<TRACE> tr(slot);
     const uint64_t t0 = (uint64_t)blockIdx.x * JT + (uint64_t)threadIdx.x * JPER;
     uint64_t lk = PAD_KEY, lg = 0;
 #pragma unroll
     for (int k = 0; k < JPER; ++k) {
         const uint64_t key = keys[t0 + k], v = vals[t0 + k];
+        tr.at(1, t0 + k);
         const bool d = is_dim(key);
         lk = d ? (key >> 32) : lk;
         lg = d ? v : lg;
@@ -89,7 +99,9 @@ __global__ __launch_bounds__(JB) void k_fill_tiles(const uint64_t *__restrict__ 
             g = has ? wg[t] : g;
         }
         tile_last[blockIdx.x] = Fill{k, g};
+        tr.at(2, blockIdx.x);
     }
+    tr.done(dev::gtid());
 }
 
 // Exclusive scan over the nt tile summaries with one 1024-thread block:
@@ -98,14 +110,18 @@ __global__ __launch_bounds__(JB) void k_fill_tiles(const uint64_t *__restrict__ 
 // prefixes.  op(a, b) is associative with identity `id`.
 constexpr uint32_t kScanThreads = 1024;
 
-template <class T, class Op>
-__device__ void block_exclusive_scan(const T *__restrict__ in, uint32_t nt, T *__restrict__ out, T id, Op op, T *sh)
+template <class T, class Op, class Tr>
+__device__ void block_exclusive_scan(const T *__restrict__ in, uint32_t nt, T *__restrict__ out, T id, Op op, T *sh,
+                                     Tr &tr)
 {
     const uint32_t per = (nt + kScanThreads - 1) / kScanThreads;
     const uint32_t lo = threadIdx.x * per;
     T s = id;
     for (uint32_t k = 0; k < per; ++k)
-        if (lo + k < nt) s = op(s, in[lo + k]);
+        if (lo + k < nt) { // public size
+            s = op(s, in[lo + k]);
+            tr.at(1, lo + k);
+        }
     sh[threadIdx.x] = s;
     __syncthreads();
     for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
@@ -119,6 +135,7 @@ __device__ void block_exclusive_scan(const T *__restrict__ in, uint32_t nt, T *_
         if (lo + k < nt) {
             out[lo + k] = run;
             run = op(run, in[lo + k]);
+            tr.at(2, lo + k);
         }
 }
 
@@ -131,19 +148,25 @@ struct FillOp {
 };
 
 // Exclusive "last dimension row" scan over tiles.
+template <bool TRACE>
 __global__ __launch_bounds__(kScanThreads) void k_fill_carry(const Fill *__restrict__ tile_last, uint32_t nt,
-                                                             Fill *__restrict__ carry)
+                                                             Fill *__restrict__ carry, TraceSlot *slot)
 {
     __shared__ Fill sh[kScanThreads];
-    block_exclusive_scan(tile_last, nt, carry, Fill{PAD_KEY, 0}, FillOp{}, sh);
+    dev::AtrT<TRACE> tr(slot);
+    block_exclusive_scan(tile_last, nt, carry, Fill{PAD_KEY, 0}, FillOp{}, sh, tr);
+    tr.done(dev::gtid());
 }
 
 // Per row: (group, value) columns and the matched mask.
+template <bool TRACE>
 __global__ __launch_bounds__(JB) void k_fill_rows(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
                                                   const Fill *__restrict__ carry, uint64_t *__restrict__ col_g,
-                                                  uint64_t *__restrict__ col_v, uint8_t *__restrict__ mask)
+                                                  uint64_t *__restrict__ col_v, uint8_t *__restrict__ mask,
+                                                  TraceSlot *slot)
 {
     __shared__ uint64_t wtk[JB / 64], wtg[JB / 64];
+    dev::AtrT<TRACE> tr(slot);
     const uint64_t t0 = (uint64_t)blockIdx.x * JT + (uint64_t)threadIdx.x * JPER;
     uint64_t lk = PAD_KEY, lg = 0;
 #pragma unroll
@@ -172,6 +195,7 @@ __global__ __launch_bounds__(JB) void k_fill_rows(const uint64_t *__restrict__ k
     }
     __syncthreads();
     Fill c = carry[blockIdx.x];
+    tr.at(4, blockIdx.x);
     for (uint32_t w = 0; w < wv; ++w) {
         const bool has = wtk[w] != PAD_KEY;
         c.key = has ? wtk[w] : c.key;
@@ -194,21 +218,28 @@ __global__ __launch_bounds__(JB) void k_fill_rows(const uint64_t *__restrict__ k
         col_g[i] = c.group;
         col_v[i] = v;
         mask[i] = matched ? 1 : 0;
+        tr.at(5, i); // keys, vals read and the three outputs written at row i
     }
+    tr.done(dev::gtid());
 }
 
 // ------------------------------------------- group-by keys (step 5)
 
+template <bool TRACE>
 __global__ void k_group_keys(const uint64_t *__restrict__ col_g, const uint64_t *__restrict__ col_v,
                              const uint64_t *__restrict__ count, uint64_t n_pad, uint64_t *__restrict__ keys,
-                             uint64_t *__restrict__ vals)
+                             uint64_t *__restrict__ vals, TraceSlot *slot)
 {
+    dev::AtrT<TRACE> tr(slot);
     const uint64_t m = *count;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t g = i < m ? (col_g[i] & 0xFFFFFFFFull) : 0xFFFFFFFFull;
+        const uint64_t gi = col_g[i]; // read for every row: the select below is on the value only
+        const uint64_t g = i < m ? (gi & 0xFFFFFFFFull) : 0xFFFFFFFFull;
         keys[i] = (g << 32) | (i & 0xFFFFFFFFull);
         vals[i] = col_v[i];
+        tr.at(1, i);
     }
+    tr.done(dev::gtid());
 }
 
 // ------------------------------------- segmented sum by group (step 6)
@@ -223,12 +254,14 @@ struct Seg {
 
 __device__ __forceinline__ uint64_t grp(uint64_t k) { return k >> 32; }
 
+template <bool TRACE>
 __global__ __launch_bounds__(JB) void k_seg_tiles(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
-                                                  Seg *__restrict__ tile_seg)
+                                                  Seg *__restrict__ tile_seg, TraceSlot *slot)
 {
     __shared__ uint64_t fg[JB], lg[JB];
     __shared__ int64_t ts[JB];
     __shared__ uint32_t one[JB];
+    dev::AtrT<TRACE> tr(slot);
     const uint64_t t0 = (uint64_t)blockIdx.x * JT + (uint64_t)threadIdx.x * JPER;
     const uint64_t g0 = grp(keys[t0]);
     uint64_t gl = g0;
@@ -238,6 +271,7 @@ __global__ __launch_bounds__(JB) void k_seg_tiles(const uint64_t *__restrict__ k
     for (int k = 0; k < JPER; ++k) {
         const uint64_t g = grp(keys[t0 + k]);
         const int64_t v = (int64_t)vals[t0 + k];
+        tr.at(1, t0 + k);
         run = (g == gl ? run : 0) + v;
         uni &= (g == g0);
         gl = g;
@@ -261,7 +295,9 @@ __global__ __launch_bounds__(JB) void k_seg_tiles(const uint64_t *__restrict__ k
             last = lg[t];
         }
         tile_seg[blockIdx.x] = Seg{last, tail, all};
+        tr.at(2, blockIdx.x);
     }
+    tr.done(dev::gtid());
 }
 
 // Summary of a range of tiles for the segmented carry: group of its first
@@ -283,37 +319,46 @@ struct SegOp {
     }
 };
 
+template <bool TRACE>
 __global__ void k_seg_summ(const Seg *__restrict__ seg, const uint64_t *__restrict__ keys, uint32_t nt,
-                           SegSum *__restrict__ out)
+                           SegSum *__restrict__ out, TraceSlot *slot)
 {
+    dev::AtrT<TRACE> tr(slot);
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
         const Seg s = seg[t];
         out[t] = SegSum{grp(keys[(uint64_t)t * JT]), s.last_g, s.tail_sum, s.all_one, 0u};
+        tr.at(1, t);
     }
+    tr.done(dev::gtid());
 }
 
 // Carry into each tile: the sum of the run of the tile's first group that
 // ends just before the tile (0 when the tiles before end in another group).
+template <bool TRACE>
 __global__ __launch_bounds__(kScanThreads) void k_seg_carry(const SegSum *__restrict__ summ, uint32_t nt,
-                                                            SegSum *__restrict__ pre, int64_t *__restrict__ carry)
+                                                            SegSum *__restrict__ pre, int64_t *__restrict__ carry,
+                                                            TraceSlot *slot)
 {
     __shared__ SegSum sh[kScanThreads];
-    block_exclusive_scan(summ, nt, pre, SegSum{0, 0, 0, 0, 1}, SegOp{}, sh);
+    dev::AtrT<TRACE> tr(slot);
+    block_exclusive_scan(summ, nt, pre, SegSum{0, 0, 0, 0, 1}, SegOp{}, sh, tr);
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < nt; t += kScanThreads) {
         const SegSum p = pre[t];
         carry[t] = (!p.empty && p.last_g == summ[t].first_g) ? p.tail : 0;
+        tr.at(3, t);
     }
+    tr.done(dev::gtid());
 }
 
+template <bool TRACE>
 __global__ __launch_bounds__(JB) void k_seg_rows(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
                                                  const int64_t *__restrict__ carry, uint64_t n_pad,
                                                  uint64_t *__restrict__ col_g, uint64_t *__restrict__ col_s,
-                                                 uint8_t *__restrict__ mask)
+                                                 uint8_t *__restrict__ mask, TraceSlot *slot)
 {
-    __shared__ uint64_t fg[JB], lg[JB];
-    __shared__ int64_t ts[JB];
-    __shared__ uint32_t one[JB];
+    __shared__ SegSum wtot[JB / 64];
+    dev::AtrT<TRACE> tr(slot);
     const uint64_t t0 = (uint64_t)blockIdx.x * JT + (uint64_t)threadIdx.x * JPER;
     const uint64_t g0 = grp(keys[t0]);
     uint64_t gl = g0;
@@ -326,20 +371,38 @@ __global__ __launch_bounds__(JB) void k_seg_rows(const uint64_t *__restrict__ ke
         uni &= (g == g0);
         gl = g;
     }
-    fg[threadIdx.x] = g0;
-    lg[threadIdx.x] = gl;
-    ts[threadIdx.x] = run;
-    one[threadIdx.x] = uni;
-    __syncthreads();
-    // running sum of the group that is open when this thread's rows start
-    const uint64_t tile_first = grp(keys[(uint64_t)blockIdx.x * JT]);
-    int64_t c = carry[blockIdx.x];
-    uint64_t cg = tile_first;
-    for (uint32_t t = 0; t < threadIdx.x; ++t) {
-        const bool cont = one[t] & (fg[t] == cg);
-        c = cont ? c + ts[t] : ts[t];
-        cg = lg[t];
+    // running sum of the group that is open when this thread's rows start:
+    // the tile carry combined with the summaries of the threads before this
+    // one (wavefront inclusive scan, then the wavefront totals through LDS)
+    const SegOp op;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    SegSum x{g0, gl, run, uni, 0u};
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        SegSum y;
+        y.first_g = __shfl_up(x.first_g, off, 64);
+        y.last_g = __shfl_up(x.last_g, off, 64);
+        y.tail = __shfl_up(x.tail, off, 64);
+        y.all_one = __shfl_up(x.all_one, off, 64);
+        y.empty = 0u;
+        const SegSum z = op(y, x);
+        if (lane >= (uint32_t)off) x = z;
     }
+    SegSum e;
+    e.first_g = __shfl_up(x.first_g, 1, 64);
+    e.last_g = __shfl_up(x.last_g, 1, 64);
+    e.tail = __shfl_up(x.tail, 1, 64);
+    e.all_one = __shfl_up(x.all_one, 1, 64);
+    e.empty = lane == 0 ? 1u : 0u;
+    if (lane == 63) wtot[wv] = x;
+    __syncthreads();
+    const uint64_t tile_first = grp(keys[(uint64_t)blockIdx.x * JT]);
+    SegSum pre{tile_first, tile_first, carry[blockIdx.x], 1u, 0u};
+    tr.at(4, blockIdx.x);
+    for (uint32_t w = 0; w < wv; ++w) pre = op(pre, wtot[w]);
+    pre = op(pre, e);
+    int64_t c = pre.tail;
+    uint64_t cg = pre.last_g;
     // rows
     uint64_t prev_g = cg;
     int64_t acc = c;
@@ -354,20 +417,31 @@ __global__ __launch_bounds__(JB) void k_seg_rows(const uint64_t *__restrict__ ke
         col_g[i] = g;
         col_s[i] = (uint64_t)acc;
         mask[i] = (last & (g != 0xFFFFFFFFull)) ? 1 : 0;
+        tr.at(5, i);
     }
+    tr.done(dev::gtid());
 }
 
+template <bool TRACE>
 __global__ void k_group_out(const uint64_t *__restrict__ col_g, const uint64_t *__restrict__ col_s,
-                            const uint64_t *__restrict__ count, uint64_t max_groups, int32_t *__restrict__ out_g,
-                            int64_t *__restrict__ out_s, uint64_t *__restrict__ out_n)
+                            const uint64_t *__restrict__ count, uint64_t n_pad, uint64_t max_groups,
+                            int32_t *__restrict__ out_g, int64_t *__restrict__ out_s, uint64_t *__restrict__ out_n,
+                            TraceSlot *slot)
 {
+    dev::AtrT<TRACE> tr(slot);
     const uint64_t G = *count;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < max_groups; i += (uint64_t)gridDim.x * blockDim.x) {
         const bool live = i < G;
-        out_g[i] = live ? (int32_t)col_g[i] : -1;
-        out_s[i] = live ? (int64_t)col_s[i] : 0;
+        // read for every output slot (row i, or row 0 past the padded size:
+        // both public), so the select below is on values only
+        const uint64_t ri = i < n_pad ? i : 0;
+        const uint64_t g = col_g[ri], s = col_s[ri];
+        out_g[i] = live ? (int32_t)g : -1;
+        out_s[i] = live ? (int64_t)s : 0;
+        tr.at(1, ri);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && out_n) *out_n = G;
+    tr.done(dev::gtid());
 }
 
 // ------------------------------- dense GROUP BY (group ids in [0, G))
@@ -384,12 +458,14 @@ __device__ __forceinline__ int64_t wave_sum64(int64_t v)
     return v;
 }
 
-template <int G>
+template <int G, bool TRACE>
 __global__ __launch_bounds__(256) void k_dense_agg(const uint64_t *__restrict__ col_g, const uint64_t *__restrict__ col_v,
                                                    const uint64_t *__restrict__ count, uint64_t n_pad,
-                                                   int64_t *__restrict__ part_sum, int64_t *__restrict__ part_cnt)
+                                                   int64_t *__restrict__ part_sum, int64_t *__restrict__ part_cnt,
+                                                   TraceSlot *slot)
 {
     __shared__ int64_t rs[4][G], rc[4][G];
+    dev::AtrT<TRACE> tr(slot);
     const uint64_t m = *count;
     int64_t acc[G];
     uint32_t cnt[G]; // rows per thread stay far below 2^32
@@ -401,6 +477,7 @@ __global__ __launch_bounds__(256) void k_dense_agg(const uint64_t *__restrict__ 
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * 256) {
         const uint64_t grp_i = col_g[i] & 0xFFFFFFFFull;
         const int64_t v = (int64_t)col_v[i];
+        tr.at(1, i);
         const bool live = i < m;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
@@ -423,7 +500,9 @@ __global__ __launch_bounds__(256) void k_dense_agg(const uint64_t *__restrict__ 
         const uint32_t g = threadIdx.x;
         part_sum[(uint64_t)blockIdx.x * G + g] = rs[0][g] + rs[1][g] + rs[2][g] + rs[3][g];
         part_cnt[(uint64_t)blockIdx.x * G + g] = rc[0][g] + rc[1][g] + rc[2][g] + rc[3][g];
+        tr.at(2, (uint64_t)blockIdx.x * G + g);
     }
+    tr.done(dev::gtid());
 }
 
 // One block: totals per group (fixed summation order), then output slot t
@@ -432,10 +511,12 @@ template <int G>
 __global__ __launch_bounds__(256) void k_dense_out(const int64_t *__restrict__ part_sum,
                                                    const int64_t *__restrict__ part_cnt, uint32_t nparts,
                                                    uint64_t max_groups, int32_t *__restrict__ out_g,
-                                                   int64_t *__restrict__ out_s, uint64_t *__restrict__ out_n)
+                                                   int64_t *__restrict__ out_s, uint64_t *__restrict__ out_n,
+                                                   TraceSlot *slot)
 {
     __shared__ int64_t ts[G], tc[G];
     __shared__ int64_t ws[4][G], wc[4][G];
+    dev::Atr tr(slot);
     const uint32_t t = threadIdx.x;
     // thread t sums partial rows t, t+256, ...; then a block reduction per group
     // (a thread's column of partials is contiguous: coalesced across lanes)
@@ -448,6 +529,7 @@ __global__ __launch_bounds__(256) void k_dense_out(const int64_t *__restrict__ p
             for (int k = 0; k < 16; ++k) {
                 s[k] += part_sum[(uint64_t)p * G + g0 + k];
                 c[k] += part_cnt[(uint64_t)p * G + g0 + k];
+                tr.at(1, (uint64_t)p * G + g0 + k);
             }
         }
 #pragma unroll
@@ -468,20 +550,22 @@ __global__ __launch_bounds__(256) void k_dense_out(const int64_t *__restrict__ p
     __syncthreads();
     uint64_t present = 0;
     for (int g = 0; g < G; ++g) present += tc[g] > 0;
-    for (uint64_t slot = t; slot < max_groups; slot += blockDim.x) {
+    for (uint64_t j = t; j < max_groups; j += blockDim.x) {
         int32_t og = -1;
         int64_t os = 0;
         uint64_t rank = 0;
         for (int g = 0; g < G; ++g) {
-            const bool here = (tc[g] > 0) & (rank == slot);
+            const bool here = (tc[g] > 0) & (rank == j);
             og = here ? g : og;
             os = here ? ts[g] : os;
             rank += tc[g] > 0;
         }
-        out_g[slot] = og;
-        out_s[slot] = os;
+        out_g[j] = og;
+        out_s[j] = os;
+        tr.at(2, j);
     }
     if (t == 0 && out_n) *out_n = present;
+    tr.done(dev::gtid());
 }
 
 dim3 grid_for(uint64_t n) { return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 65536))); }
@@ -522,18 +606,25 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
     uint64_t *cnt1 = (uint64_t *)(base + o_c1), *cnt2 = (uint64_t *)(base + o_c2);
 
     // 1-2. union and sort
-    launch(ctx, LaunchInfo{"join_build", grid_for(n_pad), dim3(256), 0, n_pad}, k_join_build, a.dim_key, a.dim_group,
-           a.n_dim, a.fact_fk, a.fact_val, a.n_fact, n_pad, keys, vals);
+    launch(ctx, LaunchInfo{"join_build", grid_for(n_pad), dim3(256), 0, n_pad},
+           ctx->tracing ? k_join_build<true> : k_join_build<false>,
+           a.dim_key, a.dim_group,
+           a.n_dim, a.fact_fk, a.fact_val, a.n_fact, n_pad, keys, vals, trace_slot(ctx));
     sort_kv(ctx, keys, vals, n_pad);
     if (a.sorted_keys) VO_HIP(hipMemcpyAsync(a.sorted_keys, keys, n_pad * 8, hipMemcpyDeviceToDevice, ctx->stream));
     if (a.sorted_vals) VO_HIP(hipMemcpyAsync(a.sorted_vals, vals, n_pad * 8, hipMemcpyDeviceToDevice, ctx->stream));
     // 3. forward fill
-    launch(ctx, LaunchInfo{"join_fill_tiles", dim3(nt), dim3(JB), 0, n_pad}, k_fill_tiles, (const uint64_t *)keys,
-           (const uint64_t *)vals, tile_last);
-    launch(ctx, LaunchInfo{"join_fill_carry", dim3(1), dim3(kScanThreads), 0, nt}, k_fill_carry,
-           (const Fill *)tile_last, nt, fcarry);
-    launch(ctx, LaunchInfo{"join_fill_rows", dim3(nt), dim3(JB), 0, n_pad}, k_fill_rows, (const uint64_t *)keys,
-           (const uint64_t *)vals, (const Fill *)fcarry, col_a, col_b, mask);
+    launch(ctx, LaunchInfo{"join_fill_tiles", dim3(nt), dim3(JB), 0, n_pad},
+           ctx->tracing ? k_fill_tiles<true> : k_fill_tiles<false>,
+           (const uint64_t *)keys,
+           (const uint64_t *)vals, tile_last, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"join_fill_carry", dim3(1), dim3(kScanThreads), 0, nt},
+           ctx->tracing ? k_fill_carry<true> : k_fill_carry<false>,
+           (const Fill *)tile_last, nt, fcarry, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"join_fill_rows", dim3(nt), dim3(JB), 0, n_pad},
+           ctx->tracing ? k_fill_rows<true> : k_fill_rows<false>,
+           (const uint64_t *)keys,
+           (const uint64_t *)vals, (const Fill *)fcarry, col_a, col_b, mask, trace_slot(ctx));
     // 4. compaction of matched rows
     compact_rows2(ctx, col_a, col_b, mask, n_pad, cnt1);
     if (a.dense_groups) {
@@ -544,39 +635,53 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
         const uint32_t G = a.max_groups <= 32 ? 32 : 64;
         if (G == 32) {
             launch(ctx, LaunchInfo{"group_dense_agg", dim3(nb), dim3(256), 0, hash_mix(n_pad, 32)},
-                   k_dense_agg<32>, (const uint64_t *)col_a, (const uint64_t *)col_b, (const uint64_t *)cnt1, n_pad,
-                   part, part + (uint64_t)nb * 32);
+                   ctx->tracing ? k_dense_agg<32, true> : k_dense_agg<32, false>, (const uint64_t *)col_a,
+                   (const uint64_t *)col_b, (const uint64_t *)cnt1, n_pad, part, part + (uint64_t)nb * 32,
+                   trace_slot(ctx));
             launch(ctx, LaunchInfo{"group_dense_out", dim3(1), dim3(256), 0, a.max_groups}, k_dense_out<32>,
                    (const int64_t *)part, (const int64_t *)(part + (uint64_t)nb * 32), nb, a.max_groups,
-                   a.out_group, a.out_sum, a.out_n_groups);
+                   a.out_group, a.out_sum, a.out_n_groups, trace_slot(ctx));
         } else {
             launch(ctx, LaunchInfo{"group_dense_agg", dim3(nb), dim3(256), 0, hash_mix(n_pad, 64)},
-                   k_dense_agg<64>, (const uint64_t *)col_a, (const uint64_t *)col_b, (const uint64_t *)cnt1, n_pad,
-                   part, part + (uint64_t)nb * 64);
+                   ctx->tracing ? k_dense_agg<64, true> : k_dense_agg<64, false>, (const uint64_t *)col_a,
+                   (const uint64_t *)col_b, (const uint64_t *)cnt1, n_pad, part, part + (uint64_t)nb * 64,
+                   trace_slot(ctx));
             launch(ctx, LaunchInfo{"group_dense_out", dim3(1), dim3(256), 0, a.max_groups}, k_dense_out<64>,
                    (const int64_t *)part, (const int64_t *)(part + (uint64_t)nb * 64), nb, a.max_groups,
-                   a.out_group, a.out_sum, a.out_n_groups);
+                   a.out_group, a.out_sum, a.out_n_groups, trace_slot(ctx));
         }
         return;
     }
     // 5. group-by sort
-    launch(ctx, LaunchInfo{"group_keys", grid_for(n_pad), dim3(256), 0, n_pad}, k_group_keys, (const uint64_t *)col_a,
-           (const uint64_t *)col_b, (const uint64_t *)cnt1, n_pad, keys, vals);
+    launch(ctx, LaunchInfo{"group_keys", grid_for(n_pad), dim3(256), 0, n_pad},
+           ctx->tracing ? k_group_keys<true> : k_group_keys<false>,
+           (const uint64_t *)col_a,
+           (const uint64_t *)col_b, (const uint64_t *)cnt1, n_pad, keys, vals, trace_slot(ctx));
     sort_kv(ctx, keys, vals, n_pad);
     // 6. segmented sum
-    launch(ctx, LaunchInfo{"group_seg_tiles", dim3(nt), dim3(JB), 0, n_pad}, k_seg_tiles, (const uint64_t *)keys,
-           (const uint64_t *)vals, seg);
-    launch(ctx, LaunchInfo{"group_seg_summ", grid_for(nt), dim3(256), 0, nt}, k_seg_summ, (const Seg *)seg,
-           (const uint64_t *)keys, nt, ssum);
-    launch(ctx, LaunchInfo{"group_seg_carry", dim3(1), dim3(kScanThreads), 0, nt}, k_seg_carry, (const SegSum *)ssum,
-           nt, spre, scarry);
-    launch(ctx, LaunchInfo{"group_seg_rows", dim3(nt), dim3(JB), 0, n_pad}, k_seg_rows, (const uint64_t *)keys,
-           (const uint64_t *)vals, (const int64_t *)scarry, n_pad, col_a, col_b, mask);
+    launch(ctx, LaunchInfo{"group_seg_tiles", dim3(nt), dim3(JB), 0, n_pad},
+           ctx->tracing ? k_seg_tiles<true> : k_seg_tiles<false>,
+           (const uint64_t *)keys,
+           (const uint64_t *)vals, seg, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"group_seg_summ", grid_for(nt), dim3(256), 0, nt},
+           ctx->tracing ? k_seg_summ<true> : k_seg_summ<false>,
+           (const Seg *)seg,
+           (const uint64_t *)keys, nt, ssum, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"group_seg_carry", dim3(1), dim3(kScanThreads), 0, nt},
+           ctx->tracing ? k_seg_carry<true> : k_seg_carry<false>,
+           (const SegSum *)ssum,
+           nt, spre, scarry, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"group_seg_rows", dim3(nt), dim3(JB), 0, n_pad},
+           ctx->tracing ? k_seg_rows<true> : k_seg_rows<false>,
+           (const uint64_t *)keys,
+           (const uint64_t *)vals, (const int64_t *)scarry, n_pad, col_a, col_b, mask, trace_slot(ctx));
     // 7. one row per group
     compact_rows2(ctx, col_a, col_b, mask, n_pad, cnt2);
     launch(ctx, LaunchInfo{"group_out", grid_for(std::max<uint64_t>(a.max_groups, 1)), dim3(256), 0, a.max_groups},
-           k_group_out, (const uint64_t *)col_a, (const uint64_t *)col_b, (const uint64_t *)cnt2, a.max_groups,
-           a.out_group, a.out_sum, a.out_n_groups);
+
+           ctx->tracing ? k_group_out<true> : k_group_out<false>,
+           (const uint64_t *)col_a, (const uint64_t *)col_b, (const uint64_t *)cnt2, n_pad, a.max_groups,
+           a.out_group, a.out_sum, a.out_n_groups, trace_slot(ctx));
 }
 
 } // namespace vo

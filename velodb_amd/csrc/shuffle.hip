@@ -36,22 +36,28 @@ __device__ __forceinline__ uint64_t philox_tag(uint64_t seed, uint64_t i)
 }
 
 __global__ void k_shuffle_tags(uint64_t seed, const uint32_t *__restrict__ payload, uint64_t n_pad,
-                               uint64_t *__restrict__ keys, uint64_t *__restrict__ vals)
+                               uint64_t *__restrict__ keys, uint64_t *__restrict__ vals, TraceSlot *slot)
 {
+    dev::Atr tr(slot);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
         keys[i] = philox_tag(seed, i);
         vals[i] = (i << 32) | payload[i];
+        tr.at(1, i);
     }
+    tr.done(dev::gtid());
 }
 
 __global__ void k_shuffle_unpack(const uint64_t *__restrict__ vals, uint64_t n_pad, uint32_t *__restrict__ payload,
-                                 uint32_t *__restrict__ perm_out)
+                                 uint32_t *__restrict__ perm_out, TraceSlot *slot)
 {
+    dev::Atr tr(slot);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t v = vals[i];
         payload[i] = (uint32_t)v;
         if (perm_out) perm_out[i] = (uint32_t)(v >> 32);
+        tr.at(1, i);
     }
+    tr.done(dev::gtid());
 }
 
 dim3 grid_for(uint64_t n) { return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 65536))); }
@@ -65,10 +71,10 @@ void op_shuffle(vo_ctx *ctx, const vo_shuffle_args &a)
     uint64_t *keys = (uint64_t *)scratch(ctx, 16 * a.n_pad + 256);
     uint64_t *vals = keys + a.n_pad;
     launch(ctx, LaunchInfo{"shuffle_tags", grid_for(a.n_pad), dim3(256), 0, a.n_pad}, k_shuffle_tags, a.seed,
-           (const uint32_t *)a.payload, a.n_pad, keys, vals);
+           (const uint32_t *)a.payload, a.n_pad, keys, vals, trace_slot(ctx));
     sort_kv(ctx, keys, vals, a.n_pad);
     launch(ctx, LaunchInfo{"shuffle_unpack", grid_for(a.n_pad), dim3(256), 0, a.n_pad}, k_shuffle_unpack,
-           (const uint64_t *)vals, a.n_pad, a.payload, a.perm_out);
+           (const uint64_t *)vals, a.n_pad, a.payload, a.perm_out, trace_slot(ctx));
 }
 
 } // namespace vo

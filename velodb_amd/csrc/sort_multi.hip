@@ -100,9 +100,11 @@ struct PackSpec {
 // writes index = row.  Pad rows take the image of max for ASC columns and
 // of lowest for DESC columns, the values sort.cuh:79-100 writes.
 __global__ void k_pack_keys(PackSpec ps, uint64_t *w0, uint64_t *w1, uint64_t *w2, uint64_t *w3, uint64_t *w4,
-                            uint64_t *w5, uint64_t *w6, uint64_t *w7, int64_t *idx, uint64_t n, uint64_t n_pad)
+                            uint64_t *w5, uint64_t *w6, uint64_t *w7, int64_t *idx, uint64_t n, uint64_t n_pad,
+                            TraceSlot *slot)
 {
     uint64_t *ws[8] = {w0, w1, w2, w3, w4, w5, w6, w7};
+    dev::Atr tr(slot);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const bool pad = i >= n;
@@ -116,14 +118,18 @@ __global__ void k_pack_keys(PackSpec ps, uint64_t *w0, uint64_t *w1, uint64_t *w
         }
         for (uint32_t w = 0; w < ps.nwrite; ++w) ws[w][i] = acc[w];
         idx[i] = (int64_t)i;
+        tr.at(1, src); // columns read at row src (row 0 for pad rows: n is public)
+        tr.at(2, i);   // key words and index written at row i
     }
+    tr.done(dev::gtid());
 }
 
 __global__ void k_unpack_keys(PackSpec ps, const uint64_t *w0, const uint64_t *w1, const uint64_t *w2,
                               const uint64_t *w3, const uint64_t *w4, const uint64_t *w5, const uint64_t *w6,
-                              const uint64_t *w7, uint64_t n_pad)
+                              const uint64_t *w7, uint64_t n_pad, TraceSlot *slot)
 {
     const uint64_t *ws[8] = {w0, w1, w2, w3, w4, w5, w6, w7};
+    dev::Atr tr(slot);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
         for (uint32_t c = 0; c < ps.ncols; ++c) {
             const uint64_t mask = ps.bits[c] == 64 ? ~0ull : ((1ull << ps.bits[c]) - 1);
@@ -131,7 +137,9 @@ __global__ void k_unpack_keys(PackSpec ps, const uint64_t *w0, const uint64_t *w
             v = ps.desc[c] ? (~v & mask) : v;
             denorm_store(const_cast<void *>(ps.cols[c]), ps.types[c], i, v);
         }
+        tr.at(1, i);
     }
+    tr.done(dev::gtid());
 }
 
 uint32_t type_bits(uint32_t t)
@@ -209,7 +217,7 @@ void op_sort_multi(vo_ctx *ctx, const vo_sort_args &a)
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n_pad + 255) / 256, 8192));
     LaunchInfo lp{"sort_pack", dim3(blocks), dim3(256), 0, hash_mix(ps.ncols, ps.nwords)};
     launch(ctx, lp, k_pack_keys, ps, wp[0], wp[1], wp[2], wp[3], wp[4], wp[5], wp[6], wp[7], a.indices, a.n_rows,
-           n_pad);
+           n_pad, trace_slot(ctx));
     const int n_log = ilog2(n_pad);
     switch (W) {
     case 1: {
@@ -242,7 +250,8 @@ void op_sort_multi(vo_ctx *ctx, const vo_sort_args &a)
     }
     }
     LaunchInfo lu{"sort_unpack", dim3(blocks), dim3(256), 0, hash_mix(ps.ncols, ps.nwords)};
-    launch(ctx, lu, k_unpack_keys, ps, wp[0], wp[1], wp[2], wp[3], wp[4], wp[5], wp[6], wp[7], n_pad);
+    launch(ctx, lu, k_unpack_keys, ps, wp[0], wp[1], wp[2], wp[3], wp[4], wp[5], wp[6], wp[7], n_pad,
+           trace_slot(ctx));
 }
 
 } // namespace vo

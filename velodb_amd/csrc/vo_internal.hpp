@@ -245,6 +245,48 @@ __device__ __forceinline__ void tcommit(TraceSlot *slot, uint64_t gtid, uint64_t
     atomicAdd(&slot->count, (unsigned long long)n);
 }
 
+// One thread's address trace, gated at run time (slot == nullptr when the
+// context is not tracing): at() folds the index of each global access where
+// it happens, done() commits the thread's hash to the launch slot.
+struct Atr {
+    TraceSlot *slot;
+    uint64_t h = 0, n = 0;
+    __device__ explicit Atr(TraceSlot *s) : slot(s) {}
+    __device__ __forceinline__ void at(uint64_t tag, uint64_t idx)
+    {
+        if (slot) {
+            h = tfold(h, tag, idx);
+            ++n;
+        }
+    }
+    __device__ __forceinline__ void done(uint64_t gtid)
+    {
+        if (slot) tcommit(slot, gtid, h, n);
+    }
+};
+
+// The same with the gate fixed at compile time, for kernels whose register
+// budget is tight (ON = false compiles to nothing).
+template <bool ON>
+struct AtrT {
+    TraceSlot *slot;
+    uint64_t h = 0, n = 0;
+    __device__ explicit AtrT(TraceSlot *s) : slot(s) {}
+    __device__ __forceinline__ void at(uint64_t tag, uint64_t idx)
+    {
+        if constexpr (ON) {
+            h = tfold(h, tag, idx);
+            ++n;
+        }
+    }
+    __device__ __forceinline__ void done(uint64_t gtid)
+    {
+        if constexpr (ON) tcommit(slot, gtid, h, n);
+    }
+};
+
+__device__ __forceinline__ uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
