@@ -283,6 +283,7 @@ def bench_batch(args):
     run(batched, max(1, args.warmup))
     tb = run(batched, args.steps)
     kb, pb = keys.clone(), pay.clone()
+    torch.cuda.synchronize()
     tp = run(per_instance, max(1, min(args.steps, 3)))
     same = bool(torch.equal(kb, keys) and torch.equal(pb, pay))
     _line("batched oblivious sorts (Mrec/s over padded records)", padded / tb / 1e6, "Mrec/s", args.steps,
@@ -474,7 +475,7 @@ def main():
     fams = {}
     # one family per kernel instantiation, so each maps to one rocprof kernel name
     for f in (["sort_local", "sort_merge", "gen_sort"] + [f"sort_global_s{s}" for s in range(1, 6)] +
-              ["sort_hybrid_s6", "sort_hybrid_s7"]):
+              ["sort_hybrid_s6", "sort_hybrid_s7", "sort_hybrid_s8"]):
         cnt, ms = ctx.kernel_stats(f)
         if cnt:
             fams[f] = (cnt, ms)

@@ -293,7 +293,8 @@ enum {
 typedef struct vo_config {
     uint32_t queue_capacity; /* ring entries, power of two; 0 -> 1024 (task_manager.hpp:14) */
     uint32_t flags;          /* VO_CFG_* */
-    void *stream;            /* hipStream_t to run on; NULL -> the context creates one */
+    void *stream;            /* hipStream_t to run on; NULL -> the context creates a blocking stream,
+                                ordered with work on the null (default) stream */
     int32_t device;          /* HIP device ordinal; -1 -> current */
     uint32_t wait_timeout_ms;/* default timeout of vo_wait when 0 is passed; 0 -> 60000 */
     uint32_t tuning;         /* 0 -> autotuned table / built-in defaults */

@@ -85,3 +85,34 @@ def test_autotuner_choices_are_valid_and_used(vctx):
         for (off, n, n_pad) in items:
             ek, ep = oracle.bitonic_sort_kp(keys[off:off + n], pay[off:off + n], n, n_pad)
             assert np.array_equal(gk[off:off + n_pad], ek) and np.array_equal(gp[off:off + n_pad], ep)
+
+
+def test_batch_config4_full_size_equals_per_instance(vctx):
+    """Config 4 at full size: 4096 sorts, sizes log-uniform in [2^10, 2^20]
+    (numpy default_rng(0)), 1.27e9 buffer rows: the batched schedule gives
+    the same bytes as one sort per instance (both against device-generated
+    inputs), and every instance comes out sorted."""
+    import torch
+    rng = np.random.default_rng(0)
+    sizes = list(np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 20), 4096)).astype(int))
+    items, total = _layout(sizes)
+    keys = torch.empty(total, dtype=torch.int64, device="cuda:0")
+    pay = torch.empty(total, dtype=torch.int32, device="cuda:0")
+    vctx.gen_sort_input(7, keys, pay, total)
+    vctx.sort_batch(keys, pay, items)
+    torch.cuda.synchronize()
+    kb, pb = keys.clone(), pay.clone()
+    torch.cuda.synchronize()
+    vctx.gen_sort_input(7, keys, pay, total)
+    for (off, n, n_pad) in items:
+        vctx.sort_kp_async(keys[off:], pay[off:], n, n_pad)
+    vctx.sync()
+    torch.cuda.synchronize()
+    assert torch.equal(kb, keys) and torch.equal(pb, pay)
+    # sortedness of the live rows of a spread of instances (unsigned order)
+    sign = -(2 ** 63)
+    for (off, n, n_pad) in items[::97]:
+        k = kb[off:off + n] ^ sign
+        assert bool(torch.all(k[1:] >= k[:-1]))
+    del kb, pb, keys, pay
+    torch.cuda.empty_cache()

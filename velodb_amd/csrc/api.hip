@@ -237,7 +237,9 @@ int vo_ctx_create(const vo_config *cfg, vo_ctx **out)
         if (c.stream) {
             ctx->stream = (hipStream_t)c.stream;
         } else {
-            VO_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+            // a blocking stream: work the caller queues on the legacy null
+            // stream (a framework's default stream) stays ordered with it
+            VO_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamDefault));
             ctx->own_stream = true;
         }
         hipDeviceProp_t prop;

@@ -356,7 +356,7 @@ __device__ __forceinline__ void body_hybrid(Bufs<R> b, IMap map, int p, int q_lo
     using TL = TileCT<R, T_LOG, R_LOG, TRACE>;
     constexpr int C = T_LOG - S;
     constexpr int A0 = T_LOG - R_LOG;
-    static_assert(C >= 6 && A0 >= 6, "runs and lanes must cover 64 contiguous records");
+    static_assert(C >= 5 && A0 >= 6, "runs of at least 32 records; lanes cover 64 tile-local records");
     const uint32_t tau = threadIdx.x;
     const uint32_t lane = tau & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tau >> 6);
@@ -368,16 +368,20 @@ __device__ __forceinline__ void body_hybrid(Bufs<R> b, IMap map, int p, int q_lo
     auto eoff = [&](uint64_t l) { return (l & ((1ull << C) - 1)) | ((l >> C) << q_lo); };
     const uint64_t vw = ublk + eoff((uint64_t)wave << 6);
     const uint64_t pw = map_uniform(map, vw);
+    // per-lane offset: with runs of 64 or more the wavefront is one run; with
+    // runs of 32 its upper half is the next run, 2^q_lo records further
+    // (below 2^29 elements, so the byte offset fits the 32-bit field)
+    const uint32_t vl = C >= 6 ? lane : ((lane & 31u) | ((lane >> 5) << q_lo));
     uint64_t h = 0;
     uint32_t nops = 0;
     R r[1 << R_LOG];
 #pragma unroll
-    for (int m = 0; m < (1 << R_LOG); ++m) gload_u<R, TRACE>(r[m], b, pw + eoff((uint64_t)m << A0), lane, h, nops);
+    for (int m = 0; m < (1 << R_LOG); ++m) gload_u<R, TRACE>(r[m], b, pw + eoff((uint64_t)m << A0), vl, h, nops);
     const bool asc = dir_asc(ublk, p, p_top);
     TL::template level<-1, T_LOG - 1, A0, C>(r, lds, tau, asc, p_top, h, nops);
     TL::template relayout<TL::end_layout(T_LOG - 1, A0, C), A0>(r, lds, tau, h, nops);
 #pragma unroll
-    for (int m = 0; m < (1 << R_LOG); ++m) gstore_u<R, TRACE>(r[m], b, pw + eoff((uint64_t)m << A0), lane, h, nops);
+    for (int m = 0; m < (1 << R_LOG); ++m) gstore_u<R, TRACE>(r[m], b, pw + eoff((uint64_t)m << A0), vl, h, nops);
     if constexpr (TRACE) tcommit(slot, blk * BLOCK + tau, h, nops);
 }
 
@@ -500,7 +504,7 @@ inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log,
 template <class R, int T_LOG, int R_LOG, int BLOCK, int S_MAX, int H_MAX = 0>
 struct SortImpl {
     static_assert(T_LOG >= 8, "global passes need q_lo >= 8 (256-thread blocks)");
-    static_assert(H_MAX == 0 || (T_LOG - H_MAX >= 6 && BLOCK << R_LOG == 1 << T_LOG), "hybrid tile shape");
+    static_assert(H_MAX == 0 || (T_LOG - H_MAX >= 5 && BLOCK << R_LOG == 1 << T_LOG), "hybrid tile shape");
 
     static int smax(vo_ctx *ctx) { return std::max(1, std::min(S_MAX, ctx->tuning.max_fused_stages)); }
     static int hmax(vo_ctx *ctx) { return std::min(H_MAX, ctx->tuning.max_hybrid_stages); }
@@ -508,10 +512,10 @@ struct SortImpl {
     template <bool TRACE, int S>
     static void launch_hybrid(vo_ctx *ctx, Bufs<R> b, IMap map, int p, int q_lo, int p_top, uint64_t n_virtual)
     {
-        if constexpr (S <= H_MAX && T_LOG - S >= 6) {
+        if constexpr (S <= H_MAX && T_LOG - S >= 5) {
             using TL = dev::TileCT<R, T_LOG, R_LOG, TRACE>;
             const size_t lds = TL::lds_bytes();
-            static const char *const names[] = {"sort_hybrid_s6", "sort_hybrid_s7"};
+            static const char *const names[] = {"sort_hybrid_s6", "sort_hybrid_s7", "sort_hybrid_s8"};
             LaunchInfo li{names[S - 6], dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BLOCK), (uint32_t)lds,
                           hash_mix(hash_mix(p, q_lo), hash_mix(S + 100, p_top))};
             auto k = dev::k_sort_hybrid<R, T_LOG, R_LOG, BLOCK, S, TRACE>;
@@ -581,6 +585,7 @@ struct SortImpl {
                 switch (ps.s) {
                 case 6: launch_hybrid<TRACE, 6>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
                 case 7: launch_hybrid<TRACE, 7>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
+                case 8: launch_hybrid<TRACE, 8>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
                 default: throw Error{VO_ERR_INVALID, "bad hybrid stage count"};
                 }
                 break;

@@ -7,7 +7,7 @@ namespace vo {
 using namespace sortimpl;
 
 namespace {
-using SortKP = SortImpl<RecKP, 13, 4, 512, 5, 7>;
+using SortKP = SortImpl<RecKP, 13, 4, 512, 5, 8>;
 using SortKV = SortImpl<RecKV, 12, 4, 256, 4, 6>;
 } // namespace
 
@@ -111,8 +111,8 @@ void autotune_sort(vo_ctx *ctx, int log_min, int log_max, uint64_t rows_per_clas
 {
     VO_CHECK(log_min >= 4 && log_max <= 29 && log_min <= log_max, VO_ERR_INVALID, "autotune class range 4..29");
     static const uint32_t candidates[] = {
-        0x5 | (8u << 4),         0x5 | (8u << 4) | 0x100, 0x5 | (7u << 4), 0x5 | (1u << 4),
-        0x4 | (8u << 4),         0x4 | (1u << 4),         0x3 | (8u << 4), 0x5 | (1u << 4) | 0x100,
+        0x5 | (9u << 4),         0x5 | (9u << 4) | 0x100, 0x5 | (8u << 4), 0x5 | (7u << 4),
+        0x5 | (1u << 4),         0x4 | (9u << 4),         0x4 | (1u << 4), 0x3 | (9u << 4),
     };
     for (int c = log_min; c <= log_max; ++c) {
         const uint64_t inst = 1ull << c;
