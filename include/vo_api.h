@@ -77,8 +77,8 @@ typedef enum vo_opcode {
     VO_OP_GATHER = 2,                   /* out[pos[i]] = in[i] if mask (data-dependent writes) */
     VO_OP_SORT = 3,                     /* multi-column bitonic sort, reference semantics */
     VO_OP_PERMUTE = 4,                  /* out[i] = in[idx[i]] (data-dependent reads) */
-    VO_OP_SORT_MERGE_JOIN_COUNT = 5,    /* unsupported: replaced by VO_OP_PKFK_JOIN_GROUPSUM */
-    VO_OP_SORT_MERGE_JOIN_WRITE = 6,    /* unsupported */
+    VO_OP_SORT_MERGE_JOIN_COUNT = 5,    /* oblivious equi-join: sort, count the pairs (vo_join_pairs_args) */
+    VO_OP_SORT_MERGE_JOIN_WRITE = 6,    /* oblivious equi-join: write the row-index pairs */
     VO_OP_HASH_JOIN_BUILD = 7,          /* unsupported: non-oblivious on the device */
     VO_OP_HASH_JOIN_COUNT = 8,          /* unsupported */
     VO_OP_HASH_JOIN_WRITE = 9,          /* unsupported */
@@ -243,6 +243,40 @@ typedef struct vo_join_args {
                               whose group is outside the bound are not counted. */
 } vo_join_args;
 
+/* OP_SORT_MERGE_JOIN_COUNT / OP_SORT_MERGE_JOIN_WRITE: oblivious equi-join
+ * of two key columns into row-index pairs; replaces the non-oblivious
+ * SortMergeJoinCountArgs / SortMergeJoinWriteArgs pair
+ * (include/cuda/commands.hpp:82-98, include/cuda/sort_merge_join.cuh:104-172).
+ * The output is every (l, r) with left_keys[l] == right_keys[r], as row
+ * indices (the rowids the reference's operator fills with 0..n-1,
+ * src/operator/sort_merge_join_operator.cpp:75-92), ordered by (key, l, r).
+ * Inputs need not be sorted.  Key types: BOOLEAN, TINYINT, SMALLINT,
+ * INTEGER, BIGINT, DATE, VARCHAR (ordinal) and UBIGINT.
+ *   COUNT: sorts the union of the n_left + n_right rows (padded to a power
+ *     of two >= 4096, at most 2^29), derives every row's match counts and
+ *     output positions, and compacts the matched rows; *out_count (device)
+ *     receives the number of pairs n_out.  The routing state stays in the
+ *     context until the next COUNT.
+ *   WRITE: expands the state of the last COUNT with the same key pointers
+ *     and sizes into out_left / out_right (device, out_cap rows each,
+ *     out_cap >= n_out, n_out <= 2^29); rows [n_out, out_cap) are -1.
+ * Every pass reads and writes every row of its padded size; n_out is the
+ * only data-dependent value that leaves the device (the reference returns
+ * it too).  Uses about 90 bytes of device memory per padded union row and
+ * 48 per padded output row. */
+typedef struct vo_join_pairs_args {
+    const void *left_keys;
+    uint64_t n_left;
+    const void *right_keys;
+    uint64_t n_right;
+    uint32_t type_id;    /* vo_type of both key columns */
+    uint32_t reserved;   /* 0 */
+    uint64_t *out_count; /* COUNT: device, receives n_out (may be NULL) */
+    int64_t *out_left;   /* WRITE: device, out_cap rows */
+    int64_t *out_right;  /* WRITE: device, out_cap rows */
+    uint64_t out_cap;
+} vo_join_pairs_args;
+
 /* OP_SORT_BATCH: independent sorts in one schedule.  Item i sorts
  * (keys + offset, payload + offset) over n rows padded to n_pad (a power of
  * two; offset must be a multiple of n_pad).  Items are bucketed by n_pad. */
@@ -273,6 +307,7 @@ typedef union vo_command_args {
     vo_filter_args filter;
     vo_join_args join;
     vo_sort_batch_args sort_batch;
+    vo_join_pairs_args join_pairs; /* OP_SORT_MERGE_JOIN_COUNT / _WRITE */
 } vo_command_args;
 
 /* Mirror of Command (commands.hpp:152-156). */
@@ -340,6 +375,10 @@ VO_EXPORT int vo_shuffle(vo_ctx *ctx, const vo_shuffle_args *args);
 VO_EXPORT int vo_filter_sum(vo_ctx *ctx, const vo_filter_args *args);
 VO_EXPORT int vo_pkfk_join_groupsum(vo_ctx *ctx, const vo_join_args *args);
 VO_EXPORT int vo_sort_batch(vo_ctx *ctx, const vo_sort_batch_args *args);
+/* OP_SORT_MERGE_JOIN_COUNT then OP_SORT_MERGE_JOIN_WRITE; *n_out (host, may
+ * be NULL) receives the pair count after COUNT. */
+VO_EXPORT int vo_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args *args, uint64_t *n_out);
+VO_EXPORT int vo_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args *args);
 
 /* Asynchronous forms: enqueue on the context stream and return. */
 VO_EXPORT int vo_sort_kp_async(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad);

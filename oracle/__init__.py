@@ -13,6 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 REF_KAT = os.path.join(_HERE, "_ref", "kks_test_sorts")
+REF_DIST = os.path.join(_HERE, "_ref", "kks_test_obliv_distribute")
 
 _lib = None
 
@@ -65,6 +66,10 @@ def lib():
         L.orc_union_key.argtypes = [i32, C.c_int, u32]
         L.orc_join_groupsum.restype = sz
         L.orc_join_groupsum.argtypes = [P, P, sz, P, P, sz, sz, P, P, P, P, sz]
+        L.orc_distribute.restype = C.c_int
+        L.orc_distribute.argtypes = [P, P, sz, sz, P, P]
+        L.orc_join_pairs.restype = sz
+        L.orc_join_pairs.argtypes = [P, sz, P, sz, P, P, sz, P]
         _lib = L
     return _lib
 
@@ -195,6 +200,48 @@ def join_groupsum(dk, dg, fk, fv, npad, max_groups=64, want_sorted=False):
                                 _p(og), _p(os_), max_groups)
     out = (og[:G].copy(), os_[:G].copy())
     return out + (sk, sv) if want_sorted else out
+
+
+# Order-preserving 64-bit codes of the join key types (vo_type ids): signed
+# types flip the sign bit, unsigned ones widen.
+_SIGNED = {2: np.int8, 3: np.int16, 4: np.int32, 5: np.int64}
+_UNSIGNED = {1: np.uint8, 10: np.uint32, 8: np.uint64, 64: np.uint64}
+
+
+def key_codes(keys, type_id):
+    if type_id in _SIGNED:
+        v = np.asarray(keys, dtype=_SIGNED[type_id]).astype(np.int64).view(np.uint64)
+        return v ^ np.uint64(1 << 63)
+    if type_id in _UNSIGNED:
+        return np.asarray(keys, dtype=_UNSIGNED[type_id]).astype(np.uint64)
+    raise ValueError("join keys are integer types")
+
+
+def distribute(targets, payload, m):
+    """KKS oblivious distribution; returns (targets, payload) of the m
+    slots (-1 / 0 for dummies), or raises if the invariant fails."""
+    t = np.ascontiguousarray(targets, dtype=np.int64)
+    p = np.ascontiguousarray(payload, dtype=np.uint64)
+    ot = np.empty(m, np.int64)
+    op = np.empty(m, np.uint64)
+    if lib().orc_distribute(_p(t), _p(p), len(t), m, _p(ot), _p(op)) != 0:
+        raise ValueError("distribution invariant failed")
+    return ot, op
+
+
+def join_pairs(lcodes, rcodes):
+    """Equi-join of two key-code columns: (left rows, right rows), int64,
+    ordered by (key, left row, right row)."""
+    lk = np.ascontiguousarray(lcodes, dtype=np.uint64)
+    rk = np.ascontiguousarray(rcodes, dtype=np.uint64)
+    st = C.c_int(0)
+    n_out = lib().orc_join_pairs(_p(lk), len(lk), _p(rk), len(rk), None, None, 0, C.byref(st))
+    ol = np.empty(max(n_out, 1), np.int64)
+    or_ = np.empty(max(n_out, 1), np.int64)
+    n2 = lib().orc_join_pairs(_p(lk), len(lk), _p(rk), len(rk), _p(ol), _p(or_), n_out, C.byref(st))
+    if st.value != 0 or n2 != n_out:
+        raise ValueError("join oracle invariant failed")
+    return ol[:n_out], or_[:n_out]
 
 
 def rand64(seed, stream, i):

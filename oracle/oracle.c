@@ -536,3 +536,238 @@ size_t orc_join_groupsum(const int32_t *dim_key, const int32_t *dim_group, size_
     free(rows);
     return G;
 }
+
+/* ------------------------------------------------------------------------ */
+/* General equi-join into row-index pairs                                    */
+/* ------------------------------------------------------------------------ */
+
+/* sort.h:18-22: the largest power of two strictly below x (0 for x <= 1) */
+static size_t prev_pow_two(size_t x)
+{
+    size_t y = 1;
+    while (y < x) y <<= 1;
+    return y >> 1;
+}
+
+/* The shifting loop of obliv_distribute (sort.h:103-121) over elements of sz
+ * bytes already ordered by target (valid first, targets ascending), with
+ * tgt[i] = -1 for dummies; buf and tgt hold m slots (n_in given, the rest
+ * dummies).  Returns -1 if a move would land on a non-dummy. */
+static int kks_shift(unsigned char *buf, int64_t *tgt, size_t m, size_t sz)
+{
+    unsigned char *tmp = (unsigned char *)malloc(sz ? sz : 1);
+    int bad = 0;
+    for (size_t j = prev_pow_two(m); j >= 1; j >>= 1) {
+        for (size_t ii = m - j; ii-- > 0;) {
+            int64_t dest = tgt[ii];
+            if (dest >= (int64_t)(ii + j)) {
+                bad |= tgt[ii + j] != -1;
+                memcpy(tmp, buf + ii * sz, sz);
+                memcpy(buf + ii * sz, buf + (ii + j) * sz, sz);
+                memcpy(buf + (ii + j) * sz, tmp, sz);
+                tgt[ii] = tgt[ii + j];
+                tgt[ii + j] = dest;
+            }
+        }
+        if (j == 1) break;
+    }
+    free(tmp);
+    return bad ? -1 : 0;
+}
+
+typedef struct {
+    int64_t t;
+    uint64_t p;
+} dist_pair;
+
+/* ind_func_comp (sort.h:78-84): dummies last, then by target */
+static int cmp_dist(const void *a, const void *b)
+{
+    const dist_pair *x = (const dist_pair *)a, *y = (const dist_pair *)b;
+    int dx = x->t == -1, dy = y->t == -1;
+    if (dx != dy) return dx - dy;
+    return (x->t > y->t) - (x->t < y->t);
+}
+
+int orc_distribute(const int64_t *target, const uint64_t *payload, size_t n, size_t m,
+                   int64_t *out_target, uint64_t *out_payload)
+{
+    size_t cap = n > m ? n : m;
+    dist_pair *d = (dist_pair *)calloc(cap ? cap : 1, sizeof(dist_pair));
+    int64_t *tg = (int64_t *)malloc((cap ? cap : 1) * sizeof(int64_t));
+    for (size_t i = 0; i < n; ++i) {
+        d[i].t = target[i] < 0 ? -1 : target[i];
+        d[i].p = payload[i];
+    }
+    qsort(d, n, sizeof(dist_pair), cmp_dist);
+    for (size_t i = n; i < cap; ++i) d[i].t = -1;
+    int rc = 0;
+    for (size_t i = 0; i < m; ++i) {
+        rc |= d[i].t >= (int64_t)m ? -1 : 0;
+        tg[i] = d[i].t;
+    }
+    if (rc == 0) rc = kks_shift((unsigned char *)d, tg, m, sizeof(dist_pair));
+    for (size_t i = 0; i < m; ++i) {
+        out_target[i] = tg[i];
+        out_payload[i] = d[i].t == -1 ? 0 : d[i].p;
+    }
+    free(d);
+    free(tg);
+    return rc;
+}
+
+/* one row of the join union (the reference's elem_t, common/elem_t.h:15-23,
+ * with the data bytes replaced by the row index) */
+typedef struct {
+    uint64_t key;
+    uint32_t tab; /* 0 left (table_0), 1 right, 2 pad */
+    uint32_t row;
+    int64_t m0, m1, j_order;
+    int64_t has_value;
+} jp_elem;
+
+static int jp_less(const jp_elem *a, const jp_elem *b)
+{
+    if (a->key != b->key) return a->key < b->key;
+    if (a->tab != b->tab) return a->tab < b->tab;
+    return a->row < b->row;
+}
+
+static void jp_bitonic(jp_elem *a, size_t npad)
+{
+    for (size_t k = 2; k <= npad; k <<= 1)
+        for (size_t j = k >> 1; j > 0; j >>= 1)
+            for (size_t i = 0; i < npad; ++i) {
+                size_t x = i ^ j;
+                if (x > i) {
+                    int asc = (i & k) == 0;
+                    int sw = asc ? jp_less(&a[x], &a[i]) : jp_less(&a[i], &a[x]);
+                    cswap((unsigned char *)&a[i], (unsigned char *)&a[x], sizeof(jp_elem), sw);
+                }
+            }
+}
+
+static int cmp_jorder(const void *a, const void *b)
+{
+    const jp_elem *x = (const jp_elem *)a, *y = (const jp_elem *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return (x->j_order > y->j_order) - (x->j_order < y->j_order);
+}
+
+/* Steps 5 of the join for one copy: the compacted rows [0, k) with targets,
+ * distributed over n_out slots, then forward fill (:305-314). */
+static int jp_expand(const jp_elem *c, const int64_t *tgt, size_t k, size_t n_out, jp_elem *out)
+{
+    int64_t *tg = (int64_t *)malloc((n_out ? n_out : 1) * sizeof(int64_t));
+    size_t take = k < n_out ? k : n_out;
+    memset(out, 0, n_out * sizeof(jp_elem));
+    for (size_t i = 0; i < n_out; ++i) {
+        tg[i] = -1;
+        if (i < take) {
+            out[i] = c[i];
+            out[i].has_value = 1;
+            tg[i] = tgt[i];
+        }
+    }
+    int rc = kks_shift((unsigned char *)out, tg, n_out, sizeof(jp_elem));
+    for (size_t i = 0; i < n_out; ++i) out[i].has_value = tg[i] != -1;
+    for (size_t i = 1; i < n_out; ++i)
+        if (!out[i].has_value) out[i] = out[i - 1];
+    free(tg);
+    return rc;
+}
+
+size_t orc_join_pairs(const uint64_t *lkey, size_t n_left, const uint64_t *rkey, size_t n_right,
+                      int64_t *out_left, int64_t *out_right, size_t cap, int *status)
+{
+    size_t n = n_left + n_right, npad = 1;
+    while (npad < n) npad <<= 1;
+    jp_elem *arr = (jp_elem *)calloc(npad, sizeof(jp_elem));
+    jp_elem *arr_ = (jp_elem *)calloc(npad, sizeof(jp_elem));
+    *status = 0;
+    /* 1. union, sort */
+    for (size_t i = 0; i < npad; ++i) {
+        if (i < n_left) {
+            arr[i].key = lkey[i];
+            arr[i].tab = 0;
+            arr[i].row = (uint32_t)i;
+        } else if (i < n) {
+            arr[i].key = rkey[i - n_left];
+            arr[i].tab = 1;
+            arr[i].row = (uint32_t)(i - n_left);
+        } else {
+            arr[i].key = UINT64_MAX;
+            arr[i].tab = 2;
+            arr[i].row = (uint32_t)i;
+        }
+    }
+    jp_bitonic(arr, npad);
+    /* 2. m0 / m1: forward counts, backward propagation (:211-224); pads
+     * belong to no key */
+    for (size_t i = 0; i < npad; ++i) {
+        int t0 = arr[i].tab == 0, t1 = arr[i].tab == 1;
+        int c = i > 0 && arr[i].key == arr[i - 1].key && arr[i].tab != 2 && arr[i - 1].tab != 2;
+        int64_t pm0 = i > 0 ? arr[i - 1].m0 : 0, pm1 = i > 0 ? arr[i - 1].m1 : 0;
+        arr[i].m0 = (t0 && c) * (pm0 + 1) + (t1 && c) * pm0 + (t0 && !c);
+        arr[i].m1 = (t0 && c) * pm1 + (t1 && c) * (pm1 + 1) + (t1 && !c);
+    }
+    for (size_t i = npad; i-- > 1;) {
+        int c = arr[i - 1].key == arr[i].key && arr[i].tab != 2 && arr[i - 1].tab != 2;
+        arr[i - 1].m0 = c ? arr[i].m0 : arr[i - 1].m0;
+        arr[i - 1].m1 = c ? arr[i].m1 : arr[i - 1].m1;
+    }
+    /* 3. control bits, two copies, compaction */
+    uint8_t *cb1 = (uint8_t *)calloc(npad, 1), *cb2 = (uint8_t *)calloc(npad, 1);
+    size_t k1 = 0, k2 = 0;
+    for (size_t i = 0; i < npad; ++i) {
+        cb1[i] = arr[i].tab == 0 && arr[i].m1 > 0;
+        cb2[i] = arr[i].tab == 1 && arr[i].m0 > 0;
+        k1 += cb1[i];
+        k2 += cb2[i];
+        arr_[i] = arr[i];
+    }
+    orc_compact(arr, sizeof(jp_elem), cb1, npad);
+    orc_compact(arr_, sizeof(jp_elem), cb2, npad);
+    /* 4. index targets */
+    int64_t *it1 = (int64_t *)malloc((k1 ? k1 : 1) * sizeof(int64_t));
+    int64_t *it2 = (int64_t *)malloc((k2 ? k2 : 1) * sizeof(int64_t));
+    size_t n_out = 0, n_out2 = 0;
+    for (size_t i = 0; i < k1; ++i) {
+        it1[i] = (int64_t)n_out;
+        n_out += (size_t)arr[i].m1;
+    }
+    for (size_t i = 0; i < k2; ++i) {
+        it2[i] = (int64_t)n_out2;
+        n_out2 += (size_t)arr_[i].m0;
+    }
+    if (n_out2 != n_out) *status = -1;
+    if (*status == 0 && n_out <= cap && n_out > 0) {
+        /* 5. distribution + fill of both copies */
+        jp_elem *e1 = (jp_elem *)malloc(n_out * sizeof(jp_elem));
+        jp_elem *e2 = (jp_elem *)malloc(n_out * sizeof(jp_elem));
+        *status |= jp_expand(arr, it1, k1, n_out, e1);
+        *status |= jp_expand(arr_, it2, k2, n_out, e2);
+        /* 6. j_order of the right copy, sort by (key, j_order) */
+        int64_t j = 0;
+        for (size_t i = 0; i < n_out; ++i) {
+            j = (i > 0 && e2[i].key == e2[i - 1].key) ? j + 1 : 0;
+            int64_t m0 = e2[i].m0 > 1 ? e2[i].m0 : 1;
+            e2[i].j_order = (j % m0) * e2[i].m1 + j / m0;
+        }
+        qsort(e2, n_out, sizeof(jp_elem), cmp_jorder);
+        for (size_t i = 0; i < n_out; ++i) {
+            out_left[i] = e1[i].row;
+            out_right[i] = e2[i].row;
+            if (e1[i].key != e2[i].key) *status = -1;
+        }
+        free(e1);
+        free(e2);
+    }
+    free(arr);
+    free(arr_);
+    free(cb1);
+    free(cb2);
+    free(it1);
+    free(it2);
+    return n_out;
+}

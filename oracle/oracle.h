@@ -187,6 +187,49 @@ size_t orc_join_groupsum(const int32_t *dim_key, const int32_t *dim_group, size_
                          uint64_t *sorted_keys, uint64_t *sorted_vals,
                          int32_t *out_group, int64_t *out_sum, size_t max_groups);
 
+
+/* ------------------------------------------------------------------------ */
+/* General equi-join into row-index pairs                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Oblivious distribution, restating obliv_distribute
+ * (obliviator/join_kks_INT_INT/Include/sort.h:87-126): sort the n elements
+ * by target (dummies, target -1, last), extend to m slots of dummies, then
+ * for j = prev_pow_two(m) (the largest power of two below m, sort.h:18-22)
+ * down to 1 and i = m-j-1 down to 0, move the element at i to i+j when its
+ * target >= i+j; the slot it moves to must hold a dummy (the network's
+ * invariant, asserted in the reference).  Every valid element ends at its
+ * target.  Returns 0, or -1 when the invariant fails (targets not distinct
+ * or >= m).  Pinned against the reference's own known-answer program
+ * (join_kks_INT_INT/test_sort/test_obliv_distribute.cpp) built in _ref/. */
+int orc_distribute(const int64_t *target, const uint64_t *payload, size_t n, size_t m,
+                   int64_t *out_target, uint64_t *out_payload);
+
+/* Equi-join of two key columns into row-index pairs, restating
+ * scalable_oblivious_join (obliviator/join/enclave/scalable_oblivious_join.c:159-317):
+ *   1. union of the left (table_0) and right rows, padded to a power of two,
+ *      bitonic sort (the reference sorts by key only, :205; here ties are
+ *      broken by (table, row) so that the output order is canonical),
+ *   2. forward / backward scans giving m0 / m1 = left / right rows of the
+ *      row's key (:211-224),
+ *   3. control bits table_0 && m1 > 0 and !table_0 && m0 > 0, two copies
+ *      (soj_scan_1 :104-122), ORCompact of each (:251-252),
+ *   4. index targets = exclusive prefix sums of m1 (left) and m0 (right)
+ *      over the compacted rows (aggregation_tree_i, aggregation_tree.c:339-386),
+ *      n_out = their total,
+ *   5. oblivious distribution of each copy to n_out slots and forward fill
+ *      of the empty slots (:305-314),
+ *   6. j_order = (j mod m0) * m1 + j / m0 for the right copy, j its index in
+ *      the key's block (aggregation_tree.c:200-221), sort by (key, j_order).
+ * Keys are order-preserving 64-bit codes.  Output row i pairs out_left[i]
+ * with out_right[i]; pairs come ordered by (key, left row, right row).
+ * Returns n_out; the outputs are written only when n_out <= cap.  *status is
+ * 0, or -1 when a distribution invariant failed.
+ * Pinned: the reference's golden join (obliviator/join/test.txt ->
+ * test_output.txt) and brute-force nested loops over random inputs. */
+size_t orc_join_pairs(const uint64_t *lkey, size_t n_left, const uint64_t *rkey, size_t n_right,
+                      int64_t *out_left, int64_t *out_right, size_t cap, int *status);
+
 #ifdef __cplusplus
 }
 #endif

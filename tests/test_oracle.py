@@ -263,3 +263,149 @@ def test_join_with_unmatched_fact_rows():
     fv = np.array([10, 1000, 5, 7, 1000, 3], np.int64)
     groups, sums = oracle.join_groupsum(dk, dg, fk, fv, 16)
     assert groups.tolist() == [0, 1] and sums.tolist() == [22, 3]
+
+
+# -------------------------------------------------- equi-join into row pairs
+
+def _brute_pairs(lk, rk):
+    """Nested loops: every (l, r) with equal keys, ordered by (key, l, r)."""
+    out = sorted((int(lk[i]), i, j) for i in range(len(lk)) for j in range(len(rk)) if lk[i] == rk[j])
+    return [o[1] for o in out], [o[2] for o in out]
+
+
+def _read_join_file(path):
+    """obliviator's join input format: "n1 n2", a blank line, then n1 and n2
+    rows of "key data"."""
+    lines = open(path).read().split("\n")
+    n1, n2 = map(int, lines[0].split())
+    rows = [ln.split() for ln in lines[1:] if ln.strip()]
+    return rows[:n1], rows[n1:n1 + n2]
+
+
+def test_distribute_matches_reference_built_kat_program():
+    """oracle/_ref/kks_test_obliv_distribute is the reference's own
+    distribution known-answer program (join_kks_INT_INT/test_sort/
+    test_obliv_distribute.cpp) compiled from its sources; the oracle's
+    distribution of the same three inputs must print the same slots."""
+    if not os.path.exists(oracle.REF_DIST):
+        if os.path.isdir("/root/reference"):
+            subprocess.check_call(["make", "-s", "-C", os.path.dirname(oracle.REF_DIST), "ref"])
+        else:
+            pytest.skip("reference KAT program not built and /root/reference absent")
+    out = subprocess.check_output([oracle.REF_DIST], timeout=30).decode().split("\n")
+    got, cur = {}, None
+    for ln in out:
+        if ln.startswith("Result for "):
+            cur = ln[len("Result for "):-1]
+            got[cur] = []
+        elif cur and ":" in ln:
+            v = ln.split(":", 1)[1].strip()
+            got[cur].append(None if v == "D" else tuple(int(x) for x in v.split(" -> ")))
+    # (size, m, {slot: (id, target)}) of the program's inputs A, B, C
+    cases = {
+        "A": (16, 16, {2: (2, 8), 5: (5, 14), 6: (6, 1), 10: (10, 4), 13: (13, 15)}),
+        "B": (17, 31, {3: (0, 0), 7: (7, 17), 8: (8, 21), 10: (10, 4), 14: (14, 29), 15: (1, 5)}),
+        "C": (14, 100, {0: (0, 8), 1: (1, 10), 2: (2, 4), 3: (3, 3), 4: (4, 9), 5: (5, 92), 6: (6, 44),
+                        7: (7, 45), 12: (12, 39)}),
+    }
+    for name, (size, m, elems) in cases.items():
+        tg = np.full(size, -1, np.int64)
+        pay = np.zeros(size, np.uint64)
+        for slot, (eid, t) in elems.items():
+            tg[slot], pay[slot] = t, eid
+        ot, op = oracle.distribute(tg, pay, m)
+        mine = [None if t == -1 else (int(p), int(t)) for t, p in zip(ot, op)]
+        assert mine == got[name], name
+        assert all(t == -1 or t == i for i, t in enumerate(ot))  # every element at its target
+
+
+def test_distribute_random_invariant():
+    """Sorted distinct targets from any positions land on their targets; the
+    shifting network never overwrites an element (the reference's assert)."""
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        m = int(rng.integers(1, 300))
+        k = int(rng.integers(0, m + 1))
+        t = np.sort(rng.choice(m, size=k, replace=False)).astype(np.int64)
+        n = int(rng.integers(k, k + 20))
+        tg = np.full(n, -1, np.int64)
+        pos = np.sort(rng.choice(n, size=k, replace=False))
+        tg[pos] = t
+        ot, op = oracle.distribute(tg, np.arange(n, dtype=np.uint64), m)
+        want = np.full(m, -1, np.int64)
+        want[t] = t
+        assert np.array_equal(ot, want)
+        assert np.array_equal(op[t], pos.astype(np.uint64))
+
+
+def test_join_pairs_reference_golden_output():
+    """obliviator/join/test.txt -> test_output.txt: 512 left rows of one key
+    and 512 right rows of distinct keys, one of them equal, give 512 pairs.
+    The multiset of (key1, data1, key2, data2) output lines is equal (the
+    reference's line order follows its unstable sort)."""
+    left, right = _read_join_file(os.path.join(GOLDEN, "obliviator_join_input.txt"))
+    expected = sorted(tuple(ln.split()) for ln in open(os.path.join(GOLDEN, "obliviator_join_expected.txt"))
+                      if ln.strip())
+    lk = oracle.key_codes([int(r[0]) for r in left], 4)
+    rk = oracle.key_codes([int(r[0]) for r in right], 4)
+    ol, or_ = oracle.join_pairs(lk, rk)
+    got = sorted((left[a][0], left[a][1], right[b][0], right[b][1]) for a, b in zip(ol, or_))
+    assert len(got) == 512 and got == expected
+
+
+def test_join_pairs_fk_join_fixture():
+    """obliviator/fk_join/test.txt (10 primary keys x 1024 foreign keys):
+    equal to nested loops, one pair per fact row."""
+    left, right = _read_join_file(os.path.join(GOLDEN, "obliviator_fk_join_input.txt"))
+    lk = [int(r[0]) for r in left]
+    rk = [int(r[0]) for r in right]
+    ol, or_ = oracle.join_pairs(oracle.key_codes(lk, 4), oracle.key_codes(rk, 4))
+    bl, br = _brute_pairs(lk, rk)
+    assert ol.tolist() == bl and or_.tolist() == br and len(bl) == 1024
+
+
+@pytest.mark.parametrize("shape", ["uniform", "skew_left", "skew_right", "diagonal", "zipf_blocks", "disjoint"])
+def test_join_pairs_brute_force(shape):
+    """Random inputs of the shapes of obliviator's example generator
+    (join_kks_INT_INT/test_scripts/gen_example.py:33-55) and more."""
+    rng = np.random.default_rng(hash(shape) % 1000)
+    for trial in range(40):
+        n1 = int(rng.integers(1, 60))
+        if shape == "uniform":
+            lk = rng.integers(-5, 5, n1)
+            rk = rng.integers(-5, 5, int(rng.integers(0, 60)))
+        elif shape == "skew_left":        # gen_example class 1
+            lk = np.zeros(n1, np.int64)
+            rk = np.array([0] + [1] * (n1 - 1))
+        elif shape == "skew_right":       # class 2
+            lk = np.array([0] + [1] * (n1 - 1))
+            rk = np.zeros(n1, np.int64)
+        elif shape == "diagonal":         # class 3
+            lk = rng.permutation(n1)
+            rk = rng.permutation(n1)
+        elif shape == "zipf_blocks":      # class 4
+            lk, rk = [], []
+            for key in range(int(rng.integers(1, 8))):
+                lk += [key] * int(min(rng.zipf(2), 9))
+                rk += [key] * int(min(rng.zipf(2), 9))
+            lk, rk = rng.permutation(lk), rng.permutation(rk)
+        else:
+            lk = rng.integers(0, 100, n1)
+            rk = rng.integers(100, 200, int(rng.integers(0, 60)))
+        for t in (4, 5):
+            ol, or_ = oracle.join_pairs(oracle.key_codes(lk, t), oracle.key_codes(rk, t))
+            bl, br = _brute_pairs(list(lk), list(rk))
+            assert ol.tolist() == bl and or_.tolist() == br, (shape, trial)
+
+
+def test_key_codes_preserve_order_and_equality():
+    rng = np.random.default_rng(9)
+    for t, dt in ((2, np.int8), (3, np.int16), (4, np.int32), (5, np.int64), (1, np.uint8), (10, np.uint32),
+                  (64, np.uint64)):
+        info = np.iinfo(dt)
+        v = rng.integers(info.min, info.max, 500, dtype=dt, endpoint=True)
+        v[:4] = [info.min, info.max, 0, info.max]
+        c = oracle.key_codes(v, t)
+        order = np.argsort(v, kind="stable")
+        assert np.array_equal(np.argsort(c, kind="stable"), order)
+        assert np.array_equal(c[:, None] == c[None, :], v[:, None] == v[None, :])

@@ -206,6 +206,38 @@ class Context:
                           _ptr(sorted_keys), _ptr(sorted_vals), 1 if dense_groups else 0)
         return self.check(self.lib.vo_pkfk_join_groupsum(self.h, C.byref(a)))
 
+    def join_pairs_count(self, left_keys, n_left, right_keys, n_right, type_id, out_count=None):
+        """OP_SORT_MERGE_JOIN_COUNT: returns the number of (left, right) row
+        pairs with equal keys; the routing state stays in the context."""
+        a = _abi.JoinPairsArgs(_ptr(left_keys), n_left, _ptr(right_keys), n_right, type_id, 0, _ptr(out_count),
+                               None, None, 0)
+        n = C.c_uint64(0)
+        self.check(self.lib.vo_join_pairs_count(self.h, C.byref(a), C.byref(n)))
+        return n.value
+
+    def join_pairs_write(self, left_keys, n_left, right_keys, n_right, type_id, out_left, out_right, out_cap):
+        """OP_SORT_MERGE_JOIN_WRITE after the COUNT of the same columns:
+        out_left/out_right (int64, out_cap rows) get the pairs ordered by
+        (key, left row, right row), then -1."""
+        a = _abi.JoinPairsArgs(_ptr(left_keys), n_left, _ptr(right_keys), n_right, type_id, 0, None,
+                               _ptr(out_left), _ptr(out_right), out_cap)
+        return self.check(self.lib.vo_join_pairs_write(self.h, C.byref(a)))
+
+    def join_pairs(self, left_keys, right_keys, type_id):
+        """COUNT, allocate next_pow2(n_out) rows as the reference's operator
+        does (sort_merge_join_operator.cpp:126-133), WRITE.  Returns
+        (n_out, out_left, out_right) as int64 device tensors."""
+        import torch
+        nl, nr = int(left_keys.numel()), int(right_keys.numel())
+        n = self.join_pairs_count(left_keys, nl, right_keys, nr, type_id)
+        cap = 1
+        while cap < n:
+            cap <<= 1
+        ol = torch.empty(cap, dtype=torch.int64, device=left_keys.device)
+        orr = torch.empty(cap, dtype=torch.int64, device=left_keys.device)
+        self.join_pairs_write(left_keys, nl, right_keys, nr, type_id, ol, orr, cap)
+        return n, ol, orr
+
     def sort_batch(self, keys, payload, items):
         arr = (_abi.BatchItem * len(items))(*[_abi.BatchItem(o, n, p) for (o, n, p) in items])
         a = _abi.SortBatchArgs(_ptr(keys), _ptr(payload), C.cast(arr, C.c_void_p), len(items))

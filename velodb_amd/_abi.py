@@ -111,11 +111,17 @@ class SortBatchArgs(C.Structure):
     _fields_ = [("keys", P), ("payload", P), ("items", P), ("n_items", u64)]
 
 
+class JoinPairsArgs(C.Structure):
+    _fields_ = [("left_keys", P), ("n_left", u64), ("right_keys", P), ("n_right", u64), ("type_id", u32),
+                ("reserved", u32), ("out_count", P), ("out_left", P), ("out_right", P), ("out_cap", u64)]
+
+
 class CommandArgs(C.Union):
     _fields_ = [("sort", SortArgs), ("sort_kp", SortKpArgs), ("permute", PermuteArgs), ("scatter", ScatterArgs),
                 ("gather", GatherArgs), ("compare_eq", CompareEqArgs), ("compose_position_map", ComposeArgs),
                 ("enrich_positions", EnrichArgs), ("compact", CompactArgs), ("shuffle", ShuffleArgs),
-                ("filter", FilterArgs), ("join", JoinArgs), ("sort_batch", SortBatchArgs)]
+                ("filter", FilterArgs), ("join", JoinArgs), ("sort_batch", SortBatchArgs),
+                ("join_pairs", JoinPairsArgs)]
 
 
 class Command(C.Structure):
@@ -175,6 +181,8 @@ PROTOTYPES = [
     ("vo_filter_sum", C.c_int, [P, C.POINTER(FilterArgs)]),
     ("vo_pkfk_join_groupsum", C.c_int, [P, C.POINTER(JoinArgs)]),
     ("vo_sort_batch", C.c_int, [P, C.POINTER(SortBatchArgs)]),
+    ("vo_join_pairs_count", C.c_int, [P, C.POINTER(JoinPairsArgs), C.POINTER(u64)]),
+    ("vo_join_pairs_write", C.c_int, [P, C.POINTER(JoinPairsArgs)]),
     ("vo_sort_kp_async", C.c_int, [P, P, P, u64, u64]),
     ("vo_trace_begin", C.c_int, [P]),
     ("vo_trace_end", C.c_int, [P, C.POINTER(Trace)]),

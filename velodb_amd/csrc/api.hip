@@ -135,6 +135,8 @@ void dispatch(vo_ctx *ctx, const vo_command &cmd)
     case VO_OP_FILTER_SUM: op_filter_sum(ctx, a.filter); break;
     case VO_OP_PKFK_JOIN_GROUPSUM: op_join_groupsum(ctx, a.join); break;
     case VO_OP_SORT_BATCH: op_sort_batch(ctx, a.sort_batch); break;
+    case VO_OP_SORT_MERGE_JOIN_COUNT: (void)op_join_pairs_count(ctx, a.join_pairs); break;
+    case VO_OP_SORT_MERGE_JOIN_WRITE: op_join_pairs_write(ctx, a.join_pairs); break;
     case VO_OP_TERMINATE: break;
     default:
         // the reference ignores unknown opcodes (persistent_kernel.cu:69-72); report them instead
@@ -267,6 +269,7 @@ int vo_ctx_destroy(vo_ctx *ctx)
         (void)hipEventDestroy(t.stop);
     }
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    join_pairs_state_free(ctx);
     if (ctx->trace_slots) (void)hipFree(ctx->trace_slots);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -387,6 +390,18 @@ VO_SYNC_OP(vo_shuffle, VO_OP_SHUFFLE, shuffle, vo_shuffle_args)
 VO_SYNC_OP(vo_filter_sum, VO_OP_FILTER_SUM, filter, vo_filter_args)
 VO_SYNC_OP(vo_pkfk_join_groupsum, VO_OP_PKFK_JOIN_GROUPSUM, join, vo_join_args)
 VO_SYNC_OP(vo_sort_batch, VO_OP_SORT_BATCH, sort_batch, vo_sort_batch_args)
+VO_SYNC_OP(vo_join_pairs_write, VO_OP_SORT_MERGE_JOIN_WRITE, join_pairs, vo_join_pairs_args)
+
+int vo_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args *args, uint64_t *n_out)
+{
+    if (!args) return VO_ERR_INVALID;
+    vo_command c{};
+    c.opcode = VO_OP_SORT_MERGE_JOIN_COUNT;
+    c.args.join_pairs = *args;
+    const int st = run_sync(ctx, c);
+    if (st == VO_OK && n_out) *n_out = join_pairs_last_count(ctx);
+    return st;
+}
 
 // ------------------------------------------------------------ diagnostics
 

@@ -728,7 +728,22 @@ void compact_rows2(vo_ctx *ctx, uint64_t *c0, uint64_t *c1, const uint8_t *mask,
     run_network<2>(ctx, cs, cs, mask, n_pad, n_pad, tile_pre);
 }
 
-// Bytes of device memory compact_rows2 needs after count_dev.
+// The same for one 8-byte column.
+void compact_rows1(vo_ctx *ctx, uint64_t *c0, const uint8_t *mask, uint64_t n_pad, uint64_t *count_dev)
+{
+    check_pad(n_pad, n_pad);
+    const uint32_t nt = (uint32_t)(n_pad / CT);
+    uint32_t *tile_cnt = (uint32_t *)(count_dev + 1);
+    uint64_t *tile_pre = (uint64_t *)(((uintptr_t)(tile_cnt + nt) + 255) & ~(uintptr_t)255);
+    launch(ctx, LaunchInfo{"compact_count", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_count_mask, mask, n_pad, tile_cnt,
+           trace_slot(ctx));
+    launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)tile_cnt, nt,
+           tile_pre, count_dev, trace_slot(ctx));
+    Cols<1> cs{{{c0, 8}}};
+    run_network<1>(ctx, cs, cs, mask, n_pad, n_pad, tile_pre);
+}
+
+// Bytes of device memory compact_rows1/2 need after count_dev.
 size_t compact_rows2_scratch(uint64_t n_pad)
 {
     const uint64_t nt = n_pad / CT;

@@ -1,0 +1,879 @@
+// join_pairs.hip -- oblivious equi-join of two key columns into row-index
+// pairs: OP_SORT_MERGE_JOIN_COUNT / OP_SORT_MERGE_JOIN_WRITE, the device
+// work of SortMergeJoinOperator (src/operator/sort_merge_join_operator.cpp:27-257).
+//
+// Reference method: the general m0 x m1 join of
+// obliviator/join/enclave/scalable_oblivious_join.c:159-317 with the
+// oblivious distribution of obliviator/join_kks_INT_INT/Include/sort.h:87-126.
+//
+// COUNT (over the union, n_pad = 2^k rows):
+//   1. union records key = order-preserving code of the key,
+//      val = (class << 62) | row, class 0 left, 2 right, 3 pad, and a sort by
+//      (key, val): a key's left rows precede its right rows;
+//   2. a backward scan (suffix minima) gives every row the end E of its key
+//      segment and, for left rows, the first right row Q of the segment:
+//      m1 = E - Q;
+//   3. a forward scan gives the segment start S, the first right row R and
+//      the exclusive sum X of m1 over the left rows before the row:
+//        left row : target T = X                    (= B + a*m1)
+//        right row: m0 = R - S, m1 = E - R, b = i - R,
+//                   B = X - m0*m1, target T = B + b*m0,
+//      B being the segment's first output row and n_out the total of m1;
+//   4. ORCompact of the left rows with m1 > 0 and of the right rows with
+//      m0 > 0 (the control bits of :104-122).
+// WRITE (over M_pad = 2^k >= n_out output rows):
+//   5. distribution of each compacted copy to its targets: levels
+//      j = M_pad/2 .. 1, an element moves from p to p + j when its target is
+//      >= p + j, and the slot it enters is always empty (the order of the
+//      targets guarantees it), so every level is one branch-free pass
+//      new[p] = moves(old[p-j]) ? old[p-j] : moves(old[p]) ? empty : old[p];
+//      levels j >= 256 run one per HBM pass, the last 8 in LDS per 2048-slot
+//      tile with a 256-slot halo (an output slot depends only on the 255
+//      slots before it in those levels);
+//   6. forward fill of the empty slots (:305-314): slot p holds the left row
+//      of its pair; for the right copy, copy a = p - T goes to final row
+//      B + a*m1 + b (the j_order of aggregation_tree.c:200-221) by a sort of
+//      (final row, right row).
+// Every launch reads and writes every row of its padded size; only n_out
+// leaves the device.
+#include "vo_internal.hpp"
+
+#include <algorithm>
+
+namespace vo {
+namespace {
+
+constexpr int PT = 4096; // scan tile (rows)
+constexpr int PB = 256;
+constexpr int PPER = PT / PB;
+constexpr uint32_t INF32 = 0xFFFFFFFFu;
+constexpr uint64_t EMPTY = ~0ull; // element word 0 of an empty slot (target field 0xFFFFFFFF)
+
+constexpr int DT = 2048;         // distribution / fill tile (slots)
+constexpr int DH = 256;          // halo: levels 128 .. 1 run in the tile kernel
+constexpr int DW = DT + DH;      // window rows
+constexpr int DPER = DW / PB;    // window rows per thread per level
+constexpr int FPER = DT / PB;    // fill slots per thread
+
+dim3 grid_for(uint64_t n) { return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 65536))); }
+
+// ------------------------------------------------------------ union build
+
+__device__ __forceinline__ uint64_t key_code(const void *p, uint32_t type, uint64_t i)
+{
+    switch (type) {
+    case VO_T_BOOLEAN: return ((const uint8_t *)p)[i];
+    case VO_T_TINYINT: return (uint64_t)(int64_t)((const int8_t *)p)[i] ^ (1ull << 63);
+    case VO_T_SMALLINT: return (uint64_t)(int64_t)((const int16_t *)p)[i] ^ (1ull << 63);
+    case VO_T_INTEGER: return (uint64_t)(int64_t)((const int32_t *)p)[i] ^ (1ull << 63);
+    case VO_T_BIGINT: return (uint64_t)((const int64_t *)p)[i] ^ (1ull << 63);
+    case VO_T_DATE: return ((const uint32_t *)p)[i];
+    default: return ((const uint64_t *)p)[i]; // VARCHAR ordinal, UBIGINT
+    }
+}
+
+template <bool TRACE>
+__global__ void k_jp_build(const void *__restrict__ lk, uint64_t nl, const void *__restrict__ rk, uint64_t nr,
+                           uint32_t type, uint64_t n_pad, uint64_t *__restrict__ keys, uint64_t *__restrict__ vals,
+                           TraceSlot *slot)
+{
+    dev::AtrT<TRACE> tr(slot);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k = ~0ull, v = (3ull << 62) | i;
+        if (i < nl) { // public sizes only
+            k = key_code(lk, type, i);
+            v = i;
+            tr.at(1, i);
+        } else if (i < nl + nr) {
+            k = key_code(rk, type, i - nl);
+            v = (2ull << 62) | (i - nl);
+            tr.at(2, i - nl);
+        }
+        keys[i] = k;
+        vals[i] = v;
+        tr.at(3, i);
+    }
+    tr.done(dev::gtid());
+}
+
+__device__ __forceinline__ uint32_t cls(uint64_t v) { return (uint32_t)(v >> 62); } // 0 left, 2 right, 3 pad
+
+__device__ __forceinline__ bool same_seg(uint64_t ka, uint64_t va, uint64_t kb, uint64_t vb)
+{
+    return (ka == kb) & (cls(va) != 3) & (cls(vb) != 3);
+}
+
+// ------------------------------------------------------ tile-level scans
+
+// Exclusive scan of nt tile summaries with one 1024-thread block (forward,
+// or backward when REV): contiguous chunks per thread, a Hillis-Steele scan
+// of the chunk totals in LDS.
+constexpr uint32_t kScanThreads = 1024;
+
+template <bool REV, class T, class Op, class Tr>
+__device__ void tile_scan(const T *__restrict__ in, uint32_t nt, T *__restrict__ out, T id, Op op, T *sh, Tr &tr)
+{
+    const uint32_t per = (nt + kScanThreads - 1) / kScanThreads;
+    const uint32_t lo = threadIdx.x * per;
+    auto at = [&](uint32_t x) { return REV ? nt - 1 - x : x; };
+    T s = id;
+    for (uint32_t k = 0; k < per; ++k)
+        if (lo + k < nt) { // public size
+            s = op(s, in[at(lo + k)]);
+            tr.at(1, at(lo + k));
+        }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
+        const T v = threadIdx.x >= off ? sh[threadIdx.x - off] : id;
+        __syncthreads();
+        if (threadIdx.x >= off) sh[threadIdx.x] = op(v, sh[threadIdx.x]);
+        __syncthreads();
+    }
+    T run = threadIdx.x ? sh[threadIdx.x - 1] : id;
+    for (uint32_t k = 0; k < per; ++k)
+        if (lo + k < nt) {
+            out[at(lo + k)] = run;
+            run = op(run, in[at(lo + k)]);
+            tr.at(2, at(lo + k));
+        }
+}
+
+struct Min2 {
+    uint32_t e, q;
+};
+struct Min2Op {
+    __device__ Min2 operator()(const Min2 &a, const Min2 &b) const { return Min2{min(a.e, b.e), min(a.q, b.q)}; }
+};
+
+struct Fwd {
+    uint32_t s, r; // latest segment start / first right row (maxima)
+    uint64_t x;    // sum of m1 over left rows
+};
+struct FwdOp {
+    __device__ Fwd operator()(const Fwd &a, const Fwd &b) const { return Fwd{max(a.s, b.s), max(a.r, b.r), a.x + b.x}; }
+};
+
+template <int NW>
+struct Elem {
+    uint64_t w[NW];
+};
+template <int NW>
+struct LastOp { // the later non-empty element wins
+    __device__ Elem<NW> operator()(const Elem<NW> &a, const Elem<NW> &b) const { return b.w[0] != EMPTY ? b : a; }
+};
+
+template <bool TRACE>
+__global__ __launch_bounds__(kScanThreads) void k_jp_bwd_carry(const Min2 *__restrict__ tiles, uint32_t nt,
+                                                               Min2 *__restrict__ carry, TraceSlot *slot)
+{
+    __shared__ Min2 sh[kScanThreads];
+    dev::AtrT<TRACE> tr(slot);
+    tile_scan<true>(tiles, nt, carry, Min2{INF32, INF32}, Min2Op{}, sh, tr);
+    tr.done(dev::gtid());
+}
+
+template <bool TRACE>
+__global__ __launch_bounds__(kScanThreads) void k_jp_fwd_carry(const Fwd *__restrict__ tiles, uint32_t nt,
+                                                               Fwd *__restrict__ carry, TraceSlot *slot)
+{
+    __shared__ Fwd sh[kScanThreads];
+    dev::AtrT<TRACE> tr(slot);
+    tile_scan<false>(tiles, nt, carry, Fwd{0, 0, 0}, FwdOp{}, sh, tr);
+    tr.done(dev::gtid());
+}
+
+template <int NW, bool TRACE>
+__global__ __launch_bounds__(kScanThreads) void k_jp_fill_carry(const Elem<NW> *__restrict__ tiles, uint32_t nt,
+                                                                Elem<NW> *__restrict__ carry, TraceSlot *slot)
+{
+    __shared__ Elem<NW> sh[kScanThreads];
+    dev::AtrT<TRACE> tr(slot);
+    Elem<NW> id;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) id.w[w] = EMPTY;
+    tile_scan<false>(tiles, nt, carry, id, LastOp<NW>{}, sh, tr);
+    tr.done(dev::gtid());
+}
+
+// --------------------------------------------- backward scan (step 2)
+
+// Row events of the backward scan: e = end of the key segment (i+1 at the
+// segment's last row), q = i at a right row, else e.
+struct RowsBuf {
+    uint64_t k[PPER + 1], v[PPER + 1]; // the thread's rows and the next one
+};
+
+__device__ __forceinline__ void load_rows(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
+                                          uint64_t t0, uint64_t n_pad, uint64_t *k, uint64_t *v)
+{
+#pragma unroll
+    for (int r = 0; r < PPER; ++r) {
+        k[r] = keys[t0 + r];
+        v[r] = vals[t0 + r];
+    }
+    const bool has_next = t0 + PPER < n_pad; // public
+    k[PPER] = has_next ? keys[t0 + PPER] : ~0ull;
+    v[PPER] = has_next ? vals[t0 + PPER] : (3ull << 62);
+}
+
+template <bool TRACE>
+__global__ __launch_bounds__(PB) void k_jp_bwd_tiles(const uint64_t *__restrict__ keys,
+                                                     const uint64_t *__restrict__ vals, uint64_t n_pad,
+                                                     Min2 *__restrict__ tiles, TraceSlot *slot)
+{
+    __shared__ Min2 red[PB / 64];
+    dev::AtrT<TRACE> tr(slot);
+    const uint64_t t0 = (uint64_t)blockIdx.x * PT + (uint64_t)threadIdx.x * PPER;
+    uint64_t k[PPER + 1], v[PPER + 1];
+    load_rows(keys, vals, t0, n_pad, k, v);
+    tr.at(1, t0);
+    uint32_t me = INF32, mq = INF32;
+#pragma unroll
+    for (int r = 0; r < PPER; ++r) {
+        const uint32_t i = (uint32_t)(t0 + r);
+        const bool end = !same_seg(k[r], v[r], k[r + 1], v[r + 1]);
+        const uint32_t e = end ? i + 1 : INF32;
+        const uint32_t q = cls(v[r]) == 2 ? i : e;
+        me = min(me, e);
+        mq = min(mq, q);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        me = min(me, (uint32_t)__shfl_xor(me, off));
+        mq = min(mq, (uint32_t)__shfl_xor(mq, off));
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = Min2{me, mq};
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Min2 m = red[0];
+        for (int w = 1; w < PB / 64; ++w) m = Min2Op{}(m, red[w]);
+        tiles[blockIdx.x] = m;
+        tr.at(2, blockIdx.x);
+    }
+    tr.done(dev::gtid());
+}
+
+// Per row: E and Q (suffix minima), aux = m1 for left rows, E for right
+// rows; per tile: the forward-scan summary (latest start, latest first right
+// row, sum of m1 over left rows).
+template <bool TRACE>
+__global__ __launch_bounds__(PB) void k_jp_bwd_rows(const uint64_t *__restrict__ keys,
+                                                    const uint64_t *__restrict__ vals, uint64_t n_pad,
+                                                    const Min2 *__restrict__ carry, uint32_t *__restrict__ aux,
+                                                    Fwd *__restrict__ ftiles, TraceSlot *slot)
+{
+    __shared__ Min2 wt[PB / 64];
+    __shared__ Fwd fr[PB / 64];
+    dev::AtrT<TRACE> tr(slot);
+    const uint64_t t0 = (uint64_t)blockIdx.x * PT + (uint64_t)threadIdx.x * PPER;
+    uint64_t k[PPER + 1], v[PPER + 1];
+    load_rows(keys, vals, t0, n_pad, k, v);
+    const bool has_prev = t0 > 0; // public
+    const uint64_t pk = has_prev ? keys[t0 - 1] : ~0ull;
+    const uint64_t pv = has_prev ? vals[t0 - 1] : (3ull << 62);
+    tr.at(1, t0);
+    uint32_t me = INF32, mq = INF32;
+#pragma unroll
+    for (int r = 0; r < PPER; ++r) {
+        const uint32_t i = (uint32_t)(t0 + r);
+        const bool end = !same_seg(k[r], v[r], k[r + 1], v[r + 1]);
+        const uint32_t e = end ? i + 1 : INF32;
+        me = min(me, e);
+        mq = min(mq, cls(v[r]) == 2 ? i : e);
+    }
+    // minima over the later lanes, waves and tiles
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t se = me, sq = mq;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t oe = __shfl_down(se, off, 64), oq = __shfl_down(sq, off, 64);
+        const bool take = lane + off < 64;
+        se = take ? min(se, oe) : se;
+        sq = take ? min(sq, oq) : sq;
+    }
+    uint32_t ce = __shfl_down(se, 1, 64), cq = __shfl_down(sq, 1, 64);
+    ce = lane == 63 ? INF32 : ce;
+    cq = lane == 63 ? INF32 : cq;
+    if (lane == 0) wt[wv] = Min2{se, sq};
+    __syncthreads();
+    const Min2 c = carry[blockIdx.x];
+    tr.at(2, blockIdx.x);
+    ce = min(ce, c.e);
+    cq = min(cq, c.q);
+    for (uint32_t w = wv + 1; w < PB / 64; ++w) {
+        ce = min(ce, wt[w].e);
+        cq = min(cq, wt[w].q);
+    }
+    uint32_t E[PPER], Q[PPER];
+#pragma unroll
+    for (int r = PPER - 1; r >= 0; --r) {
+        const uint32_t i = (uint32_t)(t0 + r);
+        const bool end = !same_seg(k[r], v[r], k[r + 1], v[r + 1]);
+        const uint32_t e = end ? i + 1 : INF32;
+        ce = min(ce, e);
+        cq = min(cq, cls(v[r]) == 2 ? i : e);
+        E[r] = ce;
+        Q[r] = cq;
+    }
+    // forward summary of this thread's rows
+    Fwd f{0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < PPER; ++r) {
+        const uint32_t i = (uint32_t)(t0 + r);
+        const uint64_t kp = r ? k[r - 1] : pk, vp = r ? v[r - 1] : pv;
+        const bool start = !same_seg(kp, vp, k[r], v[r]);
+        const uint32_t c0 = cls(v[r]);
+        const bool rstart = (c0 == 2) & (start | (cls(vp) != 2));
+        const uint32_t m1 = E[r] - Q[r];
+        f.s = start ? i : f.s;
+        f.r = rstart ? i : f.r;
+        f.x += c0 == 0 ? m1 : 0;
+        aux[t0 + r] = c0 == 0 ? m1 : (c0 == 2 ? E[r] : 0);
+    }
+    tr.at(3, t0);
+    for (int off = 32; off > 0; off >>= 1) {
+        const Fwd o{(uint32_t)__shfl_xor(f.s, off), (uint32_t)__shfl_xor(f.r, off),
+                    (uint64_t)__shfl_xor((unsigned long long)f.x, off)};
+        f = FwdOp{}(f, o);
+    }
+    if (lane == 0) fr[wv] = f;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Fwd t = fr[0];
+        for (int w = 1; w < PB / 64; ++w) t = FwdOp{}(t, fr[w]);
+        ftiles[blockIdx.x] = t;
+        tr.at(4, blockIdx.x);
+    }
+    tr.done(dev::gtid());
+}
+
+// --------------------------------------------- forward scan (step 3)
+
+// Per row: targets and the compaction columns / masks; the last thread of
+// the last tile writes n_out.
+template <bool TRACE>
+__global__ __launch_bounds__(PB) void k_jp_fwd_rows(const uint64_t *__restrict__ keys,
+                                                    const uint64_t *__restrict__ vals,
+                                                    const uint32_t *__restrict__ aux, uint64_t n_pad,
+                                                    const Fwd *__restrict__ carry, uint64_t *__restrict__ cl,
+                                                    uint8_t *__restrict__ ml, uint64_t *__restrict__ cr0,
+                                                    uint64_t *__restrict__ cr1, uint8_t *__restrict__ mr,
+                                                    uint64_t *__restrict__ n_out, TraceSlot *slot)
+{
+    __shared__ Fwd wt[PB / 64];
+    dev::AtrT<TRACE> tr(slot);
+    const uint64_t t0 = (uint64_t)blockIdx.x * PT + (uint64_t)threadIdx.x * PPER;
+    uint64_t k[PPER], v[PPER];
+    uint32_t ax[PPER];
+#pragma unroll
+    for (int r = 0; r < PPER; ++r) {
+        k[r] = keys[t0 + r];
+        v[r] = vals[t0 + r];
+        ax[r] = aux[t0 + r];
+    }
+    const bool has_prev = t0 > 0;
+    const uint64_t pk = has_prev ? keys[t0 - 1] : ~0ull;
+    const uint64_t pv = has_prev ? vals[t0 - 1] : (3ull << 62);
+    tr.at(1, t0);
+    bool st[PPER], rs[PPER];
+    Fwd f{0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < PPER; ++r) {
+        const uint32_t i = (uint32_t)(t0 + r);
+        const uint64_t kp = r ? k[r - 1] : pk, vp = r ? v[r - 1] : pv;
+        st[r] = !same_seg(kp, vp, k[r], v[r]);
+        rs[r] = (cls(v[r]) == 2) & (st[r] | (cls(vp) != 2));
+        f.s = st[r] ? i : f.s;
+        f.r = rs[r] ? i : f.r;
+        f.x += cls(v[r]) == 0 ? ax[r] : 0;
+    }
+    // exclusive prefix over earlier lanes, waves and tiles
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    Fwd in = f;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const Fwd o{(uint32_t)__shfl_up(in.s, off, 64), (uint32_t)__shfl_up(in.r, off, 64),
+                    (uint64_t)__shfl_up((unsigned long long)in.x, off, 64)};
+        in = lane >= (uint32_t)off ? FwdOp{}(o, in) : in;
+    }
+    Fwd ex{(uint32_t)__shfl_up(in.s, 1, 64), (uint32_t)__shfl_up(in.r, 1, 64),
+           (uint64_t)__shfl_up((unsigned long long)in.x, 1, 64)};
+    ex = lane == 0 ? Fwd{0, 0, 0} : ex;
+    if (lane == 63) wt[wv] = in;
+    __syncthreads();
+    Fwd c = carry[blockIdx.x];
+    tr.at(2, blockIdx.x);
+    for (uint32_t w = 0; w < wv; ++w) c = FwdOp{}(c, wt[w]);
+    c = FwdOp{}(c, ex);
+    uint32_t S = c.s, R = c.r;
+    uint64_t X = c.x;
+#pragma unroll
+    for (int r = 0; r < PPER; ++r) {
+        const uint64_t i = t0 + r;
+        S = st[r] ? (uint32_t)i : S;
+        R = rs[r] ? (uint32_t)i : R;
+        const uint32_t c0 = cls(v[r]);
+        const uint64_t row = v[r] & 0xFFFFFFFFull;
+        // left row: ax = m1, target X
+        const uint64_t tl = X;
+        // right row: ax = E
+        const uint64_t m0 = (uint64_t)(R - S), m1 = (uint64_t)(ax[r] - R), b = i - R;
+        const uint64_t B = X - m0 * m1;
+        const uint64_t tr_ = B + b * m0;
+        cl[i] = (tl << 32) | row;
+        ml[i] = (c0 == 0) & (ax[r] != 0);
+        cr0[i] = (tr_ << 32) | row;
+        cr1[i] = ((B + b) & 0xFFFFFFFFull) | (m1 << 32);
+        mr[i] = (c0 == 2) & (m0 != 0);
+        X += c0 == 0 ? ax[r] : 0;
+    }
+    tr.at(3, t0);
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == PB - 1) {
+        n_out[0] = X;
+        tr.at(4, 0);
+    }
+    tr.done(dev::gtid());
+}
+
+// ------------------------------------------------ distribution (step 5)
+
+__device__ __forceinline__ uint32_t tgt(uint64_t w0) { return (uint32_t)(w0 >> 32); }
+
+// Copy of the compacted rows into the first slots; slots >= k are empty.
+template <int NW, bool TRACE>
+__global__ void k_jp_dist_init(const uint64_t *__restrict__ s0, const uint64_t *__restrict__ s1, uint64_t src_rows,
+                               const uint64_t *__restrict__ kdev, uint64_t m_pad, uint64_t *__restrict__ d0,
+                               uint64_t *__restrict__ d1, TraceSlot *slot)
+{
+    dev::AtrT<TRACE> tr(slot);
+    const uint64_t kk = kdev[0];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m_pad; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t a = EMPTY, b = EMPTY;
+        if (i < src_rows) { // public
+            a = s0[i];
+            if constexpr (NW == 2) b = s1[i];
+            tr.at(1, i);
+        }
+        const bool valid = i < kk;
+        d0[i] = valid ? a : EMPTY;
+        if constexpr (NW == 2) d1[i] = valid ? b : EMPTY;
+        tr.at(2, i);
+    }
+    tr.done(dev::gtid());
+}
+
+// One level j (>= 256): new[p] from old[p] and old[p - j].
+template <int NW, bool TRACE>
+__global__ void k_jp_dist_level(const uint64_t *__restrict__ i0, const uint64_t *__restrict__ i1, uint64_t m_pad,
+                                uint32_t j, uint64_t *__restrict__ o0, uint64_t *__restrict__ o1, TraceSlot *slot)
+{
+    dev::AtrT<TRACE> tr(slot);
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m_pad; p += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t c0 = i0[p];
+        uint64_t c1 = 0, q1 = 0, q0 = EMPTY;
+        if constexpr (NW == 2) c1 = i1[p];
+        if (p >= j) { // public
+            q0 = i0[p - j];
+            if constexpr (NW == 2) q1 = i1[p - j];
+            tr.at(1, p - j);
+        }
+        const bool in = (q0 != EMPTY) & ((uint64_t)tgt(q0) >= p);
+        const bool out = (c0 != EMPTY) & ((uint64_t)tgt(c0) >= p + j);
+        o0[p] = in ? q0 : (out ? EMPTY : c0);
+        if constexpr (NW == 2) o1[p] = in ? q1 : c1;
+        tr.at(2, p);
+    }
+    tr.done(dev::gtid());
+}
+
+// Levels 128 .. 1 of one 2048-slot tile in LDS (window of the 256 slots
+// before it), then the tile's last non-empty element for the fill.
+template <int NW, bool TRACE>
+__global__ __launch_bounds__(PB) void k_jp_dist_tile(const uint64_t *__restrict__ i0, const uint64_t *__restrict__ i1,
+                                                     uint64_t *__restrict__ o0, uint64_t *__restrict__ o1,
+                                                     Elem<NW> *__restrict__ last, TraceSlot *slot)
+{
+    extern __shared__ uint64_t lds[]; // [2][NW][DW]
+    dev::AtrT<TRACE> tr(slot);
+    const int64_t base = (int64_t)blockIdx.x * DT - DH;
+#pragma unroll
+    for (int r = 0; r < DPER; ++r) {
+        const int x = threadIdx.x + r * PB;
+        const int64_t g = base + x;
+        uint64_t a = EMPTY, b = EMPTY;
+        if (g >= 0) { // public
+            a = i0[g];
+            if constexpr (NW == 2) b = i1[g];
+            tr.at(1, (uint64_t)g);
+        }
+        lds[x] = a;
+        if constexpr (NW == 2) lds[DW + x] = b;
+    }
+    __syncthreads();
+    int cur = 0;
+#pragma unroll
+    for (int j = DH / 2; j >= 1; j >>= 1) {
+        const uint64_t *s = lds + cur * NW * DW;
+        uint64_t *d = lds + (cur ^ 1) * NW * DW;
+#pragma unroll
+        for (int r = 0; r < DPER; ++r) {
+            const int x = threadIdx.x + r * PB;
+            const int64_t g = base + x;
+            const uint64_t c0 = s[x];
+            const int xp = x >= j ? x - j : x; // window-relative; x < j only in the halo
+            const uint64_t q0 = x >= j ? s[xp] : EMPTY;
+            const bool in = (q0 != EMPTY) & ((int64_t)tgt(q0) >= g);
+            const bool out = (c0 != EMPTY) & ((int64_t)tgt(c0) >= g + j);
+            d[x] = in ? q0 : (out ? EMPTY : c0);
+            if constexpr (NW == 2) d[DW + x] = in ? s[DW + xp] : s[DW + x];
+        }
+        cur ^= 1;
+        __syncthreads();
+    }
+    const uint64_t *s = lds + cur * NW * DW;
+    Elem<NW> lst;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) lst.w[w] = EMPTY;
+#pragma unroll
+    for (int r = 0; r < FPER; ++r) {
+        const int x = DH + threadIdx.x + r * PB;
+        const uint64_t g = (uint64_t)blockIdx.x * DT + threadIdx.x + r * PB;
+        Elem<NW> e;
+        e.w[0] = s[x];
+        if constexpr (NW == 2) e.w[1] = s[DW + x];
+        o0[g] = e.w[0];
+        if constexpr (NW == 2) o1[g] = e.w[1];
+        tr.at(2, g);
+        lst = LastOp<NW>{}(lst, e);
+    }
+    // the latest non-empty element over threads: slots are interleaved
+    // (thread t holds t, t+256, ...), so compare slot positions
+    __shared__ int wbest[PB / 64];
+    __shared__ Elem<NW> win;
+    int pos = -1;
+#pragma unroll
+    for (int r = 0; r < FPER; ++r) pos = s[DH + threadIdx.x + r * PB] != EMPTY ? (int)(threadIdx.x + r * PB) : pos;
+    int wmax = pos;
+    for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off));
+    if ((threadIdx.x & 63) == 0) wbest[threadIdx.x >> 6] = wmax;
+    __syncthreads();
+    int best = wbest[0];
+    for (int w = 1; w < PB / 64; ++w) best = max(best, wbest[w]);
+    if (pos >= 0 && pos == best) win = lst; // one thread: slot positions are distinct
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Elem<NW> r;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) r.w[w] = best >= 0 ? win.w[w] : EMPTY;
+        last[blockIdx.x] = r;
+        tr.at(3, blockIdx.x);
+    }
+    tr.done(dev::gtid());
+}
+
+// ---------------------------------------------------- fill (step 6)
+
+// SIDE 0: out_left[p] = left row of slot p (p < n_out) or -1 (p < out_cap).
+// SIDE 1: keys[p] = final row of the right copy in slot p, payload = right
+//         row; slots >= n_out get (~0, p).
+template <int NW, int SIDE, bool TRACE>
+__global__ __launch_bounds__(PB) void k_jp_fill_rows(const uint64_t *__restrict__ i0, const uint64_t *__restrict__ i1,
+                                                     const Elem<NW> *__restrict__ carry, uint64_t n_out,
+                                                     uint64_t out_cap, int64_t *__restrict__ out_left,
+                                                     uint64_t *__restrict__ skeys, uint32_t *__restrict__ spay,
+                                                     TraceSlot *slot)
+{
+    __shared__ Elem<NW> wt[PB / 64];
+    dev::AtrT<TRACE> tr(slot);
+    const uint64_t g0 = (uint64_t)blockIdx.x * DT + (uint64_t)threadIdx.x * FPER;
+    Elem<NW> e[FPER];
+    Elem<NW> lst;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) lst.w[w] = EMPTY;
+#pragma unroll
+    for (int r = 0; r < FPER; ++r) {
+        e[r].w[0] = i0[g0 + r];
+        if constexpr (NW == 2) e[r].w[1] = i1[g0 + r];
+        lst = LastOp<NW>{}(lst, e[r]);
+    }
+    tr.at(1, g0);
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    Elem<NW> in = lst;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        Elem<NW> o;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) o.w[w] = __shfl_up((unsigned long long)in.w[w], off, 64);
+        in = lane >= (uint32_t)off ? LastOp<NW>{}(o, in) : in;
+    }
+    Elem<NW> ex;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        ex.w[w] = __shfl_up((unsigned long long)in.w[w], 1, 64);
+        ex.w[w] = lane == 0 ? EMPTY : ex.w[w];
+    }
+    if (lane == 63) wt[wv] = in;
+    __syncthreads();
+    Elem<NW> c = carry[blockIdx.x];
+    tr.at(2, blockIdx.x);
+    for (uint32_t w = 0; w < wv; ++w) c = LastOp<NW>{}(c, wt[w]);
+    c = LastOp<NW>{}(c, ex);
+#pragma unroll
+    for (int r = 0; r < FPER; ++r) {
+        const uint64_t g = g0 + r;
+        c = LastOp<NW>{}(c, e[r]);
+        const bool live = g < n_out; // public
+        if constexpr (SIDE == 0) {
+            if (g < out_cap) out_left[g] = live ? (int64_t)(c.w[0] & 0xFFFFFFFFull) : -1;
+        } else {
+            const uint64_t a = g - tgt(c.w[0]);
+            const uint64_t fin = (c.w[1] & 0xFFFFFFFFull) + a * (c.w[1] >> 32);
+            skeys[g] = live ? fin : ~0ull;
+            spay[g] = live ? (uint32_t)c.w[0] : (uint32_t)g;
+        }
+    }
+    tr.at(3, g0);
+    tr.done(dev::gtid());
+}
+
+// out_right[p] = sorted right row (p < n_out) or -1; out_left[p] = -1 for
+// p >= m_pad.
+template <bool TRACE>
+__global__ void k_jp_out(const uint32_t *__restrict__ spay, uint64_t m_pad, uint64_t n_out, uint64_t out_cap,
+                         int64_t *__restrict__ out_left, int64_t *__restrict__ out_right, TraceSlot *slot)
+{
+    dev::AtrT<TRACE> tr(slot);
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < out_cap; g += (uint64_t)gridDim.x * blockDim.x) {
+        int64_t r = -1;
+        if (g < m_pad) { // public
+            const uint32_t p = spay[g];
+            r = g < n_out ? (int64_t)p : -1;
+            tr.at(1, g);
+        } else {
+            out_left[g] = -1;
+        }
+        out_right[g] = r;
+        tr.at(2, g);
+    }
+    tr.done(dev::gtid());
+}
+
+// --------------------------------------------------------- host driver
+
+bool key_type_ok(uint32_t t)
+{
+    switch (t) {
+    case VO_T_BOOLEAN: case VO_T_TINYINT: case VO_T_SMALLINT: case VO_T_INTEGER: case VO_T_BIGINT:
+    case VO_T_DATE: case VO_T_VARCHAR: case VO_T_UBIGINT: return true;
+    default: return false;
+    }
+}
+
+struct Region {
+    size_t off = 0;
+    size_t take(size_t bytes)
+    {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    }
+};
+
+// The context-owned state a COUNT leaves for the matching WRITE.
+struct JoinState {
+    char *base = nullptr;
+    size_t bytes = 0;
+    const void *lk = nullptr, *rk = nullptr;
+    uint64_t nl = 0, nr = 0, n_pad = 0, n_out = 0;
+    uint32_t type = 0;
+    bool valid = false;
+    uint64_t *cl, *cr0, *cr1, *cnt_l, *cnt_r, *n_out_dev;
+};
+
+template <int NW>
+void distribute(vo_ctx *ctx, const uint64_t *s0, const uint64_t *s1, uint64_t src_rows, const uint64_t *kdev,
+                uint64_t m_pad, uint64_t *a0, uint64_t *a1, uint64_t *b0, uint64_t *b1, Elem<NW> *tl,
+                Elem<NW> *tc, uint64_t **res0, uint64_t **res1)
+{
+    const bool tr = ctx->tracing;
+    launch(ctx, LaunchInfo{"join_pairs_dist_init", grid_for(m_pad), dim3(256), 0, hash_mix(NW, m_pad)},
+           tr ? k_jp_dist_init<NW, true> : k_jp_dist_init<NW, false>, s0, s1, src_rows, kdev, m_pad, a0, a1,
+           trace_slot(ctx));
+    uint64_t *c0 = a0, *c1 = a1, *n0 = b0, *n1 = b1;
+    for (uint64_t j = m_pad / 2; j >= (uint64_t)DH; j >>= 1) {
+        launch(ctx, LaunchInfo{"join_pairs_dist_level", grid_for(m_pad), dim3(256), 0, hash_mix(hash_mix(NW, m_pad), j)},
+               tr ? k_jp_dist_level<NW, true> : k_jp_dist_level<NW, false>, (const uint64_t *)c0,
+               (const uint64_t *)c1, m_pad, (uint32_t)j, n0, n1, trace_slot(ctx));
+        std::swap(c0, n0);
+        std::swap(c1, n1);
+    }
+    const uint32_t nt = (uint32_t)(m_pad / DT);
+    const size_t lds = (size_t)2 * NW * DW * 8;
+    auto kt = tr ? k_jp_dist_tile<NW, true> : k_jp_dist_tile<NW, false>;
+    VO_HIP(hipFuncSetAttribute((const void *)kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    launch(ctx, LaunchInfo{"join_pairs_dist_tile", dim3(nt), dim3(PB), (uint32_t)lds, hash_mix(NW, m_pad)}, kt,
+           (const uint64_t *)c0, (const uint64_t *)c1, n0, n1, tl, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"join_pairs_fill_carry", dim3(1), dim3(kScanThreads), 0, hash_mix(NW, nt)},
+           tr ? k_jp_fill_carry<NW, true> : k_jp_fill_carry<NW, false>, (const Elem<NW> *)tl, nt, tc,
+           trace_slot(ctx));
+    *res0 = n0;
+    *res1 = n1;
+}
+
+JoinState &state_of(vo_ctx *ctx)
+{
+    if (!ctx->join_pairs_state) ctx->join_pairs_state = new JoinState();
+    return *(JoinState *)ctx->join_pairs_state;
+}
+
+} // namespace
+
+void join_pairs_state_free(vo_ctx *ctx)
+{
+    if (!ctx->join_pairs_state) return;
+    JoinState *s = (JoinState *)ctx->join_pairs_state;
+    if (s->base) (void)hipFree(s->base);
+    delete s;
+    ctx->join_pairs_state = nullptr;
+}
+
+uint64_t join_pairs_last_count(vo_ctx *ctx) { return ctx->join_pairs_state ? state_of(ctx).n_out : 0; }
+
+uint64_t op_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args &a)
+{
+    VO_CHECK(key_type_ok(a.type_id), a.type_id == VO_T_FLOAT || a.type_id == VO_T_DOUBLE ? VO_ERR_UNSUPPORTED
+                                                                                          : VO_ERR_INVALID,
+             "join keys must be an integer, date or ordinal type");
+    VO_CHECK((a.left_keys || a.n_left == 0) && (a.right_keys || a.n_right == 0), VO_ERR_INVALID, "null key column");
+    VO_CHECK(a.n_left < (1ull << 31) && a.n_right < (1ull << 31), VO_ERR_INVALID, "row counts must be below 2^31");
+    uint64_t n_pad = PT;
+    while (n_pad < a.n_left + a.n_right) n_pad <<= 1;
+    VO_CHECK(n_pad <= (1ull << 29), VO_ERR_INVALID, "join union above 2^29 rows");
+    JoinState &st = state_of(ctx);
+    st.valid = false;
+    const uint32_t nt = (uint32_t)(n_pad / PT);
+    // context-owned state: compacted columns and counts
+    Region sr;
+    const size_t o_cl = sr.take(n_pad * 8), o_cr0 = sr.take(n_pad * 8), o_cr1 = sr.take(n_pad * 8),
+                 o_nout = sr.take(8), o_cntl = sr.take(compact_rows2_scratch(n_pad)),
+                 o_cntr = sr.take(compact_rows2_scratch(n_pad));
+    if (st.bytes < sr.off) {
+        VO_HIP(hipStreamSynchronize(ctx->stream));
+        if (st.base) VO_HIP(hipFree(st.base));
+        st.base = nullptr;
+        st.bytes = 0;
+        VO_HIP(hipMalloc((void **)&st.base, sr.off));
+        st.bytes = sr.off;
+    }
+    st.cl = (uint64_t *)(st.base + o_cl);
+    st.cr0 = (uint64_t *)(st.base + o_cr0);
+    st.cr1 = (uint64_t *)(st.base + o_cr1);
+    st.n_out_dev = (uint64_t *)(st.base + o_nout);
+    st.cnt_l = (uint64_t *)(st.base + o_cntl);
+    st.cnt_r = (uint64_t *)(st.base + o_cntr);
+    // per-call scratch: sorted union, aux, masks, tile summaries
+    Region r;
+    const size_t o_k = r.take(n_pad * 8), o_v = r.take(n_pad * 8), o_aux = r.take(n_pad * 4), o_ml = r.take(n_pad),
+                 o_mr = r.take(n_pad), o_bt = r.take(nt * sizeof(Min2)), o_bc = r.take(nt * sizeof(Min2)),
+                 o_ft = r.take(nt * sizeof(Fwd)), o_fc = r.take(nt * sizeof(Fwd));
+    char *base = (char *)scratch(ctx, r.off);
+    uint64_t *keys = (uint64_t *)(base + o_k), *vals = (uint64_t *)(base + o_v);
+    uint32_t *aux = (uint32_t *)(base + o_aux);
+    uint8_t *ml = (uint8_t *)(base + o_ml), *mr = (uint8_t *)(base + o_mr);
+    Min2 *bt = (Min2 *)(base + o_bt), *bc = (Min2 *)(base + o_bc);
+    Fwd *ft = (Fwd *)(base + o_ft), *fc = (Fwd *)(base + o_fc);
+    const bool tr = ctx->tracing;
+
+    launch(ctx, LaunchInfo{"join_pairs_build", grid_for(n_pad), dim3(256), 0, hash_mix(a.type_id, n_pad)},
+           tr ? k_jp_build<true> : k_jp_build<false>, a.left_keys, a.n_left, a.right_keys, a.n_right, a.type_id,
+           n_pad, keys, vals, trace_slot(ctx));
+    sort_kv(ctx, keys, vals, n_pad);
+    launch(ctx, LaunchInfo{"join_pairs_bwd_tiles", dim3(nt), dim3(PB), 0, n_pad},
+           tr ? k_jp_bwd_tiles<true> : k_jp_bwd_tiles<false>, (const uint64_t *)keys, (const uint64_t *)vals, n_pad,
+           bt, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"join_pairs_bwd_carry", dim3(1), dim3(kScanThreads), 0, nt},
+           tr ? k_jp_bwd_carry<true> : k_jp_bwd_carry<false>, (const Min2 *)bt, nt, bc, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"join_pairs_bwd_rows", dim3(nt), dim3(PB), 0, n_pad},
+           tr ? k_jp_bwd_rows<true> : k_jp_bwd_rows<false>, (const uint64_t *)keys, (const uint64_t *)vals, n_pad,
+           (const Min2 *)bc, aux, ft, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"join_pairs_fwd_carry", dim3(1), dim3(kScanThreads), 0, nt},
+           tr ? k_jp_fwd_carry<true> : k_jp_fwd_carry<false>, (const Fwd *)ft, nt, fc, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"join_pairs_fwd_rows", dim3(nt), dim3(PB), 0, n_pad},
+           tr ? k_jp_fwd_rows<true> : k_jp_fwd_rows<false>, (const uint64_t *)keys, (const uint64_t *)vals,
+           (const uint32_t *)aux, n_pad, (const Fwd *)fc, st.cl, ml, st.cr0, st.cr1, mr, st.n_out_dev,
+           trace_slot(ctx));
+    compact_rows1(ctx, st.cl, ml, n_pad, st.cnt_l);
+    compact_rows2(ctx, st.cr0, st.cr1, mr, n_pad, st.cnt_r);
+    if (a.out_count)
+        VO_HIP(hipMemcpyAsync(a.out_count, st.n_out_dev, 8, hipMemcpyDeviceToDevice, ctx->stream));
+    uint64_t n_out = 0;
+    VO_HIP(hipMemcpyAsync(&n_out, st.n_out_dev, 8, hipMemcpyDeviceToHost, ctx->stream));
+    VO_HIP(hipStreamSynchronize(ctx->stream));
+    st.lk = a.left_keys;
+    st.rk = a.right_keys;
+    st.nl = a.n_left;
+    st.nr = a.n_right;
+    st.type = a.type_id;
+    st.n_pad = n_pad;
+    st.n_out = n_out;
+    st.valid = true;
+    return n_out;
+}
+
+void op_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args &a)
+{
+    JoinState &st = state_of(ctx);
+    VO_CHECK(st.valid && st.lk == a.left_keys && st.rk == a.right_keys && st.nl == a.n_left && st.nr == a.n_right &&
+                 st.type == a.type_id,
+             VO_ERR_INVALID, "join WRITE needs the COUNT of the same key columns first");
+    const uint64_t n_out = st.n_out;
+    VO_CHECK(a.out_cap >= n_out, VO_ERR_INVALID, "out_cap below the pair count");
+    VO_CHECK(n_out <= (1ull << 29), VO_ERR_UNSUPPORTED, "join output above 2^29 pairs");
+    VO_CHECK(a.out_cap == 0 || (a.out_left && a.out_right), VO_ERR_INVALID, "null output buffer");
+    const bool tr = ctx->tracing;
+    if (n_out == 0) {
+        if (a.out_cap)
+            launch(ctx, LaunchInfo{"join_pairs_out", grid_for(a.out_cap), dim3(256), 0, hash_mix(0, a.out_cap)},
+                   tr ? k_jp_out<true> : k_jp_out<false>, (const uint32_t *)nullptr, (uint64_t)0, (uint64_t)0,
+                   a.out_cap, a.out_left, a.out_right, trace_slot(ctx));
+        return;
+    }
+    uint64_t m_pad = 8192;
+    while (m_pad < n_out) m_pad <<= 1;
+    const uint32_t nt = (uint32_t)(m_pad / DT);
+    Region r;
+    const size_t o_a0 = r.take(m_pad * 8), o_a1 = r.take(m_pad * 8), o_b0 = r.take(m_pad * 8),
+                 o_b1 = r.take(m_pad * 8), o_sk = r.take(m_pad * 8), o_sp = r.take(m_pad * 4),
+                 o_tl = r.take(nt * sizeof(Elem<2>)), o_tc = r.take(nt * sizeof(Elem<2>));
+    char *base = (char *)scratch(ctx, r.off);
+    uint64_t *a0 = (uint64_t *)(base + o_a0), *a1 = (uint64_t *)(base + o_a1);
+    uint64_t *b0 = (uint64_t *)(base + o_b0), *b1 = (uint64_t *)(base + o_b1);
+    uint64_t *skeys = (uint64_t *)(base + o_sk);
+    uint32_t *spay = (uint32_t *)(base + o_sp);
+    const uint64_t src_rows = std::min(st.n_pad, m_pad);
+    uint64_t *d0 = nullptr, *d1 = nullptr;
+    // left copy: one word (target << 32 | row)
+    {
+        Elem<1> *tl = (Elem<1> *)(base + o_tl), *tc = (Elem<1> *)(base + o_tc);
+        distribute<1>(ctx, st.cl, nullptr, src_rows, st.cnt_l, m_pad, a0, nullptr, b0, nullptr, tl, tc, &d0, &d1);
+        launch(ctx, LaunchInfo{"join_pairs_fill_rows", dim3(nt), dim3(PB), 0, hash_mix(hash_mix(1, m_pad), a.out_cap)},
+               tr ? k_jp_fill_rows<1, 0, true> : k_jp_fill_rows<1, 0, false>, (const uint64_t *)d0,
+               (const uint64_t *)nullptr, (const Elem<1> *)tc, n_out, a.out_cap, a.out_left, (uint64_t *)nullptr,
+               (uint32_t *)nullptr, trace_slot(ctx));
+    }
+    // right copy: (target << 32 | row, (B + b) | m1 << 32)
+    {
+        Elem<2> *tl = (Elem<2> *)(base + o_tl), *tc = (Elem<2> *)(base + o_tc);
+        distribute<2>(ctx, st.cr0, st.cr1, src_rows, st.cnt_r, m_pad, a0, a1, b0, b1, tl, tc, &d0, &d1);
+        launch(ctx, LaunchInfo{"join_pairs_fill_rows", dim3(nt), dim3(PB), 0, hash_mix(2, m_pad)},
+               tr ? k_jp_fill_rows<2, 1, true> : k_jp_fill_rows<2, 1, false>, (const uint64_t *)d0,
+               (const uint64_t *)d1, (const Elem<2> *)tc, n_out, a.out_cap, (int64_t *)nullptr, skeys, spay,
+               trace_slot(ctx));
+    }
+    sort_kp_full(ctx, skeys, spay, m_pad);
+    launch(ctx, LaunchInfo{"join_pairs_out", grid_for(a.out_cap), dim3(256), 0, hash_mix(m_pad, a.out_cap)},
+           tr ? k_jp_out<true> : k_jp_out<false>, (const uint32_t *)spay, m_pad, n_out, a.out_cap, a.out_left,
+           a.out_right, trace_slot(ctx));
+}
+
+} // namespace vo

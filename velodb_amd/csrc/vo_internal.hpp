@@ -110,6 +110,9 @@ struct vo_ctx {
 
     // persistent queue (queue.hip)
     void *queue = nullptr;
+
+    // equi-join routing state kept from COUNT to WRITE (join_pairs.hip)
+    void *join_pairs_state = nullptr;
 };
 
 namespace vo {
@@ -190,6 +193,10 @@ void op_compact(vo_ctx *ctx, const vo_compact_args &a);
 void op_shuffle(vo_ctx *ctx, const vo_shuffle_args &a);
 void op_filter_sum(vo_ctx *ctx, const vo_filter_args &a);
 void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a);
+uint64_t op_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args &a); // returns n_out (synchronises)
+void op_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args &a);
+void join_pairs_state_free(vo_ctx *ctx);
+uint64_t join_pairs_last_count(vo_ctx *ctx); // n_out of the last COUNT
 void op_gen_sort_input(vo_ctx *ctx, uint64_t seed, uint64_t *keys, uint32_t *payload, uint64_t n);
 void op_gen_fact(vo_ctx *ctx, uint64_t seed, int32_t *day, int64_t *qty, int64_t *price, int64_t *disc,
                  uint64_t n);
@@ -216,6 +223,7 @@ void sort_kp_full(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n_pad
 // the selected count lands in *count_dev, followed by
 // compact_rows2_scratch(n_pad) - 8 bytes of tile arrays the call owns.
 void compact_rows2(vo_ctx *ctx, uint64_t *c0, uint64_t *c1, const uint8_t *mask, uint64_t n_pad, uint64_t *count_dev);
+void compact_rows1(vo_ctx *ctx, uint64_t *c0, const uint8_t *mask, uint64_t n_pad, uint64_t *count_dev);
 size_t compact_rows2_scratch(uint64_t n_pad);
 
 } // namespace vo
