@@ -237,6 +237,70 @@ def bench_join(args):
     ctx.close()
 
 
+def bench_join_pairs(args):
+    """Oblivious equi-join into row-index pairs (COUNT + WRITE, the
+    SortMergeJoinOperator path) on the join workload's key columns: 1.5M
+    unique dimension keys x 6M foreign keys -> 6M pairs; union and output
+    both padded to 2^23."""
+    import numpy as np
+    import torch
+
+    import velodb_amd
+    from velodb_amd import _abi
+    n_dim, n_fact = 1_500_000, 6_000_000
+    rng = np.random.default_rng(1)
+    dk = (rng.permutation(n_dim) + 1).astype(np.int32)
+    fk = dk[rng.integers(0, n_dim, n_fact)]
+    dev = torch.device("cuda:0")
+    d_dk, d_fk = torch.from_numpy(dk).to(dev), torch.from_numpy(fk).to(dev)
+    cap = 1 << 23
+    ol = torch.empty(cap, dtype=torch.int64, device=dev)
+    orr = torch.empty(cap, dtype=torch.int64, device=dev)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+
+    def step():
+        n = ctx.join_pairs_count(d_dk, n_dim, d_fk, n_fact, _abi.T_INTEGER)
+        ctx.join_pairs_write(d_dk, n_dim, d_fk, n_fact, _abi.T_INTEGER, ol, orr, cap)
+        return n
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.kernel_stats_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        n = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # the output is the join: equal keys, each fact row once, ordered
+    l, r = ol[:n], orr[:n]
+    ok = n == n_fact and bool(torch.all(d_dk[l] == d_fk[r])) and int(torch.unique(r).numel()) == n_fact
+    ok = ok and bool(torch.all(d_dk[l][1:] >= d_dk[l][:-1]))
+    fams = ["join_pairs_build", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2", "sort_global_s3",
+            "sort_global_s4", "sort_hybrid_s6", "sort_hybrid_s7", "sort_hybrid_s8", "join_pairs_bwd_tiles",
+            "join_pairs_bwd_carry", "join_pairs_bwd_rows", "join_pairs_fwd_carry", "join_pairs_fwd_rows",
+            "compact_count", "compact_scan", "compact_local", "compact_hybrid", "compact_global",
+            "join_pairs_dist_init_w1", "join_pairs_dist_init_w2", "join_pairs_dist_level_w1",
+            "join_pairs_dist_level_w2", "join_pairs_dist_tile_w1", "join_pairs_dist_tile_w2",
+            "join_pairs_fill_carry", "join_pairs_fill_left", "join_pairs_fill_right", "join_pairs_out"]
+    stats = _family_stats(ctx, fams)
+    # algorithmic bytes per padded row and launch: a distribution level reads
+    # and writes every slot once (8 or 16 B each way; the row j before a slot
+    # is the same data, read by another thread); the scans read the sorted
+    # union (16 B) and write their outputs
+    bpr = {"join_pairs_dist_level_w1": 16, "join_pairs_dist_level_w2": 32, "join_pairs_bwd_rows": 20,
+           "join_pairs_fwd_rows": 46}
+    n_pad = 1 << 23
+    _line("oblivious equi-join into row pairs (Mrows/s)", (n_dim + n_fact) / dt / 1e6, "Mrows/s", args.steps,
+          args.warmup, dt * 1e3, "int32 keys / int64 row ids",
+          {"workload": "equi-join of 1.5M unique keys x 6M foreign keys into 6M (left, right) row pairs "
+                       "(union and output padded to 2^23), COUNT + WRITE", "n_left": n_dim, "n_right": n_fact,
+           "n_out": n, "output_equals_join": bool(ok)},
+          roofline=_roofline(stats, bpr, n_pad),
+          kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()})
+    ctx.close()
+
+
 def bench_batch(args):
     """4096 independent sorts, sizes log-uniform in [2^10, 2^20]
     (numpy default_rng(0)), each padded to its power of two; one batched call
@@ -417,7 +481,8 @@ def main():
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", type=int, default=0)
-    ap.add_argument("--workload", choices=["sort", "filter", "join", "batch", "serve", "queue"], default="sort",
+    ap.add_argument("--workload", choices=["sort", "filter", "join", "join_pairs", "batch", "serve", "queue"],
+                    default="sort",
                     help="sort = the headline line; the others print their own line")
     ap.add_argument("--autotune", action="store_true", help="batch: run the autotuner first")
     ap.add_argument("--slots", type=int, default=16, help="serve: concurrent query slots")
@@ -435,8 +500,8 @@ def main():
               "(no cross-GPU exchange stages are built); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.workload != "sort":
-        {"filter": bench_filter, "join": bench_join, "batch": bench_batch, "serve": bench_serve,
-         "queue": bench_queue}[args.workload](args)
+        {"filter": bench_filter, "join": bench_join, "join_pairs": bench_join_pairs, "batch": bench_batch,
+         "serve": bench_serve, "queue": bench_queue}[args.workload](args)
         return
 
     import torch

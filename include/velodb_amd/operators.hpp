@@ -12,6 +12,9 @@
  *   SortOperator            include/operator/sort_operator.hpp:15-19, src/operator/sort_operator.cpp:21-134
  *   FilterCompactionOperator include/operator/filter_compaction_operator.hpp:14-17,
  *                           src/operator/filter_compaction_operator.cpp:65-184
+ *   SortMergeJoinOperator   include/operator/sort_merge_join_operator.hpp:14-33,
+ *                           src/operator/sort_merge_join_operator.cpp:27-257
+ *   JoinType                include/operator/join_type.hpp
  * and adds PKFKJoinGroupSumOperator for the oblivious join + GROUP BY.
  *
  * Columns live in device memory with a power-of-two capacity (at least 4096
@@ -228,6 +231,17 @@ public:
         void *p = nullptr;
         hip_check(hipMalloc(&p, capacity_), "hipMalloc");
         return static_cast<uint8_t *>(p);
+    }
+    // a column over buffers the caller allocated (createFromDeviceBuffers,
+    // column.cpp:90-109): takes ownership
+    static Column createFromDeviceBuffers(uint32_t type_id, void *data, uint8_t *bitmap, size_t capacity)
+    {
+        Column c;
+        c.type_ = type_id;
+        c.capacity_ = capacity;
+        c.data_ = data;
+        c.bitmap_ = bitmap;
+        return c;
     }
     // take ownership of new buffers, freeing the old ones (setFromDeviceBuffers)
     void setFromDeviceBuffers(void *data, uint8_t *bitmap)
@@ -476,6 +490,124 @@ private:
     std::unique_ptr<AbstractOperator> child_;
     size_t mask_index_;
     std::vector<bool> compact_;
+};
+
+enum class JoinType { INNER, LEFT, RIGHT, FULL };
+
+// Equi-join of two inputs on one key column each (SortMergeJoinOperator,
+// sort_merge_join_operator.cpp:27-257): the pairs of row indices with equal
+// keys come from the oblivious OP_SORT_MERGE_JOIN_COUNT / _WRITE (every
+// input row read and written by every pass; the pair count is the one value
+// that leaves the device, as in the reference), then every column and null
+// bitmap of both inputs is gathered by OP_PERMUTE, left columns first, as
+// :151-235 does.  The pair computation is oblivious; the permute reads each
+// input at the pair indices, which depend on the data, exactly as the
+// reference's does.  The inputs need not be
+// sorted (the reference requires both sorted by key, :58-59).  The output
+// rows come ordered by (key, left row, right row) with a power-of-two
+// capacity; one batch, then an empty one.  INNER joins only, as in the
+// reference.
+class SortMergeJoinOperator : public AbstractOperator {
+public:
+    SortMergeJoinOperator(ExecutionContext &context, std::unique_ptr<AbstractOperator> left_child,
+                          std::unique_ptr<AbstractOperator> right_child, std::pair<size_t, size_t> join_key_indices,
+                          JoinType join_type = JoinType::INNER)
+        : AbstractOperator(context), left_(std::move(left_child)), right_(std::move(right_child)),
+          keys_(join_key_indices), type_(join_type)
+    {
+    }
+
+    Result<RowBatch> next() override
+    {
+        if (joined_) return Result<RowBatch>::success(RowBatch());
+        joined_ = true;
+        if (type_ != JoinType::INNER) return Result<RowBatch>::failure("SortMergeJoinOperator supports INNER joins");
+        auto l = left_->next();
+        if (!l) return l;
+        RowBatch lb = std::move(l).value();
+        if (lb.getRowCount() == 0) return Result<RowBatch>::success(RowBatch());
+        auto r = right_->next();
+        if (!r) return r;
+        RowBatch rb = std::move(r).value();
+        if (rb.getRowCount() == 0) return Result<RowBatch>::success(RowBatch());
+        if (keys_.first >= lb.getColumnCount()) return Result<RowBatch>::failure("Left join key index out of bounds");
+        if (keys_.second >= rb.getColumnCount())
+            return Result<RowBatch>::failure("Right join key index out of bounds");
+        try {
+            return Result<RowBatch>::success(run(lb, rb));
+        } catch (const std::exception &e) {
+            return Result<RowBatch>::failure(e.what());
+        }
+    }
+
+private:
+    RowBatch run(RowBatch &lb, RowBatch &rb)
+    {
+        Column &lk = lb.getColumn(keys_.first), &rk = rb.getColumn(keys_.second);
+        if (lk.getTypeId() != rk.getTypeId()) throw ExecutionError("SortMergeJoinOperator: key types differ");
+        TaskManager &tm = context_.getTaskManager();
+        uint64_t *d_count = nullptr;
+        hip_check(hipMalloc(reinterpret_cast<void **>(&d_count), sizeof(uint64_t)), "hipMalloc");
+        std::unique_ptr<uint64_t, void (*)(uint64_t *)> guard(d_count, [](uint64_t *p) { (void)hipFree(p); });
+        vo_join_pairs_args a{};
+        a.left_keys = lk.rawData();
+        a.n_left = lb.getRowCount();
+        a.right_keys = rk.rawData();
+        a.n_right = rb.getRowCount();
+        a.type_id = lk.getTypeId();
+        a.out_count = d_count;
+        vo_command count{};
+        count.opcode = VO_OP_SORT_MERGE_JOIN_COUNT;
+        count.args.join_pairs = a;
+        tm.waitCommand(tm.submitCommand(count));
+        uint64_t n_out = 0;
+        hip_check(hipMemcpy(&n_out, d_count, sizeof(n_out), hipMemcpyDeviceToHost), "hipMemcpy");
+        if (n_out == 0) return RowBatch();
+        const size_t cap = next_pow2(n_out);
+        Column li = Column(VO_T_BIGINT, cap, false), ri = Column(VO_T_BIGINT, cap, false);
+        a.out_count = nullptr;
+        a.out_left = static_cast<int64_t *>(li.rawData());
+        a.out_right = static_cast<int64_t *>(ri.rawData());
+        a.out_cap = cap;
+        vo_command write{};
+        write.opcode = VO_OP_SORT_MERGE_JOIN_WRITE;
+        write.args.join_pairs = a;
+        tm.waitCommand(tm.submitCommand(write));
+        std::vector<Column> out;
+        out.reserve(lb.getColumnCount() + rb.getColumnCount());
+        uint64_t last = 0;
+        auto gather = [&](RowBatch &b, const Column &idx) {
+            for (size_t i = 0; i < b.getColumnCount(); ++i) {
+                Column &c = b.getColumn(i);
+                void *data = nullptr;
+                hip_check(hipMalloc(&data, cap * type_width(c.getTypeId())), "hipMalloc");
+                uint8_t *bits = nullptr;
+                if (c.rawBitmapData()) hip_check(hipMalloc(reinterpret_cast<void **>(&bits), cap), "hipMalloc");
+                out.push_back(Column::createFromDeviceBuffers(c.getTypeId(), data, bits, cap));
+                vo_command p{};
+                p.opcode = VO_OP_PERMUTE;
+                p.args.permute = {data, c.rawData(), static_cast<const int64_t *>(idx.rawData()), n_out,
+                                  c.getTypeId()};
+                last = tm.submitCommand(p);
+                if (bits) {
+                    vo_command pb{};
+                    pb.opcode = VO_OP_PERMUTE;
+                    pb.args.permute = {bits, c.rawBitmapData(), static_cast<const int64_t *>(idx.rawData()), n_out,
+                                       VO_T_BOOLEAN};
+                    last = tm.submitCommand(pb);
+                }
+            }
+        };
+        gather(lb, li);
+        gather(rb, ri);
+        if (last) tm.waitCommand(last);
+        return RowBatch(std::move(out), n_out);
+    }
+
+    std::unique_ptr<AbstractOperator> left_, right_;
+    std::pair<size_t, size_t> keys_;
+    JoinType type_;
+    bool joined_ = false;
 };
 
 // Oblivious PK/FK join + SUM(value) GROUP BY group: dimension batch (key

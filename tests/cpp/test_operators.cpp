@@ -15,6 +15,7 @@
 #include <numeric>
 #include <random>
 #include <string>
+#include <tuple>
 #include <vector>
 
 using namespace velodb_amd;
@@ -338,6 +339,104 @@ static void test_join_groupsum(ExecutionContext &ctx, bool dense)
     CHECK(b2.getColumn(1).toHost<int64_t>(25) == want);
 }
 
+// two-column INTEGER batch (id, value)
+static RowBatch id_value(const std::vector<int32_t> &id, const std::vector<int32_t> &v)
+{
+    std::vector<Column> c;
+    c.push_back(Column::fromHost(VO_T_INTEGER, id.data(), id.size()));
+    c.push_back(Column::fromHost(VO_T_INTEGER, v.data(), v.size()));
+    return RowBatch(std::move(c), id.size());
+}
+
+static void test_sort_merge_join(ExecutionContext &ctx)
+{
+    // tests/query/test_join_execution.cpp:12-66: A(id 0..4, x = 10 id) JOIN
+    // B(id 0..4, y = 100 + id) ON A.id = B.id -> 5 rows, 4 columns
+    std::vector<int32_t> id = {0, 1, 2, 3, 4}, x = {0, 10, 20, 30, 40}, y = {100, 101, 102, 103, 104};
+    SortMergeJoinOperator j(ctx, std::make_unique<BatchSourceOperator>(ctx, id_value(id, x)),
+                            std::make_unique<BatchSourceOperator>(ctx, id_value(id, y)), {0, 0});
+    auto r = j.next();
+    CHECK(static_cast<bool>(r));
+    if (!r) return;
+    RowBatch b = std::move(r).value();
+    CHECK(b.getRowCount() == 5);
+    CHECK(b.getColumnCount() == 4);
+    CHECK(b.getColumn(0).toHost<int32_t>(5) == id);
+    CHECK(b.getColumn(1).toHost<int32_t>(5) == x);
+    CHECK(b.getColumn(2).toHost<int32_t>(5) == id);
+    CHECK(b.getColumn(3).toHost<int32_t>(5) == y);
+    auto end = j.next();
+    CHECK(static_cast<bool>(end) && end.value().getRowCount() == 0);
+
+    // tests/query/test_multi_join.cpp:14-86: A(1,2,3) JOIN B(1,2,4) JOIN C(1,3,5) -> 1 row (id 1)
+    auto ab = std::make_unique<SortMergeJoinOperator>(
+        ctx, std::make_unique<BatchSourceOperator>(ctx, id_value({1, 2, 3}, {10, 20, 30})),
+        std::make_unique<BatchSourceOperator>(ctx, id_value({1, 2, 4}, {100, 200, 400})), std::make_pair(0, 0));
+    SortMergeJoinOperator abc(ctx, std::move(ab),
+                              std::make_unique<BatchSourceOperator>(ctx, id_value({1, 3, 5}, {1000, 3000, 5000})),
+                              {2, 0});
+    auto r3 = abc.next();
+    CHECK(static_cast<bool>(r3));
+    if (!r3) return;
+    RowBatch b3 = std::move(r3).value();
+    CHECK(b3.getRowCount() == 1);
+    CHECK(b3.getColumnCount() == 6);
+    std::vector<int32_t> row;
+    for (size_t c = 0; c < 6; ++c) row.push_back(b3.getColumn(c).toHost<int32_t>(1)[0]);
+    CHECK((row == std::vector<int32_t>{1, 10, 1, 100, 1, 1000}));
+    auto e3 = abc.next();
+    CHECK(static_cast<bool>(e3) && e3.value().getRowCount() == 0);
+
+    // many-to-many with a nullable payload against host nested loops
+    const size_t nl = 3000, nr = 2000;
+    std::mt19937_64 rng(5);
+    std::vector<int64_t> lk(nl), rk(nr), lv(nl), rv(nr);
+    std::vector<uint8_t> lvalid(nl);
+    for (size_t i = 0; i < nl; ++i) {
+        lk[i] = (int64_t)(rng() % 300) - 150;
+        lv[i] = (int64_t)i * 7;
+        lvalid[i] = (uint8_t)(rng() & 1);
+    }
+    for (size_t i = 0; i < nr; ++i) {
+        rk[i] = (int64_t)(rng() % 300) - 150;
+        rv[i] = -(int64_t)i;
+    }
+    std::vector<std::tuple<int64_t, size_t, size_t>> want;
+    for (size_t i = 0; i < nl; ++i)
+        for (size_t k = 0; k < nr; ++k)
+            if (lk[i] == rk[k]) want.emplace_back(lk[i], i, k);
+    std::sort(want.begin(), want.end());
+    std::vector<Column> lc, rc;
+    lc.push_back(Column::fromHost(VO_T_BIGINT, lk.data(), nl));
+    lc.push_back(Column::fromHost(VO_T_BIGINT, lv.data(), nl, lvalid.data()));
+    rc.push_back(Column::fromHost(VO_T_BIGINT, rv.data(), nr));
+    rc.push_back(Column::fromHost(VO_T_BIGINT, rk.data(), nr));
+    SortMergeJoinOperator mm(ctx, std::make_unique<BatchSourceOperator>(ctx, RowBatch(std::move(lc), nl)),
+                             std::make_unique<BatchSourceOperator>(ctx, RowBatch(std::move(rc), nr)), {0, 1});
+    auto rm = mm.next();
+    CHECK(static_cast<bool>(rm));
+    if (!rm) return;
+    RowBatch bm = std::move(rm).value();
+    const size_t n = want.size();
+    CHECK(bm.getRowCount() == n);
+    if (bm.getRowCount() != n) return;
+    const auto gk = bm.getColumn(0).toHost<int64_t>(n), gv = bm.getColumn(1).toHost<int64_t>(n);
+    const auto gb = bm.getColumn(1).bitmapToHost(n);
+    const auto grv = bm.getColumn(2).toHost<int64_t>(n), grk = bm.getColumn(3).toHost<int64_t>(n);
+    bool ok = true;
+    for (size_t i = 0; i < n; ++i) {
+        const auto &[k, li, ri] = want[i];
+        ok = ok && gk[i] == k && gv[i] == lv[li] && gb[i] == lvalid[li] && grv[i] == rv[ri] && grk[i] == k;
+    }
+    CHECK(ok);
+    // a key index out of bounds is a failure Result, as in the reference
+    SortMergeJoinOperator bad(ctx, std::make_unique<BatchSourceOperator>(ctx, id_value({1}, {1})),
+                              std::make_unique<BatchSourceOperator>(ctx, id_value({1}, {1})), {5, 0});
+    auto rb = bad.next();
+    CHECK(!rb);
+    if (!rb) CHECK(rb.error() == "Left join key index out of bounds");
+}
+
 static void test_errors_are_results(ExecutionContext &ctx)
 {
     // a failure of the library comes back as Result::failure with its message
@@ -398,6 +497,7 @@ int main()
         {"where_order_by", [&] { test_where_order_by(ctx); }},
         {"nullable_bitmaps_travel", [&] { test_nullable_bitmaps_travel(ctx); }},
         {"join_groupsum_sorted", [&] { test_join_groupsum(ctx, false); }},
+        {"sort_merge_join", [&] { test_sort_merge_join(ctx); }},
         {"join_groupsum_dense", [&] { test_join_groupsum(ctx, true); }},
         {"errors_are_results", [&] { test_errors_are_results(ctx); }},
         {"task_manager_queue", [&] { test_task_manager_queue(ctx); }},

@@ -696,12 +696,14 @@ void distribute(vo_ctx *ctx, const uint64_t *s0, const uint64_t *s1, uint64_t sr
                 Elem<NW> *tc, uint64_t **res0, uint64_t **res1)
 {
     const bool tr = ctx->tracing;
-    launch(ctx, LaunchInfo{"join_pairs_dist_init", grid_for(m_pad), dim3(256), 0, hash_mix(NW, m_pad)},
+    launch(ctx, LaunchInfo{NW == 1 ? "join_pairs_dist_init_w1" : "join_pairs_dist_init_w2", grid_for(m_pad), dim3(256),
+                           0, hash_mix(NW, m_pad)},
            tr ? k_jp_dist_init<NW, true> : k_jp_dist_init<NW, false>, s0, s1, src_rows, kdev, m_pad, a0, a1,
            trace_slot(ctx));
     uint64_t *c0 = a0, *c1 = a1, *n0 = b0, *n1 = b1;
     for (uint64_t j = m_pad / 2; j >= (uint64_t)DH; j >>= 1) {
-        launch(ctx, LaunchInfo{"join_pairs_dist_level", grid_for(m_pad), dim3(256), 0, hash_mix(hash_mix(NW, m_pad), j)},
+        launch(ctx, LaunchInfo{NW == 1 ? "join_pairs_dist_level_w1" : "join_pairs_dist_level_w2", grid_for(m_pad),
+                               dim3(256), 0, hash_mix(hash_mix(NW, m_pad), j)},
                tr ? k_jp_dist_level<NW, true> : k_jp_dist_level<NW, false>, (const uint64_t *)c0,
                (const uint64_t *)c1, m_pad, (uint32_t)j, n0, n1, trace_slot(ctx));
         std::swap(c0, n0);
@@ -711,7 +713,9 @@ void distribute(vo_ctx *ctx, const uint64_t *s0, const uint64_t *s1, uint64_t sr
     const size_t lds = (size_t)2 * NW * DW * 8;
     auto kt = tr ? k_jp_dist_tile<NW, true> : k_jp_dist_tile<NW, false>;
     VO_HIP(hipFuncSetAttribute((const void *)kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    launch(ctx, LaunchInfo{"join_pairs_dist_tile", dim3(nt), dim3(PB), (uint32_t)lds, hash_mix(NW, m_pad)}, kt,
+    launch(ctx, LaunchInfo{NW == 1 ? "join_pairs_dist_tile_w1" : "join_pairs_dist_tile_w2", dim3(nt), dim3(PB),
+                           (uint32_t)lds, hash_mix(NW, m_pad)},
+           kt,
            (const uint64_t *)c0, (const uint64_t *)c1, n0, n1, tl, trace_slot(ctx));
     launch(ctx, LaunchInfo{"join_pairs_fill_carry", dim3(1), dim3(kScanThreads), 0, hash_mix(NW, nt)},
            tr ? k_jp_fill_carry<NW, true> : k_jp_fill_carry<NW, false>, (const Elem<NW> *)tl, nt, tc,
@@ -856,7 +860,7 @@ void op_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args &a)
     {
         Elem<1> *tl = (Elem<1> *)(base + o_tl), *tc = (Elem<1> *)(base + o_tc);
         distribute<1>(ctx, st.cl, nullptr, src_rows, st.cnt_l, m_pad, a0, nullptr, b0, nullptr, tl, tc, &d0, &d1);
-        launch(ctx, LaunchInfo{"join_pairs_fill_rows", dim3(nt), dim3(PB), 0, hash_mix(hash_mix(1, m_pad), a.out_cap)},
+        launch(ctx, LaunchInfo{"join_pairs_fill_left", dim3(nt), dim3(PB), 0, hash_mix(hash_mix(1, m_pad), a.out_cap)},
                tr ? k_jp_fill_rows<1, 0, true> : k_jp_fill_rows<1, 0, false>, (const uint64_t *)d0,
                (const uint64_t *)nullptr, (const Elem<1> *)tc, n_out, a.out_cap, a.out_left, (uint64_t *)nullptr,
                (uint32_t *)nullptr, trace_slot(ctx));
@@ -865,7 +869,7 @@ void op_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args &a)
     {
         Elem<2> *tl = (Elem<2> *)(base + o_tl), *tc = (Elem<2> *)(base + o_tc);
         distribute<2>(ctx, st.cr0, st.cr1, src_rows, st.cnt_r, m_pad, a0, a1, b0, b1, tl, tc, &d0, &d1);
-        launch(ctx, LaunchInfo{"join_pairs_fill_rows", dim3(nt), dim3(PB), 0, hash_mix(2, m_pad)},
+        launch(ctx, LaunchInfo{"join_pairs_fill_right", dim3(nt), dim3(PB), 0, hash_mix(2, m_pad)},
                tr ? k_jp_fill_rows<2, 1, true> : k_jp_fill_rows<2, 1, false>, (const uint64_t *)d0,
                (const uint64_t *)d1, (const Elem<2> *)tc, n_out, a.out_cap, (int64_t *)nullptr, skeys, spay,
                trace_slot(ctx));
