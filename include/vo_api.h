@@ -258,12 +258,15 @@ typedef struct vo_join_args {
  *     receives the number of pairs n_out.  The routing state stays in the
  *     context until the next COUNT.
  *   WRITE: expands the state of the last COUNT with the same key pointers
- *     and sizes into out_left / out_right (device, out_cap rows each,
- *     out_cap >= n_out, n_out <= 2^29); rows [n_out, out_cap) are -1.
+ *     and sizes (type_id 0: the COUNT's) into out_left / out_right (device,
+ *     out_cap rows each, out_cap >= n_out, n_out <= 2^29); rows
+ *     [n_out, out_cap) are -1.  out_cap 0 means the power of two >= n_out,
+ *     the size the reference's operator allocates
+ *     (sort_merge_join_operator.cpp:126-133).
  * Every pass reads and writes every row of its padded size; n_out is the
  * only data-dependent value that leaves the device (the reference returns
- * it too).  Uses about 90 bytes of device memory per padded union row and
- * 48 per padded output row. */
+ * it too).  Uses about 46 bytes of device memory per padded union row and
+ * 44 per padded output row (at least 8192 output rows). */
 typedef struct vo_join_pairs_args {
     const void *left_keys;
     uint64_t n_left;

@@ -188,8 +188,10 @@ def test_through_command_queue(vctx):
     ol, orr = empty_dev(8, np.int64), empty_dev(8, np.int64)
     cmd = _abi.Command()
     cmd.opcode = _abi.OP_SORT_MERGE_JOIN_WRITE
-    cmd.args.join_pairs = _abi.JoinPairsArgs(dl.data_ptr(), 4, dr.data_ptr(), 4, T_INTEGER, 0, None,
-                                             ol.data_ptr(), orr.data_ptr(), 8)
+    # as translated from the reference's SortMergeJoinWriteArgs: no type, no
+    # capacity (0: the power of two >= n_out the reference's operator allocates)
+    cmd.args.join_pairs = _abi.JoinPairsArgs(dl.data_ptr(), 4, dr.data_ptr(), 4, 0, 0, None,
+                                             ol.data_ptr(), orr.data_ptr(), 0)
     vctx.wait(vctx.submit(cmd))
     assert from_dev(ol, np.int64).tolist() == [2, 0, 0, 3, 3, -1, -1, -1]
     assert from_dev(orr, np.int64).tolist() == [2, 0, 1, 0, 1, -1, -1, -1]

@@ -828,18 +828,23 @@ void op_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args &a)
 {
     JoinState &st = state_of(ctx);
     VO_CHECK(st.valid && st.lk == a.left_keys && st.rk == a.right_keys && st.nl == a.n_left && st.nr == a.n_right &&
-                 st.type == a.type_id,
+                 (a.type_id == 0 || st.type == a.type_id),
              VO_ERR_INVALID, "join WRITE needs the COUNT of the same key columns first");
     const uint64_t n_out = st.n_out;
-    VO_CHECK(a.out_cap >= n_out, VO_ERR_INVALID, "out_cap below the pair count");
     VO_CHECK(n_out <= (1ull << 29), VO_ERR_UNSUPPORTED, "join output above 2^29 pairs");
-    VO_CHECK(a.out_cap == 0 || (a.out_left && a.out_right), VO_ERR_INVALID, "null output buffer");
+    // out_cap 0: the reference's allocation, the power of two >= n_out
+    uint64_t out_cap = a.out_cap;
+    if (out_cap == 0 && n_out > 0)
+        for (out_cap = 1; out_cap < n_out; out_cap <<= 1) {
+        }
+    VO_CHECK(out_cap >= n_out, VO_ERR_INVALID, "out_cap below the pair count");
+    VO_CHECK(out_cap == 0 || (a.out_left && a.out_right), VO_ERR_INVALID, "null output buffer");
     const bool tr = ctx->tracing;
     if (n_out == 0) {
-        if (a.out_cap)
-            launch(ctx, LaunchInfo{"join_pairs_out", grid_for(a.out_cap), dim3(256), 0, hash_mix(0, a.out_cap)},
+        if (out_cap)
+            launch(ctx, LaunchInfo{"join_pairs_out", grid_for(out_cap), dim3(256), 0, hash_mix(0, out_cap)},
                    tr ? k_jp_out<true> : k_jp_out<false>, (const uint32_t *)nullptr, (uint64_t)0, (uint64_t)0,
-                   a.out_cap, a.out_left, a.out_right, trace_slot(ctx));
+                   out_cap, a.out_left, a.out_right, trace_slot(ctx));
         return;
     }
     uint64_t m_pad = 8192;
@@ -860,9 +865,9 @@ void op_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args &a)
     {
         Elem<1> *tl = (Elem<1> *)(base + o_tl), *tc = (Elem<1> *)(base + o_tc);
         distribute<1>(ctx, st.cl, nullptr, src_rows, st.cnt_l, m_pad, a0, nullptr, b0, nullptr, tl, tc, &d0, &d1);
-        launch(ctx, LaunchInfo{"join_pairs_fill_left", dim3(nt), dim3(PB), 0, hash_mix(hash_mix(1, m_pad), a.out_cap)},
+        launch(ctx, LaunchInfo{"join_pairs_fill_left", dim3(nt), dim3(PB), 0, hash_mix(hash_mix(1, m_pad), out_cap)},
                tr ? k_jp_fill_rows<1, 0, true> : k_jp_fill_rows<1, 0, false>, (const uint64_t *)d0,
-               (const uint64_t *)nullptr, (const Elem<1> *)tc, n_out, a.out_cap, a.out_left, (uint64_t *)nullptr,
+               (const uint64_t *)nullptr, (const Elem<1> *)tc, n_out, out_cap, a.out_left, (uint64_t *)nullptr,
                (uint32_t *)nullptr, trace_slot(ctx));
     }
     // right copy: (target << 32 | row, (B + b) | m1 << 32)
@@ -871,12 +876,12 @@ void op_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args &a)
         distribute<2>(ctx, st.cr0, st.cr1, src_rows, st.cnt_r, m_pad, a0, a1, b0, b1, tl, tc, &d0, &d1);
         launch(ctx, LaunchInfo{"join_pairs_fill_right", dim3(nt), dim3(PB), 0, hash_mix(2, m_pad)},
                tr ? k_jp_fill_rows<2, 1, true> : k_jp_fill_rows<2, 1, false>, (const uint64_t *)d0,
-               (const uint64_t *)d1, (const Elem<2> *)tc, n_out, a.out_cap, (int64_t *)nullptr, skeys, spay,
+               (const uint64_t *)d1, (const Elem<2> *)tc, n_out, out_cap, (int64_t *)nullptr, skeys, spay,
                trace_slot(ctx));
     }
     sort_kp_full(ctx, skeys, spay, m_pad);
-    launch(ctx, LaunchInfo{"join_pairs_out", grid_for(a.out_cap), dim3(256), 0, hash_mix(m_pad, a.out_cap)},
-           tr ? k_jp_out<true> : k_jp_out<false>, (const uint32_t *)spay, m_pad, n_out, a.out_cap, a.out_left,
+    launch(ctx, LaunchInfo{"join_pairs_out", grid_for(out_cap), dim3(256), 0, hash_mix(m_pad, out_cap)},
+           tr ? k_jp_out<true> : k_jp_out<false>, (const uint32_t *)spay, m_pad, n_out, out_cap, a.out_left,
            a.out_right, trace_slot(ctx));
 }
 
