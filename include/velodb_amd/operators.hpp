@@ -15,6 +15,9 @@
  *   SortMergeJoinOperator   include/operator/sort_merge_join_operator.hpp:14-33,
  *                           src/operator/sort_merge_join_operator.cpp:27-257
  *   JoinType                include/operator/join_type.hpp
+ *   ObliviousTable          include/data/oblivious_table.hpp, src/data/oblivious_table.cpp:16-255
+ *   ObliviousTableManager   include/data/oblivious_table_manager.hpp,
+ *                           src/data/oblivious_table_manager.cpp:14-221
  * and adds PKFKJoinGroupSumOperator for the oblivious join + GROUP BY.
  *
  * Columns live in device memory with a power-of-two capacity (at least 4096
@@ -39,9 +42,15 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <random>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -679,6 +688,232 @@ private:
     size_t dk_, dg_, fk_, fv_, max_groups_;
     bool dense_;
     bool done_ = false;
+};
+
+// ------------------------------------------------- oblivious position maps
+
+// A table's position map (record id -> position), re-shuffled after every
+// query that read the table (ObliviousTable, src/data/oblivious_table.cpp).
+// The reference draws sigma with a host Fisher-Yates over std::mt19937_64,
+// uploads it and composes new[i] = old[sigma[i]] on the device, a gather
+// whose addresses follow sigma (:175-205, oblivious_shuffle.cuh:10-21).  Here
+// the map itself goes through OP_SHUFFLE: Philox4x32-10 tags of a fresh
+// 64-bit seed sorted by the bitonic network with the map entries as
+// payload, so new = old composed with a uniform permutation, computed without
+// a host permutation, an upload or a data-dependent address.  The map has a
+// power-of-two capacity; rows >= getRowCount() are not part of it.
+class ObliviousTable {
+public:
+    ObliviousTable(std::string name, const RowBatch *source, TaskManager &tm,
+                   uint64_t seed = std::random_device{}())
+        : name_(std::move(name)), source_(source), tm_(tm), rows_(source ? source->getRowCount() : 0), rng_(seed)
+    {
+        if (rows_ >= (1ull << 32)) throw ExecutionError("ObliviousTable: more than 2^32 rows");
+        pad_ = 1;
+        while (pad_ < rows_) pad_ <<= 1;
+        hip_check(hipMalloc(reinterpret_cast<void **>(&d_active_), pad_ * sizeof(uint32_t)), "hipMalloc");
+        hip_check(hipMalloc(reinterpret_cast<void **>(&d_pending_), pad_ * sizeof(uint32_t)), "hipMalloc");
+        // identity, then the initial shuffle (:22-33)
+        h_map_.resize(pad_);
+        for (size_t i = 0; i < pad_; ++i) h_map_[i] = static_cast<uint32_t>(i);
+        hip_check(hipMemcpy(d_active_, h_map_.data(), pad_ * sizeof(uint32_t), hipMemcpyHostToDevice), "hipMemcpy");
+        vo_shuffle_args a{d_active_, nullptr, pad_, rng_(), rows_};
+        if (rows_) tm_.check(vo_shuffle(tm_.native(), &a));
+        pull();
+    }
+    ObliviousTable(const ObliviousTable &) = delete;
+    ObliviousTable &operator=(const ObliviousTable &) = delete;
+    ~ObliviousTable()
+    {
+        try {
+            waitForShuffle();
+        } catch (...) {
+        }
+        (void)hipFree(d_active_);
+        (void)hipFree(d_pending_);
+    }
+
+    const std::string &getName() const { return name_; }
+    size_t getRowCount() const { return rows_; }
+    const RowBatch *getSourceTable() const { return source_; }
+    uint32_t getPosition(uint32_t record_id) const { return h_map_.at(record_id); }
+    uint32_t *getPositionMapDevice() const { return d_active_; }
+    // upload the host map (after host-side edits), as :152-165
+    void syncPositionMapToGPU()
+    {
+        if (!shuffling_.load())
+            hip_check(hipMemcpy(d_active_, h_map_.data(), pad_ * sizeof(uint32_t), hipMemcpyHostToDevice),
+                      "hipMemcpy");
+    }
+
+    // pending = active, shuffled through the command queue; the active map
+    // stays readable until waitForShuffle swaps them (:171-206)
+    void startAsyncShuffle()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (shuffling_.load()) return;
+        hip_check(hipMemcpy(d_pending_, d_active_, pad_ * sizeof(uint32_t), hipMemcpyDeviceToDevice), "hipMemcpy");
+        vo_command c{};
+        c.opcode = VO_OP_SHUFFLE;
+        c.args.shuffle = {d_pending_, nullptr, pad_, rng_(), rows_};
+        pending_id_ = rows_ ? tm_.submitCommand(c) : 0;
+        shuffling_.store(true);
+        ++shuffles_;
+    }
+    void waitForShuffle()
+    {
+        if (!shuffling_.load()) return;
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!shuffling_.load()) return;
+        if (pending_id_) tm_.waitCommand(pending_id_);
+        std::swap(d_active_, d_pending_);
+        pull();
+        shuffling_.store(false);
+    }
+    bool isShuffling() const { return shuffling_.load(); }
+    size_t getShuffleCount() const { return shuffles_; }
+
+private:
+    void pull()
+    {
+        hip_check(hipMemcpy(h_map_.data(), d_active_, pad_ * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+
+    std::string name_;
+    const RowBatch *source_;
+    TaskManager &tm_;
+    size_t rows_, pad_ = 1;
+    std::vector<uint32_t> h_map_;
+    uint32_t *d_active_ = nullptr, *d_pending_ = nullptr;
+    std::atomic<bool> shuffling_{false};
+    uint64_t pending_id_ = 0;
+    std::mutex mu_;
+    std::mt19937_64 rng_; // seeds of the device shuffles
+    size_t shuffles_ = 0;
+};
+
+// Query lifecycle over the registered tables (ObliviousTableManager,
+// src/data/oblivious_table_manager.cpp): beginQuery waits for the pending
+// shuffles of the tables a query names, endQuery starts a shuffle of every
+// table marked accessed, enrichPositions maps record ids to positions with
+// OP_ENRICH_POSITIONS.
+class ObliviousTableManager {
+public:
+    struct Stats {
+        size_t total_shuffles = 0;
+        size_t tables_registered = 0;
+    };
+
+    explicit ObliviousTableManager(TaskManager &tm) : tm_(tm) {}
+    ~ObliviousTableManager()
+    {
+        try {
+            syncAllShuffles();
+        } catch (...) {
+        }
+    }
+
+    void registerTable(const std::string &name, const RowBatch *source, uint64_t seed = std::random_device{}())
+    {
+        std::lock_guard<std::mutex> lk(tables_mu_);
+        tables_[name] = std::make_unique<ObliviousTable>(name, source, tm_, seed);
+        std::lock_guard<std::mutex> sl(stats_mu_);
+        stats_.tables_registered++;
+    }
+    bool hasTable(const std::string &name) const
+    {
+        std::lock_guard<std::mutex> lk(tables_mu_);
+        return tables_.count(name) != 0;
+    }
+    ObliviousTable &getTable(const std::string &name)
+    {
+        std::lock_guard<std::mutex> lk(tables_mu_);
+        auto it = tables_.find(name);
+        if (it == tables_.end()) throw std::runtime_error("Oblivious table not found: " + name);
+        return *it->second;
+    }
+
+    void beginQuery(const std::vector<std::string> &table_names)
+    {
+        std::lock_guard<std::mutex> lk(query_mu_);
+        accessed_.clear();
+        for (const auto &name : table_names)
+            if (hasTable(name)) {
+                ObliviousTable &t = getTable(name);
+                if (t.isShuffling()) t.waitForShuffle();
+                t.syncPositionMapToGPU();
+            }
+    }
+    void markAccessed(const std::string &name)
+    {
+        std::lock_guard<std::mutex> lk(query_mu_);
+        accessed_.insert(name);
+    }
+    void endQuery()
+    {
+        std::lock_guard<std::mutex> lk(query_mu_);
+        for (const auto &name : accessed_)
+            if (hasTable(name)) {
+                getTable(name).startAsyncShuffle();
+                std::lock_guard<std::mutex> sl(stats_mu_);
+                stats_.total_shuffles++;
+            }
+    }
+    // finish the background shuffles, then one timed shuffle of every table
+    // the last query accessed (:157-181); the times are kept per table
+    void profileAccessedTableShuffles()
+    {
+        std::lock_guard<std::mutex> lk(query_mu_);
+        syncAllShuffles();
+        for (const auto &name : accessed_) {
+            if (!hasTable(name)) continue;
+            ObliviousTable &t = getTable(name);
+            const auto t0 = std::chrono::steady_clock::now();
+            t.startAsyncShuffle();
+            t.waitForShuffle();
+            const auto us =
+                std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count();
+            std::lock_guard<std::mutex> sl(stats_mu_);
+            shuffle_us_[name] = static_cast<double>(us);
+        }
+    }
+    double controlledShuffleMicros(const std::string &name) const
+    {
+        std::lock_guard<std::mutex> sl(stats_mu_);
+        auto it = shuffle_us_.find(name);
+        return it == shuffle_us_.end() ? -1.0 : it->second;
+    }
+
+    void enrichPositions(const std::string &table_name, const uint32_t *d_record_ids, uint32_t *d_positions, size_t n)
+    {
+        ObliviousTable &t = getTable(table_name);
+        vo_command c{};
+        c.opcode = VO_OP_ENRICH_POSITIONS;
+        c.args.enrich_positions = {d_record_ids, t.getPositionMapDevice(), d_positions, n};
+        tm_.waitCommand(tm_.submitCommand(c));
+    }
+
+    void syncAllShuffles()
+    {
+        std::lock_guard<std::mutex> lk(tables_mu_);
+        for (auto &kv : tables_)
+            if (kv.second->isShuffling()) kv.second->waitForShuffle();
+    }
+    Stats getStats() const
+    {
+        std::lock_guard<std::mutex> sl(stats_mu_);
+        return stats_;
+    }
+
+private:
+    TaskManager &tm_;
+    std::map<std::string, std::unique_ptr<ObliviousTable>> tables_;
+    mutable std::mutex tables_mu_;
+    std::set<std::string> accessed_;
+    std::mutex query_mu_;
+    mutable std::mutex stats_mu_;
+    Stats stats_;
+    std::map<std::string, double> shuffle_us_;
 };
 
 } // namespace velodb_amd

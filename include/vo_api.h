@@ -188,12 +188,15 @@ typedef struct vo_compact_args {
 
 /* OP_SHUFFLE: oblivious shuffle of n_pad rows: tag_i = Philox4x32-10(i, seed),
  * sort by (tag, i) and apply the permutation to the payload column.
- * perm_out (optional) receives the source row of each output row. */
+ * perm_out (optional) receives the source row of each output row.  With
+ * 0 < n < n_pad, rows >= n keep their place and the first n rows receive a
+ * permutation of themselves (tag of a row >= n: the maximum). */
 typedef struct vo_shuffle_args {
     uint32_t *payload; /* n_pad rows, shuffled in place */
     uint32_t *perm_out;
     uint64_t n_pad;
     uint64_t seed;
+    uint64_t n;        /* live rows; 0 -> n_pad */
 } vo_shuffle_args;
 
 /* OP_FILTER_SUM: Q6-style predicate
@@ -363,7 +366,11 @@ VO_EXPORT int vo_queue_running(const vo_ctx *ctx);
 /* TaskManager::submitCommand / waitCommand.  vo_submit returns the sequence
  * id (> 0) and stores it in cmd->sequence_id, or 0 on error (see
  * vo_last_error).  vo_wait returns VO_ERR_TIMEOUT after timeout_ms
- * (0 -> the context default). */
+ * (0 -> the context default).  In persistent mode an opcode the resident
+ * kernel does not run is executed when vo_submit is called: the commands
+ * submitted before it finish, it runs on the context stream, and its
+ * sequence id is that of a NOP queued after it, so the order of commands
+ * and the meaning of vo_wait are unchanged. */
 VO_EXPORT uint64_t vo_submit(vo_ctx *ctx, vo_command *cmd);
 VO_EXPORT int vo_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms);
 VO_EXPORT int vo_poll(vo_ctx *ctx, uint64_t seq); /* 1 done, 0 pending, <0 -error */

@@ -14,6 +14,7 @@
 #include <functional>
 #include <numeric>
 #include <random>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -437,6 +438,158 @@ static void test_sort_merge_join(ExecutionContext &ctx)
     if (!rb) CHECK(rb.error() == "Left join key index out of bounds");
 }
 
+// tests/data/test_oblivious_table.cpp: a 1000-row table (id INTEGER, value BIGINT)
+static RowBatch thousand_rows()
+{
+    std::vector<int32_t> id(1000);
+    std::vector<int64_t> v(1000);
+    for (int i = 0; i < 1000; ++i) {
+        id[i] = i;
+        v[i] = (int64_t)i * 100;
+    }
+    std::vector<Column> c;
+    c.push_back(Column::fromHost(VO_T_INTEGER, id.data(), 1000));
+    c.push_back(Column::fromHost(VO_T_BIGINT, v.data(), 1000));
+    return RowBatch(std::move(c), 1000);
+}
+
+static bool is_permutation_of_rows(const ObliviousTable &t)
+{
+    std::set<uint32_t> pos;
+    for (size_t i = 0; i < t.getRowCount(); ++i) {
+        const uint32_t p = t.getPosition((uint32_t)i);
+        if (p >= t.getRowCount()) return false;
+        pos.insert(p);
+    }
+    return pos.size() == t.getRowCount();
+}
+
+static void test_oblivious_table(TaskManager &tm)
+{
+    RowBatch src = thousand_rows();
+    // BasicCreation (:32-51)
+    ObliviousTable ot("test_table", &src, tm, 1);
+    CHECK(ot.getName() == "test_table");
+    CHECK(ot.getRowCount() == 1000);
+    CHECK(ot.getSourceTable() == &src);
+    CHECK(is_permutation_of_rows(ot));
+    // the initial shuffle moved most records
+    size_t moved0 = 0;
+    for (uint32_t i = 0; i < 1000; ++i) moved0 += ot.getPosition(i) != i;
+    CHECK(moved0 > 900);
+    // ShuffleChangesPositionMap (:54-84)
+    std::vector<uint32_t> before(1000);
+    for (uint32_t i = 0; i < 1000; ++i) before[i] = ot.getPosition(i);
+    ot.startAsyncShuffle();
+    CHECK(ot.isShuffling());
+    CHECK(ot.getPosition(0) == before[0]); // the active map is unchanged until the wait
+    ot.waitForShuffle();
+    CHECK(!ot.isShuffling());
+    size_t changed = 0;
+    for (uint32_t i = 0; i < 1000; ++i) changed += ot.getPosition(i) != before[i];
+    CHECK(changed > 900);
+    CHECK(is_permutation_of_rows(ot));
+    // MultipleShuffles (:150-170)
+    ObliviousTable ot2("t2", &src, tm, 2);
+    for (int round = 0; round < 5; ++round) {
+        ot2.startAsyncShuffle();
+        ot2.waitForShuffle();
+        CHECK(is_permutation_of_rows(ot2));
+    }
+    CHECK(ot2.getShuffleCount() == 5);
+    // a row count that is not a power of two keeps its positions in range
+    // (checked above); a one-row table stays put
+    std::vector<int32_t> one = {7};
+    std::vector<Column> oc;
+    oc.push_back(Column::fromHost(VO_T_INTEGER, one.data(), 1));
+    RowBatch ob(std::move(oc), 1);
+    ObliviousTable ot1("one", &ob, tm, 3);
+    ot1.startAsyncShuffle();
+    ot1.waitForShuffle();
+    CHECK(ot1.getPosition(0) == 0);
+}
+
+static void test_oblivious_table_manager(TaskManager &tm)
+{
+    RowBatch src = thousand_rows();
+    ObliviousTableManager mgr(tm);
+    mgr.registerTable("test_table", &src, 4);
+    // ManagerBasicUsage (:87-98)
+    CHECK(mgr.hasTable("test_table"));
+    CHECK(!mgr.hasTable("other"));
+    CHECK(mgr.getStats().tables_registered == 1);
+    // QueryLifecycle (:100-126)
+    const uint32_t p0 = mgr.getTable("test_table").getPosition(0);
+    mgr.beginQuery({"test_table"});
+    mgr.markAccessed("test_table");
+    mgr.endQuery(); // starts the shuffle
+    CHECK(mgr.getTable("test_table").isShuffling());
+    mgr.beginQuery({"test_table"}); // waits for it
+    CHECK(!mgr.getTable("test_table").isShuffling());
+    CHECK(is_permutation_of_rows(mgr.getTable("test_table")));
+    (void)p0;
+    mgr.endQuery();
+    CHECK(mgr.getStats().total_shuffles >= 1);
+    // ProfilesControlledShuffleForAccessedTables (:128-148)
+    mgr.beginQuery({"test_table"});
+    mgr.markAccessed("test_table");
+    mgr.endQuery();
+    const size_t count = mgr.getTable("test_table").getShuffleCount();
+    mgr.profileAccessedTableShuffles();
+    CHECK(mgr.getTable("test_table").getShuffleCount() == count + 1);
+    CHECK(mgr.controlledShuffleMicros("test_table") > 0);
+    // PositionEnrichment (:172-214)
+    mgr.beginQuery({"test_table"});
+    const size_t n = 100;
+    std::vector<uint32_t> ids(n);
+    std::iota(ids.begin(), ids.end(), 0u);
+    uint32_t *d_ids = nullptr, *d_pos = nullptr;
+    hip_check(hipMalloc(reinterpret_cast<void **>(&d_ids), n * 4), "hipMalloc");
+    hip_check(hipMalloc(reinterpret_cast<void **>(&d_pos), n * 4), "hipMalloc");
+    hip_check(hipMemcpy(d_ids, ids.data(), n * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    mgr.enrichPositions("test_table", d_ids, d_pos, n);
+    std::vector<uint32_t> pos(n);
+    hip_check(hipMemcpy(pos.data(), d_pos, n * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    bool ok = true;
+    for (size_t i = 0; i < n; ++i) ok = ok && pos[i] == mgr.getTable("test_table").getPosition((uint32_t)i);
+    CHECK(ok);
+    (void)hipFree(d_ids);
+    (void)hipFree(d_pos);
+    mgr.endQuery();
+    bool threw = false;
+    try {
+        mgr.getTable("missing");
+    } catch (const std::runtime_error &) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
+// The operators submit every command to the TaskManager, as the reference's
+// do; with the resident queue the launch-only ones (OP_SORT, the join, the
+// shuffle) run in order between the queue's commands.
+static void test_operators_over_resident_queue()
+{
+    TaskManager tm(64, true);
+    CHECK(tm.start());
+    ExecutionContext ctx(tm);
+    std::vector<int32_t> id = {0, 1, 2, 3, 4}, x = {0, 10, 20, 30, 40}, y = {100, 101, 102, 103, 104};
+    SortMergeJoinOperator j(ctx, std::make_unique<BatchSourceOperator>(ctx, id_value(id, x)),
+                            std::make_unique<BatchSourceOperator>(ctx, id_value(id, y)), {0, 0});
+    auto r = j.next();
+    CHECK(static_cast<bool>(r));
+    if (r) {
+        RowBatch b = std::move(r).value();
+        CHECK(b.getRowCount() == 5 && b.getColumnCount() == 4);
+        CHECK(b.getColumn(3).toHost<int32_t>(5) == y);
+    }
+    auto sorted = order_by(ctx, all_rows(), {C_AGE, C_ID}, {false, true});
+    CHECK(sorted.getRowCount() == 8);
+    CHECK((sorted.getColumn(C_ID).toHost<int32_t>(8) == std::vector<int32_t>{5, 4, 1, 7, 3, 6, 8, 2}));
+    test_oblivious_table_manager(tm);
+    CHECK(tm.stop());
+}
+
 static void test_errors_are_results(ExecutionContext &ctx)
 {
     // a failure of the library comes back as Result::failure with its message
@@ -498,6 +651,9 @@ int main()
         {"nullable_bitmaps_travel", [&] { test_nullable_bitmaps_travel(ctx); }},
         {"join_groupsum_sorted", [&] { test_join_groupsum(ctx, false); }},
         {"sort_merge_join", [&] { test_sort_merge_join(ctx); }},
+        {"oblivious_table", [&] { test_oblivious_table(tm); }},
+        {"oblivious_table_manager", [&] { test_oblivious_table_manager(tm); }},
+        {"operators_over_resident_queue", [&] { test_operators_over_resident_queue(); }},
         {"join_groupsum_dense", [&] { test_join_groupsum(ctx, true); }},
         {"errors_are_results", [&] { test_errors_are_results(ctx); }},
         {"task_manager_queue", [&] { test_task_manager_queue(ctx); }},

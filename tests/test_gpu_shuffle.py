@@ -73,3 +73,21 @@ def test_shuffle_is_seed_deterministic_and_seed_sensitive(vctx):
     b, _ = _shuffle(vctx, p, 11)
     c, _ = _shuffle(vctx, p, 12)
     assert np.array_equal(a, b) and not np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("n,n_pad", [(1000, 1024), (3, 4), (5000, 8192), (1, 1)])
+def test_shuffle_live_rows_only(vctx, n, n_pad):
+    """With n < n_pad live rows (a position map of n records), rows >= n stay
+    in place and the first n rows are the live rows in the order of their
+    sorted Philox tags."""
+    pay = np.arange(n_pad, dtype=np.uint32) * 3 + 1
+    d = to_dev(pay)
+    perm = empty_dev(n_pad, np.uint32)
+    vctx.shuffle(d, n_pad, 21, perm, n=n)
+    got = from_dev(d, np.uint32)
+    pm = from_dev(perm, np.uint32)
+    tags = [oracle.philox_tag(21, i) for i in range(n)]
+    want = sorted(range(n), key=lambda i: (tags[i], i))
+    assert pm[:n].tolist() == want
+    assert pm[n:].tolist() == list(range(n, n_pad))
+    assert np.array_equal(got, pay[pm])

@@ -188,8 +188,8 @@ class Context:
         a.out_count = _ptr(out_count)
         return self.check(self.lib.vo_compact(self.h, C.byref(a)))
 
-    def shuffle(self, payload, n_pad, seed, perm_out=None):
-        a = _abi.ShuffleArgs(_ptr(payload), _ptr(perm_out), n_pad, seed)
+    def shuffle(self, payload, n_pad, seed, perm_out=None, n=0):
+        a = _abi.ShuffleArgs(_ptr(payload), _ptr(perm_out), n_pad, seed, n)
         return self.check(self.lib.vo_shuffle(self.h, C.byref(a)))
 
     def filter_sum(self, day, qty, price, disc, n, n_pad, out, out_price=None, out_disc=None,

@@ -88,7 +88,7 @@ class CompactArgs(C.Structure):
 
 
 class ShuffleArgs(C.Structure):
-    _fields_ = [("payload", P), ("perm_out", P), ("n_pad", u64), ("seed", u64)]
+    _fields_ = [("payload", P), ("perm_out", P), ("n_pad", u64), ("seed", u64), ("n", u64)]
 
 
 class FilterArgs(C.Structure):

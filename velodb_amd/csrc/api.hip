@@ -278,10 +278,25 @@ int vo_ctx_destroy(vo_ctx *ctx)
 
 const char *vo_last_error(const vo_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
 
+static int run_sync(vo_ctx *ctx, vo_command &cmd);
+
 uint64_t vo_submit(vo_ctx *ctx, vo_command *cmd)
 {
     if (!ctx || !cmd) return 0;
-    if (ctx->queue) return queue_submit(ctx, cmd);
+    if (ctx->queue) {
+        if (queue_supports(cmd->opcode)) return queue_submit(ctx, cmd);
+        // an operator the resident kernel does not run: the submitted
+        // commands finish, it runs on the context stream to completion, and a
+        // NOP through the ring gives it the next sequence id, so waits and
+        // ordering stay those of the queue (the reference submits every
+        // operator to its queue, sort_merge_join_operator.cpp:113-149)
+        if (run_sync(ctx, *cmd) != VO_OK) return 0;
+        vo_command nop{};
+        nop.opcode = VO_OP_NOP;
+        const uint64_t seq = queue_submit(ctx, &nop);
+        cmd->sequence_id = seq;
+        return seq;
+    }
     std::lock_guard<std::mutex> lk(ctx->mu);
     try {
         dispatch(ctx, *cmd);

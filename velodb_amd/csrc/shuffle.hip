@@ -6,7 +6,10 @@
 // Philox4x32-10(counter = (i, 0), key = seed) (Salmon et al., SC'11); the rows
 // are sorted by (tag, i) with the oblivious bitonic network, and the payload
 // travels with its row, so the permutation is never materialised as
-// addresses.  perm_out[j] receives the source row of output row j.
+// addresses.  perm_out[j] receives the source row of output row j.  With
+// n < n_pad live rows, rows >= n get the largest tag and stay in place, so the
+// first n rows receive a uniform permutation of the live rows (a position map
+// of n records padded to a power of two).
 #include "vo_internal.hpp"
 
 #include <algorithm>
@@ -35,12 +38,13 @@ __device__ __forceinline__ uint64_t philox_tag(uint64_t seed, uint64_t i)
     return ((uint64_t)c1 << 32) | c0;
 }
 
-__global__ void k_shuffle_tags(uint64_t seed, const uint32_t *__restrict__ payload, uint64_t n_pad,
+__global__ void k_shuffle_tags(uint64_t seed, const uint32_t *__restrict__ payload, uint64_t n, uint64_t n_pad,
                                uint64_t *__restrict__ keys, uint64_t *__restrict__ vals, TraceSlot *slot)
 {
     dev::Atr tr(slot);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
-        keys[i] = philox_tag(seed, i);
+        const uint64_t t = philox_tag(seed, i);
+        keys[i] = i < n ? t : ~0ull; // a tie with a live tag of ~0 breaks by row: live rows first
         vals[i] = (i << 32) | payload[i];
         tr.at(1, i);
     }
@@ -68,10 +72,12 @@ void op_shuffle(vo_ctx *ctx, const vo_shuffle_args &a)
 {
     VO_CHECK(is_pow2(a.n_pad) && a.n_pad <= (1ull << 32), VO_ERR_INVALID, "n_pad must be a power of two <= 2^32");
     VO_CHECK(a.payload != nullptr, VO_ERR_INVALID, "null payload");
+    VO_CHECK(a.n <= a.n_pad, VO_ERR_INVALID, "n above n_pad");
+    const uint64_t n = a.n ? a.n : a.n_pad;
     uint64_t *keys = (uint64_t *)scratch(ctx, 16 * a.n_pad + 256);
     uint64_t *vals = keys + a.n_pad;
     launch(ctx, LaunchInfo{"shuffle_tags", grid_for(a.n_pad), dim3(256), 0, a.n_pad}, k_shuffle_tags, a.seed,
-           (const uint32_t *)a.payload, a.n_pad, keys, vals, trace_slot(ctx));
+           (const uint32_t *)a.payload, n, a.n_pad, keys, vals, trace_slot(ctx));
     sort_kv(ctx, keys, vals, a.n_pad);
     launch(ctx, LaunchInfo{"shuffle_unpack", grid_for(a.n_pad), dim3(256), 0, a.n_pad}, k_shuffle_unpack,
            (const uint64_t *)vals, a.n_pad, a.payload, a.perm_out, trace_slot(ctx));
