@@ -191,6 +191,14 @@ public:
         if (valid) hip_check(hipMemcpy(c.bitmap_, valid, n, hipMemcpyHostToDevice), "hipMemcpy");
         return c;
     }
+    // the same through the context's padded staged transfer: the whole
+    // capacity crosses the link (pad rows as zeros), so only the padded
+    // size is visible on it (ValueVector::to, value_vector.hpp:140-294)
+    static Column fromHost(TaskManager &tm, uint32_t type_id, const void *values, size_t n,
+                           const uint8_t *valid = nullptr);
+    template <typename T>
+    std::vector<T> toHost(TaskManager &tm, size_t n) const;
+
     template <typename T>
     std::vector<T> toHost(size_t n) const
     {
@@ -293,6 +301,24 @@ private:
     std::vector<Column> cols_;
     size_t rows_ = 0;
 };
+
+inline Column Column::fromHost(TaskManager &tm, uint32_t type_id, const void *values, size_t n, const uint8_t *valid)
+{
+    Column c(type_id, next_pow2(n), valid != nullptr);
+    const size_t w = type_width(type_id);
+    tm.check(vo_transfer_to_device(tm.native(), c.data_, values, n * w, c.capacity_ * w, nullptr));
+    if (valid) tm.check(vo_transfer_to_device(tm.native(), c.bitmap_, valid, n, c.capacity_, nullptr));
+    return c;
+}
+
+template <typename T>
+std::vector<T> Column::toHost(TaskManager &tm, size_t n) const
+{
+    if (sizeof(T) != type_width(type_)) throw ExecutionError("toHost: element size does not match the column type");
+    std::vector<T> v(n);
+    tm.check(vo_transfer_to_host(tm.native(), v.data(), data_, n * sizeof(T), capacity_ * sizeof(T), nullptr));
+    return v;
+}
 
 // --------------------------------------------------------------- operators
 

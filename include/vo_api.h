@@ -390,6 +390,22 @@ VO_EXPORT int vo_sort_batch(vo_ctx *ctx, const vo_sort_batch_args *args);
 VO_EXPORT int vo_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args *args, uint64_t *n_out);
 VO_EXPORT int vo_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args *args);
 
+/* Host <-> device wire format (StagedTransfer, include/cuda/staged_transfer.hpp:101-225,
+ * and the padded moves of ValueVector::to, include/data/value_vector.hpp:140-294).
+ * Copies `bytes` between pageable host memory and a device buffer of at
+ * least padded_bytes, and always moves padded_bytes over the link: to the
+ * device the pad region receives zeros, from the device the pad rows are read
+ * and discarded, so the link shows the padded size only (callers pad to the
+ * power-of-two capacity of the column).  Staged through a ring of pinned
+ * chunks owned by the context, with the host copies on worker threads
+ * overlapping the DMA.  Ordered after the work enqueued on the context
+ * stream and complete on return.  *link_bytes (may be NULL) receives the
+ * bytes moved over the link. */
+VO_EXPORT int vo_transfer_to_device(vo_ctx *ctx, void *dst_dev, const void *src_host, uint64_t bytes,
+                                    uint64_t padded_bytes, uint64_t *link_bytes);
+VO_EXPORT int vo_transfer_to_host(vo_ctx *ctx, void *dst_host, const void *src_dev, uint64_t bytes,
+                                  uint64_t padded_bytes, uint64_t *link_bytes);
+
 /* Asynchronous forms: enqueue on the context stream and return. */
 VO_EXPORT int vo_sort_kp_async(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad);
 

@@ -590,6 +590,23 @@ static void test_operators_over_resident_queue()
     CHECK(tm.stop());
 }
 
+static void test_padded_transfers(TaskManager &tm)
+{
+    // columns through the padded staged transfer: the values round-trip, the
+    // pad rows arrive as zeros, and the sort sees the same data
+    const size_t n = 100000;
+    std::vector<int64_t> v(n);
+    std::mt19937_64 rng(9);
+    for (auto &x : v) x = (int64_t)(rng() >> 1) - (1ll << 61);
+    Column c = Column::fromHost(tm, VO_T_BIGINT, v.data(), n);
+    CHECK(c.capacity() == 131072);
+    CHECK(c.toHost<int64_t>(tm, n) == v);
+    const auto all = c.toHost<int64_t>(tm, c.capacity());
+    bool zeros = true;
+    for (size_t i = n; i < all.size(); ++i) zeros = zeros && all[i] == 0;
+    CHECK(zeros);
+}
+
 static void test_errors_are_results(ExecutionContext &ctx)
 {
     // a failure of the library comes back as Result::failure with its message
@@ -652,6 +669,7 @@ int main()
         {"join_groupsum_sorted", [&] { test_join_groupsum(ctx, false); }},
         {"sort_merge_join", [&] { test_sort_merge_join(ctx); }},
         {"oblivious_table", [&] { test_oblivious_table(tm); }},
+        {"padded_transfers", [&] { test_padded_transfers(tm); }},
         {"oblivious_table_manager", [&] { test_oblivious_table_manager(tm); }},
         {"operators_over_resident_queue", [&] { test_operators_over_resident_queue(); }},
         {"join_groupsum_dense", [&] { test_join_groupsum(ctx, true); }},

@@ -8,6 +8,8 @@ with an error.
 import ctypes as C
 import os
 
+import numpy as np
+
 from . import _abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -237,6 +239,27 @@ class Context:
         orr = torch.empty(cap, dtype=torch.int64, device=left_keys.device)
         self.join_pairs_write(left_keys, nl, right_keys, nr, type_id, ol, orr, cap)
         return n, ol, orr
+
+    def transfer_to_device(self, dst, src_host, padded_bytes=None):
+        """Pageable numpy array -> device tensor, padded_bytes over the link
+        (default: the tensor's size); returns the bytes moved."""
+        src_host = np.ascontiguousarray(src_host)
+        nb = src_host.nbytes
+        pb = dst.numel() * dst.element_size() if padded_bytes is None else padded_bytes
+        moved = C.c_uint64(0)
+        self.check(self.lib.vo_transfer_to_device(self.h, _ptr(dst), src_host.ctypes.data_as(C.c_void_p), nb, pb,
+                                                  C.byref(moved)))
+        return moved.value
+
+    def transfer_to_host(self, dst_host, src, padded_bytes=None):
+        """Device tensor -> pageable numpy array (dst_host.nbytes bytes),
+        padded_bytes over the link; returns the bytes moved."""
+        nb = dst_host.nbytes
+        pb = src.numel() * src.element_size() if padded_bytes is None else padded_bytes
+        moved = C.c_uint64(0)
+        self.check(self.lib.vo_transfer_to_host(self.h, dst_host.ctypes.data_as(C.c_void_p), _ptr(src), nb, pb,
+                                                C.byref(moved)))
+        return moved.value
 
     def sort_batch(self, keys, payload, items):
         arr = (_abi.BatchItem * len(items))(*[_abi.BatchItem(o, n, p) for (o, n, p) in items])

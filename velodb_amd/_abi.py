@@ -182,6 +182,8 @@ PROTOTYPES = [
     ("vo_pkfk_join_groupsum", C.c_int, [P, C.POINTER(JoinArgs)]),
     ("vo_sort_batch", C.c_int, [P, C.POINTER(SortBatchArgs)]),
     ("vo_join_pairs_count", C.c_int, [P, C.POINTER(JoinPairsArgs), C.POINTER(u64)]),
+    ("vo_transfer_to_device", C.c_int, [P, P, P, u64, u64, C.POINTER(u64)]),
+    ("vo_transfer_to_host", C.c_int, [P, P, P, u64, u64, C.POINTER(u64)]),
     ("vo_join_pairs_write", C.c_int, [P, C.POINTER(JoinPairsArgs)]),
     ("vo_sort_kp_async", C.c_int, [P, P, P, u64, u64]),
     ("vo_trace_begin", C.c_int, [P]),

@@ -270,6 +270,7 @@ int vo_ctx_destroy(vo_ctx *ctx)
     }
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     join_pairs_state_free(ctx);
+    staging_free(ctx);
     if (ctx->trace_slots) (void)hipFree(ctx->trace_slots);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -406,6 +407,26 @@ VO_SYNC_OP(vo_filter_sum, VO_OP_FILTER_SUM, filter, vo_filter_args)
 VO_SYNC_OP(vo_pkfk_join_groupsum, VO_OP_PKFK_JOIN_GROUPSUM, join, vo_join_args)
 VO_SYNC_OP(vo_sort_batch, VO_OP_SORT_BATCH, sort_batch, vo_sort_batch_args)
 VO_SYNC_OP(vo_join_pairs_write, VO_OP_SORT_MERGE_JOIN_WRITE, join_pairs, vo_join_pairs_args)
+
+int vo_transfer_to_device(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes, uint64_t padded_bytes,
+                          uint64_t *link_bytes)
+{
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        op_transfer_to_device(ctx, dst, src, bytes, padded_bytes);
+        if (link_bytes) *link_bytes = last_transfer_bytes(ctx);
+    });
+}
+
+int vo_transfer_to_host(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes, uint64_t padded_bytes,
+                        uint64_t *link_bytes)
+{
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        op_transfer_to_host(ctx, dst, src, bytes, padded_bytes);
+        if (link_bytes) *link_bytes = last_transfer_bytes(ctx);
+    });
+}
 
 int vo_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args *args, uint64_t *n_out)
 {

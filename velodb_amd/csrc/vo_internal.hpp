@@ -113,6 +113,9 @@ struct vo_ctx {
 
     // equi-join routing state kept from COUNT to WRITE (join_pairs.hip)
     void *join_pairs_state = nullptr;
+
+    // pinned staging ring and copy threads of the host transfers (transfer.hip)
+    void *staging = nullptr;
 };
 
 namespace vo {
@@ -197,6 +200,10 @@ uint64_t op_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args &a); // retur
 void op_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args &a);
 void join_pairs_state_free(vo_ctx *ctx);
 uint64_t join_pairs_last_count(vo_ctx *ctx); // n_out of the last COUNT
+void op_transfer_to_device(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes, uint64_t padded_bytes);
+void op_transfer_to_host(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes, uint64_t padded_bytes);
+uint64_t last_transfer_bytes(vo_ctx *ctx);
+void staging_free(vo_ctx *ctx);
 void op_gen_sort_input(vo_ctx *ctx, uint64_t seed, uint64_t *keys, uint32_t *payload, uint64_t n);
 void op_gen_fact(vo_ctx *ctx, uint64_t seed, int32_t *day, int64_t *qty, int64_t *price, int64_t *disc,
                  uint64_t n);
