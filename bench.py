@@ -361,6 +361,48 @@ def bench_transfer(args):
     ctx.close()
 
 
+def bench_gather(args):
+    """Oblivious gather (the device form of the host oblivious
+    materialization): one int64 column of a 2^22-row table at 2^23 record
+    ids with repeats."""
+    import torch
+
+    import velodb_amd
+    from velodb_amd import _abi
+    nt, n = 1 << 22, 1 << 23
+    g = torch.Generator(device="cuda:0").manual_seed(5)
+    table = torch.randint(-(1 << 62), 1 << 62, (nt,), device="cuda:0", generator=g, dtype=torch.int64)
+    ids = torch.randint(0, nt, (n,), device="cuda:0", generator=g, dtype=torch.int64)
+    ids[: n // 8] = 7
+    out = torch.empty(n, dtype=torch.int64, device="cuda:0")
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+
+    def step():
+        ctx.gather_oblivious(table, nt, ids, n, out, 8)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.kernel_stats_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    ok = bool(torch.all(out == table[ids]))
+    fams = ["gather_obl_build", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2", "sort_global_s3",
+            "sort_global_s4", "sort_hybrid_s6", "gather_obl_tiles", "gather_obl_carry", "gather_obl_rows",
+            "compact_count", "compact_scan", "compact_local", "compact_hybrid", "compact_global", "gather_obl_order",
+            "gather_obl_out"]
+    stats = _family_stats(ctx, fams)
+    _line("oblivious gather by record id (Mrows/s)", n / dt / 1e6, "Mrows/s", args.steps, args.warmup, dt * 1e3,
+          "int64",
+          {"workload": "one int64 column of a 2^22-row table gathered at 2^23 record ids (1/8 of them one id); "
+                       "union of 2^22 + 2^23 rows padded to 2^24", "n_table": nt, "n": n, "equals_indexing": ok},
+          kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()})
+    ctx.close()
+
+
 def bench_batch(args):
     """4096 independent sorts, sizes log-uniform in [2^10, 2^20]
     (numpy default_rng(0)), each padded to its power of two; one batched call
@@ -542,7 +584,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", type=int, default=0)
     ap.add_argument("--workload", choices=["sort", "filter", "join", "join_pairs", "batch", "serve", "queue",
-                                           "transfer"],
+                                           "transfer", "gather"],
                     default="sort",
                     help="sort = the headline line; the others print their own line")
     ap.add_argument("--autotune", action="store_true", help="batch: run the autotuner first")
@@ -562,7 +604,8 @@ def main():
         sys.exit(2)
     if args.workload != "sort":
         {"filter": bench_filter, "join": bench_join, "join_pairs": bench_join_pairs, "batch": bench_batch,
-         "serve": bench_serve, "queue": bench_queue, "transfer": bench_transfer}[args.workload](args)
+         "serve": bench_serve, "queue": bench_queue, "transfer": bench_transfer,
+         "gather": bench_gather}[args.workload](args)
         return
 
     import torch

@@ -15,6 +15,8 @@
  *   SortMergeJoinOperator   include/operator/sort_merge_join_operator.hpp:14-33,
  *                           src/operator/sort_merge_join_operator.cpp:27-257
  *   JoinType                include/operator/join_type.hpp
+ *   MaterializationOperator include/operator/materialization_operator.hpp,
+ *                           src/operator/materialization_operator.cpp:362-517
  *   ObliviousTable          include/data/oblivious_table.hpp, src/data/oblivious_table.cpp:16-255
  *   ObliviousTableManager   include/data/oblivious_table_manager.hpp,
  *                           src/data/oblivious_table_manager.cpp:14-221
@@ -235,6 +237,7 @@ public:
     void *rawData() { return data_; }
     const void *rawData() const { return data_; }
     uint8_t *rawBitmapData() { return bitmap_; }
+    const uint8_t *bitmap() const { return bitmap_; }
     // new device buffers for a permuted copy (Column::getDeviceBuffer analogue)
     void *getDeviceBuffer() const
     {
@@ -544,11 +547,13 @@ enum class JoinType { INNER, LEFT, RIGHT, FULL };
 // reference.
 class SortMergeJoinOperator : public AbstractOperator {
 public:
+    // oblivious_materialization: gather the columns with OP_GATHER_OBLIVIOUS
+    // (no address depends on the pairs) instead of the reference's permute
     SortMergeJoinOperator(ExecutionContext &context, std::unique_ptr<AbstractOperator> left_child,
                           std::unique_ptr<AbstractOperator> right_child, std::pair<size_t, size_t> join_key_indices,
-                          JoinType join_type = JoinType::INNER)
+                          JoinType join_type = JoinType::INNER, bool oblivious_materialization = false)
         : AbstractOperator(context), left_(std::move(left_child)), right_(std::move(right_child)),
-          keys_(join_key_indices), type_(join_type)
+          keys_(join_key_indices), type_(join_type), oblivious_(oblivious_materialization)
     {
     }
 
@@ -619,16 +624,21 @@ private:
                 uint8_t *bits = nullptr;
                 if (c.rawBitmapData()) hip_check(hipMalloc(reinterpret_cast<void **>(&bits), cap), "hipMalloc");
                 out.push_back(Column::createFromDeviceBuffers(c.getTypeId(), data, bits, cap));
+                const int64_t *ids = static_cast<const int64_t *>(idx.rawData());
+                if (oblivious_) {
+                    last = submit_gather(tm, c.rawData(), b.getRowCount(), ids, n_out, data,
+                                         (uint32_t)type_width(c.getTypeId()));
+                    if (bits) last = submit_gather(tm, c.rawBitmapData(), b.getRowCount(), ids, n_out, bits, 1);
+                    continue;
+                }
                 vo_command p{};
                 p.opcode = VO_OP_PERMUTE;
-                p.args.permute = {data, c.rawData(), static_cast<const int64_t *>(idx.rawData()), n_out,
-                                  c.getTypeId()};
+                p.args.permute = {data, c.rawData(), ids, n_out, c.getTypeId()};
                 last = tm.submitCommand(p);
                 if (bits) {
                     vo_command pb{};
                     pb.opcode = VO_OP_PERMUTE;
-                    pb.args.permute = {bits, c.rawBitmapData(), static_cast<const int64_t *>(idx.rawData()), n_out,
-                                       VO_T_BOOLEAN};
+                    pb.args.permute = {bits, c.rawBitmapData(), ids, n_out, VO_T_BOOLEAN};
                     last = tm.submitCommand(pb);
                 }
             }
@@ -639,10 +649,95 @@ private:
         return RowBatch(std::move(out), n_out);
     }
 
+    static uint64_t submit_gather(TaskManager &tm, const void *table, size_t n_table, const int64_t *ids, size_t n,
+                                  void *out, uint32_t width)
+    {
+        vo_command g{};
+        g.opcode = VO_OP_GATHER_OBLIVIOUS;
+        g.args.gather_oblivious = {table, n_table, ids, n, out, nullptr, width, 0};
+        return tm.submitCommand(g);
+    }
+
     std::unique_ptr<AbstractOperator> left_, right_;
     std::pair<size_t, size_t> keys_;
     JoinType type_;
+    bool oblivious_;
     bool joined_ = false;
+};
+
+// Oblivious materialization of columns by record id
+// (MaterializationOperator, src/operator/materialization_operator.cpp:362-517):
+// for each requested column, out[i] = source[rowid[i]] through
+// OP_GATHER_OBLIVIOUS, which sorts the table rows and the requests together
+// instead of reading rows at positions (the reference's host bitonic sort,
+// cached reads with dummy reads, restore).  The child yields the row-id
+// columns (BIGINT); each output column names a row-id column of the child and
+// a column of a source batch.  Null bitmaps are gathered with their values;
+// an id outside the source gives 0 and, when the source column is nullable,
+// a null.
+struct MaterializedColumn {
+    size_t rowid_index;     // BIGINT column of the child
+    const RowBatch *source; // source table
+    size_t column_index;    // column of the source
+};
+
+class MaterializationOperator : public AbstractOperator {
+public:
+    MaterializationOperator(ExecutionContext &context, std::unique_ptr<AbstractOperator> child,
+                            std::vector<MaterializedColumn> columns)
+        : AbstractOperator(context), child_(std::move(child)), cols_(std::move(columns))
+    {
+    }
+
+    Result<RowBatch> next() override
+    {
+        auto in = child_->next();
+        if (!in) return in;
+        RowBatch ids = std::move(in).value();
+        if (ids.getRowCount() == 0) return Result<RowBatch>::success(RowBatch());
+        try {
+            return Result<RowBatch>::success(run(ids));
+        } catch (const std::exception &e) {
+            return Result<RowBatch>::failure(e.what());
+        }
+    }
+
+private:
+    RowBatch run(RowBatch &ids)
+    {
+        const size_t n = ids.getRowCount(), cap = next_pow2(n);
+        TaskManager &tm = context_.getTaskManager();
+        std::vector<Column> out;
+        uint64_t last = 0;
+        for (const MaterializedColumn &m : cols_) {
+            if (!m.source || m.rowid_index >= ids.getColumnCount() || m.column_index >= m.source->getColumnCount())
+                throw ExecutionError("MaterializationOperator: column out of range");
+            const Column &rid = ids.getColumn(m.rowid_index);
+            if (rid.getTypeId() != VO_T_BIGINT) throw ExecutionError("MaterializationOperator: row ids must be BIGINT");
+            const Column &src = m.source->getColumn(m.column_index);
+            const size_t nt = m.source->getRowCount();
+            Column o(src.getTypeId(), cap, src.bitmap() != nullptr);
+            vo_command g{};
+            g.opcode = VO_OP_GATHER_OBLIVIOUS;
+            g.args.gather_oblivious = {src.rawData(), nt, static_cast<const int64_t *>(rid.rawData()), n, o.rawData(),
+                                       nullptr, (uint32_t)type_width(src.getTypeId()), 0};
+            last = tm.submitCommand(g);
+            if (src.bitmap()) {
+                // an id outside the source gathers 0: a null
+                vo_command gb{};
+                gb.opcode = VO_OP_GATHER_OBLIVIOUS;
+                gb.args.gather_oblivious = {src.bitmap(), nt, static_cast<const int64_t *>(rid.rawData()), n,
+                                            o.rawBitmapData(), nullptr, 1, 0};
+                last = tm.submitCommand(gb);
+            }
+            out.push_back(std::move(o));
+        }
+        if (last) tm.waitCommand(last);
+        return RowBatch(std::move(out), n);
+    }
+
+    std::unique_ptr<AbstractOperator> child_;
+    std::vector<MaterializedColumn> cols_;
 };
 
 // Oblivious PK/FK join + SUM(value) GROUP BY group: dimension batch (key

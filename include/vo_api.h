@@ -91,6 +91,7 @@ typedef enum vo_opcode {
     VO_OP_FILTER_SUM = 0x103,           /* predicate + compaction + int128 SUM */
     VO_OP_PKFK_JOIN_GROUPSUM = 0x104,   /* PK/FK join + GROUP BY SUM */
     VO_OP_SORT_BATCH = 0x105,           /* many independent (u64,u32) sorts */
+    VO_OP_GATHER_OBLIVIOUS = 0x106,     /* table values by record id, no data-dependent address */
     VO_OP_TERMINATE = 0xFFFFFFFFu
 } vo_opcode;
 
@@ -283,6 +284,25 @@ typedef struct vo_join_pairs_args {
     uint64_t out_cap;
 } vo_join_pairs_args;
 
+/* OP_GATHER_OBLIVIOUS: out[i] = table[record_ids[i]] for i < n, the device
+ * form of the host oblivious materialization
+ * (src/operator/materialization_operator.cpp:362-517).  The table rows and
+ * the requests are sorted together by record id, each value is filled forward
+ * to its requests, the requests are compacted and sorted back to request
+ * order: no address depends on a record id or on how often it repeats.  An
+ * id outside [0, n_table) gives 0 and out_found[i] = 0 (out_found optional).
+ * n_table + n padded to a power of two must be at most 2^29. */
+typedef struct vo_gather_oblivious_args {
+    const void *table; /* n_table rows of elem_bytes */
+    uint64_t n_table;
+    const int64_t *record_ids; /* n requests */
+    uint64_t n;
+    void *out;                 /* n rows of elem_bytes, request order */
+    uint8_t *out_found;        /* optional, n bytes */
+    uint32_t elem_bytes;       /* 1, 2, 4 or 8 */
+    uint32_t reserved;         /* 0 */
+} vo_gather_oblivious_args;
+
 /* OP_SORT_BATCH: independent sorts in one schedule.  Item i sorts
  * (keys + offset, payload + offset) over n rows padded to n_pad (a power of
  * two; offset must be a multiple of n_pad).  Items are bucketed by n_pad. */
@@ -314,6 +334,7 @@ typedef union vo_command_args {
     vo_join_args join;
     vo_sort_batch_args sort_batch;
     vo_join_pairs_args join_pairs; /* OP_SORT_MERGE_JOIN_COUNT / _WRITE */
+    vo_gather_oblivious_args gather_oblivious;
 } vo_command_args;
 
 /* Mirror of Command (commands.hpp:152-156). */
@@ -389,6 +410,7 @@ VO_EXPORT int vo_sort_batch(vo_ctx *ctx, const vo_sort_batch_args *args);
  * be NULL) receives the pair count after COUNT. */
 VO_EXPORT int vo_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args *args, uint64_t *n_out);
 VO_EXPORT int vo_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args *args);
+VO_EXPORT int vo_gather_oblivious(vo_ctx *ctx, const vo_gather_oblivious_args *args);
 
 /* Host <-> device wire format (StagedTransfer, include/cuda/staged_transfer.hpp:101-225,
  * and the padded moves of ValueVector::to, include/data/value_vector.hpp:140-294).

@@ -68,6 +68,8 @@ def lib():
         L.orc_join_groupsum.argtypes = [P, P, sz, P, P, sz, sz, P, P, P, P, sz]
         L.orc_distribute.restype = C.c_int
         L.orc_distribute.argtypes = [P, P, sz, sz, P, P]
+        L.orc_gather_recover.restype = None
+        L.orc_gather_recover.argtypes = [P, sz, P, sz, P, P]
         L.orc_join_pairs.restype = sz
         L.orc_join_pairs.argtypes = [P, sz, P, sz, P, P, sz, P]
         _lib = L
@@ -242,6 +244,16 @@ def join_pairs(lcodes, rcodes):
     if st.value != 0 or n2 != n_out:
         raise ValueError("join oracle invariant failed")
     return ol[:n_out], or_[:n_out]
+
+
+def gather_recover(table, ids):
+    """Oblivious materialization of one column: (values as uint64, found)."""
+    t = np.ascontiguousarray(table, dtype=np.uint64)
+    r = np.ascontiguousarray(ids, dtype=np.int64)
+    out = np.empty(max(len(r), 1), np.uint64)
+    found = np.empty(max(len(r), 1), np.uint8)
+    lib().orc_gather_recover(_p(t), len(t), _p(r), len(r), _p(out), _p(found))
+    return out[:len(r)], found[:len(r)]
 
 
 def rand64(seed, stream, i):

@@ -771,3 +771,67 @@ size_t orc_join_pairs(const uint64_t *lkey, size_t n_left, const uint64_t *rkey,
     free(it2);
     return n_out;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Oblivious materialization                                                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int64_t record_id;
+    uint32_t position;
+    uint32_t pairing_idx;
+} sortable_tuple; /* SortableTuple of include/operator/materialization_operator.hpp */
+
+void orc_gather_recover(const uint64_t *table, size_t n_table, const int64_t *ids, size_t n,
+                        uint64_t *out, uint8_t *found)
+{
+    size_t npad = 1;
+    while (npad < n) npad <<= 1;
+    sortable_tuple *t = (sortable_tuple *)malloc((npad ? npad : 1) * sizeof(sortable_tuple));
+    for (size_t i = 0; i < npad; ++i) {
+        if (i < n) {
+            t[i].record_id = ids[i];
+            t[i].position = (ids[i] >= 0 && (uint64_t)ids[i] < n_table) ? (uint32_t)ids[i] : 0;
+            t[i].pairing_idx = (uint32_t)i;
+        } else { /* sentinels sort last (:147-152) */
+            t[i].record_id = INT64_MAX;
+            t[i].position = 0;
+            t[i].pairing_idx = UINT32_MAX;
+        }
+    }
+    /* bitonic sort by record_id (:156-169) */
+    for (size_t k = 2; k <= npad; k <<= 1)
+        for (size_t j = k >> 1; j > 0; j >>= 1)
+            for (size_t i = 0; i < npad; ++i) {
+                size_t x = i ^ j;
+                if (x > i) {
+                    int asc = (i & k) == 0;
+                    int sw = asc ? (t[i].record_id > t[x].record_id) : (t[i].record_id < t[x].record_id);
+                    cswap((unsigned char *)&t[i], (unsigned char *)&t[x], sizeof(sortable_tuple), sw);
+                }
+            }
+    /* gather in sorted order with a one-entry cache (:476-503) */
+    uint64_t *sv = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+    uint8_t *sf = (uint8_t *)malloc(n ? n : 1);
+    int64_t cached_id = INT64_MIN;
+    uint64_t cached = 0;
+    uint8_t cached_found = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (t[i].record_id != cached_id) {
+            int ok = t[i].record_id >= 0 && (uint64_t)t[i].record_id < n_table;
+            cached = ok ? table[t[i].position] : 0;
+            cached_found = (uint8_t)ok;
+            cached_id = t[i].record_id;
+        }
+        sv[i] = cached;
+        sf[i] = cached_found;
+    }
+    /* restore the request order (:505-516) */
+    for (size_t i = 0; i < n; ++i) {
+        out[t[i].pairing_idx] = sv[i];
+        if (found) found[t[i].pairing_idx] = sf[i];
+    }
+    free(t);
+    free(sv);
+    free(sf);
+}

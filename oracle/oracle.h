@@ -230,6 +230,20 @@ int orc_distribute(const int64_t *target, const uint64_t *payload, size_t n, siz
 size_t orc_join_pairs(const uint64_t *lkey, size_t n_left, const uint64_t *rkey, size_t n_right,
                       int64_t *out_left, int64_t *out_right, size_t cap, int *status);
 
+
+/* Oblivious materialization of one column, restating obliviousRecoverSingle
+ * (src/operator/materialization_operator.cpp:362-455) with
+ * obliviousGatherTyped (:461-517): tuples (record id, position, pairing
+ * index), bitonic sort by record id padded with INT64_MAX
+ * (bitonicSortTuples :137-173), reads in sorted order with a one-entry cache
+ * for repeated ids, and the restore permutation by pairing index.  Positions
+ * are the record ids (no position map).  An id outside [0, n_table) reads
+ * nothing and gives value 0, found 0 (the reference wraps it modulo the
+ * table size, :395).  Values are 64-bit.
+ * Pinned: numpy take on random ids with repetitions. */
+void orc_gather_recover(const uint64_t *table, size_t n_table, const int64_t *ids, size_t n,
+                        uint64_t *out, uint8_t *found);
+
 #ifdef __cplusplus
 }
 #endif

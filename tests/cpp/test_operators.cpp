@@ -607,6 +607,72 @@ static void test_padded_transfers(TaskManager &tm)
     CHECK(zeros);
 }
 
+static void test_materialization(ExecutionContext &ctx)
+{
+    // MaterializationOperator: a source table (value BIGINT nullable, tag
+    // INTEGER) and row ids with repeats and one id outside the table
+    const size_t nt = 5000, n = 20000;
+    std::mt19937_64 rng(21);
+    std::vector<int64_t> val(nt);
+    std::vector<int32_t> tag(nt);
+    std::vector<uint8_t> valid(nt);
+    for (size_t i = 0; i < nt; ++i) {
+        val[i] = (int64_t)(rng() >> 2);
+        tag[i] = (int32_t)(rng() % 1000);
+        valid[i] = (uint8_t)(rng() % 4 != 0);
+    }
+    std::vector<Column> sc;
+    sc.push_back(Column::fromHost(VO_T_BIGINT, val.data(), nt, valid.data()));
+    sc.push_back(Column::fromHost(VO_T_INTEGER, tag.data(), nt));
+    RowBatch source(std::move(sc), nt);
+    std::vector<int64_t> ids(n);
+    for (auto &x : ids) x = (int64_t)(rng() % nt);
+    for (size_t i = 0; i < 3000; ++i) ids[i] = 42;
+    ids[n - 1] = (int64_t)nt + 9; // outside the table
+    std::vector<Column> ic;
+    ic.push_back(Column::fromHost(VO_T_BIGINT, ids.data(), n));
+    MaterializationOperator m(ctx, std::make_unique<BatchSourceOperator>(ctx, RowBatch(std::move(ic), n)),
+                              {{0, &source, 1}, {0, &source, 0}});
+    auto r = m.next();
+    CHECK(static_cast<bool>(r));
+    if (!r) return;
+    RowBatch b = std::move(r).value();
+    CHECK(b.getRowCount() == n && b.getColumnCount() == 2);
+    const auto gt = b.getColumn(0).toHost<int32_t>(n);
+    const auto gv = b.getColumn(1).toHost<int64_t>(n);
+    const auto gb = b.getColumn(1).bitmapToHost(n);
+    bool ok = true;
+    for (size_t i = 0; i + 1 < n; ++i)
+        ok = ok && gt[i] == tag[ids[i]] && gv[i] == val[ids[i]] && (gb[i] != 0) == (valid[ids[i]] != 0);
+    CHECK(ok);
+    CHECK(gt[n - 1] == 0 && gv[n - 1] == 0 && gb[n - 1] == 0);
+
+    // the join with oblivious materialization equals the permute form
+    std::vector<int32_t> lk(3000), rk(2000), lv(3000), rv(2000);
+    for (size_t i = 0; i < 3000; ++i) {
+        lk[i] = (int32_t)(rng() % 500);
+        lv[i] = (int32_t)i;
+    }
+    for (size_t i = 0; i < 2000; ++i) {
+        rk[i] = (int32_t)(rng() % 500);
+        rv[i] = -(int32_t)i;
+    }
+    auto join = [&](bool obl) {
+        SortMergeJoinOperator j(ctx, std::make_unique<BatchSourceOperator>(ctx, id_value(lk, lv)),
+                                std::make_unique<BatchSourceOperator>(ctx, id_value(rk, rv)), {0, 0},
+                                JoinType::INNER, obl);
+        auto res = j.next();
+        CHECK(static_cast<bool>(res));
+        std::vector<std::vector<int32_t>> cols;
+        if (!res) return cols;
+        RowBatch jb = std::move(res).value();
+        for (size_t c = 0; c < 4; ++c) cols.push_back(jb.getColumn(c).toHost<int32_t>(jb.getRowCount()));
+        return cols;
+    };
+    const auto a = join(false), o = join(true);
+    CHECK(!a.empty() && a == o);
+}
+
 static void test_errors_are_results(ExecutionContext &ctx)
 {
     // a failure of the library comes back as Result::failure with its message
@@ -670,6 +736,7 @@ int main()
         {"sort_merge_join", [&] { test_sort_merge_join(ctx); }},
         {"oblivious_table", [&] { test_oblivious_table(tm); }},
         {"padded_transfers", [&] { test_padded_transfers(tm); }},
+        {"materialization", [&] { test_materialization(ctx); }},
         {"oblivious_table_manager", [&] { test_oblivious_table_manager(tm); }},
         {"operators_over_resident_queue", [&] { test_operators_over_resident_queue(); }},
         {"join_groupsum_dense", [&] { test_join_groupsum(ctx, true); }},

@@ -409,3 +409,22 @@ def test_key_codes_preserve_order_and_equality():
         order = np.argsort(v, kind="stable")
         assert np.array_equal(np.argsort(c, kind="stable"), order)
         assert np.array_equal(c[:, None] == c[None, :], v[:, None] == v[None, :])
+
+
+# ------------------------------------------------- oblivious materialization
+
+@pytest.mark.parametrize("n_table,n", [(1, 1), (10, 0), (1000, 1), (1000, 5000), (3000, 777), (0, 40)])
+def test_gather_recover_equals_take(n_table, n):
+    """The restated materialization (tuple bitonic sort, cached reads,
+    restore) equals numpy take, with repeated and out-of-range ids."""
+    rng = np.random.default_rng(n_table * 7 + n)
+    table = rng.integers(0, 1 << 63, n_table, dtype=np.uint64)
+    ids = rng.integers(-3, max(n_table, 1) + 3, n).astype(np.int64)
+    if n > 10:
+        ids[:10] = ids[0]  # a run of one repeated id
+    out, found = oracle.gather_recover(table, ids)
+    ok = (ids >= 0) & (ids < n_table)
+    want = np.zeros(n, np.uint64)
+    want[ok] = table[ids[ok]]
+    assert np.array_equal(found.astype(bool), ok)
+    assert np.array_equal(out, want)

@@ -240,6 +240,13 @@ class Context:
         self.join_pairs_write(left_keys, nl, right_keys, nr, type_id, ol, orr, cap)
         return n, ol, orr
 
+    def gather_oblivious(self, table, n_table, record_ids, n, out, elem_bytes, out_found=None):
+        """OP_GATHER_OBLIVIOUS: out[i] = table[record_ids[i]] (0 and
+        found = 0 for ids outside the table)."""
+        a = _abi.GatherObliviousArgs(_ptr(table), n_table, _ptr(record_ids), n, _ptr(out), _ptr(out_found),
+                                     elem_bytes, 0)
+        return self.check(self.lib.vo_gather_oblivious(self.h, C.byref(a)))
+
     def transfer_to_device(self, dst, src_host, padded_bytes=None):
         """Pageable numpy array -> device tensor, padded_bytes over the link
         (default: the tensor's size); returns the bytes moved."""

@@ -38,6 +38,7 @@ OP_SHUFFLE = 0x102
 OP_FILTER_SUM = 0x103
 OP_PKFK_JOIN_GROUPSUM = 0x104
 OP_SORT_BATCH = 0x105
+OP_GATHER_OBLIVIOUS = 0x106
 OP_TERMINATE = 0xFFFFFFFF
 
 CFG_PERSISTENT = 1 << 0
@@ -116,12 +117,17 @@ class JoinPairsArgs(C.Structure):
                 ("reserved", u32), ("out_count", P), ("out_left", P), ("out_right", P), ("out_cap", u64)]
 
 
+class GatherObliviousArgs(C.Structure):
+    _fields_ = [("table", P), ("n_table", u64), ("record_ids", P), ("n", u64), ("out", P), ("out_found", P),
+                ("elem_bytes", u32), ("reserved", u32)]
+
+
 class CommandArgs(C.Union):
     _fields_ = [("sort", SortArgs), ("sort_kp", SortKpArgs), ("permute", PermuteArgs), ("scatter", ScatterArgs),
                 ("gather", GatherArgs), ("compare_eq", CompareEqArgs), ("compose_position_map", ComposeArgs),
                 ("enrich_positions", EnrichArgs), ("compact", CompactArgs), ("shuffle", ShuffleArgs),
                 ("filter", FilterArgs), ("join", JoinArgs), ("sort_batch", SortBatchArgs),
-                ("join_pairs", JoinPairsArgs)]
+                ("join_pairs", JoinPairsArgs), ("gather_oblivious", GatherObliviousArgs)]
 
 
 class Command(C.Structure):
@@ -183,6 +189,7 @@ PROTOTYPES = [
     ("vo_sort_batch", C.c_int, [P, C.POINTER(SortBatchArgs)]),
     ("vo_join_pairs_count", C.c_int, [P, C.POINTER(JoinPairsArgs), C.POINTER(u64)]),
     ("vo_transfer_to_device", C.c_int, [P, P, P, u64, u64, C.POINTER(u64)]),
+    ("vo_gather_oblivious", C.c_int, [P, C.POINTER(GatherObliviousArgs)]),
     ("vo_transfer_to_host", C.c_int, [P, P, P, u64, u64, C.POINTER(u64)]),
     ("vo_join_pairs_write", C.c_int, [P, C.POINTER(JoinPairsArgs)]),
     ("vo_sort_kp_async", C.c_int, [P, P, P, u64, u64]),
