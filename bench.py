@@ -284,10 +284,10 @@ def bench_join_pairs(args):
             "join_pairs_dist_level_w2", "join_pairs_dist_tile_w1", "join_pairs_dist_tile_w2",
             "join_pairs_fill_carry", "join_pairs_fill_left", "join_pairs_fill_right", "join_pairs_out"]
     stats = _family_stats(ctx, fams)
-    # algorithmic bytes per padded row and launch: a distribution level reads
-    # and writes every slot once (8 or 16 B each way; the row j before a slot
-    # is the same data, read by another thread); the scans read the sorted
-    # union (16 B) and write their outputs
+    # algorithmic bytes per padded row and launch: a distribution pass (three
+    # levels) reads and writes every slot once (8 or 16 B each way; the rows
+    # before a slot that it also reads are the same data, read by other
+    # threads); the scans read the sorted union (16 B) and write their outputs
     bpr = {"join_pairs_dist_level_w1": 16, "join_pairs_dist_level_w2": 32, "join_pairs_bwd_rows": 20,
            "join_pairs_fwd_rows": 46}
     n_pad = 1 << 23

@@ -462,25 +462,59 @@ __global__ void k_jp_dist_init(const uint64_t *__restrict__ s0, const uint64_t *
     tr.done(dev::gtid());
 }
 
-// One level j (>= 256): new[p] from old[p] and old[p - j].
-template <int NW, bool TRACE>
-__global__ void k_jp_dist_level(const uint64_t *__restrict__ i0, const uint64_t *__restrict__ i1, uint64_t m_pad,
-                                uint32_t j, uint64_t *__restrict__ o0, uint64_t *__restrict__ o1, TraceSlot *slot)
+// S levels j_top, j_top/2, .., j_top/2^(S-1) in one pass: the value of slot
+// p after them depends only on the 2^S slots p - m * (j_top >> (S-1)),
+// m < 2^S, so each output reads those (the neighbours' reads are the same
+// rows and come from the caches) and applies the levels in order.
+#ifndef VO_DIST_FUSE
+#define VO_DIST_FUSE 3
+#endif
+constexpr int kDistFuse = VO_DIST_FUSE;
+static_assert(kDistFuse >= 1 && kDistFuse <= 4, "1 to 4 distribution levels per pass");
+
+template <int NW>
+__device__ __forceinline__ Elem<NW> dist_step(const Elem<NW> &cur, const Elem<NW> &prv, int64_t p, uint64_t j)
+{
+    const bool in = (prv.w[0] != EMPTY) & ((int64_t)tgt(prv.w[0]) >= p);
+    const bool out = (cur.w[0] != EMPTY) & ((int64_t)tgt(cur.w[0]) >= p + (int64_t)j);
+    Elem<NW> r;
+    r.w[0] = in ? prv.w[0] : (out ? EMPTY : cur.w[0]);
+    if constexpr (NW == 2) r.w[1] = in ? prv.w[1] : cur.w[1];
+    return r;
+}
+
+template <int S, int NW, class Tr>
+__device__ __forceinline__ Elem<NW> dist_val(const uint64_t *__restrict__ i0, const uint64_t *__restrict__ i1,
+                                             int64_t p, uint64_t j_top, Tr &tr)
+{
+    if constexpr (S == 0) {
+        Elem<NW> e;
+        e.w[0] = EMPTY;
+        if constexpr (NW == 2) e.w[1] = EMPTY;
+        if (p >= 0) { // public: the position only
+            e.w[0] = i0[p];
+            if constexpr (NW == 2) e.w[1] = i1[p];
+            tr.at(1, (uint64_t)p);
+        }
+        return e;
+    } else {
+        const uint64_t j = j_top >> (S - 1);
+        const Elem<NW> a = dist_val<S - 1, NW>(i0, i1, p, j_top, tr);
+        const Elem<NW> b = dist_val<S - 1, NW>(i0, i1, p - (int64_t)j, j_top, tr);
+        return dist_step<NW>(a, b, p, j);
+    }
+}
+
+template <int NW, int S, bool TRACE>
+__global__ void k_jp_dist_levels(const uint64_t *__restrict__ i0, const uint64_t *__restrict__ i1, uint64_t m_pad,
+                                 uint64_t j_top, uint64_t *__restrict__ o0, uint64_t *__restrict__ o1,
+                                 TraceSlot *slot)
 {
     dev::AtrT<TRACE> tr(slot);
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m_pad; p += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t c0 = i0[p];
-        uint64_t c1 = 0, q1 = 0, q0 = EMPTY;
-        if constexpr (NW == 2) c1 = i1[p];
-        if (p >= j) { // public
-            q0 = i0[p - j];
-            if constexpr (NW == 2) q1 = i1[p - j];
-            tr.at(1, p - j);
-        }
-        const bool in = (q0 != EMPTY) & ((uint64_t)tgt(q0) >= p);
-        const bool out = (c0 != EMPTY) & ((uint64_t)tgt(c0) >= p + j);
-        o0[p] = in ? q0 : (out ? EMPTY : c0);
-        if constexpr (NW == 2) o1[p] = in ? q1 : c1;
+        const Elem<NW> e = dist_val<S, NW>(i0, i1, (int64_t)p, j_top, tr);
+        o0[p] = e.w[0];
+        if constexpr (NW == 2) o1[p] = e.w[1];
         tr.at(2, p);
     }
     tr.done(dev::gtid());
@@ -701,11 +735,24 @@ void distribute(vo_ctx *ctx, const uint64_t *s0, const uint64_t *s1, uint64_t sr
            tr ? k_jp_dist_init<NW, true> : k_jp_dist_init<NW, false>, s0, s1, src_rows, kdev, m_pad, a0, a1,
            trace_slot(ctx));
     uint64_t *c0 = a0, *c1 = a1, *n0 = b0, *n1 = b1;
-    for (uint64_t j = m_pad / 2; j >= (uint64_t)DH; j >>= 1) {
-        launch(ctx, LaunchInfo{NW == 1 ? "join_pairs_dist_level_w1" : "join_pairs_dist_level_w2", grid_for(m_pad),
-                               dim3(256), 0, hash_mix(hash_mix(NW, m_pad), j)},
-               tr ? k_jp_dist_level<NW, true> : k_jp_dist_level<NW, false>, (const uint64_t *)c0,
-               (const uint64_t *)c1, m_pad, (uint32_t)j, n0, n1, trace_slot(ctx));
+    // levels m_pad/2 .. 256, up to kDistFuse per pass, as even as possible
+    const int count = ilog2(m_pad / 2) - ilog2(DH) + 1;
+    const int groups = (count + kDistFuse - 1) / kDistFuse;
+    uint64_t j = m_pad / 2;
+    for (int gi = 0; gi < groups; ++gi) {
+        const int sl = count / groups + (gi < count % groups ? 1 : 0);
+        LaunchInfo li{NW == 1 ? "join_pairs_dist_level_w1" : "join_pairs_dist_level_w2", grid_for(m_pad), dim3(256), 0,
+                      hash_mix(hash_mix(NW, m_pad), hash_mix(j, sl))};
+        auto go = [&](auto k) {
+            launch(ctx, li, k, (const uint64_t *)c0, (const uint64_t *)c1, m_pad, j, n0, n1, trace_slot(ctx));
+        };
+        switch (sl) {
+        case 1: go(tr ? k_jp_dist_levels<NW, 1, true> : k_jp_dist_levels<NW, 1, false>); break;
+        case 2: go(tr ? k_jp_dist_levels<NW, 2, true> : k_jp_dist_levels<NW, 2, false>); break;
+        case 3: go(tr ? k_jp_dist_levels<NW, 3, true> : k_jp_dist_levels<NW, 3, false>); break;
+        default: go(tr ? k_jp_dist_levels<NW, 4, true> : k_jp_dist_levels<NW, 4, false>); break;
+        }
+        j >>= sl;
         std::swap(c0, n0);
         std::swap(c1, n1);
     }
