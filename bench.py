@@ -222,7 +222,7 @@ def bench_join(args):
     ok_sort = ng_sort == ng and np.array_equal(sums_sort, got_s)
     fams = ["join_build", "join_fill_tiles", "join_fill_rows", "group_dense_agg", "group_dense_out",
             "compact_count", "compact_local", "compact_hybrid", "compact_global", "sort_local", "sort_merge",
-            "sort_global_s1", "sort_global_s2", "sort_global_s3", "sort_global_s4", "sort_hybrid_s6"]
+            "sort_global_s1", "sort_global_s2", "sort_global_s3", "sort_global_s4", "sort_hybrid_s6", "sort_hybrid_s7"]
     stats = _family_stats(ctx, fams)
     bpr = {f: 32 for f in fams if f.startswith("sort_") or f in ("compact_global", "compact_hybrid")}
     _line("oblivious PK/FK join + GROUP BY SUM (Mrows/s)", (n_dim + n_fact) / dt / 1e6, "Mrows/s", args.steps,
@@ -391,7 +391,7 @@ def bench_gather(args):
     dt = (time.perf_counter() - t0) / args.steps
     ok = bool(torch.all(out == table[ids]))
     fams = ["gather_obl_build", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2", "sort_global_s3",
-            "sort_global_s4", "sort_hybrid_s6", "gather_obl_tiles", "gather_obl_carry", "gather_obl_rows",
+            "sort_global_s4", "sort_hybrid_s6", "sort_hybrid_s7", "gather_obl_tiles", "gather_obl_carry", "gather_obl_rows",
             "compact_count", "compact_scan", "compact_local", "compact_hybrid", "compact_global", "gather_obl_order",
             "gather_obl_out"]
     stats = _family_stats(ctx, fams)
