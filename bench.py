@@ -220,7 +220,7 @@ def bench_join(args):
     got_s = osum[:ng].cpu().numpy()
     ok = ng == int((exp > 0).sum()) and np.array_equal(got_s, sums[got_g])
     ok_sort = ng_sort == ng and np.array_equal(sums_sort, got_s)
-    fams = ["join_build", "join_fill_tiles", "join_fill_rows", "group_dense_agg", "group_dense_out",
+    fams = ["join_build", "join_fill_tiles", "join_fill_rows", "join_fill_agg", "group_dense_out",
             "compact_count", "compact_local", "compact_hybrid", "compact_global", "sort_local", "sort_merge",
             "sort_global_s1", "sort_global_s2", "sort_global_s3", "sort_global_s4", "sort_hybrid_s6", "sort_hybrid_s7"]
     stats = _family_stats(ctx, fams)

@@ -20,8 +20,9 @@
 //   6. segmented inclusive sum by group; the last row of each group selected
 //   7. ORCompact -> the first G rows are (group, sum), ascending by group.
 // With dense_groups (group ids known to lie in [0, max_groups), max_groups
-// <= 64), steps 5-7 become one oblivious dense aggregation over the
-// compacted rows: every row updates all max_groups accumulators by selects.
+// <= 64), steps 4-7 become one oblivious dense aggregation fused with the
+// fill of step 3: every union row updates all G per-thread sums by selects,
+// so the matched rows need no compaction.
 // Every pass reads and writes every row; only the group count G and the
 // match count leave the device.
 #include "vo_internal.hpp"
@@ -446,11 +447,6 @@ __global__ void k_group_out(const uint64_t *__restrict__ col_g, const uint64_t *
 
 // ------------------------------- dense GROUP BY (group ids in [0, G))
 
-// Oblivious dense aggregation over the compacted matched rows: every row
-// (all n_pad of them; rows >= count contribute zero) updates all G per-thread
-// accumulators through selects, so no address depends on a group id.
-constexpr uint32_t kAggBlocks = 1024;
-
 __device__ __forceinline__ int64_t wave_sum64(int64_t v)
 {
 #pragma unroll
@@ -458,94 +454,140 @@ __device__ __forceinline__ int64_t wave_sum64(int64_t v)
     return v;
 }
 
+// The dense GROUP BY fused with the forward fill: per tile, the fill of
+// k_fill_rows, and every row adds its value to all G per-thread sums through
+// selects (matched fact rows whose group is g); a presence bitmask stands in
+// for the per-group counts.  One partial row per tile; no compaction is
+// needed because every union row is visited anyway.
 template <int G, bool TRACE>
-__global__ __launch_bounds__(256) void k_dense_agg(const uint64_t *__restrict__ col_g, const uint64_t *__restrict__ col_v,
-                                                   const uint64_t *__restrict__ count, uint64_t n_pad,
-                                                   int64_t *__restrict__ part_sum, int64_t *__restrict__ part_cnt,
-                                                   TraceSlot *slot)
+__global__ __launch_bounds__(JB) void k_fill_agg(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
+                                                 const Fill *__restrict__ carry, int64_t *__restrict__ part_sum,
+                                                 int64_t *__restrict__ part_cnt, TraceSlot *slot)
 {
-    __shared__ int64_t rs[4][G], rc[4][G];
+    static_assert(G <= 64, "presence bits in one word");
+    __shared__ uint64_t wtk[JB / 64], wtg[JB / 64];
+    __shared__ int64_t rs[JB / 64][G];
+    __shared__ uint64_t rb[JB / 64];
     dev::AtrT<TRACE> tr(slot);
-    const uint64_t m = *count;
-    int64_t acc[G];
-    uint32_t cnt[G]; // rows per thread stay far below 2^32
+    const uint64_t t0 = (uint64_t)blockIdx.x * JT + (uint64_t)threadIdx.x * JPER;
+    uint64_t lk = PAD_KEY, lg = 0;
 #pragma unroll
-    for (int g = 0; g < G; ++g) {
-        acc[g] = 0;
-        cnt[g] = 0;
+    for (int k = 0; k < JPER; ++k) {
+        const uint64_t key = keys[t0 + k], v = vals[t0 + k];
+        const bool d = is_dim(key);
+        lk = d ? (key >> 32) : lk;
+        lg = d ? v : lg;
     }
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * 256) {
-        const uint64_t grp_i = col_g[i] & 0xFFFFFFFFull;
-        const int64_t v = (int64_t)col_v[i];
-        tr.at(1, i);
-        const bool live = i < m;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t ik = lk, ig = lg;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t pk = __shfl_up(ik, off, 64), pg = __shfl_up(ig, off, 64);
+        const bool take = (lane >= (uint32_t)off) & (ik == PAD_KEY);
+        ik = take ? pk : ik;
+        ig = take ? pg : ig;
+    }
+    const uint64_t ek = __shfl_up(ik, 1, 64), eg = __shfl_up(ig, 1, 64);
+    if (lane == 63) {
+        wtk[wv] = ik;
+        wtg[wv] = ig;
+    }
+    __syncthreads();
+    Fill c = carry[blockIdx.x];
+    tr.at(4, blockIdx.x);
+    for (uint32_t w = 0; w < wv; ++w) {
+        const bool has = wtk[w] != PAD_KEY;
+        c.key = has ? wtk[w] : c.key;
+        c.group = has ? wtg[w] : c.group;
+    }
+    {
+        const bool has = (lane > 0) & (ek != PAD_KEY);
+        c.key = has ? ek : c.key;
+        c.group = has ? eg : c.group;
+    }
+    int64_t acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g] = 0;
+    uint64_t bits = 0;
+#pragma unroll
+    for (int k = 0; k < JPER; ++k) {
+        const uint64_t i = t0 + k;
+        const uint64_t key = keys[i], v = vals[i];
+        const bool d = is_dim(key);
+        c.key = d ? (key >> 32) : c.key;
+        c.group = d ? v : c.group;
+        const bool fact = key != PAD_KEY && ((key >> 31) & 1);
+        const bool matched = fact & (c.key == (key >> 32));
+        const uint64_t grp = c.group & 0xFFFFFFFFull;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            const bool hit = live & (grp_i == (uint64_t)g);
-            acc[g] += hit ? v : 0;
-            cnt[g] += hit ? 1 : 0;
+            const bool hit = matched & (grp == (uint64_t)g);
+            acc[g] += hit ? (int64_t)v : 0;
+            bits |= hit ? (1ull << g) : 0ull;
         }
+        tr.at(5, i);
     }
-    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const int64_t s = wave_sum64(acc[g]), c = wave_sum64((int64_t)cnt[g]);
-        if (lane == 0) {
-            rs[w][g] = s;
-            rc[w][g] = c;
-        }
+        const int64_t sg = wave_sum64(acc[g]);
+        if (lane == 0) rs[wv][g] = sg;
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) bits |= (uint64_t)__shfl_xor((unsigned long long)bits, off, 64);
+    if (lane == 0) rb[wv] = bits;
     __syncthreads();
     if (threadIdx.x < G) {
         const uint32_t g = threadIdx.x;
-        part_sum[(uint64_t)blockIdx.x * G + g] = rs[0][g] + rs[1][g] + rs[2][g] + rs[3][g];
-        part_cnt[(uint64_t)blockIdx.x * G + g] = rc[0][g] + rc[1][g] + rc[2][g] + rc[3][g];
-        tr.at(2, (uint64_t)blockIdx.x * G + g);
+        int64_t sg = 0;
+        uint64_t b = 0;
+#pragma unroll
+        for (int w = 0; w < JB / 64; ++w) {
+            sg += rs[w][g];
+            b |= rb[w];
+        }
+        part_sum[(uint64_t)blockIdx.x * G + g] = sg;
+        part_cnt[(uint64_t)blockIdx.x * G + g] = (int64_t)((b >> g) & 1);
+        tr.at(6, (uint64_t)blockIdx.x * G + g);
     }
     tr.done(dev::gtid());
 }
 
-// One block: totals per group (fixed summation order), then output slot t
-// takes the t-th present group, chosen by selects over all groups.
+// One block of 1024 threads: totals per group, then output slot t takes the
+// t-th present group, chosen by selects over all groups.  The nparts x G
+// partials are read contiguously; thread t only ever meets group t mod G
+// (G divides 1024), and the threads of one group are summed in a fixed order.
+constexpr uint32_t kOutThreads = 1024;
+
 template <int G>
-__global__ __launch_bounds__(256) void k_dense_out(const int64_t *__restrict__ part_sum,
-                                                   const int64_t *__restrict__ part_cnt, uint32_t nparts,
-                                                   uint64_t max_groups, int32_t *__restrict__ out_g,
-                                                   int64_t *__restrict__ out_s, uint64_t *__restrict__ out_n,
-                                                   TraceSlot *slot)
+__global__ __launch_bounds__(kOutThreads) void k_dense_out(const int64_t *__restrict__ part_sum,
+                                                           const int64_t *__restrict__ part_cnt, uint32_t nparts,
+                                                           uint64_t max_groups, int32_t *__restrict__ out_g,
+                                                           int64_t *__restrict__ out_s, uint64_t *__restrict__ out_n,
+                                                           TraceSlot *slot)
 {
+    static_assert(kOutThreads % G == 0, "G divides the block");
+    __shared__ int64_t ss[kOutThreads], sc[kOutThreads];
     __shared__ int64_t ts[G], tc[G];
-    __shared__ int64_t ws[4][G], wc[4][G];
     dev::Atr tr(slot);
     const uint32_t t = threadIdx.x;
-    // thread t sums partial rows t, t+256, ...; then a block reduction per group
-    // (a thread's column of partials is contiguous: coalesced across lanes)
-    for (int g0 = 0; g0 < G; g0 += 16) {
-        int64_t s[16], c[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) s[k] = c[k] = 0;
-        for (uint32_t p = t; p < nparts; p += 256) {
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                s[k] += part_sum[(uint64_t)p * G + g0 + k];
-                c[k] += part_cnt[(uint64_t)p * G + g0 + k];
-                tr.at(1, (uint64_t)p * G + g0 + k);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int64_t rs_ = wave_sum64(s[k]), rc_ = wave_sum64(c[k]);
-            if ((t & 63) == 0) {
-                ws[t >> 6][g0 + k] = rs_;
-                wc[t >> 6][g0 + k] = rc_;
-            }
-        }
+    int64_t s = 0, c = 0;
+    const uint64_t total = (uint64_t)nparts * G;
+    for (uint64_t i = t; i < total; i += kOutThreads) {
+        s += part_sum[i];
+        c += part_cnt[i];
+        tr.at(1, i);
     }
+    ss[t] = s;
+    sc[t] = c;
     __syncthreads();
     if (t < G) {
-        ts[t] = ws[0][t] + ws[1][t] + ws[2][t] + ws[3][t];
-        const int64_t c = wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
-        tc[t] = (uint64_t)t < max_groups ? c : 0;
+        int64_t a = 0, b = 0;
+        for (uint32_t u = t; u < kOutThreads; u += G) {
+            a += ss[u];
+            b += sc[u];
+        }
+        ts[t] = a;
+        tc[t] = (uint64_t)t < max_groups ? b : 0;
     }
     __syncthreads();
     uint64_t present = 0;
@@ -594,7 +636,7 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
                  o_m = take(n_pad), o_fl = take(nt * sizeof(Fill)), o_fc = take(nt * sizeof(Fill)),
                  o_sg = take(nt * sizeof(Seg)), o_sc = take(nt * 8), o_ss = take(nt * sizeof(SegSum)),
                  o_sp = take(nt * sizeof(SegSum)), o_c1 = take(compact_rows2_scratch(n_pad)),
-                 o_c2 = take(compact_rows2_scratch(n_pad)), o_part = take(kAggBlocks * 64 * 16);
+                 o_c2 = take(compact_rows2_scratch(n_pad)), o_part = take((size_t)nt * 64 * 16);
     char *base = (char *)scratch(ctx, off);
     uint64_t *keys = (uint64_t *)(base + o_k), *vals = (uint64_t *)(base + o_v);
     uint64_t *col_a = (uint64_t *)(base + o_a), *col_b = (uint64_t *)(base + o_b);
@@ -621,37 +663,33 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
     launch(ctx, LaunchInfo{"join_fill_carry", dim3(1), dim3(kScanThreads), 0, nt},
            ctx->tracing ? k_fill_carry<true> : k_fill_carry<false>,
            (const Fill *)tile_last, nt, fcarry, trace_slot(ctx));
+    if (a.dense_groups) {
+        // 4'-5'. dense GROUP BY fused with the fill: one partial row per tile
+        int64_t *part = (int64_t *)(base + o_part);
+        const uint32_t G = a.max_groups <= 32 ? 32 : 64;
+        if (G == 32) {
+            launch(ctx, LaunchInfo{"join_fill_agg", dim3(nt), dim3(JB), 0, hash_mix(n_pad, 32)},
+                   ctx->tracing ? k_fill_agg<32, true> : k_fill_agg<32, false>, (const uint64_t *)keys,
+                   (const uint64_t *)vals, (const Fill *)fcarry, part, part + (uint64_t)nt * 32, trace_slot(ctx));
+            launch(ctx, LaunchInfo{"group_dense_out", dim3(1), dim3(kOutThreads), 0, a.max_groups}, k_dense_out<32>,
+                   (const int64_t *)part, (const int64_t *)(part + (uint64_t)nt * 32), nt, a.max_groups, a.out_group,
+                   a.out_sum, a.out_n_groups, trace_slot(ctx));
+        } else {
+            launch(ctx, LaunchInfo{"join_fill_agg", dim3(nt), dim3(JB), 0, hash_mix(n_pad, 64)},
+                   ctx->tracing ? k_fill_agg<64, true> : k_fill_agg<64, false>, (const uint64_t *)keys,
+                   (const uint64_t *)vals, (const Fill *)fcarry, part, part + (uint64_t)nt * 64, trace_slot(ctx));
+            launch(ctx, LaunchInfo{"group_dense_out", dim3(1), dim3(kOutThreads), 0, a.max_groups}, k_dense_out<64>,
+                   (const int64_t *)part, (const int64_t *)(part + (uint64_t)nt * 64), nt, a.max_groups, a.out_group,
+                   a.out_sum, a.out_n_groups, trace_slot(ctx));
+        }
+        return;
+    }
     launch(ctx, LaunchInfo{"join_fill_rows", dim3(nt), dim3(JB), 0, n_pad},
            ctx->tracing ? k_fill_rows<true> : k_fill_rows<false>,
            (const uint64_t *)keys,
            (const uint64_t *)vals, (const Fill *)fcarry, col_a, col_b, mask, trace_slot(ctx));
     // 4. compaction of matched rows
     compact_rows2(ctx, col_a, col_b, mask, n_pad, cnt1);
-    if (a.dense_groups) {
-        // 5'. dense GROUP BY over the compacted rows
-        int64_t *part = (int64_t *)(base + o_part);
-        // 8 rows per thread at least; at most kAggBlocks partial rows
-        const uint32_t nb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kAggBlocks, n_pad / 2048));
-        const uint32_t G = a.max_groups <= 32 ? 32 : 64;
-        if (G == 32) {
-            launch(ctx, LaunchInfo{"group_dense_agg", dim3(nb), dim3(256), 0, hash_mix(n_pad, 32)},
-                   ctx->tracing ? k_dense_agg<32, true> : k_dense_agg<32, false>, (const uint64_t *)col_a,
-                   (const uint64_t *)col_b, (const uint64_t *)cnt1, n_pad, part, part + (uint64_t)nb * 32,
-                   trace_slot(ctx));
-            launch(ctx, LaunchInfo{"group_dense_out", dim3(1), dim3(256), 0, a.max_groups}, k_dense_out<32>,
-                   (const int64_t *)part, (const int64_t *)(part + (uint64_t)nb * 32), nb, a.max_groups,
-                   a.out_group, a.out_sum, a.out_n_groups, trace_slot(ctx));
-        } else {
-            launch(ctx, LaunchInfo{"group_dense_agg", dim3(nb), dim3(256), 0, hash_mix(n_pad, 64)},
-                   ctx->tracing ? k_dense_agg<64, true> : k_dense_agg<64, false>, (const uint64_t *)col_a,
-                   (const uint64_t *)col_b, (const uint64_t *)cnt1, n_pad, part, part + (uint64_t)nb * 64,
-                   trace_slot(ctx));
-            launch(ctx, LaunchInfo{"group_dense_out", dim3(1), dim3(256), 0, a.max_groups}, k_dense_out<64>,
-                   (const int64_t *)part, (const int64_t *)(part + (uint64_t)nb * 64), nb, a.max_groups,
-                   a.out_group, a.out_sum, a.out_n_groups, trace_slot(ctx));
-        }
-        return;
-    }
     // 5. group-by sort
     launch(ctx, LaunchInfo{"group_keys", grid_for(n_pad), dim3(256), 0, n_pad},
            ctx->tracing ? k_group_keys<true> : k_group_keys<false>,

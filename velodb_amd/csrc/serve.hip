@@ -16,6 +16,8 @@
 // schedule looks at (arrival times, table sizes) is public.
 #include "vo_internal.hpp"
 
+#include <map>
+
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -113,6 +115,8 @@ __global__ __launch_bounds__(256) void k_sort_checksum(const uint64_t *__restric
 constexpr uint32_t kChecksumBlocks = 256;
 constexpr uint64_t kMaxGroups = 64;
 constexpr uint64_t kJoinGroups = 32; // public bound of the generated group attribute (0..24)
+// pinned result words of one query: filter 32 B, join groups, sort partial sums
+constexpr size_t kResBytes = 2 * kChecksumBlocks * 8 > kMaxGroups * 12 + 8 ? 2 * kChecksumBlocks * 8 : kMaxGroups * 12 + 8;
 
 // Query intermediates that live outside the slot context's scratch arena.
 // In transfer mode the buffer also holds the query's input columns.
@@ -155,6 +159,15 @@ struct vo_server {
     std::vector<void *> host_allocations;
     std::vector<void *> slot_hbuf; // pinned host staging per slot (transfer mode)
     std::vector<uint64_t> slot_hbuf_bytes;
+    std::vector<void *> slot_res; // pinned result words per slot (the last node of a query)
+    // per slot: captured query graphs by (kind, table), valid while the slot's
+    // scratch arena and query buffer are the allocations they were captured on
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        void *scr = nullptr, *buf = nullptr;
+        bool failed = false;
+    };
+    std::vector<std::map<uint64_t, Graph>> graphs;
 };
 
 namespace {
@@ -213,6 +226,10 @@ vo_ctx *slot_ctx(vo_server *s, uint32_t i)
         s->slot_buf_bytes.push_back(0);
         s->slot_hbuf.push_back(nullptr);
         s->slot_hbuf_bytes.push_back(0);
+        void *res = nullptr;
+        VO_HIP(hipHostMalloc(&res, kResBytes, hipHostMallocDefault));
+        s->slot_res.push_back(res);
+        s->graphs.emplace_back();
     }
     return s->slots[i];
 }
@@ -238,11 +255,12 @@ void h2d(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes)
     if (bytes) VO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
 }
 
-// Runs one query on a slot and fills its result words (blocking).  With
+// Enqueues one query on a slot's stream, ending with the copy of its result
+// words to the slot's pinned result buffer `res`; nothing here waits.  With
 // `transfer`, the input columns are first copied from the table's pinned host
 // copy into the slot buffer and the query runs on those copies; a sort also
 // copies its sorted columns back to host memory (hbuf).
-void run_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_query &q, bool transfer, uint64_t r[4])
+void enqueue_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_query &q, bool transfer, void *res)
 {
     char *b = (char *)buf;
     switch (q.kind) {
@@ -277,13 +295,7 @@ void run_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_query 
         a.disc_hi = 7;
         a.qty_lt = 24;
         op_filter_sum(ctx, a);
-        uint64_t h[4];
-        VO_HIP(hipMemcpyAsync(h, a.out, 32, hipMemcpyDeviceToHost, ctx->stream));
-        VO_HIP(hipStreamSynchronize(ctx->stream));
-        r[0] = h[0];
-        r[1] = h[2];
-        r[2] = h[3];
-        r[3] = 0;
+        VO_HIP(hipMemcpyAsync(res, a.out, 32, hipMemcpyDeviceToHost, ctx->stream));
         break;
     }
     case VO_Q_JOIN_GROUPSUM: {
@@ -319,21 +331,7 @@ void run_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_query 
         a.max_groups = kJoinGroups;
         a.dense_groups = 1;
         op_join_groupsum(ctx, a);
-        std::vector<char> h(kMaxGroups * 12 + 8);
-        VO_HIP(hipMemcpyAsync(h.data(), o, h.size(), hipMemcpyDeviceToHost, ctx->stream));
-        VO_HIP(hipStreamSynchronize(ctx->stream));
-        const int32_t *g = (const int32_t *)h.data();
-        const int64_t *sm = (const int64_t *)(h.data() + kMaxGroups * 4);
-        const uint64_t G = *(const uint64_t *)(h.data() + kMaxGroups * 12);
-        uint64_t w = 0, tot = 0;
-        for (uint64_t i = 0; i < std::min<uint64_t>(G, kMaxGroups); ++i) {
-            w += (uint64_t)(g[i] + 1) * (uint64_t)sm[i];
-            tot += (uint64_t)sm[i];
-        }
-        r[0] = G;
-        r[1] = w;
-        r[2] = tot;
-        r[3] = 0;
+        VO_HIP(hipMemcpyAsync(res, o, kMaxGroups * 12 + 8, hipMemcpyDeviceToHost, ctx->stream));
         break;
     }
     case VO_Q_SORT: {
@@ -357,22 +355,116 @@ void run_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_query 
             VO_HIP(hipMemcpyAsync(hb, k, 8 * t.n, hipMemcpyDeviceToHost, ctx->stream));
             VO_HIP(hipMemcpyAsync(hb + 8 * t.n, p, 4 * t.n, hipMemcpyDeviceToHost, ctx->stream));
         }
-        std::vector<uint64_t> h(2 * kChecksumBlocks);
-        VO_HIP(hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-        VO_HIP(hipStreamSynchronize(ctx->stream));
+        VO_HIP(hipMemcpyAsync(res, part, 2 * kChecksumBlocks * 8, hipMemcpyDeviceToHost, ctx->stream));
+        break;
+    }
+    default: throw Error{VO_ERR_INVALID, "unknown query kind"};
+    }
+}
+
+// The result words of a finished query from its result buffer.
+void finish_query(vo_server *s, const vo_query &q, const void *res, uint64_t r[4])
+{
+    switch (q.kind) {
+    case VO_Q_FILTER_SUM: {
+        const uint64_t *h = (const uint64_t *)res;
+        r[0] = h[0];
+        r[1] = h[2];
+        r[2] = h[3];
+        r[3] = 0;
+        break;
+    }
+    case VO_Q_JOIN_GROUPSUM: {
+        const char *h = (const char *)res;
+        const int32_t *g = (const int32_t *)h;
+        const int64_t *sm = (const int64_t *)(h + kMaxGroups * 4);
+        const uint64_t G = *(const uint64_t *)(h + kMaxGroups * 12);
+        uint64_t w = 0, tot = 0;
+        for (uint64_t i = 0; i < std::min<uint64_t>(G, kMaxGroups); ++i) {
+            w += (uint64_t)(g[i] + 1) * (uint64_t)sm[i];
+            tot += (uint64_t)sm[i];
+        }
+        r[0] = G;
+        r[1] = w;
+        r[2] = tot;
+        r[3] = 0;
+        break;
+    }
+    default: { // VO_Q_SORT
+        const uint64_t *h = (const uint64_t *)res;
         uint64_t x = 0, y = 0;
         for (uint32_t i = 0; i < kChecksumBlocks; ++i) {
             x += h[2 * i];
             y += h[2 * i + 1];
         }
-        r[0] = t.n;
+        r[0] = s->sorts[q.table].n;
         r[1] = x;
         r[2] = y;
         r[3] = 0;
         break;
     }
-    default: throw Error{VO_ERR_INVALID, "unknown query kind"};
     }
+}
+
+// Records the query's launches and copies on the slot stream as a graph
+// (nothing runs); false if the stream cannot be captured for it.
+bool capture_query(vo_server *s, vo_ctx *ctx, void *buf, void *res, const vo_query &q, hipGraphExec_t *exec)
+{
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    bool ok = true;
+    try {
+        enqueue_query(s, ctx, buf, nullptr, q, false, res);
+    } catch (const Error &) {
+        ok = false;
+    }
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    ok = ok && e == hipSuccess && graph != nullptr;
+    if (ok) ok = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!ok) {
+        (void)hipGetLastError();
+        *exec = nullptr;
+    }
+    return ok;
+}
+
+// Runs one query on a slot and fills its result words (blocking).  Over the
+// resident tables, a (kind, table) pair seen before on this slot replays the
+// graph captured after its first run: one launch instead of one per kernel.
+void run_query(vo_server *s, uint32_t slot, const vo_query &q, bool transfer, uint64_t r[4])
+{
+    vo_ctx *ctx = s->slots[slot];
+    void *buf = s->slot_buf[slot], *res = s->slot_res[slot];
+    if (transfer) {
+        enqueue_query(s, ctx, buf, s->slot_hbuf[slot], q, true, res);
+        VO_HIP(hipStreamSynchronize(ctx->stream));
+        finish_query(s, q, res, r);
+        return;
+    }
+    auto &cache = s->graphs[slot];
+    const uint64_t key = ((uint64_t)q.kind << 32) | q.table;
+    auto it = cache.find(key);
+    const bool fresh = it != cache.end() && it->second.scr == ctx->scratch && it->second.buf == buf;
+    if (fresh && it->second.exec) {
+        VO_HIP(hipGraphLaunch(it->second.exec, ctx->stream));
+        VO_HIP(hipStreamSynchronize(ctx->stream));
+        finish_query(s, q, res, r);
+        return;
+    }
+    enqueue_query(s, ctx, buf, nullptr, q, false, res);
+    VO_HIP(hipStreamSynchronize(ctx->stream));
+    finish_query(s, q, res, r);
+    if (fresh) return; // a capture of this pair failed before: it stays eager
+    vo_server::Graph g;
+    g.scr = ctx->scratch;
+    g.buf = buf;
+    if (it != cache.end() && it->second.exec) (void)hipGraphExecDestroy(it->second.exec);
+    g.failed = !capture_query(s, ctx, buf, res, q, &g.exec);
+    cache[key] = g;
 }
 
 } // namespace
@@ -402,7 +494,12 @@ int vo_server_destroy(vo_server *s)
 {
     if (!s) return VO_ERR_INVALID;
     (void)hipSetDevice(s->device);
+    for (auto &cache : s->graphs)
+        for (auto &kv : cache)
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
     for (vo_ctx *c : s->slots) vo_ctx_destroy(c);
+    for (void *p : s->slot_res)
+        if (p) (void)hipHostFree(p);
     for (void *p : s->slot_buf)
         if (p) (void)hipFree(p);
     for (void *p : s->allocations) (void)hipFree(p);
@@ -593,7 +690,7 @@ int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_s
                 r.footprint_bytes = fp[i];
                 r.slot = slot;
                 try {
-                    run_query(s, ctx, s->slot_buf[slot], s->slot_hbuf[slot], q[i], transfer, r.r);
+                    run_query(s, slot, q[i], transfer, r.r);
                     r.status = VO_OK;
                 } catch (const Error &e) {
                     r.status = e.status;
