@@ -8,7 +8,7 @@ using namespace sortimpl;
 
 namespace {
 using SortKP = SortImpl<RecKP, 13, 4, 512, 5, 8>;
-using SortKV = SortImpl<RecKV, 12, 4, 256, 4, 7>;
+using SortKV = SortImpl<RecKV, 12, 4, 256, 5, 7>;
 } // namespace
 
 void op_sort_kp(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad)
