@@ -541,7 +541,8 @@ def bench_serve(args):
 
     def summ(st, res=None, qs=None):
         d = {"qps": round(st.qps, 1), "p50_ms": round(st.p50_s * 1e3, 2), "p99_ms": round(st.p99_s * 1e3, 2),
-             "max_concurrent": st.max_concurrent, "peak_GB": round(st.peak_bytes_in_flight / 1e9, 2)}
+             "max_concurrent": st.max_concurrent, "peak_GB": round(st.peak_bytes_in_flight / 1e9, 2),
+             "graph_replays": st.graph_replays}
         if res is not None:
             # mean time from admission to completion, per query kind
             per = {}

@@ -516,6 +516,7 @@ typedef struct vo_run_stats {
     double makespan_s, qps, p50_s, p99_s, mean_s;
     uint64_t peak_bytes_in_flight;
     uint32_t max_concurrent;
+    uint32_t graph_replays; /* resident queries run as one replay of a captured HIP graph */
 } vo_run_stats;
 
 typedef struct vo_server vo_server;

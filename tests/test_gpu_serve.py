@@ -85,3 +85,32 @@ def test_concurrency_raises_throughput():
         assert batched.qps > 1.5 * plain.qps, (batched.qps, plain.qps)
     finally:
         s.close()
+
+
+def test_graph_replays_equal_eager_results():
+    """Resident runs replay the HIP graph captured per (slot, kind, table).
+    On one slot the first run of each (kind, table) pair is eager and every
+    later one a replay; runs on one and on four slots, with eager
+    transfer-mode work and synchronizations between them, each give the eager
+    results for every query."""
+    s, tables = _server(n_slots=4)
+    try:
+        qs = _mix(tables, 48, 0, 7)
+        pairs = len({(k, t) for k, t, _ in qs})
+        eager, _ = s.run(qs, n_slots=1, transfer=True)
+        assert all(r.status == 0 for r in eager)
+        want = [list(r.r) for r in eager]
+        got, st = s.run(qs, n_slots=1)
+        assert [list(r.r) for r in got] == want
+        assert st.graph_replays == len(qs) - pairs  # the replay path runs
+        other = _mix(tables, 12, 0, 8)
+        replays = 0
+        for rnd in range(3):
+            between, _ = s.run(other, n_slots=1, transfer=True)  # eager work between replays
+            assert all(r.status == 0 for r in between)
+            got, st = s.run(qs, n_slots=4 if rnd < 2 else 1)
+            assert [list(r.r) for r in got] == want, rnd
+            replays += st.graph_replays
+        assert replays > 0
+    finally:
+        s.close()

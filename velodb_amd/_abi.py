@@ -158,7 +158,8 @@ class QueryResult(C.Structure):
 
 class RunStats(C.Structure):
     _fields_ = [("makespan_s", C.c_double), ("qps", C.c_double), ("p50_s", C.c_double), ("p99_s", C.c_double),
-                ("mean_s", C.c_double), ("peak_bytes_in_flight", u64), ("max_concurrent", u32)]
+                ("mean_s", C.c_double), ("peak_bytes_in_flight", u64), ("max_concurrent", u32),
+                ("graph_replays", u32)]
 
 
 class Trace(C.Structure):

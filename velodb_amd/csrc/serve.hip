@@ -435,7 +435,7 @@ bool capture_query(vo_server *s, vo_ctx *ctx, void *buf, void *res, const vo_que
 // Runs one query on a slot and fills its result words (blocking).  Over the
 // resident tables, a (kind, table) pair seen before on this slot replays the
 // graph captured after its first run: one launch instead of one per kernel.
-void run_query(vo_server *s, uint32_t slot, const vo_query &q, bool transfer, uint64_t r[4])
+bool run_query(vo_server *s, uint32_t slot, const vo_query &q, bool transfer, uint64_t r[4])
 {
     vo_ctx *ctx = s->slots[slot];
     void *buf = s->slot_buf[slot], *res = s->slot_res[slot];
@@ -443,7 +443,7 @@ void run_query(vo_server *s, uint32_t slot, const vo_query &q, bool transfer, ui
         enqueue_query(s, ctx, buf, s->slot_hbuf[slot], q, true, res);
         VO_HIP(hipStreamSynchronize(ctx->stream));
         finish_query(s, q, res, r);
-        return;
+        return false;
     }
     auto &cache = s->graphs[slot];
     const uint64_t key = ((uint64_t)q.kind << 32) | q.table;
@@ -453,18 +453,19 @@ void run_query(vo_server *s, uint32_t slot, const vo_query &q, bool transfer, ui
         VO_HIP(hipGraphLaunch(it->second.exec, ctx->stream));
         VO_HIP(hipStreamSynchronize(ctx->stream));
         finish_query(s, q, res, r);
-        return;
+        return true;
     }
     enqueue_query(s, ctx, buf, nullptr, q, false, res);
     VO_HIP(hipStreamSynchronize(ctx->stream));
     finish_query(s, q, res, r);
-    if (fresh) return; // a capture of this pair failed before: it stays eager
+    if (fresh) return false; // a capture of this pair failed before: it stays eager
     vo_server::Graph g;
     g.scr = ctx->scratch;
     g.buf = buf;
     if (it != cache.end() && it->second.exec) (void)hipGraphExecDestroy(it->second.exec);
     g.failed = !capture_query(s, ctx, buf, res, q, &g.exec);
     cache[key] = g;
+    return false;
 }
 
 } // namespace
@@ -645,6 +646,7 @@ int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_s
         std::condition_variable cv;
         uint64_t next = 0, in_flight = 0, peak = 0;
         uint32_t running = 0, max_running = 0;
+        std::atomic<uint32_t> replays{0};
         int first_error = VO_OK;
         std::string err_msg;
         using clk = std::chrono::steady_clock;
@@ -690,7 +692,7 @@ int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_s
                 r.footprint_bytes = fp[i];
                 r.slot = slot;
                 try {
-                    run_query(s, slot, q[i], transfer, r.r);
+                    if (run_query(s, slot, q[i], transfer, r.r)) replays.fetch_add(1);
                     r.status = VO_OK;
                 } catch (const Error &e) {
                     r.status = e.status;
@@ -738,6 +740,7 @@ int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_s
             }
             st.peak_bytes_in_flight = peak;
             st.max_concurrent = max_running;
+            st.graph_replays = replays.load();
             *stats = st;
         }
     });
