@@ -1,6 +1,7 @@
 """Edge cases of every operator: empty and ragged inputs, the largest
-supported sort, sizes that must be rejected (a status code, no launch), and
-inputs whose every row is a pad or unmatched."""
+supported sort, sizes that must be rejected (a status code, no launch),
+inputs whose every row is a pad or unmatched, and the pairs join, oblivious
+gather and wire-format transfers at their smallest and degenerate sizes."""
 import numpy as np
 import pytest
 
@@ -140,3 +141,65 @@ def test_sort_batch_empty_and_degenerate(vctx):
     from velodb_amd import VoError
     with pytest.raises(VoError):
         vctx.sort_batch(dk, dp, [(2, 3, 4)])
+
+
+def test_join_pairs_edges(vctx):
+    """Pairs join: WRITE without a COUNT, the reference's write arguments
+    (type 0, capacity 0), one row per side, a single key on both sides."""
+    import velodb_amd
+    from velodb_amd import VoError, _abi
+    ctx2 = velodb_amd.Context()
+    try:
+        d = to_dev(np.array([1], np.int32))
+        o = empty_dev(4, np.int64)
+        with pytest.raises(VoError):  # no COUNT on this context yet
+            ctx2.join_pairs_write(d, 1, d, 1, _abi.T_INTEGER, o, o, 4)
+    finally:
+        ctx2.close()
+    one = to_dev(np.array([5], np.int64))
+    assert vctx.join_pairs_count(one, 1, one, 1, _abi.T_BIGINT) == 1
+    ol, orr = empty_dev(1, np.int64), empty_dev(1, np.int64)
+    vctx.join_pairs_write(one, 1, one, 1, 0, ol, orr, 0)  # type and capacity of the COUNT
+    assert from_dev(ol, np.int64).tolist() == [0] and from_dev(orr, np.int64).tolist() == [0]
+    # 3 x 4 rows of one key: 12 pairs in (left, right) order
+    lk, rk = to_dev(np.full(3, -9, np.int16)), to_dev(np.full(4, -9, np.int16))
+    assert vctx.join_pairs_count(lk, 3, rk, 4, _abi.T_SMALLINT) == 12
+    ol, orr = empty_dev(16, np.int64), empty_dev(16, np.int64)
+    vctx.join_pairs_write(lk, 3, rk, 4, _abi.T_SMALLINT, ol, orr, 16)
+    assert from_dev(ol, np.int64)[:12].tolist() == [a for a in range(3) for _ in range(4)]
+    assert from_dev(orr, np.int64)[:12].tolist() == [b for _ in range(3) for b in range(4)]
+
+
+def test_gather_oblivious_edges(vctx):
+    """Oblivious gather: no requests, an empty table, a one-row table, and a
+    bad element width."""
+    from velodb_amd import VoError
+    t = to_dev(np.array([77], np.uint64))
+    ids = to_dev(np.array([0, 0, 1, -1], np.int64))
+    out = empty_dev(4, np.uint64)
+    found = empty_dev(4, np.uint8)
+    vctx.gather_oblivious(t, 1, ids, 0, out, 8)  # nothing to do
+    vctx.gather_oblivious(t, 1, ids, 4, out, 8, out_found=found)
+    assert from_dev(out, np.uint64).tolist() == [77, 77, 0, 0]
+    assert from_dev(found, np.uint8).tolist() == [1, 1, 0, 0]
+    vctx.gather_oblivious(t, 0, ids, 4, out, 8, out_found=found)  # empty table: nothing found
+    assert from_dev(out, np.uint64).tolist() == [0, 0, 0, 0]
+    assert from_dev(found, np.uint8).tolist() == [0, 0, 0, 0]
+    with pytest.raises(VoError):
+        vctx.gather_oblivious(t, 1, ids, 4, out, 3)
+
+
+def test_transfer_edges(vctx):
+    """Wire format: zero bytes, a padded size that is not a multiple of the
+    staging chunk, an odd byte count."""
+    import torch
+    dev = torch.full((1000,), 0x55, dtype=torch.uint8, device="cuda:0")
+    torch.cuda.synchronize()
+    assert vctx.transfer_to_device(dev, np.zeros(0, np.uint8), padded_bytes=0) == 0
+    src = (np.arange(777) % 251).astype(np.uint8)
+    assert vctx.transfer_to_device(dev, src, padded_bytes=1000) == 1000
+    got = dev.cpu().numpy()
+    assert np.array_equal(got[:777], src) and np.all(got[777:] == 0)
+    back = np.empty(333, np.uint8)
+    assert vctx.transfer_to_host(back, dev, padded_bytes=1000) == 1000
+    assert np.array_equal(back, src[:333])
