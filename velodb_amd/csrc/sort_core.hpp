@@ -364,14 +364,64 @@ struct TileCT {
     static constexpr uint32_t TPAD = T + (T >> 4);
     using P = typename R::P;
 
-    // layout reached after stages (q_start .. 0) starting from layout a
+    // layout reached after stages (q_start .. 0) starting from layout a;
+    // stage 0 in layout 1 runs across lane pairs and keeps the layout
     static constexpr int end_layout(int q, int a, int qlo = 0)
     {
         while (q >= qlo) {
+            if (q == 0 && a == 1) break;
             if (!(a <= q && q < a + R_LOG)) a = q - R_LOG + 1 > 0 ? q - R_LOG + 1 : 0;
             q = (a > qlo ? a : qlo) - 1;
         }
         return a;
+    }
+
+    // The record in register m of the lane whose index differs in bit 0: a
+    // DPP quad permutation (lanes 1 0 3 2) per 32-bit word, no LDS.
+    static __device__ __forceinline__ uint32_t xor1(uint32_t v)
+    {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+    }
+    static __device__ __forceinline__ R lane_partner(const R &a)
+    {
+        R o;
+#pragma unroll
+        for (int w = 0; w < R::KW; ++w)
+            o.k[w] = (uint64_t)xor1((uint32_t)a.k[w]) | ((uint64_t)xor1((uint32_t)(a.k[w] >> 32)) << 32);
+        if constexpr (sizeof(P) == 8)
+            o.p = (P)((uint64_t)xor1((uint32_t)a.p) | ((uint64_t)xor1((uint32_t)((uint64_t)a.p >> 32)) << 32));
+        else
+            o.p = (P)xor1((uint32_t)a.p);
+        return o;
+    }
+
+    // Stage 0 of level P in layout 1: record index bit 0 is thread index
+    // bit 0, so the pair of register m is register m of lanes (2t, 2t+1).
+    // Each lane keeps the smaller or the larger record of its pair, as the
+    // compare-exchange would leave it (ties stay in place).
+    template <int P>
+    static __device__ __forceinline__ void lane_stage0(R (&r)[RR], uint32_t tau, bool asc_hi, int p_top)
+    {
+        const uint32_t base = lay_base<R_LOG>(tau, 1);
+        const bool lower = (tau & 1u) == 0;
+#pragma unroll
+        for (int m = 0; m < RR; ++m) {
+            bool asc;
+            if constexpr (P < 0) {
+                asc = asc_hi;
+            } else {
+                const uint32_t l = base | ((uint32_t)m << 1);
+                asc = (P >= p_top) | (P >= T_LOG ? asc_hi : (((l >> P) & 1u) == 0));
+            }
+            const R o = lane_partner(r[m]);
+            // swap when this lane should keep the smaller and holds the
+            // larger, or the other way round
+            const bool keep_small = asc == lower;
+            const bool sw = keep_small ? rec_gt(r[m], o) : rec_gt(o, r[m]);
+#pragma unroll
+            for (int w = 0; w < R::KW; ++w) r[m].k[w] = sw ? o.k[w] : r[m].k[w];
+            r[m].p = sw ? o.p : r[m].p;
+        }
     }
 
     template <int A>
@@ -504,7 +554,9 @@ struct TileCT {
     static __device__ __forceinline__ void level(R (&r)[RR], char *lds, uint32_t tau, bool asc_hi, int p_top,
                                                  uint64_t &h, uint32_t &n)
     {
-        if constexpr (Q >= QLO) {
+        if constexpr (Q == 0 && A == 1 && QLO == 0) {
+            lane_stage0<P>(r, tau, asc_hi, p_top);
+        } else if constexpr (Q >= QLO) {
             constexpr bool covered = (A <= Q && Q < A + R_LOG);
             constexpr int AN = covered ? A : (Q - R_LOG + 1 > 0 ? Q - R_LOG + 1 : 0);
             constexpr int LO = AN > QLO ? AN : QLO;
