@@ -403,6 +403,62 @@ def bench_gather(args):
     ctx.close()
 
 
+def bench_order_by(args):
+    """SortOperator path (sort_operator.cpp:44-123): OP_SORT of (INTEGER ASC,
+    DOUBLE DESC) over 12M rows padded to 2^24 with int64 row indices, then
+    OP_PERMUTE of two BIGINT payload columns by those indices."""
+    import numpy as np
+    import torch
+
+    import velodb_amd
+    from velodb_amd import _abi
+    n, n_pad = 12_000_000, 1 << 24
+    g = torch.Generator(device="cuda:0").manual_seed(9)
+    a0 = torch.randint(-1000, 1000, (n_pad,), device="cuda:0", generator=g, dtype=torch.int32)
+    b0 = torch.randn(n_pad, device="cuda:0", generator=g, dtype=torch.float64)
+    p1 = torch.arange(n_pad, device="cuda:0", dtype=torch.int64)
+    p2 = p1 * 3
+    a, b = a0.clone(), b0.clone()
+    idx = torch.empty(n_pad, dtype=torch.int64, device="cuda:0")
+    o1, o2 = torch.empty_like(p1), torch.empty_like(p2)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+
+    def step():
+        a.copy_(a0)
+        b.copy_(b0)
+        ctx.sort([a, b], [_abi.T_INTEGER, _abi.T_DOUBLE], [True, False], idx, n, n_pad)
+        ctx.permute(o1, p1, idx, n, _abi.T_BIGINT)
+        ctx.permute(o2, p2, idx, n, _abi.T_BIGINT)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.kernel_stats_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # checked against torch: the first column ascending, the second descending
+    # within equal first keys, and the permuted payload follows the indices
+    i = idx[:n]
+    ka, kb = a0[i], b0[i]
+    ok = bool(torch.all(ka[1:] >= ka[:-1]))
+    same = ka[1:] == ka[:-1]
+    ok = ok and bool(torch.all(~same | (kb[1:] <= kb[:-1])))
+    ok = ok and bool(torch.all(a[:n] == ka)) and bool(torch.all(o1[:n] == i)) and bool(torch.all(o2[:n] == i * 3))
+    fams = ["sort_pack", "sort_unpack", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2",
+            "sort_global_s3", "sort_global_s4", "sort_global_s5", "sort_hybrid_s6", "sort_hybrid_s7", "permute"]
+    stats = _family_stats(ctx, fams)
+    _line("ORDER BY two keys + permute of two columns (Mrows/s)", n / dt / 1e6, "Mrows/s", args.steps,
+          args.warmup, dt * 1e3, "int32/f64 keys, int64 indices",
+          {"workload": "OP_SORT of (INTEGER ASC, DOUBLE DESC) over 12M rows padded to 2^24 with int64 row "
+                       "indices, then OP_PERMUTE of two BIGINT columns (the SortOperator path)", "n": n,
+           "n_pad": n_pad, "order_equals_torch_check": ok},
+          kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()})
+    ctx.close()
+
+
 def bench_batch(args):
     """4096 independent sorts, sizes log-uniform in [2^10, 2^20]
     (numpy default_rng(0)), each padded to its power of two; one batched call
@@ -585,7 +641,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", type=int, default=0)
     ap.add_argument("--workload", choices=["sort", "filter", "join", "join_pairs", "batch", "serve", "queue",
-                                           "transfer", "gather"],
+                                           "transfer", "gather", "order_by"],
                     default="sort",
                     help="sort = the headline line; the others print their own line")
     ap.add_argument("--autotune", action="store_true", help="batch: run the autotuner first")
@@ -606,7 +662,7 @@ def main():
     if args.workload != "sort":
         {"filter": bench_filter, "join": bench_join, "join_pairs": bench_join_pairs, "batch": bench_batch,
          "serve": bench_serve, "queue": bench_queue, "transfer": bench_transfer,
-         "gather": bench_gather}[args.workload](args)
+         "gather": bench_gather, "order_by": bench_order_by}[args.workload](args)
         return
 
     import torch

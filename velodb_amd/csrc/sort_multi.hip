@@ -10,9 +10,9 @@ using namespace sortimpl;
 
 namespace {
 
-using SortM1 = SortImpl<RecM1, 12, 4, 256, 4>;
-using SortM2 = SortImpl<RecM2, 12, 4, 256, 3>;
-using SortM4 = SortImpl<RecM4, 11, 3, 256, 2>;
+using SortM1 = SortImpl<RecM1, 12, 4, 256, 5, 7>;
+using SortM2 = SortImpl<RecM2, 12, 4, 256, 4, 7>;
+using SortM4 = SortImpl<RecM4, 11, 3, 256, 3, 6>;
 using SortM8 = SortImpl<RecM8, 10, 2, 256, 2>;
 
 // ------------------------------------------------- multi-column (OP_SORT)
