@@ -23,6 +23,32 @@ METRIC = "oblivious sort Mrec/s at 2^28 (%HBM peak); served QPS + p50/p99 vs pla
 REC_BYTES = 12  # u64 key + u32 payload, structure of arrays
 
 
+def cpu_baseline_parallel(budget_s=10.0):
+    """The multi-threaded restatement of obliviator's recursive parallel
+    bitonic (obliviator/join/enclave/bitonic.c:72-242) on the host cores the
+    box gives this GPU (at most 16 threads), 2^23-record samples."""
+    import numpy as np
+
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n = 1 << 23
+    k, p = oracle.gen_sort_input(1, n)
+    oracle.bitonic_sort_kp_mt(k, p, n, n, threads)  # warm-up
+    times = []
+    t_end = time.time() + budget_s
+    while time.time() < t_end or len(times) < 3:
+        t0 = time.perf_counter()
+        oracle.bitonic_sort_kp_mt(k, p, n, n, threads)
+        times.append(time.perf_counter() - t0)
+        if len(times) >= 20:
+            break
+    rate = n / float(np.median(times)) / 1e6
+    return {"value": round(rate, 3), "unit": "Mrec/s", "cores": threads, "kind": "port",
+            "sample": f"oracle multi-threaded recursive bitonic (obliviator bitonic.c structure) on 2^23 records, "
+                      f"{threads} threads, median of {len(times)} runs; at 2^28 the per-record work is "
+                      f"{406 / 276:.2f}x that of 2^23"}
+
+
 def cpu_baseline(budget_s=12.0):
     """Oracle bitonic network (reference host path semantics, one core) on
     2^20-record samples of the same workload, repeated for ~budget_s."""
@@ -158,8 +184,31 @@ def bench_filter(args):
                        f"exact int128 SUM(price*disc)", "n": n, "n_pad": n_pad,
            "selected": int(o[0]), "sum": str(int(o[2]) | (int(o[3]) << 64))},
           roofline=_roofline(stats, bpr, n_pad),
-          kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()})
+          kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()},
+          cpu_baseline=None if args.no_cpu_baseline else cpu_filter_baseline())
     ctx.close()
+
+
+def cpu_filter_baseline(budget_s=8.0):
+    """The oracle's filter + ORCompact + int128 SUM restatement on one core,
+    on 2^20-row samples of the same column distributions."""
+    import numpy as np
+
+    import oracle
+    n, npad = 786_432, 1 << 20
+    cols = oracle.gen_fact(1, n)
+    oracle.filter_sum(*cols, n, npad)
+    times = []
+    t_end = time.time() + budget_s
+    while time.time() < t_end or len(times) < 3:
+        t0 = time.perf_counter()
+        oracle.filter_sum(*cols, n, npad)
+        times.append(time.perf_counter() - t0)
+        if len(times) >= 20:
+            break
+    return {"value": round(n / float(np.median(times)) / 1e6, 3), "unit": "Mrows/s", "cores": 1, "kind": "port",
+            "sample": f"oracle predicate + ORCompact (obliviator oblivious_compact.c restated) + int128 SUM, "
+                      f"{n} rows padded to 2^20, one thread, median of {len(times)} runs"}
 
 
 def ilog2(v):
@@ -233,8 +282,34 @@ def bench_join(args):
            "matches_numpy_groupby": bool(ok)},
           sort_groupby_ms=round(dt_sort * 1e3, 3), sort_groupby_equal=bool(ok_sort),
           roofline=_roofline(stats, bpr, n_pad),
-          kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()})
+          kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()},
+          cpu_baseline=None if args.no_cpu_baseline else cpu_join_baseline())
     ctx.close()
+
+
+def cpu_join_baseline(budget_s=8.0):
+    """The oracle's join + GROUP BY restatement (union bitonic sort, fill,
+    ORCompact, group sort, segmented sum, ORCompact) on one core, on 1/8 of
+    the join workload (187,500 x 750,000 rows, union padded to 2^20)."""
+    import numpy as np
+
+    import oracle
+    nd, nf, npad = 187_500, 750_000, 1 << 20
+    dk, dg, fk, fv = oracle.gen_join(1, nd, nf)
+    oracle.join_groupsum(dk, dg, fk, fv, npad)
+    times = []
+    t_end = time.time() + budget_s
+    while time.time() < t_end or len(times) < 3:
+        t0 = time.perf_counter()
+        oracle.join_groupsum(dk, dg, fk, fv, npad)
+        times.append(time.perf_counter() - t0)
+        if len(times) >= 20:
+            break
+    return {"value": round((nd + nf) / float(np.median(times)) / 1e6, 3), "unit": "Mrows/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle PK/FK join + sort-based GROUP BY (obliviator scalable_oblivious_join.c and "
+                      f"tpch_q3 group-by restated), {nd} x {nf} rows padded to 2^20, one thread, median of "
+                      f"{len(times)} runs"}
 
 
 def bench_join_pairs(args):
@@ -744,6 +819,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline_parallel"] = cpu_baseline_parallel()
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -44,6 +44,8 @@ def lib():
         L.orc_gen_sort_input.argtypes = [u64, sz, P, P]
         L.orc_bitonic_sort_kp.restype = None
         L.orc_bitonic_sort_kp.argtypes = [P, P, sz, sz, P]
+        L.orc_bitonic_sort_kp_mt.restype = None
+        L.orc_bitonic_sort_kp_mt.argtypes = [P, P, sz, sz, C.c_int]
         L.orc_bitonic_sort_kv.restype = None
         L.orc_bitonic_sort_kv.argtypes = [P, P, sz]
         L.orc_bitonic_sort_multi.restype = C.c_int
@@ -108,6 +110,17 @@ def bitonic_sort_kp(keys, payload, n, npad, trace=False):
     t = Trace() if trace else None
     lib().orc_bitonic_sort_kp(_p(k), _p(p), n, npad, C.byref(t) if trace else None)
     return (k, p, t) if trace else (k, p)
+
+
+def bitonic_sort_kp_mt(keys, payload, n, npad, threads):
+    """Multi-threaded recursive bitonic (the CPU baseline); same output as
+    bitonic_sort_kp."""
+    k = np.empty(npad, np.uint64)
+    p = np.empty(npad, np.uint32)
+    k[:n] = keys[:n]
+    p[:n] = payload[:n]
+    lib().orc_bitonic_sort_kp_mt(_p(k), _p(p), n, npad, threads)
+    return k, p
 
 
 def bitonic_sort_kv(keys, vals):

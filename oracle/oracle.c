@@ -6,6 +6,7 @@
 #include "oracle.h"
 
 #include <limits.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -834,4 +835,114 @@ void orc_gather_recover(const uint64_t *table, size_t n_table, const int64_t *id
     free(t);
     free(sv);
     free(sf);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multi-threaded bitonic sort (CPU baseline)                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    uint64_t *k;
+    uint32_t *p;
+    size_t lo, hi, c;
+    int asc, threads;
+} mt_args;
+
+static inline void mt_cmpx(uint64_t *k, uint32_t *p, size_t i, size_t j, int asc)
+{
+    uint64_t ka = k[i], kb = k[j];
+    uint32_t pa = p[i], pb = p[j];
+    int gt = (ka > kb) | ((ka == kb) & (pa > pb));
+    int sw = asc ? gt : !gt;
+    uint64_t mk = (uint64_t)0 - (uint64_t)sw;
+    uint32_t mp = (uint32_t)0 - (uint32_t)sw;
+    uint64_t xk = (ka ^ kb) & mk;
+    uint32_t xp = (pa ^ pb) & mp;
+    k[i] = ka ^ xk;
+    k[j] = kb ^ xk;
+    p[i] = pa ^ xp;
+    p[j] = pb ^ xp;
+}
+
+/* bitonic_merge_2 (bitonic.c:57-68): compares i, i + c for i in [lo, hi) */
+static void *mt_pass(void *v)
+{
+    mt_args *a = (mt_args *)v;
+    for (size_t i = a->lo; i < a->hi; ++i) mt_cmpx(a->k, a->p, i, i + a->c, a->asc);
+    return NULL;
+}
+
+static void *mt_merge(void *v);
+
+/* bitonic_merge (bitonic.c:70-155) */
+static void *mt_merge(void *v)
+{
+    mt_args *a = (mt_args *)v;
+    if (a->hi <= a->lo + 1) return NULL;
+    size_t mid = prev_pow_two(a->hi - a->lo);
+    size_t len = a->hi - mid - a->lo;
+    if (a->threads <= 1) {
+        for (size_t i = a->lo; i < a->hi - mid; ++i) mt_cmpx(a->k, a->p, i, i + mid, a->asc);
+    } else {
+        int t = a->threads;
+        pthread_t th[64];
+        mt_args sub[64];
+        size_t start = a->lo;
+        for (int i = 0; i < t; ++i) {
+            size_t l = len / t + ((size_t)i < len % t);
+            sub[i] = (mt_args){a->k, a->p, start, start + l, mid, a->asc, 1};
+            start += l;
+            if (i < t - 1) pthread_create(&th[i], NULL, mt_pass, &sub[i]);
+        }
+        mt_pass(&sub[t - 1]);
+        for (int i = 0; i < t - 1; ++i) pthread_join(th[i], NULL);
+    }
+    int tl = a->threads / 2, tr = a->threads - tl;
+    mt_args left = {a->k, a->p, a->lo, a->lo + mid, 0, a->asc, a->threads > 1 ? tl : 1};
+    mt_args right = {a->k, a->p, a->lo + mid, a->hi, 0, a->asc, a->threads > 1 ? tr : 1};
+    if (a->threads > 1) {
+        pthread_t th;
+        pthread_create(&th, NULL, mt_merge, &left);
+        mt_merge(&right);
+        pthread_join(th, NULL);
+    } else {
+        mt_merge(&left);
+        mt_merge(&right);
+    }
+    return NULL;
+}
+
+/* bitonic_sort_new (bitonic.c:157-230) */
+static void *mt_sort(void *v)
+{
+    mt_args *a = (mt_args *)v;
+    size_t mid = a->lo + (a->hi - a->lo) / 2;
+    if (mid == a->lo) return NULL;
+    int tl = a->threads / 2, tr = a->threads - tl;
+    mt_args left = {a->k, a->p, a->lo, mid, 0, !a->asc, a->threads > 1 ? tl : 1};
+    mt_args right = {a->k, a->p, mid, a->hi, 0, a->asc, a->threads > 1 ? tr : 1};
+    if (a->threads > 1) {
+        pthread_t th;
+        pthread_create(&th, NULL, mt_sort, &left);
+        mt_sort(&right);
+        pthread_join(th, NULL);
+    } else {
+        mt_sort(&left);
+        mt_sort(&right);
+    }
+    mt_args m = {a->k, a->p, a->lo, a->hi, 0, a->asc, a->threads};
+    mt_merge(&m);
+    return NULL;
+}
+
+void orc_bitonic_sort_kp_mt(uint64_t *keys, uint32_t *payload, size_t n, size_t npad, int nthreads)
+{
+    for (size_t i = n; i < npad; ++i) {
+        keys[i] = UINT64_MAX;
+        payload[i] = (uint32_t)i;
+    }
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    mt_args a = {keys, payload, 0, npad, 0, 1, nthreads};
+    mt_sort(&a);
 }

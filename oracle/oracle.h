@@ -98,6 +98,16 @@ int orc_bitonic_sort_multi(void *const *cols, const uint32_t *types, const uint8
 /* OP_PERMUTE (include/cuda/permute.cuh:10-22): out[i] = in[idx[i]], i < n. */
 void orc_permute(void *out, const void *in, const int64_t *idx, size_t n, size_t elem_size);
 
+/* Multi-threaded bitonic sort of (key, payload) records, restating the
+ * recursive parallel bitonic of obliviator/join/enclave/bitonic.c:72-242
+ * (bitonic_sort_new: halves sorted in opposite directions on half the
+ * threads each, then bitonic_merge: the compare pass at distance
+ * prev_pow_two(len) split over the threads, then both halves merged in
+ * parallel).  Rows [n, npad) become pads (key max, payload = row) first.
+ * Pinned: equal to orc_bitonic_sort_kp (the same unique sorted order) for
+ * every thread count.  Used as the multi-core CPU baseline. */
+void orc_bitonic_sort_kp_mt(uint64_t *keys, uint32_t *payload, size_t n, size_t npad, int nthreads);
+
 /* ------------------------------------------------------------------------ */
 /* Oblivious compaction (ORCompact)                                          */
 /* ------------------------------------------------------------------------ */

@@ -428,3 +428,14 @@ def test_gather_recover_equals_take(n_table, n):
     want[ok] = table[ids[ok]]
     assert np.array_equal(found.astype(bool), ok)
     assert np.array_equal(out, want)
+
+
+@pytest.mark.parametrize("threads", [1, 2, 3, 8, 16])
+def test_parallel_bitonic_equals_single_thread(threads):
+    """The multi-threaded recursive bitonic (the multi-core CPU baseline)
+    gives the unique sorted order of the single-thread network."""
+    for n, npad in [(1000, 1024), (3000, 4096), (1 << 15, 1 << 15)]:
+        k, p = oracle.gen_sort_input(5, n)
+        ek, ep = oracle.bitonic_sort_kp(k, p, n, npad)
+        gk, gp = oracle.bitonic_sort_kp_mt(k, p, n, npad, threads)
+        assert np.array_equal(gk, ek) and np.array_equal(gp, ep)
