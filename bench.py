@@ -74,11 +74,12 @@ def cpu_baseline(budget_s=12.0):
                       f"log2N(log2N+1)/2, so at 2^28 the same code would run at ~{rate * 210 / 406:.2f} Mrec/s"}
 
 
-def measured_traffic(family, n):
-    """HBM bytes per launch of `family` from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", "sort_2p28_traffic.md")
-    if n != 1 << 28 or not os.path.exists(path):
+def measured_traffic(family, n, profile="sort_2p28_traffic.md", n_profiled=1 << 28):
+    """HBM bytes per launch of `family` from a committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated) taken at
+    n_profiled rows, or None at other sizes."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if n != n_profiled or not os.path.exists(path):
         return None
     for line in open(path):
         parts = line.split()
@@ -120,16 +121,18 @@ def _family_stats(ctx, fams):
     return out
 
 
-def _roofline(stats, bytes_per_row, n_pad):
+def _roofline(stats, bytes_per_row, n_pad, traffic=None):
     """Roofline of the family with the largest total time; bytes_per_row maps
-    a family to its algorithmic bytes per padded row and launch."""
+    a family to its algorithmic bytes per padded row and launch; traffic =
+    (profile file, rows it was taken at) of the measured HBM bytes."""
     cand = {f: v for f, v in stats.items() if f in bytes_per_row}
     dom = max(cand, key=lambda f: cand[f][1])
     cnt, ms = cand[dom]
     alg = bytes_per_row[dom] * n_pad
     ach = alg / (ms / cnt * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(dom, n_pad, *traffic) if traffic else None, "kernel": dom,
             "avg_launch_ms": round(ms / cnt, 4), "alg_bytes_per_launch": alg}
 
 
@@ -183,7 +186,7 @@ def bench_filter(args):
           {"workload": f"Q6-style filter over {n} rows padded to 2^{ilog2(n_pad)}, ORCompact of (price, disc), "
                        f"exact int128 SUM(price*disc)", "n": n, "n_pad": n_pad,
            "selected": int(o[0]), "sum": str(int(o[2]) | (int(o[3]) << 64))},
-          roofline=_roofline(stats, bpr, n_pad),
+          roofline=_roofline(stats, bpr, n_pad, traffic=("filter_2p23_traffic.md", 1 << 23)),
           kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()},
           cpu_baseline=None if args.no_cpu_baseline else cpu_filter_baseline())
     ctx.close()
@@ -281,7 +284,7 @@ def bench_join(args):
            "groups": ng, "group_by": "dense oblivious aggregation over the 25-id group domain",
            "matches_numpy_groupby": bool(ok)},
           sort_groupby_ms=round(dt_sort * 1e3, 3), sort_groupby_equal=bool(ok_sort),
-          roofline=_roofline(stats, bpr, n_pad),
+          roofline=_roofline(stats, bpr, n_pad, traffic=("join_2p23_traffic.md", 1 << 23)),
           kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()},
           cpu_baseline=None if args.no_cpu_baseline else cpu_join_baseline())
     ctx.close()
@@ -534,6 +537,66 @@ def bench_order_by(args):
     ctx.close()
 
 
+def bench_shuffle(args):
+    """Oblivious shuffle of a position map (ObliviousTable::generatePermutation
+    + compose, src/data/oblivious_table.cpp:240-255): Philox4x32-10 tag per
+    row, bitonic sort of (tag, row<<32 | payload), unpack of the payload and
+    the permutation.  2^28 rows by default."""
+    import torch
+
+    import velodb_amd
+    from velodb_amd import _abi
+    n_pad = 1 << args.log2n
+    dev = torch.device("cuda:0")
+    p0 = torch.randint(0, 2 ** 31 - 1, (n_pad,), device=dev, dtype=torch.int32)
+    pay = torch.empty_like(p0)
+    perm = torch.empty(n_pad, dtype=torch.int32, device=dev)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+
+    def step(i):
+        pay.copy_(p0)
+        ctx.shuffle(pay, n_pad, 77 + i, perm_out=perm)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    ctx.kernel_stats_reset()
+    # the copy that restores the payload is outside the timed path; only the
+    # library's launches are on its stream in between the events below
+    t_ms = 0.0
+    for i in range(args.steps):
+        pay.copy_(p0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ctx.shuffle(pay, n_pad, 77 + args.warmup + i, perm_out=perm)
+        e1.record()
+        torch.cuda.synchronize()
+        t_ms += e0.elapsed_time(e1)
+    dt = t_ms / 1e3 / args.steps
+    # the map is a permutation and the payload followed its row
+    pl = perm.long()
+    ok = bool(torch.equal(torch.sort(pl).values, torch.arange(n_pad, device=dev)))
+    ok = ok and bool(torch.equal(pay, p0[pl]))
+    moved = float((pl != torch.arange(n_pad, device=dev)).float().mean())
+    del pl
+    fams = ["shuffle_tags", "shuffle_unpack", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2",
+            "sort_global_s3", "sort_global_s4", "sort_global_s5", "sort_hybrid_s6", "sort_hybrid_s7",
+            "sort_hybrid_s8"]
+    stats = _family_stats(ctx, fams)
+    # per padded row and launch: a sort pass reads and writes the 16-byte
+    # (tag, row|payload) record; tags read the payload and write the record;
+    # unpack reads the record and writes payload and map
+    bpr = {f: 32 for f in fams if f.startswith("sort_")}
+    bpr.update({"shuffle_tags": 4 + 16, "shuffle_unpack": 8 + 4 + 4})
+    _line("oblivious shuffle of a position map (Mrows/s)", n_pad / dt / 1e6, "Mrows/s", args.steps, args.warmup,
+          dt * 1e3, "u64 tags, u32 payload",
+          {"workload": f"Philox4x32-10 tag shuffle of 2^{args.log2n} rows (u32 payload + u32 position map)",
+           "n_pad": n_pad, "permutation_check": ok, "fraction_moved": round(moved, 6)},
+          roofline=_roofline(stats, bpr, n_pad, traffic=("shuffle_2p28_traffic.md", 1 << 28)),
+          kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()})
+    ctx.close()
+
+
 def bench_batch(args):
     """4096 independent sorts, sizes log-uniform in [2^10, 2^20]
     (numpy default_rng(0)), each padded to its power of two; one batched call
@@ -716,7 +779,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", type=int, default=0)
     ap.add_argument("--workload", choices=["sort", "filter", "join", "join_pairs", "batch", "serve", "queue",
-                                           "transfer", "gather", "order_by"],
+                                           "transfer", "gather", "order_by", "shuffle"],
                     default="sort",
                     help="sort = the headline line; the others print their own line")
     ap.add_argument("--autotune", action="store_true", help="batch: run the autotuner first")
@@ -737,7 +800,7 @@ def main():
     if args.workload != "sort":
         {"filter": bench_filter, "join": bench_join, "join_pairs": bench_join_pairs, "batch": bench_batch,
          "serve": bench_serve, "queue": bench_queue, "transfer": bench_transfer,
-         "gather": bench_gather, "order_by": bench_order_by}[args.workload](args)
+         "gather": bench_gather, "order_by": bench_order_by, "shuffle": bench_shuffle}[args.workload](args)
         return
 
     import torch
