@@ -123,6 +123,14 @@ def bitonic_sort_kp_mt(keys, payload, n, npad, threads):
     return k, p
 
 
+def bitonic_sort_kp_mt_inplace(k, p, n, threads):
+    """bitonic_sort_kp_mt on caller arrays (uint64 keys, uint32 payload, both
+    of the padded length), sorted in place with no copy."""
+    assert k.dtype == np.uint64 and p.dtype == np.uint32 and len(k) == len(p)
+    assert k.flags.c_contiguous and p.flags.c_contiguous and n <= len(k)
+    lib().orc_bitonic_sort_kp_mt(_p(k), _p(p), n, len(k), threads)
+
+
 def bitonic_sort_kv(keys, vals):
     k = np.ascontiguousarray(keys, np.uint64).copy()
     v = np.ascontiguousarray(vals, np.uint64).copy()

@@ -439,3 +439,7 @@ def test_parallel_bitonic_equals_single_thread(threads):
         ek, ep = oracle.bitonic_sort_kp(k, p, n, npad)
         gk, gp = oracle.bitonic_sort_kp_mt(k, p, n, npad, threads)
         assert np.array_equal(gk, ek) and np.array_equal(gp, ep)
+        if n == npad:  # the in-place form the bench times
+            wk, wp = k.copy(), p.copy()
+            oracle.bitonic_sort_kp_mt_inplace(wk, wp, n, threads)
+            assert np.array_equal(wk, ek) and np.array_equal(wp, ep)
