@@ -20,7 +20,13 @@
  *   ObliviousTable          include/data/oblivious_table.hpp, src/data/oblivious_table.cpp:16-255
  *   ObliviousTableManager   include/data/oblivious_table_manager.hpp,
  *                           src/data/oblivious_table_manager.cpp:14-221
- * and adds PKFKJoinGroupSumOperator for the oblivious join + GROUP BY.
+ *   AbstractPlanNode, SortPlanNode, FilterCompactionPlanNode, SortMergeJoinPlanNode,
+ *   MaterializationPlanNode  include/planner/{sort,filter_compaction,sort_merge_join,
+ *                           materialization}_plan_node.hpp and their .cpp files, and the
+ *                           planner's wiring of them (src/planner/query_planner.cpp:247-252,
+ *                           443-476, 518-544)
+ * and adds PKFKJoinGroupSumOperator (and its plan node) for the oblivious
+ * join + GROUP BY.
  *
  * Columns live in device memory with a power-of-two capacity (at least 4096
  * rows), as the reference's device ValueVector does (value_vector.hpp:43,
@@ -48,6 +54,7 @@
 #include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -810,6 +817,227 @@ private:
     bool dense_;
     bool done_ = false;
 };
+
+// ------------------------------------------------------------- plan nodes
+
+// Plan nodes of this path: each builds its operator, and its children's,
+// from createOperator(context), as the reference's planner nodes do
+// (include/planner/abstract_plan_node.hpp:19-59).  The PlanType values are
+// the reference's (:19-33), so code that switches on them keeps working.
+// Without a Schema, a node names columns by index; toString() prints "#i".
+// createOperator may be called more than once (the nodes keep their
+// arguments and copy them into each operator).
+enum class PlanType : uint16_t {
+    INVALID = 0,
+    SEQ_SCAN = 0b0000'0001,
+    COMPACTION = 0b0000'0010,
+    JOIN = 0b0000'1000,
+    SORT_MERGE_JOIN = 0b0011'1000,
+    MATERIALIZATION = 0b0100'0000,
+    SORT = 0b1000'0000,
+    AGGREGATE = 0b10'1000'0000,
+};
+
+class AbstractPlanNode {
+public:
+    explicit AbstractPlanNode(PlanType type) : type_(type) {}
+    virtual ~AbstractPlanNode() = default;
+    AbstractPlanNode(const AbstractPlanNode &) = delete;
+    AbstractPlanNode &operator=(const AbstractPlanNode &) = delete;
+
+    PlanType getPlanType() const { return type_; }
+    const std::vector<std::unique_ptr<AbstractPlanNode>> &getChildren() const { return children_; }
+    void addChild(std::unique_ptr<AbstractPlanNode> child) { children_.push_back(std::move(child)); }
+
+    virtual std::unique_ptr<AbstractOperator> createOperator(ExecutionContext &context) const = 0;
+    virtual std::string toString() const = 0;
+
+protected:
+    // children_[i]->createOperator(context), after checking the child count
+    // (the reference asserts it, e.g. sort_plan_node.cpp:19)
+    std::unique_ptr<AbstractOperator> child_op(ExecutionContext &context, size_t i, size_t expect,
+                                               const char *node) const
+    {
+        if (children_.size() != expect)
+            throw ExecutionError(std::string(node) + " must have exactly " + std::to_string(expect) +
+                                 (expect == 1 ? " child" : " children"));
+        return children_[i]->createOperator(context);
+    }
+
+    PlanType type_;
+    std::vector<std::unique_ptr<AbstractPlanNode>> children_;
+};
+
+// Leaf that produces one batch per operator it creates; `scan` stands in for
+// the sequential scan of device columns (seq_scan_plan_node.cpp).
+class ScanPlanNode : public AbstractPlanNode {
+public:
+    explicit ScanPlanNode(std::function<RowBatch()> scan, std::string name = "batch")
+        : AbstractPlanNode(PlanType::SEQ_SCAN), scan_(std::move(scan)), name_(std::move(name))
+    {
+    }
+    std::unique_ptr<AbstractOperator> createOperator(ExecutionContext &context) const override
+    {
+        return std::make_unique<BatchSourceOperator>(context, scan_());
+    }
+    std::string toString() const override { return "Scan(" + name_ + ")"; }
+
+private:
+    std::function<RowBatch()> scan_;
+    std::string name_;
+};
+
+// SortPlanNode (include/planner/sort_plan_node.hpp, src/planner/sort_plan_node.cpp:17-27)
+class SortPlanNode : public AbstractPlanNode {
+public:
+    SortPlanNode(std::vector<size_t> order_indices, std::vector<bool> ascending_flags)
+        : AbstractPlanNode(PlanType::SORT), order_(std::move(order_indices)), asc_(std::move(ascending_flags))
+    {
+    }
+    std::unique_ptr<AbstractOperator> createOperator(ExecutionContext &context) const override
+    {
+        return std::make_unique<SortOperator>(context, child_op(context, 0, 1, "SortPlanNode"), order_, asc_);
+    }
+    std::string toString() const override
+    {
+        std::string s = "Sort(";
+        for (size_t i = 0; i < order_.size(); ++i)
+            s += (i ? ", #" : "#") + std::to_string(order_[i]) + (i < asc_.size() && !asc_[i] ? " DESC" : " ASC");
+        return s + ")";
+    }
+
+private:
+    std::vector<size_t> order_;
+    std::vector<bool> asc_;
+};
+
+// FilterCompactionPlanNode (include/planner/filter_compaction_plan_node.hpp:14-27,
+// src/planner/filter_compaction_plan_node.cpp): compacts the rows whose mask
+// byte (column mask_index) is set; setCompactColumns chooses the columns.
+class FilterCompactionPlanNode : public AbstractPlanNode {
+public:
+    explicit FilterCompactionPlanNode(size_t mask_index) : AbstractPlanNode(PlanType::COMPACTION), mask_(mask_index) {}
+    void setCompactColumns(std::vector<bool> compact_columns) { compact_ = std::move(compact_columns); }
+    const std::vector<bool> &getCompactColumns() const { return compact_; }
+    std::unique_ptr<AbstractOperator> createOperator(ExecutionContext &context) const override
+    {
+        return std::make_unique<FilterCompactionOperator>(
+            context, child_op(context, 0, 1, "FilterCompactionPlanNode"), mask_, compact_);
+    }
+    std::string toString() const override { return "FilterCompaction(mask #" + std::to_string(mask_) + ")"; }
+
+private:
+    size_t mask_;
+    std::vector<bool> compact_;
+};
+
+// SortMergeJoinPlanNode (include/planner/sort_merge_join_plan_node.hpp:13-35,
+// src/planner/sort_merge_join_plan_node.cpp): children are (left, right).
+class SortMergeJoinPlanNode : public AbstractPlanNode {
+public:
+    SortMergeJoinPlanNode(std::pair<size_t, size_t> join_key_indices, JoinType join_type = JoinType::INNER,
+                          bool oblivious_materialization = false)
+        : AbstractPlanNode(PlanType::SORT_MERGE_JOIN), keys_(join_key_indices), type_(join_type),
+          oblivious_(oblivious_materialization)
+    {
+    }
+    JoinType getJoinType() const { return type_; }
+    std::unique_ptr<AbstractOperator> createOperator(ExecutionContext &context) const override
+    {
+        auto left = child_op(context, 0, 2, "SortMergeJoinPlanNode");
+        auto right = child_op(context, 1, 2, "SortMergeJoinPlanNode");
+        return std::make_unique<SortMergeJoinOperator>(context, std::move(left), std::move(right), keys_, type_,
+                                                       oblivious_);
+    }
+    std::string toString() const override
+    {
+        return "SortMergeJoin(#" + std::to_string(keys_.first) + " = #" + std::to_string(keys_.second) + ")";
+    }
+
+private:
+    std::pair<size_t, size_t> keys_;
+    JoinType type_;
+    bool oblivious_;
+};
+
+// MaterializationPlanNode (include/planner/materialization_plan_node.hpp,
+// src/planner/materialization_plan_node.cpp)
+class MaterializationPlanNode : public AbstractPlanNode {
+public:
+    explicit MaterializationPlanNode(std::vector<MaterializedColumn> columns)
+        : AbstractPlanNode(PlanType::MATERIALIZATION), cols_(std::move(columns))
+    {
+    }
+    std::unique_ptr<AbstractOperator> createOperator(ExecutionContext &context) const override
+    {
+        return std::make_unique<MaterializationOperator>(context, child_op(context, 0, 1, "MaterializationPlanNode"),
+                                                         cols_);
+    }
+    std::string toString() const override { return "Materialization(" + std::to_string(cols_.size()) + " columns)"; }
+
+private:
+    std::vector<MaterializedColumn> cols_;
+};
+
+// PK/FK join + SUM GROUP BY; children are (dimension, fact).
+class PKFKJoinGroupSumPlanNode : public AbstractPlanNode {
+public:
+    PKFKJoinGroupSumPlanNode(size_t dim_key, size_t dim_group, size_t fact_fk, size_t fact_val, size_t max_groups,
+                             bool dense_groups)
+        : AbstractPlanNode(PlanType::AGGREGATE), dk_(dim_key), dg_(dim_group), fk_(fact_fk), fv_(fact_val),
+          max_groups_(max_groups), dense_(dense_groups)
+    {
+    }
+    std::unique_ptr<AbstractOperator> createOperator(ExecutionContext &context) const override
+    {
+        auto dim = child_op(context, 0, 2, "PKFKJoinGroupSumPlanNode");
+        auto fact = child_op(context, 1, 2, "PKFKJoinGroupSumPlanNode");
+        return std::make_unique<PKFKJoinGroupSumOperator>(context, std::move(dim), std::move(fact), dk_, dg_, fk_, fv_,
+                                                          max_groups_, dense_);
+    }
+    std::string toString() const override
+    {
+        return "PKFKJoinGroupSum(#" + std::to_string(dk_) + " = #" + std::to_string(fk_) + ", group #" +
+               std::to_string(dg_) + ", sum #" + std::to_string(fv_) + ")";
+    }
+
+private:
+    size_t dk_, dg_, fk_, fv_, max_groups_;
+    bool dense_;
+};
+
+// The planner's sort-merge join wiring (src/planner/query_planner.cpp:443-476):
+// each input sorted ascending on its key, then the join of the two sorted
+// inputs.
+inline std::unique_ptr<AbstractPlanNode> planSortMergeJoin(std::unique_ptr<AbstractPlanNode> left,
+                                                           std::unique_ptr<AbstractPlanNode> right,
+                                                           std::pair<size_t, size_t> join_key_indices,
+                                                           bool oblivious_materialization = false)
+{
+    auto ls = std::make_unique<SortPlanNode>(std::vector<size_t>{join_key_indices.first}, std::vector<bool>{true});
+    ls->addChild(std::move(left));
+    auto rs = std::make_unique<SortPlanNode>(std::vector<size_t>{join_key_indices.second}, std::vector<bool>{true});
+    rs->addChild(std::move(right));
+    auto j = std::make_unique<SortMergeJoinPlanNode>(join_key_indices, JoinType::INNER, oblivious_materialization);
+    j->addChild(std::move(ls));
+    j->addChild(std::move(rs));
+    return j;
+}
+
+// The planner's single-table and ORDER BY wiring (query_planner.cpp:247-252,
+// 518-544): compaction of the scan by its mask column, then, with order
+// columns given, a sort over the compacted rows.
+inline std::unique_ptr<AbstractPlanNode> planFilterOrderBy(std::unique_ptr<AbstractPlanNode> scan, size_t mask_index,
+                                                           std::vector<size_t> order_indices = {},
+                                                           std::vector<bool> ascending_flags = {})
+{
+    auto f = std::make_unique<FilterCompactionPlanNode>(mask_index);
+    f->addChild(std::move(scan));
+    if (order_indices.empty()) return f;
+    auto s = std::make_unique<SortPlanNode>(std::move(order_indices), std::move(ascending_flags));
+    s->addChild(std::move(f));
+    return s;
+}
 
 // ------------------------------------------------- oblivious position maps
 

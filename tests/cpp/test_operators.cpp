@@ -438,6 +438,133 @@ static void test_sort_merge_join(ExecutionContext &ctx)
     if (!rb) CHECK(rb.error() == "Left join key index out of bounds");
 }
 
+// Plan nodes build the same operators as the direct calls above: the
+// single-table WHERE + ORDER BY wiring, the planner's sort-merge join (sort
+// each side on its key, then join), a materialization and a PK/FK GROUP BY
+// plan; a plan runs again from a second createOperator; a node with the wrong
+// number of children throws, as the reference's assertion does.
+static std::vector<std::vector<int64_t>> rows_of(const RowBatch &b)
+{
+    std::vector<std::vector<int64_t>> out(b.getRowCount());
+    for (size_t c = 0; c < b.getColumnCount(); ++c) {
+        const Column &col = b.getColumn(c);
+        std::vector<int64_t> v(b.getRowCount());
+        if (col.getTypeId() == VO_T_BIGINT) {
+            v = col.toHost<int64_t>(b.getRowCount());
+        } else {
+            const auto w = col.toHost<int32_t>(b.getRowCount());
+            std::copy(w.begin(), w.end(), v.begin());
+        }
+        for (size_t r = 0; r < v.size(); ++r) out[r].push_back(v[r]);
+    }
+    return out;
+}
+
+static RowBatch run_plan(ExecutionContext &ctx, const AbstractPlanNode &plan)
+{
+    auto op = plan.createOperator(ctx);
+    auto r = op->next();
+    if (!r) {
+        std::printf("  plan failed: %s\n", r.error().c_str());
+        ++g_fail;
+        return RowBatch();
+    }
+    return std::move(r).value();
+}
+
+static void test_plan_nodes(ExecutionContext &ctx)
+{
+    static_assert(static_cast<uint16_t>(PlanType::SORT_MERGE_JOIN) == 0x38 &&
+                      static_cast<uint16_t>(PlanType::SORT) == 0x80 &&
+                      static_cast<uint16_t>(PlanType::COMPACTION) == 0x02,
+                  "reference PlanType values");
+    // WHERE department = 'Engineering' ORDER BY salary DESC, as a plan
+    const auto dept_o = ordinals(kDept);
+    auto scan = std::make_unique<ScanPlanNode>(
+        [&] {
+            RowBatch b = employees(all_rows());
+            std::vector<uint8_t> mask;
+            for (size_t r = 0; r < kId.size(); ++r) mask.push_back(dept_o[r] == 0 ? 1 : 0);
+            std::vector<Column> cols;
+            for (size_t i = 0; i < b.getColumnCount(); ++i) cols.push_back(std::move(b.getColumn(i)));
+            cols.push_back(Column::fromHost(VO_T_BOOLEAN, mask.data(), mask.size()));
+            return RowBatch(std::move(cols), kId.size());
+        },
+        "employees");
+    auto q = planFilterOrderBy(std::move(scan), 6, {C_SALARY}, {false});
+    CHECK(q->getPlanType() == PlanType::SORT);
+    CHECK(q->toString() == "Sort(#2 DESC)");
+    CHECK(q->getChildren().size() == 1 && q->getChildren()[0]->toString() == "FilterCompaction(mask #6)");
+    for (int run = 0; run < 2; ++run) { // the same plan twice
+        RowBatch b = run_plan(ctx, *q);
+        CHECK(b.getRowCount() == 3);
+        if (b.getRowCount() == 3) CHECK((b.getColumn(C_ID).toHost<int32_t>(3) == std::vector<int32_t>{5, 7, 3}));
+    }
+
+    // sort-merge join of unsorted inputs through the planner's wiring
+    // (tests/query/test_join_execution.cpp:12-66 data, rows shuffled)
+    const std::vector<int32_t> lid = {3, 0, 4, 1, 2}, rid = {2, 4, 0, 3, 1};
+    std::vector<int32_t> lx, ry;
+    for (int32_t i : lid) lx.push_back(10 * i);
+    for (int32_t i : rid) ry.push_back(100 + i);
+    for (bool obl : {false, true}) {
+        auto j = planSortMergeJoin(std::make_unique<ScanPlanNode>([&] { return id_value(lid, lx); }, "A"),
+                                   std::make_unique<ScanPlanNode>([&] { return id_value(rid, ry); }, "B"), {0, 0},
+                                   obl);
+        CHECK(j->getPlanType() == PlanType::SORT_MERGE_JOIN && j->toString() == "SortMergeJoin(#0 = #0)");
+        CHECK(j->getChildren().size() == 2 && j->getChildren()[0]->toString() == "Sort(#0 ASC)");
+        RowBatch b = run_plan(ctx, *j);
+        std::vector<std::vector<int64_t>> want;
+        for (int64_t i = 0; i < 5; ++i) want.push_back({i, 10 * i, i, 100 + i});
+        CHECK(b.getRowCount() == 5 && rows_of(b) == want);
+    }
+
+    // materialization of row ids through a plan
+    std::vector<int64_t> val = {7, 8, 9, 10};
+    std::vector<Column> sc;
+    sc.push_back(Column::fromHost(VO_T_BIGINT, val.data(), val.size()));
+    RowBatch source(std::move(sc), val.size());
+    std::vector<int64_t> ids = {3, 0, 0, 2};
+    MaterializationPlanNode mp({{0, &source, 0}});
+    mp.addChild(std::make_unique<ScanPlanNode>([&] {
+        std::vector<Column> c;
+        c.push_back(Column::fromHost(VO_T_BIGINT, ids.data(), ids.size()));
+        return RowBatch(std::move(c), ids.size());
+    }));
+    RowBatch mb = run_plan(ctx, mp);
+    CHECK(mb.getRowCount() == 4 && (mb.getColumn(0).toHost<int64_t>(4) == std::vector<int64_t>{10, 7, 7, 9}));
+
+    // PK/FK join + GROUP BY as a plan (same data as test_join_groupsum)
+    const std::vector<int32_t> dk = {1, 2, 3, 10}, dg = {0, 1, 0, 7}, fk = {1, 4, 3, 3, 9, 2, 11, 11};
+    const std::vector<int64_t> fv = {10, 1000, 5, 7, 1000, 3, 99, 99};
+    PKFKJoinGroupSumPlanNode gp(0, 1, 0, 1, 8, true);
+    gp.addChild(std::make_unique<ScanPlanNode>([&] {
+        std::vector<Column> d;
+        d.push_back(Column::fromHost(VO_T_INTEGER, dk.data(), dk.size()));
+        d.push_back(Column::fromHost(VO_T_INTEGER, dg.data(), dg.size()));
+        return RowBatch(std::move(d), dk.size());
+    }));
+    gp.addChild(std::make_unique<ScanPlanNode>([&] {
+        std::vector<Column> f;
+        f.push_back(Column::fromHost(VO_T_INTEGER, fk.data(), fk.size()));
+        f.push_back(Column::fromHost(VO_T_BIGINT, fv.data(), fv.size()));
+        return RowBatch(std::move(f), fk.size());
+    }));
+    RowBatch gb = run_plan(ctx, gp);
+    CHECK(gb.getRowCount() == 2 && (rows_of(gb) == std::vector<std::vector<int64_t>>{{0, 22}, {1, 3}}));
+
+    // a join node with one child
+    SortMergeJoinPlanNode lone({0, 0});
+    lone.addChild(std::make_unique<ScanPlanNode>([&] { return id_value({1}, {1}); }));
+    bool threw = false;
+    try {
+        (void)lone.createOperator(ctx);
+    } catch (const ExecutionError &e) {
+        threw = std::string(e.what()) == "SortMergeJoinPlanNode must have exactly 2 children";
+    }
+    CHECK(threw);
+}
+
 // tests/data/test_oblivious_table.cpp: a 1000-row table (id INTEGER, value BIGINT)
 static RowBatch thousand_rows()
 {
@@ -742,6 +869,7 @@ int main()
         {"join_groupsum_dense", [&] { test_join_groupsum(ctx, true); }},
         {"errors_are_results", [&] { test_errors_are_results(ctx); }},
         {"task_manager_queue", [&] { test_task_manager_queue(ctx); }},
+        {"plan_nodes", [&] { test_plan_nodes(ctx); }},
     };
     for (const auto &t : tests) {
         const int before = g_fail;
