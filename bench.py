@@ -180,6 +180,22 @@ def bench_filter(args):
     o = out.cpu().numpy().view(np.uint64)
     stats = _family_stats(ctx, ["filter_count", "compact_scan", "compact_local", "compact_hybrid", "compact_global",
                                 "filter_sum"])
+    # the same WHERE clause as five generic device terms (OP_PREDICATE_MASK):
+    # reads day, disc, qty once each (20 B/row) and writes the mask (1 B/row)
+    mask = torch.empty(n_pad, dtype=torch.uint8, device=dev)
+    terms = [(day, _abi.T_INTEGER, _abi.CMP_GE, 8766), (day, _abi.T_INTEGER, _abi.CMP_LT, 9131),
+             (disc, _abi.T_BIGINT, _abi.CMP_GE, 5), (disc, _abi.T_BIGINT, _abi.CMP_LE, 7),
+             (qty, _abi.T_BIGINT, _abi.CMP_LT, 24)]
+    ctx.predicate_mask(terms, mask, n, n_pad)
+    torch.cuda.synchronize()
+    ctx.kernel_stats_reset()
+    for _ in range(args.steps):
+        ctx.predicate_mask(terms, mask, n, n_pad)
+    torch.cuda.synchronize()
+    pc, pms = ctx.kernel_stats("predicate_mask")
+    pred_ms = pms / max(pc, 1)
+    pred = {"ms": round(pred_ms, 4), "GBps": round(21 * n_pad / (pred_ms * 1e-3) / 1e9, 1),
+            "selected_equal": int(mask.sum().item()) == int(o[0])}
     # algorithmic bytes per padded row and launch: predicate reads day+qty+disc
     # and writes the mask; the local network reads the mask and both columns
     # and writes the columns; a global or LDS-staged pass reads and writes both
@@ -190,6 +206,7 @@ def bench_filter(args):
           {"workload": f"Q6-style filter over {n} rows padded to 2^{ilog2(n_pad)}, ORCompact of (price, disc), "
                        f"exact int128 SUM(price*disc)", "n": n, "n_pad": n_pad,
            "selected": int(o[0]), "sum": str(int(o[2]) | (int(o[3]) << 64))},
+          generic_predicate_mask=pred,
           roofline=_roofline(stats, bpr, n_pad, traffic=("filter_2p23_traffic.md", 1 << 23)),
           kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()},
           cpu_baseline=None if args.no_cpu_baseline else cpu_filter_baseline())

@@ -447,17 +447,55 @@ private:
     bool sorted_ = false;
 };
 
-// Oblivious filter compaction: the rows whose mask byte (column mask_index,
-// BOOLEAN) is non-zero move to the front, in order, in every column marked in
-// compact_columns (data and null bitmap); the batch's row count becomes the
-// selected count.  Columns not marked keep their rows, as the reference
-// leaves them (filter_compaction_operator.cpp:112-143).
+// Predicate terms evaluated on the device (OP_PREDICATE_MASK): a column
+// compared with a constant in the column's type, as ComparisonExpression
+// does with a scalar (src/expression/comparison_expression.cpp:13-92); the
+// terms of one predicate are joined by one ConnectiveType
+// (src/expression/logical_expression.cpp:61-150).  A column's NULL rows
+// (bitmap byte 0) make the row unselected.  The enums keep the reference's
+// order (include/expression/expression.hpp:34-43, without LIKE).
+enum class ComparisonType { EQUAL, NOT_EQUAL, LESS_THAN, LESS_THAN_OR_EQUAL, GREATER_THAN, GREATER_THAN_OR_EQUAL };
+enum class ConnectiveType { AND, OR };
+
+struct PredicateTerm {
+    size_t column;       // column index in the batch
+    ComparisonType cmp;
+    uint64_t constant;   // the constant's bit pattern in the column's type
+
+    // make(column, cmp, value): value of the column's C type (int32_t for
+    // INTEGER, double for DOUBLE, ...)
+    template <typename T>
+    static PredicateTerm make(size_t column, ComparisonType cmp, T value)
+    {
+        static_assert(sizeof(T) <= 8, "scalar constant");
+        uint64_t bits = 0;
+        std::memcpy(&bits, &value, sizeof(T));
+        return PredicateTerm{column, cmp, bits};
+    }
+};
+
+// Oblivious filter compaction: the selected rows move to the front, in
+// order, in every column marked in compact_columns (data and null bitmap);
+// the batch's row count becomes the selected count.  Columns not marked keep
+// their rows, as the reference leaves them
+// (filter_compaction_operator.cpp:112-143).  The selection is either a mask
+// column (BOOLEAN, non-zero = selected; the scan's "$_mask") or a predicate
+// evaluated on the device, which the reference evaluates on the host
+// (seq_scan_operator.cpp:38-45).
 class FilterCompactionOperator : public AbstractOperator {
 public:
     FilterCompactionOperator(ExecutionContext &context, std::unique_ptr<AbstractOperator> child, size_t mask_index,
                              std::vector<bool> compact_columns = {})
         : AbstractOperator(context), child_(std::move(child)), mask_index_(mask_index),
           compact_(std::move(compact_columns))
+    {
+    }
+
+    FilterCompactionOperator(ExecutionContext &context, std::unique_ptr<AbstractOperator> child,
+                             std::vector<PredicateTerm> predicate, ConnectiveType connective = ConnectiveType::AND,
+                             std::vector<bool> compact_columns = {})
+        : AbstractOperator(context), child_(std::move(child)), mask_index_(kNoMask), pred_(std::move(predicate)),
+          conn_(connective), compact_(std::move(compact_columns))
     {
     }
 
@@ -476,12 +514,57 @@ public:
     }
 
 private:
+    static constexpr size_t kNoMask = ~(size_t)0;
+
+    // the predicate's mask, one byte per padded row, in a new device buffer
+    uint8_t *evaluate(RowBatch &batch, size_t n_pad)
+    {
+        if (pred_.empty() || pred_.size() > VO_MAX_PREDICATE_TERMS)
+            throw ExecutionError("FilterCompactionOperator: 1..8 predicate terms");
+        std::vector<vo_predicate_term> terms;
+        for (const PredicateTerm &t : pred_) {
+            if (t.column >= batch.getColumnCount())
+                throw ExecutionError("FilterCompactionOperator: predicate column out of range");
+            const Column &c = batch.getColumn(t.column);
+            terms.push_back(vo_predicate_term{c.rawData(), c.bitmap(), t.constant, c.getTypeId(),
+                                              static_cast<uint32_t>(t.cmp)});
+        }
+        uint8_t *mask = nullptr;
+        hip_check(hipMalloc(reinterpret_cast<void **>(&mask), n_pad), "hipMalloc");
+        vo_predicate_args a{};
+        a.terms = terms.data();
+        a.n_terms = (uint32_t)terms.size();
+        a.connective = conn_ == ConnectiveType::AND ? VO_CONN_AND : VO_CONN_OR;
+        a.out_mask = mask;
+        a.n = batch.getRowCount();
+        a.n_pad = n_pad;
+        TaskManager &tm = context_.getTaskManager();
+        const int st = vo_predicate_mask(tm.native(), &a);
+        if (st != VO_OK) {
+            (void)hipFree(mask);
+            tm.check(st);
+        }
+        return mask;
+    }
+
     void run(RowBatch &batch)
     {
         const size_t n = batch.getRowCount();
-        Column &mask = batch.getColumn(mask_index_);
-        if (type_width(mask.getTypeId()) != 1) throw ExecutionError("FilterCompactionOperator: mask must be 1 byte/row");
-        const size_t n_pad = mask.capacity();
+        const uint8_t *mask = nullptr;
+        size_t n_pad = 0;
+        std::unique_ptr<uint8_t, void (*)(uint8_t *)> pred_mask(nullptr, [](uint8_t *p) { (void)hipFree(p); });
+        if (mask_index_ != kNoMask) {
+            if (mask_index_ >= batch.getColumnCount())
+                throw ExecutionError("FilterCompactionOperator: mask column out of range");
+            Column &m = batch.getColumn(mask_index_);
+            if (type_width(m.getTypeId()) != 1) throw ExecutionError("FilterCompactionOperator: mask must be 1 byte/row");
+            mask = static_cast<const uint8_t *>(m.rawData());
+            n_pad = m.capacity();
+        } else {
+            n_pad = batch.getColumn(0).capacity();
+            pred_mask.reset(evaluate(batch, n_pad));
+            mask = pred_mask.get();
+        }
         std::vector<std::pair<void *, uint32_t>> cols; // every buffer moved by the network
         for (size_t i = 0; i < batch.getColumnCount(); ++i) {
             if (i == mask_index_) continue;
@@ -497,11 +580,10 @@ private:
         TaskManager &tm = context_.getTaskManager();
         // the routing depends on the mask only: each group of up to 8
         // buffers is moved by the same network
-        for (size_t at = 0; at < std::max<size_t>(cols.size(), 1); at += VO_MAX_SORT_COLUMNS) {
+        for (size_t at = 0; at < cols.size(); at += VO_MAX_SORT_COLUMNS) {
             vo_compact_args a{};
-            a.mask = static_cast<const uint8_t *>(mask.rawData());
-            const size_t k = std::min<size_t>(VO_MAX_SORT_COLUMNS, cols.size() - std::min(at, cols.size()));
-            if (k == 0) break;
+            a.mask = mask;
+            const size_t k = std::min<size_t>(VO_MAX_SORT_COLUMNS, cols.size() - at);
             for (size_t j = 0; j < k; ++j) {
                 a.cols[j] = cols[at + j].first;
                 a.col_bytes[j] = cols[at + j].second;
@@ -512,14 +594,14 @@ private:
             a.out_count = d_count;
             tm.check(vo_compact(tm.native(), &a));
         }
-        uint64_t count = 0;
         if (cols.empty()) {
-            // nothing to move: the count alone (a compaction of the mask copy)
-            void *tmp = mask.getDeviceBuffer();
+            // nothing to move: the count alone (a compaction of a mask copy)
+            void *tmp = nullptr;
+            hip_check(hipMalloc(&tmp, n_pad), "hipMalloc");
             std::unique_ptr<void, void (*)(void *)> g2(tmp, [](void *p) { (void)hipFree(p); });
-            hip_check(hipMemcpy(tmp, mask.rawData(), n_pad, hipMemcpyDeviceToDevice), "hipMemcpy");
+            hip_check(hipMemcpy(tmp, mask, n_pad, hipMemcpyDeviceToDevice), "hipMemcpy");
             vo_compact_args a{};
-            a.mask = static_cast<const uint8_t *>(mask.rawData());
+            a.mask = mask;
             a.cols[0] = tmp;
             a.col_bytes[0] = 1;
             a.n_cols = 1;
@@ -528,12 +610,15 @@ private:
             a.out_count = d_count;
             tm.check(vo_compact(tm.native(), &a));
         }
+        uint64_t count = 0;
         hip_check(hipMemcpy(&count, d_count, sizeof(count), hipMemcpyDeviceToHost), "hipMemcpy");
         batch.setRowCount(count);
     }
 
     std::unique_ptr<AbstractOperator> child_;
     size_t mask_index_;
+    std::vector<PredicateTerm> pred_;
+    ConnectiveType conn_ = ConnectiveType::AND;
     std::vector<bool> compact_;
 };
 
@@ -914,20 +999,41 @@ private:
 // FilterCompactionPlanNode (include/planner/filter_compaction_plan_node.hpp:14-27,
 // src/planner/filter_compaction_plan_node.cpp): compacts the rows whose mask
 // byte (column mask_index) is set; setCompactColumns chooses the columns.
+// The node selects either by a mask column or by a predicate evaluated on
+// the device.
 class FilterCompactionPlanNode : public AbstractPlanNode {
 public:
     explicit FilterCompactionPlanNode(size_t mask_index) : AbstractPlanNode(PlanType::COMPACTION), mask_(mask_index) {}
+    FilterCompactionPlanNode(std::vector<PredicateTerm> predicate, ConnectiveType connective = ConnectiveType::AND)
+        : AbstractPlanNode(PlanType::COMPACTION), mask_(kNoMask), pred_(std::move(predicate)), conn_(connective)
+    {
+    }
     void setCompactColumns(std::vector<bool> compact_columns) { compact_ = std::move(compact_columns); }
     const std::vector<bool> &getCompactColumns() const { return compact_; }
     std::unique_ptr<AbstractOperator> createOperator(ExecutionContext &context) const override
     {
-        return std::make_unique<FilterCompactionOperator>(
-            context, child_op(context, 0, 1, "FilterCompactionPlanNode"), mask_, compact_);
+        auto child = child_op(context, 0, 1, "FilterCompactionPlanNode");
+        if (mask_ == kNoMask)
+            return std::make_unique<FilterCompactionOperator>(context, std::move(child), pred_, conn_, compact_);
+        return std::make_unique<FilterCompactionOperator>(context, std::move(child), mask_, compact_);
     }
-    std::string toString() const override { return "FilterCompaction(mask #" + std::to_string(mask_) + ")"; }
+    std::string toString() const override
+    {
+        if (mask_ != kNoMask) return "FilterCompaction(mask #" + std::to_string(mask_) + ")";
+        static const char *const ops[] = {"=", "<>", "<", "<=", ">", ">="};
+        std::string s = "FilterCompaction(";
+        for (size_t i = 0; i < pred_.size(); ++i) {
+            if (i) s += conn_ == ConnectiveType::AND ? " AND " : " OR ";
+            s += "#" + std::to_string(pred_[i].column) + " " + ops[static_cast<int>(pred_[i].cmp)] + " c";
+        }
+        return s + ")";
+    }
 
 private:
+    static constexpr size_t kNoMask = ~(size_t)0;
     size_t mask_;
+    std::vector<PredicateTerm> pred_;
+    ConnectiveType conn_ = ConnectiveType::AND;
     std::vector<bool> compact_;
 };
 

@@ -92,6 +92,7 @@ typedef enum vo_opcode {
     VO_OP_PKFK_JOIN_GROUPSUM = 0x104,   /* PK/FK join + GROUP BY SUM */
     VO_OP_SORT_BATCH = 0x105,           /* many independent (u64,u32) sorts */
     VO_OP_GATHER_OBLIVIOUS = 0x106,     /* table values by record id, no data-dependent address */
+    VO_OP_PREDICATE_MASK = 0x107,       /* WHERE terms (column vs constant, AND / OR) -> u8 mask */
     VO_OP_TERMINATE = 0xFFFFFFFFu
 } vo_opcode;
 
@@ -203,8 +204,10 @@ typedef struct vo_shuffle_args {
 /* OP_FILTER_SUM: Q6-style predicate
  *   day_lo <= day < day_hi && disc_lo <= disc <= disc_hi && qty < qty_lt
  * over n rows padded to n_pad, oblivious compaction of (price, disc) into
- * out_price / out_disc (n_pad each, may be NULL to skip writing them), and
- * exact int128 SUM(price * disc) over the selected rows.
+ * out_price / out_disc (n_pad each), and exact int128 SUM(price * disc) over
+ * the selected rows.  With out_price and out_disc both NULL the compaction
+ * runs in place: price and disc (n_pad each) are overwritten with the
+ * compacted rows.
  * out (device pointer, 32 bytes): { u64 count, u64 pad, u64 sum_lo, i64 sum_hi }. */
 typedef struct vo_filter_args {
     const int32_t *day;
@@ -284,6 +287,45 @@ typedef struct vo_join_pairs_args {
     uint64_t out_cap;
 } vo_join_pairs_args;
 
+/* OP_PREDICATE_MASK: the device form of the host predicate evaluation
+ * (ComparisonExpression::evaluateBatch, src/expression/comparison_expression.cpp:13-92,
+ * and BinaryLogicalExpression::evaluateBatch, src/expression/logical_expression.cpp:61-150).
+ * out_mask[i] = 1 when row i < n satisfies every term (VO_CONN_AND) or any
+ * term (VO_CONN_OR) and no term's row is NULL, else 0; rows [n, n_pad) get 0.
+ * A term compares its column with a constant in the column's own type:
+ * `cmp` takes the reference's ComparisonType order (EQUAL, NOT_EQUAL,
+ * LESS_THAN, LESS_THAN_OR_EQUAL, GREATER_THAN, GREATER_THAN_OR_EQUAL);
+ * `constant` holds the constant's bit pattern in its low bytes (an integer
+ * of the column's width, a FLOAT's 32 bits, a DOUBLE's 64 bits); `valid` is
+ * an optional byte per row, 0 = NULL.  Every row of every term column is
+ * read.  `terms` is host memory, read before vo_submit / vo_predicate_mask
+ * returns. */
+#define VO_MAX_PREDICATE_TERMS 8
+typedef enum vo_cmp {
+    VO_CMP_EQ = 0,
+    VO_CMP_NE = 1,
+    VO_CMP_LT = 2,
+    VO_CMP_LE = 3,
+    VO_CMP_GT = 4,
+    VO_CMP_GE = 5
+} vo_cmp;
+typedef enum vo_connective { VO_CONN_AND = 0, VO_CONN_OR = 1 } vo_connective;
+typedef struct vo_predicate_term {
+    const void *col;      /* n rows of the column type */
+    const uint8_t *valid; /* optional, n bytes, 0 = NULL */
+    uint64_t constant;    /* bit pattern of the constant */
+    uint32_t type_id;     /* vo_type */
+    uint32_t cmp;         /* vo_cmp */
+} vo_predicate_term;
+typedef struct vo_predicate_args {
+    const vo_predicate_term *terms; /* host array of n_terms (1..8) */
+    uint32_t n_terms;
+    uint32_t connective;            /* vo_connective */
+    uint8_t *out_mask;              /* device, n_pad bytes */
+    uint64_t n;
+    uint64_t n_pad;
+} vo_predicate_args;
+
 /* OP_GATHER_OBLIVIOUS: out[i] = table[record_ids[i]] for i < n, the device
  * form of the host oblivious materialization
  * (src/operator/materialization_operator.cpp:362-517).  The table rows and
@@ -335,6 +377,7 @@ typedef union vo_command_args {
     vo_sort_batch_args sort_batch;
     vo_join_pairs_args join_pairs; /* OP_SORT_MERGE_JOIN_COUNT / _WRITE */
     vo_gather_oblivious_args gather_oblivious;
+    vo_predicate_args predicate;   /* OP_PREDICATE_MASK */
 } vo_command_args;
 
 /* Mirror of Command (commands.hpp:152-156). */
@@ -411,6 +454,10 @@ VO_EXPORT int vo_sort_batch(vo_ctx *ctx, const vo_sort_batch_args *args);
 VO_EXPORT int vo_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args *args, uint64_t *n_out);
 VO_EXPORT int vo_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args *args);
 VO_EXPORT int vo_gather_oblivious(vo_ctx *ctx, const vo_gather_oblivious_args *args);
+/* OP_PREDICATE_MASK (replaces the host ComparisonExpression /
+ * BinaryLogicalExpression::evaluateBatch, comparison_expression.cpp:13-92,
+ * logical_expression.cpp:61-150, for column-vs-constant terms). */
+VO_EXPORT int vo_predicate_mask(vo_ctx *ctx, const vo_predicate_args *args);
 
 /* Host <-> device wire format (StagedTransfer, include/cuda/staged_transfer.hpp:101-225,
  * and the padded moves of ValueVector::to, include/data/value_vector.hpp:140-294).

@@ -74,6 +74,8 @@ def lib():
         L.orc_gather_recover.argtypes = [P, sz, P, sz, P, P]
         L.orc_join_pairs.restype = sz
         L.orc_join_pairs.argtypes = [P, sz, P, sz, P, P, sz, P]
+        L.orc_predicate_mask.restype = C.c_int
+        L.orc_predicate_mask.argtypes = [P, P, P, P, P, u32, u32, sz, sz, P]
         _lib = L
     return _lib
 
@@ -291,3 +293,32 @@ def philox4x32_10(ctr, key):
 
 def philox_tag(seed, i):
     return lib().orc_philox_tag(seed, i)
+
+
+def const_bits(type_id, value):
+    """Bit pattern of a constant in a column type (vo_type ids)."""
+    import struct
+    if type_id == 6:
+        return struct.unpack("<I", struct.pack("<f", value))[0]
+    if type_id == 7:
+        return struct.unpack("<Q", struct.pack("<d", value))[0]
+    width = {1: 1, 2: 1, 3: 2, 4: 4, 10: 4}.get(type_id, 8)
+    return int(value) & ((1 << (8 * width)) - 1)
+
+
+def predicate_mask(terms, n, npad, conn=0):
+    """terms: (numpy column, type id, cmp 0..5, constant value[, valid bytes or
+    None]); conn 0 = AND, 1 = OR.  Returns the uint8 mask of npad rows."""
+    k = len(terms)
+    cols = [np.ascontiguousarray(t[0]) for t in terms]
+    valids = [None if len(t) < 5 or t[4] is None else np.ascontiguousarray(t[4], np.uint8) for t in terms]
+    cp = (C.c_void_p * max(1, k))(*[c.ctypes.data for c in cols])
+    vp = (C.c_void_p * max(1, k))(*[None if v is None else v.ctypes.data for v in valids])
+    ty = (C.c_uint32 * max(1, k))(*[t[1] for t in terms])
+    cm = (C.c_uint32 * max(1, k))(*[t[2] for t in terms])
+    kb = (C.c_uint64 * max(1, k))(*[const_bits(t[1], t[3]) for t in terms])
+    m = np.empty(npad, np.uint8)
+    rc = lib().orc_predicate_mask(cp, vp, ty, cm, kb, k, conn, n, npad, _p(m))
+    if rc != 0:
+        raise ValueError("unsupported predicate term")
+    return m

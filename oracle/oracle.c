@@ -946,3 +946,70 @@ void orc_bitonic_sort_kp_mt(uint64_t *keys, uint32_t *payload, size_t n, size_t 
     mt_args a = {keys, payload, 0, npad, 0, 1, nthreads};
     mt_sort(&a);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Predicate mask (comparison_expression.cpp:13-92, logical_expression.cpp:61-150) */
+/* ------------------------------------------------------------------------ */
+
+#define ORC_CMP(v, k, c)                                                                                \
+    ((c) == 0 ? (v) == (k) : (c) == 1 ? (v) != (k) : (c) == 2 ? (v) < (k) : (c) == 3 ? (v) <= (k)    \
+     : (c) == 4 ? (v) > (k) : (v) >= (k))
+
+/* One comparison of row i; *ok = 0 for an unsupported type. */
+static int orc_cmp_row(const void *col, uint32_t type, uint32_t cmp, uint64_t kb, size_t i, int *ok)
+{
+    switch (type) {
+    case 1: { uint8_t v = ((const uint8_t *)col)[i], k = (uint8_t)kb; return ORC_CMP(v, k, cmp); }
+    case 2: { int8_t v = ((const int8_t *)col)[i], k = (int8_t)(uint8_t)kb; return ORC_CMP(v, k, cmp); }
+    case 3: { int16_t v = ((const int16_t *)col)[i], k = (int16_t)(uint16_t)kb; return ORC_CMP(v, k, cmp); }
+    case 4: { int32_t v = ((const int32_t *)col)[i], k = (int32_t)(uint32_t)kb; return ORC_CMP(v, k, cmp); }
+    case 10: { uint32_t v = ((const uint32_t *)col)[i], k = (uint32_t)kb; return ORC_CMP(v, k, cmp); }
+    case 5: { int64_t v = ((const int64_t *)col)[i], k = (int64_t)kb; return ORC_CMP(v, k, cmp); }
+    case 6: {
+        float v = ((const float *)col)[i], k;
+        uint32_t b = (uint32_t)kb;
+        memcpy(&k, &b, 4);
+        return ORC_CMP(v, k, cmp);
+    }
+    case 7: {
+        double v = ((const double *)col)[i], k;
+        memcpy(&k, &kb, 8);
+        return ORC_CMP(v, k, cmp);
+    }
+    case 8: case 64: { uint64_t v = ((const uint64_t *)col)[i]; return ORC_CMP(v, kb, cmp); }
+    default: *ok = 0; return 0;
+    }
+}
+
+int orc_predicate_mask(const void *const *cols, const uint8_t *const *valid, const uint32_t *types,
+                       const uint32_t *cmps, const uint64_t *consts, uint32_t n_terms, uint32_t conn,
+                       size_t n, size_t npad, uint8_t *mask)
+{
+    if (n_terms == 0 || conn > 1) return -1;
+    for (uint32_t j = 0; j < n_terms; ++j)
+        if (cmps[j] > 5) return -1;
+    int ok = 1;
+    for (size_t i = 0; i < npad; ++i) {
+        if (i >= n) {
+            mask[i] = 0;
+            continue;
+        }
+        /* left-deep chain: (data, null) of the running result */
+        int data = 0, null = 0;
+        for (uint32_t j = 0; j < n_terms; ++j) {
+            const int tnull = valid[j] != NULL && valid[j][i] == 0;
+            const int tdata = tnull ? 0 : orc_cmp_row(cols[j], types[j], cmps[j], consts[j], i, &ok);
+            if (j == 0) {
+                data = tdata;
+                null = tnull;
+            } else if (null || tnull) {
+                data = 0;
+                null = 1;
+            } else {
+                data = conn == 0 ? (data & tdata) : (data | tdata);
+            }
+        }
+        mask[i] = (uint8_t)(null ? 0 : data);
+    }
+    return ok ? 0 : -1;
+}

@@ -254,6 +254,23 @@ size_t orc_join_pairs(const uint64_t *lkey, size_t n_left, const uint64_t *rkey,
 void orc_gather_recover(const uint64_t *table, size_t n_table, const int64_t *ids, size_t n,
                         uint64_t *out, uint8_t *found);
 
+/* Predicate mask over column-vs-constant terms, restating the host
+ * ComparisonExpression::evaluateBatch with a scalar
+ * (src/expression/comparison_expression.cpp:13-92: the comparison in the
+ * column's type; a NULL row gives NULL with data 0) and the AND / OR of
+ * BinaryLogicalExpression::evaluateBatch (src/expression/logical_expression.cpp:61-150:
+ * a NULL operand makes the result NULL with data 0), evaluated as the
+ * left-deep chain (((t0 op t1) op t2) ...) a parser builds.  mask[i] = data
+ * byte of row i (0 for NULL), rows >= n are 0.  types: vo_type ids;
+ * cmps: 0..5 in ComparisonType order; consts: bit patterns in the column
+ * type; valid[j] may be NULL (no NULLs), else 0 marks a NULL row; conn 0 =
+ * AND, 1 = OR.  Returns 0, or -1 for an unsupported type or comparison.
+ * Pinned: the reference's test_expression_batch.cpp vectors and the Q6 mask
+ * of orc_filter_mask. */
+int orc_predicate_mask(const void *const *cols, const uint8_t *const *valid, const uint32_t *types,
+                       const uint32_t *cmps, const uint64_t *consts, uint32_t n_terms, uint32_t conn,
+                       size_t n, size_t npad, uint8_t *mask);
+
 #ifdef __cplusplus
 }
 #endif

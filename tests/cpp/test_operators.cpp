@@ -501,6 +501,40 @@ static void test_plan_nodes(ExecutionContext &ctx)
         if (b.getRowCount() == 3) CHECK((b.getColumn(C_ID).toHost<int32_t>(3) == std::vector<int32_t>{5, 7, 3}));
     }
 
+    // the same WHERE as a predicate evaluated on the device (no mask column):
+    // department = 'Engineering' AND salary >= 76000.0 -> Eve, Grace
+    auto pf = std::make_unique<FilterCompactionPlanNode>(
+        std::vector<PredicateTerm>{PredicateTerm::make<uint64_t>(C_DEPT, ComparisonType::EQUAL, 0),
+                                   PredicateTerm::make<double>(C_SALARY, ComparisonType::GREATER_THAN_OR_EQUAL,
+                                                               76000.0)});
+    CHECK(pf->toString() == "FilterCompaction(#3 = c AND #2 >= c)");
+    pf->addChild(std::make_unique<ScanPlanNode>([] { return employees(all_rows()); }));
+    auto ps = std::make_unique<SortPlanNode>(std::vector<size_t>{C_SALARY}, std::vector<bool>{false});
+    ps->addChild(std::move(pf));
+    RowBatch pb = run_plan(ctx, *ps);
+    CHECK(pb.getRowCount() == 2);
+    if (pb.getRowCount() == 2) CHECK((pb.getColumn(C_ID).toHost<int32_t>(2) == std::vector<int32_t>{5, 7}));
+    // OR of two terms over a nullable column: a NULL row is never selected
+    {
+        std::vector<int32_t> v = {5, 50, 7, 70}, w = {1, 1, 0, 0};
+        std::vector<uint8_t> vvalid = {1, 0, 1, 1};
+        std::vector<Column> c;
+        c.push_back(Column::fromHost(VO_T_INTEGER, v.data(), 4, vvalid.data()));
+        c.push_back(Column::fromHost(VO_T_INTEGER, w.data(), 4));
+        FilterCompactionOperator f(ctx, std::make_unique<BatchSourceOperator>(ctx, RowBatch(std::move(c), 4)),
+                                   std::vector<PredicateTerm>{PredicateTerm::make<int32_t>(0, ComparisonType::GREATER_THAN, 60),
+                                                              PredicateTerm::make<int32_t>(1, ComparisonType::EQUAL, 1)},
+                                   ConnectiveType::OR);
+        auto r = f.next();
+        CHECK(static_cast<bool>(r));
+        if (r) {
+            RowBatch b = std::move(r).value();
+            // row 0 (5 > 60 OR 1 = 1), row 1 NULL, row 2 neither, row 3 (70 > 60)
+            CHECK(b.getRowCount() == 2);
+            if (b.getRowCount() == 2) CHECK((b.getColumn(0).toHost<int32_t>(2) == std::vector<int32_t>{5, 70}));
+        }
+    }
+
     // sort-merge join of unsorted inputs through the planner's wiring
     // (tests/query/test_join_execution.cpp:12-66 data, rows shuffled)
     const std::vector<int32_t> lid = {3, 0, 4, 1, 2}, rid = {2, 4, 0, 3, 1};

@@ -443,3 +443,64 @@ def test_parallel_bitonic_equals_single_thread(threads):
             wk, wp = k.copy(), p.copy()
             oracle.bitonic_sort_kp_mt_inplace(wk, wp, n, threads)
             assert np.array_equal(wk, ek) and np.array_equal(wp, ep)
+
+
+# ---------------------------------------------------------------- predicates
+
+def test_predicate_reference_vectors():
+    """tests/expression/test_expression_batch.cpp:46-48, 97-118, 145-184 of
+    the reference: val > 20 -> F F F T T; (val > 10) AND active -> F F F T F."""
+    val = np.array([10, 20, 5, 30, 25], np.int32)
+    act = np.array([1, 0, 1, 1, 0], np.uint8)
+    assert oracle.predicate_mask([(val, 4, 4, 20)], 5, 5).tolist() == [0, 0, 0, 1, 1]
+    assert oracle.predicate_mask([(val, 4, 4, 10), (act, 1, 1, 0)], 5, 5).tolist() == [0, 0, 0, 1, 0]
+
+
+def test_predicate_q6_equals_filter_mask():
+    """The Q6 clause as five generic terms equals the oracle's fused Q6 mask."""
+    n, npad = 50_000, 1 << 16
+    day, qty, _, disc = oracle.gen_fact(3, n)
+    terms = [(day, 4, 5, 8766), (day, 4, 2, 9131), (disc, 5, 5, 5), (disc, 5, 3, 7), (qty, 5, 2, 24)]
+    got = oracle.predicate_mask(terms, n, npad)
+    assert np.array_equal(got, oracle.filter_mask(day, qty, disc, n, npad))
+    assert 0 < got.sum() < n
+
+
+@pytest.mark.parametrize("type_id,dtype", [(1, np.uint8), (2, np.int8), (3, np.int16), (4, np.int32),
+                                           (10, np.uint32), (5, np.int64), (6, np.float32), (7, np.float64),
+                                           (8, np.uint64), (64, np.uint64)])
+def test_predicate_comparisons_follow_the_type(type_id, dtype):
+    """Each comparison in the column's own type (signedness, IEEE order with
+    NaN and -0.0) against numpy's typed comparison operators."""
+    import operator
+    rng = np.random.default_rng(type_id)
+    n = 4000
+    if np.dtype(dtype).kind == "f":
+        col = (rng.integers(-8, 8, n) / 2).astype(dtype)
+        col[::37] = np.nan
+        col[::41] = -0.0
+        k = dtype(0.0)
+    else:
+        info = np.iinfo(dtype)
+        col = rng.integers(info.min, info.max, n, dtype=dtype, endpoint=True)
+        col[::5] = info.max // 2
+        k = dtype(info.max // 2)
+    ops = [operator.eq, operator.ne, operator.lt, operator.le, operator.gt, operator.ge]
+    for c, op in enumerate(ops):
+        got = oracle.predicate_mask([(col, type_id, c, k.item())], n, n)
+        assert np.array_equal(got, op(col, k).astype(np.uint8)), c
+
+
+def test_predicate_nulls_and_chain_order():
+    """A NULL operand makes the row NULL (unselected) under AND and OR
+    (logical_expression.cpp:138-143 sets NULL, data 0): TRUE OR NULL and
+    FALSE AND NULL both give 0; rows past n are 0."""
+    a = np.array([1, 1, 0, 0, 1], np.int32)
+    b = np.array([1, 0, 1, 0, 1], np.int32)
+    bv = np.array([1, 0, 0, 1, 1], np.uint8)
+    t = [(a, 4, 0, 1), (b, 4, 0, 1, bv)]
+    assert oracle.predicate_mask(t, 4, 6, conn=1).tolist() == [1, 0, 0, 0, 0, 0]
+    assert oracle.predicate_mask(t, 4, 6, conn=0).tolist() == [1, 0, 0, 0, 0, 0]
+    assert oracle.predicate_mask([(b, 4, 0, 1, bv)], 5, 5).tolist() == [1, 0, 0, 0, 1]
+    with pytest.raises(ValueError):
+        oracle.predicate_mask([(a, 99, 0, 1)], 5, 5)

@@ -247,6 +247,18 @@ class Context:
                                      elem_bytes, 0)
         return self.check(self.lib.vo_gather_oblivious(self.h, C.byref(a)))
 
+    def predicate_mask(self, terms, out_mask, n, n_pad, connective=_abi.CONN_AND):
+        """OP_PREDICATE_MASK: out_mask[i] = 1 for rows i < n meeting all
+        (CONN_AND) or any (CONN_OR) of the terms with no NULL term, else 0.
+        terms: (column tensor, type id, cmp, constant value[, valid bytes])."""
+        arr = (_abi.PredicateTerm * max(1, len(terms)))()
+        for j, t in enumerate(terms):
+            col, type_id, cmp, value = t[:4]
+            valid = t[4] if len(t) > 4 else None
+            arr[j] = _abi.PredicateTerm(_ptr(col), _ptr(valid), _abi.constant_bits(type_id, value), type_id, cmp)
+        a = _abi.PredicateArgs(arr, len(terms), connective, _ptr(out_mask), n, n_pad)
+        return self.check(self.lib.vo_predicate_mask(self.h, C.byref(a)))
+
     def transfer_to_device(self, dst, src_host, padded_bytes=None):
         """Pageable numpy array -> device tensor, padded_bytes over the link
         (default: the tensor's size); returns the bytes moved."""

@@ -39,6 +39,7 @@ OP_FILTER_SUM = 0x103
 OP_PKFK_JOIN_GROUPSUM = 0x104
 OP_SORT_BATCH = 0x105
 OP_GATHER_OBLIVIOUS = 0x106
+OP_PREDICATE_MASK = 0x107
 OP_TERMINATE = 0xFFFFFFFF
 
 CFG_PERSISTENT = 1 << 0
@@ -122,12 +123,39 @@ class GatherObliviousArgs(C.Structure):
                 ("elem_bytes", u32), ("reserved", u32)]
 
 
+# ComparisonType order (reference include/expression/expression.hpp:34-41)
+CMP_EQ, CMP_NE, CMP_LT, CMP_LE, CMP_GT, CMP_GE = 0, 1, 2, 3, 4, 5
+CONN_AND, CONN_OR = 0, 1
+MAX_PREDICATE_TERMS = 8
+
+
+class PredicateTerm(C.Structure):
+    _fields_ = [("col", P), ("valid", P), ("constant", u64), ("type_id", u32), ("cmp", u32)]
+
+
+class PredicateArgs(C.Structure):
+    _fields_ = [("terms", C.POINTER(PredicateTerm)), ("n_terms", u32), ("connective", u32), ("out_mask", P),
+                ("n", u64), ("n_pad", u64)]
+
+
+def constant_bits(type_id, value):
+    """Bit pattern of a constant in a column type (the `constant` field)."""
+    import struct
+    if type_id == T_FLOAT:
+        return struct.unpack("<I", struct.pack("<f", value))[0]
+    if type_id == T_DOUBLE:
+        return struct.unpack("<Q", struct.pack("<d", value))[0]
+    width = {T_BOOLEAN: 1, T_TINYINT: 1, T_SMALLINT: 2, T_INTEGER: 4, T_DATE: 4}.get(type_id, 8)
+    return int(value) & ((1 << (8 * width)) - 1)
+
+
 class CommandArgs(C.Union):
     _fields_ = [("sort", SortArgs), ("sort_kp", SortKpArgs), ("permute", PermuteArgs), ("scatter", ScatterArgs),
                 ("gather", GatherArgs), ("compare_eq", CompareEqArgs), ("compose_position_map", ComposeArgs),
                 ("enrich_positions", EnrichArgs), ("compact", CompactArgs), ("shuffle", ShuffleArgs),
                 ("filter", FilterArgs), ("join", JoinArgs), ("sort_batch", SortBatchArgs),
-                ("join_pairs", JoinPairsArgs), ("gather_oblivious", GatherObliviousArgs)]
+                ("join_pairs", JoinPairsArgs), ("gather_oblivious", GatherObliviousArgs),
+                ("predicate", PredicateArgs)]
 
 
 class Command(C.Structure):
@@ -191,6 +219,7 @@ PROTOTYPES = [
     ("vo_join_pairs_count", C.c_int, [P, C.POINTER(JoinPairsArgs), C.POINTER(u64)]),
     ("vo_transfer_to_device", C.c_int, [P, P, P, u64, u64, C.POINTER(u64)]),
     ("vo_gather_oblivious", C.c_int, [P, C.POINTER(GatherObliviousArgs)]),
+    ("vo_predicate_mask", C.c_int, [P, C.POINTER(PredicateArgs)]),
     ("vo_transfer_to_host", C.c_int, [P, P, P, u64, u64, C.POINTER(u64)]),
     ("vo_join_pairs_write", C.c_int, [P, C.POINTER(JoinPairsArgs)]),
     ("vo_sort_kp_async", C.c_int, [P, P, P, u64, u64]),

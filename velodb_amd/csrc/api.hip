@@ -138,6 +138,7 @@ void dispatch(vo_ctx *ctx, const vo_command &cmd)
     case VO_OP_SORT_MERGE_JOIN_COUNT: (void)op_join_pairs_count(ctx, a.join_pairs); break;
     case VO_OP_SORT_MERGE_JOIN_WRITE: op_join_pairs_write(ctx, a.join_pairs); break;
     case VO_OP_GATHER_OBLIVIOUS: op_gather_oblivious(ctx, a.gather_oblivious); break;
+    case VO_OP_PREDICATE_MASK: op_predicate_mask(ctx, a.predicate); break;
     case VO_OP_TERMINATE: break;
     default:
         // the reference ignores unknown opcodes (persistent_kernel.cu:69-72); report them instead
@@ -409,6 +410,7 @@ VO_SYNC_OP(vo_pkfk_join_groupsum, VO_OP_PKFK_JOIN_GROUPSUM, join, vo_join_args)
 VO_SYNC_OP(vo_sort_batch, VO_OP_SORT_BATCH, sort_batch, vo_sort_batch_args)
 VO_SYNC_OP(vo_join_pairs_write, VO_OP_SORT_MERGE_JOIN_WRITE, join_pairs, vo_join_pairs_args)
 VO_SYNC_OP(vo_gather_oblivious, VO_OP_GATHER_OBLIVIOUS, gather_oblivious, vo_gather_oblivious_args)
+VO_SYNC_OP(vo_predicate_mask, VO_OP_PREDICATE_MASK, predicate, vo_predicate_args)
 
 int vo_transfer_to_device(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes, uint64_t padded_bytes,
                           uint64_t *link_bytes)

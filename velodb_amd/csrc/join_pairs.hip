@@ -100,7 +100,7 @@ __device__ __forceinline__ uint32_t cls(uint64_t v) { return (uint32_t)(v >> 62)
 
 __device__ __forceinline__ bool same_seg(uint64_t ka, uint64_t va, uint64_t kb, uint64_t vb)
 {
-    return (ka == kb) & (cls(va) != 3) & (cls(vb) != 3);
+    return (int)(ka == kb) & (int)(cls(va) != 3) & (int)(cls(vb) != 3);
 }
 
 // ------------------------------------------------------ tile-level scans
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(PB) void k_jp_fwd_rows(const uint64_t *__restrict__
         const uint32_t i = (uint32_t)(t0 + r);
         const uint64_t kp = r ? k[r - 1] : pk, vp = r ? v[r - 1] : pv;
         st[r] = !same_seg(kp, vp, k[r], v[r]);
-        rs[r] = (cls(v[r]) == 2) & (st[r] | (cls(vp) != 2));
+        rs[r] = (int)(cls(v[r]) == 2) & ((int)st[r] | (int)(cls(vp) != 2));
         f.s = st[r] ? i : f.s;
         f.r = rs[r] ? i : f.r;
         f.x += cls(v[r]) == 0 ? ax[r] : 0;

@@ -655,7 +655,6 @@ int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_s
 
         auto worker = [&](uint32_t slot) {
             (void)hipSetDevice(s->device);
-            vo_ctx *ctx = s->slots[slot];
             for (;;) {
                 uint64_t i;
                 {

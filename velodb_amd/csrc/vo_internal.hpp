@@ -201,6 +201,7 @@ void op_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args &a);
 void join_pairs_state_free(vo_ctx *ctx);
 uint64_t join_pairs_last_count(vo_ctx *ctx); // n_out of the last COUNT
 void op_gather_oblivious(vo_ctx *ctx, const vo_gather_oblivious_args &a);
+void op_predicate_mask(vo_ctx *ctx, const vo_predicate_args &a);
 void op_transfer_to_device(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes, uint64_t padded_bytes);
 void op_transfer_to_host(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes, uint64_t padded_bytes);
 uint64_t last_transfer_bytes(vo_ctx *ctx);
