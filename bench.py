@@ -667,6 +667,32 @@ def bench_batch(args):
     torch.cuda.synchronize()
     tp = run(per_instance, max(1, min(args.steps, 3)))
     same = bool(torch.equal(kb, keys) and torch.equal(pb, pay))
+    # roofline over every sort pass of one batched call, from a second context
+    # with per-launch events (the timed runs above carry none): each instance
+    # of padded size p moves 24 B x p per pass, and a class runs the pass plan
+    # of its size, so the call's algorithmic bytes are sum(24 p passes(p))
+    from velodb_amd import _abi
+    tctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+    plan = {}
+    for _, _, p in items:
+        if p not in plan:
+            plan[p] = tctx.sort_plan(p)
+    alg = sum(2 * 12 * p * plan[p] for _, _, p in items)
+    ctx.gen_sort_input(7, keys, pay, total)
+    tctx.kernel_stats_reset()
+    tctx.sort_batch(keys, pay, items)
+    torch.cuda.synchronize()
+    fams = ["sort_tiny", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2", "sort_global_s3",
+            "sort_global_s4", "sort_global_s5", "sort_hybrid_s6", "sort_hybrid_s7", "sort_hybrid_s8"]
+    st = _family_stats(tctx, fams)
+    k_ms = sum(v[1] for v in st.values())
+    tctx.close()
+    ach = alg / (k_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "every sort pass of one batched call (local, merge, global, hybrid)",
+            "kernel_ms_per_call": round(k_ms, 3), "alg_bytes_per_call": alg,
+            "passes_by_class": {str(p): plan[p] for p in sorted(plan)}, "tuning": "default"}
     _line("batched oblivious sorts (Mrec/s over padded records)", padded / tb / 1e6, "Mrec/s", args.steps,
           args.warmup, tb * 1e3, "u64",
           {"workload": "4096 sorts, sizes log-uniform in [2^10, 2^20] (numpy default_rng(0)), u64 keys + u32 "
@@ -674,7 +700,7 @@ def bench_batch(args):
            "instances": len(items), "raw_records": int(sizes.sum()), "padded_records": padded,
            "autotuned": bool(args.autotune)},
           per_instance_ms=round(tp * 1e3, 2), speedup_vs_per_instance=round(tp / tb, 2),
-          batched_equals_per_instance=same)
+          batched_equals_per_instance=same, roofline=roof)
     ctx.close()
 
 
