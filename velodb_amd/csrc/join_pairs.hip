@@ -520,6 +520,62 @@ __global__ void k_jp_dist_levels(const uint64_t *__restrict__ i0, const uint64_t
     tr.done(dev::gtid());
 }
 
+// The same levels on the lattice of the smallest stride u = j_top >> (S-1):
+// slot p = q + k u with residue q < u and row k.  Output row k reads rows
+// k, k-1, .., k-(2^S-1) of the same residue only, so a block takes 256
+// consecutive residues and K rows plus the 2^S - 1 rows before them into
+// registers and runs the S levels there, largest stride first (each slot
+// read (K + 2^S - 1) / K times instead of 2^S).  A level with lattice
+// stride s updates rows from the top down, so row r - s still holds the
+// previous level; rows below the running sum of strides hold partial values
+// that no output row reads.  Rows k < 0 (p < 0) are empty, as in dist_val.
+// Needs u >= 256 and K <= m_pad / u.
+template <int NW, int S, int K, bool TRACE>
+__global__ __launch_bounds__(256) void k_jp_dist_lattice(const uint64_t *__restrict__ i0,
+                                                         const uint64_t *__restrict__ i1, uint64_t u,
+                                                         uint64_t *__restrict__ o0, uint64_t *__restrict__ o1,
+                                                         TraceSlot *slot)
+{
+    constexpr int H = (1 << S) - 1;
+    constexpr int W = K + H;
+    dev::AtrT<TRACE> tr(slot);
+    const uint64_t res_blocks = u / 256;
+    const uint64_t q = (blockIdx.x % res_blocks) * 256 + threadIdx.x;
+    const int64_t k0 = (int64_t)(blockIdx.x / res_blocks) * K - H;
+    Elem<NW> v[W];
+#pragma unroll
+    for (int r = 0; r < W; ++r) {
+        const int64_t k = k0 + r;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) v[r].w[w] = EMPTY;
+        if (k >= 0) { // public: the row only
+            const uint64_t pp = q + (uint64_t)k * u;
+            v[r].w[0] = i0[pp];
+            if constexpr (NW == 2) v[r].w[1] = i1[pp];
+            tr.at(1, pp);
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < S; ++l) {
+        const int st = 1 << (S - 1 - l); // lattice stride of this level
+        const uint64_t j = (uint64_t)st * u;
+#pragma unroll
+        for (int r = W - 1; r >= 1; --r) {
+            if (r < st) continue; // compile-time after unrolling
+            const int64_t pp = (int64_t)q + (k0 + r) * (int64_t)u;
+            v[r] = dist_step<NW>(v[r], v[r - st], pp, j);
+        }
+    }
+#pragma unroll
+    for (int r = H; r < W; ++r) {
+        const uint64_t pp = q + (uint64_t)(k0 + r) * u;
+        o0[pp] = v[r].w[0];
+        if constexpr (NW == 2) o1[pp] = v[r].w[1];
+        tr.at(2, pp);
+    }
+    tr.done(dev::gtid());
+}
+
 // Levels 128 .. 1 of one 2048-slot tile in LDS (window of the 256 slots
 // before it), then the tile's last non-empty element for the fill.
 template <int NW, bool TRACE>
@@ -724,6 +780,42 @@ struct JoinState {
     uint64_t *cl, *cr0, *cr1, *cnt_l, *cnt_r, *n_out_dev;
 };
 
+// k_jp_dist_lattice<NW, S, K> for runtime S in 1..3 and K in {2, .., 32}
+// (K >= 2^S always holds: rows = m_pad / u >= 2^S).  Two-word elements stop
+// at K = 16: at 32 the rows take every VGPR and one wave per SIMD remains.
+template <int NW>
+constexpr uint64_t kLatticeRows = NW == 1 ? 32 : 16;
+template <int NW, int S, int K, class F>
+void lattice_k(bool tr, F &&f)
+{
+    if (tr) f(k_jp_dist_lattice<NW, S, K, true>);
+    else f(k_jp_dist_lattice<NW, S, K, false>);
+}
+
+template <int NW, int S, class F>
+void lattice_s(int K, bool tr, F &&f)
+{
+    switch (K) {
+    case 2: if constexpr (S <= 1) { lattice_k<NW, S, 2>(tr, f); return; } break;
+    case 4: if constexpr (S <= 2) { lattice_k<NW, S, 4>(tr, f); return; } break;
+    case 8: lattice_k<NW, S, 8>(tr, f); return;
+    case 16: lattice_k<NW, S, 16>(tr, f); return;
+    case 32: if constexpr (NW == 1) { lattice_k<NW, S, 32>(tr, f); return; } break;
+    default: break;
+    }
+    throw Error{VO_ERR_INVALID, "distribution lattice: bad row count"};
+}
+
+template <int NW, class F>
+void dist_lattice_launch(int S, int K, bool tr, F &&f)
+{
+    switch (S) {
+    case 1: lattice_s<NW, 1>(K, tr, f); break;
+    case 2: lattice_s<NW, 2>(K, tr, f); break;
+    default: lattice_s<NW, 3>(K, tr, f); break;
+    }
+}
+
 template <int NW>
 void distribute(vo_ctx *ctx, const uint64_t *s0, const uint64_t *s1, uint64_t src_rows, const uint64_t *kdev,
                 uint64_t m_pad, uint64_t *a0, uint64_t *a1, uint64_t *b0, uint64_t *b1, Elem<NW> *tl,
@@ -746,11 +838,21 @@ void distribute(vo_ctx *ctx, const uint64_t *s0, const uint64_t *s1, uint64_t sr
         auto go = [&](auto k) {
             launch(ctx, li, k, (const uint64_t *)c0, (const uint64_t *)c1, m_pad, j, n0, n1, trace_slot(ctx));
         };
-        switch (sl) {
-        case 1: go(tr ? k_jp_dist_levels<NW, 1, true> : k_jp_dist_levels<NW, 1, false>); break;
-        case 2: go(tr ? k_jp_dist_levels<NW, 2, true> : k_jp_dist_levels<NW, 2, false>); break;
-        case 3: go(tr ? k_jp_dist_levels<NW, 3, true> : k_jp_dist_levels<NW, 3, false>); break;
-        default: go(tr ? k_jp_dist_levels<NW, 4, true> : k_jp_dist_levels<NW, 4, false>); break;
+        if (sl <= 3) {
+            // lattice form: u = smallest stride of the group (>= DH = 256),
+            // rows = m_pad / u >= 2^sl, K = min(kLatticeRows, rows)
+            const uint64_t u = j >> (sl - 1);
+            const uint64_t rows = m_pad / u;
+            const int K = (int)std::min<uint64_t>(kLatticeRows<NW>, rows);
+            LaunchInfo ll = li;
+            ll.grid = dim3((uint32_t)(m_pad / (256ull * (uint64_t)K)));
+            ll.block = dim3(256);
+            auto lat = [&](auto k) {
+                launch(ctx, ll, k, (const uint64_t *)c0, (const uint64_t *)c1, u, n0, n1, trace_slot(ctx));
+            };
+            dist_lattice_launch<NW>(sl, K, tr, lat);
+        } else {
+            go(tr ? k_jp_dist_levels<NW, 4, true> : k_jp_dist_levels<NW, 4, false>);
         }
         j >>= sl;
         std::swap(c0, n0);
