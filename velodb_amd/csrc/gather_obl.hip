@@ -12,7 +12,9 @@
 //   3. forward fill of the last table row (row, value) to the requests;
 //   4. ORCompact of the request rows: the first n rows hold
 //      (request index << 1 | found, value) in record-id order;
-//   5. KV sort of those n rows (padded to 2^k) by request index.
+//   5. sort of those n rows (padded to 2^k) by request index, as
+//      (u64 key, u32 payload) records: the value's low word rides below the
+//      index in the key and its high word is the payload.
 // Every pass reads and writes every row of its padded size; the record ids
 // and their repetitions never reach an address.
 #include "vo_internal.hpp"
@@ -211,33 +213,42 @@ __global__ __launch_bounds__(GB) void k_og_rows(const uint64_t *__restrict__ key
 }
 
 // keys of the final sort: request rows (the first n) by index, pads last
+// Requests back in request order by the (u64 key, u32 payload) sort: c0 =
+// index << 1 | found is unique and below 2^30 (table + requests <= 2^29), so
+// key = c0 << 32 | value low word orders by index alone, and the value's high
+// word rides as the payload: 12 bytes per record instead of the 16 of a
+// (u64, u64) sort.  Pad rows get distinct keys above every request key.
 template <bool TRACE>
 __global__ void k_og_order(const uint64_t *__restrict__ c0, const uint64_t *__restrict__ c1, uint64_t n,
-                           uint64_t m_pad, uint64_t *__restrict__ keys, uint64_t *__restrict__ vals, TraceSlot *slot)
+                           uint64_t m_pad, uint64_t *__restrict__ keys, uint32_t *__restrict__ hi,
+                           TraceSlot *slot)
 {
     dev::AtrT<TRACE> tr(slot);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m_pad; i += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t k = NONE - m_pad + i, v = 0; // distinct pad keys above every request key
-        if (i < n) {                           // public
-            k = c0[i];
-            v = c1[i];
+        uint64_t k = NONE - m_pad + i;
+        uint32_t h = 0;
+        if (i < n) { // public
+            const uint64_t v = c1[i];
+            k = (c0[i] << 32) | (v & 0xFFFFFFFFull);
+            h = (uint32_t)(v >> 32);
             tr.at(1, i);
         }
         keys[i] = k;
-        vals[i] = v;
+        hi[i] = h;
         tr.at(2, i);
     }
     tr.done(dev::gtid());
 }
 
 template <bool TRACE>
-__global__ void k_og_out(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals, uint64_t n,
+__global__ void k_og_out(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ hi, uint64_t n,
                          uint32_t eb, void *__restrict__ out, uint8_t *__restrict__ found, TraceSlot *slot)
 {
     dev::AtrT<TRACE> tr(slot);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        store_elem(out, eb, i, vals[i]);
-        if (found) found[i] = (uint8_t)(keys[i] & 1);
+        const uint64_t k = keys[i];
+        store_elem(out, eb, i, ((uint64_t)hi[i] << 32) | (k & 0xFFFFFFFFull));
+        if (found) found[i] = (uint8_t)((k >> 32) & 1);
         tr.at(1, i);
     }
     tr.done(dev::gtid());
@@ -286,13 +297,15 @@ void op_gather_oblivious(vo_ctx *ctx, const vo_gather_oblivious_args &a)
     launch(ctx, LaunchInfo{"gather_obl_rows", dim3(nt), dim3(GB), 0, n_pad}, tr ? k_og_rows<true> : k_og_rows<false>,
            (const uint64_t *)keys, (const uint64_t *)vals, (const Last *)carry, c0, c1, mask, trace_slot(ctx));
     compact_rows2(ctx, c0, c1, mask, n_pad, cnt);
-    // the first n rows are the requests; reuse keys/vals for the final sort
+    // the first n rows are the requests; reuse keys (and vals as the u32
+    // payload column) for the final sort
+    uint32_t *hi = reinterpret_cast<uint32_t *>(vals);
     launch(ctx, LaunchInfo{"gather_obl_order", grid_for(m_pad), dim3(256), 0, hash_mix(a.n, m_pad)},
            tr ? k_og_order<true> : k_og_order<false>, (const uint64_t *)c0, (const uint64_t *)c1, a.n, m_pad, keys,
-           vals, trace_slot(ctx));
-    sort_kv(ctx, keys, vals, m_pad);
+           hi, trace_slot(ctx));
+    sort_kp_full(ctx, keys, hi, m_pad);
     launch(ctx, LaunchInfo{"gather_obl_out", grid_for(a.n), dim3(256), 0, hash_mix(a.n, eb)},
-           tr ? k_og_out<true> : k_og_out<false>, (const uint64_t *)keys, (const uint64_t *)vals, a.n, eb, a.out,
+           tr ? k_og_out<true> : k_og_out<false>, (const uint64_t *)keys, (const uint32_t *)hi, a.n, eb, a.out,
            a.out_found, trace_slot(ctx));
 }
 
