@@ -297,7 +297,9 @@ def bench_join(args):
             "compact_count", "compact_local", "compact_hybrid", "compact_global", "sort_local", "sort_merge",
             "sort_global_s1", "sort_global_s2", "sort_global_s3", "sort_global_s4", "sort_hybrid_s6", "sort_hybrid_s7"]
     stats = _family_stats(ctx, fams)
-    bpr = {f: 32 for f in fams if f.startswith("sort_") or f in ("compact_global", "compact_hybrid")}
+    # the stats are the dense run's: its union sort moves (u64 key, u32
+    # payload) records, 2 x 12 B per row and pass
+    bpr = {f: 24 for f in fams if f.startswith("sort_")}
     _line("oblivious PK/FK join + GROUP BY SUM (Mrows/s)", (n_dim + n_fact) / dt / 1e6, "Mrows/s", args.steps,
           args.warmup, dt * 1e3, "int64",
           {"workload": "PK/FK join of 1.5M dimension x 6M fact rows (union padded to 2^23) + exact int64 "

@@ -891,6 +891,7 @@ private:
         tm.check(vo_pkfk_join_groupsum(tm.native(), &a));
         uint64_t g = 0;
         hip_check(hipMemcpy(&g, d_n, sizeof(g), hipMemcpyDeviceToHost), "hipMemcpy");
+        if (g == ~0ull) throw ExecutionError("PKFKJoinGroupSumOperator: a fact value above INT64_MAX - 65");
         std::vector<Column> cols;
         cols.push_back(std::move(og));
         cols.push_back(std::move(os));

@@ -245,9 +245,12 @@ typedef struct vo_join_args {
     uint64_t *sorted_keys;
     uint64_t *sorted_vals;
     uint32_t dense_groups; /* 1: group ids lie in [0, max_groups) (a public bound, max_groups <= 64);
-                              the GROUP BY is an oblivious dense aggregation over the compacted
-                              matched rows instead of sort + segmented scan + compaction.  Rows
-                              whose group is outside the bound are not counted. */
+                              the GROUP BY is an oblivious dense aggregation fused with the join's
+                              fill instead of sort + segmented scan + compaction.  Rows whose group
+                              is outside the bound are not counted.  With max_groups <= 63 and no
+                              sorted_keys / sorted_vals, the union is sorted as 12-byte records;
+                              there fact values must be <= INT64_MAX - 65, and a larger one makes
+                              *out_n_groups = UINT64_MAX (no result). */
 } vo_join_args;
 
 /* OP_SORT_MERGE_JOIN_COUNT / OP_SORT_MERGE_JOIN_WRITE: oblivious equi-join

@@ -90,3 +90,91 @@ def test_join_single_group_spans_tiles(vctx):
     fv = rng.integers(100, 10_000_000, n_fact).astype(np.int64)
     g, s, _, _ = _run(vctx, dk, dg, fk, fv, 1 << 17)
     assert g.tolist() == [0] and s.tolist() == [int(fv.sum())]
+
+
+# ------------------------------------------------ 12-byte union records
+# dense_groups with max_groups <= 63 and no sorted-union outputs sorts the
+# union as (u64 key, u32 payload) records; fact values up to INT64_MAX - 65.
+
+_I64_MAX = np.iinfo(np.int64).max
+_I64_MIN = np.iinfo(np.int64).min
+
+
+def _run_compact(vctx, dk, dg, fk, fv, n_pad, max_groups):
+    """Returns (groups, sums), or None when the library reports UINT64_MAX
+    groups (a fact value the 12-byte union cannot hold)."""
+    d = [to_dev(x) for x in (dk, dg, fk, fv)]
+    og = empty_dev(max_groups, np.int32)
+    os_ = empty_dev(max_groups, np.int64)
+    on = empty_dev(1, np.uint64)
+    vctx.join_groupsum(d[0], d[1], len(dk), d[2], d[3], len(fk), n_pad, og, os_, on, max_groups, dense_groups=True)
+    G = int(from_dev(on, np.uint64)[0])
+    if G == (1 << 64) - 1:
+        return None
+    g = from_dev(og, np.int32)
+    s = from_dev(os_, np.int64)
+    assert G <= max_groups and np.all(g[G:] == -1) and np.all(s[G:] == 0)
+    return g[:G], s[:G]
+
+
+@pytest.mark.parametrize("n_dim,n_fact,n_pad", [(100, 400, 4096), (1500, 6000, 8192), (20000, 50000, 1 << 17),
+                                                (150000, 600000, 1 << 20)])
+def test_join_compact_union_bit_exact(vctx, n_dim, n_fact, n_pad):
+    print(f"join compact union n_dim={n_dim} n_fact={n_fact}", flush=True)
+    dk, dg, fk, fv = oracle.gen_join(4, n_dim, n_fact)
+    g, s = _run_compact(vctx, dk, dg, fk, fv, n_pad, 25)
+    eg, es = oracle.join_groupsum(dk, dg, fk, fv, n_pad, max_groups=25)
+    assert np.array_equal(g, eg) and np.array_equal(s, es)
+
+
+def test_join_compact_union_config3_full_size(vctx):
+    """1.5M dimension rows, 6M fact rows, union padded to 2^23, 25 groups:
+    the closed-form sums."""
+    n_dim, n_fact, n_pad = 1_500_000, 6_000_000, 1 << 23
+    dk, dg, fk, fv = oracle.gen_join(1, n_dim, n_fact)
+    g, s = _run_compact(vctx, dk, dg, fk, fv, n_pad, 25)
+    g_of_key = np.zeros(n_dim + 1, np.int64)
+    g_of_key[dk] = dg
+    sums = np.zeros(25, np.int64)
+    np.add.at(sums, g_of_key[fk], fv)
+    assert g.tolist() == list(range(25)) and s.tolist() == sums.tolist()
+
+
+def test_join_compact_union_value_extremes_and_bounds(vctx):
+    """INT64_MIN and INT64_MAX - 65 are exact; groups outside [0, max_groups)
+    (negative, 63 and above) are not counted, as on the 16-byte path; a fact
+    value of INT64_MAX - 64, matched or not, gives UINT64_MAX groups."""
+    dk = np.array([1, 2, 3, 4, 5, 6, 7], np.int32)
+    dg = np.array([0, 1, 2, 3, 70, -5, 63], np.int32)
+    fk = np.array([1, 2, 3, 4, 1, 5, 6, 7, 9], np.int32)
+    fv = np.array([_I64_MIN, _I64_MAX - 65, -1, 0, 5, 11, 12, 13, 77], np.int64)
+    g, s = _run_compact(vctx, dk, dg, fk, fv, 4096, 25)
+    assert g.tolist() == [0, 1, 2, 3]
+    assert s.tolist() == [_I64_MIN + 5, _I64_MAX - 65, -1, 0]
+    # the same rows through the 16-byte path (sorted outputs requested)
+    gk, sk, _, _ = _run(vctx, dk, dg, fk, fv, 4096, max_groups=25, dense=True)
+    assert np.array_equal(g, gk) and np.array_equal(s, sk)
+    for bad_key in (1, 9):  # matched, unmatched
+        fk2 = np.append(fk, np.int32(bad_key))
+        fv2 = np.append(fv, np.int64(_I64_MAX - 64))
+        assert _run_compact(vctx, dk, dg, fk2, fv2, 4096, 25) is None
+    # the context is still usable and the next result is exact
+    g, s = _run_compact(vctx, dk, dg, fk, fv, 4096, 25)
+    assert s.tolist() == [_I64_MIN + 5, _I64_MAX - 65, -1, 0]
+
+
+def test_join_compact_union_trace_value_independent(vctx):
+    """Two inputs of equal sizes, different keys, groups and values: the
+    same launches and address digests on the 12-byte path."""
+    def run(seed):
+        dk, dg, fk, fv = oracle.gen_join(seed, 3000, 12000)
+        d = [to_dev(x) for x in (dk, dg, fk, fv)]
+        og, os_, on = empty_dev(25, np.int32), empty_dev(25, np.int64), empty_dev(1, np.uint64)
+        vctx.trace_begin()
+        vctx.join_groupsum(d[0], d[1], 3000, d[2], d[3], 12000, 1 << 14, og, os_, on, 25, dense_groups=True)
+        t = vctx.trace_end()
+        return t, vctx.trace_launches(t.n_launches)
+
+    (t1, l1), (t2, l2) = run(5), run(6)
+    assert t1.n_launches == t2.n_launches > 0 and l1 == l2
+    assert t1.launch_digest == t2.launch_digest and t1.address_digest == t2.address_digest

@@ -63,6 +63,56 @@ __global__ void k_join_build(const int32_t *__restrict__ dk, const int32_t *__re
     tr.done(dev::gtid());
 }
 
+// The (u64 key, u32 payload) union of the dense GROUP BY, chosen on public
+// arguments only (dense_groups, max_groups <= 63, no sorted-union outputs).
+// There the row index is not needed (the sums do not depend on the order of
+// equal keys), so the 96 bits hold key32 and a 64-bit field L, the key's low
+// word then the payload:
+//   dimension row  L = group, or 63 for a group outside [0, 63) (not counted,
+//                  as max_groups <= 63)
+//   fact row       L = 64 + (value + 2^63) mod 2^64
+// so each key's dimension row sorts before its facts, in 12 bytes instead of
+// 16.  A value above INT64_MAX - 65 does not fit (L would wrap into the
+// dimension codes or reach the pad record): the block that reads one sets
+// its flag and the output reports UINT64_MAX groups.
+constexpr int64_t kKpMaxValue = INT64_MAX - 65;
+
+template <bool TRACE>
+__global__ __launch_bounds__(256) void k_join_build_kp(const int32_t *__restrict__ dk, const int32_t *__restrict__ dg,
+                                                       uint64_t n_dim, const int32_t *__restrict__ fk,
+                                                       const int64_t *__restrict__ fv, uint64_t n_fact, uint64_t n_pad,
+                                                       uint64_t *__restrict__ keys, uint32_t *__restrict__ pay,
+                                                       uint32_t *__restrict__ bad, TraceSlot *slot)
+{
+    dev::AtrT<TRACE> tr(slot);
+    int over = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k = PAD_KEY;
+        uint32_t p = 0xFFFFFFFFu;
+        if (i < n_dim) { // public sizes only
+            const uint32_t g = (uint32_t)dg[i];
+            const uint64_t L = g < 63u ? g : 63u;
+            k = ((uint64_t)(uint32_t)dk[i] << 32) | (L >> 32);
+            p = (uint32_t)L;
+            tr.at(1, i);
+        } else if (i < n_dim + n_fact) {
+            const uint64_t f = i - n_dim;
+            const int64_t v = fv[f];
+            over |= (int)(v > kKpMaxValue);
+            const uint64_t L = 64u + ((uint64_t)v ^ (1ull << 63));
+            k = ((uint64_t)(uint32_t)fk[f] << 32) | (L >> 32);
+            p = (uint32_t)L;
+            tr.at(2, f);
+        }
+        keys[i] = k;
+        pay[i] = p;
+        tr.at(3, i);
+    }
+    over = __syncthreads_or(over);
+    if (threadIdx.x == 0) bad[blockIdx.x] = (uint32_t)over;
+    tr.done(dev::gtid());
+}
+
 // ------------------------------------------------- forward fill (step 3)
 
 struct Fill {
@@ -72,9 +122,31 @@ struct Fill {
 
 __device__ __forceinline__ bool is_dim(uint64_t k) { return k != PAD_KEY && ((k >> 31) & 1) == 0; }
 
+// Union row i in either form: key32, the dimension / fact flags and the
+// 64-bit field (the group of a dimension row, the value of a fact row).
+template <bool KP>
+__device__ __forceinline__ void union_row(const uint64_t *__restrict__ keys, const void *__restrict__ vals, uint64_t i,
+                                          uint64_t &key32, bool &dim, bool &fact, uint64_t &v)
+{
+    const uint64_t k = keys[i];
+    if constexpr (KP) {
+        const uint32_t p = static_cast<const uint32_t *>(vals)[i];
+        const uint64_t L = (k << 32) | p;
+        const bool pad = (k == PAD_KEY) & (p == 0xFFFFFFFFu);
+        dim = !pad & (L < 64u);
+        fact = !pad & (L >= 64u);
+        v = dim ? L : ((L - 64u) ^ (1ull << 63));
+    } else {
+        v = static_cast<const uint64_t *>(vals)[i];
+        dim = is_dim(k);
+        fact = (k != PAD_KEY) & (((k >> 31) & 1) != 0);
+    }
+    key32 = k >> 32;
+}
+
 // Per tile: the last dimension row (key, group) or none.
-template <bool TRACE>
-__global__ __launch_bounds__(JB) void k_fill_tiles(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
+template <bool TRACE, bool KP>
+__global__ __launch_bounds__(JB) void k_fill_tiles(const uint64_t *__restrict__ keys, const void *__restrict__ vals,
                                                    Fill *__restrict__ tile_last, TraceSlot *slot)
 {
     __shared__ uint64_t wk[JB], wg[JB];
@@ -83,10 +155,11 @@ __global__ __launch_bounds__(JB) void k_fill_tiles(const uint64_t *__restrict__ 
     uint64_t lk = PAD_KEY, lg = 0;
 #pragma unroll
     for (int k = 0; k < JPER; ++k) {
-        const uint64_t key = keys[t0 + k], v = vals[t0 + k];
+        uint64_t k32, v;
+        bool d, f;
+        union_row<KP>(keys, vals, t0 + k, k32, d, f, v);
         tr.at(1, t0 + k);
-        const bool d = is_dim(key);
-        lk = d ? (key >> 32) : lk;
+        lk = d ? k32 : lk;
         lg = d ? v : lg;
     }
     wk[threadIdx.x] = lk;
@@ -459,8 +532,8 @@ __device__ __forceinline__ int64_t wave_sum64(int64_t v)
 // selects (matched fact rows whose group is g); a presence bitmask stands in
 // for the per-group counts.  One partial row per tile; no compaction is
 // needed because every union row is visited anyway.
-template <int G, bool TRACE>
-__global__ __launch_bounds__(JB) void k_fill_agg(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
+template <int G, bool TRACE, bool KP>
+__global__ __launch_bounds__(JB) void k_fill_agg(const uint64_t *__restrict__ keys, const void *__restrict__ vals,
                                                  const Fill *__restrict__ carry, int64_t *__restrict__ part_sum,
                                                  int64_t *__restrict__ part_cnt, TraceSlot *slot)
 {
@@ -473,9 +546,10 @@ __global__ __launch_bounds__(JB) void k_fill_agg(const uint64_t *__restrict__ ke
     uint64_t lk = PAD_KEY, lg = 0;
 #pragma unroll
     for (int k = 0; k < JPER; ++k) {
-        const uint64_t key = keys[t0 + k], v = vals[t0 + k];
-        const bool d = is_dim(key);
-        lk = d ? (key >> 32) : lk;
+        uint64_t k32, v;
+        bool d, f;
+        union_row<KP>(keys, vals, t0 + k, k32, d, f, v);
+        lk = d ? k32 : lk;
         lg = d ? v : lg;
     }
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -512,12 +586,12 @@ __global__ __launch_bounds__(JB) void k_fill_agg(const uint64_t *__restrict__ ke
 #pragma unroll
     for (int k = 0; k < JPER; ++k) {
         const uint64_t i = t0 + k;
-        const uint64_t key = keys[i], v = vals[i];
-        const bool d = is_dim(key);
-        c.key = d ? (key >> 32) : c.key;
+        uint64_t k32, v;
+        bool d, fact;
+        union_row<KP>(keys, vals, i, k32, d, fact, v);
+        c.key = d ? k32 : c.key;
         c.group = d ? v : c.group;
-        const bool fact = key != PAD_KEY && ((key >> 31) & 1);
-        const bool matched = fact & (c.key == (key >> 32));
+        const bool matched = fact & (c.key == k32);
         const uint64_t grp = c.group & 0xFFFFFFFFull;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
@@ -563,6 +637,7 @@ __global__ __launch_bounds__(kOutThreads) void k_dense_out(const int64_t *__rest
                                                            const int64_t *__restrict__ part_cnt, uint32_t nparts,
                                                            uint64_t max_groups, int32_t *__restrict__ out_g,
                                                            int64_t *__restrict__ out_s, uint64_t *__restrict__ out_n,
+                                                           const uint32_t *__restrict__ bad, uint32_t n_bad,
                                                            TraceSlot *slot)
 {
     static_assert(kOutThreads % G == 0, "G divides the block");
@@ -606,7 +681,11 @@ __global__ __launch_bounds__(kOutThreads) void k_dense_out(const int64_t *__rest
         out_s[j] = os;
         tr.at(2, j);
     }
-    if (t == 0 && out_n) *out_n = present;
+    // a fact value that the 12-byte union cannot hold: report UINT64_MAX
+    int over = 0;
+    for (uint32_t b = t; b < n_bad; b += kOutThreads) over |= (int)(bad[b] != 0);
+    over = __syncthreads_or(over);
+    if (t == 0 && out_n) *out_n = over ? ~0ull : present;
     tr.done(dev::gtid());
 }
 
@@ -636,7 +715,8 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
                  o_m = take(n_pad), o_fl = take(nt * sizeof(Fill)), o_fc = take(nt * sizeof(Fill)),
                  o_sg = take(nt * sizeof(Seg)), o_sc = take(nt * 8), o_ss = take(nt * sizeof(SegSum)),
                  o_sp = take(nt * sizeof(SegSum)), o_c1 = take(compact_rows2_scratch(n_pad)),
-                 o_c2 = take(compact_rows2_scratch(n_pad)), o_part = take((size_t)nt * 64 * 16);
+                 o_c2 = take(compact_rows2_scratch(n_pad)), o_part = take((size_t)nt * 64 * 16),
+                 o_bad = take((size_t)65536 * 4);
     char *base = (char *)scratch(ctx, off);
     uint64_t *keys = (uint64_t *)(base + o_k), *vals = (uint64_t *)(base + o_v);
     uint64_t *col_a = (uint64_t *)(base + o_a), *col_b = (uint64_t *)(base + o_b);
@@ -647,19 +727,30 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
     SegSum *ssum = (SegSum *)(base + o_ss), *spre = (SegSum *)(base + o_sp);
     uint64_t *cnt1 = (uint64_t *)(base + o_c1), *cnt2 = (uint64_t *)(base + o_c2);
 
-    // 1-2. union and sort
-    launch(ctx, LaunchInfo{"join_build", grid_for(n_pad), dim3(256), 0, n_pad},
-           ctx->tracing ? k_join_build<true> : k_join_build<false>,
-           a.dim_key, a.dim_group,
-           a.n_dim, a.fact_fk, a.fact_val, a.n_fact, n_pad, keys, vals, trace_slot(ctx));
-    sort_kv(ctx, keys, vals, n_pad);
+    // 1-2. union and sort: 12-byte records for the dense GROUP BY when the
+    // caller wants no sorted union and max_groups <= 63 (public arguments)
+    const bool kp = a.dense_groups && a.max_groups <= 63 && !a.sorted_keys && !a.sorted_vals;
+    uint32_t *bad = (uint32_t *)(base + o_bad);
+    const dim3 bgrid = grid_for(n_pad);
+    if (kp) {
+        launch(ctx, LaunchInfo{"join_build", bgrid, dim3(256), 0, hash_mix(n_pad, 1)},
+               ctx->tracing ? k_join_build_kp<true> : k_join_build_kp<false>, a.dim_key, a.dim_group, a.n_dim,
+               a.fact_fk, a.fact_val, a.n_fact, n_pad, keys, (uint32_t *)vals, bad, trace_slot(ctx));
+        sort_kp_full(ctx, keys, (uint32_t *)vals, n_pad);
+    } else {
+        launch(ctx, LaunchInfo{"join_build", bgrid, dim3(256), 0, n_pad},
+               ctx->tracing ? k_join_build<true> : k_join_build<false>,
+               a.dim_key, a.dim_group,
+               a.n_dim, a.fact_fk, a.fact_val, a.n_fact, n_pad, keys, vals, trace_slot(ctx));
+        sort_kv(ctx, keys, vals, n_pad);
+    }
     if (a.sorted_keys) VO_HIP(hipMemcpyAsync(a.sorted_keys, keys, n_pad * 8, hipMemcpyDeviceToDevice, ctx->stream));
     if (a.sorted_vals) VO_HIP(hipMemcpyAsync(a.sorted_vals, vals, n_pad * 8, hipMemcpyDeviceToDevice, ctx->stream));
     // 3. forward fill
     launch(ctx, LaunchInfo{"join_fill_tiles", dim3(nt), dim3(JB), 0, n_pad},
-           ctx->tracing ? k_fill_tiles<true> : k_fill_tiles<false>,
-           (const uint64_t *)keys,
-           (const uint64_t *)vals, tile_last, trace_slot(ctx));
+           kp ? (ctx->tracing ? k_fill_tiles<true, true> : k_fill_tiles<false, true>)
+              : (ctx->tracing ? k_fill_tiles<true, false> : k_fill_tiles<false, false>),
+           (const uint64_t *)keys, (const void *)vals, tile_last, trace_slot(ctx));
     launch(ctx, LaunchInfo{"join_fill_carry", dim3(1), dim3(kScanThreads), 0, nt},
            ctx->tracing ? k_fill_carry<true> : k_fill_carry<false>,
            (const Fill *)tile_last, nt, fcarry, trace_slot(ctx));
@@ -667,20 +758,25 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a)
         // 4'-5'. dense GROUP BY fused with the fill: one partial row per tile
         int64_t *part = (int64_t *)(base + o_part);
         const uint32_t G = a.max_groups <= 32 ? 32 : 64;
+        const uint32_t n_bad = kp ? bgrid.x : 0u;
         if (G == 32) {
-            launch(ctx, LaunchInfo{"join_fill_agg", dim3(nt), dim3(JB), 0, hash_mix(n_pad, 32)},
-                   ctx->tracing ? k_fill_agg<32, true> : k_fill_agg<32, false>, (const uint64_t *)keys,
-                   (const uint64_t *)vals, (const Fill *)fcarry, part, part + (uint64_t)nt * 32, trace_slot(ctx));
+            auto agg = kp ? (ctx->tracing ? k_fill_agg<32, true, true> : k_fill_agg<32, false, true>)
+                          : (ctx->tracing ? k_fill_agg<32, true, false> : k_fill_agg<32, false, false>);
+            launch(ctx, LaunchInfo{"join_fill_agg", dim3(nt), dim3(JB), 0, hash_mix(n_pad, 32)}, agg,
+                   (const uint64_t *)keys, (const void *)vals, (const Fill *)fcarry, part,
+                   part + (uint64_t)nt * 32, trace_slot(ctx));
             launch(ctx, LaunchInfo{"group_dense_out", dim3(1), dim3(kOutThreads), 0, a.max_groups}, k_dense_out<32>,
                    (const int64_t *)part, (const int64_t *)(part + (uint64_t)nt * 32), nt, a.max_groups, a.out_group,
-                   a.out_sum, a.out_n_groups, trace_slot(ctx));
+                   a.out_sum, a.out_n_groups, (const uint32_t *)bad, n_bad, trace_slot(ctx));
         } else {
-            launch(ctx, LaunchInfo{"join_fill_agg", dim3(nt), dim3(JB), 0, hash_mix(n_pad, 64)},
-                   ctx->tracing ? k_fill_agg<64, true> : k_fill_agg<64, false>, (const uint64_t *)keys,
-                   (const uint64_t *)vals, (const Fill *)fcarry, part, part + (uint64_t)nt * 64, trace_slot(ctx));
+            auto agg = kp ? (ctx->tracing ? k_fill_agg<64, true, true> : k_fill_agg<64, false, true>)
+                          : (ctx->tracing ? k_fill_agg<64, true, false> : k_fill_agg<64, false, false>);
+            launch(ctx, LaunchInfo{"join_fill_agg", dim3(nt), dim3(JB), 0, hash_mix(n_pad, 64)}, agg,
+                   (const uint64_t *)keys, (const void *)vals, (const Fill *)fcarry, part,
+                   part + (uint64_t)nt * 64, trace_slot(ctx));
             launch(ctx, LaunchInfo{"group_dense_out", dim3(1), dim3(kOutThreads), 0, a.max_groups}, k_dense_out<64>,
                    (const int64_t *)part, (const int64_t *)(part + (uint64_t)nt * 64), nt, a.max_groups, a.out_group,
-                   a.out_sum, a.out_n_groups, trace_slot(ctx));
+                   a.out_sum, a.out_n_groups, (const uint32_t *)bad, n_bad, trace_slot(ctx));
         }
         return;
     }
