@@ -190,9 +190,11 @@ typedef struct vo_compact_args {
 
 /* OP_SHUFFLE: oblivious shuffle of n_pad rows: tag_i = Philox4x32-10(i, seed),
  * sort by (tag, i) and apply the permutation to the payload column.
- * perm_out (optional) receives the source row of each output row.  With
- * 0 < n < n_pad, rows >= n keep their place and the first n rows receive a
- * permutation of themselves (tag of a row >= n: the maximum). */
+ * perm_out (optional) receives the source row of each output row.  Without
+ * perm_out the payload is sorted alone as (tag >> 1, payload) records, 12
+ * bytes per row instead of 16: the same permutation unless two live tags
+ * share their top 63 bits.  With 0 < n < n_pad, rows >= n keep their place
+ * and the first n rows receive a permutation of themselves. */
 typedef struct vo_shuffle_args {
     uint32_t *payload; /* n_pad rows, shuffled in place */
     uint32_t *perm_out;

@@ -91,3 +91,39 @@ def test_shuffle_live_rows_only(vctx, n, n_pad):
     assert pm[:n].tolist() == want
     assert pm[n:].tolist() == list(range(n, n_pad))
     assert np.array_equal(got, pay[pm])
+
+
+@pytest.mark.parametrize("n,n_pad,seed", [(8, 8, 1), (1024, 1024, 2), (1000, 1024, 21), (5000, 8192, 3),
+                                          (1 << 16, 1 << 16, 4), (3, 4, 5), (1, 1, 6)])
+def test_payload_only_shuffle_is_the_same_permutation(vctx, n, n_pad, seed):
+    """Without perm_out the payload is shuffled as 12-byte (tag >> 1,
+    payload) records: the same rows land in the same places as with the
+    16-byte (tag, row) form whenever no two live tags share their top 63
+    bits (checked here), and rows >= n stay in place."""
+    tags = [oracle.philox_tag(seed, i) for i in range(n)]
+    assert len({t >> 1 for t in tags}) == n  # no 63-bit tie in this case
+    pay = (np.arange(n_pad, dtype=np.uint64) * 2654435761 % (1 << 32)).astype(np.uint32)
+    a = to_dev(pay)
+    vctx.shuffle(a, n_pad, seed, n=n)
+    b = to_dev(pay)
+    perm = empty_dev(n_pad, np.uint32)
+    vctx.shuffle(b, n_pad, seed, perm, n=n)
+    got, want = from_dev(a, np.uint32), from_dev(b, np.uint32)
+    assert np.array_equal(got, want)
+    order = sorted(range(n), key=lambda i: (tags[i], i))
+    assert np.array_equal(got[:n], pay[order]) and np.array_equal(got[n:], pay[n:])
+
+
+def test_payload_only_shuffle_trace_value_independent(vctx):
+    n = 1 << 14
+
+    def run(vals):
+        d = to_dev(vals)
+        vctx.trace_begin()
+        vctx.shuffle(d, n, 99)
+        t = vctx.trace_end()
+        return t, vctx.trace_launches(t.n_launches)
+
+    (t1, l1), (t2, l2) = run(np.zeros(n, np.uint32)), run(np.random.default_rng(1).integers(0, 2**32, n, np.uint32))
+    assert t1.n_launches == t2.n_launches > 0 and l1 == l2
+    assert t1.launch_digest == t2.launch_digest and t1.address_digest == t2.address_digest

@@ -6,7 +6,8 @@
 // Philox4x32-10(counter = (i, 0), key = seed) (Salmon et al., SC'11); the rows
 // are sorted by (tag, i) with the oblivious bitonic network, and the payload
 // travels with its row, so the permutation is never materialised as
-// addresses.  perm_out[j] receives the source row of output row j.  With
+// addresses.  perm_out[j] receives the source row of output row j (without
+// perm_out the payload is shuffled alone, as 12-byte records).  With
 // n < n_pad live rows, rows >= n get the largest tag and stay in place, so the
 // first n rows receive a uniform permutation of the live rows (a position map
 // of n records padded to a power of two).
@@ -64,6 +65,22 @@ __global__ void k_shuffle_unpack(const uint64_t *__restrict__ vals, uint64_t n_p
     tr.done(dev::gtid());
 }
 
+// Payload-only form (no perm_out): the payload column itself rides as the
+// u32 payload of (u64 key, u32 payload) records, 12 bytes per row instead of
+// 16, sorted in place.  Live rows get key = tag >> 1, pad rows 2^63 | row, so
+// pads stay in place after every live row.  The order is the (tag, row) order
+// of the 16-byte form unless two live tags share their top 63 bits (a tie
+// then falls to the payload values).
+__global__ void k_shuffle_keys(uint64_t seed, uint64_t n, uint64_t n_pad, uint64_t *__restrict__ keys, TraceSlot *slot)
+{
+    dev::Atr tr(slot);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
+        keys[i] = i < n ? (philox_tag(seed, i) >> 1) : ((1ull << 63) | i);
+        tr.at(1, i);
+    }
+    tr.done(dev::gtid());
+}
+
 dim3 grid_for(uint64_t n) { return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 65536))); }
 
 } // namespace
@@ -74,6 +91,13 @@ void op_shuffle(vo_ctx *ctx, const vo_shuffle_args &a)
     VO_CHECK(a.payload != nullptr, VO_ERR_INVALID, "null payload");
     VO_CHECK(a.n <= a.n_pad, VO_ERR_INVALID, "n above n_pad");
     const uint64_t n = a.n ? a.n : a.n_pad;
+    if (a.perm_out == nullptr) {
+        uint64_t *keys = (uint64_t *)scratch(ctx, 8 * a.n_pad + 256);
+        launch(ctx, LaunchInfo{"shuffle_tags", grid_for(a.n_pad), dim3(256), 0, hash_mix(a.n_pad, 1)}, k_shuffle_keys,
+               a.seed, n, a.n_pad, keys, trace_slot(ctx));
+        sort_kp_full(ctx, keys, (uint32_t *)a.payload, a.n_pad);
+        return;
+    }
     uint64_t *keys = (uint64_t *)scratch(ctx, 16 * a.n_pad + 256);
     uint64_t *vals = keys + a.n_pad;
     launch(ctx, LaunchInfo{"shuffle_tags", grid_for(a.n_pad), dim3(256), 0, a.n_pad}, k_shuffle_tags, a.seed,
