@@ -434,14 +434,25 @@ __global__ __launch_bounds__(JB) void k_seg_rows(const uint64_t *__restrict__ ke
     __shared__ SegSum wtot[JB / 64];
     dev::AtrT<TRACE> tr(slot);
     const uint64_t t0 = (uint64_t)blockIdx.x * JT + (uint64_t)threadIdx.x * JPER;
-    const uint64_t g0 = grp(keys[t0]);
+    // this thread's rows in registers (contiguous: vector loads), and the
+    // group of the row after them (a public index; none past n_pad)
+    uint64_t kg[JPER];
+    int64_t kv[JPER];
+#pragma unroll
+    for (int k = 0; k < JPER; ++k) {
+        kg[k] = grp(keys[t0 + k]);
+        kv[k] = (int64_t)vals[t0 + k];
+    }
+    const uint64_t nx = t0 + JPER;
+    const uint64_t g_next = nx < n_pad ? grp(keys[nx < n_pad ? nx : t0]) : ~0ull;
+    const uint64_t g0 = kg[0];
     uint64_t gl = g0;
     int64_t run = 0;
     uint32_t uni = 1;
 #pragma unroll
     for (int k = 0; k < JPER; ++k) {
-        const uint64_t g = grp(keys[t0 + k]);
-        run = (g == gl ? run : 0) + (int64_t)vals[t0 + k];
+        const uint64_t g = kg[k];
+        run = (g == gl ? run : 0) + kv[k];
         uni &= (g == g0);
         gl = g;
     }
@@ -480,19 +491,28 @@ __global__ __launch_bounds__(JB) void k_seg_rows(const uint64_t *__restrict__ ke
     // rows
     uint64_t prev_g = cg;
     int64_t acc = c;
+    uint8_t mk[JPER];
 #pragma unroll
     for (int k = 0; k < JPER; ++k) {
         const uint64_t i = t0 + k;
-        const uint64_t g = grp(keys[i]);
-        acc = (g == prev_g ? acc : 0) + (int64_t)vals[i];
+        const uint64_t g = kg[k];
+        acc = (g == prev_g ? acc : 0) + kv[k];
         prev_g = g;
-        const uint64_t gn = (i + 1 < n_pad) ? grp(keys[i + 1]) : ~0ull;
+        const uint64_t gn = k + 1 < JPER ? kg[k + 1] : g_next;
         const bool last = gn != g;
         col_g[i] = g;
         col_s[i] = (uint64_t)acc;
-        mask[i] = (last & (g != 0xFFFFFFFFull)) ? 1 : 0;
+        mk[k] = (last & (g != 0xFFFFFFFFull)) ? 1 : 0;
         tr.at(5, i);
     }
+    static_assert(JPER == 16, "one 16-byte mask store per thread");
+    typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+    u32x4_t mw;
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+        mw[w] = (uint32_t)mk[4 * w] | ((uint32_t)mk[4 * w + 1] << 8) | ((uint32_t)mk[4 * w + 2] << 16) |
+                ((uint32_t)mk[4 * w + 3] << 24);
+    *reinterpret_cast<u32x4_t *>(mask + t0) = mw; // t0 is a multiple of 16
     tr.done(dev::gtid());
 }
 
