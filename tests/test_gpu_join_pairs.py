@@ -197,15 +197,18 @@ def test_through_command_queue(vctx):
     assert from_dev(orr, np.int64).tolist() == [2, 0, 1, 0, 1, -1, -1, -1]
 
 
-def test_trace_value_independent(vctx):
+@pytest.mark.parametrize("ty", [T_BIGINT, T_INTEGER])  # 16-byte and 12-byte union records
+def test_trace_value_independent(vctx, ty):
     """Two inputs of equal sizes and equal pair count (the one public output)
     but different key structure: identical launches and address digests."""
+    dt = np.int64 if ty == T_BIGINT else np.int32
+
     def run(lk, rk):
-        dl, dr = to_dev(np.asarray(lk, np.int64)), to_dev(np.asarray(rk, np.int64))
+        dl, dr = to_dev(np.asarray(lk, dt)), to_dev(np.asarray(rk, dt))
         vctx.trace_begin()
-        n = vctx.join_pairs_count(dl, len(lk), dr, len(rk), T_BIGINT)
+        n = vctx.join_pairs_count(dl, len(lk), dr, len(rk), ty)
         ol, orr = empty_dev(16384, np.int64), empty_dev(16384, np.int64)
-        vctx.join_pairs_write(dl, len(lk), dr, len(rk), T_BIGINT, ol, orr, 16384)
+        vctx.join_pairs_write(dl, len(lk), dr, len(rk), ty, ol, orr, 16384)
         t = vctx.trace_end()
         return n, t, vctx.trace_launches(t.n_launches)
 

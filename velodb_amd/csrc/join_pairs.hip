@@ -96,6 +96,68 @@ __global__ void k_jp_build(const void *__restrict__ lk, uint64_t nl, const void 
     tr.done(dev::gtid());
 }
 
+// Key types up to 32 bits: the union as 12-byte (u64 key, u32 payload)
+// records, key = code32 << 32 | class << 30 (class 0 left, 2 right, 3 pad),
+// payload = row.  code32 orders like key_code, so the sort order (key,
+// class, row) is that of the 16-byte form, and ld_union below decodes a row
+// to the same (code, class << 62 | row) pair.
+__device__ __forceinline__ uint32_t key_code32(const void *p, uint32_t type, uint64_t i)
+{
+    switch (type) {
+    case VO_T_BOOLEAN: return ((const uint8_t *)p)[i];
+    case VO_T_TINYINT: return (uint32_t)(int32_t)((const int8_t *)p)[i] ^ 0x80000000u;
+    case VO_T_SMALLINT: return (uint32_t)(int32_t)((const int16_t *)p)[i] ^ 0x80000000u;
+    case VO_T_INTEGER: return (uint32_t)((const int32_t *)p)[i] ^ 0x80000000u;
+    default: return ((const uint32_t *)p)[i]; // DATE
+    }
+}
+
+bool key_fits_32(uint32_t type)
+{
+    return type == VO_T_BOOLEAN || type == VO_T_TINYINT || type == VO_T_SMALLINT || type == VO_T_INTEGER ||
+           type == VO_T_DATE;
+}
+
+template <bool TRACE>
+__global__ void k_jp_build_kp(const void *__restrict__ lk, uint64_t nl, const void *__restrict__ rk, uint64_t nr,
+                              uint32_t type, uint64_t n_pad, uint64_t *__restrict__ keys, uint32_t *__restrict__ pay,
+                              TraceSlot *slot)
+{
+    dev::AtrT<TRACE> tr(slot);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k = ~0ull;
+        uint32_t r = (uint32_t)i;
+        if (i < nl) { // public sizes only
+            k = (uint64_t)key_code32(lk, type, i) << 32;
+            tr.at(1, i);
+        } else if (i < nl + nr) {
+            r = (uint32_t)(i - nl);
+            k = ((uint64_t)key_code32(rk, type, i - nl) << 32) | (2ull << 30);
+            tr.at(2, i - nl);
+        }
+        keys[i] = k;
+        pay[i] = r;
+        tr.at(3, i);
+    }
+    tr.done(dev::gtid());
+}
+
+// Union row i as (key code, class << 62 | row) in either form.
+template <bool KP>
+__device__ __forceinline__ void ld_union(const uint64_t *__restrict__ keys, const void *__restrict__ vals, uint64_t i,
+                                         uint64_t &k, uint64_t &v)
+{
+    if constexpr (KP) {
+        const uint64_t kk = keys[i];
+        const uint64_t c = (kk >> 30) & 3;
+        k = c == 3 ? ~0ull : (kk >> 32);
+        v = (c << 62) | static_cast<const uint32_t *>(vals)[i];
+    } else {
+        k = keys[i];
+        v = static_cast<const uint64_t *>(vals)[i];
+    }
+}
+
 __device__ __forceinline__ uint32_t cls(uint64_t v) { return (uint32_t)(v >> 62); } // 0 left, 2 right, 3 pad
 
 __device__ __forceinline__ bool same_seg(uint64_t ka, uint64_t va, uint64_t kb, uint64_t vb)
@@ -204,29 +266,29 @@ struct RowsBuf {
     uint64_t k[PPER + 1], v[PPER + 1]; // the thread's rows and the next one
 };
 
-__device__ __forceinline__ void load_rows(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
+template <bool KP>
+__device__ __forceinline__ void load_rows(const uint64_t *__restrict__ keys, const void *__restrict__ vals,
                                           uint64_t t0, uint64_t n_pad, uint64_t *k, uint64_t *v)
 {
 #pragma unroll
-    for (int r = 0; r < PPER; ++r) {
-        k[r] = keys[t0 + r];
-        v[r] = vals[t0 + r];
-    }
+    for (int r = 0; r < PPER; ++r) ld_union<KP>(keys, vals, t0 + r, k[r], v[r]);
     const bool has_next = t0 + PPER < n_pad; // public
-    k[PPER] = has_next ? keys[t0 + PPER] : ~0ull;
-    v[PPER] = has_next ? vals[t0 + PPER] : (3ull << 62);
+    uint64_t nk, nv;
+    ld_union<KP>(keys, vals, has_next ? t0 + PPER : t0, nk, nv);
+    k[PPER] = has_next ? nk : ~0ull;
+    v[PPER] = has_next ? nv : (3ull << 62);
 }
 
-template <bool TRACE>
+template <bool TRACE, bool KP>
 __global__ __launch_bounds__(PB) void k_jp_bwd_tiles(const uint64_t *__restrict__ keys,
-                                                     const uint64_t *__restrict__ vals, uint64_t n_pad,
+                                                     const void *__restrict__ vals, uint64_t n_pad,
                                                      Min2 *__restrict__ tiles, TraceSlot *slot)
 {
     __shared__ Min2 red[PB / 64];
     dev::AtrT<TRACE> tr(slot);
     const uint64_t t0 = (uint64_t)blockIdx.x * PT + (uint64_t)threadIdx.x * PPER;
     uint64_t k[PPER + 1], v[PPER + 1];
-    load_rows(keys, vals, t0, n_pad, k, v);
+    load_rows<KP>(keys, vals, t0, n_pad, k, v);
     tr.at(1, t0);
     uint32_t me = INF32, mq = INF32;
 #pragma unroll
@@ -256,9 +318,9 @@ __global__ __launch_bounds__(PB) void k_jp_bwd_tiles(const uint64_t *__restrict_
 // Per row: E and Q (suffix minima), aux = m1 for left rows, E for right
 // rows; per tile: the forward-scan summary (latest start, latest first right
 // row, sum of m1 over left rows).
-template <bool TRACE>
+template <bool TRACE, bool KP>
 __global__ __launch_bounds__(PB) void k_jp_bwd_rows(const uint64_t *__restrict__ keys,
-                                                    const uint64_t *__restrict__ vals, uint64_t n_pad,
+                                                    const void *__restrict__ vals, uint64_t n_pad,
                                                     const Min2 *__restrict__ carry, uint32_t *__restrict__ aux,
                                                     Fwd *__restrict__ ftiles, TraceSlot *slot)
 {
@@ -267,10 +329,12 @@ __global__ __launch_bounds__(PB) void k_jp_bwd_rows(const uint64_t *__restrict__
     dev::AtrT<TRACE> tr(slot);
     const uint64_t t0 = (uint64_t)blockIdx.x * PT + (uint64_t)threadIdx.x * PPER;
     uint64_t k[PPER + 1], v[PPER + 1];
-    load_rows(keys, vals, t0, n_pad, k, v);
+    load_rows<KP>(keys, vals, t0, n_pad, k, v);
     const bool has_prev = t0 > 0; // public
-    const uint64_t pk = has_prev ? keys[t0 - 1] : ~0ull;
-    const uint64_t pv = has_prev ? vals[t0 - 1] : (3ull << 62);
+    uint64_t pk, pv;
+    ld_union<KP>(keys, vals, has_prev ? t0 - 1 : 0, pk, pv);
+    pk = has_prev ? pk : ~0ull;
+    pv = has_prev ? pv : (3ull << 62);
     tr.at(1, t0);
     uint32_t me = INF32, mq = INF32;
 #pragma unroll
@@ -351,9 +415,9 @@ __global__ __launch_bounds__(PB) void k_jp_bwd_rows(const uint64_t *__restrict__
 
 // Per row: targets and the compaction columns / masks; the last thread of
 // the last tile writes n_out.
-template <bool TRACE>
+template <bool TRACE, bool KP>
 __global__ __launch_bounds__(PB) void k_jp_fwd_rows(const uint64_t *__restrict__ keys,
-                                                    const uint64_t *__restrict__ vals,
+                                                    const void *__restrict__ vals,
                                                     const uint32_t *__restrict__ aux, uint64_t n_pad,
                                                     const Fwd *__restrict__ carry, uint64_t *__restrict__ cl,
                                                     uint8_t *__restrict__ ml, uint64_t *__restrict__ cr0,
@@ -367,13 +431,14 @@ __global__ __launch_bounds__(PB) void k_jp_fwd_rows(const uint64_t *__restrict__
     uint32_t ax[PPER];
 #pragma unroll
     for (int r = 0; r < PPER; ++r) {
-        k[r] = keys[t0 + r];
-        v[r] = vals[t0 + r];
+        ld_union<KP>(keys, vals, t0 + r, k[r], v[r]);
         ax[r] = aux[t0 + r];
     }
     const bool has_prev = t0 > 0;
-    const uint64_t pk = has_prev ? keys[t0 - 1] : ~0ull;
-    const uint64_t pv = has_prev ? vals[t0 - 1] : (3ull << 62);
+    uint64_t pk, pv;
+    ld_union<KP>(keys, vals, has_prev ? t0 - 1 : 0, pk, pv);
+    pk = has_prev ? pk : ~0ull;
+    pv = has_prev ? pv : (3ull << 62);
     tr.at(1, t0);
     bool st[PPER], rs[PPER];
     Fwd f{0, 0, 0};
@@ -937,24 +1002,37 @@ uint64_t op_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args &a)
     Fwd *ft = (Fwd *)(base + o_ft), *fc = (Fwd *)(base + o_fc);
     const bool tr = ctx->tracing;
 
-    launch(ctx, LaunchInfo{"join_pairs_build", grid_for(n_pad), dim3(256), 0, hash_mix(a.type_id, n_pad)},
-           tr ? k_jp_build<true> : k_jp_build<false>, a.left_keys, a.n_left, a.right_keys, a.n_right, a.type_id,
-           n_pad, keys, vals, trace_slot(ctx));
-    sort_kv(ctx, keys, vals, n_pad);
+    // key types up to 32 bits sort the union as 12-byte records (public: the type)
+    const bool kp = key_fits_32(a.type_id);
+    if (kp) {
+        launch(ctx, LaunchInfo{"join_pairs_build", grid_for(n_pad), dim3(256), 0, hash_mix(a.type_id, n_pad)},
+               tr ? k_jp_build_kp<true> : k_jp_build_kp<false>, a.left_keys, a.n_left, a.right_keys, a.n_right,
+               a.type_id, n_pad, keys, (uint32_t *)vals, trace_slot(ctx));
+        sort_kp_full(ctx, keys, (uint32_t *)vals, n_pad);
+    } else {
+        launch(ctx, LaunchInfo{"join_pairs_build", grid_for(n_pad), dim3(256), 0, hash_mix(a.type_id, n_pad)},
+               tr ? k_jp_build<true> : k_jp_build<false>, a.left_keys, a.n_left, a.right_keys, a.n_right, a.type_id,
+               n_pad, keys, vals, trace_slot(ctx));
+        sort_kv(ctx, keys, vals, n_pad);
+    }
+    auto pick = [&](auto kt_f, auto kt_t, auto kp_f, auto kp_t) { return kp ? (tr ? kp_t : kp_f) : (tr ? kt_t : kt_f); };
     launch(ctx, LaunchInfo{"join_pairs_bwd_tiles", dim3(nt), dim3(PB), 0, n_pad},
-           tr ? k_jp_bwd_tiles<true> : k_jp_bwd_tiles<false>, (const uint64_t *)keys, (const uint64_t *)vals, n_pad,
-           bt, trace_slot(ctx));
+           pick(k_jp_bwd_tiles<false, false>, k_jp_bwd_tiles<true, false>, k_jp_bwd_tiles<false, true>,
+                k_jp_bwd_tiles<true, true>),
+           (const uint64_t *)keys, (const void *)vals, n_pad, bt, trace_slot(ctx));
     launch(ctx, LaunchInfo{"join_pairs_bwd_carry", dim3(1), dim3(kScanThreads), 0, nt},
            tr ? k_jp_bwd_carry<true> : k_jp_bwd_carry<false>, (const Min2 *)bt, nt, bc, trace_slot(ctx));
     launch(ctx, LaunchInfo{"join_pairs_bwd_rows", dim3(nt), dim3(PB), 0, n_pad},
-           tr ? k_jp_bwd_rows<true> : k_jp_bwd_rows<false>, (const uint64_t *)keys, (const uint64_t *)vals, n_pad,
-           (const Min2 *)bc, aux, ft, trace_slot(ctx));
+           pick(k_jp_bwd_rows<false, false>, k_jp_bwd_rows<true, false>, k_jp_bwd_rows<false, true>,
+                k_jp_bwd_rows<true, true>),
+           (const uint64_t *)keys, (const void *)vals, n_pad, (const Min2 *)bc, aux, ft, trace_slot(ctx));
     launch(ctx, LaunchInfo{"join_pairs_fwd_carry", dim3(1), dim3(kScanThreads), 0, nt},
            tr ? k_jp_fwd_carry<true> : k_jp_fwd_carry<false>, (const Fwd *)ft, nt, fc, trace_slot(ctx));
     launch(ctx, LaunchInfo{"join_pairs_fwd_rows", dim3(nt), dim3(PB), 0, n_pad},
-           tr ? k_jp_fwd_rows<true> : k_jp_fwd_rows<false>, (const uint64_t *)keys, (const uint64_t *)vals,
-           (const uint32_t *)aux, n_pad, (const Fwd *)fc, st.cl, ml, st.cr0, st.cr1, mr, st.n_out_dev,
-           trace_slot(ctx));
+           pick(k_jp_fwd_rows<false, false>, k_jp_fwd_rows<true, false>, k_jp_fwd_rows<false, true>,
+                k_jp_fwd_rows<true, true>),
+           (const uint64_t *)keys, (const void *)vals, (const uint32_t *)aux, n_pad, (const Fwd *)fc, st.cl, ml,
+           st.cr0, st.cr1, mr, st.n_out_dev, trace_slot(ctx));
     compact_rows1(ctx, st.cl, ml, n_pad, st.cnt_l);
     compact_rows2(ctx, st.cr0, st.cr1, mr, n_pad, st.cnt_r);
     if (a.out_count)
