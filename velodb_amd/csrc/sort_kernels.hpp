@@ -642,10 +642,12 @@ struct SortImpl {
 // Record kinds and their tile shapes (LDS = 2^T_LOG * 17/16 * record bytes).
 using RecKP = dev::Rec<1, uint32_t, true>; // 12 B: u64 key + u32 payload
 using RecKV = dev::Rec<1, uint64_t, true>; // 16 B: u64 key + u64 value
-using RecM1 = dev::Rec<1, int64_t, false>; // multi-column: packed key words + index
-using RecM2 = dev::Rec<2, int64_t, false>;
-using RecM4 = dev::Rec<4, int64_t, false>;
-using RecM8 = dev::Rec<8, int64_t, false>;
+// multi-column: packed key words + the u32 row index (the unpack pass widens
+// it into the caller's int64 index column)
+using RecM1 = dev::Rec<1, uint32_t, false>;
+using RecM2 = dev::Rec<2, uint32_t, false>;
+using RecM4 = dev::Rec<4, uint32_t, false>;
+using RecM8 = dev::Rec<8, uint32_t, false>;
 
 } // namespace sortimpl
 } // namespace vo

@@ -100,7 +100,7 @@ struct PackSpec {
 // writes index = row.  Pad rows take the image of max for ASC columns and
 // of lowest for DESC columns, the values sort.cuh:79-100 writes.
 __global__ void k_pack_keys(PackSpec ps, uint64_t *w0, uint64_t *w1, uint64_t *w2, uint64_t *w3, uint64_t *w4,
-                            uint64_t *w5, uint64_t *w6, uint64_t *w7, int64_t *idx, uint64_t n, uint64_t n_pad,
+                            uint64_t *w5, uint64_t *w6, uint64_t *w7, uint32_t *idx, uint64_t n, uint64_t n_pad,
                             TraceSlot *slot)
 {
     uint64_t *ws[8] = {w0, w1, w2, w3, w4, w5, w6, w7};
@@ -117,7 +117,7 @@ __global__ void k_pack_keys(PackSpec ps, uint64_t *w0, uint64_t *w1, uint64_t *w
             acc[ps.word[c]] |= v << ps.shift[c];
         }
         for (uint32_t w = 0; w < ps.nwrite; ++w) ws[w][i] = acc[w];
-        idx[i] = (int64_t)i;
+        idx[i] = (uint32_t)i;
         tr.at(1, src); // columns read at row src (row 0 for pad rows: n is public)
         tr.at(2, i);   // key words and index written at row i
     }
@@ -126,7 +126,8 @@ __global__ void k_pack_keys(PackSpec ps, uint64_t *w0, uint64_t *w1, uint64_t *w
 
 __global__ void k_unpack_keys(PackSpec ps, const uint64_t *w0, const uint64_t *w1, const uint64_t *w2,
                               const uint64_t *w3, const uint64_t *w4, const uint64_t *w5, const uint64_t *w6,
-                              const uint64_t *w7, uint64_t n_pad, TraceSlot *slot)
+                              const uint64_t *w7, const uint32_t *idx, int64_t *out_idx, uint64_t n_pad,
+                              TraceSlot *slot)
 {
     const uint64_t *ws[8] = {w0, w1, w2, w3, w4, w5, w6, w7};
     dev::Atr tr(slot);
@@ -137,6 +138,7 @@ __global__ void k_unpack_keys(PackSpec ps, const uint64_t *w0, const uint64_t *w
             v = ps.desc[c] ? (~v & mask) : v;
             denorm_store(const_cast<void *>(ps.cols[c]), ps.types[c], i, v);
         }
+        out_idx[i] = (int64_t)idx[i];
         tr.at(1, i);
     }
     tr.done(dev::gtid());
@@ -178,6 +180,7 @@ void op_sort_multi(vo_ctx *ctx, const vo_sort_args &a)
              "n_sort_columns must be 1..8");
     VO_CHECK(is_pow2(a.n_padded_rows) && a.n_rows <= a.n_padded_rows, VO_ERR_INVALID,
              "n_padded_rows must be a power of two >= n_rows");
+    VO_CHECK(a.n_padded_rows <= kMaxSortRows, VO_ERR_INVALID, "n_padded_rows above 2^29");
     VO_CHECK(a.indices != nullptr, VO_ERR_INVALID, "null indices");
     PackSpec ps{};
     ps.ncols = (uint32_t)a.n_sort_columns;
@@ -211,47 +214,49 @@ void op_sort_multi(vo_ctx *ctx, const vo_sort_args &a)
     const uint32_t W = ps.nwords <= 1 ? 1 : ps.nwords <= 2 ? 2 : ps.nwords <= 4 ? 4 : 8;
     ps.nwrite = W;
     const uint64_t n_pad = a.n_padded_rows;
-    uint64_t *words = (uint64_t *)scratch(ctx, sizeof(uint64_t) * W * n_pad);
+    // key words, then the u32 row indices the sort carries
+    uint64_t *words = (uint64_t *)scratch(ctx, sizeof(uint64_t) * W * n_pad + sizeof(uint32_t) * n_pad);
+    uint32_t *idx = reinterpret_cast<uint32_t *>(words + (uint64_t)W * n_pad);
     uint64_t *wp[8];
     for (int w = 0; w < 8; ++w) wp[w] = words + (uint64_t)std::min<uint32_t>(w, W - 1) * n_pad;
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n_pad + 255) / 256, 8192));
     LaunchInfo lp{"sort_pack", dim3(blocks), dim3(256), 0, hash_mix(ps.ncols, ps.nwords)};
-    launch(ctx, lp, k_pack_keys, ps, wp[0], wp[1], wp[2], wp[3], wp[4], wp[5], wp[6], wp[7], a.indices, a.n_rows,
+    launch(ctx, lp, k_pack_keys, ps, wp[0], wp[1], wp[2], wp[3], wp[4], wp[5], wp[6], wp[7], idx, a.n_rows,
            n_pad, trace_slot(ctx));
     const int n_log = ilog2(n_pad);
     switch (W) {
     case 1: {
         Bufs<RecM1> b;
         b.k[0] = wp[0];
-        b.p = a.indices;
+        b.p = idx;
         SortM1::run(ctx, b, n_log, n_pad, 0);
         break;
     }
     case 2: {
         Bufs<RecM2> b;
         for (int w = 0; w < 2; ++w) b.k[w] = wp[w];
-        b.p = a.indices;
+        b.p = idx;
         SortM2::run(ctx, b, n_log, n_pad, 0);
         break;
     }
     case 4: {
         Bufs<RecM4> b;
         for (int w = 0; w < 4; ++w) b.k[w] = wp[w];
-        b.p = a.indices;
+        b.p = idx;
         SortM4::run(ctx, b, n_log, n_pad, 0);
         break;
     }
     default: {
         Bufs<RecM8> b;
         for (int w = 0; w < 8; ++w) b.k[w] = wp[w];
-        b.p = a.indices;
+        b.p = idx;
         SortM8::run(ctx, b, n_log, n_pad, 0);
         break;
     }
     }
     LaunchInfo lu{"sort_unpack", dim3(blocks), dim3(256), 0, hash_mix(ps.ncols, ps.nwords)};
-    launch(ctx, lu, k_unpack_keys, ps, wp[0], wp[1], wp[2], wp[3], wp[4], wp[5], wp[6], wp[7], n_pad,
-           trace_slot(ctx));
+    launch(ctx, lu, k_unpack_keys, ps, wp[0], wp[1], wp[2], wp[3], wp[4], wp[5], wp[6], wp[7],
+           (const uint32_t *)idx, a.indices, n_pad, trace_slot(ctx));
 }
 
 } // namespace vo
