@@ -496,10 +496,22 @@ def bench_gather(args):
             "compact_count", "compact_scan", "compact_local", "compact_hybrid", "compact_global", "gather_obl_order",
             "gather_obl_out"]
     stats = _family_stats(ctx, fams)
+    # the same ids over a 4-byte (int32) column: 12-byte union records
+    t32 = (table & 0x7FFFFFFF).to(torch.int32)
+    o32 = torch.empty(n, dtype=torch.int32, device="cuda:0")
+    ctx.gather_oblivious(t32, nt, ids, n, o32, 4)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.gather_oblivious(t32, nt, ids, n, o32, 4)
+    torch.cuda.synchronize()
+    dt32 = (time.perf_counter() - t1) / args.steps
+    ok32 = bool(torch.all(o32 == t32[ids]))
     _line("oblivious gather by record id (Mrows/s)", n / dt / 1e6, "Mrows/s", args.steps, args.warmup, dt * 1e3,
           "int64",
           {"workload": "one int64 column of a 2^22-row table gathered at 2^23 record ids (1/8 of them one id); "
                        "union of 2^22 + 2^23 rows padded to 2^24", "n_table": nt, "n": n, "equals_indexing": ok},
+          int32_column={"ms": round(dt32 * 1e3, 3), "equals_indexing": ok32},
           kernels_ms_per_step={f: round(v[1] / args.steps, 3) for f, v in stats.items()})
     ctx.close()
 

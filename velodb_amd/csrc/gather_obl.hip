@@ -8,7 +8,9 @@
 // Here no address depends on a record id:
 //   1. union of the table rows (key = row << 33) and the requests
 //      (key = id << 33 | 1 << 32 | request index), padded to N = 2^k;
-//   2. KV sort by key: each request follows the table row it asks for;
+//   2. sort by key: each request follows the table row it asks for (values
+//      of at most 4 bytes ride as the u32 payload of 12-byte records, wider
+//      ones as a u64 column);
 //   3. forward fill of the last table row (row, value) to the requests;
 //   4. ORCompact of the request rows: the first n rows hold
 //      (request index << 1 | found, value) in record-id order;
@@ -51,10 +53,10 @@ __device__ __forceinline__ void store_elem(void *p, uint32_t eb, uint64_t i, uin
     }
 }
 
-template <bool TRACE>
+template <bool TRACE, bool KP>
 __global__ void k_og_build(const void *__restrict__ table, uint64_t n_table, uint32_t eb,
                            const int64_t *__restrict__ ids, uint64_t n, uint64_t n_pad, uint64_t *__restrict__ keys,
-                           uint64_t *__restrict__ vals, TraceSlot *slot)
+                           void *__restrict__ vals, TraceSlot *slot)
 {
     dev::AtrT<TRACE> tr(slot);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -73,10 +75,21 @@ __global__ void k_og_build(const void *__restrict__ table, uint64_t n_table, uin
             tr.at(2, r);
         }
         keys[i] = k;
-        vals[i] = v;
+        if constexpr (KP) static_cast<uint32_t *>(vals)[i] = (uint32_t)v; // values of at most 4 bytes
+        else static_cast<uint64_t *>(vals)[i] = v;
         tr.at(3, i);
     }
     tr.done(dev::gtid());
+}
+
+// Union row value: u64 column, or the u32 payload column of the 12-byte
+// records used for values of at most 4 bytes (the keys are unique, pads
+// included through their row, so the order is that of the 16-byte form).
+template <bool KP>
+__device__ __forceinline__ uint64_t union_val(const void *__restrict__ vals, uint64_t i)
+{
+    if constexpr (KP) return static_cast<const uint32_t *>(vals)[i];
+    else return static_cast<const uint64_t *>(vals)[i];
 }
 
 // last table row of each tile: (row, value) or NONE
@@ -89,8 +102,8 @@ struct LastOp {
 
 __device__ __forceinline__ bool is_table(uint64_t k) { return k != NONE && ((k >> 32) & 1) == 0; }
 
-template <bool TRACE>
-__global__ __launch_bounds__(GB) void k_og_tiles(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
+template <bool TRACE, bool KP>
+__global__ __launch_bounds__(GB) void k_og_tiles(const uint64_t *__restrict__ keys, const void *__restrict__ vals,
                                                  Last *__restrict__ tiles, TraceSlot *slot)
 {
     __shared__ Last red[GB / 64];
@@ -99,7 +112,7 @@ __global__ __launch_bounds__(GB) void k_og_tiles(const uint64_t *__restrict__ ke
     Last l{NONE, 0};
 #pragma unroll
     for (int r = 0; r < GPER; ++r) {
-        const uint64_t k = keys[t0 + r], v = vals[t0 + r];
+        const uint64_t k = keys[t0 + r], v = union_val<KP>(vals, t0 + r);
         const bool t = is_table(k);
         l.row = t ? (k >> 33) : l.row;
         l.val = t ? v : l.val;
@@ -159,8 +172,8 @@ __global__ __launch_bounds__(kScanThreads) void k_og_carry(const Last *__restric
 }
 
 // per row: request rows get (index << 1 | found, value) and mask 1
-template <bool TRACE>
-__global__ __launch_bounds__(GB) void k_og_rows(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
+template <bool TRACE, bool KP>
+__global__ __launch_bounds__(GB) void k_og_rows(const uint64_t *__restrict__ keys, const void *__restrict__ vals,
                                                 const Last *__restrict__ carry, uint64_t *__restrict__ c0,
                                                 uint64_t *__restrict__ c1, uint8_t *__restrict__ mask,
                                                 TraceSlot *slot)
@@ -173,7 +186,7 @@ __global__ __launch_bounds__(GB) void k_og_rows(const uint64_t *__restrict__ key
 #pragma unroll
     for (int r = 0; r < GPER; ++r) {
         k[r] = keys[t0 + r];
-        v[r] = vals[t0 + r];
+        v[r] = union_val<KP>(vals, t0 + r);
         const bool t = is_table(k[r]);
         l.row = t ? (k[r] >> 33) : l.row;
         l.val = t ? v[r] : l.val;
@@ -286,16 +299,23 @@ void op_gather_oblivious(vo_ctx *ctx, const vo_gather_oblivious_args &a)
     Last *tiles = (Last *)(base + o_t), *carry = (Last *)(base + o_c);
     uint64_t *cnt = (uint64_t *)(base + o_cnt);
     const bool tr = ctx->tracing;
+    // values of at most 4 bytes (a public width) ride as the u32 payload of
+    // 12-byte union records
+    const bool kp = eb <= 4;
+    auto pick = [&](auto f0, auto f1, auto k0, auto k1) { return kp ? (tr ? k1 : k0) : (tr ? f1 : f0); };
     launch(ctx, LaunchInfo{"gather_obl_build", grid_for(n_pad), dim3(256), 0, hash_mix(hash_mix(eb, a.n_table), a.n)},
-           tr ? k_og_build<true> : k_og_build<false>, a.table, a.n_table, eb, a.record_ids, a.n, n_pad, keys, vals,
-           trace_slot(ctx));
-    sort_kv(ctx, keys, vals, n_pad);
-    launch(ctx, LaunchInfo{"gather_obl_tiles", dim3(nt), dim3(GB), 0, n_pad}, tr ? k_og_tiles<true> : k_og_tiles<false>,
-           (const uint64_t *)keys, (const uint64_t *)vals, tiles, trace_slot(ctx));
+           pick(k_og_build<false, false>, k_og_build<true, false>, k_og_build<false, true>, k_og_build<true, true>),
+           a.table, a.n_table, eb, a.record_ids, a.n, n_pad, keys, (void *)vals, trace_slot(ctx));
+    if (kp) sort_kp_full(ctx, keys, reinterpret_cast<uint32_t *>(vals), n_pad);
+    else sort_kv(ctx, keys, vals, n_pad);
+    launch(ctx, LaunchInfo{"gather_obl_tiles", dim3(nt), dim3(GB), 0, n_pad},
+           pick(k_og_tiles<false, false>, k_og_tiles<true, false>, k_og_tiles<false, true>, k_og_tiles<true, true>),
+           (const uint64_t *)keys, (const void *)vals, tiles, trace_slot(ctx));
     launch(ctx, LaunchInfo{"gather_obl_carry", dim3(1), dim3(kScanThreads), 0, nt},
            tr ? k_og_carry<true> : k_og_carry<false>, (const Last *)tiles, nt, carry, trace_slot(ctx));
-    launch(ctx, LaunchInfo{"gather_obl_rows", dim3(nt), dim3(GB), 0, n_pad}, tr ? k_og_rows<true> : k_og_rows<false>,
-           (const uint64_t *)keys, (const uint64_t *)vals, (const Last *)carry, c0, c1, mask, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"gather_obl_rows", dim3(nt), dim3(GB), 0, n_pad},
+           pick(k_og_rows<false, false>, k_og_rows<true, false>, k_og_rows<false, true>, k_og_rows<true, true>),
+           (const uint64_t *)keys, (const void *)vals, (const Last *)carry, c0, c1, mask, trace_slot(ctx));
     compact_rows2(ctx, c0, c1, mask, n_pad, cnt);
     // the first n rows are the requests; reuse keys (and vals as the u32
     // payload column) for the final sort
