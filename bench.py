@@ -700,7 +700,7 @@ def bench_batch(args):
     tctx.sort_batch(keys, pay, items)
     torch.cuda.synchronize()
     fams = ["sort_tiny", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2", "sort_global_s3",
-            "sort_global_s4", "sort_global_s5", "sort_hybrid_s6", "sort_hybrid_s7", "sort_hybrid_s8"]
+            "sort_global_s4", "sort_global_s5", "sort_hybrid_s4", "sort_hybrid_s5", "sort_hybrid_s6", "sort_hybrid_s7", "sort_hybrid_s8"]
     st = _family_stats(tctx, fams)
     k_ms = sum(v[1] for v in st.values())
     tctx.close()
@@ -903,7 +903,7 @@ def main():
     fams = {}
     # one family per kernel instantiation, so each maps to one rocprof kernel name
     for f in (["sort_local", "sort_merge", "gen_sort"] + [f"sort_global_s{s}" for s in range(1, 6)] +
-              ["sort_hybrid_s6", "sort_hybrid_s7", "sort_hybrid_s8"]):
+              [f"sort_hybrid_s{s}" for s in range(4, 9)]):
         cnt, ms = ctx.kernel_stats(f)
         if cnt:
             fams[f] = (cnt, ms)

@@ -209,7 +209,7 @@ __device__ bool exec_sort_kp(const vo_sort_kp_args &a, char *lds, QDev *qd, uint
     if (!qbarrier(qd, nb, epoch, bar)) return false;
     for (int p = QT_LOG + 1; p <= n_log; ++p) {
         const int count = p - QT_LOG;
-        const int groups = plan_groups(count, kQueueSMax, kQueueHMax);
+        const int groups = plan_groups(count, kQueueSMax, kQueueHMax, 6); // widths built below: 1-4, 6-7
         int q = p - 1;
         for (int gi = 0; gi < groups; ++gi) {
             const int s = plan_width(count, groups, gi);
@@ -224,10 +224,11 @@ __device__ bool exec_sort_kp(const vo_sort_kp_args &a, char *lds, QDev *qd, uint
                 for (uint32_t w = blockIdx.x; w < tiles; w += nb)
                     dev::body_hybrid<RecKP, QT_LOG, QR_LOG, QBLOCK, 6, false>(b, map, p, q_lo, n_log, nullptr, lds, w);
                 break;
-            default:
+            case 7:
                 for (uint32_t w = blockIdx.x; w < tiles; w += nb)
                     dev::body_hybrid<RecKP, QT_LOG, QR_LOG, QBLOCK, 7, false>(b, map, p, q_lo, n_log, nullptr, lds, w);
                 break;
+            default: break; // plan_groups above yields only the widths built here
             }
             if (!qbarrier(qd, nb, epoch, bar)) return false;
         }

@@ -463,16 +463,17 @@ __host__ __device__ constexpr int plan_width(int count, int groups, int gi)
     return count / groups + (gi < count % groups ? 1 : 0);
 }
 
-// Fewest passes for `count` stages above the tile such that every width is
-// buildable: a register pass (<= s_max) or a hybrid pass (6 .. h_max).
-__host__ __device__ inline int plan_groups(int count, int s_max, int h_max)
+// Fewest groups of one level's `count` stages such that every group is a
+// register pass (width <= s_max) or a hybrid pass (h_min <= width <= h_max):
+// a caller passes the widths it has built.
+__host__ __device__ inline int plan_groups(int count, int s_max, int h_max, int h_min)
 {
     const int g_max = s_max > h_max ? s_max : h_max;
     for (int groups = (count + g_max - 1) / g_max;; ++groups) {
         bool ok = true;
         for (int gi = 0; gi < groups; ++gi) {
             const int s = plan_width(count, groups, gi);
-            if (!(s <= s_max || (s >= 6 && s <= h_max))) ok = false;
+            if (!(s <= s_max || (s >= h_min && s <= h_max))) ok = false;
         }
         if (ok) return groups;
     }
@@ -489,7 +490,7 @@ inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log,
     v.push_back({1, t, 0, t});
     for (int p = t_log + 1; p <= n_log; ++p) {
         const int count = p - t_log;
-        const int groups = plan_groups(count, s_max, h_max);
+        const int groups = plan_groups(count, s_max, h_max, 4); // launch path: hybrid widths 4..8
         int q = p - 1;
         for (int gi = 0; gi < groups; ++gi) {
             const int s = plan_width(count, groups, gi);
@@ -515,8 +516,10 @@ struct SortImpl {
         if constexpr (S <= H_MAX && T_LOG - S >= 5) {
             using TL = dev::TileCT<R, T_LOG, R_LOG, TRACE>;
             const size_t lds = TL::lds_bytes();
-            static const char *const names[] = {"sort_hybrid_s6", "sort_hybrid_s7", "sort_hybrid_s8"};
-            LaunchInfo li{names[S - 6], dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BLOCK), (uint32_t)lds,
+            static_assert(S >= 4 && S <= 8, "hybrid pass widths 4..8");
+            static const char *const names[] = {"sort_hybrid_s4", "sort_hybrid_s5", "sort_hybrid_s6", "sort_hybrid_s7",
+                                                "sort_hybrid_s8"};
+            LaunchInfo li{names[S - 4], dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BLOCK), (uint32_t)lds,
                           hash_mix(hash_mix(p, q_lo), hash_mix(S + 100, p_top))};
             auto k = dev::k_sort_hybrid<R, T_LOG, R_LOG, BLOCK, S, TRACE>;
             VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -583,6 +586,8 @@ struct SortImpl {
             case 2: global_pass<TRACE>(ctx, b, map, ps.p, ps.q_lo, ps.s, p_top, n_virtual); break;
             case 4:
                 switch (ps.s) {
+                case 4: launch_hybrid<TRACE, 4>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
+                case 5: launch_hybrid<TRACE, 5>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
                 case 6: launch_hybrid<TRACE, 6>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
                 case 7: launch_hybrid<TRACE, 7>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
                 case 8: launch_hybrid<TRACE, 8>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;

@@ -7,7 +7,10 @@ namespace vo {
 using namespace sortimpl;
 
 namespace {
-using SortKP = SortImpl<RecKP, 13, 4, 512, 5, 8>;
+// at most 3 register-only stages per global pass: groups of 4 to 8 stages
+// above the tile take the LDS-staged pass, which measured faster at 4 and 5
+// stages too (5455 vs 5162 and 5165 vs 4955 GB/s at 2^28)
+using SortKP = SortImpl<RecKP, 13, 4, 512, 3, 8>;
 using SortKV = SortImpl<RecKV, 12, 4, 256, 5, 7>;
 } // namespace
 

@@ -64,7 +64,7 @@ struct TimingRecord {
 // Public-size-only schedule parameters (chosen by the autotuner or defaults).
 struct SortTuning {
     int max_fused_stages = 5;  // stages per register-only global pass (1..5)
-    int max_hybrid_stages = 8; // stages per LDS-staged global pass (0 = off, 6..8)
+    int max_hybrid_stages = 8; // stages per LDS-staged global pass (0 = off, 4..8; used above max_fused_stages)
     int pipe_merge = 0;        // resident merge blocks with next-tile prefetch
 };
 
