@@ -10,8 +10,10 @@ using namespace sortimpl;
 
 namespace {
 
-using SortM1 = SortImpl<RecM1, 12, 4, 256, 5, 7>;
-using SortM2 = SortImpl<RecM2, 12, 4, 256, 4, 7>;
+// 1- and 2-word keys: register passes of at most 3 stages, wider groups as
+// LDS-staged hybrid passes (ORDER BY of two keys at 2^24: 5.66 -> 5.39 ms)
+using SortM1 = SortImpl<RecM1, 12, 4, 256, 3, 7>;
+using SortM2 = SortImpl<RecM2, 12, 4, 256, 3, 7>;
 using SortM4 = SortImpl<RecM4, 11, 3, 256, 3, 6>;
 using SortM8 = SortImpl<RecM8, 10, 2, 256, 2>;
 
