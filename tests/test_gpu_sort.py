@@ -144,6 +144,9 @@ def _rand_col(rng, t, n, distinct):
     ([6, 10, 5, 4], [False, True, True, False], 9000, 16384, 4),
     ([4, 4, 4, 4, 4, 4, 4, 4], [True, False] * 4, 2000, 2048, 2),
     ([64], [True], 70000, 1 << 17, 1 << 30),
+    # levels of 4-5 stages above the tile run as hybrid passes: 2 and 4 key words
+    ([5, 4], [True, False], 100000, 1 << 17, 10),
+    ([6, 10, 5, 4], [False, True, True, False], 70000, 1 << 17, 4),
 ])
 def test_sort_multi_bit_exact_with_ties(vctx, types, asc, n, npad, distinct):
     """Reference OP_SORT semantics incl. ties (decided by the network) and the
