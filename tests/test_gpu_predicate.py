@@ -65,7 +65,7 @@ def test_single_term_every_type_and_comparison(vctx, type_id, cmp):
 
 @pytest.mark.parametrize("conn", [_abi.CONN_AND, _abi.CONN_OR])
 @pytest.mark.parametrize("n,n_pad", [(0, 4), (1, 1), (2, 2), (3, 4), (1000, 1001), (70000, 1 << 17),
-                                     (1 << 16, 1 << 16)])
+                                     (1 << 16, 1 << 16), (300001, 300001)])
 def test_chains_with_nulls(vctx, conn, n, n_pad):
     """Up to eight terms of mixed types, some with NULL rows, against the
     left-deep chain of the reference's binary AND / OR."""

@@ -154,15 +154,17 @@ void op_predicate_mask(vo_ctx *ctx, const vo_predicate_args &a)
     uint64_t h = hash_mix(a.n_pad, a.n_terms);
     for (uint32_t j = 0; j < a.n_terms; ++j) h = hash_mix(h, ((uint64_t)t.type[j] << 1) | (t.valid[j] != nullptr));
     constexpr uint32_t kBlock = 256;
-    if (a.n_pad % 4 == 0) {
-        const uint64_t threads = a.n_pad / 4;
-        const dim3 grid((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((threads + kBlock - 1) / kBlock, 1u << 16)));
+    // 4 rows and one 32-bit mask store per thread when n_pad is a multiple of
+    // 4 (the mask buffer is then 4-byte aligned in length); else 1 row each.
+    const uint32_t rows = a.n_pad % 4 == 0 ? 4 : 1;
+    const uint64_t threads = a.n_pad / rows;
+    const dim3 grid((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((threads + kBlock - 1) / kBlock, 1u << 16)));
+    if (rows == 4)
         launch(ctx, LaunchInfo{"predicate_mask", grid, dim3(kBlock), 0, h}, k_predicate_mask<4>, t, a.n_terms,
                (uint32_t)(a.connective == VO_CONN_AND), a.n, a.n_pad, a.out_mask, trace_slot(ctx));
-    } else {
-        launch(ctx, LaunchInfo{"predicate_mask", dim3(1), dim3(kBlock), 0, h}, k_predicate_mask<1>, t, a.n_terms,
+    else
+        launch(ctx, LaunchInfo{"predicate_mask", grid, dim3(kBlock), 0, h}, k_predicate_mask<1>, t, a.n_terms,
                (uint32_t)(a.connective == VO_CONN_AND), a.n, a.n_pad, a.out_mask, trace_slot(ctx));
-    }
 }
 
 } // namespace vo
