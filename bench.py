@@ -21,6 +21,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# launch families of the sort passes: local, merge, register passes of 1-5
+# stages, LDS-staged passes of 4-8 stages
+SORT_FAMS = (["sort_local", "sort_merge"] + [f"sort_global_s{s}" for s in range(1, 6)] +
+             [f"sort_hybrid_s{s}" for s in range(4, 9)])
 METRIC = "oblivious sort Mrec/s at 2^28 (%HBM peak); served QPS + p50/p99 vs plain"
 REC_BYTES = 12  # u64 key + u32 payload, structure of arrays
 
@@ -296,8 +300,7 @@ def bench_join(args):
     ok = ng == int((exp > 0).sum()) and np.array_equal(got_s, sums[got_g])
     ok_sort = ng_sort == ng and np.array_equal(sums_sort, got_s)
     fams = ["join_build", "join_fill_tiles", "join_fill_rows", "join_fill_agg", "group_dense_out",
-            "compact_count", "compact_local", "compact_hybrid", "compact_global", "sort_local", "sort_merge",
-            "sort_global_s1", "sort_global_s2", "sort_global_s3", "sort_global_s4", "sort_hybrid_s6", "sort_hybrid_s7"]
+            "compact_count", "compact_local", "compact_hybrid", "compact_global"] + SORT_FAMS
     stats = _family_stats(ctx, fams)
     # the stats are the dense run's: its union sort moves (u64 key, u32
     # payload) records, 2 x 12 B per row and pass
@@ -379,8 +382,7 @@ def bench_join_pairs(args):
     l, r = ol[:n], orr[:n]
     ok = n == n_fact and bool(torch.all(d_dk[l] == d_fk[r])) and int(torch.unique(r).numel()) == n_fact
     ok = ok and bool(torch.all(d_dk[l][1:] >= d_dk[l][:-1]))
-    fams = ["join_pairs_build", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2", "sort_global_s3",
-            "sort_global_s4", "sort_hybrid_s6", "sort_hybrid_s7", "sort_hybrid_s8", "join_pairs_bwd_tiles",
+    fams = ["join_pairs_build"] + SORT_FAMS + ["join_pairs_bwd_tiles",
             "join_pairs_bwd_carry", "join_pairs_bwd_rows", "join_pairs_fwd_carry", "join_pairs_fwd_rows",
             "compact_count", "compact_scan", "compact_local", "compact_hybrid", "compact_global",
             "join_pairs_dist_init_w1", "join_pairs_dist_init_w2", "join_pairs_dist_level_w1",
@@ -493,8 +495,7 @@ def bench_gather(args):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     ok = bool(torch.all(out == table[ids]))
-    fams = ["gather_obl_build", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2", "sort_global_s3",
-            "sort_global_s4", "sort_hybrid_s6", "sort_hybrid_s7", "gather_obl_tiles", "gather_obl_carry", "gather_obl_rows",
+    fams = ["gather_obl_build"] + SORT_FAMS + ["gather_obl_tiles", "gather_obl_carry", "gather_obl_rows",
             "compact_count", "compact_scan", "compact_local", "compact_hybrid", "compact_global", "gather_obl_order",
             "gather_obl_out"]
     stats = _family_stats(ctx, fams)
@@ -562,8 +563,7 @@ def bench_order_by(args):
     same = ka[1:] == ka[:-1]
     ok = ok and bool(torch.all(~same | (kb[1:] <= kb[:-1])))
     ok = ok and bool(torch.all(a[:n] == ka)) and bool(torch.all(o1[:n] == i)) and bool(torch.all(o2[:n] == i * 3))
-    fams = ["sort_pack", "sort_unpack", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2",
-            "sort_global_s3", "sort_global_s4", "sort_global_s5", "sort_hybrid_s6", "sort_hybrid_s7", "permute"]
+    fams = ["sort_pack", "sort_unpack", "permute"] + SORT_FAMS
     stats = _family_stats(ctx, fams)
     _line("ORDER BY two keys + permute of two columns (Mrows/s)", n / dt / 1e6, "Mrows/s", args.steps,
           args.warmup, dt * 1e3, "int32/f64 keys, int64 indices",
@@ -611,9 +611,7 @@ def bench_shuffle(args):
     torch.cuda.synchronize()
     ctx.kernel_stats_reset()
     dt = timed(77 + args.warmup, False)
-    fams = ["shuffle_tags", "shuffle_unpack", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2",
-            "sort_global_s3", "sort_global_s4", "sort_global_s5", "sort_hybrid_s6", "sort_hybrid_s7",
-            "sort_hybrid_s8"]
+    fams = ["shuffle_tags", "shuffle_unpack"] + SORT_FAMS
     stats = _family_stats(ctx, fams)
     # the map after the last timed shuffle is a permutation of the rows
     pl = pay.long()
@@ -701,8 +699,7 @@ def bench_batch(args):
     tctx.kernel_stats_reset()
     tctx.sort_batch(keys, pay, items)
     torch.cuda.synchronize()
-    fams = ["sort_tiny", "sort_local", "sort_merge", "sort_global_s1", "sort_global_s2", "sort_global_s3",
-            "sort_global_s4", "sort_global_s5", "sort_hybrid_s4", "sort_hybrid_s5", "sort_hybrid_s6", "sort_hybrid_s7", "sort_hybrid_s8"]
+    fams = ["sort_tiny"] + SORT_FAMS
     st = _family_stats(tctx, fams)
     k_ms = sum(v[1] for v in st.values())
     tctx.close()

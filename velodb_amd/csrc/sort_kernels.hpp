@@ -502,12 +502,19 @@ inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log,
     return v;
 }
 
-template <class R, int T_LOG, int R_LOG, int BLOCK, int S_MAX, int H_MAX = 0>
+// S_LARGE / LARGE_LOG: from n_pad = 2^LARGE_LOG up, register passes take at
+// most S_LARGE stages (wider groups go to the hybrid pass); below, S_MAX.
+template <class R, int T_LOG, int R_LOG, int BLOCK, int S_MAX, int H_MAX = 0, int S_LARGE = S_MAX, int LARGE_LOG = 64>
 struct SortImpl {
     static_assert(T_LOG >= 8, "global passes need q_lo >= 8 (256-thread blocks)");
     static_assert(H_MAX == 0 || (T_LOG - H_MAX >= 5 && BLOCK << R_LOG == 1 << T_LOG), "hybrid tile shape");
+    static_assert(S_LARGE >= 1 && S_LARGE <= S_MAX, "register cap for large sorts");
 
-    static int smax(vo_ctx *ctx) { return std::max(1, std::min(S_MAX, ctx->tuning.max_fused_stages)); }
+    static int smax(vo_ctx *ctx, int n_log)
+    {
+        const int cap = n_log >= LARGE_LOG ? S_LARGE : S_MAX;
+        return std::max(1, std::min(cap, ctx->tuning.max_fused_stages));
+    }
     static int hmax(vo_ctx *ctx) { return std::min(H_MAX, ctx->tuning.max_hybrid_stages); }
 
     template <bool TRACE, int S>
@@ -559,7 +566,7 @@ struct SortImpl {
     static void run_t(vo_ctx *ctx, Bufs<R> b, int n_log, uint64_t n_valid, int fill, IMap map, int p_top,
                       uint64_t n_virtual)
     {
-        for (const Pass &ps : plan_passes(n_log, T_LOG, smax(ctx), R_LOG, hmax(ctx))) {
+        for (const Pass &ps : plan_passes(n_log, T_LOG, smax(ctx, n_log), R_LOG, hmax(ctx))) {
             switch (ps.kind) {
             case 0: {
                 LaunchInfo li{"sort_tiny", dim3(1), dim3(64), 0, hash_mix(n_log, fill)};
@@ -641,7 +648,7 @@ struct SortImpl {
             run_t<false>(ctx, b, c, 0, 1, map, c, count << c);
     }
 
-    static int passes(vo_ctx *ctx, int n_log) { return (int)plan_passes(n_log, T_LOG, smax(ctx), R_LOG, hmax(ctx)).size(); }
+    static int passes(vo_ctx *ctx, int n_log) { return (int)plan_passes(n_log, T_LOG, smax(ctx, n_log), R_LOG, hmax(ctx)).size(); }
 };
 
 // Record kinds and their tile shapes (LDS = 2^T_LOG * 17/16 * record bytes).

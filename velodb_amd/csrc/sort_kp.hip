@@ -7,10 +7,11 @@ namespace vo {
 using namespace sortimpl;
 
 namespace {
-// at most 3 register-only stages per global pass: groups of 4 to 8 stages
-// above the tile take the LDS-staged pass, which measured faster at 4 and 5
-// stages too (5455 vs 5162 and 5165 vs 4955 GB/s at 2^28)
-using SortKP = SortImpl<RecKP, 13, 4, 512, 3, 8>;
+// Register-only global passes take up to 5 stages below 2^27 records and up
+// to 3 from 2^27 on, where groups of 4 and 5 stages run faster as LDS-staged
+// passes (2^28: 5455 vs 5162 and 5165 vs 4955 GB/s); below 2^27 the register
+// passes win (2^25: 5.06 vs 5.13 ms per sort, 2^23: 1.107 vs 1.128 ms).
+using SortKP = SortImpl<RecKP, 13, 4, 512, 5, 8, 3, 27>;
 using SortKV = SortImpl<RecKV, 12, 4, 256, 5, 7>;
 } // namespace
 
