@@ -85,6 +85,19 @@ def test_chains_with_nulls(vctx, conn, n, n_pad):
         assert np.array_equal(got, want), n_terms
 
 
+def test_unaligned_mask_pointer(vctx):
+    """A mask that starts one byte into its buffer (n_pad a multiple of 4)
+    takes the byte-store path and leaves the byte before it alone."""
+    import torch
+    n = 4096
+    col = np.random.default_rng(3).integers(-50, 50, n).astype(np.int64)
+    buf = torch.full((n + 8,), 0xEE, dtype=torch.uint8, device="cuda:0")
+    vctx.predicate_mask([(to_dev(col), _abi.T_BIGINT, _abi.CMP_GE, 0)], buf[1:1 + n], n, n)
+    got = buf.cpu().numpy()
+    assert got[0] == 0xEE and got[1 + n] == 0xEE
+    assert np.array_equal(got[1:1 + n], (col >= 0).astype(np.uint8))
+
+
 def test_reference_expression_vectors(vctx):
     """tests/expression/test_expression_batch.cpp:46-48, 97-184 of the
     reference: val > 20 and (val > 10) AND active."""

@@ -155,8 +155,9 @@ void op_predicate_mask(vo_ctx *ctx, const vo_predicate_args &a)
     for (uint32_t j = 0; j < a.n_terms; ++j) h = hash_mix(h, ((uint64_t)t.type[j] << 1) | (t.valid[j] != nullptr));
     constexpr uint32_t kBlock = 256;
     // 4 rows and one 32-bit mask store per thread when n_pad is a multiple of
-    // 4 (the mask buffer is then 4-byte aligned in length); else 1 row each.
-    const uint32_t rows = a.n_pad % 4 == 0 ? 4 : 1;
+    // 4 and the mask starts on a 4-byte boundary; else 1 row and byte stores.
+    const bool words = a.n_pad % 4 == 0 && reinterpret_cast<uintptr_t>(a.out_mask) % 4 == 0;
+    const uint32_t rows = words ? 4 : 1;
     const uint64_t threads = a.n_pad / rows;
     const dim3 grid((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((threads + kBlock - 1) / kBlock, 1u << 16)));
     if (rows == 4)
