@@ -127,3 +127,20 @@ def test_payload_only_shuffle_trace_value_independent(vctx):
     (t1, l1), (t2, l2) = run(np.zeros(n, np.uint32)), run(np.random.default_rng(1).integers(0, 2**32, n, np.uint32))
     assert t1.n_launches == t2.n_launches > 0 and l1 == l2
     assert t1.launch_digest == t2.launch_digest and t1.address_digest == t2.address_digest
+
+
+@pytest.mark.parametrize("tuning", [0x3 | (8 << 4), 0x5 | (8 << 4), 0x2 | (6 << 4)])
+def test_kv_sort_schedules_give_the_same_shuffle(vctx, tuning):
+    """The 16-byte sort behind perm_out with register passes capped at 3
+    (groups of 4-7 stages as LDS-staged passes), 5, or 2 (hybrid up to 5):
+    every schedule applies the same network, so payload and permutation
+    equal the default context's at 2^20 and 2^17."""
+    import torch
+    import velodb_amd
+    print(f"kv schedule tuning={tuning:#x}", flush=True)
+    with velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, tuning=tuning) as ctx:
+        for n_pad, seed in [(1 << 20, 99), (1 << 17, 5)]:
+            payload = np.random.default_rng(seed).integers(0, 2 ** 32, n_pad, dtype=np.uint64).astype(np.uint32)
+            want_p, want_perm = _shuffle(vctx, payload, seed)
+            got_p, got_perm = _shuffle(ctx, payload, seed)
+            assert np.array_equal(got_p, want_p) and np.array_equal(got_perm, want_perm)

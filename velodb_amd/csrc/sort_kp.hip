@@ -12,7 +12,9 @@ namespace {
 // passes (2^28: 5455 vs 5162 and 5165 vs 4955 GB/s); below 2^27 the register
 // passes win (2^25: 5.06 vs 5.13 ms per sort, 2^23: 1.107 vs 1.128 ms).
 using SortKP = SortImpl<RecKP, 13, 4, 512, 5, 8, 3, 27>;
-using SortKV = SortImpl<RecKV, 12, 4, 256, 5, 7>;
+// the 16-byte sort likewise: at 2^28 its 4-5 stage register passes ran at
+// 4.80-4.99 TB/s against 5.11-5.36 for its LDS-staged passes
+using SortKV = SortImpl<RecKV, 12, 4, 256, 5, 7, 3, 27>;
 } // namespace
 
 void op_sort_kp(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad)
