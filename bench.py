@@ -33,8 +33,9 @@ def cpu_baseline_parallel(log2n=28, runs=3):
     """The multi-threaded restatement of obliviator's recursive parallel
     bitonic (obliviator/join/enclave/bitonic.c:72-242) on the host cores the
     box gives this GPU (at most 16 threads), on the full headline workload:
-    2^log2n records from the same generator, median of `runs` sorts, each
-    on a fresh copy (the copy is not timed)."""
+    2^log2n records from the same generator (seed 1), median of `runs`
+    sorts, each on a fresh copy (the copy is not timed).  Returns the
+    baseline record and the sorted (keys, payload) of the last run."""
     import numpy as np
 
     import oracle
@@ -55,8 +56,8 @@ def cpu_baseline_parallel(log2n=28, runs=3):
     rate = n / float(np.median(times)) / 1e6
     return {"value": round(rate, 3), "unit": "Mrec/s", "cores": threads, "kind": "port",
             "sample": f"oracle multi-threaded recursive bitonic (obliviator bitonic.c structure) on the full "
-                      f"2^{log2n}-record workload, {threads} threads, median of {runs} runs "
-                      f"({', '.join(f'{t:.2f}' for t in times)} s); sorted: {ok}"}
+                      f"2^{log2n}-record workload (seed 1), {threads} threads, median of {runs} runs "
+                      f"({', '.join(f'{t:.2f}' for t in times)} s); sorted: {ok}"}, wk, wp
 
 
 def cpu_baseline(budget_s=12.0):
@@ -766,19 +767,26 @@ def bench_queue(args):
     print(json.dumps(d), flush=True)
 
 
-def bench_serve(args):
-    """Served mix (50% filter+SUM, 30% join+group-by, 20% sort) over 12 tables
-    of each kind with row counts log-uniform in [1e4, 1e7]; all queries
-    arriving at once, then Poisson arrivals at 50/200/800 QPS; 16 slots vs the
-    plain one-query-at-a-time loop, 200 GB admission budget."""
+def served_workload(slots=16, rates=(0, 800), n_tables=12, check_queries=200, log=None):
+    """The served mix (50% filter+SUM, 30% join+group-by, 20% sort) over
+    n_tables tables of each kind with row counts log-uniform in [1e4, 1e7],
+    200 GB admission budget.  For each offered load (0 = every query arrives
+    at once) the same query list runs three ways: served (`slots` concurrent
+    slots over the resident tables), plain (one query at a time, inputs
+    shipped from pinned host memory per query: the reference engine's path)
+    and plain_resident (one at a time over the resident tables).  The first
+    `check_queries` queries of the first load then run served again with
+    their whole results copied out, each checked against the oracle's closed
+    form of its table."""
     import numpy as np
 
+    import oracle
     import velodb_amd
     S = velodb_amd.Server
-    srv = S(n_slots=args.slots, hbm_budget_bytes=200 << 30)
+    srv = S(n_slots=slots, hbm_budget_bytes=200 << 30)
     rng = np.random.default_rng(0)
     tables = {"f": [], "j": [], "s": []}
-    for i in range(12):
+    for i in range(n_tables):
         n = int(np.exp(rng.uniform(np.log(1e4), np.log(1e7))))
         tables["f"].append(srv.add_fact_table(100 + i, n))
         tables["j"].append(srv.add_join_tables(200 + i, max(1, n // 5), n - n // 5))
@@ -810,27 +818,87 @@ def bench_serve(args):
 
     srv.run(mix(64, 0, 99))  # warm-up
     srv.run(mix(16, 0, 98), n_slots=1, transfer=True)
-    for rate in [int(x) for x in args.rates.split(",")]:
+    out = {"slots": slots, "tables_per_kind": n_tables, "loads": []}
+    first_qs = None
+    all_equal = True
+    for rate in rates:
         count = 600 if rate == 0 else max(200, rate * 2)
         qs = mix(count, rate, 7 + rate)
-        # plain: one query at a time, inputs shipped host->device per query
-        # (the reference engine's path); plain_resident: one at a time over the
-        # resident tables; served: 16 slots over the resident tables
+        first_qs = first_qs or qs
         rp, sp = srv.run(qs, n_slots=1, transfer=True)
         rr, sr = srv.run(qs, n_slots=1)
         rb, sb = srv.run(qs)
         same = [list(a.r) for a in rp] == [list(b.r) for b in rb] == [list(c.r) for c in rr]
-        _line(f"served queries/s (continuous batching, {args.slots} slots)", sb.qps, "QPS", count, 64,
-              1e3 / max(sb.qps, 1e-9),
-              "int64/int128",
+        all_equal = all_equal and same
+        ld = {"offered_qps": rate if rate else "all at t=0", "queries": count,
+              "served": summ(sb, rb, qs), "plain": summ(sp, rp, qs), "plain_resident": summ(sr, rr, qs),
+              "throughput_ratio": round(sb.qps / max(sp.qps, 1e-9), 2),
+              "throughput_ratio_vs_plain_resident": round(sb.qps / max(sr.qps, 1e-9), 2),
+              "p99_ratio_plain_over_served": round(sp.p99_s / max(sb.p99_s, 1e-12), 2),
+              "results_equal": same}
+        out["loads"].append(ld)
+        if log:
+            log(ld)
+
+    # oracle check of whole results, served concurrently
+    qs = first_qs[:check_queries]
+    want = {}
+    for k, t, _ in qs:
+        if (k, t) in want:
+            continue
+        cols = [srv.table_column(k, t, c) for c in range(2 if k == S.SORT else 4)]
+        if k == S.FILTER_SUM:
+            want[(k, t)] = (len(cols[0]),) + oracle.filter_closed_form(*cols)
+        elif k == S.JOIN_GROUPSUM:
+            want[(k, t)] = (len(cols[2]),) + oracle.join_groupsum_closed_form(*cols, max_groups=32)
+        else:
+            want[(k, t)] = (len(cols[0]),) + oracle.sort_closed_form(*cols)
+    outs = [np.zeros(srv.query_out_bytes(k, t), np.uint8) for k, t, _ in qs]
+    res, _ = srv.run(qs, outputs=outs)
+    ok = all(r.status == 0 for r in res)
+    for (k, t, _), o in zip(qs, outs):
+        w = want[(k, t)]
+        n = w[0]
+        if k == S.FILTER_SUM:
+            m, sm, pr, di = w[1:]
+            h = o[:32].view(np.uint64)
+            c = o[32:].view(np.int64)
+            ok = ok and int(h[0]) == m and (int(h[2]) | (int(h[3]) << 64)) == sm
+            ok = ok and np.array_equal(c[:m], pr) and np.array_equal(c[n:n + m], di)
+        elif k == S.JOIN_GROUPSUM:
+            g, sm = w[1:]
+            G = int(o[768:776].view(np.uint64)[0])
+            ok = ok and G == len(g) and np.array_equal(o[:256].view(np.int32)[:G], g)
+            ok = ok and np.array_equal(o[256:768].view(np.int64)[:G], sm)
+        else:
+            kk, pp = w[1:]
+            ok = ok and np.array_equal(o[:8 * n].view(np.uint64), kk) and np.array_equal(o[8 * n:].view(np.uint32), pp)
+    out["results_equal"] = all_equal
+    out["oracle_checked_queries"] = len(qs)
+    out["results_equal_oracle"] = bool(ok)
+    srv.close()
+    return out
+
+
+def bench_serve(args):
+    """One line per offered load (the served loop vs the plain loops)."""
+    rates = tuple(int(x) for x in args.rates.split(","))
+
+    def log(ld):
+        sb = ld["served"]
+        _line(f"served queries/s (continuous batching, {args.slots} slots)", sb["qps"], "QPS", ld["queries"], 64,
+              1e3 / max(sb["qps"], 1e-9), "int64/int128",
               {"workload": "served mix 50% filter+SUM / 30% join+group-by / 20% sort, tables log-uniform "
                            "in [1e4, 1e7] rows, 200 GB admission budget",
-               "offered_qps": rate if rate else "all at t=0", "queries": count},
-              served=summ(sb, rb, qs), plain=summ(sp, rp, qs), plain_resident=summ(sr, rr, qs),
-              throughput_ratio=round(sb.qps / max(sp.qps, 1e-9), 2),
-              throughput_ratio_vs_plain_resident=round(sb.qps / max(sr.qps, 1e-9), 2),
-              results_equal=same)
-    srv.close()
+               "offered_qps": ld["offered_qps"], "queries": ld["queries"]},
+              **{k: v for k, v in ld.items() if k not in ("offered_qps", "queries")})
+
+    res = served_workload(args.slots, rates, log=log)
+    print(json.dumps({"served_oracle_check": {k: res[k] for k in ("oracle_checked_queries",
+                                                                  "results_equal_oracle", "results_equal")}}),
+          flush=True)
+    if not (res["results_equal"] and res["results_equal_oracle"]):
+        sys.exit(1)
 
 
 def main():
@@ -840,6 +908,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-served", action="store_true", help="sort line without the served sub-object")
     ap.add_argument("--tuning", type=int, default=0)
     ap.add_argument("--workload", choices=["sort", "filter", "join", "join_pairs", "batch", "serve", "queue",
                                            "transfer", "gather", "order_by", "shuffle"],
@@ -918,12 +987,42 @@ def main():
     sort_ms = sum(v[1] for v in sort_fams.values()) / args.steps
     per_pass = {f: round(alg_bytes / (v[1] / v[0] * 1e-3) / 1e9, 1) for f, v in sort_fams.items()}
 
-    # sanity: every batch ends sorted (device-side check)
+    # parity of every timed batch, on the device: ascending by (key, payload)
+    # as unsigned integers, the payloads a permutation of the row indices,
+    # and every key the input key of its payload's row (regenerated)
     ok = True
-    for keys, _ in batches:
-        k = keys.view(torch.int64)
-        ok = ok and bool(torch.all((k[1:] ^ (-(2 ** 63))) >= (k[:-1] ^ (-(2 ** 63)))).item())
-    del batches
+    ref_k = torch.empty(n, dtype=torch.int64, device=dev)
+    ref_p = torch.empty(n, dtype=torch.int32, device=dev)
+    top = -(2 ** 63)
+    for b, (keys, pay) in enumerate(batches):
+        ku = keys ^ top
+        pl = pay.long()
+        asc = (ku[1:] > ku[:-1]) | ((ku[1:] == ku[:-1]) & (pl[1:] > pl[:-1]))
+        ok = ok and bool(torch.all(asc).item())
+        ok = ok and bool(torch.equal(torch.sort(pay)[0], torch.arange(n, dtype=torch.int32, device=dev)))
+        ctx.gen_sort_input(1000 + args.warmup + b, ref_k, ref_p, n)
+        ok = ok and bool(torch.equal(ref_k[pl], keys))
+        del ku, pl, asc
+    del ref_k, ref_p
+
+    # the GPU against the CPU baseline's own output: seed 1 sorted on both
+    cpu_equal = None
+    cpu_par = None
+    if not args.no_cpu_baseline:
+        import numpy as np
+        keys, pay = batches[0]
+        ctx.gen_sort_input(1, keys, pay, n)
+        ctx.sort_kp_async(keys, pay, n, n)
+        torch.cuda.synchronize()
+        gk = keys.cpu().numpy().view(np.uint64)
+        gp = pay.cpu().numpy().view(np.uint32)
+        del batches
+        torch.cuda.empty_cache()
+        cpu_par, ck, cp = cpu_baseline_parallel(args.log2n)
+        cpu_equal = bool(np.array_equal(gk, ck) and np.array_equal(gp, cp))
+        del gk, gp, ck, cp
+    else:
+        del batches
 
     line = {
         "metric": METRIC,
@@ -942,7 +1041,8 @@ def main():
         "config": {"workload": f"oblivious bitonic sort of 2^{args.log2n} records (u64 key, u32 payload), "
                                f"ascending by key, ties by payload; {n_batches} input batches resident in HBM",
                    "n": n, "record_bytes": REC_BYTES, "layout": "SoA", "hbm_passes": passes,
-                   "sort_ms_per_step": round(sort_ms, 3), "output_sorted": ok},
+                   "sort_ms_per_step": round(sort_ms, 3), "output_sorted_permutation_keys_travel": ok,
+                   "bit_exact_vs_cpu_baseline_seed1": cpu_equal},
         "pct_hbm_peak": round(100 * achieved / HBM_PEAK_GBS, 1),
         "per_pass_GBps": per_pass,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -951,9 +1051,18 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
-        line["cpu_baseline_parallel"] = cpu_baseline_parallel(args.log2n)
+        line["cpu_baseline_parallel"] = cpu_par
     ctx.close()
+    served_ok = True
+    if not args.no_served:
+        # the served half of the metric: saturation and 800 QPS, oracle-checked
+        sv = served_workload(args.slots, (0, 800))
+        line["served"] = sv
+        served_ok = sv["results_equal"] and sv["results_equal_oracle"]
     print(json.dumps(line), flush=True)
+    if not ok or cpu_equal is False or not served_ok:
+        print("bench.py: an output check failed (see the line above)", file=sys.stderr)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

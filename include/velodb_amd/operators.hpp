@@ -37,10 +37,10 @@
  * Differences from the reference, by design of the oblivious path:
  *   - FilterCompactionOperator moves the selected rows with the ORCompact
  *     routing network (every row read and written regardless of the mask)
- *     instead of scatter + gather, and takes the mask column's index instead
- *     of looking up "$_mask" by name (there is no schema here);
- *   - SortOperator returns the whole sorted batch from its first next()
- *     rather than slices of MAX_BATCH_SIZE;
+ *     instead of scatter + gather; with a Schema it finds the mask column by
+ *     the reference's "$_mask" name, without one it takes the column index;
+ *   - a Schema carries names and types only (ColumnInfo), the catalog's
+ *     other duties are out of this path;
  *   - failures come back as Result::failure with the library's message;
  *     TaskManager methods throw ExecutionError, as CHECKED_CALL_THROW does
  *     (include/cuda/helper.hpp:42-48).
@@ -55,13 +55,16 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <optional>
 #include <random>
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <variant>
 #include <vector>
@@ -297,6 +300,9 @@ private:
     uint8_t *bitmap_ = nullptr;
 };
 
+// Largest batch an operator hands downstream (include/common/constants.hpp.in:16).
+constexpr size_t MAX_BATCH_SIZE = size_t(1) << 23;
+
 class RowBatch {
 public:
     RowBatch() = default;
@@ -305,12 +311,91 @@ public:
     size_t getColumnCount() const { return cols_.size(); }
     Column &getColumn(size_t i) { return cols_.at(i); }
     const Column &getColumn(size_t i) const { return cols_.at(i); }
+    std::vector<Column> &getColumns() { return cols_; }
     void setRowCount(size_t n) { rows_ = n; }
+
+    // Rows [start, end) as a new batch with its own device columns of
+    // power-of-two capacity (RowBatch::slice, src/catalog/row_batch.cpp:67-79).
+    RowBatch slice(size_t start, size_t end) const
+    {
+        if (start > end || end > rows_) throw ExecutionError("Invalid slice range");
+        const size_t n = end - start, cap = next_pow2(n);
+        std::vector<Column> out;
+        out.reserve(cols_.size());
+        for (const Column &c : cols_) {
+            const size_t w = type_width(c.getTypeId());
+            Column o(c.getTypeId(), cap, c.bitmap() != nullptr);
+            if (n) {
+                hip_check(hipMemcpy(o.rawData(), static_cast<const char *>(c.rawData()) + start * w, n * w,
+                                    hipMemcpyDeviceToDevice),
+                          "hipMemcpy");
+                if (c.bitmap())
+                    hip_check(hipMemcpy(o.rawBitmapData(), c.bitmap() + start, n, hipMemcpyDeviceToDevice),
+                              "hipMemcpy");
+            }
+            out.push_back(std::move(o));
+        }
+        return RowBatch(std::move(out), n);
+    }
 
 private:
     std::vector<Column> cols_;
     size_t rows_ = 0;
 };
+
+// ----------------------------------------------------------------- Schema
+
+// A column's name and type (ColumnInfo, include/catalog/column.hpp:15-40).
+class ColumnInfo {
+public:
+    ColumnInfo(std::string name, uint32_t type_id, bool is_nullable = true, bool is_unique = false,
+               bool is_primary_key = false)
+        : name_(std::move(name)), type_(type_id), nullable_(is_nullable), unique_(is_unique), pk_(is_primary_key)
+    {
+    }
+    const std::string &getName() const { return name_; }
+    uint32_t getTypeId() const { return type_; }
+    bool isNullable() const { return nullable_; }
+    bool isUnique() const { return unique_; }
+    bool isPrimaryKey() const { return pk_; }
+
+private:
+    std::string name_;
+    uint32_t type_;
+    bool nullable_, unique_, pk_;
+};
+
+// An operator's output columns (Schema, include/catalog/schema.hpp:13-45).
+class Schema {
+public:
+    Schema() = default;
+    explicit Schema(std::vector<ColumnInfo> columns)
+    {
+        for (ColumnInfo &c : columns) addColumnInfo(std::move(c));
+    }
+    void addColumnInfo(ColumnInfo column)
+    {
+        index_[column.getName()] = cols_.size();
+        cols_.push_back(std::move(column));
+    }
+    const ColumnInfo &getColumnInfo(size_t index) const { return cols_.at(index); }
+    const ColumnInfo &getColumnInfo(const std::string &name) const { return cols_.at(getColumnIndex(name)); }
+    size_t getColumnIndex(const std::string &name) const
+    {
+        auto it = index_.find(name);
+        if (it == index_.end()) throw ExecutionError("Column not found: " + name);
+        return it->second;
+    }
+    size_t getColumnCount() const { return cols_.size(); }
+    bool hasColumn(const std::string &name) const { return index_.count(name) != 0; }
+    Schema clone() const { return Schema(cols_); }
+
+private:
+    std::vector<ColumnInfo> cols_;
+    std::unordered_map<std::string, size_t> index_;
+};
+
+class Table; // source tables of a join (catalog), passed through unused as in the reference
 
 inline Column Column::fromHost(TaskManager &tm, uint32_t type_id, const void *values, size_t n, const uint8_t *valid)
 {
@@ -335,59 +420,154 @@ std::vector<T> Column::toHost(TaskManager &tm, size_t n) const
 class AbstractOperator {
 public:
     explicit AbstractOperator(ExecutionContext &context) : context_(context) {}
+    AbstractOperator(ExecutionContext &context, Schema output_schema)
+        : context_(context), output_schema_(std::move(output_schema))
+    {
+    }
     virtual ~AbstractOperator() = default;
     AbstractOperator(const AbstractOperator &) = delete;
     AbstractOperator &operator=(const AbstractOperator &) = delete;
     // an empty batch (no rows) ends the stream
     virtual Result<RowBatch> next() = 0;
+    const Schema &getOutputSchema() const { return output_schema_; }
 
 protected:
+    // Every batch of the child, end to end, in one batch of power-of-two
+    // capacity (AbstractOperator::collectBatches, src/operator/abstract_operator.cpp:17-81):
+    // the first batch is kept when it is the only one; otherwise each column
+    // is copied batch after batch into a new buffer.  A column that has a
+    // null bitmap in any batch gets one, with the rows of the batches
+    // without one valid.  A child failure throws ExecutionError, as the
+    // reference does.
+    RowBatch collectBatches(AbstractOperator &child)
+    {
+        std::vector<RowBatch> buffer;
+        size_t n_rows = 0;
+        for (;;) {
+            auto r = child.next();
+            if (!r) throw ExecutionError("Error fetching batch from child operator: " + r.error());
+            RowBatch b = std::move(r).value();
+            if (b.getRowCount() == 0) break;
+            n_rows += b.getRowCount();
+            buffer.push_back(std::move(b));
+        }
+        if (buffer.empty()) return RowBatch();
+        if (buffer.size() == 1) return std::move(buffer.front());
+        const size_t cap = next_pow2(n_rows), n_cols = buffer.front().getColumnCount();
+        std::vector<Column> cols;
+        cols.reserve(n_cols);
+        for (size_t c = 0; c < n_cols; ++c) {
+            const uint32_t type = buffer.front().getColumn(c).getTypeId();
+            bool nullable = false;
+            for (RowBatch &b : buffer) {
+                if (b.getColumnCount() != n_cols || b.getColumn(c).getTypeId() != type)
+                    throw ExecutionError("collectBatches: batches differ in their columns");
+                nullable = nullable || b.getColumn(c).bitmap() != nullptr;
+            }
+            Column out(type, cap, nullable);
+            const size_t w = type_width(type);
+            size_t off = 0;
+            for (RowBatch &b : buffer) {
+                const Column &src = b.getColumn(c);
+                const size_t rows = b.getRowCount();
+                hip_check(hipMemcpyAsync(static_cast<char *>(out.rawData()) + off * w, src.rawData(), rows * w,
+                                         hipMemcpyDeviceToDevice, nullptr),
+                          "hipMemcpyAsync");
+                if (nullable) {
+                    if (src.bitmap())
+                        hip_check(hipMemcpyAsync(out.rawBitmapData() + off, src.bitmap(), rows,
+                                                 hipMemcpyDeviceToDevice, nullptr),
+                                  "hipMemcpyAsync");
+                    else
+                        hip_check(hipMemsetAsync(out.rawBitmapData() + off, 1, rows, nullptr), "hipMemsetAsync");
+                }
+                off += rows;
+            }
+            cols.push_back(std::move(out));
+        }
+        hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+        return RowBatch(std::move(cols), n_rows);
+    }
+
     ExecutionContext &context_;
+    Schema output_schema_;
 };
 
-// Source of one prepared batch (stands in for a scan of device columns).
+// Source of prepared batches, one per next() (stands in for a scan of
+// device columns).
 class BatchSourceOperator : public AbstractOperator {
 public:
-    BatchSourceOperator(ExecutionContext &context, RowBatch batch) : AbstractOperator(context), batch_(std::move(batch))
+    BatchSourceOperator(ExecutionContext &context, RowBatch batch) : AbstractOperator(context)
+    {
+        batches_.push_back(std::move(batch));
+    }
+    BatchSourceOperator(ExecutionContext &context, std::vector<RowBatch> batches)
+        : AbstractOperator(context), batches_(std::move(batches))
     {
     }
     Result<RowBatch> next() override
     {
-        if (done_) return Result<RowBatch>::success(RowBatch());
-        done_ = true;
-        return Result<RowBatch>::success(std::move(batch_));
+        if (at_ >= batches_.size()) return Result<RowBatch>::success(RowBatch());
+        return Result<RowBatch>::success(std::move(batches_[at_++]));
     }
 
 private:
-    RowBatch batch_;
-    bool done_ = false;
+    std::vector<RowBatch> batches_;
+    size_t at_ = 0;
 };
 
-// Oblivious multi-column sort: OP_SORT over the key columns (pads written in
-// place, int64 row indices produced), then OP_PERMUTE of every other column
-// and of every null bitmap (sort_operator.cpp:44-123).  One input batch.
+// Oblivious multi-column sort: every child batch gathered into one
+// (collectBatches), OP_SORT over the key columns (pads written in place,
+// int64 row indices produced), then OP_PERMUTE of every other column and of
+// every null bitmap (sort_operator.cpp:21-134).  The sorted rows come back
+// in slices of at most MAX_BATCH_SIZE rows, one per next(), then an empty
+// batch (sort_operator.cpp:24-33, 127-133).
 class SortOperator : public AbstractOperator {
 public:
+    SortOperator(ExecutionContext &context, Schema output_schema, std::unique_ptr<AbstractOperator> child,
+                 std::vector<size_t> order_indices, std::vector<bool> ascending_flags)
+        : AbstractOperator(context, std::move(output_schema)), child_(std::move(child)),
+          order_(std::move(order_indices)), asc_(std::move(ascending_flags))
+    {
+    }
     SortOperator(ExecutionContext &context, std::unique_ptr<AbstractOperator> child, std::vector<size_t> order_indices,
                  std::vector<bool> ascending_flags)
-        : AbstractOperator(context), child_(std::move(child)), order_(std::move(order_indices)),
-          asc_(std::move(ascending_flags))
+        : SortOperator(context, Schema(), std::move(child), std::move(order_indices), std::move(ascending_flags))
     {
     }
 
+    // rows per output slice (MAX_BATCH_SIZE unless set smaller, e.g. by tests)
+    void setMaxBatchSize(size_t rows) { max_batch_ = std::max<size_t>(1, rows); }
+
     Result<RowBatch> next() override
     {
-        if (sorted_) return Result<RowBatch>::success(RowBatch());
-        sorted_ = true;
-        auto in = child_->next();
-        if (!in) return in;
-        RowBatch batch = std::move(in).value();
+        if (!sorted_) {
+            try {
+                RowBatch gathered = collectBatches(*child_);
+                sorted_ = true;
+                if (gathered.getRowCount() == 0) return Result<RowBatch>::success(RowBatch());
+                run(gathered);
+                sorted_batch_ = std::move(gathered);
+                offset_ = 0;
+            } catch (const std::exception &e) {
+                sorted_ = true;
+                return Result<RowBatch>::failure(e.what());
+            }
+        }
+        const size_t total = sorted_batch_.getRowCount();
+        if (offset_ >= total) return Result<RowBatch>::success(RowBatch());
+        if (offset_ == 0 && total <= max_batch_) { // one slice: the batch itself
+            offset_ = total;
+            return Result<RowBatch>::success(std::move(sorted_batch_));
+        }
+        const size_t end = std::min(offset_ + max_batch_, total);
         try {
-            run(batch);
+            RowBatch slice = sorted_batch_.slice(offset_, end);
+            offset_ = end;
+            return Result<RowBatch>::success(std::move(slice));
         } catch (const std::exception &e) {
             return Result<RowBatch>::failure(e.what());
         }
-        return Result<RowBatch>::success(std::move(batch));
     }
 
 private:
@@ -445,6 +625,9 @@ private:
     std::vector<size_t> order_;
     std::vector<bool> asc_;
     bool sorted_ = false;
+    RowBatch sorted_batch_; // the whole sorted result, handed out in slices
+    size_t offset_ = 0;
+    size_t max_batch_ = MAX_BATCH_SIZE;
 };
 
 // Predicate terms evaluated on the device (OP_PREDICATE_MASK): a column
@@ -482,8 +665,30 @@ struct PredicateTerm {
 // column (BOOLEAN, non-zero = selected; the scan's "$_mask") or a predicate
 // evaluated on the device, which the reference evaluates on the host
 // (seq_scan_operator.cpp:38-45).
+// Batches stream through one at a time; while one is compacted on the
+// device, the next is fetched from the child on another thread, as the
+// reference's prefetch does (filter_compaction_operator.cpp:38-62, 186-233).
 class FilterCompactionOperator : public AbstractOperator {
 public:
+    // The reference's form: the mask column is the schema's "$_mask" (or a
+    // name ending in ".$_mask"); compact_columns defaults to every column and
+    // must match the schema (filter_compaction_operator.cpp:16-28, 69-82).
+    FilterCompactionOperator(ExecutionContext &context, Schema output_schema, std::unique_ptr<AbstractOperator> child,
+                             std::vector<bool> compact_columns = {})
+        : AbstractOperator(context, std::move(output_schema)), child_(std::move(child)), mask_index_(kNoMask),
+          compact_(std::move(compact_columns))
+    {
+        const size_t nc = output_schema_.getColumnCount();
+        for (size_t i = 0; i < nc && mask_index_ == kNoMask; ++i) {
+            const std::string &name = output_schema_.getColumnInfo(i).getName();
+            if (name == "$_mask" || (name.size() > 7 && name.compare(name.size() - 7, 7, ".$_mask") == 0))
+                mask_index_ = i;
+        }
+        if (mask_index_ == kNoMask) mask_index_ = output_schema_.getColumnIndex("$_mask"); // throws, as the reference
+        if (compact_.empty()) compact_.assign(nc, true);
+        if (compact_.size() != nc) throw ExecutionError("Compaction mask must match the output schema");
+    }
+
     FilterCompactionOperator(ExecutionContext &context, std::unique_ptr<AbstractOperator> child, size_t mask_index,
                              std::vector<bool> compact_columns = {})
         : AbstractOperator(context), child_(std::move(child)), mask_index_(mask_index),
@@ -499,12 +704,26 @@ public:
     {
     }
 
+    ~FilterCompactionOperator() override
+    {
+        if (prefetch_.valid()) prefetch_.wait();
+    }
+
     Result<RowBatch> next() override
     {
-        auto in = child_->next();
+        Result<RowBatch> in = Result<RowBatch>::success(RowBatch());
+        if (first_call_) {
+            first_call_ = false;
+            in = child_->next();
+        } else {
+            if (!prefetch_.valid()) return Result<RowBatch>::success(RowBatch()); // the stream ended
+            in = prefetch_.get();
+        }
         if (!in) return in;
         RowBatch batch = std::move(in).value();
         if (batch.getRowCount() == 0) return Result<RowBatch>::success(std::move(batch));
+        // the next batch is fetched while this one is compacted
+        prefetch_ = std::async(std::launch::async, [this] { return child_->next(); });
         try {
             run(batch);
         } catch (const std::exception &e) {
@@ -620,6 +839,8 @@ private:
     std::vector<PredicateTerm> pred_;
     ConnectiveType conn_ = ConnectiveType::AND;
     std::vector<bool> compact_;
+    bool first_call_ = true;
+    std::future<Result<RowBatch>> prefetch_; // the child's next batch, fetched on another thread
 };
 
 enum class JoinType { INNER, LEFT, RIGHT, FULL };
@@ -635,17 +856,30 @@ enum class JoinType { INNER, LEFT, RIGHT, FULL };
 // reference's does.  The inputs need not be
 // sorted (the reference requires both sorted by key, :58-59).  The output
 // rows come ordered by (key, left row, right row) with a power-of-two
-// capacity; one batch, then an empty one.  INNER joins only, as in the
+// capacity; one batch, then an empty one.  Every batch of each child is
+// gathered first (collectBatches, :37-49).  INNER joins only, as in the
 // reference.
 class SortMergeJoinOperator : public AbstractOperator {
 public:
+    // The reference's form (include/operator/sort_merge_join_operator.hpp:16-23);
+    // the source tables are kept and not used, as there.
+    SortMergeJoinOperator(ExecutionContext &context, Schema output_schema,
+                          std::unique_ptr<AbstractOperator> left_child, std::unique_ptr<AbstractOperator> right_child,
+                          std::pair<size_t, size_t> join_key_indices, std::vector<const Table *> left_source_tables,
+                          std::vector<const Table *> right_source_tables, JoinType join_type = JoinType::INNER,
+                          bool oblivious_materialization = false)
+        : AbstractOperator(context, std::move(output_schema)), left_(std::move(left_child)),
+          right_(std::move(right_child)), keys_(join_key_indices), left_tables_(std::move(left_source_tables)),
+          right_tables_(std::move(right_source_tables)), type_(join_type), oblivious_(oblivious_materialization)
+    {
+    }
     // oblivious_materialization: gather the columns with OP_GATHER_OBLIVIOUS
     // (no address depends on the pairs) instead of the reference's permute
     SortMergeJoinOperator(ExecutionContext &context, std::unique_ptr<AbstractOperator> left_child,
                           std::unique_ptr<AbstractOperator> right_child, std::pair<size_t, size_t> join_key_indices,
                           JoinType join_type = JoinType::INNER, bool oblivious_materialization = false)
-        : AbstractOperator(context), left_(std::move(left_child)), right_(std::move(right_child)),
-          keys_(join_key_indices), type_(join_type), oblivious_(oblivious_materialization)
+        : SortMergeJoinOperator(context, Schema(), std::move(left_child), std::move(right_child), join_key_indices, {},
+                                {}, join_type, oblivious_materialization)
     {
     }
 
@@ -654,14 +888,15 @@ public:
         if (joined_) return Result<RowBatch>::success(RowBatch());
         joined_ = true;
         if (type_ != JoinType::INNER) return Result<RowBatch>::failure("SortMergeJoinOperator supports INNER joins");
-        auto l = left_->next();
-        if (!l) return l;
-        RowBatch lb = std::move(l).value();
-        if (lb.getRowCount() == 0) return Result<RowBatch>::success(RowBatch());
-        auto r = right_->next();
-        if (!r) return r;
-        RowBatch rb = std::move(r).value();
-        if (rb.getRowCount() == 0) return Result<RowBatch>::success(RowBatch());
+        RowBatch lb, rb;
+        try {
+            lb = collectBatches(*left_);
+            if (lb.getRowCount() == 0) return Result<RowBatch>::success(RowBatch());
+            rb = collectBatches(*right_);
+            if (rb.getRowCount() == 0) return Result<RowBatch>::success(RowBatch());
+        } catch (const std::exception &e) {
+            return Result<RowBatch>::failure(e.what());
+        }
         if (keys_.first >= lb.getColumnCount()) return Result<RowBatch>::failure("Left join key index out of bounds");
         if (keys_.second >= rb.getColumnCount())
             return Result<RowBatch>::failure("Right join key index out of bounds");
@@ -752,6 +987,7 @@ private:
 
     std::unique_ptr<AbstractOperator> left_, right_;
     std::pair<size_t, size_t> keys_;
+    std::vector<const Table *> left_tables_, right_tables_;
     JoinType type_;
     bool oblivious_;
     bool joined_ = false;
@@ -851,12 +1087,11 @@ public:
     {
         if (done_) return Result<RowBatch>::success(RowBatch());
         done_ = true;
-        auto d = dim_->next();
-        if (!d) return d;
-        auto f = fact_->next();
-        if (!f) return f;
-        RowBatch db = std::move(d).value(), fb = std::move(f).value();
         try {
+            // every batch of each side, as the sort-merge join gathers them
+            RowBatch db = collectBatches(*dim_);
+            RowBatch fb = collectBatches(*fact_);
+            if (db.getRowCount() == 0 || fb.getRowCount() == 0) return Result<RowBatch>::success(RowBatch());
             return Result<RowBatch>::success(run(db, fb));
         } catch (const std::exception &e) {
             return Result<RowBatch>::failure(e.what());
@@ -932,6 +1167,10 @@ public:
     AbstractPlanNode &operator=(const AbstractPlanNode &) = delete;
 
     PlanType getPlanType() const { return type_; }
+    // the output columns (AbstractPlanNode::getOutputSchema); empty when the
+    // node names columns by index only
+    const Schema &getOutputSchema() const { return output_schema_; }
+    void setOutputSchema(Schema schema) { output_schema_ = std::move(schema); }
     const std::vector<std::unique_ptr<AbstractPlanNode>> &getChildren() const { return children_; }
     void addChild(std::unique_ptr<AbstractPlanNode> child) { children_.push_back(std::move(child)); }
 
@@ -951,6 +1190,7 @@ protected:
     }
 
     PlanType type_;
+    Schema output_schema_;
     std::vector<std::unique_ptr<AbstractPlanNode>> children_;
 };
 
@@ -982,7 +1222,8 @@ public:
     }
     std::unique_ptr<AbstractOperator> createOperator(ExecutionContext &context) const override
     {
-        return std::make_unique<SortOperator>(context, child_op(context, 0, 1, "SortPlanNode"), order_, asc_);
+        return std::make_unique<SortOperator>(context, output_schema_.clone(), child_op(context, 0, 1, "SortPlanNode"),
+                                              order_, asc_);
     }
     std::string toString() const override
     {
@@ -1005,6 +1246,13 @@ private:
 class FilterCompactionPlanNode : public AbstractPlanNode {
 public:
     explicit FilterCompactionPlanNode(size_t mask_index) : AbstractPlanNode(PlanType::COMPACTION), mask_(mask_index) {}
+    // the reference's form (filter_compaction_plan_node.cpp:17-28): the mask
+    // is the output schema's "$_mask" column
+    FilterCompactionPlanNode(Schema output_schema, std::vector<bool> compact_columns = {})
+        : AbstractPlanNode(PlanType::COMPACTION), mask_(kByName), compact_(std::move(compact_columns))
+    {
+        output_schema_ = std::move(output_schema);
+    }
     FilterCompactionPlanNode(std::vector<PredicateTerm> predicate, ConnectiveType connective = ConnectiveType::AND)
         : AbstractPlanNode(PlanType::COMPACTION), mask_(kNoMask), pred_(std::move(predicate)), conn_(connective)
     {
@@ -1014,12 +1262,16 @@ public:
     std::unique_ptr<AbstractOperator> createOperator(ExecutionContext &context) const override
     {
         auto child = child_op(context, 0, 1, "FilterCompactionPlanNode");
+        if (mask_ == kByName)
+            return std::make_unique<FilterCompactionOperator>(context, output_schema_.clone(), std::move(child),
+                                                              compact_);
         if (mask_ == kNoMask)
             return std::make_unique<FilterCompactionOperator>(context, std::move(child), pred_, conn_, compact_);
         return std::make_unique<FilterCompactionOperator>(context, std::move(child), mask_, compact_);
     }
     std::string toString() const override
     {
+        if (mask_ == kByName) return "FilterCompaction()";
         if (mask_ != kNoMask) return "FilterCompaction(mask #" + std::to_string(mask_) + ")";
         static const char *const ops[] = {"=", "<>", "<", "<=", ">", ">="};
         std::string s = "FilterCompaction(";
@@ -1031,7 +1283,7 @@ public:
     }
 
 private:
-    static constexpr size_t kNoMask = ~(size_t)0;
+    static constexpr size_t kNoMask = ~(size_t)0, kByName = ~(size_t)1;
     size_t mask_;
     std::vector<PredicateTerm> pred_;
     ConnectiveType conn_ = ConnectiveType::AND;
@@ -1048,12 +1300,22 @@ public:
           oblivious_(oblivious_materialization)
     {
     }
+    // the reference's form (sort_merge_join_plan_node.cpp:26-41)
+    SortMergeJoinPlanNode(Schema output_schema, std::pair<size_t, size_t> join_key_indices,
+                          std::vector<const Table *> left_source_tables, std::vector<const Table *> right_source_tables,
+                          JoinType join_type = JoinType::INNER)
+        : AbstractPlanNode(PlanType::SORT_MERGE_JOIN), keys_(join_key_indices), type_(join_type), oblivious_(false),
+          left_tables_(std::move(left_source_tables)), right_tables_(std::move(right_source_tables))
+    {
+        output_schema_ = std::move(output_schema);
+    }
     JoinType getJoinType() const { return type_; }
     std::unique_ptr<AbstractOperator> createOperator(ExecutionContext &context) const override
     {
         auto left = child_op(context, 0, 2, "SortMergeJoinPlanNode");
         auto right = child_op(context, 1, 2, "SortMergeJoinPlanNode");
-        return std::make_unique<SortMergeJoinOperator>(context, std::move(left), std::move(right), keys_, type_,
+        return std::make_unique<SortMergeJoinOperator>(context, output_schema_.clone(), std::move(left),
+                                                       std::move(right), keys_, left_tables_, right_tables_, type_,
                                                        oblivious_);
     }
     std::string toString() const override
@@ -1065,6 +1327,7 @@ private:
     std::pair<size_t, size_t> keys_;
     JoinType type_;
     bool oblivious_;
+    std::vector<const Table *> left_tables_, right_tables_;
 };
 
 // MaterializationPlanNode (include/planner/materialization_plan_node.hpp,

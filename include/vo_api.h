@@ -99,8 +99,22 @@ typedef enum vo_opcode {
 /* ------------------------------------------------------------------ args */
 
 /* OP_SORT: mirror of CommandArgs::SortArgs (include/cuda/commands.hpp:64-72).
- * Sorts the columns in place over n_padded_rows rows (pad rows filled with
- * max for ASC / lowest for DESC), and writes indices[i] = original row. */
+ * Sorts the columns in place over n_padded_rows rows and writes
+ * indices[i] = original row, with the reference's semantics
+ * (include/cuda/sort.cuh:11-118), bit-exact:
+ *   - pad rows [n_rows, n_padded_rows) are filled with numeric max (ASC
+ *     column) or lowest (DESC column) first (sort.cuh:79-100);
+ *   - rows equal in every key column are not exchanged: ties, including a
+ *     real row whose keys equal the pad values, are decided by the bitonic
+ *     network, so a pad row can precede such a real row and the first n_rows
+ *     indices need not all be < n_rows (as in the reference);
+ *   - FLOAT/DOUBLE: -0.0 and +0.0 compare equal and every value keeps its own
+ *     bits (the sign of a zero travels with its row).  With more than
+ *     32 - log2(n_padded_rows) FLOAT/DOUBLE key columns the call returns
+ *     VO_ERR_UNSUPPORTED;
+ *   - NaN (deviation): the reference's compareSingle has no order for NaN
+ *     (it compares neither equal nor less); here a NaN takes its IEEE
+ *     total-order place, above +inf (below -inf when its sign bit is set). */
 typedef struct vo_sort_args {
     void *sort_cols[VO_MAX_SORT_COLUMNS];
     int64_t *indices;
@@ -483,6 +497,7 @@ VO_EXPORT int vo_transfer_to_host(vo_ctx *ctx, void *dst_host, const void *src_d
 /* Asynchronous forms: enqueue on the context stream and return. */
 VO_EXPORT int vo_sort_kp_async(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad);
 
+
 /* --------------------------------------------------------- diagnostics */
 
 /* Obliviousness trace (modelled on TraceMem,
@@ -553,6 +568,12 @@ typedef struct vo_query {
     uint32_t kind;  /* VO_Q_* */
     uint32_t table; /* table id of that kind */
     double arrival_s;
+    void *out;      /* optional host memory of vo_server_query_out_bytes() bytes: the query's whole
+                       result, copied out of the device after the query ends (outside its latency):
+                         filter: { u64 count, u64 0, u64 sum_lo, i64 sum_hi } then price[n], disc[n]
+                                 (the compacted columns: the selected rows first, in row order);
+                         join:   { i32 group[64], i64 sum[64], u64 n_groups } (groups ascending);
+                         sort:   keys[n] (u64) then payload[n] (u32), sorted */
 } vo_query;
 
 typedef struct vo_query_result {
@@ -596,6 +617,14 @@ enum {
 };
 VO_EXPORT int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_slots_override,
                                  uint32_t mode, vo_query_result *out, vo_run_stats *stats);
+/* Bytes of a query's whole result (vo_query.out), 0 for an unknown table. */
+VO_EXPORT uint64_t vo_server_query_out_bytes(vo_server *s, const vo_query *q);
+/* The host copy of a table column (the data the device copy was generated
+ * with), for checking results: fact tables day (i32), qty, price, disc (i64);
+ * join tables dim_key, dim_group, fact_fk (i32), fact_val (i64); sort tables
+ * keys (u64), payload (u32).  *rows receives the column's row count. */
+VO_EXPORT int vo_server_table_column(vo_server *s, uint32_t kind, uint32_t table, uint32_t column,
+                                     const void **host, uint64_t *rows);
 
 /* ------------------------------------------------------ data generators */
 

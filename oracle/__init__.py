@@ -322,3 +322,49 @@ def predicate_mask(terms, n, npad, conn=0):
     if rc != 0:
         raise ValueError("unsupported predicate term")
     return m
+
+
+# ------------------------------------------------------------ closed forms
+# The results the oracle's restatements are pinned against (tests/test_oracle.py
+# checks each one equal to its restatement on random inputs).  They run in
+# O(n log n) numpy, so tests can check results at the configs' full sizes.
+
+def filter_closed_form(day, qty, price, disc, spec=Q6):
+    """Filter + ORCompact + SUM as a stable partition: (count, exact sum,
+    selected price, selected disc), the selected rows in row order
+    (oblivious_compact.c:59-153 keeps their order; the sum is exact in Python
+    integers)."""
+    sel = ((day >= spec["day_lo"]) & (day < spec["day_hi"]) & (disc >= spec["disc_lo"]) &
+           (disc <= spec["disc_hi"]) & (qty < spec["qty_lt"]))
+    p, d = price[sel], disc[sel]
+    hi = np.abs(p).max(initial=0) * np.abs(d).max(initial=0)
+    if hi < (1 << 62) // max(1, len(p)):  # the int64 dot product cannot overflow
+        s = int(np.dot(p, d))
+    else:
+        s = sum(int(a) * int(b) for a, b in zip(p, d))
+    return int(sel.sum()), s, p, d
+
+
+def join_groupsum_closed_form(dk, dg, fk, fv, max_groups=64):
+    """PK/FK join + SUM GROUP BY as an array lookup: every fact row joins the
+    one dimension row with its key (keys are unique); (groups ascending,
+    int64 sums) over the joined rows whose group lies in [0, max_groups)."""
+    dk = np.asarray(dk, np.int64)
+    order = np.argsort(dk, kind="stable")
+    pos = np.searchsorted(dk[order], fk)
+    pos = np.minimum(pos, len(dk) - 1)
+    hit = dk[order][pos] == fk
+    g = np.asarray(dg, np.int64)[order][pos][hit]
+    v = np.asarray(fv, np.int64)[hit]
+    keep = (g >= 0) & (g < max_groups)
+    g, v = g[keep], v[keep]
+    groups = np.unique(g)
+    sums = np.array([int(v[g == x].sum()) for x in groups], np.int64)
+    return groups.astype(np.int32), sums
+
+
+def sort_closed_form(keys, payload):
+    """Ascending by key, ties by payload: the unique order of distinct
+    (key, payload) pairs, which the bitonic network produces."""
+    order = np.lexsort((payload, keys))
+    return keys[order], payload[order]

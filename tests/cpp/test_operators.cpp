@@ -21,6 +21,17 @@
 
 using namespace velodb_amd;
 
+// The CPU oracle (oracle/oracle.h), linked from oracle/_build/liboracle.so:
+// the reference's multi-column bitonic sort, ORCompact and the oblivious
+// equi-join restated on the host.
+extern "C" {
+int orc_bitonic_sort_multi(void *const *cols, const uint32_t *types, const uint8_t *asc, size_t ncols,
+                           int64_t *indices, size_t n, size_t npad);
+void orc_compact(void *buf, size_t elem_size, const uint8_t *selected, size_t n);
+size_t orc_join_pairs(const uint64_t *lkey, size_t n_left, const uint64_t *rkey, size_t n_right, int64_t *out_left,
+                      int64_t *out_right, size_t cap, int *status);
+}
+
 static int g_fail = 0;
 #define CHECK(cond)                                                                                                    \
     do {                                                                                                               \
@@ -436,6 +447,185 @@ static void test_sort_merge_join(ExecutionContext &ctx)
     auto rb = bad.next();
     CHECK(!rb);
     if (!rb) CHECK(rb.error() == "Left join key index out of bounds");
+}
+
+// ---- children that yield several batches (collectBatches, slicing, prefetch)
+
+struct KV {
+    std::vector<int32_t> key;
+    std::vector<int64_t> val;
+    std::vector<uint8_t> valid;
+};
+
+static KV random_kv(size_t n, uint64_t seed, int32_t key_range)
+{
+    std::mt19937_64 rng(seed);
+    KV kv;
+    for (size_t i = 0; i < n; ++i) {
+        kv.key.push_back((int32_t)(rng() % (uint64_t)key_range) - key_range / 2);
+        kv.val.push_back((int64_t)(rng() >> 8) - (int64_t)(1ull << 54));
+        kv.valid.push_back((uint8_t)(rng() % 5 != 0));
+    }
+    return kv;
+}
+
+// rows [a, b) of kv as a batch (key INTEGER, value BIGINT nullable)
+static RowBatch kv_batch(const KV &kv, size_t a, size_t b)
+{
+    std::vector<Column> c;
+    c.push_back(Column::fromHost(VO_T_INTEGER, kv.key.data() + a, b - a));
+    c.push_back(Column::fromHost(VO_T_BIGINT, kv.val.data() + a, b - a, kv.valid.data() + a));
+    return RowBatch(std::move(c), b - a);
+}
+
+static std::vector<RowBatch> kv_batches(const KV &kv, const std::vector<size_t> &cuts)
+{
+    std::vector<RowBatch> out;
+    for (size_t i = 0; i + 1 < cuts.size(); ++i) out.push_back(kv_batch(kv, cuts[i], cuts[i + 1]));
+    return out;
+}
+
+static void test_multi_batch_sort(ExecutionContext &ctx)
+{
+    // three child batches, 9100 rows; output in slices of at most 4000 rows
+    const KV kv = random_kv(9100, 21, 600);
+    Schema schema({ColumnInfo("k", VO_T_INTEGER), ColumnInfo("v", VO_T_BIGINT)});
+    SortOperator op(ctx, schema.clone(), std::make_unique<BatchSourceOperator>(ctx, kv_batches(kv, {0, 3000, 7500, 9100})),
+                    {0}, {false});
+    op.setMaxBatchSize(4000);
+    std::vector<int32_t> gk;
+    std::vector<int64_t> gv;
+    std::vector<uint8_t> gb;
+    std::vector<size_t> sizes;
+    for (;;) {
+        auto r = op.next();
+        CHECK(static_cast<bool>(r));
+        if (!r) return;
+        RowBatch b = std::move(r).value();
+        if (b.getRowCount() == 0) break;
+        const size_t n = b.getRowCount();
+        sizes.push_back(n);
+        const auto k = b.getColumn(0).toHost<int32_t>(n);
+        const auto v = b.getColumn(1).toHost<int64_t>(n);
+        const auto bits = b.getColumn(1).bitmapToHost(n);
+        gk.insert(gk.end(), k.begin(), k.end());
+        gv.insert(gv.end(), v.begin(), v.end());
+        gb.insert(gb.end(), bits.begin(), bits.end());
+    }
+    CHECK((sizes == std::vector<size_t>{4000, 4000, 1100}));
+    CHECK(op.getOutputSchema().getColumnIndex("v") == 1);
+    // the oracle: the reference's network over the gathered rows (padded to 2^14), then permute
+    const size_t n = 9100, npad = 16384;
+    std::vector<int32_t> key(npad, 0);
+    std::copy(kv.key.begin(), kv.key.end(), key.begin());
+    std::vector<int64_t> idx(npad);
+    void *cols[1] = {key.data()};
+    const uint32_t types[1] = {VO_T_INTEGER};
+    const uint8_t asc[1] = {0};
+    CHECK(orc_bitonic_sort_multi(cols, types, asc, 1, idx.data(), n, npad) == 0);
+    bool ok = gk.size() == n;
+    for (size_t i = 0; ok && i < n; ++i)
+        ok = gk[i] == key[i] && idx[i] < (int64_t)n && gv[i] == kv.val[idx[i]] && gb[i] == kv.valid[idx[i]];
+    CHECK(ok);
+}
+
+static void test_multi_batch_filter(ExecutionContext &ctx)
+{
+    // three batches with a "$_mask" column; each compacted on its own while
+    // the next is prefetched; the selected rows match ORCompact's prefix
+    const std::vector<size_t> cuts = {0, 5000, 5001, 12000};
+    const KV kv = random_kv(12000, 22, 1000);
+    std::vector<uint8_t> mask(12000);
+    std::mt19937_64 rng(23);
+    for (auto &m : mask) m = (uint8_t)(rng() % 3 == 0);
+    std::vector<RowBatch> batches;
+    for (size_t i = 0; i + 1 < cuts.size(); ++i) {
+        RowBatch b = kv_batch(kv, cuts[i], cuts[i + 1]);
+        std::vector<Column> c;
+        c.push_back(std::move(b.getColumn(0)));
+        c.push_back(std::move(b.getColumn(1)));
+        c.push_back(Column::fromHost(VO_T_BOOLEAN, mask.data() + cuts[i], cuts[i + 1] - cuts[i]));
+        batches.emplace_back(std::move(c), cuts[i + 1] - cuts[i]);
+    }
+    Schema schema({ColumnInfo("t.k", VO_T_INTEGER), ColumnInfo("t.v", VO_T_BIGINT), ColumnInfo("t.$_mask", VO_T_BOOLEAN)});
+    FilterCompactionOperator op(ctx, std::move(schema), std::make_unique<BatchSourceOperator>(ctx, std::move(batches)));
+    for (size_t i = 0; i + 1 < cuts.size(); ++i) {
+        auto r = op.next();
+        CHECK(static_cast<bool>(r));
+        if (!r) return;
+        RowBatch b = std::move(r).value();
+        const size_t a = cuts[i], n = cuts[i + 1] - a;
+        // the oracle: ORCompact of (key, value, valid) rows of this batch
+        struct Row {
+            int32_t k;
+            uint8_t valid, pad[3];
+            int64_t v;
+        };
+        std::vector<Row> rows(n);
+        for (size_t j = 0; j < n; ++j) rows[j] = Row{kv.key[a + j], kv.valid[a + j], {0, 0, 0}, kv.val[a + j]};
+        orc_compact(rows.data(), sizeof(Row), mask.data() + a, n);
+        size_t m = 0;
+        for (size_t j = 0; j < n; ++j) m += mask[a + j] != 0;
+        CHECK(b.getRowCount() == m);
+        if (b.getRowCount() != m) return;
+        const auto k = b.getColumn(0).toHost<int32_t>(m);
+        const auto v = b.getColumn(1).toHost<int64_t>(m);
+        const auto bits = b.getColumn(1).bitmapToHost(m);
+        bool ok = true;
+        for (size_t j = 0; j < m; ++j) ok = ok && k[j] == rows[j].k && v[j] == rows[j].v && bits[j] == rows[j].valid;
+        CHECK(ok);
+    }
+    auto end = op.next();
+    CHECK(static_cast<bool>(end) && end.value().getRowCount() == 0);
+    // the reference's form needs a "$_mask" column in the schema
+    bool threw = false;
+    try {
+        FilterCompactionOperator bad(ctx, Schema({ColumnInfo("k", VO_T_INTEGER)}),
+                                     std::make_unique<BatchSourceOperator>(ctx, RowBatch()));
+    } catch (const ExecutionError &) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
+static void test_multi_batch_sort_merge_join(ExecutionContext &ctx)
+{
+    // left in two batches, right in three; the pairs match the oracle's join
+    const KV l = random_kv(3000, 24, 400), r = random_kv(4000, 25, 400);
+    Schema schema({ColumnInfo("l.k", VO_T_INTEGER), ColumnInfo("l.v", VO_T_BIGINT), ColumnInfo("r.k", VO_T_INTEGER),
+                   ColumnInfo("r.v", VO_T_BIGINT)});
+    SortMergeJoinOperator op(ctx, std::move(schema),
+                             std::make_unique<BatchSourceOperator>(ctx, kv_batches(l, {0, 1700, 3000})),
+                             std::make_unique<BatchSourceOperator>(ctx, kv_batches(r, {0, 100, 2600, 4000})), {0, 0},
+                             {}, {}, JoinType::INNER);
+    auto res = op.next();
+    CHECK(static_cast<bool>(res));
+    if (!res) return;
+    RowBatch b = std::move(res).value();
+    // oracle key codes: INTEGER keys sign-extended, sign bit flipped
+    auto code = [](int32_t k) { return (uint64_t)(int64_t)k ^ (1ull << 63); };
+    std::vector<uint64_t> lc, rc;
+    for (int32_t k : l.key) lc.push_back(code(k));
+    for (int32_t k : r.key) rc.push_back(code(k));
+    int status = 0;
+    std::vector<int64_t> ol(1), orr(1);
+    const size_t n_out = orc_join_pairs(lc.data(), lc.size(), rc.data(), rc.size(), ol.data(), orr.data(), 0, &status);
+    ol.resize(n_out);
+    orr.resize(n_out);
+    CHECK(orc_join_pairs(lc.data(), lc.size(), rc.data(), rc.size(), ol.data(), orr.data(), n_out, &status) == n_out);
+    CHECK(status == 0);
+    CHECK(b.getRowCount() == n_out);
+    if (b.getRowCount() != n_out) return;
+    const auto glk = b.getColumn(0).toHost<int32_t>(n_out), grk = b.getColumn(2).toHost<int32_t>(n_out);
+    const auto glv = b.getColumn(1).toHost<int64_t>(n_out), grv = b.getColumn(3).toHost<int64_t>(n_out);
+    const auto glb = b.getColumn(1).bitmapToHost(n_out), grb = b.getColumn(3).bitmapToHost(n_out);
+    bool ok = true;
+    for (size_t i = 0; ok && i < n_out; ++i)
+        ok = glk[i] == l.key[ol[i]] && glv[i] == l.val[ol[i]] && glb[i] == l.valid[ol[i]] && grk[i] == r.key[orr[i]] &&
+             grv[i] == r.val[orr[i]] && grb[i] == r.valid[orr[i]];
+    CHECK(ok);
+    auto end = op.next();
+    CHECK(static_cast<bool>(end) && end.value().getRowCount() == 0);
 }
 
 // Plan nodes build the same operators as the direct calls above: the
@@ -904,6 +1094,9 @@ int main()
         {"errors_are_results", [&] { test_errors_are_results(ctx); }},
         {"task_manager_queue", [&] { test_task_manager_queue(ctx); }},
         {"plan_nodes", [&] { test_plan_nodes(ctx); }},
+        {"multi_batch_sort", [&] { test_multi_batch_sort(ctx); }},
+        {"multi_batch_filter", [&] { test_multi_batch_filter(ctx); }},
+        {"multi_batch_sort_merge_join", [&] { test_multi_batch_sort_merge_join(ctx); }},
     };
     for (const auto &t : tests) {
         const int before = g_fail;

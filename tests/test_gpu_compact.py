@@ -133,3 +133,32 @@ def test_scatter_positions_stable(vctx):
     assert np.all(got[mask == 0] == -1)
     assert int(from_dev(dcnt, np.uint64)[0]) == int(mask.sum())
     del ctypes
+
+
+def test_filter_sum_config2_3e8_rows_2p29(vctx):
+    """Config: 3.0e8 fact rows padded to 2^29 (the roofline size), generated
+    on the device by the generator the oracle pins (orc_gen_fact).  The
+    selected count, the exact int128 SUM(price*disc) and the compacted
+    (price, disc) prefix equal their closed forms computed from the same
+    columns: the count and sum of a mask, and the masked rows in row order
+    (a stable partition, the ORCompact invariant).  Every product is below
+    1.05e9 and the total below 2^63, so int64 torch arithmetic is exact."""
+    import torch
+    n, n_pad = 300_000_000, 1 << 29
+    print(f"filter_sum config2 n={n} n_pad={n_pad}", flush=True)
+    day = empty_dev(n_pad, np.int32)
+    qty, price, disc = (empty_dev(n_pad, np.int64) for _ in range(3))
+    vctx.gen_fact(11, day, qty, price, disc, n)
+    out = empty_dev(4, np.uint64)
+    op, od = empty_dev(n_pad, np.int64), empty_dev(n_pad, np.int64)
+    vctx.filter_sum(day, qty, price, disc, n, n_pad, out, op, od)
+    o = from_dev(out, np.uint64)
+    d, q, p, c = day[:n], qty[:n], price[:n], disc[:n]
+    sel = (d >= 8766) & (d < 9131) & (c >= 5) & (c <= 7) & (q < 24)
+    m = int(sel.sum().item())
+    s = int((p * c)[sel].sum().item())
+    assert int(o[0]) == m and abs(m / n - 0.0181) < 0.0005
+    assert int(o[2]) == s and int(o[3]) == 0
+    assert torch.equal(op[:m], p[sel]) and torch.equal(od[:m], c[sel])
+    del day, qty, price, disc, op, od, d, q, p, c, sel
+    torch.cuda.empty_cache()

@@ -224,6 +224,35 @@ def test_queue_restarts_after_idle_exit(torch_dev):
         ctx.close()
 
 
+def test_queue_poll_alone_restarts_an_idle_kernel(torch_dev):
+    """A caller that only polls (never waits) still sees its command finish
+    after the resident kernel left on idle: vo_poll restarts it."""
+    import velodb_amd
+    from velodb_amd import _abi
+    ctx = velodb_amd.Context(flags=_abi.CFG_PERSISTENT, queue_idle_us=2000)
+    try:
+        ctx.queue_start()
+        for rnd in range(2):
+            t0 = time.time()
+            while ctx.queue_running() and time.time() - t0 < 5:
+                time.sleep(0.01)
+            assert not ctx.queue_running()
+            n = 3000 + rnd
+            k, p = oracle.gen_sort_input(40 + rnd, n)
+            dk, dp = to_dev(k, 4096), to_dev(p, 4096)
+            _sync()
+            seq = ctx.submit(_sort_cmd(dk, dp, n, 4096))
+            t0 = time.time()
+            while not ctx.poll(seq):
+                assert time.time() - t0 < 20, "command never finished under polling alone"
+                time.sleep(0.001)
+            ek, ep = oracle.bitonic_sort_kp(k, p, n, 4096)
+            assert np.array_equal(_host(dk, np.uint64), ek) and np.array_equal(_host(dp, np.uint32), ep)
+        ctx.queue_stop()
+    finally:
+        ctx.close()
+
+
 def test_queue_parks_for_launch_only_call(qctx):
     """A synchronous launch-only operator in persistent mode first finishes
     the submitted commands and lets the resident kernel leave; the next

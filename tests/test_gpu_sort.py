@@ -176,3 +176,96 @@ def test_sort_schedule_variants_bit_exact(tuning):
             gk, gp = _run_sort(ctx, k, p, n, npad)
             ek, ep = oracle.bitonic_sort_kp(k, p, n, npad)
             assert np.array_equal(gk, ek) and np.array_equal(gp, ep)
+
+
+# ------------------------------------- OP_SORT edge cases vs the oracle
+
+def _extremes(t):
+    """Values at and next to the pad values of a type: lowest, lowest+1, 0,
+    max-1, max (and +-0.0, +-inf for floating point)."""
+    dt = TYPES[t]
+    if t in (6, 7):
+        fi = np.finfo(dt)
+        return np.array([fi.min, np.nextafter(fi.min, dt(0)), -0.0, 0.0, np.nextafter(fi.max, dt(0)), fi.max,
+                         -np.inf, np.inf], dtype=dt)
+    ii = np.iinfo(dt)
+    return np.array([ii.min, ii.min + 1, 0, ii.max - 1, ii.max], dtype=dt)
+
+
+@pytest.mark.parametrize("t", [1, 2, 3, 4, 5, 6, 7, 8, 10, 64])
+@pytest.mark.parametrize("asc", [True, False])
+def test_sort_multi_rows_at_pad_values(vctx, t, asc):
+    """Real rows at numeric max / lowest tie with the pad rows (sort.cuh:79-100):
+    the network decides where they land, so the indices and the columns must
+    match the reference's rule bit for bit, for either direction."""
+    print(f"sort_multi extremes type={t} asc={asc}", flush=True)
+    rng = np.random.default_rng(t * 2 + asc)
+    n, npad = 3000, 4096
+    vals = _extremes(t)
+    c0 = vals[rng.integers(0, len(vals), npad)]
+    c1 = rng.integers(0, 3, npad).astype(np.int32)  # second key: partial tie-breaker
+    for types, cols, ascs in (([t], [c0], [asc]), ([t, 4], [c0, c1], [asc, not asc])):
+        ecols, eidx = oracle.bitonic_sort_multi(cols, types, ascs, n, npad)
+        dcols = [to_dev(c) for c in cols]
+        didx = empty_dev(npad, np.int64)
+        vctx.sort(dcols, types, ascs, didx, n, npad)
+        assert np.array_equal(from_dev(didx, np.int64), eidx)
+        for dc, ec, tt in zip(dcols, ecols, types):
+            assert np.array_equal(from_dev(dc, TYPES[tt]).view(np.uint8), ec.view(np.uint8))
+
+
+@pytest.mark.parametrize("types,asc", [([6], [True]), ([7], [False]), ([6, 7], [True, False]),
+                                       ([4, 7, 6], [False, True, True])])
+def test_sort_multi_negative_zero_keeps_its_bits(vctx, types, asc):
+    """-0.0 and +0.0 compare equal (compareSingle, sort.cuh:11-29) and every
+    row keeps its own bits: the output equals the oracle's byte for byte."""
+    print(f"sort_multi -0.0 types={types}", flush=True)
+    rng = np.random.default_rng(len(types))
+    n, npad = 7000, 8192
+    cols = []
+    for t in types:
+        if t in (6, 7):
+            cols.append(np.array([-0.0, 0.0, 1.5, -1.5], dtype=TYPES[t])[rng.integers(0, 4, npad)])
+        else:
+            cols.append(rng.integers(0, 3, npad).astype(TYPES[t]))
+    ecols, eidx = oracle.bitonic_sort_multi(cols, types, asc, n, npad)
+    dcols = [to_dev(c) for c in cols]
+    didx = empty_dev(npad, np.int64)
+    vctx.sort(dcols, types, asc, didx, n, npad)
+    gidx = from_dev(didx, np.int64)
+    assert np.array_equal(gidx, eidx)
+    for dc, ec, c, t in zip(dcols, ecols, cols, types):
+        g = from_dev(dc, TYPES[t])
+        assert np.array_equal(g.view(np.uint8), ec.view(np.uint8))
+        if t in (6, 7):  # every real row carries its own sign of zero
+            live = gidx < n
+            assert np.array_equal(np.signbit(g[live]), np.signbit(c[gidx[live]]))
+
+
+def test_sort_multi_nan_takes_its_total_order_place(vctx):
+    """The documented deviation: NaN (unordered in the reference) sorts by its
+    IEEE total-order image, above +inf and, with the sign bit, below -inf."""
+    neg_nan = np.array([0xFFC00000], np.uint32).view(np.float32)[0]
+    vals = np.array([np.nan, np.inf, -np.inf, 1.0, -2.0, neg_nan, 0.5, 3.0], np.float32)
+    n = npad = len(vals)  # no pad rows: every value is distinct, so the order is unique
+    for asc in (True, False):
+        dcol = to_dev(vals, npad)
+        didx = empty_dev(npad, np.int64)
+        vctx.sort([dcol], [6], [asc], didx, n, npad)
+        gidx = from_dev(didx, np.int64)
+        order = [5, 2, 4, 6, 3, 7, 1, 0]  # -NaN, -inf, -2, 0.5, 1, 3, inf, NaN
+        expect = order if asc else order[::-1]
+        assert list(gidx) == expect
+        got = from_dev(dcol, np.float32)
+        assert np.array_equal(got.view(np.uint32), vals[expect].view(np.uint32))
+
+
+def test_sort_multi_rejects_more_fp_columns_than_sign_bits(vctx):
+    """At 2^29 rows the payload has 3 spare bits: a fourth FLOAT/DOUBLE key is
+    refused on the host before any launch."""
+    import velodb_amd
+    dummy = [empty_dev(16, np.float32) for _ in range(4)]
+    didx = empty_dev(16, np.int64)
+    with pytest.raises(velodb_amd.VoError) as e:
+        vctx.sort(dummy, [6, 6, 7, 6], [True] * 4, didx, 0, 1 << 29)
+    assert e.value.status == 5  # VO_ERR_UNSUPPORTED

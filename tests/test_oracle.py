@@ -504,3 +504,44 @@ def test_predicate_nulls_and_chain_order():
     assert oracle.predicate_mask([(b, 4, 0, 1, bv)], 5, 5).tolist() == [1, 0, 0, 0, 1]
     with pytest.raises(ValueError):
         oracle.predicate_mask([(a, 99, 0, 1)], 5, 5)
+
+
+# ---- closed forms that check the configs' full sizes, pinned to the restatements
+
+@pytest.mark.parametrize("n,npad", [(3000, 4096), (20000, 1 << 15)])
+def test_filter_closed_form_equals_restatement(n, npad):
+    day, qty, price, disc = oracle.gen_fact(21 + n, n)
+    m, s, op, od = oracle.filter_sum(day, qty, price, disc, n, npad, want_rows=True)
+    cm, cs, cp, cd = oracle.filter_closed_form(day, qty, price, disc)
+    assert (cm, cs) == (m, s)
+    assert np.array_equal(cp, op[:m]) and np.array_equal(cd, od[:m])
+    # a planted mistake (the first selected row dropped) is seen
+    first = int(np.flatnonzero(oracle.filter_mask(day, qty, disc, n, npad)[:n])[0])
+    keep = np.arange(n) != first
+    assert oracle.filter_closed_form(day[keep], qty[keep], price[keep], disc[keep])[:2] != (m, s)
+
+
+@pytest.mark.parametrize("n_dim,n_fact", [(500, 2000), (3000, 9000)])
+def test_join_closed_form_equals_restatement(n_dim, n_fact):
+    dk, dg, fk, fv = oracle.gen_join(5 + n_dim, n_dim, n_fact)
+    npad = 1 << int(np.ceil(np.log2(n_dim + n_fact)))
+    g, s = oracle.join_groupsum(dk, dg, fk, fv, npad)
+    cg, cs = oracle.join_groupsum_closed_form(dk, dg, fk, fv)
+    assert np.array_equal(cg, g) and np.array_equal(cs, s)
+    # unmatched fact rows and groups beyond the bound drop out of both
+    fk2 = fk.copy()
+    fk2[::7] = n_dim * 10 + 3
+    g2, s2 = oracle.join_groupsum(dk, dg, fk2, fv, npad)
+    cg2, cs2 = oracle.join_groupsum_closed_form(dk, dg, fk2, fv)
+    assert np.array_equal(cg2, g2) and np.array_equal(cs2, s2)
+    cg3, _ = oracle.join_groupsum_closed_form(dk, dg, fk, fv, max_groups=10)
+    assert cg3.max() < 10
+
+
+def test_sort_closed_form_equals_restatement():
+    n = 5000
+    k, p = oracle.gen_sort_input(9, n)
+    k[::5] = k[1::5][: len(k[::5])]  # duplicate keys: ties by payload
+    ek, ep = oracle.bitonic_sort_kp(k, p, n, 8192)
+    ck, cp = oracle.sort_closed_form(k, p)
+    assert np.array_equal(ck, ek[:n]) and np.array_equal(cp, ep[:n])

@@ -92,17 +92,40 @@ class Server:
         self._check(self.lib.vo_server_add_sort_table(self.h, seed, n_rows, C.byref(i)))
         return i.value
 
-    def run(self, queries, n_slots=0, transfer=False):
+    def run(self, queries, n_slots=0, transfer=False, outputs=None):
         """queries: list of (kind, table, arrival_s).  Returns (results, stats).
         transfer=True ships each query's inputs from pinned host memory (the
         reference engine's per-query path) instead of reading the resident
-        tables."""
-        arr = (_abi.Query * len(queries))(*[_abi.Query(k, t, a) for (k, t, a) in queries])
+        tables.  outputs: optional list (one entry per query) of numpy uint8
+        arrays of query_out_bytes() bytes, or None, that receive each query's
+        whole result (vo_query.out)."""
+        qs = []
+        for i, (k, t, a) in enumerate(queries):
+            o = outputs[i] if outputs is not None else None
+            qs.append(_abi.Query(k, t, a, o.ctypes.data if o is not None else None))
+        arr = (_abi.Query * len(queries))(*qs)
         res = (_abi.QueryResult * len(queries))()
         st = _abi.RunStats()
         mode = _abi.RUN_TRANSFER if transfer else _abi.RUN_RESIDENT
         self._check(self.lib.vo_server_run_mode(self.h, arr, len(queries), n_slots, mode, res, C.byref(st)))
         return list(res), st
+
+    def query_out_bytes(self, kind, table):
+        q = _abi.Query(kind, table, 0.0, None)
+        return int(self.lib.vo_server_query_out_bytes(self.h, C.byref(q)))
+
+    def table_column(self, kind, table, column):
+        """A copy of a table column's host data (numpy): fact day/qty/price/disc,
+        join dim_key/dim_group/fact_fk/fact_val, sort keys/payload."""
+        import numpy as np
+        dts = {_abi.Q_FILTER_SUM: (np.int32, np.int64, np.int64, np.int64),
+               _abi.Q_JOIN_GROUPSUM: (np.int32, np.int32, np.int32, np.int64),
+               _abi.Q_SORT: (np.uint64, np.uint32)}
+        ptr, rows = C.c_void_p(), C.c_uint64()
+        self._check(self.lib.vo_server_table_column(self.h, kind, table, column, C.byref(ptr), C.byref(rows)))
+        dt = np.dtype(dts[kind][column])
+        buf = (C.c_char * (rows.value * dt.itemsize)).from_address(ptr.value)
+        return np.frombuffer(buf, dtype=dt).copy()
 
     def close(self):
         if self.h:

@@ -176,7 +176,7 @@ class ServerConfig(C.Structure):
 
 
 class Query(C.Structure):
-    _fields_ = [("kind", u32), ("table", u32), ("arrival_s", C.c_double)]
+    _fields_ = [("kind", u32), ("table", u32), ("arrival_s", C.c_double), ("out", P)]
 
 
 class QueryResult(C.Structure):
@@ -242,6 +242,8 @@ PROTOTYPES = [
     ("vo_server_run", C.c_int, [P, C.POINTER(Query), u64, u32, C.POINTER(QueryResult), C.POINTER(RunStats)]),
     ("vo_server_run_mode", C.c_int, [P, C.POINTER(Query), u64, u32, u32, C.POINTER(QueryResult),
                                      C.POINTER(RunStats)]),
+    ("vo_server_query_out_bytes", u64, [P, C.POINTER(Query)]),
+    ("vo_server_table_column", C.c_int, [P, u32, u32, u32, C.POINTER(P), C.POINTER(u64)]),
     ("vo_gen_sort_input", C.c_int, [P, u64, P, P, u64]),
     ("vo_gen_fact", C.c_int, [P, u64, P, P, P, P, u64]),
 ]

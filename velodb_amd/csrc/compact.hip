@@ -603,7 +603,9 @@ struct Workspace {
 
 constexpr uint32_t kSumBlocks = 2048;
 
-Workspace workspace(vo_ctx *ctx, uint64_t n_pad, bool want_mask)
+// The workspace laid out from `base` (nullptr: only the size); returns its
+// bytes.
+size_t workspace_at(char *base, uint64_t n_pad, bool want_mask, Workspace *ws)
 {
     const uint64_t nt = n_pad / CT;
     size_t off = 0;
@@ -614,9 +616,18 @@ Workspace workspace(vo_ctx *ctx, uint64_t n_pad, bool want_mask)
     };
     const size_t o_cnt = take(nt * 4), o_pre = take((nt + 1) * 8), o_count = take(8),
                  o_part = take(kSumBlocks * sizeof(__int128)), o_mask = take(want_mask ? n_pad : 0);
-    char *base = (char *)scratch(ctx, off);
-    return Workspace{(uint32_t *)(base + o_cnt), (uint64_t *)(base + o_pre), (uint64_t *)(base + o_count),
-                     want_mask ? (uint8_t *)(base + o_mask) : nullptr, (__int128 *)(base + o_part)};
+    if (ws)
+        *ws = Workspace{(uint32_t *)(base + o_cnt), (uint64_t *)(base + o_pre), (uint64_t *)(base + o_count),
+                        want_mask ? (uint8_t *)(base + o_mask) : nullptr, (__int128 *)(base + o_part)};
+    return off;
+}
+
+Workspace workspace(vo_ctx *ctx, uint64_t n_pad, bool want_mask)
+{
+    Workspace ws;
+    char *base = (char *)scratch(ctx, workspace_at(nullptr, n_pad, want_mask, nullptr));
+    workspace_at(base, n_pad, want_mask, &ws);
+    return ws;
 }
 
 template <int NW>
@@ -776,14 +787,32 @@ void op_compact(vo_ctx *ctx, const vo_compact_args &a)
     }
 }
 
-void op_filter_sum(vo_ctx *ctx, const vo_filter_args &a)
+namespace {
+
+void check_filter(const vo_filter_args &a)
 {
     check_pad(a.n, a.n_pad);
     VO_CHECK(a.day && a.qty && a.price && a.disc && a.out, VO_ERR_INVALID, "null buffer");
     VO_CHECK((a.out_price == nullptr) == (a.out_disc == nullptr), VO_ERR_INVALID,
              "out_price and out_disc are given together");
+}
+
+// The filter's passes over a workspace the caller provides.
+void filter_sum_with(vo_ctx *ctx, const vo_filter_args &a, const Workspace &ws);
+
+} // namespace
+
+void op_filter_sum(vo_ctx *ctx, const vo_filter_args &a)
+{
+    check_filter(a);
+    filter_sum_with(ctx, a, workspace(ctx, a.n_pad, true));
+}
+
+namespace {
+
+void filter_sum_with(vo_ctx *ctx, const vo_filter_args &a, const Workspace &ws)
+{
     const bool in_place_copy = a.out_price != nullptr;
-    Workspace ws = workspace(ctx, a.n_pad, true);
     const uint32_t nt = (uint32_t)(a.n_pad / CT);
     Pred pr{a.day_lo, a.day_hi, a.disc_lo, a.disc_hi, a.qty_lt};
     launch(ctx, LaunchInfo{"filter_count", dim3(nt), dim3(CBLOCK), 0, a.n_pad}, k_filter_count, a.day, a.qty, a.disc,
@@ -804,6 +833,8 @@ void op_filter_sum(vo_ctx *ctx, const vo_filter_args &a)
     launch(ctx, LaunchInfo{"filter_sum_final", dim3(1), dim3(256), 0, 0}, k_sum_final, (const __int128 *)ws.partial,
            sb, (const uint64_t *)ws.count, a.out, trace_slot(ctx));
 }
+
+} // namespace
 
 void op_scatter(vo_ctx *ctx, const vo_scatter_args &a)
 {

@@ -604,6 +604,13 @@ int queue_poll(vo_ctx *ctx, uint64_t seq)
     QueueState *q = (QueueState *)ctx->queue;
     try {
         reap_error(ctx, q);
+        if (__atomic_load_n(&q->ctrl->done, __ATOMIC_ACQUIRE) >= seq) return 1;
+        // a kernel that left on idle just as this command was published is
+        // started again here too, so a caller that only polls makes progress
+        if (vctrl(q)->alive == 0) {
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            ensure_running_locked(ctx, q);
+        }
         return __atomic_load_n(&q->ctrl->done, __ATOMIC_ACQUIRE) >= seq ? 1 : 0;
     } catch (const Error &e) {
         return -fail_q(ctx, e);

@@ -22,6 +22,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
+#include <cstring>
 #include <thread>
 
 struct vo_server;
@@ -56,6 +57,13 @@ struct SortTable {
 };
 
 uint64_t align256(uint64_t b) { return (b + 255) & ~255ull; }
+
+// Where a query leaves its whole result on the device: the filter's
+// compacted (price, disc) columns, the sort's sorted (keys, payload).
+struct OutLoc {
+    const void *c0 = nullptr, *c1 = nullptr;
+    uint64_t b0 = 0, b1 = 0; // bytes of each
+};
 
 uint64_t pow2_at_least(uint64_t n, uint64_t floor_)
 {
@@ -166,6 +174,7 @@ struct vo_server {
         hipGraphExec_t exec = nullptr;
         void *scr = nullptr, *buf = nullptr;
         bool failed = false;
+        vo::OutLoc loc; // where a replay leaves the query's whole result
     };
     std::vector<std::map<uint64_t, Graph>> graphs;
 };
@@ -259,10 +268,13 @@ void h2d(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes)
 // words to the slot's pinned result buffer `res`; nothing here waits.  With
 // `transfer`, the input columns are first copied from the table's pinned host
 // copy into the slot buffer and the query runs on those copies; a sort also
-// copies its sorted columns back to host memory (hbuf).
-void enqueue_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_query &q, bool transfer, void *res)
+// copies its sorted columns back to host memory (hbuf).  *loc (may be NULL)
+// receives where the whole result stays until the slot's next query.
+void enqueue_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_query &q, bool transfer, void *res,
+                   OutLoc *loc = nullptr)
 {
     char *b = (char *)buf;
+    OutLoc l;
     switch (q.kind) {
     case VO_Q_FILTER_SUM: {
         const FactTable &t = s->facts[q.table];
@@ -296,6 +308,10 @@ void enqueue_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_qu
         a.qty_lt = 24;
         op_filter_sum(ctx, a);
         VO_HIP(hipMemcpyAsync(res, a.out, 32, hipMemcpyDeviceToHost, ctx->stream));
+        // without output columns the network compacts the private copies in place
+        l.c0 = transfer ? (const void *)a.price : (const void *)a.out_price;
+        l.c1 = transfer ? (const void *)a.disc : (const void *)a.out_disc;
+        l.b0 = l.b1 = 8 * t.n;
         break;
     }
     case VO_Q_JOIN_GROUPSUM: {
@@ -356,10 +372,46 @@ void enqueue_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_qu
             VO_HIP(hipMemcpyAsync(hb + 8 * t.n, p, 4 * t.n, hipMemcpyDeviceToHost, ctx->stream));
         }
         VO_HIP(hipMemcpyAsync(res, part, 2 * kChecksumBlocks * 8, hipMemcpyDeviceToHost, ctx->stream));
+        l.c0 = k;
+        l.c1 = p;
+        l.b0 = 8 * t.n;
+        l.b1 = 4 * t.n;
         break;
     }
     default: throw Error{VO_ERR_INVALID, "unknown query kind"};
     }
+    if (loc) *loc = l;
+}
+
+// Bytes of a query's whole result (vo_query.out).
+uint64_t out_bytes(vo_server *s, const vo_query &q)
+{
+    switch (q.kind) {
+    case VO_Q_FILTER_SUM: return q.table < s->facts.size() ? 32 + 16 * s->facts[q.table].n : 0;
+    case VO_Q_JOIN_GROUPSUM: return q.table < s->joins.size() ? kMaxGroups * 12 + 8 : 0;
+    case VO_Q_SORT: return q.table < s->sorts.size() ? 12 * s->sorts[q.table].n : 0;
+    default: return 0;
+    }
+}
+
+// Copies a finished query's whole result into q.out (host memory): the
+// result words from the pinned buffer, the columns from the slot's device
+// buffer, which holds them until the slot's next query.
+void copy_out(vo_server *s, const vo_query &q, const void *res, const OutLoc &l)
+{
+    if (!q.out) return;
+    char *o = (char *)q.out;
+    switch (q.kind) {
+    case VO_Q_FILTER_SUM:
+        std::memcpy(o, res, 32);
+        o += 32;
+        break;
+    case VO_Q_JOIN_GROUPSUM: std::memcpy(o, res, kMaxGroups * 12 + 8); return;
+    default: break;
+    }
+    if (l.b0) VO_HIP(hipMemcpy(o, l.c0, l.b0, hipMemcpyDeviceToHost));
+    if (l.b1) VO_HIP(hipMemcpy(o + l.b0, l.c1, l.b1, hipMemcpyDeviceToHost));
+    (void)s;
 }
 
 // The result words of a finished query from its result buffer.
@@ -408,7 +460,8 @@ void finish_query(vo_server *s, const vo_query &q, const void *res, uint64_t r[4
 
 // Records the query's launches and copies on the slot stream as a graph
 // (nothing runs); false if the stream cannot be captured for it.
-bool capture_query(vo_server *s, vo_ctx *ctx, void *buf, void *res, const vo_query &q, hipGraphExec_t *exec)
+bool capture_query(vo_server *s, vo_ctx *ctx, void *buf, void *res, const vo_query &q, hipGraphExec_t *exec,
+                   OutLoc *loc)
 {
     hipGraph_t graph = nullptr;
     if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -417,7 +470,7 @@ bool capture_query(vo_server *s, vo_ctx *ctx, void *buf, void *res, const vo_que
     }
     bool ok = true;
     try {
-        enqueue_query(s, ctx, buf, nullptr, q, false, res);
+        enqueue_query(s, ctx, buf, nullptr, q, false, res, loc);
     } catch (const Error &) {
         ok = false;
     }
@@ -439,10 +492,12 @@ bool run_query(vo_server *s, uint32_t slot, const vo_query &q, bool transfer, ui
 {
     vo_ctx *ctx = s->slots[slot];
     void *buf = s->slot_buf[slot], *res = s->slot_res[slot];
+    OutLoc loc;
     if (transfer) {
-        enqueue_query(s, ctx, buf, s->slot_hbuf[slot], q, true, res);
+        enqueue_query(s, ctx, buf, s->slot_hbuf[slot], q, true, res, &loc);
         VO_HIP(hipStreamSynchronize(ctx->stream));
         finish_query(s, q, res, r);
+        copy_out(s, q, res, loc);
         return false;
     }
     auto &cache = s->graphs[slot];
@@ -453,17 +508,19 @@ bool run_query(vo_server *s, uint32_t slot, const vo_query &q, bool transfer, ui
         VO_HIP(hipGraphLaunch(it->second.exec, ctx->stream));
         VO_HIP(hipStreamSynchronize(ctx->stream));
         finish_query(s, q, res, r);
+        copy_out(s, q, res, it->second.loc);
         return true;
     }
-    enqueue_query(s, ctx, buf, nullptr, q, false, res);
+    enqueue_query(s, ctx, buf, nullptr, q, false, res, &loc);
     VO_HIP(hipStreamSynchronize(ctx->stream));
     finish_query(s, q, res, r);
+    copy_out(s, q, res, loc);
     if (fresh) return false; // a capture of this pair failed before: it stays eager
     vo_server::Graph g;
     g.scr = ctx->scratch;
     g.buf = buf;
     if (it != cache.end() && it->second.exec) (void)hipGraphExecDestroy(it->second.exec);
-    g.failed = !capture_query(s, ctx, buf, res, q, &g.exec);
+    g.failed = !capture_query(s, ctx, buf, res, q, &g.exec, &g.loc);
     cache[key] = g;
     return false;
 }
@@ -595,6 +652,45 @@ int vo_server_add_sort_table(vo_server *s, uint64_t seed, uint64_t n_rows, uint3
         VO_HIP(hipStreamSynchronize(ctx->stream));
         s->sorts.push_back(t);
         if (id) *id = (uint32_t)(s->sorts.size() - 1);
+    });
+}
+
+uint64_t vo_server_query_out_bytes(vo_server *s, const vo_query *q)
+{
+    return s && q ? out_bytes(s, *q) : 0;
+}
+
+int vo_server_table_column(vo_server *s, uint32_t kind, uint32_t table, uint32_t column, const void **host,
+                           uint64_t *rows)
+{
+    return sguard(s, [&] {
+        VO_CHECK(host && rows, VO_ERR_INVALID, "null output");
+        switch (kind) {
+        case VO_Q_FILTER_SUM: {
+            VO_CHECK(table < s->facts.size() && column < 4, VO_ERR_INVALID, "unknown fact table column");
+            const FactTable &t = s->facts[table];
+            const void *c[4] = {t.h_day, t.h_qty, t.h_price, t.h_disc};
+            *host = c[column];
+            *rows = t.n;
+            break;
+        }
+        case VO_Q_JOIN_GROUPSUM: {
+            VO_CHECK(table < s->joins.size() && column < 4, VO_ERR_INVALID, "unknown join table column");
+            const JoinTable &t = s->joins[table];
+            const void *c[4] = {t.h_dk, t.h_dg, t.h_fk, t.h_fv};
+            *host = c[column];
+            *rows = column < 2 ? t.n_dim : t.n_fact;
+            break;
+        }
+        case VO_Q_SORT: {
+            VO_CHECK(table < s->sorts.size() && column < 2, VO_ERR_INVALID, "unknown sort table column");
+            const SortTable &t = s->sorts[table];
+            *host = column == 0 ? (const void *)t.h_keys : (const void *)t.h_pay;
+            *rows = t.n;
+            break;
+        }
+        default: throw Error{VO_ERR_INVALID, "unknown query kind"};
+        }
     });
 }
 

@@ -22,8 +22,12 @@ using SortM8 = SortImpl<RecM8, 10, 2, 256, 2>;
 // Order-preserving unsigned image of a column value.  DESC columns are
 // bit-inverted, so lexicographic ascending order of the packed words is the
 // reference's column-by-column order with per-column direction
-// (sort.cuh:31-40).  FLOAT/DOUBLE map -0.0 to +0.0 (the reference treats
-// them as equal); NaN is not supported.
+// (sort.cuh:31-40).  FLOAT/DOUBLE map -0.0 to the image of +0.0, since the
+// reference compares them equal (compareSingle, sort.cuh:11-29); the sign of
+// a zero travels in a payload bit the comparison ignores, so the unpack pass
+// writes back the row's own bits.  NaN has no place in the reference's order
+// (it compares unequal and not less than everything); here it takes the IEEE
+// total-order image (above +inf, or below -inf for a negative NaN).
 __device__ __forceinline__ uint64_t load_raw(const void *col, uint32_t type, uint64_t i)
 {
     switch (type) {
@@ -93,6 +97,8 @@ struct PackSpec {
     uint32_t shift[VO_MAX_SORT_COLUMNS]; // left shift inside the word
     uint32_t desc[VO_MAX_SORT_COLUMNS];
     uint64_t fill[VO_MAX_SORT_COLUMNS];  // raw bits of the pad value
+    uint32_t negz[VO_MAX_SORT_COLUMNS];  // FLOAT/DOUBLE: payload bit that marks a -0.0 (0 for other types)
+    uint32_t idx_mask;                   // payload bits holding the row index
     uint32_t ncols;
     uint32_t nwords;  // words holding columns
     uint32_t nwrite;  // words written (>= nwords; the rest are zero)
@@ -111,15 +117,19 @@ __global__ void k_pack_keys(PackSpec ps, uint64_t *w0, uint64_t *w1, uint64_t *w
         uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const bool pad = i >= n;
         const uint64_t src = pad ? 0 : i;
+        uint32_t flags = 0;
         for (uint32_t c = 0; c < ps.ncols; ++c) {
             const uint64_t mask = ps.bits[c] == 64 ? ~0ull : ((1ull << ps.bits[c]) - 1);
-            const uint64_t raw = load_raw(ps.cols[c], ps.types[c], src);
-            uint64_t v = norm_bits(ps.types[c], pad ? ps.fill[c] : raw);
+            const uint64_t raw = pad ? ps.fill[c] : load_raw(ps.cols[c], ps.types[c], src);
+            // the sign of a zero: -0.0 is the sign bit alone
+            const uint64_t sign = ps.bits[c] == 64 ? 0x8000000000000000ull : 0x80000000ull;
+            flags |= (raw == sign) ? ps.negz[c] : 0u;
+            uint64_t v = norm_bits(ps.types[c], raw);
             v = ps.desc[c] ? (~v & mask) : v;
             acc[ps.word[c]] |= v << ps.shift[c];
         }
         for (uint32_t w = 0; w < ps.nwrite; ++w) ws[w][i] = acc[w];
-        idx[i] = (uint32_t)i;
+        idx[i] = (uint32_t)i | flags;
         tr.at(1, src); // columns read at row src (row 0 for pad rows: n is public)
         tr.at(2, i);   // key words and index written at row i
     }
@@ -134,13 +144,18 @@ __global__ void k_unpack_keys(PackSpec ps, const uint64_t *w0, const uint64_t *w
     const uint64_t *ws[8] = {w0, w1, w2, w3, w4, w5, w6, w7};
     dev::Atr tr(slot);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t pay = idx[i];
         for (uint32_t c = 0; c < ps.ncols; ++c) {
             const uint64_t mask = ps.bits[c] == 64 ? ~0ull : ((1ull << ps.bits[c]) - 1);
             uint64_t v = (ws[ps.word[c]][i] >> ps.shift[c]) & mask;
             v = ps.desc[c] ? (~v & mask) : v;
+            // a -0.0 row: the image of +0.0 with the sign bit put back (the
+            // image of -0.0 is that of +0.0 with the top bit flipped: 0x0..0
+            // -> 0x80..0 denormalises to +0.0, 0x7F..F to -0.0)
+            if (pay & ps.negz[c]) v = ps.bits[c] == 64 ? 0x7FFFFFFFFFFFFFFFull : 0x7FFFFFFFull;
             denorm_store(const_cast<void *>(ps.cols[c]), ps.types[c], i, v);
         }
-        out_idx[i] = (int64_t)idx[i];
+        out_idx[i] = (int64_t)(pay & ps.idx_mask);
         tr.at(1, i);
     }
     tr.done(dev::gtid());
@@ -216,6 +231,20 @@ void op_sort_multi(vo_ctx *ctx, const vo_sort_args &a)
     const uint32_t W = ps.nwords <= 1 ? 1 : ps.nwords <= 2 ? 2 : ps.nwords <= 4 ? 4 : 8;
     ps.nwrite = W;
     const uint64_t n_pad = a.n_padded_rows;
+    // The u32 payload holds the row index in its low log2(n_pad) bits; the
+    // bits above carry, per FLOAT/DOUBLE key column, whether the row's value
+    // is -0.0, so the unpack pass restores the sign the comparison ignored.
+    const int idx_bits = ilog2(n_pad);
+    ps.idx_mask = idx_bits >= 32 ? ~0u : (uint32_t)((1ull << idx_bits) - 1);
+    uint32_t n_fp = 0;
+    for (uint32_t c = 0; c < ps.ncols; ++c) {
+        if (a.col_types[c] != VO_T_FLOAT && a.col_types[c] != VO_T_DOUBLE) continue;
+        VO_CHECK((int)n_fp < 32 - idx_bits, VO_ERR_UNSUPPORTED,
+                 "more FLOAT/DOUBLE key columns than 32 - log2(n_padded_rows) payload bits can carry the sign "
+                 "of a zero for");
+        ps.negz[c] = 1u << (31 - n_fp);
+        ++n_fp;
+    }
     // key words, then the u32 row indices the sort carries
     uint64_t *words = (uint64_t *)scratch(ctx, sizeof(uint64_t) * W * n_pad + sizeof(uint32_t) * n_pad);
     uint32_t *idx = reinterpret_cast<uint32_t *>(words + (uint64_t)W * n_pad);
