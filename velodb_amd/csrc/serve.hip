@@ -396,8 +396,10 @@ uint64_t out_bytes(vo_server *s, const vo_query &q)
 
 // Copies a finished query's whole result into q.out (host memory): the
 // result words from the pinned buffer, the columns from the slot's device
-// buffer, which holds them until the slot's next query.
-void copy_out(vo_server *s, const vo_query &q, const void *res, const OutLoc &l)
+// buffer, which holds them until the slot's next query.  The copies run on
+// the slot's stream: a legacy-stream copy would have to wait for every
+// blocking stream, and another slot may be capturing its stream as a graph.
+void copy_out(vo_server *s, vo_ctx *ctx, const vo_query &q, const void *res, const OutLoc &l)
 {
     if (!q.out) return;
     char *o = (char *)q.out;
@@ -409,8 +411,9 @@ void copy_out(vo_server *s, const vo_query &q, const void *res, const OutLoc &l)
     case VO_Q_JOIN_GROUPSUM: std::memcpy(o, res, kMaxGroups * 12 + 8); return;
     default: break;
     }
-    if (l.b0) VO_HIP(hipMemcpy(o, l.c0, l.b0, hipMemcpyDeviceToHost));
-    if (l.b1) VO_HIP(hipMemcpy(o + l.b0, l.c1, l.b1, hipMemcpyDeviceToHost));
+    if (l.b0) VO_HIP(hipMemcpyAsync(o, l.c0, l.b0, hipMemcpyDeviceToHost, ctx->stream));
+    if (l.b1) VO_HIP(hipMemcpyAsync(o + l.b0, l.c1, l.b1, hipMemcpyDeviceToHost, ctx->stream));
+    if (l.b0 || l.b1) VO_HIP(hipStreamSynchronize(ctx->stream));
     (void)s;
 }
 
@@ -497,7 +500,7 @@ bool run_query(vo_server *s, uint32_t slot, const vo_query &q, bool transfer, ui
         enqueue_query(s, ctx, buf, s->slot_hbuf[slot], q, true, res, &loc);
         VO_HIP(hipStreamSynchronize(ctx->stream));
         finish_query(s, q, res, r);
-        copy_out(s, q, res, loc);
+        copy_out(s, ctx, q, res, loc);
         return false;
     }
     auto &cache = s->graphs[slot];
@@ -508,13 +511,13 @@ bool run_query(vo_server *s, uint32_t slot, const vo_query &q, bool transfer, ui
         VO_HIP(hipGraphLaunch(it->second.exec, ctx->stream));
         VO_HIP(hipStreamSynchronize(ctx->stream));
         finish_query(s, q, res, r);
-        copy_out(s, q, res, it->second.loc);
+        copy_out(s, ctx, q, res, it->second.loc);
         return true;
     }
     enqueue_query(s, ctx, buf, nullptr, q, false, res, &loc);
     VO_HIP(hipStreamSynchronize(ctx->stream));
     finish_query(s, q, res, r);
-    copy_out(s, q, res, loc);
+    copy_out(s, ctx, q, res, loc);
     if (fresh) return false; // a capture of this pair failed before: it stays eager
     vo_server::Graph g;
     g.scr = ctx->scratch;
