@@ -497,6 +497,19 @@ VO_EXPORT int vo_transfer_to_host(vo_ctx *ctx, void *dst_host, const void *src_d
 /* Asynchronous forms: enqueue on the context stream and return. */
 VO_EXPORT int vo_sort_kp_async(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad);
 
+/* Allocation-free compaction.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as vo_compact, but asynchronous (enqueued on the context stream;
+ * synchronise the stream or call vo_sync before reading the results), and
+ * its tile arrays live in a fixed 2 MiB device array of the library instead
+ * of the context's scratch arena, so the call allocates nothing.
+ * Limits: n_pad <= 2^29 (VO_ERR_INVALID above); tracing and kernel timing
+ * must be off (they allocate).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other contexts,
+ * other threads) race and must be serialised by the caller. */
+VO_EXPORT int vo_compact_noalloc(vo_ctx *ctx, const vo_compact_args *args);
+
 
 /* --------------------------------------------------------- diagnostics */
 

@@ -202,7 +202,9 @@ class Context:
         a = _abi.PermuteArgs(_ptr(out), _ptr(inp), _ptr(idx), n, type_id)
         return self.check(self.lib.vo_permute(self.h, C.byref(a)))
 
-    def compact(self, mask, cols, col_bytes, n, n_pad, out_count=None):
+    def compact(self, mask, cols, col_bytes, n, n_pad, out_count=None, noalloc=False):
+        """vo_compact; with noalloc, vo_compact_noalloc (asynchronous: the
+        results are ready after sync())."""
         a = _abi.CompactArgs()
         a.mask = _ptr(mask)
         for i, c in enumerate(cols):
@@ -211,7 +213,8 @@ class Context:
         a.n_cols = len(cols)
         a.n, a.n_pad = n, n_pad
         a.out_count = _ptr(out_count)
-        return self.check(self.lib.vo_compact(self.h, C.byref(a)))
+        fn = self.lib.vo_compact_noalloc if noalloc else self.lib.vo_compact
+        return self.check(fn(self.h, C.byref(a)))
 
     def shuffle(self, payload, n_pad, seed, perm_out=None, n=0):
         a = _abi.ShuffleArgs(_ptr(payload), _ptr(perm_out), n_pad, seed, n)

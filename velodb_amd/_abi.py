@@ -212,6 +212,7 @@ PROTOTYPES = [
     ("vo_sort", C.c_int, [P, C.POINTER(SortArgs)]),
     ("vo_permute", C.c_int, [P, C.POINTER(PermuteArgs)]),
     ("vo_compact", C.c_int, [P, C.POINTER(CompactArgs)]),
+    ("vo_compact_noalloc", C.c_int, [P, C.POINTER(CompactArgs)]),
     ("vo_shuffle", C.c_int, [P, C.POINTER(ShuffleArgs)]),
     ("vo_filter_sum", C.c_int, [P, C.POINTER(FilterArgs)]),
     ("vo_pkfk_join_groupsum", C.c_int, [P, C.POINTER(JoinArgs)]),

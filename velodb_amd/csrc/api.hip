@@ -250,6 +250,7 @@ int vo_ctx_create(const vo_config *cfg, vo_ctx **out)
         VO_HIP(hipGetDeviceProperties(&prop, dev));
         ctx->num_cus = prop.multiProcessorCount;
         ctx->tuning = apply_tuning_word(ctx->tuning, c.tuning);
+        noalloc_scratch_init(dev);
     } catch (const Error &e) {
         delete ctx;
         return e.status;
@@ -377,6 +378,19 @@ int vo_sort_kp(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint6
     c.opcode = VO_OP_SORT_KP;
     c.args.sort_kp = {keys, payload, n, n_pad};
     return run_sync(ctx, c);
+}
+
+int vo_compact_noalloc(vo_ctx *ctx, const vo_compact_args *args)
+{
+    if (!args) return VO_ERR_INVALID;
+    if (ctx && ctx->queue) {
+        const int st = queue_park(ctx);
+        if (st != VO_OK) return st;
+    }
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        op_compact_noalloc(ctx, *args);
+    });
 }
 
 int vo_sort_kp_async(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad)

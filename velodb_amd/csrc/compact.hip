@@ -20,6 +20,16 @@
 #include "vo_internal.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
+
+// Fixed device scratch of vo_compact_noalloc: the tile counts, the tile
+// prefix array and the count of one compaction of at most 2^29 rows
+// (1.53 MiB at 2^29; the array keeps 2 MiB).  Each device holds its own copy
+// for the life of the process.  Host code never passes the array's name to a
+// kernel (in host code it is a stand-in, not a device address): it uses the
+// address hipGetSymbolAddress returned on that device (noalloc_scratch_init).
+static __device__ __align__(256) unsigned char scratch[2u << 20];
 
 namespace vo {
 namespace {
@@ -761,21 +771,22 @@ size_t compact_rows2_scratch(uint64_t n_pad)
     return 8 + nt * 4 + 256 + (nt + 1) * 8 + 256;
 }
 
-void op_compact(vo_ctx *ctx, const vo_compact_args &a)
+namespace {
+
+// The compaction's passes over a workspace the caller provides.
+void compact_with(vo_ctx *ctx, const vo_compact_args &a, const Workspace &ws)
 {
-    check_pad(a.n, a.n_pad);
     VO_CHECK(a.mask && a.n_cols >= 1 && a.n_cols <= VO_MAX_SORT_COLUMNS, VO_ERR_INVALID, "bad compaction columns");
-    Workspace ws = workspace(ctx, a.n_pad, false);
+    for (uint32_t c = 0; c < a.n_cols; ++c) {
+        const uint32_t w = a.col_bytes[c];
+        VO_CHECK(a.cols[c] && (w == 1 || w == 2 || w == 4 || w == 8), VO_ERR_INVALID, "bad column");
+    }
     count_and_scan(ctx, a.mask, a.n, a.n_pad, ws, a.out_count);
     // columns in pairs of two words per network run; the network depends on
     // the mask only, so each run applies the same routing
     uint32_t c = 0;
     while (c < a.n_cols) {
         const uint32_t take = std::min<uint32_t>(2, a.n_cols - c);
-        for (uint32_t k = 0; k < take; ++k) {
-            const uint32_t w = a.col_bytes[c + k];
-            VO_CHECK(a.cols[c + k] && (w == 1 || w == 2 || w == 4 || w == 8), VO_ERR_INVALID, "bad column");
-        }
         if (take == 2) {
             Cols<2> cs{{{a.cols[c], a.col_bytes[c]}, {a.cols[c + 1], a.col_bytes[c + 1]}}};
             run_network<2>(ctx, cs, cs, a.mask, a.n, a.n_pad, ws.tile_pre);
@@ -785,6 +796,57 @@ void op_compact(vo_ctx *ctx, const vo_compact_args &a)
         }
         c += take;
     }
+}
+
+} // namespace
+
+void op_compact(vo_ctx *ctx, const vo_compact_args &a)
+{
+    check_pad(a.n, a.n_pad);
+    compact_with(ctx, a, workspace(ctx, a.n_pad, false));
+}
+
+namespace {
+
+constexpr uint64_t kNoAllocMaxRows = 1ull << 29;
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+std::atomic<char *> g_noalloc_ptr[kMaxDevices];
+
+} // namespace
+
+// Called by vo_ctx_create with `device` current: looks up this device's copy
+// of the fixed scratch once and keeps its address.
+void noalloc_scratch_init(int device)
+{
+    VO_CHECK(device >= 0 && device < kMaxDevices, VO_ERR_INVALID, "device ordinal above 63");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device].load(std::memory_order_acquire)) return;
+    void *p = nullptr;
+    VO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+    VO_CHECK(p != nullptr, VO_ERR_HIP, "hipGetSymbolAddress returned no address for the fixed scratch");
+    g_noalloc_ptr[device].store((char *)p, std::memory_order_release);
+}
+
+// op_compact over the fixed scratch instead of the context arena: nothing is
+// allocated.  Every array in it is written by the count and scan passes
+// before it is read, so nothing needs clearing between calls.
+void op_compact_noalloc(vo_ctx *ctx, const vo_compact_args &a)
+{
+    check_pad(a.n, a.n_pad);
+    VO_CHECK(a.n_pad <= kNoAllocMaxRows, VO_ERR_INVALID,
+             "vo_compact_noalloc: n_pad above 2^29 (its fixed scratch holds the tile arrays of at most 2^29 rows; "
+             "use vo_compact)");
+    VO_CHECK(!ctx->tracing && !(ctx->flags & VO_CFG_KERNEL_TIMING), VO_ERR_INVALID,
+             "vo_compact_noalloc: tracing and kernel timing allocate; turn them off for this entry point");
+    VO_CHECK(ctx->device >= 0 && ctx->device < kMaxDevices, VO_ERR_INVALID, "device ordinal above 63");
+    char *base = g_noalloc_ptr[ctx->device].load(std::memory_order_acquire);
+    VO_CHECK(base != nullptr, VO_ERR_INVALID, "vo_compact_noalloc: fixed scratch not set up on this device");
+    VO_CHECK(workspace_at(nullptr, a.n_pad, false, nullptr) <= sizeof(::scratch), VO_ERR_INVALID,
+             "vo_compact_noalloc: workspace exceeds the fixed scratch");
+    Workspace ws;
+    workspace_at(base, a.n_pad, false, &ws);
+    compact_with(ctx, a, ws);
 }
 
 namespace {

@@ -193,6 +193,10 @@ void op_compare_eq(vo_ctx *ctx, const vo_compare_eq_args &a);
 void op_compose(vo_ctx *ctx, const vo_compose_args &a);
 void op_enrich(vo_ctx *ctx, const vo_enrich_args &a);
 void op_compact(vo_ctx *ctx, const vo_compact_args &a);
+// op_compact over the fixed device scratch of compact.hip (allocates nothing)
+void op_compact_noalloc(vo_ctx *ctx, const vo_compact_args &a);
+// looks up `device`'s copy of that scratch (device current); once per device
+void noalloc_scratch_init(int device);
 void op_shuffle(vo_ctx *ctx, const vo_shuffle_args &a);
 void op_filter_sum(vo_ctx *ctx, const vo_filter_args &a);
 void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a);
