@@ -424,6 +424,9 @@ typedef struct vo_config {
     uint32_t tuning;         /* 0 -> autotuned table / built-in defaults */
     uint32_t queue_idle_us;  /* the resident queue kernel leaves after this long without a
                                 command (0 -> 20000); the next vo_submit starts it again */
+    uint32_t queue_lanes;    /* lanes of the resident queue kernel, 1..8 (0 -> 1): the resident
+                                blocks are split into lanes that run their own commands side by
+                                side (vo_submit_lane) */
 } vo_config;
 
 typedef struct vo_ctx vo_ctx;
@@ -455,6 +458,15 @@ VO_EXPORT int vo_queue_running(const vo_ctx *ctx);
  * sequence id is that of a NOP queued after it, so the order of commands
  * and the meaning of vo_wait are unchanged. */
 VO_EXPORT uint64_t vo_submit(vo_ctx *ctx, vo_command *cmd);
+/* vo_submit on one lane of the resident queue (0 <= lane < vo_queue_lanes).
+ * Commands of one lane run in submission order; commands of different lanes
+ * run side by side and may finish in any order (vo_wait / vo_poll take each
+ * command's own sequence id).  Lanes of one context share its buffers, so
+ * commands that touch the same memory go on the same lane.  No counterpart
+ * in the reference, whose queue runs one command at a time
+ * (persistent_kernel.cu:18-124); used by the serving loop. */
+VO_EXPORT uint64_t vo_submit_lane(vo_ctx *ctx, vo_command *cmd, uint32_t lane);
+VO_EXPORT int vo_queue_lanes(const vo_ctx *ctx); /* 0 when the resident queue is not running */
 VO_EXPORT int vo_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms);
 VO_EXPORT int vo_poll(vo_ctx *ctx, uint64_t seq); /* 1 done, 0 pending, <0 -error */
 VO_EXPORT int vo_sync(vo_ctx *ctx);               /* wait for everything submitted */

@@ -254,19 +254,23 @@ def test_queue_poll_alone_restarts_an_idle_kernel(torch_dev):
 
 
 def test_queue_parks_for_launch_only_call(qctx):
-    """A synchronous launch-only operator in persistent mode first finishes
-    the submitted commands and lets the resident kernel leave; the next
-    submit starts it again."""
+    """An entry point that runs in launch mode only (here the allocation-free
+    compaction) first finishes the submitted commands and lets the resident
+    kernel leave; vo_sync also waits for its work; the next submit starts the
+    resident kernel again."""
     n = 6000
     k, p = oracle.gen_sort_input(77, n)
     dk, dp = to_dev(k, 8192), to_dev(p, 8192)
-    pay = to_dev(np.arange(1 << 12, dtype=np.uint32))
+    vals = np.arange(1 << 12, dtype=np.uint64) * 3
+    mask = (np.arange(1 << 12) % 3 == 0).astype(np.uint8)
+    dv, dm = to_dev(vals), to_dev(mask)
     _sync()
     s1 = qctx.submit(_sort_cmd(dk, dp, n, 8192))
-    qctx.shuffle(pay, 1 << 12, 5)  # launch-only: parks the queue first
+    qctx.compact(dm, [dv], [8], 1 << 12, 1 << 12, noalloc=True)  # launch mode: parks the queue first
     assert qctx.poll(s1)
     assert not qctx.queue_running()
-    assert sorted(_host(pay, np.uint32).tolist()) == list(range(1 << 12))
+    qctx.sync()
+    assert np.array_equal(_host(dv, np.uint64)[:int(mask.sum())], vals[mask != 0])
     ek, ep = oracle.bitonic_sort_kp(k, p, n, 8192)
     assert np.array_equal(_host(dk, np.uint64), ek) and np.array_equal(_host(dp, np.uint32), ep)
     k2, p2 = oracle.gen_sort_input(78, n)

@@ -148,9 +148,10 @@ class Context:
     """
 
     def __init__(self, stream=None, flags=0, device=-1, queue_capacity=0, wait_timeout_ms=0, tuning=0,
-                 queue_idle_us=0):
+                 queue_idle_us=0, queue_lanes=0):
         self.lib = load_library()
-        cfg = _abi.Config(queue_capacity, flags, stream, device, wait_timeout_ms, tuning, queue_idle_us)
+        cfg = _abi.Config(queue_capacity, flags, stream, device, wait_timeout_ms, tuning, queue_idle_us,
+                          queue_lanes)
         h = C.c_void_p()
         st = self.lib.vo_ctx_create(C.byref(cfg), C.byref(h))
         if st != 0:
@@ -326,11 +327,18 @@ class Context:
             raise VoError(-r, self.lib.vo_last_error(self.h).decode())
         return bool(r)
 
-    def submit(self, cmd):
-        s = self.lib.vo_submit(self.h, C.byref(cmd))
+    def submit(self, cmd, lane=None):
+        """vo_submit, or vo_submit_lane on one lane of the resident queue."""
+        if lane is None:
+            s = self.lib.vo_submit(self.h, C.byref(cmd))
+        else:
+            s = self.lib.vo_submit_lane(self.h, C.byref(cmd), lane)
         if s == 0:
             raise VoError(-1, self.lib.vo_last_error(self.h).decode())
         return s
+
+    def queue_lanes(self):
+        return self.lib.vo_queue_lanes(self.h)
 
     def wait(self, seq, timeout_ms=0):
         return self.check(self.lib.vo_wait(self.h, seq, timeout_ms))

@@ -164,7 +164,8 @@ class Command(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("queue_capacity", u32), ("flags", u32), ("stream", P), ("device", i32),
-                ("wait_timeout_ms", u32), ("tuning", u32), ("queue_idle_us", u32)]
+                ("wait_timeout_ms", u32), ("tuning", u32), ("queue_idle_us", u32),
+                ("queue_lanes", u32)]
 
 
 Q_FILTER_SUM, Q_JOIN_GROUPSUM, Q_SORT = 1, 2, 3
@@ -205,6 +206,8 @@ PROTOTYPES = [
     ("vo_queue_stop", C.c_int, [P, u32]),
     ("vo_queue_running", C.c_int, [P]),
     ("vo_submit", u64, [P, C.POINTER(Command)]),
+    ("vo_submit_lane", u64, [P, C.POINTER(Command), u32]),
+    ("vo_queue_lanes", C.c_int, [P]),
     ("vo_wait", C.c_int, [P, u64, u32]),
     ("vo_poll", C.c_int, [P, u64]),
     ("vo_sync", C.c_int, [P]),

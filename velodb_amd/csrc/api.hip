@@ -16,22 +16,135 @@ using namespace vo;
 
 namespace vo {
 
+// The arena of the current command: the context's own in launch mode and
+// on queue lane 0, a lane's own on the other lanes (commands of different
+// lanes run at the same time).
 void *scratch(vo_ctx *ctx, size_t bytes)
 {
-    if (bytes <= ctx->scratch_bytes) return ctx->scratch;
-    VO_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->scratch) VO_HIP(hipFree(ctx->scratch));
-    ctx->scratch = nullptr;
-    ctx->scratch_bytes = 0;
+    const uint32_t lane = ctx->rec ? ctx->rec->lane : 0;
+    void *&arena = lane ? ctx->lane_scratch[lane] : ctx->scratch;
+    size_t &have = lane ? ctx->lane_scratch_bytes[lane] : ctx->scratch_bytes;
+    if (bytes <= have) return arena;
+    // commands already submitted may still use the old arena
+    if (ctx->rec)
+        queue_drain_locked(ctx);
+    else
+        VO_HIP(hipStreamSynchronize(ctx->stream));
+    if (arena) VO_HIP(hipFree(arena));
+    arena = nullptr;
+    have = 0;
     size_t want = bytes + (bytes >> 3);
     want = (want + 0xFFFFF) & ~(size_t)0xFFFFF;
-    hipError_t e = hipMalloc(&ctx->scratch, want);
+    hipError_t e = hipMalloc(&arena, want);
     if (e != hipSuccess) {
         (void)hipGetLastError();
+        arena = nullptr;
         throw Error{VO_ERR_NOMEM, "scratch arena: hipMalloc of " + std::to_string(want) + " bytes failed"};
     }
-    ctx->scratch_bytes = want;
-    return ctx->scratch;
+    have = want;
+    return arena;
+}
+
+// ------------------------------------------------------ resident programs
+
+namespace {
+std::mutex g_body_mu;
+std::vector<std::pair<const void *, BodyInfo>> &body_table()
+{
+    static std::vector<std::pair<const void *, BodyInfo>> t;
+    return t;
+}
+} // namespace
+
+void register_bodies(const std::pair<const void *, BodyInfo> *entries, size_t n)
+{
+    std::lock_guard<std::mutex> lk(g_body_mu);
+    auto &t = body_table();
+    t.insert(t.end(), entries, entries + n);
+}
+
+const BodyInfo *find_body(const void *kernel)
+{
+    std::lock_guard<std::mutex> lk(g_body_mu);
+    for (const auto &e : body_table())
+        if (e.first == kernel) return &e.second;
+    return nullptr;
+}
+
+void fill_async(vo_ctx *ctx, void *dst, int value, uint64_t bytes)
+{
+    if (bytes == 0) return;
+    if (!ctx->rec) {
+        VO_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+        return;
+    }
+    Step s{};
+    s.id = kStepFill;
+    s.grid = 1;
+    s.slots = 1;
+    s.aux = (uint16_t)(value & 0xFF);
+    const FillArgs a{dst, bytes};
+    std::memcpy(s.blob, &a, sizeof a);
+    ctx->rec->steps.push_back(s);
+}
+
+void copy_async(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    if (bytes == 0) return;
+    if (!ctx->rec) {
+        VO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        return;
+    }
+    Step s{};
+    s.id = kStepCopy;
+    s.grid = 1;
+    s.slots = 1;
+    const CopyArgs a{dst, src, bytes};
+    std::memcpy(s.blob, &a, sizeof a);
+    ctx->rec->steps.push_back(s);
+}
+
+void upload_async(vo_ctx *ctx, void *dst, const void *host_src, uint64_t bytes)
+{
+    if (bytes == 0) return;
+    if (!ctx->rec) {
+        VO_HIP(hipMemcpyAsync(dst, host_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return;
+    }
+    // the bytes travel with the program (host-mapped); a copy step moves them
+    Recorder &r = *ctx->rec;
+    const size_t off = (r.data.size() + 15) & ~(size_t)15;
+    r.data.resize(off + bytes);
+    std::memcpy(r.data.data() + off, host_src, bytes);
+    Step s{};
+    s.id = kStepCopy;
+    s.grid = 1;
+    s.slots = 1;
+    const CopyArgs a{dst, nullptr, bytes};
+    std::memcpy(s.blob, &a, sizeof a);
+    r.steps.push_back(s);
+    r.data_refs.emplace_back(r.steps.size() - 1, off);
+}
+
+} // namespace vo
+
+namespace {
+void dispatch(vo_ctx *ctx, const vo_command &cmd);
+} // namespace
+
+namespace vo {
+
+void record_command(vo_ctx *ctx, const vo_command &cmd, Recorder &rec)
+{
+    VO_CHECK(!ctx->tracing, VO_ERR_UNSUPPORTED, "tracing runs in launch mode only");
+    ctx->rec = &rec;
+    try {
+        dispatch(ctx, cmd);
+    } catch (...) {
+        ctx->rec = nullptr;
+        throw;
+    }
+    ctx->rec = nullptr;
 }
 
 TraceSlot *trace_slot(vo_ctx *ctx)
@@ -238,6 +351,7 @@ int vo_ctx_create(const vo_config *cfg, vo_ctx **out)
         VO_CHECK(is_pow2(ctx->queue_capacity), VO_ERR_INVALID, "queue_capacity must be a power of two");
         ctx->wait_timeout_ms = c.wait_timeout_ms ? c.wait_timeout_ms : 60000;
         ctx->queue_idle_us = c.queue_idle_us ? c.queue_idle_us : 20000;
+        ctx->queue_lanes = c.queue_lanes ? c.queue_lanes : 1;
         if (c.stream) {
             ctx->stream = (hipStream_t)c.stream;
         } else {
@@ -272,6 +386,8 @@ int vo_ctx_destroy(vo_ctx *ctx)
         (void)hipEventDestroy(t.stop);
     }
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    for (void *p : ctx->lane_scratch)
+        if (p) (void)hipFree(p);
     join_pairs_state_free(ctx);
     staging_free(ctx);
     if (ctx->trace_slots) (void)hipFree(ctx->trace_slots);
@@ -284,22 +400,34 @@ const char *vo_last_error(const vo_ctx *ctx) { return ctx ? ctx->last_error.c_st
 
 static int run_sync(vo_ctx *ctx, vo_command &cmd);
 
+// Persistent mode, a command whose program cannot be recorded (a kernel
+// without a resident body): the submitted commands finish, the resident
+// kernel leaves, the command runs on the context stream to completion, and a
+// NOP through the ring gives it the next sequence id, so waits and ordering
+// stay those of the queue.
+static int run_parked(vo_ctx *ctx, const vo_command &cmd)
+{
+    const int st = queue_park(ctx);
+    if (st != VO_OK) return st;
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        dispatch(ctx, cmd);
+        VO_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 uint64_t vo_submit(vo_ctx *ctx, vo_command *cmd)
 {
     if (!ctx || !cmd) return 0;
     if (ctx->queue) {
-        if (queue_supports(cmd->opcode)) return queue_submit(ctx, cmd);
-        // an operator the resident kernel does not run: the submitted
-        // commands finish, it runs on the context stream to completion, and a
-        // NOP through the ring gives it the next sequence id, so waits and
-        // ordering stay those of the queue (the reference submits every
-        // operator to its queue, sort_merge_join_operator.cpp:113-149)
-        if (run_sync(ctx, *cmd) != VO_OK) return 0;
+        const uint64_t seq = queue_submit(ctx, cmd);
+        if (seq != 0 || ctx->last_status != VO_ERR_UNSUPPORTED) return seq;
+        if (run_parked(ctx, *cmd) != VO_OK) return 0;
         vo_command nop{};
         nop.opcode = VO_OP_NOP;
-        const uint64_t seq = queue_submit(ctx, &nop);
-        cmd->sequence_id = seq;
-        return seq;
+        const uint64_t s2 = queue_submit(ctx, &nop);
+        cmd->sequence_id = s2;
+        return s2;
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
     try {
@@ -343,6 +471,8 @@ int vo_sync(vo_ctx *ctx)
         if (ctx->queue) {
             int st = queue_sync(ctx);
             if (st != VO_OK) throw Error{st, ctx->last_error};
+            // and the asynchronous launch-mode calls (they park the queue)
+            VO_HIP(hipStreamSynchronize(ctx->stream));
             return;
         }
         VO_HIP(hipStreamSynchronize(ctx->stream));
@@ -354,19 +484,6 @@ int vo_sync(vo_ctx *ctx)
 static int run_sync(vo_ctx *ctx, vo_command &cmd)
 {
     if (!ctx) return VO_ERR_INVALID;
-    if (ctx->queue && !queue_supports(cmd.opcode)) {
-        // persistent mode, operator the resident kernel does not run: finish
-        // the submitted commands, let the kernel leave (its registers and LDS
-        // would keep these kernels from starting) and launch on the context
-        // stream; the next vo_submit starts the resident kernel again
-        const int st = queue_park(ctx);
-        if (st != VO_OK) return st;
-        return guarded(ctx, [&] {
-            std::lock_guard<std::mutex> lk(ctx->mu);
-            dispatch(ctx, cmd);
-            VO_HIP(hipStreamSynchronize(ctx->stream));
-        });
-    }
     uint64_t s = vo_submit(ctx, &cmd);
     if (s == 0) return ctx->last_status ? ctx->last_status : VO_ERR_INVALID;
     return vo_wait(ctx, s, 0);

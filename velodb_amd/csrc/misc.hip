@@ -8,12 +8,18 @@
 // PERMUTE, GATHER, COMPOSE and ENRICH read or write at data-dependent
 // addresses exactly as the reference does; the oblivious paths (sort,
 // compaction, shuffle) never call them.
-#include "vo_internal.hpp"
+#include "misc_kernels.hpp"
+#include "resident_tables.hpp"
 
 #include <algorithm>
 
 namespace vo {
 namespace {
+
+using namespace misc_k;
+
+const std::pair<const void *, BodyInfo> kBodies[] = {VO_MISC_BODIES(VO_BODY_ENTRY)};
+const bool kRegistered = (register_bodies(kBodies, sizeof kBodies / sizeof kBodies[0]), true);
 
 size_t type_bytes(uint32_t t)
 {
@@ -30,44 +36,6 @@ dim3 grid_for(uint64_t n, uint32_t block = 256, uint64_t cap = 16384)
 {
     uint64_t b = (n + block - 1) / block;
     return dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, cap)));
-}
-
-template <typename T>
-__global__ void k_permute(T *__restrict__ out, const T *__restrict__ in, const int64_t *__restrict__ idx, uint64_t n)
-{
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        out[i] = in[idx[i]];
-}
-
-template <typename T>
-__global__ void k_gather(T *__restrict__ out, const T *__restrict__ in, const int32_t *__restrict__ pos,
-                         const uint8_t *__restrict__ mask, uint64_t n)
-{
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const bool keep = (mask == nullptr) || (mask[i] != 0);
-        if (keep) out[pos[i]] = in[i];
-    }
-}
-
-template <typename T>
-__global__ void k_compare_eq(const T *__restrict__ a, const T *__restrict__ b, uint8_t *__restrict__ out, uint64_t n)
-{
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        out[i] = a[i] == b[i] ? 1 : 0;
-}
-
-__global__ void k_compose(const uint32_t *__restrict__ old_map, const uint32_t *__restrict__ sigma,
-                          uint32_t *__restrict__ new_map, uint64_t n)
-{
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        new_map[i] = old_map[sigma[i]];
-}
-
-__global__ void k_enrich(const uint32_t *__restrict__ rid, const uint32_t *__restrict__ map,
-                         uint32_t *__restrict__ pos, uint64_t n)
-{
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        pos[i] = map[rid[i]];
 }
 
 __global__ void k_gen_sort(uint64_t seed, uint64_t *__restrict__ keys, uint32_t *__restrict__ payload, uint64_t n)

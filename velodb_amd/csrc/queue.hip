@@ -4,29 +4,33 @@
 // (include/cuda/commands.hpp:16-201, src/cuda/commands.cpp:11-91,
 // src/cuda/persistent_kernel.cu:18-124, src/cuda/task_manager.cu:26-124):
 // a ring of Commands in mapped host memory, one cooperative kernel polling
-// it, grid-wide barriers between commands.
+// it, grid-wide barriers between commands, in-order completion.
 //
 // Here:
-//   * ring and control words live in fine-grained host-coherent memory
-//     (hipHostMalloc coherent + mapped); the host writes a command, then
-//     publishes `head` with a release store; the device leader reads `head`
-//     with a system-scope acquire load and copies the command into device
-//     memory for every block;
-//   * one cooperative launch of 512-thread blocks, as many as fit on every
-//     CU (all co-resident), runs the same device bodies as the launch-mode
-//     kernels, separated by a sense-counting grid barrier in device memory;
-//   * completion is a per-command count (`done`) published with a system
-//     release after the results are written back;
-//   * every wait on the device is bounded: the leader exits after an idle
-//     period, and a barrier that does not complete in time aborts the
-//     kernel, so the grid always drains.  The host relaunches on the next
-//     submit after an idle exit.
-// Commands available in the resident kernel: NOP, SORT_KP, PERMUTE, GATHER,
-// COMPARE_EQ, COMPOSE_POSITION_MAP, ENRICH_POSITIONS, TERMINATE; the others
-// run in launch mode.
+//   * every opcode runs resident: vo_submit records the command's program
+//     (resident.hpp: the launches its host driver would make, as steps) into
+//     a host-mapped arena and publishes a ring entry; the resident kernel
+//     copies the program into device memory and runs it step by step, each
+//     step's virtual blocks spread over the lane's blocks, with a barrier of
+//     the lane's blocks between steps;
+//   * the grid is split into lanes (block b serves lane b mod L, so with
+//     L | 8 a lane keeps to its own XCDs and their L2), each with its own
+//     ring, barrier and leader: commands of different lanes run side by side
+//     and finish out of order.  Commands of one lane run in order;
+//   * completion is per command: the lane leader stores the command's
+//     sequence id in flag[seq mod capacity] (fine-grained host memory) after
+//     the results are written back, and bumps its lane's `done`;
+//   * every wait on the device is bounded: the kernel leaves when every lane
+//     has been idle for the idle period, and a barrier that does not
+//     complete in time aborts the kernel, so the grid always drains.  The
+//     host relaunches on the next submit or wait after an idle exit.
+#include "misc_kernels.hpp"
+#include "resident_tables.hpp"
 #include "sort_kernels.hpp"
+#include "sort_multi_kernels.hpp"
 
 #include <chrono>
+#include <deque>
 #include <thread>
 
 namespace vo {
@@ -34,40 +38,74 @@ namespace {
 
 using namespace sortimpl;
 
-constexpr int QBLOCK = 512;
-constexpr int QT_LOG = 13, QR_LOG = 4;
-constexpr int kQueueSMax = 4, kQueueHMax = 7; // widths 1-4 and 6-7 are built here
-using QTile = dev::TileCT<RecKP, QT_LOG, QR_LOG, false>;
+constexpr uint32_t RB = kResidentThreads;
+constexpr uint32_t kMaxLanes = 8;
+constexpr uint32_t kMaxQueueBlocks = 2048;
+constexpr uint32_t kFlagStride = 16;   // u32 words per arrival flag: one 64-byte line
+constexpr uint32_t kMaxSteps = 4096;   // steps of one command (1 MiB of device program per lane)
+constexpr uint64_t kArenaSteps = 32768; // host program arena (8 MiB)
+
+// One ring entry (fine-grained host memory).
+struct QEntry {
+    uint64_t seq;      // global sequence id
+    uint32_t opcode;
+    uint32_t n_steps;  // program length (0 for NOP / TERMINATE)
+    const Step *prog;  // host-mapped program
+    uint64_t pad_[5];
+};
+static_assert(sizeof(QEntry) == 64, "ring entries are one line");
+
+struct QLaneHost {
+    uint64_t head; // entries published on the lane
+    uint64_t done; // entries completed on the lane
+    uint64_t pad_[6];
+};
 
 // Control words shared with the host (fine-grained host memory).
 struct QCtrl {
-    uint64_t head;  // commands published by the host
-    uint64_t done;  // commands completed by the device
-    uint32_t stop;  // host asks the kernel to exit
-    uint32_t alive; // 1 while the kernel runs
-    uint32_t error; // device error code (0 = none, 1 = barrier timeout)
+    QLaneHost lane[kMaxLanes];
+    uint32_t stop;        // host asks the kernel to leave (after draining published entries)
+    uint32_t alive;       // 1 while the kernel runs
+    uint32_t error;       // 1 = a barrier timed out
     uint32_t exit_reason; // 1 stop, 2 idle, 3 terminate, 4 abort
 };
 
-static_assert(sizeof(vo_command) % 8 == 0, "commands are copied in 8-byte words");
-
-// Grid coordination state (device memory).
-// Each block announces its arrival at a barrier in its own 64-byte line of
-// `arrive` (no two blocks contend for one address); block 0 collects the
-// arrivals with one wavefront of parallel loads and releases the grid with
-// one store to `release`.
-constexpr uint32_t kMaxQueueBlocks = 2048;
-constexpr uint32_t kFlagStride = 16; // u32 words per flag: one 64-byte line
+// Per-lane device state (persists across launches).
+struct QLaneDev {
+    uint32_t release; // barrier generation released by the lane leader
+    uint32_t exit;    // exit code decided by the leader's fetch
+    uint32_t epoch;   // barrier generation when the lane last left
+    uint32_t n_steps; // program length of the current entry
+    uint64_t consumed;
+    uint64_t seq;     // sequence id of the current entry
+    uint64_t prog;    // host address of its program
+    uint32_t opcode;
+    uint32_t pad_[5];
+};
+static_assert(sizeof(QLaneDev) == 64, "lane state is one line");
 
 struct QDev {
-    uint32_t release;
     uint32_t abort;
-    uint32_t exit;
-    uint32_t epoch; // barrier generation when the kernel last left
-    uint64_t consumed;
-    uint64_t pad_[5];
-    vo_command cmd;
+    uint32_t exit_all;   // every lane leaves (idle vote complete, or TERMINATE)
+    uint32_t idle_votes; // lanes idle for the idle period
+    uint32_t blocks_out; // blocks that have left this launch
+    uint32_t reason;     // exit reason of the launch (max over lanes)
+    uint32_t pad_[11];
+    QLaneDev lane[kMaxLanes];
     uint32_t arrive[kMaxQueueBlocks * kFlagStride];
+};
+
+struct QArgs {
+    const QEntry *ring; // lanes x cap_lane entries
+    QCtrl *ctrl;
+    QDev *qd;
+    Step *prog;         // lanes x kMaxSteps (device)
+    uint64_t *flag;     // cap completion flags (host)
+    uint32_t cap;       // flags
+    uint32_t cap_lane;  // ring entries per lane
+    uint32_t lanes;
+    uint32_t pad_;
+    uint64_t idle_ticks, bar_ticks;
 };
 
 __device__ __forceinline__ uint64_t ld_sys(const uint64_t *p)
@@ -90,13 +128,18 @@ __device__ __forceinline__ void st_sys32(uint32_t *p, uint32_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+__device__ __forceinline__ uint32_t ld_ag(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 struct NoHook {
     __device__ void operator()() const {}
 };
 
 __device__ __forceinline__ bool timed_out(QDev *qd, uint64_t t0, uint64_t limit)
 {
-    if (__hip_atomic_load(&qd->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return true;
+    if (ld_ag(&qd->abort)) return true;
     if (wall_clock64() - t0 > limit) {
         __hip_atomic_store(&qd->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return true;
@@ -104,24 +147,30 @@ __device__ __forceinline__ bool timed_out(QDev *qd, uint64_t t0, uint64_t limit)
     return false;
 }
 
-// Grid barrier number ++epoch.  Block 0 waits until every other block has
-// arrived, runs hook() with all its threads (the hook sees every block's
-// writes), then releases the grid.  Every wait is bounded: a barrier that
-// does not complete within limit_ticks sets `abort` and returns false in
-// every block, so the grid drains.
+// Where a block serves: lane = block mod L, rank within the lane, lane size.
+struct LanePos {
+    uint32_t lane, rank, nbl, L;
+};
+
+// Barrier number ++epoch of one lane.  The lane leader (rank 0) waits until
+// every other block of the lane has arrived, runs hook() with all its
+// threads (the hook sees every block's writes), then releases the lane.
+// Every wait is bounded: a barrier that does not complete within
+// limit_ticks sets `abort` and returns false in every block.
 template <class Hook>
-__device__ bool qbarrier(QDev *qd, uint32_t nb, uint32_t &epoch, uint64_t limit_ticks, Hook hook)
+__device__ bool lbarrier(QDev *qd, const LanePos &lp, uint32_t &epoch, uint64_t limit_ticks, Hook hook)
 {
     __shared__ int ok;
+    QLaneDev *ld = &qd->lane[lp.lane];
     const uint32_t e = ++epoch;
     __syncthreads();
-    if (blockIdx.x != 0) {
+    if (lp.rank != 0) {
         if (threadIdx.x == 0) {
             __threadfence();
             __hip_atomic_store(&qd->arrive[blockIdx.x * kFlagStride], e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             int good = 1;
             const uint64_t t0 = wall_clock64();
-            while ((int32_t)(__hip_atomic_load(&qd->release, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - e) < 0) {
+            while ((int32_t)(__hip_atomic_load(&ld->release, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - e) < 0) {
                 if (timed_out(qd, t0, limit_ticks)) {
                     good = 0;
                     break;
@@ -139,9 +188,11 @@ __device__ bool qbarrier(QDev *qd, uint32_t nb, uint32_t &epoch, uint64_t limit_
         const uint64_t t0 = wall_clock64();
         for (;;) {
             int all = 1;
-            for (uint32_t blk = 1 + threadIdx.x; blk < nb; blk += 64)
+            for (uint32_t k = 1 + threadIdx.x; k < lp.nbl; k += 64) {
+                const uint32_t blk = lp.lane + k * lp.L;
                 all &= (int32_t)(__hip_atomic_load(&qd->arrive[blk * kFlagStride], __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT) - e) >= 0;
+            }
             if (__all(all)) break;
             if (timed_out(qd, t0, limit_ticks)) { // uniform: scalar clock, same flag for every lane
                 good = 0;
@@ -158,227 +209,253 @@ __device__ bool qbarrier(QDev *qd, uint32_t nb, uint32_t &epoch, uint64_t limit_
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
-        __hip_atomic_store(&qd->release, e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ld->release, e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
     return true;
 }
 
-__device__ __forceinline__ bool qbarrier(QDev *qd, uint32_t nb, uint32_t &epoch, uint64_t limit_ticks)
+// ------------------------------------------------------------ step bodies
+
+// A kernel parameter read from the program with every 32-bit word made
+// wave-uniform, so pointers and sizes stay in scalar registers.
+template <class T>
+__device__ __forceinline__ T uread(const char *p)
 {
-    return qbarrier(qd, nb, epoch, limit_ticks, NoHook{});
+    T v;
+    __builtin_memcpy(&v, p, sizeof(T));
+    if constexpr (sizeof(T) % 4 == 0) {
+        uint32_t w[sizeof(T) / 4];
+        __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+        for (uint32_t i = 0; i < sizeof(T) / 4; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+        __builtin_memcpy(&v, w, sizeof(T));
+    } else {
+        v = (T)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    }
+    return v;
 }
 
-// ------------------------------------------------------------ op bodies
-
-template <int S>
-__device__ __forceinline__ void sort_global_items(const dev::Bufs<RecKP> &b, const dev::IMap &map, int p, int q_lo,
-                                                  int p_top, uint64_t n_pad, uint32_t nb)
+template <class S, class F, size_t... I>
+__device__ __forceinline__ void call_blob_u_i(const char *blob, F &&f, std::index_sequence<I...>)
 {
-    // 256-thread items of the launch-mode pass; a 512-thread block runs two
-    const uint32_t items = (uint32_t)((n_pad >> S) / 256);
-    const uint32_t half = threadIdx.x >> 8;
-    for (uint32_t w = blockIdx.x; 2 * w < items; w += nb) {
-        const uint32_t it = 2 * w + half;
-        if (it < items) dev::body_global<RecKP, S, false>(b, map, p, q_lo, p_top, nullptr, it, threadIdx.x & 255);
-    }
+    f(uread<typename S::template T<I>>(blob + S::off(I))...);
 }
 
-// SORT_KP: the launch-mode pass schedule (plan_passes with register passes
-// of at most 4 stages and hybrid passes of 6-7), one grid barrier per pass.
-__device__ bool exec_sort_kp(const vo_sort_kp_args &a, char *lds, QDev *qd, uint32_t nb, uint32_t &epoch,
-                             uint64_t bar)
+template <class K, class F>
+__device__ __forceinline__ void call_blob_u(const char *blob, F &&f)
 {
-    dev::Bufs<RecKP> b;
-    b.k[0] = a.keys;
-    b.p = a.payload;
-    const dev::IMap map{nullptr, 0, nullptr};
-    int n_log = 0;
-    while ((1ull << n_log) < a.n_pad) ++n_log;
-    if (n_log < QR_LOG) {
-        dev::body_tiny<RecKP, false>(b, n_log, a.n, 1, nullptr, blockIdx.x);
-        return qbarrier(qd, nb, epoch, bar);
-    }
-    if (n_log < QT_LOG) {
-        if (blockIdx.x == 0)
-            dev::body_local<RecKP, QT_LOG, QR_LOG, QBLOCK, false>(b, map, n_log, n_log, n_log, a.n, 1, nullptr, lds, 0);
-        return qbarrier(qd, nb, epoch, bar);
-    }
-    const uint32_t tiles = (uint32_t)(a.n_pad >> QT_LOG);
-    for (uint32_t w = blockIdx.x; w < tiles; w += nb)
-        dev::body_local_ct<RecKP, QT_LOG, QR_LOG, QBLOCK, false>(b, map, n_log, a.n, 1, nullptr, lds, w);
-    if (!qbarrier(qd, nb, epoch, bar)) return false;
-    for (int p = QT_LOG + 1; p <= n_log; ++p) {
-        const int count = p - QT_LOG;
-        const int groups = plan_groups(count, kQueueSMax, kQueueHMax, 6); // widths built below: 1-4, 6-7
-        int q = p - 1;
-        for (int gi = 0; gi < groups; ++gi) {
-            const int s = plan_width(count, groups, gi);
-            const int q_lo = q - s + 1;
-            q -= s;
-            switch (s) {
-            case 1: sort_global_items<1>(b, map, p, q_lo, n_log, a.n_pad, nb); break;
-            case 2: sort_global_items<2>(b, map, p, q_lo, n_log, a.n_pad, nb); break;
-            case 3: sort_global_items<3>(b, map, p, q_lo, n_log, a.n_pad, nb); break;
-            case 4: sort_global_items<4>(b, map, p, q_lo, n_log, a.n_pad, nb); break;
-            case 6:
-                for (uint32_t w = blockIdx.x; w < tiles; w += nb)
-                    dev::body_hybrid<RecKP, QT_LOG, QR_LOG, QBLOCK, 6, false>(b, map, p, q_lo, n_log, nullptr, lds, w);
-                break;
-            case 7:
-                for (uint32_t w = blockIdx.x; w < tiles; w += nb)
-                    dev::body_hybrid<RecKP, QT_LOG, QR_LOG, QBLOCK, 7, false>(b, map, p, q_lo, n_log, nullptr, lds, w);
-                break;
-            default: break; // plan_groups above yields only the widths built here
-            }
-            if (!qbarrier(qd, nb, epoch, bar)) return false;
-        }
-        for (uint32_t w = blockIdx.x; w < tiles; w += nb)
-            dev::body_merge<RecKP, QT_LOG, QR_LOG, QBLOCK, false>(b, map, p, n_log, nullptr, lds, w);
-        if (!qbarrier(qd, nb, epoch, bar)) return false;
-    }
-    return true;
+    using S = KSigOf<K>;
+    call_blob_u_i<S>(blob, f, std::make_index_sequence<S::n>{});
 }
 
-__device__ __forceinline__ uint32_t width_of(uint32_t t)
+// A pointer made wave-uniform (function arguments arrive in vector registers).
+__device__ __forceinline__ const char *uptr(const char *p)
 {
-    switch (t) {
-    case VO_T_BOOLEAN: case VO_T_TINYINT: return 1;
-    case VO_T_SMALLINT: return 2;
-    case VO_T_INTEGER: case VO_T_FLOAT: case VO_T_DATE: return 4;
-    default: return 8;
-    }
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (const char *)(((uint64_t)hi << 32) | lo);
 }
 
-__device__ __forceinline__ uint64_t ld_w(const void *p, uint64_t i, uint32_t w)
+// One runner per table row: the step's arguments once, then every virtual
+// block of the step that falls to this block.  Not inlined, so each body is
+// compiled as its own function.
+// The resident block's dynamic LDS (every body's `lds`).
+extern __shared__ __align__(16) char q_lds[];
+
+#define VO_RUNNER(ID, K, CALL, L)                                                                                  \
+    __device__ __noinline__ void run_##ID(const char *blob_v, uint32_t grid_v, uint32_t rank_v, uint32_t nbl_v)     \
+    {                                                                                                              \
+        char *lds = q_lds;                                                                                         \
+        const char *blob = uptr(blob_v);                                                                           \
+        const uint32_t grid = __builtin_amdgcn_readfirstlane(grid_v);                                              \
+        const uint32_t rank = __builtin_amdgcn_readfirstlane(rank_v);                                              \
+        const uint32_t nbl = __builtin_amdgcn_readfirstlane(nbl_v);                                                \
+        call_blob_u<decltype(K)>(blob, [&](auto... a) {                                                            \
+            const uint32_t nvb = grid;                                                                             \
+            (void)nvb;                                                                                             \
+            (void)lds;                                                                                             \
+            for (uint32_t vb = rank; vb < grid; vb += nbl) {                                                       \
+                CALL;                                                                                              \
+                __syncthreads();                                                                                   \
+            }                                                                                                      \
+        });                                                                                                        \
+    }
+VO_ALL_BODIES(VO_RUNNER)
+#undef VO_RUNNER
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Built-in fill and copy: the whole lane strides over the bytes.
+__device__ void step_fill(const FillArgs &a, uint32_t value, uint32_t rank, uint32_t nbl)
 {
-    switch (w) {
-    case 1: return ((const uint8_t *)p)[i];
-    case 2: return ((const uint16_t *)p)[i];
-    case 4: return ((const uint32_t *)p)[i];
-    default: return ((const uint64_t *)p)[i];
+    const uint64_t stride = (uint64_t)nbl * RB;
+    const uint64_t t = (uint64_t)rank * RB + threadIdx.x;
+    char *d = (char *)a.dst;
+    const uint32_t b = value & 0xFF;
+    if ((((uintptr_t)d | a.bytes) & 15) == 0) {
+        const uint32_t w = b * 0x01010101u;
+        const u32x4 v = {w, w, w, w};
+        for (uint64_t i = t; i < a.bytes / 16; i += stride) ((u32x4 *)d)[i] = v;
+    } else {
+        for (uint64_t i = t; i < a.bytes; i += stride) d[i] = (char)b;
     }
 }
 
-__device__ __forceinline__ void st_w(void *p, uint64_t i, uint32_t w, uint64_t v)
+__device__ void step_copy(const CopyArgs &a, uint32_t rank, uint32_t nbl)
 {
-    switch (w) {
-    case 1: ((uint8_t *)p)[i] = (uint8_t)v; break;
-    case 2: ((uint16_t *)p)[i] = (uint16_t)v; break;
-    case 4: ((uint32_t *)p)[i] = (uint32_t)v; break;
-    default: ((uint64_t *)p)[i] = v; break;
+    const uint64_t stride = (uint64_t)nbl * RB;
+    const uint64_t t = (uint64_t)rank * RB + threadIdx.x;
+    char *d = (char *)a.dst;
+    const char *s = (const char *)a.src;
+    if ((((uintptr_t)d | (uintptr_t)s | a.bytes) & 15) == 0) {
+        for (uint64_t i = t; i < a.bytes / 16; i += stride) ((u32x4 *)d)[i] = ((const u32x4 *)s)[i];
+    } else if ((((uintptr_t)d | (uintptr_t)s | a.bytes) & 7) == 0) {
+        for (uint64_t i = t; i < a.bytes / 8; i += stride) ((uint64_t *)d)[i] = ((const uint64_t *)s)[i];
+    } else {
+        for (uint64_t i = t; i < a.bytes; i += stride) d[i] = s[i];
     }
 }
 
-// Element-wise commands (reference permute.cuh, gather.cuh, compare.cuh,
-// oblivious_shuffle.cuh) as grid-stride loops over the resident grid.
-__device__ void exec_elementwise(const vo_command &c, uint32_t nb)
+__device__ void run_step(const Step *st, uint32_t rank, uint32_t nbl)
 {
-    const uint64_t stride = (uint64_t)nb * QBLOCK;
-    const uint64_t i0 = (uint64_t)blockIdx.x * QBLOCK + threadIdx.x;
-    switch (c.opcode) {
-    case VO_OP_PERMUTE: {
-        const vo_permute_args &a = c.args.permute;
-        const uint32_t w = width_of(a.type_id);
-        for (uint64_t i = i0; i < a.n; i += stride) st_w(a.out_data, i, w, ld_w(a.in_data, (uint64_t)a.in_indices[i], w));
-        break;
-    }
-    case VO_OP_GATHER: {
-        const vo_gather_args &a = c.args.gather;
-        const uint32_t w = width_of(a.type_id);
-        for (uint64_t i = i0; i < a.n; i += stride)
-            if (a.in_mask == nullptr || a.in_mask[i]) st_w(a.out_data, (uint64_t)a.in_indices[i], w, ld_w(a.in_data, i, w));
-        break;
-    }
-    case VO_OP_COMPARE_EQ: {
-        const vo_compare_eq_args &a = c.args.compare_eq;
-        const uint32_t w = width_of(a.type_id);
-        for (uint64_t i = i0; i < a.n; i += stride) {
-            bool eq;
-            if (a.type_id == VO_T_FLOAT)
-                eq = ((const float *)a.left_data)[i] == ((const float *)a.right_data)[i];
-            else if (a.type_id == VO_T_DOUBLE)
-                eq = ((const double *)a.left_data)[i] == ((const double *)a.right_data)[i];
-            else
-                eq = ld_w(a.left_data, i, w) == ld_w(a.right_data, i, w);
-            a.out_mask[i] = eq ? 1 : 0;
-        }
-        break;
-    }
-    case VO_OP_COMPOSE_POSITION_MAP: {
-        const vo_compose_args &a = c.args.compose_position_map;
-        for (uint64_t i = i0; i < a.n; i += stride) a.new_map[i] = a.old_map[a.sigma[i]];
-        break;
-    }
-    case VO_OP_ENRICH_POSITIONS: {
-        const vo_enrich_args &a = c.args.enrich_positions;
-        for (uint64_t i = i0; i < a.n; i += stride) a.positions[i] = a.position_map[a.record_ids[i]];
-        break;
-    }
-    default: break;
+    const uint32_t id = __builtin_amdgcn_readfirstlane(st->id);
+    const uint32_t grid = __builtin_amdgcn_readfirstlane(st->grid);
+    switch (id) {
+    case kStepFill: step_fill(uread<FillArgs>(st->blob), __builtin_amdgcn_readfirstlane((uint32_t)st->aux), rank, nbl); break;
+    case kStepCopy: step_copy(uread<CopyArgs>(st->blob), rank, nbl); break;
+#define VO_CASE(ID, K, CALL, L)                                                                                    \
+    case ID: run_##ID(st->blob, grid, rank, nbl); break;
+        VO_ALL_BODIES(VO_CASE)
+#undef VO_CASE
+    default: break; // ids are checked on the host before a program is published
     }
 }
 
-// Block 0's work at the barrier that ends one command and starts the next:
-// publish completion of the command every block just finished, then wait
-// (bounded by the idle time) for the host to publish the next one and copy it
-// into device memory with one wavefront (one PCIe round trip per 64 words).
+// ------------------------------------------------------------ fetch
+
+// The lane leader's work at the barrier that ends one command and starts
+// the next: publish completion of the command every block of the lane just
+// finished, wait (bounded) for the next ring entry, and copy its program into
+// the lane's device buffer (one PCIe round trip per 4 KiB).
 struct FetchHook {
-    const vo_command *ring;
-    QCtrl *ctrl;
-    QDev *qd;
-    uint32_t cap;
-    uint64_t consumed; // commands completed so far
-    bool publish;      // a command finished since the last fetch
-    uint64_t idle_ticks;
+    const QArgs *qa;
+    uint32_t lane;
+    uint64_t consumed; // entries of this lane completed so far
+    uint64_t done_seq; // sequence id of the command just finished (0: none)
 
     __device__ void operator()() const
     {
-        if (threadIdx.x >= 64) return;
-        const uint32_t lane = threadIdx.x;
-        if (publish && lane == 0) st_sys(&ctrl->done, consumed);
-        const uint64_t t0 = wall_clock64();
-        uint32_t ex = 0;
-        for (;;) {
-            if (ld_sys(&ctrl->head) > consumed) break; // same address in every lane: uniform
-            if (ld_sys32(&ctrl->stop)) {
-                ex = 1;
-                break;
+        QDev *qd = qa->qd;
+        QLaneDev *ld = &qd->lane[lane];
+        QCtrl *ctrl = qa->ctrl;
+        if (threadIdx.x < 64) {
+            if (done_seq && threadIdx.x == 0) {
+                st_sys(&qa->flag[done_seq % qa->cap], done_seq);
+                st_sys(&ctrl->lane[lane].done, consumed);
             }
-            if (wall_clock64() - t0 > idle_ticks) {
-                ex = 2;
-                break;
+            const uint64_t t0 = wall_clock64();
+            uint32_t ex = 0;
+            bool voted = false;
+            for (;;) { // every exit condition reads one address for all lanes: uniform
+                if (ld_ag(&qd->abort)) {
+                    ex = 4;
+                    break;
+                }
+                if (ld_sys(&ctrl->lane[lane].head) > consumed) {
+                    if (voted && threadIdx.x == 0)
+                        __hip_atomic_fetch_sub(&qd->idle_votes, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    voted = false;
+                    // the others may have decided to leave meanwhile: leave too
+                    // (the host starts the kernel again for this entry)
+                    if (ld_ag(&qd->exit_all)) ex = 2;
+                    break;
+                }
+                if (ld_ag(&qd->exit_all)) {
+                    ex = 2;
+                    break;
+                }
+                if (ld_sys32(&ctrl->stop)) {
+                    ex = 1;
+                    break;
+                }
+                if (!voted && wall_clock64() - t0 > qa->idle_ticks) {
+                    voted = true;
+                    uint32_t v = 0;
+                    if (threadIdx.x == 0)
+                        v = __hip_atomic_fetch_add(&qd->idle_votes, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+                    v = __builtin_amdgcn_readfirstlane(v);
+                    if (v >= qa->lanes) {
+                        if (threadIdx.x == 0)
+                            __hip_atomic_store(&qd->exit_all, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ex = 2;
+                        break;
+                    }
+                }
+                __builtin_amdgcn_s_sleep(2);
             }
-            __builtin_amdgcn_s_sleep(2);
+            if (threadIdx.x == 0) {
+                if (ex == 0) {
+                    const QEntry *en = &qa->ring[lane * qa->cap_lane + consumed % qa->cap_lane];
+                    const uint64_t *w = reinterpret_cast<const uint64_t *>(en);
+                    const uint64_t w0 = __hip_atomic_load(w + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    const uint64_t w1 = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    const uint64_t w2 = __hip_atomic_load(w + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    ld->seq = w0;
+                    ld->opcode = (uint32_t)w1;
+                    const uint32_t ns = (uint32_t)(w1 >> 32);
+                    ld->n_steps = ns <= kMaxSteps ? ns : 0;
+                    ld->prog = w2;
+                }
+                ld->exit = ex;
+            }
         }
-        if (ex == 0) {
-            const uint64_t *src = reinterpret_cast<const uint64_t *>(&ring[consumed % cap]);
-            uint64_t *dst = reinterpret_cast<uint64_t *>(&qd->cmd);
-            for (uint32_t k = lane; k < sizeof(vo_command) / 8; k += 64)
-                dst[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __syncthreads();
+        if (ld->exit != 0) return;
+        const uint32_t ns = ld->n_steps;
+        if (ns == 0) return;
+        const u32x4 *src = (const u32x4 *)ld->prog;
+        u32x4 *dst = (u32x4 *)(qa->prog + (uint64_t)lane * kMaxSteps);
+        const uint32_t n16 = ns * (sizeof(Step) / 16);
+        for (uint32_t i = threadIdx.x; i < n16; i += 4 * RB) {
+            u32x4 v[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t j = i + m * RB;
+                if (j < n16) v[m] = __builtin_nontemporal_load(src + j);
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t j = i + m * RB;
+                if (j < n16) dst[j] = v[m];
+            }
         }
-        if (lane == 0) qd->exit = ex;
     }
 };
 
-__global__ __launch_bounds__(QBLOCK) void k_queue(const vo_command *ring, QCtrl *ctrl, QDev *qd, uint32_t cap,
-                                                   uint64_t idle_ticks, uint64_t bar_ticks)
+__global__ __launch_bounds__(RB, 2) void k_queue(QArgs qa)
 {
-    extern __shared__ __align__(16) char lds[];
-    const uint32_t nb = gridDim.x;
-    const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
-    if (leader) st_sys32(&ctrl->alive, 1);
-    uint64_t consumed = qd->consumed;
-    uint32_t epoch = qd->epoch;
+    QDev *qd = qa.qd;
+    LanePos lp;
+    lp.L = qa.lanes;
+    lp.lane = blockIdx.x % lp.L;
+    lp.rank = blockIdx.x / lp.L;
+    lp.nbl = (gridDim.x - lp.lane + lp.L - 1) / lp.L;
+    QLaneDev *ld = &qd->lane[lp.lane];
+    if (blockIdx.x == 0 && threadIdx.x == 0) st_sys32(&qa.ctrl->alive, 1);
+    uint64_t consumed = ld->consumed;
+    uint32_t epoch = ld->epoch;
     uint32_t reason = 0;
-    bool publish = false;
+    uint64_t done_seq = 0;
+    const Step *prog = qa.prog + (uint64_t)lp.lane * kMaxSteps;
     for (;;) {
-        // the other blocks wait here while block 0 polls the ring: allow the idle time
-        const FetchHook hook{ring, ctrl, qd, cap, consumed, publish, idle_ticks};
-        if (!qbarrier(qd, nb, epoch, idle_ticks + bar_ticks, hook)) {
+        // the other blocks wait here while the leader polls the ring: allow the idle time
+        const FetchHook hook{&qa, lp.lane, consumed, done_seq};
+        if (!lbarrier(qd, lp, epoch, qa.idle_ticks + qa.bar_ticks, hook)) {
             reason = 4;
             break;
         }
-        const uint32_t ex = __hip_atomic_load(&qd->exit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t ex = ld->exit;
         if (ex) {
             reason = ex;
             break;
@@ -387,68 +464,118 @@ __global__ __launch_bounds__(QBLOCK) void k_queue(const vo_command *ring, QCtrl 
         // this CU last looked: drop cached lines (every XCD has blocks here)
         if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
         __syncthreads();
-        const vo_command cmd = qd->cmd;
-        if (cmd.opcode == VO_OP_TERMINATE) {
-            ++consumed;
-            if (leader) st_sys(&ctrl->done, consumed);
+        const uint32_t opcode = __builtin_amdgcn_readfirstlane(ld->opcode);
+        const uint32_t ns = __builtin_amdgcn_readfirstlane(ld->n_steps);
+        const uint64_t seq = ld->seq;
+        ++consumed;
+        done_seq = seq;
+        if (opcode == VO_OP_TERMINATE) {
+            if (lp.rank == 0 && threadIdx.x == 0) {
+                st_sys(&qa.flag[seq % qa.cap], seq);
+                st_sys(&qa.ctrl->lane[lp.lane].done, consumed);
+                __hip_atomic_store(&qd->exit_all, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             reason = 3;
             break;
         }
         bool ok = true;
-        if (cmd.opcode == VO_OP_SORT_KP)
-            ok = exec_sort_kp(cmd.args.sort_kp, lds, qd, nb, epoch, bar_ticks);
-        else
-            exec_elementwise(cmd, nb);
+        for (uint32_t s = 0; s < ns && ok;) {
+            run_step(prog + s, lp.rank, lp.nbl);
+            const uint32_t sl = __builtin_amdgcn_readfirstlane((uint32_t)prog[s].slots);
+            s += sl ? sl : 1;
+            if (s < ns) ok = lbarrier(qd, lp, epoch, qa.bar_ticks, NoHook{});
+        }
         if (!ok) {
             reason = 4;
             break;
         }
-        // results out of every XCD's L2 before block 0 tells the host
+        // results out of every XCD's L2 before the leader tells the host
         __syncthreads();
         if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-        ++consumed;
-        publish = true;
     }
-    if (leader) {
-        qd->consumed = consumed;
-        qd->epoch = epoch;
-        if (reason == 4) st_sys32(&ctrl->error, 1);
-        st_sys32(&ctrl->exit_reason, reason);
-        st_sys32(&ctrl->alive, 0);
+    if (lp.rank == 0 && threadIdx.x == 0) {
+        ld->consumed = consumed;
+        ld->epoch = epoch;
+        __hip_atomic_fetch_max(&qd->reason, reason, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // the last block out reports the launch's end and resets the launch state
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t out = __hip_atomic_fetch_add(&qd->blocks_out, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+        if (out == gridDim.x) {
+            const uint32_t r = ld_ag(&qd->reason);
+            qd->blocks_out = 0;
+            qd->exit_all = 0;
+            qd->idle_votes = 0;
+            qd->reason = 0;
+            qd->abort = 0;
+            __threadfence();
+            if (r == 4) st_sys32(&qa.ctrl->error, 1);
+            st_sys32(&qa.ctrl->exit_reason, r);
+            st_sys32(&qa.ctrl->alive, 0);
+        }
     }
 }
 
 // ----------------------------------------------------------------- host
 
 struct QueueState {
-    vo_command *ring = nullptr;
+    QEntry *ring = nullptr;
     QCtrl *ctrl = nullptr;
     QDev *qd = nullptr;
+    Step *dprog = nullptr;
+    uint64_t *flag = nullptr;
+    Step *arena = nullptr;
     hipStream_t stream = nullptr;
-    uint32_t cap = 0;
-    uint64_t submitted = 0;
+    uint32_t cap = 0, cap_lane = 0, lanes = 1;
+    uint64_t submitted = 0;                 // global sequence ids handed out
+    uint64_t lane_submitted[kMaxLanes] = {}; // entries published per lane
+    uint64_t arena_head = 0;                // steps handed out (absolute)
+    struct Live {
+        uint64_t seq, start, end;
+    };
+    std::deque<Live> live;                  // arena regions of commands not yet known complete
     uint32_t grid = 0;
     size_t lds = 0;
     uint64_t idle_ticks = 0, bar_ticks = 0;
+    bool terminated = false;
 };
 
 constexpr double kBarrierSeconds = 10.0;
 
 volatile QCtrl *vctrl(QueueState *q) { return q->ctrl; }
 
+uint64_t lane_done(QueueState *q, uint32_t l) { return __atomic_load_n(&q->ctrl->lane[l].done, __ATOMIC_ACQUIRE); }
+
+bool seq_done(QueueState *q, uint64_t seq)
+{
+    return __atomic_load_n(&q->flag[seq % q->cap], __ATOMIC_ACQUIRE) >= seq;
+}
+
+bool all_done(QueueState *q)
+{
+    for (uint32_t l = 0; l < q->lanes; ++l)
+        if (lane_done(q, l) < q->lane_submitted[l]) return false;
+    return true;
+}
+
 void launch_queue_kernel(vo_ctx *ctx, QueueState *q)
 {
     vctrl(q)->alive = 0;
     vctrl(q)->stop = 0;
     vctrl(q)->exit_reason = 0;
-    void *args[] = {&q->ring, &q->ctrl, &q->qd, &q->cap, &q->idle_ticks, &q->bar_ticks};
-    VO_HIP(hipLaunchCooperativeKernel((const void *)k_queue, dim3(q->grid), dim3(QBLOCK), args, (unsigned)q->lds,
+    QArgs qa{q->ring, q->ctrl, q->qd, q->dprog, q->flag, q->cap, q->cap_lane, q->lanes, 0, q->idle_ticks,
+             q->bar_ticks};
+    void *args[] = {&qa};
+    VO_HIP(hipLaunchCooperativeKernel((const void *)k_queue, dim3(q->grid), dim3(RB), args, (unsigned)q->lds,
                                       q->stream));
     // wait until the kernel reports that it runs
     const auto t0 = std::chrono::steady_clock::now();
     while (vctrl(q)->alive == 0) {
         hipError_t e = hipStreamQuery(q->stream);
-        if (e != hipErrorNotReady && e != hipSuccess) throw Error{VO_ERR_HIP, std::string("queue kernel: ") + hipGetErrorString(e)};
+        if (e != hipErrorNotReady && e != hipSuccess)
+            throw Error{VO_ERR_HIP, std::string("queue kernel: ") + hipGetErrorString(e)};
+        if (e == hipSuccess) break; // it already left (nothing to do): the caller checks again
         if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30))
             throw Error{VO_ERR_TIMEOUT, "queue kernel did not start"};
         std::this_thread::yield();
@@ -456,74 +583,12 @@ void launch_queue_kernel(vo_ctx *ctx, QueueState *q)
     (void)ctx;
 }
 
-bool supported(uint32_t op)
-{
-    switch (op) {
-    case VO_OP_NOP: case VO_OP_SORT_KP: case VO_OP_PERMUTE: case VO_OP_GATHER: case VO_OP_COMPARE_EQ:
-    case VO_OP_COMPOSE_POSITION_MAP: case VO_OP_ENRICH_POSITIONS: case VO_OP_TERMINATE: return true;
-    default: return false;
-    }
-}
-
-uint32_t elem_bytes(uint32_t t)
-{
-    switch (t) {
-    case VO_T_BOOLEAN: case VO_T_TINYINT: return 1;
-    case VO_T_SMALLINT: return 2;
-    case VO_T_INTEGER: case VO_T_FLOAT: case VO_T_DATE: return 4;
-    case VO_T_BIGINT: case VO_T_DOUBLE: case VO_T_VARCHAR: case VO_T_UBIGINT: return 8;
-    default: return 0;
-    }
-}
-
-void validate(const vo_command &c)
-{
-    VO_CHECK(supported(c.opcode), VO_ERR_UNSUPPORTED,
-             "opcode " + std::to_string(c.opcode) + " runs in launch mode only (not in the resident queue)");
-    switch (c.opcode) {
-    case VO_OP_PERMUTE: {
-        const vo_permute_args &a = c.args.permute;
-        VO_CHECK(elem_bytes(a.type_id) && (a.n == 0 || (a.out_data && a.in_data && a.in_indices)), VO_ERR_INVALID,
-                 "permute: bad type or buffers");
-        break;
-    }
-    case VO_OP_GATHER: {
-        const vo_gather_args &a = c.args.gather;
-        VO_CHECK(elem_bytes(a.type_id) && (a.n == 0 || (a.out_data && a.in_data && a.in_indices)), VO_ERR_INVALID,
-                 "gather: bad type or buffers");
-        break;
-    }
-    case VO_OP_COMPARE_EQ: {
-        const vo_compare_eq_args &a = c.args.compare_eq;
-        VO_CHECK(elem_bytes(a.type_id) && (a.n == 0 || (a.left_data && a.right_data && a.out_mask)), VO_ERR_INVALID,
-                 "compare_eq: bad type or buffers");
-        break;
-    }
-    case VO_OP_COMPOSE_POSITION_MAP: {
-        const vo_compose_args &a = c.args.compose_position_map;
-        VO_CHECK(a.n == 0 || (a.old_map && a.sigma && a.new_map), VO_ERR_INVALID, "compose: null buffer");
-        break;
-    }
-    case VO_OP_ENRICH_POSITIONS: {
-        const vo_enrich_args &a = c.args.enrich_positions;
-        VO_CHECK(a.n == 0 || (a.record_ids && a.position_map && a.positions), VO_ERR_INVALID, "enrich: null buffer");
-        break;
-    }
-    default: break;
-    }
-    if (c.opcode == VO_OP_SORT_KP) {
-        const vo_sort_kp_args &a = c.args.sort_kp;
-        VO_CHECK(a.keys && a.payload && is_pow2(a.n_pad) && a.n <= a.n_pad && a.n_pad <= (1ull << 29), VO_ERR_INVALID,
-                 "sort_kp: bad buffers or sizes");
-    }
-}
-
-void reap_error(vo_ctx *ctx, QueueState *q)
+void reap_error(QueueState *q)
 {
     if (vctrl(q)->error) throw Error{VO_ERR_HIP, "queue kernel aborted (a grid barrier timed out)"};
     hipError_t e = hipStreamQuery(q->stream);
-    if (e != hipErrorNotReady && e != hipSuccess) throw Error{VO_ERR_HIP, std::string("queue kernel: ") + hipGetErrorString(e)};
-    (void)ctx;
+    if (e != hipErrorNotReady && e != hipSuccess)
+        throw Error{VO_ERR_HIP, std::string("queue kernel: ") + hipGetErrorString(e)};
 }
 
 int fail_q(vo_ctx *ctx, const Error &e)
@@ -533,42 +598,111 @@ int fail_q(vo_ctx *ctx, const Error &e)
     return e.status;
 }
 
-} // namespace
-
-// Restart the kernel when it left after an idle period with commands still
-// unconsumed (it may have decided to leave just before a command was
-// published).  Caller holds ctx->mu.
+// Restart the kernel when it left after an idle period with entries still
+// unconsumed (it may have decided to leave just before one was published).
+// Caller holds q's submission lock (ctx->mu).
 void ensure_running_locked(vo_ctx *ctx, QueueState *q)
 {
-    reap_error(ctx, q);
+    reap_error(q);
     if (vctrl(q)->alive) return;
-    VO_HIP(hipStreamSynchronize(q->stream)); // the previous kernel has drained
-    if (vctrl(q)->exit_reason == 3) throw Error{VO_ERR_QUEUE, "queue terminated"};
-    if (__atomic_load_n(&q->ctrl->done, __ATOMIC_ACQUIRE) < q->submitted)
-        launch_queue_kernel(ctx, q);
+    VO_HIP(hipStreamSynchronize(q->stream)); // the previous launch has drained
+    reap_error(q);
+    if (q->terminated || vctrl(q)->exit_reason == 3) {
+        q->terminated = true;
+        if (!all_done(q)) throw Error{VO_ERR_QUEUE, "queue terminated"};
+        return;
+    }
+    if (!all_done(q)) launch_queue_kernel(ctx, q);
 }
 
-uint64_t queue_submit(vo_ctx *ctx, vo_command *cmd)
+// Waits (relaunching an idle kernel) until pred() holds.  Caller holds ctx->mu.
+template <class P>
+void wait_locked(vo_ctx *ctx, QueueState *q, P pred, const char *what)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    int spins = 0;
+    while (!pred()) {
+        ensure_running_locked(ctx, q);
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(ctx->wait_timeout_ms))
+            throw Error{VO_ERR_TIMEOUT, what};
+        if (++spins > 2000) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+// Copies a recorded program into the host arena; returns its address.
+const Step *place_program(vo_ctx *ctx, QueueState *q, Recorder &rec, uint64_t seq)
+{
+    const uint64_t data_steps = (rec.data.size() + sizeof(Step) - 1) / sizeof(Step);
+    const uint64_t n = rec.steps.size() + data_steps;
+    VO_CHECK(rec.steps.size() <= kMaxSteps, VO_ERR_UNSUPPORTED,
+             "command program of " + std::to_string(rec.steps.size()) + " steps exceeds the resident limit");
+    VO_CHECK(n <= kArenaSteps / 2, VO_ERR_UNSUPPORTED, "command program exceeds the program arena");
+    uint64_t start = q->arena_head;
+    if (start % kArenaSteps + n > kArenaSteps) start += kArenaSteps - start % kArenaSteps; // no wrap inside
+    const uint64_t end = start + n;
+    auto reclaim = [&] {
+        while (!q->live.empty() && seq_done(q, q->live.front().seq)) q->live.pop_front();
+        const uint64_t oldest = q->live.empty() ? end : q->live.front().start;
+        return end - oldest <= kArenaSteps;
+    };
+    wait_locked(ctx, q, reclaim, "program arena full");
+    Step *dst = q->arena + start % kArenaSteps;
+    char *data = (char *)(dst + rec.steps.size());
+    for (const auto &ref : rec.data_refs) {
+        CopyArgs ca;
+        std::memcpy(&ca, rec.steps[ref.first].blob, sizeof ca);
+        ca.src = data + ref.second;
+        std::memcpy(rec.steps[ref.first].blob, &ca, sizeof ca);
+    }
+    std::memcpy(dst, rec.steps.data(), rec.steps.size() * sizeof(Step));
+    if (!rec.data.empty()) std::memcpy(data, rec.data.data(), rec.data.size());
+    q->arena_head = end;
+    q->live.push_back({seq, start, end});
+    return dst;
+}
+
+// Publishes one entry on `lane`.  Caller holds ctx->mu.
+uint64_t publish_locked(vo_ctx *ctx, QueueState *q, uint32_t lane, uint32_t opcode, Recorder *rec)
+{
+    VO_CHECK(!q->terminated, VO_ERR_QUEUE, "queue terminated");
+    const uint64_t seq = q->submitted + 1;
+    // the lane's ring slot and the sequence id's completion flag must be free
+    wait_locked(ctx, q, [&] { return q->lane_submitted[lane] - lane_done(q, lane) < q->cap_lane; },
+                "command ring full");
+    if (seq > q->cap) wait_locked(ctx, q, [&] { return seq_done(q, seq - q->cap); }, "completion flags full");
+    const Step *prog = nullptr;
+    uint32_t ns = 0;
+    if (rec && !rec->steps.empty()) {
+        prog = place_program(ctx, q, *rec, seq);
+        ns = (uint32_t)rec->steps.size();
+    }
+    QEntry *en = &q->ring[lane * q->cap_lane + q->lane_submitted[lane] % q->cap_lane];
+    en->seq = seq;
+    en->opcode = opcode;
+    en->n_steps = ns;
+    en->prog = prog;
+    q->submitted = seq;
+    q->lane_submitted[lane] += 1;
+    __atomic_store_n(&q->ctrl->lane[lane].head, q->lane_submitted[lane], __ATOMIC_RELEASE);
+    // checked after publishing: a kernel that left before it saw the entry
+    // is started again
+    ensure_running_locked(ctx, q);
+    return seq;
+}
+
+} // namespace
+
+uint64_t queue_submit_lane(vo_ctx *ctx, vo_command *cmd, uint32_t lane)
 {
     QueueState *q = (QueueState *)ctx->queue;
     try {
-        validate(*cmd);
+        VO_CHECK(lane < q->lanes, VO_ERR_INVALID, "lane " + std::to_string(lane) + " of " + std::to_string(q->lanes));
         std::lock_guard<std::mutex> lk(ctx->mu);
-        const auto t0 = std::chrono::steady_clock::now();
-        while (q->submitted - __atomic_load_n(&q->ctrl->done, __ATOMIC_ACQUIRE) >= q->cap) { // ring full
-            ensure_running_locked(ctx, q);
-            if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(ctx->wait_timeout_ms))
-                throw Error{VO_ERR_TIMEOUT, "command ring full"};
-            std::this_thread::yield();
-        }
-        const uint64_t seq = q->submitted + 1;
+        Recorder rec;
+        rec.lane = lane;
+        if (cmd->opcode != VO_OP_NOP && cmd->opcode != VO_OP_TERMINATE) record_command(ctx, *cmd, rec);
+        const uint64_t seq = publish_locked(ctx, q, lane, cmd->opcode, &rec);
         cmd->sequence_id = seq;
-        q->ring[q->submitted % q->cap] = *cmd;
-        __atomic_store_n(&q->ctrl->head, seq, __ATOMIC_RELEASE);
-        q->submitted = seq;
-        // checked after publishing: a kernel that left before it saw the
-        // command is started again
-        ensure_running_locked(ctx, q);
         return seq;
     } catch (const Error &e) {
         fail_q(ctx, e);
@@ -576,15 +710,17 @@ uint64_t queue_submit(vo_ctx *ctx, vo_command *cmd)
     }
 }
 
+uint64_t queue_submit(vo_ctx *ctx, vo_command *cmd) { return queue_submit_lane(ctx, cmd, 0); }
+
 int queue_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms)
 {
     QueueState *q = (QueueState *)ctx->queue;
     try {
         if (timeout_ms == 0) timeout_ms = ctx->wait_timeout_ms;
-        VO_CHECK(seq <= q->submitted, VO_ERR_INVALID, "wait on a sequence id never submitted");
+        VO_CHECK(seq >= 1 && seq <= q->submitted, VO_ERR_INVALID, "wait on a sequence id never submitted");
         const auto t0 = std::chrono::steady_clock::now();
         int spins = 0;
-        while (__atomic_load_n(&q->ctrl->done, __ATOMIC_ACQUIRE) < seq) {
+        while (!seq_done(q, seq)) {
             if (vctrl(q)->alive == 0 || vctrl(q)->error) {
                 std::lock_guard<std::mutex> lk(ctx->mu);
                 ensure_running_locked(ctx, q);
@@ -603,29 +739,35 @@ int queue_poll(vo_ctx *ctx, uint64_t seq)
 {
     QueueState *q = (QueueState *)ctx->queue;
     try {
-        reap_error(ctx, q);
-        if (__atomic_load_n(&q->ctrl->done, __ATOMIC_ACQUIRE) >= seq) return 1;
+        reap_error(q);
+        if (seq_done(q, seq)) return 1;
         // a kernel that left on idle just as this command was published is
         // started again here too, so a caller that only polls makes progress
         if (vctrl(q)->alive == 0) {
             std::lock_guard<std::mutex> lk(ctx->mu);
             ensure_running_locked(ctx, q);
         }
-        return __atomic_load_n(&q->ctrl->done, __ATOMIC_ACQUIRE) >= seq ? 1 : 0;
+        return seq_done(q, seq) ? 1 : 0;
     } catch (const Error &e) {
         return -fail_q(ctx, e);
     }
 }
 
-bool queue_supports(uint32_t opcode) { return supported(opcode); }
+// Waits for every submitted command; caller holds ctx->mu (used by the
+// scratch arena before it is replaced).
+void queue_drain_locked(vo_ctx *ctx)
+{
+    QueueState *q = (QueueState *)ctx->queue;
+    wait_locked(ctx, q, [&] { return all_done(q); }, "queue drain");
+}
 
 int queue_park(vo_ctx *ctx)
 {
     QueueState *q = (QueueState *)ctx->queue;
     try {
         std::lock_guard<std::mutex> lk(ctx->mu);
-        reap_error(ctx, q);
-        // the kernel takes every published command before it looks at `stop`
+        reap_error(q);
+        // the kernel takes every published entry before it looks at `stop`
         vctrl(q)->stop = 1;
         const auto t0 = std::chrono::steady_clock::now();
         for (;;) {
@@ -637,9 +779,16 @@ int queue_park(vo_ctx *ctx)
             std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
         vctrl(q)->stop = 0;
-        reap_error(ctx, q);
-        VO_CHECK(__atomic_load_n(&q->ctrl->done, __ATOMIC_ACQUIRE) == q->submitted, VO_ERR_QUEUE,
-                 "queue kernel left with commands outstanding");
+        reap_error(q);
+        // an idle exit may have raced with the last publish: run what is left
+        if (!all_done(q)) {
+            launch_queue_kernel(ctx, q);
+            vctrl(q)->stop = 1;
+            VO_HIP(hipStreamSynchronize(q->stream));
+            vctrl(q)->stop = 0;
+            reap_error(q);
+        }
+        VO_CHECK(all_done(q), VO_ERR_QUEUE, "queue kernel left with commands outstanding");
         return VO_OK;
     } catch (const Error &e) {
         return fail_q(ctx, e);
@@ -648,9 +797,16 @@ int queue_park(vo_ctx *ctx)
 
 int queue_sync(vo_ctx *ctx)
 {
-    QueueState *q = (QueueState *)ctx->queue;
-    return q->submitted ? queue_wait(ctx, q->submitted, 0) : VO_OK;
+    try {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        queue_drain_locked(ctx);
+        return VO_OK;
+    } catch (const Error &e) {
+        return fail_q(ctx, e);
+    }
 }
+
+uint32_t queue_lanes(vo_ctx *ctx) { return ctx->queue ? ((QueueState *)ctx->queue)->lanes : 0; }
 
 } // namespace vo
 
@@ -666,21 +822,31 @@ int vo_queue_start(vo_ctx *ctx)
     QueueState *q = new QueueState();
     try {
         VO_HIP(hipSetDevice(ctx->device));
+        q->lanes = std::max<uint32_t>(1, ctx->queue_lanes);
+        VO_CHECK(q->lanes <= kMaxLanes, VO_ERR_INVALID, "queue_lanes above 8");
         q->cap = ctx->queue_capacity;
-        VO_HIP(hipHostMalloc((void **)&q->ring, sizeof(vo_command) * q->cap, hipHostMallocCoherent | hipHostMallocMapped));
+        VO_CHECK(q->cap >= q->lanes, VO_ERR_INVALID, "queue_capacity below the lane count");
+        q->cap_lane = q->cap / q->lanes;
+        VO_HIP(hipHostMalloc((void **)&q->ring, sizeof(QEntry) * q->lanes * q->cap_lane,
+                             hipHostMallocCoherent | hipHostMallocMapped));
         VO_HIP(hipHostMalloc((void **)&q->ctrl, sizeof(QCtrl), hipHostMallocCoherent | hipHostMallocMapped));
+        VO_HIP(hipHostMalloc((void **)&q->flag, sizeof(uint64_t) * q->cap, hipHostMallocCoherent | hipHostMallocMapped));
+        VO_HIP(hipHostMalloc((void **)&q->arena, sizeof(Step) * kArenaSteps, hipHostMallocCoherent | hipHostMallocMapped));
         std::memset((void *)q->ctrl, 0, sizeof(QCtrl));
-        std::memset((void *)q->ring, 0, sizeof(vo_command) * q->cap);
+        std::memset((void *)q->ring, 0, sizeof(QEntry) * q->lanes * q->cap_lane);
+        std::memset((void *)q->flag, 0, sizeof(uint64_t) * q->cap);
         VO_HIP(hipMalloc((void **)&q->qd, sizeof(QDev)));
-        VO_HIP(hipMemset(q->qd, 0, sizeof(QDev)));
+        VO_HIP(hipMalloc((void **)&q->dprog, sizeof(Step) * kMaxSteps * q->lanes));
         VO_HIP(hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking));
-        // LDS: the larger of the tile column buffer and the small-array tile
-        q->lds = std::max(QTile::lds_bytes(), dev::lds_bytes<RecKP>(QT_LOG - 1));
+        VO_HIP(hipMemsetAsync(q->qd, 0, sizeof(QDev), q->stream));
+        VO_HIP(hipStreamSynchronize(q->stream));
+        q->lds = kResidentLds;
         VO_HIP(hipFuncSetAttribute((const void *)k_queue, hipFuncAttributeMaxDynamicSharedMemorySize, (int)q->lds));
         int per_cu = 0;
-        VO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_queue, QBLOCK, q->lds));
+        VO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_queue, RB, q->lds));
         VO_CHECK(per_cu >= 1, VO_ERR_UNSUPPORTED, "queue kernel does not fit on a CU");
         q->grid = std::min<uint32_t>((uint32_t)per_cu * (uint32_t)ctx->num_cus, kMaxQueueBlocks);
+        VO_CHECK(q->grid >= q->lanes, VO_ERR_INVALID, "fewer resident blocks than lanes");
         int rate_khz = 0;
         VO_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
         const double hz = rate_khz > 0 ? rate_khz * 1e3 : 1e8;
@@ -691,8 +857,11 @@ int vo_queue_start(vo_ctx *ctx)
     } catch (const Error &e) {
         if (q->stream) (void)hipStreamDestroy(q->stream);
         if (q->qd) (void)hipFree(q->qd);
+        if (q->dprog) (void)hipFree(q->dprog);
         if (q->ctrl) (void)hipHostFree(q->ctrl);
         if (q->ring) (void)hipHostFree(q->ring);
+        if (q->flag) (void)hipHostFree(q->flag);
+        if (q->arena) (void)hipHostFree(q->arena);
         delete q;
         return fail_q(ctx, e);
     }
@@ -706,7 +875,7 @@ int vo_queue_stop(vo_ctx *ctx, uint32_t timeout_ms)
     QueueState *q = (QueueState *)ctx->queue;
     if (!q) return VO_OK;
     int rc = VO_OK;
-    if (vctrl(q)->alive) {
+    if (vctrl(q)->alive && !q->terminated) {
         vo_command term{};
         term.opcode = VO_OP_TERMINATE;
         if (queue_submit(ctx, &term) == 0) rc = ctx->last_status;
@@ -736,8 +905,11 @@ int vo_queue_stop(vo_ctx *ctx, uint32_t timeout_ms)
     }
     (void)hipStreamDestroy(q->stream);
     (void)hipFree(q->qd);
+    (void)hipFree(q->dprog);
     (void)hipHostFree(q->ctrl);
     (void)hipHostFree(q->ring);
+    (void)hipHostFree(q->flag);
+    (void)hipHostFree(q->arena);
     delete q;
     ctx->queue = nullptr;
     return rc;
@@ -748,5 +920,20 @@ int vo_queue_running(const vo_ctx *ctx)
     if (!ctx || !ctx->queue) return 0;
     return ((QueueState *)ctx->queue)->ctrl->alive ? 1 : 0;
 }
+
+uint64_t vo_submit_lane(vo_ctx *ctx, vo_command *cmd, uint32_t lane)
+{
+    if (!ctx || !cmd) return 0;
+    if (!ctx->queue) {
+        if (lane != 0) {
+            fail_q(ctx, Error{VO_ERR_INVALID, "lanes need the resident queue (VO_CFG_PERSISTENT)"});
+            return 0;
+        }
+        return vo_submit(ctx, cmd);
+    }
+    return queue_submit_lane(ctx, cmd, lane);
+}
+
+int vo_queue_lanes(const vo_ctx *ctx) { return ctx && ctx->queue ? (int)((QueueState *)ctx->queue)->lanes : 0; }
 
 } // extern "C"

@@ -24,11 +24,18 @@
 namespace vo {
 namespace dev {
 
-// Uniform virtual index -> physical index (identity without a table).
+// Uniform virtual index -> physical index (identity without a table).  The
+// table entry is made wave-uniform explicitly: where the compiler cannot
+// prove the table unwritten (the resident queue kernel's bodies) it would
+// load it into vector registers, and every buffer access based on it would
+// become a loop over lanes.
 __device__ __forceinline__ uint64_t map_uniform(const IMap &m, uint64_t v)
 {
     if (m.offsets == nullptr) return v;
-    return m.offsets[v >> m.c] + (v & ((1ull << m.c) - 1));
+    const uint64_t o = m.offsets[v >> m.c];
+    const uint64_t ou = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(o >> 32)) << 32) |
+                        __builtin_amdgcn_readfirstlane((uint32_t)o);
+    return ou + (v & ((1ull << m.c) - 1));
 }
 
 // Raw buffer access: the uniform base goes into a buffer resource (scalar
@@ -604,7 +611,7 @@ struct SortImpl {
             case 3: {
                 const size_t lds = dev::TileCT<R, T_LOG, R_LOG, TRACE>::lds_bytes();
                 const uint32_t n_tiles = (uint32_t)(n_virtual >> T_LOG);
-                if (ctx->tuning.pipe_merge) {
+                if (ctx->tuning.pipe_merge && !ctx->rec) { // resident: plain merge blocks
                     auto k = dev::k_sort_merge_pipe<R, T_LOG, R_LOG, BLOCK, TRACE>;
                     VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                     int per_cu = 0;

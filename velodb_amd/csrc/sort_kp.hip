@@ -1,5 +1,6 @@
 // sort_kp.hip -- the (u64 key, u32 payload) oblivious sort (the headline
 // path) and the (u64 key, u64 value) sort used by the join.
+#include "resident_tables.hpp"
 #include "sort_kernels.hpp"
 
 namespace vo {
@@ -7,6 +8,10 @@ namespace vo {
 using namespace sortimpl;
 
 namespace {
+
+const std::pair<const void *, BodyInfo> kBodies[] = {VO_SORT_KP_BODIES(VO_BODY_ENTRY) VO_SORT_KV_BODIES(VO_BODY_ENTRY)};
+const bool kRegistered = (register_bodies(kBodies, sizeof kBodies / sizeof kBodies[0]), true);
+
 // Register-only global passes take up to 5 stages below 2^27 records and up
 // to 3 from 2^27 on, where groups of 4 and 5 stages run faster as LDS-staged
 // passes (2^28: 5455 vs 5162 and 5165 vs 4955 GB/s); below 2^27 the register
@@ -15,6 +20,17 @@ using SortKP = SortImpl<RecKP, 13, 4, 512, 5, 8, 3, 27>;
 // the 16-byte sort likewise: at 2^28 its 4-5 stage register passes ran at
 // 4.80-4.99 TB/s against 5.11-5.36 for its LDS-staged passes
 using SortKV = SortImpl<RecKV, 12, 4, 256, 5, 7, 3, 27>;
+// The resident queue kernel runs 256-thread blocks: the 12-byte sort runs
+// there on 2^12-record tiles (the same network, so the same output).
+using SortKP_R = SortImpl<RecKP, 12, 4, 256, 5, 7, 3, 27>;
+
+void run_kp(vo_ctx *ctx, Bufs<RecKP> b, int n_log, uint64_t n_valid, int fill)
+{
+    if (ctx->rec)
+        SortKP_R::run(ctx, b, n_log, n_valid, fill);
+    else
+        SortKP::run(ctx, b, n_log, n_valid, fill);
+}
 } // namespace
 
 void op_sort_kp(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad)
@@ -25,7 +41,7 @@ void op_sort_kp(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint
     Bufs<RecKP> b;
     b.k[0] = keys;
     b.p = payload;
-    SortKP::run(ctx, b, ilog2(n_pad), n, 1);
+    run_kp(ctx, b, ilog2(n_pad), n, 1);
 }
 
 void sort_kp_full(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n_pad)
@@ -33,7 +49,7 @@ void sort_kp_full(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n_pad
     Bufs<RecKP> b;
     b.k[0] = keys;
     b.p = payload;
-    SortKP::run(ctx, b, ilog2(n_pad), n_pad, 0);
+    run_kp(ctx, b, ilog2(n_pad), n_pad, 0);
 }
 
 void sort_kv(vo_ctx *ctx, uint64_t *keys, uint64_t *vals, uint64_t n_pad)
@@ -56,7 +72,10 @@ void run_class(vo_ctx *ctx, Bufs<RecKP> b, int c, uint64_t count, const uint64_t
     const SortTuning saved = ctx->tuning;
     ctx->tuning = apply_tuning_word(saved, ctx->class_tuning[c]);
     try {
-        SortKP::run_mapped(ctx, b, c, count, offsets, ns);
+        if (ctx->rec)
+            SortKP_R::run_mapped(ctx, b, c, count, offsets, ns);
+        else
+            SortKP::run_mapped(ctx, b, c, count, offsets, ns);
     } catch (...) {
         ctx->tuning = saved;
         throw;
@@ -92,7 +111,7 @@ void op_sort_batch(vo_ctx *ctx, const vo_sort_batch_args &a)
     }
     if (host.empty()) return;
     uint64_t *dtab = (uint64_t *)scratch(ctx, host.size() * 8);
-    VO_HIP(hipMemcpyAsync(dtab, host.data(), host.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    upload_async(ctx, dtab, host.data(), host.size() * 8);
     Bufs<RecKP> b;
     b.k[0] = a.keys;
     b.p = a.payload;
@@ -106,8 +125,8 @@ void op_sort_batch(vo_ctx *ctx, const vo_sort_batch_args &a)
         }
         run_class(ctx, b, c, cnt, dtab + at[c], dtab + at[c] + cnt);
     }
-    // the host table must outlive its upload
-    VO_HIP(hipStreamSynchronize(ctx->stream));
+    // the host table must outlive its upload (recorded: it travels with the program)
+    if (!ctx->rec) VO_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 // Autotuner: for each size class in [log_min, log_max], times every

@@ -68,6 +68,8 @@ struct SortTuning {
     int pipe_merge = 0;        // resident merge blocks with next-tile prefetch
 };
 
+struct Recorder;
+
 } // namespace vo
 
 // ------------------------------------------------------------------ context
@@ -80,6 +82,7 @@ struct vo_ctx {
     uint32_t queue_capacity = 1024;
     uint32_t wait_timeout_ms = 60000;
     uint32_t queue_idle_us = 20000;
+    uint32_t queue_lanes = 1;
     int num_cus = 0;
     std::string last_error;
     int last_status = 0;
@@ -91,9 +94,11 @@ struct vo_ctx {
     std::vector<std::pair<uint64_t, hipEvent_t>> pending;
     std::vector<hipEvent_t> event_pool;
 
-    // scratch arena
+    // scratch arena (launch mode and queue lane 0), and the other lanes' own
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    void *lane_scratch[8] = {};
+    size_t lane_scratch_bytes[8] = {};
 
     // tracing
     bool tracing = false;
@@ -110,6 +115,7 @@ struct vo_ctx {
 
     // persistent queue (queue.hip)
     void *queue = nullptr;
+    vo::Recorder *rec = nullptr; // set while a command's program is recorded (resident.hpp)
 
     // equi-join routing state kept from COUNT to WRITE (join_pairs.hip)
     void *join_pairs_state = nullptr;
@@ -139,17 +145,6 @@ struct LaunchInfo {
 void begin_launch(vo_ctx *ctx, const LaunchInfo &li, hipEvent_t *ev_stop);
 void end_launch(vo_ctx *ctx, hipEvent_t ev_stop);
 
-template <typename K, typename... Args>
-inline void launch(vo_ctx *ctx, const LaunchInfo &li, K kernel, Args... args)
-{
-    hipEvent_t ev = nullptr;
-    begin_launch(ctx, li, &ev);
-    hipLaunchKernelGGL(kernel, li.grid, li.block, li.lds, ctx->stream, args...);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-        throw Error{VO_ERR_HIP, std::string("launch ") + li.family + ": " + hipGetErrorString(e)};
-    end_launch(ctx, ev);
-}
 
 inline uint64_t hash_mix(uint64_t h, uint64_t v)
 {
@@ -222,8 +217,10 @@ uint64_t queue_submit(vo_ctx *ctx, vo_command *cmd);
 int queue_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms);
 int queue_poll(vo_ctx *ctx, uint64_t seq);
 int queue_sync(vo_ctx *ctx);
-bool queue_supports(uint32_t opcode); // runs in the resident kernel
 int queue_park(vo_ctx *ctx);          // finish submitted commands, let the resident kernel leave
+uint64_t queue_submit_lane(vo_ctx *ctx, vo_command *cmd, uint32_t lane);
+void queue_drain_locked(vo_ctx *ctx); // every submitted command complete (caller holds ctx->mu)
+uint32_t queue_lanes(vo_ctx *ctx);
 
 // ---- shared device helpers (sort.hip) used by other ops
 // Sort (u64 key, u64 value) records by (key, value) ascending, n_pad = 2^k.
@@ -331,3 +328,5 @@ __device__ __forceinline__ int64_t rand_range(uint64_t seed, uint32_t stream, ui
 
 } // namespace dev
 } // namespace vo
+
+#include "resident.hpp"
