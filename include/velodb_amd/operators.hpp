@@ -923,6 +923,13 @@ private:
         a.n_right = rb.getRowCount();
         a.type_id = lk.getTypeId();
         a.out_count = d_count;
+        // the COUNT -> WRITE routing state in a buffer of this operator, as
+        // the reference's MergeJoinBlock buffers (commands.hpp:82-98): joins
+        // of other operators on the same TaskManager may run in between
+        void *state = nullptr;
+        hip_check(hipMalloc(&state, vo_join_pairs_state_bytes(a.n_left, a.n_right)), "hipMalloc");
+        std::unique_ptr<void, void (*)(void *)> state_guard(state, [](void *p) { (void)hipFree(p); });
+        a.state = state;
         vo_command count{};
         count.opcode = VO_OP_SORT_MERGE_JOIN_COUNT;
         count.args.join_pairs = a;
@@ -933,6 +940,7 @@ private:
         const size_t cap = next_pow2(n_out);
         Column li = Column(VO_T_BIGINT, cap, false), ri = Column(VO_T_BIGINT, cap, false);
         a.out_count = nullptr;
+        a.n_out = n_out;
         a.out_left = static_cast<int64_t *>(li.rawData());
         a.out_right = static_cast<int64_t *>(ri.rawData());
         a.out_cap = cap;

@@ -281,14 +281,21 @@ typedef struct vo_join_args {
  *   COUNT: sorts the union of the n_left + n_right rows (padded to a power
  *     of two >= 4096, at most 2^29), derives every row's match counts and
  *     output positions, and compacts the matched rows; *out_count (device)
- *     receives the number of pairs n_out.  The routing state stays in the
- *     context until the next COUNT.
- *   WRITE: expands the state of the last COUNT with the same key pointers
- *     and sizes (type_id 0: the COUNT's) into out_left / out_right (device,
+ *     receives the number of pairs n_out.  The routing state goes into
+ *     `state`, a device buffer of vo_join_pairs_state_bytes(n_left, n_right)
+ *     bytes the caller owns (as the reference passes MergeJoinBlock buffers
+ *     from COUNT to WRITE, commands.hpp:82-98), or, with state NULL, stays in
+ *     the context until the next COUNT.
+ *   WRITE: expands a COUNT's state into out_left / out_right (device,
  *     out_cap rows each, out_cap >= n_out, n_out <= 2^29); rows
- *     [n_out, out_cap) are -1.  out_cap 0 means the power of two >= n_out,
- *     the size the reference's operator allocates
- *     (sort_merge_join_operator.cpp:126-133).
+ *     [n_out, out_cap) are -1.  With `state`: the buffer the COUNT of the same
+ *     key columns and sizes filled, and n_out = the pair count it produced
+ *     (the caller reads *out_count, as the reference's operator reads its
+ *     counts, sort_merge_join_operator.cpp:113-133), so joins with their own
+ *     state buffers may interleave their COUNTs and WRITEs.  With state NULL:
+ *     the context's last COUNT with the same key pointers and sizes (type_id
+ *     0: the COUNT's).  out_cap 0 means the power of two >= n_out, the size
+ *     the reference's operator allocates (sort_merge_join_operator.cpp:126-133).
  * Every pass reads and writes every row of its padded size; n_out is the
  * only data-dependent value that leaves the device (the reference returns
  * it too).  Uses about 46 bytes of device memory per padded union row and
@@ -304,6 +311,8 @@ typedef struct vo_join_pairs_args {
     int64_t *out_left;   /* WRITE: device, out_cap rows */
     int64_t *out_right;  /* WRITE: device, out_cap rows */
     uint64_t out_cap;
+    void *state;         /* device, vo_join_pairs_state_bytes() bytes, caller-owned; NULL: context state */
+    uint64_t n_out;      /* WRITE with state: the COUNT's pair count */
 } vo_join_pairs_args;
 
 /* OP_PREDICATE_MASK: the device form of the host predicate evaluation
@@ -484,6 +493,8 @@ VO_EXPORT int vo_sort_batch(vo_ctx *ctx, const vo_sort_batch_args *args);
  * be NULL) receives the pair count after COUNT. */
 VO_EXPORT int vo_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args *args, uint64_t *n_out);
 VO_EXPORT int vo_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args *args);
+/* Bytes of the caller-owned COUNT -> WRITE state for n_left + n_right rows. */
+VO_EXPORT uint64_t vo_join_pairs_state_bytes(uint64_t n_left, uint64_t n_right);
 VO_EXPORT int vo_gather_oblivious(vo_ctx *ctx, const vo_gather_oblivious_args *args);
 /* OP_PREDICATE_MASK (replaces the host ComparisonExpression /
  * BinaryLogicalExpression::evaluateBatch, comparison_expression.cpp:13-92,

@@ -2,8 +2,8 @@
 src/cuda/task_manager.cu:26-124, src/cuda/persistent_kernel.cu:18-124):
 commands run inside one cooperative kernel and give the same bytes as launch
 mode and the oracle; ring back-pressure, restart after an idle exit,
-commands the resident kernel does not run (executed in order between
-resident ones), rejection of invalid commands, and stop.
+commands run in order, rejection of invalid commands, an entry point that
+parks the queue, and stop.
 
 The queue kernel stays resident between commands, so these tests never call
 a device-wide synchronize (it would wait for the resident kernel to leave):
@@ -150,10 +150,10 @@ def test_queue_elementwise_ops(qctx):
     assert np.array_equal(_host(d_enr, np.uint32), old[sig])
 
 
-def test_queue_runs_launch_only_opcodes_in_order(qctx):
-    """A command the resident kernel does not run (here OP_SHUFFLE) is
-    executed in order between resident commands; invalid commands of either
-    kind are rejected and the queue keeps running."""
+def test_queue_runs_commands_in_order(qctx):
+    """An oblivious operator (OP_SHUFFLE) and element-wise commands run in
+    submission order in the resident kernel; invalid commands are rejected
+    at submission and the queue keeps running."""
     import torch
 
     from velodb_amd import VoError, _abi
@@ -168,14 +168,15 @@ def test_queue_runs_launch_only_opcodes_in_order(qctx):
     sh = _abi.Command()
     sh.opcode = _abi.OP_SHUFFLE
     sh.args.shuffle = _abi.ShuffleArgs(pay.data_ptr(), 0, n, 99, 0)
-    s1 = qctx.submit(sh)                      # launch-only, runs now
     # the shuffled rows as a permutation: COMPOSE with the identity map
     ident = torch.arange(n, dtype=torch.int32, device="cuda:0")
+    perm32 = torch.empty(n, dtype=torch.int32, device="cuda:0")
+    torch.cuda.current_stream().synchronize()
+    s1 = qctx.submit(sh)
     comp = _abi.Command()
     comp.opcode = _abi.OP_COMPOSE_POSITION_MAP
-    perm32 = torch.empty(n, dtype=torch.int32, device="cuda:0")
     comp.args.compose_position_map = _abi.ComposeArgs(ident.data_ptr(), pay.data_ptr(), perm32.data_ptr(), n)
-    s2 = qctx.submit(comp)                    # resident, after the shuffle
+    s2 = qctx.submit(comp)                    # after the shuffle, on the same lane
     qctx.wait(s2, 20000)
     assert s2 > s1
     p = perm32.cpu().numpy()
@@ -188,7 +189,7 @@ def test_queue_runs_launch_only_opcodes_in_order(qctx):
     qctx.wait(qctx.submit(pm), 20000)
     assert np.array_equal(out.cpu().numpy(), src[p])
     c = _abi.Command()
-    c.opcode = _abi.OP_COMPACT                 # launch-only with null buffers
+    c.opcode = _abi.OP_COMPACT                 # null buffers
     with pytest.raises(VoError):
         qctx.submit(c)
     c = _abi.Command()

@@ -235,22 +235,28 @@ class Context:
                           _ptr(sorted_keys), _ptr(sorted_vals), 1 if dense_groups else 0)
         return self.check(self.lib.vo_pkfk_join_groupsum(self.h, C.byref(a)))
 
-    def join_pairs_count(self, left_keys, n_left, right_keys, n_right, type_id, out_count=None):
+    def join_pairs_count(self, left_keys, n_left, right_keys, n_right, type_id, out_count=None, state=None):
         """OP_SORT_MERGE_JOIN_COUNT: returns the number of (left, right) row
-        pairs with equal keys; the routing state stays in the context."""
+        pairs with equal keys; the routing state goes into `state` (a device
+        buffer of join_pairs_state_bytes() bytes) or stays in the context."""
         a = _abi.JoinPairsArgs(_ptr(left_keys), n_left, _ptr(right_keys), n_right, type_id, 0, _ptr(out_count),
-                               None, None, 0)
+                               None, None, 0, _ptr(state), 0)
         n = C.c_uint64(0)
         self.check(self.lib.vo_join_pairs_count(self.h, C.byref(a), C.byref(n)))
         return n.value
 
-    def join_pairs_write(self, left_keys, n_left, right_keys, n_right, type_id, out_left, out_right, out_cap):
+    def join_pairs_write(self, left_keys, n_left, right_keys, n_right, type_id, out_left, out_right, out_cap,
+                         state=None, n_out=0):
         """OP_SORT_MERGE_JOIN_WRITE after the COUNT of the same columns:
         out_left/out_right (int64, out_cap rows) get the pairs ordered by
-        (key, left row, right row), then -1."""
+        (key, left row, right row), then -1.  With `state`, n_out is the
+        count that COUNT returned."""
         a = _abi.JoinPairsArgs(_ptr(left_keys), n_left, _ptr(right_keys), n_right, type_id, 0, None,
-                               _ptr(out_left), _ptr(out_right), out_cap)
+                               _ptr(out_left), _ptr(out_right), out_cap, _ptr(state), n_out)
         return self.check(self.lib.vo_join_pairs_write(self.h, C.byref(a)))
+
+    def join_pairs_state_bytes(self, n_left, n_right):
+        return int(self.lib.vo_join_pairs_state_bytes(n_left, n_right))
 
     def join_pairs(self, left_keys, right_keys, type_id):
         """COUNT, allocate next_pow2(n_out) rows as the reference's operator

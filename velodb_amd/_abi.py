@@ -115,7 +115,8 @@ class SortBatchArgs(C.Structure):
 
 class JoinPairsArgs(C.Structure):
     _fields_ = [("left_keys", P), ("n_left", u64), ("right_keys", P), ("n_right", u64), ("type_id", u32),
-                ("reserved", u32), ("out_count", P), ("out_left", P), ("out_right", P), ("out_cap", u64)]
+                ("reserved", u32), ("out_count", P), ("out_left", P), ("out_right", P), ("out_cap", u64),
+                ("state", P), ("n_out", u64)]
 
 
 class GatherObliviousArgs(C.Structure):
@@ -226,6 +227,7 @@ PROTOTYPES = [
     ("vo_predicate_mask", C.c_int, [P, C.POINTER(PredicateArgs)]),
     ("vo_transfer_to_host", C.c_int, [P, P, P, u64, u64, C.POINTER(u64)]),
     ("vo_join_pairs_write", C.c_int, [P, C.POINTER(JoinPairsArgs)]),
+    ("vo_join_pairs_state_bytes", u64, [u64, u64]),
     ("vo_sort_kp_async", C.c_int, [P, P, P, u64, u64]),
     ("vo_trace_begin", C.c_int, [P]),
     ("vo_trace_end", C.c_int, [P, C.POINTER(Trace)]),

@@ -570,8 +570,22 @@ int vo_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args *args, uint64_t *n
     c.opcode = VO_OP_SORT_MERGE_JOIN_COUNT;
     c.args.join_pairs = *args;
     const int st = run_sync(ctx, c);
-    if (st == VO_OK && n_out) *n_out = join_pairs_last_count(ctx);
-    return st;
+    if (st != VO_OK || !n_out) return st;
+    if (!args->state) {
+        *n_out = join_pairs_last_count(ctx);
+        return VO_OK;
+    }
+    // the count sits in the caller's state buffer, after the three copies
+    return guarded(ctx, [&] {
+        const uint64_t off = join_pairs_count_offset(args->n_left, args->n_right);
+        VO_HIP(hipMemcpy(n_out, (const char *)args->state + off, 8, hipMemcpyDeviceToHost));
+    });
+}
+
+uint64_t vo_join_pairs_state_bytes(uint64_t n_left, uint64_t n_right)
+{
+    if (n_left >= (1ull << 31) || n_right >= (1ull << 31)) return 0;
+    return join_pairs_state_bytes(n_left, n_right);
 }
 
 // ------------------------------------------------------------ diagnostics

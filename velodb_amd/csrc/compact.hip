@@ -1,23 +1,8 @@
-// compact.hip -- oblivious order-preserving compaction (ORCompact routing
-// network), the Q6-style filter + exact int128 SUM, and OP_SCATTER.
-//
-// Reference method: oblivious_compact_elem / _inner_elem / _2power_inner_elem
-// (obliviator/join/enclave/oblivious_compact.c:59-153, 327-338).  For a
-// power-of-two n the recursion unrolls into log2(n) levels: at level q
-// (block size s = 2^(q+1), half = 2^q) every aligned block [b, b+s) with
-// offset o = C[b] mod s (C = exclusive prefix count of the selected rows)
-// conditionally swaps rows b+i and b+half+i for all i < half, with
-//   s == 2 : swap = (!sel[b] & sel[b+1]) ^ o
-//   s  > 2 : m1 = C[b+half] - C[b], om = o mod half, om1 = (o+m1) mod half,
-//            swap = ((om + m1 >= half) ^ (o >= half)) ^ (i >= om1)
-// (the offset form of :88-105; the chain of :121-153 reduces to o = C[b]
-// mod s for the blocks it creates).  Every row is read and written at every
-// level; the swap decision is a select, so addresses depend on n_pad only.
-//
-// Passes: count (selected rows per tile) -> tile scan -> local levels
-// q < T_LOG inside each tile (LDS) -> fused global levels, S per HBM pass,
-// rows at stride 2^q_lo as in the sort's global passes.
-#include "vo_internal.hpp"
+// compact.hip -- host drivers of the oblivious order-preserving compaction
+// (ORCompact), the Q6-style filter + exact int128 SUM and OP_SCATTER; the
+// device bodies and the method are in compact_kernels.hpp.
+#include "compact_kernels.hpp"
+#include "resident_tables.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -34,572 +19,10 @@ static __device__ __align__(256) unsigned char scratch[2u << 20];
 namespace vo {
 namespace {
 
-constexpr int CT_LOG = 12; // tile: 4096 rows
-constexpr int CT = 1 << CT_LOG;
-constexpr int CBLOCK = 256;
+using namespace compact_k;
 
-// A column of the compaction: pointer and element width (1, 2, 4, 8 bytes).
-struct Col {
-    void *p;
-    uint32_t w;
-};
-
-template <int NW>
-struct Cols {
-    Col c[NW];
-};
-
-__device__ __forceinline__ uint64_t ld_col(const Col &c, uint64_t i)
-{
-    switch (c.w) {
-    case 8: return ((const uint64_t *)c.p)[i];
-    case 4: return ((const uint32_t *)c.p)[i];
-    case 2: return ((const uint16_t *)c.p)[i];
-    default: return ((const uint8_t *)c.p)[i];
-    }
-}
-
-__device__ __forceinline__ void st_col(const Col &c, uint64_t i, uint64_t v)
-{
-    switch (c.w) {
-    case 8: ((uint64_t *)c.p)[i] = v; break;
-    case 4: ((uint32_t *)c.p)[i] = (uint32_t)v; break;
-    case 2: ((uint16_t *)c.p)[i] = (uint16_t)v; break;
-    default: ((uint8_t *)c.p)[i] = (uint8_t)v; break;
-    }
-}
-
-// swap flag of the pair at offset i in the block starting at prefix count
-// cb, with cbh = prefix count at the block's middle; s2 = half size
-__device__ __forceinline__ bool level_flag(uint64_t cb, uint64_t cbh, uint64_t half, uint64_t i)
-{
-    const uint64_t s = half * 2;
-    const uint64_t o = cb & (s - 1);
-    const uint64_t m1 = cbh - cb;
-    const uint64_t om = o & (half - 1);
-    const uint64_t om1 = (o + m1) & (half - 1);
-    const bool base = ((om + m1) >= half) ^ (o >= half);
-    return base ^ (i >= om1);
-}
-
-// The same flag in 32-bit arithmetic, for n_pad <= 2^32 (q <= 31): only the
-// low 32 bits of the prefix counts matter (o is their low q+1 bits, and the
-// count difference m1 <= half is exact modulo 2^32).
-__device__ __forceinline__ bool level_flag32(uint32_t cb, uint32_t cbh, int q, uint32_t i)
-{
-    const uint32_t half = 1u << q;
-    const uint32_t smask = half | (half - 1);
-    const uint32_t o = cb & smask;
-    const uint32_t m1 = cbh - cb;
-    const uint32_t om = o & (half - 1);
-    const uint32_t om1 = (o + m1) & (half - 1);
-    const bool base = ((om + m1) >= half) ^ (o >= half);
-    return base ^ (i >= om1);
-}
-
-// ---------------------------------------------------------------- counting
-
-// Selected rows per tile: sel[i] = mask[i] != 0 && i < n.
-__global__ __launch_bounds__(CBLOCK) void k_count_mask(const uint8_t *__restrict__ mask, uint64_t n,
-                                                       uint32_t *__restrict__ tile_cnt, TraceSlot *slot)
-{
-    __shared__ uint32_t red[CBLOCK / 64];
-    dev::Atr tr(slot);
-    const uint64_t t0 = (uint64_t)blockIdx.x * CT;
-    uint32_t c = 0;
-#pragma unroll 4
-    for (int k = 0; k < CT / CBLOCK; ++k) {
-        const uint64_t i = t0 + k * CBLOCK + threadIdx.x;
-        c += (mask[i] != 0) & (i < n);
-        tr.at(1, i);
-    }
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (int w = 0; w < CBLOCK / 64; ++w) s += red[w];
-        tile_cnt[blockIdx.x] = s;
-        tr.at(2, blockIdx.x);
-    }
-    tr.done(dev::gtid());
-}
-
-// Q6-style predicate: writes mask[i] and per-tile counts in one pass.
-struct Pred {
-    int32_t day_lo, day_hi;
-    int64_t disc_lo, disc_hi, qty_lt;
-};
-
-__global__ __launch_bounds__(CBLOCK) void k_filter_count(const int32_t *__restrict__ day,
-                                                         const int64_t *__restrict__ qty,
-                                                         const int64_t *__restrict__ disc, uint64_t n, Pred pr,
-                                                         uint8_t *__restrict__ mask, uint32_t *__restrict__ tile_cnt,
-                                                         TraceSlot *slot)
-{
-    __shared__ uint32_t red[CBLOCK / 64];
-    dev::Atr tr(slot);
-    const uint64_t t0 = (uint64_t)blockIdx.x * CT;
-    uint32_t c = 0;
-#pragma unroll 4
-    for (int k = 0; k < CT / CBLOCK; ++k) {
-        const uint64_t i = t0 + k * CBLOCK + threadIdx.x;
-        const int32_t d = day[i];
-        const int64_t q = qty[i], ds = disc[i];
-        const uint32_t m = (d >= pr.day_lo) & (d < pr.day_hi) & (ds >= pr.disc_lo) & (ds <= pr.disc_hi) &
-                           (q < pr.qty_lt) & (i < n);
-        mask[i] = (uint8_t)m;
-        c += m;
-        tr.at(3, i); // day, qty, disc read and mask written at row i
-    }
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (int w = 0; w < CBLOCK / 64; ++w) s += red[w];
-        tile_cnt[blockIdx.x] = s;
-        tr.at(2, blockIdx.x);
-    }
-    tr.done(dev::gtid());
-}
-
-// Exclusive scan of the tile counts (one block): pre[0..nt], pre[nt] = total.
-__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict__ cnt, uint32_t nt,
-                                                     uint64_t *__restrict__ pre, uint64_t *__restrict__ out_count,
-                                                     TraceSlot *slot)
-{
-    __shared__ uint64_t part[1024];
-    dev::Atr tr(slot);
-    const uint32_t per = (nt + 1023) / 1024;
-    const uint32_t lo = threadIdx.x * per;
-    uint64_t s = 0;
-    for (uint32_t k = 0; k < per; ++k) {
-        const uint32_t i = lo + k;
-        if (i < nt) {
-            s += cnt[i];
-            tr.at(4, i);
-        }
-    }
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint64_t v = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint64_t run = part[threadIdx.x] - s; // exclusive prefix of this thread's chunk
-    for (uint32_t k = 0; k < per; ++k) {
-        const uint32_t i = lo + k;
-        if (i < nt) {
-            pre[i] = run;
-            run += cnt[i];
-            tr.at(5, i);
-        }
-    }
-    if (threadIdx.x == 1023) {
-        pre[nt] = part[1023];
-        if (out_count) *out_count = part[1023];
-        tr.at(6, nt);
-    }
-    tr.done(dev::gtid());
-}
-
-// ---------------------------------------------------------- local levels
-
-// Levels q = 0 .. CT_LOG-1 inside each 4096-row tile.  Thread tau holds 16
-// rows of each column in registers under layout a: row base(tau, a) | m << a
-// for m < 16, base(tau, a) = the 8 bits of tau around the 4-bit hole at a.
-// Levels a .. a+3 pair rows inside a thread; layouts a = 0, 4, 8 cover the 12
-// levels, with two LDS transposes between them (and one after the coalesced
-// load, layout 8).  Rows are read from `src` and written to `dst` (which may
-// be the same columns).  cpre (LDS) holds the tile's exclusive prefix counts.
-constexpr uint32_t CTPAD = CT + CT / 16; // transpose buffer: row l at l + l/16
-
-__device__ __forceinline__ uint32_t cpos(uint32_t l) { return l + (l >> 4); }
-
-__device__ __forceinline__ uint32_t clay(uint32_t tau, int a) { return (tau & ((1u << a) - 1)) | ((tau >> a) << (a + 4)); }
-
-template <int NW>
-__device__ __forceinline__ void ctranspose(uint64_t (&r)[16][NW], uint64_t *buf, uint32_t tau, int a_from, int a_to)
-{
-    const uint32_t bf = clay(tau, a_from), bt = clay(tau, a_to);
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-#pragma unroll
-        for (int m = 0; m < 16; ++m) buf[cpos(bf | ((uint32_t)m << a_from))] = r[m][w];
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < 16; ++m) r[m][w] = buf[cpos(bt | ((uint32_t)m << a_to))];
-        __syncthreads();
-    }
-}
-
-template <int NW, int A>
-__device__ __forceinline__ void clevels(uint64_t (&r)[16][NW], const uint16_t *cpre, uint32_t tbase, uint32_t tau)
-{
-    const uint32_t base = clay(tau, A);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int q = A + k;
-        const uint32_t half = 1u << q;
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            if (m & (1 << k)) continue;
-            const int m2 = m | (1 << k);
-            const uint32_t e0 = base | ((uint32_t)m << A);
-            const uint32_t b = e0 & ~(2 * half - 1);
-            const uint32_t cb = tbase + cpre[b];
-            bool sw;
-            if (q == 0) {
-                const uint32_t e1 = e0 | 1;
-                const bool s0 = cpre[e0 + 1] != cpre[e0];
-                const bool s1 = cpre[e1 + 1] != cpre[e1];
-                sw = ((!s0) & s1) ^ (bool)(cb & 1);
-            } else {
-                sw = level_flag32(cb, tbase + cpre[b + half], q, e0 & (half - 1));
-            }
-#pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                const uint64_t x = r[m][w], y = r[m2][w];
-                r[m][w] = sw ? y : x;
-                r[m2][w] = sw ? x : y;
-            }
-        }
-    }
-}
-
-// W8: every column is 8 bytes wide (no per-load width dispatch)
-template <int NW, bool W8, bool TRACE>
-__global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> src, Cols<NW> dst, const uint8_t *__restrict__ mask,
-                                                          uint64_t n, const uint64_t *__restrict__ tile_pre,
-                                                          TraceSlot *slot)
-{
-    static_assert(CT == 16 * CBLOCK, "16 rows per thread");
-    extern __shared__ __align__(16) char lds_raw[];
-    uint64_t *buf = reinterpret_cast<uint64_t *>(lds_raw);          // one column, CTPAD words
-    uint16_t *cpre = reinterpret_cast<uint16_t *>(buf + CTPAD);      // CT + 1 counts <= 4096
-    __shared__ uint32_t wsum[CBLOCK / 64];
-    const uint32_t tid = threadIdx.x;
-    const uint64_t t0 = (uint64_t)blockIdx.x * CT;
-    const uint64_t tbase = tile_pre[blockIdx.x];
-    constexpr int PER = CT / CBLOCK; // 16 consecutive rows per thread for the scan
-    dev::AtrT<TRACE> tr(slot);
-    tr.at(7, blockIdx.x);
-
-    // rows, coalesced (layout 8: row tid + 256 m); issued before the scan
-    uint64_t r[16][NW];
-#pragma unroll
-    for (int m = 0; m < 16; ++m)
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            r[m][w] = W8 ? ((const uint64_t *)src.c[w].p)[t0 + tid + 256u * m] : ld_col(src.c[w], t0 + tid + 256u * m);
-            tr.at(8 + w, t0 + tid + 256u * m);
-        }
-
-    // exclusive prefix of sel inside the tile: thread tid owns rows [16 tid, 16 tid + 16)
-    uint32_t sel[PER];
-    uint32_t run = 0;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const uint64_t i = t0 + tid * PER + k;
-        sel[k] = (mask[i] != 0) & (i < n);
-        run += sel[k];
-        tr.at(1, i);
-    }
-    uint32_t incl = run;
-    const uint32_t lane = tid & 63;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t v = __shfl_up(incl, off);
-        incl += lane >= (uint32_t)off ? v : 0;
-    }
-    if (lane == 63) wsum[tid >> 6] = incl;
-    __syncthreads();
-    uint32_t wpre = 0;
-    for (uint32_t w = 0; w < (tid >> 6); ++w) wpre += wsum[w];
-    uint32_t c = wpre + incl - run;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        cpre[tid * PER + k] = (uint16_t)c;
-        c += sel[k];
-    }
-    if (tid == CBLOCK - 1) cpre[CT] = (uint16_t)c;
-    // (the first transpose's barriers also publish cpre)
-    ctranspose<NW>(r, buf, tid, 8, 0);
-    clevels<NW, 0>(r, cpre, (uint32_t)tbase, tid);
-    ctranspose<NW>(r, buf, tid, 0, 4);
-    clevels<NW, 4>(r, cpre, (uint32_t)tbase, tid);
-    ctranspose<NW>(r, buf, tid, 4, 8);
-    clevels<NW, 8>(r, cpre, (uint32_t)tbase, tid);
-    uint32_t tid_s = tid; // store addresses recomputed here (register pressure)
-    asm volatile("" : "+v"(tid_s));
-#pragma unroll
-    for (int m = 0; m < 16; ++m)
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            if (W8)
-                ((uint64_t *)dst.c[w].p)[t0 + tid_s + 256u * m] = r[m][w];
-            else
-                st_col(dst.c[w], t0 + tid_s + 256u * m, r[m][w]);
-            tr.at(16 + w, t0 + tid_s + 256u * m);
-        }
-    tr.done(dev::gtid());
-}
-
-// ---------------------------------------------------------- global levels
-
-// Levels q_lo .. q_lo+S-1 (ascending, q_lo >= CT_LOG): thread g owns the 2^S
-// rows base + m * 2^q_lo.  The block start of any pair has every bit below
-// q_lo + 1 clear, so its prefix counts are the same for all lanes of a
-// wavefront and come from the tile prefix array.
-template <int NW, int S, bool TRACE>
-__global__ __launch_bounds__(256) void k_compact_global(Cols<NW> cols, const uint64_t *__restrict__ tile_pre, int q_lo,
-                                                        TraceSlot *slot)
-{
-    dev::AtrT<TRACE> tr(slot);
-    const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const uint64_t base = (g & ((1ull << q_lo) - 1)) | ((g >> q_lo) << (q_lo + S));
-    uint64_t r[1 << S][NW];
-#pragma unroll
-    for (int m = 0; m < (1 << S); ++m)
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            r[m][w] = ld_col(cols.c[w], base + ((uint64_t)m << q_lo));
-            tr.at(8 + w, base + ((uint64_t)m << q_lo));
-        }
-    // uniform part of base (bits >= 8 are shared by the wavefront)
-    const uint64_t ubase = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
-                           __builtin_amdgcn_readfirstlane((uint32_t)base & ~0xFFu);
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-        const int q = q_lo + k;
-        const uint64_t half = 1ull << q;
-#pragma unroll
-        for (int m = 0; m < (1 << S); ++m) {
-            if (m & (1 << k)) continue;
-            const uint64_t e = base + ((uint64_t)m << q_lo);
-            const uint64_t ue = ubase + ((uint64_t)m << q_lo);
-            const uint64_t b = ue & ~(2 * half - 1);
-            const uint64_t cb = tile_pre[b >> CT_LOG];
-            const uint64_t cbh = tile_pre[(b + half) >> CT_LOG];
-            tr.at(7, b >> CT_LOG);
-            tr.at(7, (b + half) >> CT_LOG);
-            const bool sw = level_flag(cb, cbh, half, e & (half - 1));
-            const int m2 = m | (1 << k);
-#pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                const uint64_t x = r[m][w], y = r[m2][w];
-                r[m][w] = sw ? y : x;
-                r[m2][w] = sw ? x : y;
-            }
-        }
-    }
-#pragma unroll
-    for (int m = 0; m < (1 << S); ++m)
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            st_col(cols.c[w], base + ((uint64_t)m << q_lo), r[m][w]);
-            tr.at(16 + w, base + ((uint64_t)m << q_lo));
-        }
-    tr.done(dev::gtid());
-}
-
-// ------------------------------------------------- LDS-staged global levels
-
-// S = 5..8 levels q_lo .. q_lo+S-1 (q_lo >= CT_LOG) in one pass.  A block
-// takes the 4096 rows hibase + m * 2^q_lo + j0 + c (m < 2^S, c < 2^(12-S)):
-// local row l = m << (12-S) | c, so level q_lo+k pairs local bit 12-S+k.
-// Registers + LDS transposes as in the local pass (layout 4 for local bits
-// 12-S .. 7, layout 8 for bits 8 .. 11); the prefix counts at every block
-// boundary the levels need, tile_pre at hibase + m * 2^q_lo, go to LDS.
-template <int NW, int A, int S>
-__device__ __forceinline__ void hlevels(uint64_t (&r)[16][NW], const uint32_t *P, uint32_t tau, int q_lo, uint32_t j0)
-{
-    constexpr int CB = CT_LOG - S;
-    const uint32_t base = clay(tau, A);
-#pragma unroll
-    for (int bit = A; bit < A + 4; ++bit) {
-        if (bit < CB) continue;
-        const int k = bit - CB; // level q_lo + k
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            if (m & (1 << (bit - A))) continue;
-            const int m2 = m | (1 << (bit - A));
-            const uint32_t l0 = base | ((uint32_t)m << A);
-            const uint32_t ml = l0 >> CB;
-            const uint32_t mb = ml & ~((2u << k) - 1);
-            const uint32_t i = ((ml & ((1u << k) - 1)) << q_lo) + j0 + (l0 & ((1u << CB) - 1));
-            const bool sw = level_flag32(P[mb], P[mb + (1u << k)], q_lo + k, i);
-#pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                const uint64_t x = r[m][w], y = r[m2][w];
-                r[m][w] = sw ? y : x;
-                r[m2][w] = sw ? x : y;
-            }
-        }
-    }
-}
-
-template <int NW, int S, bool W8, bool TRACE>
-__global__ __launch_bounds__(CBLOCK) void k_compact_hybrid(Cols<NW> cols, const uint64_t *__restrict__ tile_pre,
-                                                           int q_lo, TraceSlot *slot)
-{
-    dev::AtrT<TRACE> tr(slot);
-    static_assert(S >= 5 && S <= 8, "hybrid widths 5..8");
-    constexpr int CB = CT_LOG - S;
-    extern __shared__ __align__(16) char lds_raw[];
-    uint64_t *buf = reinterpret_cast<uint64_t *>(lds_raw);     // CTPAD words
-    uint32_t *P = reinterpret_cast<uint32_t *>(buf + CTPAD); // low words of 2^S + 1 prefix counts
-    const uint32_t tid = threadIdx.x;
-    // row indices fit 32 bits (n_pad <= 2^32)
-    const uint32_t runs = (1u << q_lo) >> CB; // blocks per group of 2^(q_lo+S) rows
-    const uint32_t hibase = (blockIdx.x / runs) << (q_lo + S);
-    const uint32_t j0 = (blockIdx.x % runs) << CB;
-    for (uint32_t k = tid; k <= (1u << S); k += CBLOCK) {
-        P[k] = (uint32_t)tile_pre[((uint64_t)hibase + ((uint64_t)k << q_lo)) >> CT_LOG];
-        tr.at(7, ((uint64_t)hibase + ((uint64_t)k << q_lo)) >> CT_LOG);
-    }
-    uint64_t r[16][NW];
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const uint32_t l = tid | ((uint32_t)m << 8);
-        const uint32_t g = hibase + ((l >> CB) << q_lo) + j0 + (l & ((1u << CB) - 1));
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            r[m][w] = W8 ? ((const uint64_t *)cols.c[w].p)[g] : ld_col(cols.c[w], g);
-            tr.at(8 + w, g);
-        }
-    }
-    ctranspose<NW>(r, buf, tid, 8, 4); // its barriers also publish P
-    hlevels<NW, 4, S>(r, P, tid, q_lo, j0);
-    ctranspose<NW>(r, buf, tid, 4, 8);
-    hlevels<NW, 8, S>(r, P, tid, q_lo, j0);
-    // recompute the store addresses here rather than keep the load addresses
-    // live across the levels (register pressure)
-    uint32_t tid_s = tid;
-    asm volatile("" : "+v"(tid_s));
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const uint32_t l = tid_s | ((uint32_t)m << 8);
-        const uint32_t g = hibase + ((l >> CB) << q_lo) + j0 + (l & ((1u << CB) - 1));
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            if (W8)
-                ((uint64_t *)cols.c[w].p)[g] = r[m][w];
-            else
-                st_col(cols.c[w], g, r[m][w]);
-            tr.at(16 + w, g);
-        }
-    }
-    tr.done(dev::gtid());
-}
-
-// ------------------------------------------------------- int128 SUM pass
-
-// Exact SUM(a[i] * b[i]) over rows i < count (count read on the device);
-// every row is read, rows >= count contribute a multiplied-out zero.
-__global__ __launch_bounds__(256) void k_sum_prod_i128(const int64_t *__restrict__ a, const int64_t *__restrict__ b,
-                                                       uint64_t n_pad, const uint64_t *__restrict__ count,
-                                                       __int128 *__restrict__ partial, TraceSlot *slot)
-{
-    __shared__ __int128 red[256];
-    dev::Atr tr(slot);
-    const uint64_t m = *count;
-    __int128 acc = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n_pad; i += (uint64_t)gridDim.x * 256) {
-        const __int128 prod = (__int128)a[i] * (__int128)b[i];
-        acc += prod * (__int128)(i < m);
-        tr.at(8, i);
-    }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < (uint32_t)off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        partial[blockIdx.x] = red[0];
-        tr.at(16, blockIdx.x);
-    }
-    tr.done(dev::gtid());
-}
-
-// One 256-thread block: strided partial sums, then a tree in LDS (exact:
-// int128 addition is associative).
-__global__ __launch_bounds__(256) void k_sum_final(const __int128 *__restrict__ partial, uint32_t np,
-                                                   const uint64_t *__restrict__ count, uint64_t *__restrict__ out,
-                                                   TraceSlot *slot)
-{
-    __shared__ __int128 red[256];
-    dev::Atr tr(slot);
-    __int128 s = 0;
-    for (uint32_t i = threadIdx.x; i < np; i += 256) {
-        s += partial[i];
-        tr.at(8, i);
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < (uint32_t)off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const __int128 t = red[0];
-        out[0] = *count;
-        out[1] = 0;
-        out[2] = (uint64_t)t;
-        out[3] = (uint64_t)(t >> 64);
-        tr.at(16, 0);
-    }
-    tr.done(dev::gtid());
-}
-
-// ------------------------------------------------------------ OP_SCATTER
-
-__global__ __launch_bounds__(CBLOCK) void k_scatter_positions(const uint8_t *__restrict__ mask, uint64_t n,
-                                                              const uint64_t *__restrict__ tile_pre,
-                                                              int32_t *__restrict__ out, TraceSlot *slot)
-{
-    __shared__ uint32_t wsum[CBLOCK / 64];
-    dev::Atr tr(slot);
-    const uint32_t tid = threadIdx.x;
-    const uint64_t t0 = (uint64_t)blockIdx.x * CT;
-    constexpr int PER = CT / CBLOCK;
-    uint32_t sel[PER], run = 0;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const uint64_t i = t0 + tid * PER + k;
-        if (i < n) {
-            sel[k] = mask[i] != 0;
-            tr.at(1, i);
-        } else {
-            sel[k] = 0;
-        }
-        run += sel[k];
-    }
-    uint32_t incl = run;
-    const uint32_t lane = tid & 63;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t v = __shfl_up(incl, off);
-        incl += lane >= (uint32_t)off ? v : 0;
-    }
-    if (lane == 63) wsum[tid >> 6] = incl;
-    __syncthreads();
-    uint32_t wpre = 0;
-    for (uint32_t w = 0; w < (tid >> 6); ++w) wpre += wsum[w];
-    uint64_t c = tile_pre[blockIdx.x] + wpre + incl - run;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const uint64_t i = t0 + tid * PER + k;
-        if (sel[k]) { // reference semantics: only selected rows are written (not oblivious)
-            out[i] = (int32_t)c;
-            tr.at(16, i);
-        }
-        c += sel[k];
-    }
-    tr.done(dev::gtid());
-}
+const std::pair<const void *, BodyInfo> kBodies[] = {VO_COMPACT_BODIES(VO_BODY_ENTRY)};
+const bool kRegistered = (register_bodies(kBodies, sizeof kBodies / sizeof kBodies[0]), true);
 
 // --------------------------------------------------------------- drivers
 
@@ -902,15 +325,15 @@ void op_scatter(vo_ctx *ctx, const vo_scatter_args &a)
 {
     VO_CHECK(a.in_mask && a.out_indices, VO_ERR_INVALID, "null buffer");
     if (a.n == 0) {
-        if (a.out_count) VO_HIP(hipMemsetAsync(a.out_count, 0, 8, ctx->stream));
+        if (a.out_count) fill_async(ctx, a.out_count, 0, 8);
         return;
     }
     // pad the count/scan grid to whole tiles; rows >= n are never read
     const uint64_t n_pad = ((a.n + CT - 1) / CT) * CT;
     Workspace ws = workspace(ctx, n_pad, true);
     const uint32_t nt = (uint32_t)(n_pad / CT);
-    VO_HIP(hipMemsetAsync(ws.mask, 0, n_pad, ctx->stream));
-    VO_HIP(hipMemcpyAsync(ws.mask, a.in_mask, a.n, hipMemcpyDeviceToDevice, ctx->stream));
+    fill_async(ctx, ws.mask, 0, n_pad);
+    copy_async(ctx, ws.mask, a.in_mask, a.n);
     count_and_scan(ctx, ws.mask, a.n, n_pad, ws, a.out_count);
     launch(ctx, LaunchInfo{"scatter_positions", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_scatter_positions,
            (const uint8_t *)ws.mask, a.n, (const uint64_t *)ws.tile_pre, a.out_indices, trace_slot(ctx));

@@ -24,8 +24,14 @@
 //     has been idle for the idle period, and a barrier that does not
 //     complete in time aborts the kernel, so the grid always drains.  The
 //     host relaunches on the next submit or wait after an idle exit.
+#include "compact_kernels.hpp"
+#include "gather_obl_kernels.hpp"
+#include "join_kernels.hpp"
+#include "join_pairs_kernels.hpp"
 #include "misc_kernels.hpp"
+#include "predicate_kernels.hpp"
 #include "resident_tables.hpp"
+#include "shuffle_kernels.hpp"
 #include "sort_kernels.hpp"
 #include "sort_multi_kernels.hpp"
 
@@ -807,6 +813,18 @@ int queue_sync(vo_ctx *ctx)
 }
 
 uint32_t queue_lanes(vo_ctx *ctx) { return ctx->queue ? ((QueueState *)ctx->queue)->lanes : 0; }
+
+// The sequence id the command being recorded will receive (ids are handed
+// out in order under ctx->mu).
+uint64_t queue_next_seq(vo_ctx *ctx) { return ((QueueState *)ctx->queue)->submitted + 1; }
+
+// Waits for one submitted command; caller holds ctx->mu.
+void queue_wait_seq_locked(vo_ctx *ctx, uint64_t seq)
+{
+    QueueState *q = (QueueState *)ctx->queue;
+    VO_CHECK(seq >= 1 && seq <= q->submitted, VO_ERR_INVALID, "wait on a sequence id never submitted");
+    wait_locked(ctx, q, [&] { return seq_done(q, seq); }, "command not finished");
+}
 
 } // namespace vo
 

@@ -198,7 +198,9 @@ void op_join_groupsum(vo_ctx *ctx, const vo_join_args &a);
 uint64_t op_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args &a); // returns n_out (synchronises)
 void op_join_pairs_write(vo_ctx *ctx, const vo_join_pairs_args &a);
 void join_pairs_state_free(vo_ctx *ctx);
-uint64_t join_pairs_last_count(vo_ctx *ctx); // n_out of the last COUNT
+uint64_t join_pairs_last_count(vo_ctx *ctx); // n_out of the last COUNT (context state)
+uint64_t join_pairs_state_bytes(uint64_t n_left, uint64_t n_right);
+uint64_t join_pairs_count_offset(uint64_t n_left, uint64_t n_right); // of n_out inside the state
 void op_gather_oblivious(vo_ctx *ctx, const vo_gather_oblivious_args &a);
 void op_predicate_mask(vo_ctx *ctx, const vo_predicate_args &a);
 void op_transfer_to_device(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes, uint64_t padded_bytes);
@@ -221,6 +223,8 @@ int queue_park(vo_ctx *ctx);          // finish submitted commands, let the resi
 uint64_t queue_submit_lane(vo_ctx *ctx, vo_command *cmd, uint32_t lane);
 void queue_drain_locked(vo_ctx *ctx); // every submitted command complete (caller holds ctx->mu)
 uint32_t queue_lanes(vo_ctx *ctx);
+uint64_t queue_next_seq(vo_ctx *ctx);                  // id of the command being recorded
+void queue_wait_seq_locked(vo_ctx *ctx, uint64_t seq); // caller holds ctx->mu
 
 // ---- shared device helpers (sort.hip) used by other ops
 // Sort (u64 key, u64 value) records by (key, value) ascending, n_pad = 2^k.
