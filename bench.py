@@ -774,7 +774,9 @@ def served_workload(slots=16, rates=(0, 800), n_tables=12, check_queries=200, lo
     at once) the same query list runs three ways: served (`slots` concurrent
     slots over the resident tables), plain (one query at a time, inputs
     shipped from pinned host memory per query: the reference engine's path)
-    and plain_resident (one at a time over the resident tables).  The first
+    plain_resident (one at a time over the resident tables) and served_queue
+    (every query one program in the resident command queue, a lane per slot
+    up to 8 lanes).  The first
     `check_queries` queries of the first load then run served again with
     their whole results copied out, each checked against the oracle's closed
     form of its table."""
@@ -818,6 +820,7 @@ def served_workload(slots=16, rates=(0, 800), n_tables=12, check_queries=200, lo
 
     srv.run(mix(64, 0, 99))  # warm-up
     srv.run(mix(16, 0, 98), n_slots=1, transfer=True)
+    srv.run(mix(64, 0, 97), queue=True)
     out = {"slots": slots, "tables_per_kind": n_tables, "loads": []}
     first_qs = None
     all_equal = True
@@ -828,11 +831,15 @@ def served_workload(slots=16, rates=(0, 800), n_tables=12, check_queries=200, lo
         rp, sp = srv.run(qs, n_slots=1, transfer=True)
         rr, sr = srv.run(qs, n_slots=1)
         rb, sb = srv.run(qs)
-        same = [list(a.r) for a in rp] == [list(b.r) for b in rb] == [list(c.r) for c in rr]
+        rq, sq = srv.run(qs, queue=True)
+        same = [list(a.r) for a in rp] == [list(b.r) for b in rb] == [list(c.r) for c in rr] == \
+            [list(d.r) for d in rq]
         all_equal = all_equal and same
         ld = {"offered_qps": rate if rate else "all at t=0", "queries": count,
               "served": summ(sb, rb, qs), "plain": summ(sp, rp, qs), "plain_resident": summ(sr, rr, qs),
+              "served_queue": summ(sq, rq, qs),
               "throughput_ratio": round(sb.qps / max(sp.qps, 1e-9), 2),
+              "throughput_ratio_queue": round(sq.qps / max(sp.qps, 1e-9), 2),
               "throughput_ratio_vs_plain_resident": round(sb.qps / max(sr.qps, 1e-9), 2),
               "p99_ratio_plain_over_served": round(sp.p99_s / max(sb.p99_s, 1e-12), 2),
               "results_equal": same}

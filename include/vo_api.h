@@ -646,10 +646,14 @@ VO_EXPORT int vo_server_run(vo_server *s, const vo_query *q, uint64_t n, uint32_
 /* How a run reaches the tables. */
 enum {
     VO_RUN_RESIDENT = 0, /* queries read the device-resident tables in place */
-    VO_RUN_TRANSFER = 1  /* the reference engine's per-query path: each query copies its input
+    VO_RUN_TRANSFER = 1, /* the reference engine's per-query path: each query copies its input
                             columns from pinned host memory into device buffers
                             (ValueVector::to(CUDA), include/data/value_vector.hpp:248-272), and a
                             sort copies its sorted columns back to host memory */
+    VO_RUN_QUEUE = 2     /* over the resident tables, every admitted query's operators go into the
+                            resident command queue as one program, on its slot's lane of a
+                            persistent context (min(n_slots, 8) lanes): the resident kernel runs
+                            the lanes' queries side by side */
 };
 VO_EXPORT int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_slots_override,
                                  uint32_t mode, vo_query_result *out, vo_run_stats *stats);

@@ -114,3 +114,22 @@ def test_graph_replays_equal_eager_results():
         assert replays > 0
     finally:
         s.close()
+
+
+def test_queue_mode_matches_the_graph_loop():
+    """Queue mode: every admitted query runs as one program in the resident
+    command queue, on its slot's lane; the results equal the graph-replay
+    loop's and the eager transfer loop's, with queries in flight on several
+    lanes at once."""
+    s, tables = _server(n_slots=8)
+    try:
+        qs = _mix(tables, 60, 0, 9)
+        want, _ = s.run(qs, n_slots=1, transfer=True)
+        got, st = s.run(qs, queue=True)
+        assert all(r.status == 0 for r in got)
+        assert [list(r.r) for r in got] == [list(r.r) for r in want]
+        assert st.max_concurrent > 1
+        again, _ = s.run(qs, n_slots=3, queue=True)  # 3 lanes
+        assert [list(r.r) for r in again] == [list(r.r) for r in want]
+    finally:
+        s.close()

@@ -3,9 +3,9 @@ rows (one of each kind at 1e7), the 50/30/20 filter+SUM / join+group-by /
 sort mix, Poisson arrivals at 800 QPS and a 200 GB admission budget.  Every
 query's whole result is checked against the oracle's closed forms of its
 table (filter count, exact SUM and compacted rows; the join's groups and
-sums; the sorted keys and payloads), for the concurrent mix and for the plain
-one-query-at-a-time loop over shipped inputs, so each result is the same
-alone and in the mix."""
+sums; the sorted keys and payloads), for the concurrent mix, for the plain
+one-query-at-a-time loop over shipped inputs and for the mix run through the
+resident command queue, so each result is the same alone and in the mix."""
 import numpy as np
 import pytest
 
@@ -99,6 +99,9 @@ def test_served_mix_at_config5_shape_matches_the_oracle():
         assert st.max_concurrent > 1 and st.peak_bytes_in_flight <= 200 << 30
         print("config5 plain loop", flush=True)
         _run_checked(srv, S, qs, want, rows, n_slots=1, transfer=True)
+        print("config5 through the resident command queue", flush=True)
+        _, stq = _run_checked(srv, S, qs, want, rows, queue=True)
+        assert stq.max_concurrent > 1
         # the timed form (no output copies) returns the same result words
         res2, _ = srv.run(qs)
         assert [list(a.r) for a in res2] == [list(b.r) for b in res]

@@ -92,13 +92,14 @@ class Server:
         self._check(self.lib.vo_server_add_sort_table(self.h, seed, n_rows, C.byref(i)))
         return i.value
 
-    def run(self, queries, n_slots=0, transfer=False, outputs=None):
+    def run(self, queries, n_slots=0, transfer=False, outputs=None, queue=False):
         """queries: list of (kind, table, arrival_s).  Returns (results, stats).
         transfer=True ships each query's inputs from pinned host memory (the
         reference engine's per-query path) instead of reading the resident
-        tables.  outputs: optional list (one entry per query) of numpy uint8
-        arrays of query_out_bytes() bytes, or None, that receive each query's
-        whole result (vo_query.out)."""
+        tables.  queue=True runs every query as one program in the resident
+        command queue, a lane per slot.  outputs: optional list (one entry per
+        query) of numpy uint8 arrays of query_out_bytes() bytes, or None, that
+        receive each query's whole result (vo_query.out)."""
         qs = []
         for i, (k, t, a) in enumerate(queries):
             o = outputs[i] if outputs is not None else None
@@ -106,7 +107,7 @@ class Server:
         arr = (_abi.Query * len(queries))(*qs)
         res = (_abi.QueryResult * len(queries))()
         st = _abi.RunStats()
-        mode = _abi.RUN_TRANSFER if transfer else _abi.RUN_RESIDENT
+        mode = _abi.RUN_QUEUE if queue else _abi.RUN_TRANSFER if transfer else _abi.RUN_RESIDENT
         self._check(self.lib.vo_server_run_mode(self.h, arr, len(queries), n_slots, mode, res, C.byref(st)))
         return list(res), st
 

@@ -170,7 +170,7 @@ class Config(C.Structure):
 
 
 Q_FILTER_SUM, Q_JOIN_GROUPSUM, Q_SORT = 1, 2, 3
-RUN_RESIDENT, RUN_TRANSFER = 0, 1
+RUN_RESIDENT, RUN_TRANSFER, RUN_QUEUE = 0, 1, 2
 
 
 class ServerConfig(C.Structure):

@@ -45,6 +45,21 @@ void *scratch(vo_ctx *ctx, size_t bytes)
     return arena;
 }
 
+void reserve_lane_scratch(vo_ctx *ctx, uint32_t lane, size_t bytes)
+{
+    Recorder r;
+    r.lane = lane;
+    Recorder *saved = ctx->rec;
+    ctx->rec = &r;
+    try {
+        (void)scratch(ctx, bytes);
+    } catch (...) {
+        ctx->rec = saved;
+        throw;
+    }
+    ctx->rec = saved;
+}
+
 // ------------------------------------------------------ resident programs
 
 namespace {
@@ -102,6 +117,16 @@ void copy_async(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes)
     const CopyArgs a{dst, src, bytes};
     std::memcpy(s.blob, &a, sizeof a);
     ctx->rec->steps.push_back(s);
+}
+
+void to_host_async(vo_ctx *ctx, void *pinned_dst, const void *src, uint64_t bytes)
+{
+    if (bytes == 0) return;
+    if (!ctx->rec) {
+        VO_HIP(hipMemcpyAsync(pinned_dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        return;
+    }
+    copy_async(ctx, pinned_dst, src, bytes);
 }
 
 void upload_async(vo_ctx *ctx, void *dst, const void *host_src, uint64_t bytes)

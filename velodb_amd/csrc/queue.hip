@@ -31,6 +31,7 @@
 #include "misc_kernels.hpp"
 #include "predicate_kernels.hpp"
 #include "resident_tables.hpp"
+#include "serve_kernels.hpp"
 #include "shuffle_kernels.hpp"
 #include "sort_kernels.hpp"
 #include "sort_multi_kernels.hpp"
@@ -717,6 +718,15 @@ uint64_t queue_submit_lane(vo_ctx *ctx, vo_command *cmd, uint32_t lane)
 }
 
 uint64_t queue_submit(vo_ctx *ctx, vo_command *cmd) { return queue_submit_lane(ctx, cmd, 0); }
+
+// Publishes a program recorded by the caller (several operators as one
+// entry); caller holds ctx->mu.
+uint64_t queue_publish_locked(vo_ctx *ctx, uint32_t lane, Recorder &rec)
+{
+    QueueState *q = (QueueState *)ctx->queue;
+    VO_CHECK(lane < q->lanes, VO_ERR_INVALID, "lane " + std::to_string(lane) + " of " + std::to_string(q->lanes));
+    return publish_locked(ctx, q, lane, kOpProgram, &rec);
+}
 
 int queue_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms)
 {

@@ -185,6 +185,8 @@ inline void launch(vo_ctx *ctx, const LaunchInfo &li, K kernel, Args... args)
 void fill_async(vo_ctx *ctx, void *dst, int value, uint64_t bytes);
 void copy_async(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes); // device to device
 void upload_async(vo_ctx *ctx, void *dst, const void *host_src, uint64_t bytes); // host to device
+// device to pinned (host-mapped) memory; while recording a copy step writes it
+void to_host_async(vo_ctx *ctx, void *pinned_dst, const void *src, uint64_t bytes);
 
 // The recorded program of one command (persistent mode): runs the command's
 // host driver with a recorder attached.

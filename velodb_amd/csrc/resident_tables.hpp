@@ -208,10 +208,15 @@
     X(kB_og_order, (og_k::k_og_order<false>), (og_k::b_og_order<false>(a..., vb, nvb)), 0)                         \
     X(kB_og_out, (og_k::k_og_out<false>), (og_k::b_og_out<false>(a..., vb, nvb)), 0)
 
+// ------------------------------------------------------------ serving loop
+#define VO_SERVE_BODIES(X)                                                                                         \
+    X(kB_sort_checksum, (serve_k::k_sort_checksum), (serve_k::b_sort_checksum(a..., (uint64_t *)lds, vb, nvb)),   \
+      512 * 8)
+
 // --------------------------------------------------------------- all tables
 #define VO_ALL_BODIES(X)                                                                                           \
     VO_MISC_BODIES(X) VO_SORT_BODIES(X) VO_COMPACT_BODIES(X) VO_PRED_BODIES(X) VO_SHUFFLE_BODIES(X)                \
-    VO_JOIN_BODIES(X) VO_JP_BODIES(X) VO_OG_BODIES(X)
+    VO_JOIN_BODIES(X) VO_JP_BODIES(X) VO_OG_BODIES(X) VO_SERVE_BODIES(X)
 
 namespace vo {
 

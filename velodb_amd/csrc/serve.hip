@@ -14,7 +14,8 @@
 //   * completion frees the slot and the footprint; latency = end - arrival.
 // n_slots = 1 is the plain one-query-at-a-time loop.  Everything the
 // schedule looks at (arrival times, table sizes) is public.
-#include "vo_internal.hpp"
+#include "resident_tables.hpp"
+#include "serve_kernels.hpp"
 
 #include <map>
 
@@ -29,6 +30,11 @@ struct vo_server;
 
 namespace vo {
 namespace {
+
+using namespace serve_k;
+
+const std::pair<const void *, BodyInfo> kBodies[] = {VO_SERVE_BODIES(VO_BODY_ENTRY)};
+const bool kRegistered = (register_bodies(kBodies, sizeof kBodies / sizeof kBodies[0]), true);
 
 // Every table also keeps a pinned host copy: the transfer-mode run ships it
 // to the device per query, as the reference's engine does.
@@ -94,33 +100,7 @@ __global__ void k_iota1(uint32_t *__restrict__ p, uint64_t n)
         p[i] = (uint32_t)(i + 1);
 }
 
-// Order-sensitive checksums of a sorted table: sum key[i]*(2i+1), sum pay[i]*(i+1) mod 2^64.
-__global__ __launch_bounds__(256) void k_sort_checksum(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ pay,
-                                                       uint64_t n, uint64_t *__restrict__ partial)
-{
-    __shared__ uint64_t a[256], b[256];
-    uint64_t x = 0, y = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-        x += keys[i] * (2 * i + 1);
-        y += (uint64_t)pay[i] * (i + 1);
-    }
-    a[threadIdx.x] = x;
-    b[threadIdx.x] = y;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < (uint32_t)off) {
-            a[threadIdx.x] += a[threadIdx.x + off];
-            b[threadIdx.x] += b[threadIdx.x + off];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = a[0];
-        partial[2 * blockIdx.x + 1] = b[0];
-    }
-}
 
-constexpr uint32_t kChecksumBlocks = 256;
 constexpr uint64_t kMaxGroups = 64;
 constexpr uint64_t kJoinGroups = 32; // public bound of the generated group attribute (0..24)
 // pinned result words of one query: filter 32 B, join groups, sort partial sums
@@ -177,6 +157,10 @@ struct vo_server {
         vo::OutLoc loc; // where a replay leaves the query's whole result
     };
     std::vector<std::map<uint64_t, Graph>> graphs;
+    // queue mode: one persistent context whose resident kernel runs a lane
+    // per slot (slot i on lane i mod lanes)
+    vo_ctx *qctx = nullptr;
+    uint32_t q_lanes = 0;
 };
 
 namespace {
@@ -307,7 +291,7 @@ void enqueue_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_qu
         a.disc_hi = 7;
         a.qty_lt = 24;
         op_filter_sum(ctx, a);
-        VO_HIP(hipMemcpyAsync(res, a.out, 32, hipMemcpyDeviceToHost, ctx->stream));
+        to_host_async(ctx, res, a.out, 32);
         // without output columns the network compacts the private copies in place
         l.c0 = transfer ? (const void *)a.price : (const void *)a.out_price;
         l.c1 = transfer ? (const void *)a.disc : (const void *)a.out_disc;
@@ -347,7 +331,7 @@ void enqueue_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_qu
         a.max_groups = kJoinGroups;
         a.dense_groups = 1;
         op_join_groupsum(ctx, a);
-        VO_HIP(hipMemcpyAsync(res, o, kMaxGroups * 12 + 8, hipMemcpyDeviceToHost, ctx->stream));
+        to_host_async(ctx, res, o, kMaxGroups * 12 + 8);
         break;
     }
     case VO_Q_SORT: {
@@ -360,8 +344,8 @@ void enqueue_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_qu
             h2d(ctx, p, t.h_pay, 4 * t.n);
         } else {
             // the resident table stays unsorted: the query sorts its own copy
-            VO_HIP(hipMemcpyAsync(k, t.keys, 8 * t.n_pad, hipMemcpyDeviceToDevice, ctx->stream));
-            VO_HIP(hipMemcpyAsync(p, t.pay, 4 * t.n_pad, hipMemcpyDeviceToDevice, ctx->stream));
+            copy_async(ctx, k, t.keys, 8 * t.n_pad);
+            copy_async(ctx, p, t.pay, 4 * t.n_pad);
         }
         op_sort_kp(ctx, k, p, t.n, t.n_pad);
         launch(ctx, LaunchInfo{"sort_checksum", dim3(kChecksumBlocks), dim3(256), 0, t.n_pad}, k_sort_checksum,
@@ -371,7 +355,7 @@ void enqueue_query(vo_server *s, vo_ctx *ctx, void *buf, void *hbuf, const vo_qu
             VO_HIP(hipMemcpyAsync(hb, k, 8 * t.n, hipMemcpyDeviceToHost, ctx->stream));
             VO_HIP(hipMemcpyAsync(hb + 8 * t.n, p, 4 * t.n, hipMemcpyDeviceToHost, ctx->stream));
         }
-        VO_HIP(hipMemcpyAsync(res, part, 2 * kChecksumBlocks * 8, hipMemcpyDeviceToHost, ctx->stream));
+        to_host_async(ctx, res, part, 2 * kChecksumBlocks * 8);
         l.c0 = k;
         l.c1 = p;
         l.b0 = 8 * t.n;
@@ -488,6 +472,60 @@ bool capture_query(vo_server *s, vo_ctx *ctx, void *buf, void *res, const vo_que
     return ok;
 }
 
+// Queue mode: the query's operators are recorded as one program and run by
+// the resident kernel on the slot's lane, beside the other lanes' queries
+// (blocking until this query's own completion flag is set).
+void run_query_queue(vo_server *s, uint32_t slot, uint32_t lane, const vo_query &q, uint64_t r[4])
+{
+    vo_ctx *ctx = s->qctx;
+    void *buf = s->slot_buf[slot], *res = s->slot_res[slot];
+    OutLoc loc;
+    uint64_t seq;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        Recorder rec;
+        rec.lane = lane;
+        ctx->rec = &rec;
+        try {
+            enqueue_query(s, ctx, buf, nullptr, q, false, res, &loc);
+        } catch (...) {
+            ctx->rec = nullptr;
+            throw;
+        }
+        ctx->rec = nullptr;
+        seq = queue_publish_locked(ctx, lane, rec);
+    }
+    const int st = queue_wait(ctx, seq, 0);
+    if (st != VO_OK) throw Error{st, ctx->last_error};
+    finish_query(s, q, res, r);
+    copy_out(s, ctx, q, res, loc);
+}
+
+// The queue-mode context with `lanes` lanes, its resident kernel started.
+void queue_ctx(vo_server *s, uint32_t lanes)
+{
+    if (s->qctx && s->q_lanes == lanes) return;
+    if (s->qctx) vo_ctx_destroy(s->qctx);
+    s->qctx = nullptr;
+    s->q_lanes = 0;
+    vo_config c{};
+    c.device = s->device;
+    c.flags = VO_CFG_PERSISTENT;
+    c.queue_capacity = 1024;
+    c.queue_lanes = lanes;
+    vo_ctx *ctx = nullptr;
+    int st = vo_ctx_create(&c, &ctx);
+    if (st != VO_OK) throw Error{st, "server queue context creation failed"};
+    st = vo_queue_start(ctx);
+    if (st != VO_OK) {
+        const std::string m = vo_last_error(ctx);
+        vo_ctx_destroy(ctx);
+        throw Error{st, "server queue: " + m};
+    }
+    s->qctx = ctx;
+    s->q_lanes = lanes;
+}
+
 // Runs one query on a slot and fills its result words (blocking).  Over the
 // resident tables, a (kind, table) pair seen before on this slot replays the
 // graph captured after its first run: one launch instead of one per kernel.
@@ -555,6 +593,7 @@ int vo_server_destroy(vo_server *s)
 {
     if (!s) return VO_ERR_INVALID;
     (void)hipSetDevice(s->device);
+    if (s->qctx) vo_ctx_destroy(s->qctx);
     for (auto &cache : s->graphs)
         for (auto &kv : cache)
             if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
@@ -708,9 +747,12 @@ int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_s
 {
     return sguard(s, [&] {
         VO_CHECK(q && out, VO_ERR_INVALID, "null query list");
-        VO_CHECK(mode == VO_RUN_RESIDENT || mode == VO_RUN_TRANSFER, VO_ERR_INVALID, "unknown run mode");
+        VO_CHECK(mode == VO_RUN_RESIDENT || mode == VO_RUN_TRANSFER || mode == VO_RUN_QUEUE, VO_ERR_INVALID,
+                 "unknown run mode");
         const bool transfer = mode == VO_RUN_TRANSFER;
+        const bool queued = mode == VO_RUN_QUEUE;
         const uint32_t slots = n_slots_override ? n_slots_override : s->n_slots;
+        const uint32_t lanes = std::min<uint32_t>(slots, 8);
         // size every slot for the largest query it may take (no allocation while timing)
         std::vector<uint64_t> fp(n);
         uint64_t max_buf = 64, max_scr = 1 << 20, max_hbuf = 0;
@@ -731,13 +773,18 @@ int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_s
                 VO_HIP(hipMalloc(&s->slot_buf[i], max_buf));
                 s->slot_buf_bytes[i] = max_buf;
             }
-            scratch(ctx, max_scr);
+            if (!queued) scratch(ctx, max_scr);
             if (s->slot_hbuf_bytes[i] < max_hbuf) {
                 if (s->slot_hbuf[i]) VO_HIP(hipHostFree(s->slot_hbuf[i]));
                 s->slot_hbuf[i] = nullptr;
                 VO_HIP(hipHostMalloc(&s->slot_hbuf[i], max_hbuf, hipHostMallocDefault));
                 s->slot_hbuf_bytes[i] = max_hbuf;
             }
+        }
+        if (queued) {
+            queue_ctx(s, lanes);
+            std::lock_guard<std::mutex> lk(s->qctx->mu);
+            for (uint32_t l = 0; l < lanes; ++l) reserve_lane_scratch(s->qctx, l, max_scr);
         }
         VO_HIP(hipDeviceSynchronize());
 
@@ -790,7 +837,10 @@ int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_s
                 r.footprint_bytes = fp[i];
                 r.slot = slot;
                 try {
-                    if (run_query(s, slot, q[i], transfer, r.r)) replays.fetch_add(1);
+                    if (queued)
+                        run_query_queue(s, slot, slot % lanes, q[i], r.r);
+                    else if (run_query(s, slot, q[i], transfer, r.r))
+                        replays.fetch_add(1);
                     r.status = VO_OK;
                 } catch (const Error &e) {
                     r.status = e.status;

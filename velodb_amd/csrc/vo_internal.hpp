@@ -224,6 +224,12 @@ uint64_t queue_submit_lane(vo_ctx *ctx, vo_command *cmd, uint32_t lane);
 void queue_drain_locked(vo_ctx *ctx); // every submitted command complete (caller holds ctx->mu)
 uint32_t queue_lanes(vo_ctx *ctx);
 uint64_t queue_next_seq(vo_ctx *ctx);                  // id of the command being recorded
+// A recorded program of several operators as one ring entry (internal opcode
+// kOpProgram); caller holds ctx->mu.
+constexpr uint32_t kOpProgram = 0x1000;
+uint64_t queue_publish_locked(vo_ctx *ctx, uint32_t lane, Recorder &rec);
+// Sizes a queue lane's scratch arena ahead of its first command.
+void reserve_lane_scratch(vo_ctx *ctx, uint32_t lane, size_t bytes);
 void queue_wait_seq_locked(vo_ctx *ctx, uint64_t seq); // caller holds ctx->mu
 
 // ---- shared device helpers (sort.hip) used by other ops
