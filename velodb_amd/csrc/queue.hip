@@ -467,12 +467,13 @@ __global__ __launch_bounds__(RB, 2) void k_queue(QArgs qa)
             reason = ex;
             break;
         }
+        const uint32_t ns = __builtin_amdgcn_readfirstlane(ld->n_steps);
         // inputs may have been written by the host or a copy engine since
-        // this CU last looked: drop cached lines (every XCD has blocks here)
-        if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+        // this CU last looked: drop cached lines (every XCD has blocks here);
+        // a command without steps reads nothing
+        if (ns && threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
         __syncthreads();
         const uint32_t opcode = __builtin_amdgcn_readfirstlane(ld->opcode);
-        const uint32_t ns = __builtin_amdgcn_readfirstlane(ld->n_steps);
         const uint64_t seq = ld->seq;
         ++consumed;
         done_seq = seq;
@@ -498,7 +499,7 @@ __global__ __launch_bounds__(RB, 2) void k_queue(QArgs qa)
         }
         // results out of every XCD's L2 before the leader tells the host
         __syncthreads();
-        if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        if (ns && threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
     }
     if (lp.rank == 0 && threadIdx.x == 0) {
         ld->consumed = consumed;
@@ -627,12 +628,14 @@ template <class P>
 void wait_locked(vo_ctx *ctx, QueueState *q, P pred, const char *what)
 {
     const auto t0 = std::chrono::steady_clock::now();
-    int spins = 0;
     while (!pred()) {
         ensure_running_locked(ctx, q);
-        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(ctx->wait_timeout_ms))
-            throw Error{VO_ERR_TIMEOUT, what};
-        if (++spins > 2000) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        const auto el = std::chrono::steady_clock::now() - t0;
+        if (el > std::chrono::milliseconds(ctx->wait_timeout_ms)) throw Error{VO_ERR_TIMEOUT, what};
+        if (el > std::chrono::milliseconds(2))
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        else if (el > std::chrono::microseconds(50))
+            std::this_thread::yield();
     }
 }
 
@@ -734,16 +737,22 @@ int queue_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms)
     try {
         if (timeout_ms == 0) timeout_ms = ctx->wait_timeout_ms;
         VO_CHECK(seq >= 1 && seq <= q->submitted, VO_ERR_INVALID, "wait on a sequence id never submitted");
+        // Spin, then yield, then sleep: a sleep lasts the timer slack (tens of
+        // microseconds) at least, longer than most resident commands.
         const auto t0 = std::chrono::steady_clock::now();
-        int spins = 0;
-        while (!seq_done(q, seq)) {
+        for (uint64_t spins = 0; !seq_done(q, seq); ++spins) {
             if (vctrl(q)->alive == 0 || vctrl(q)->error) {
                 std::lock_guard<std::mutex> lk(ctx->mu);
                 ensure_running_locked(ctx, q);
             }
-            if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(timeout_ms))
+            if ((spins & 63) != 63) continue;
+            const auto el = std::chrono::steady_clock::now() - t0;
+            if (el > std::chrono::milliseconds(timeout_ms))
                 throw Error{VO_ERR_TIMEOUT, "vo_wait: command " + std::to_string(seq) + " not finished"};
-            if (++spins > 2000) std::this_thread::sleep_for(std::chrono::microseconds(20));
+            if (el > std::chrono::milliseconds(2))
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            else if (el > std::chrono::microseconds(50))
+                std::this_thread::yield();
         }
         return VO_OK;
     } catch (const Error &e) {
