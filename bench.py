@@ -710,6 +710,8 @@ def bench_batch(args):
             "kernel": "every sort pass of one batched call (local, merge, global, hybrid)",
             "kernel_ms_per_call": round(k_ms, 3), "alg_bytes_per_call": alg,
             "passes_by_class": {str(p): plan[p] for p in sorted(plan)}, "tuning": "default"}
+    resident = _batch_resident(args, ctx, keys, pay, total, items, kb, pb, tp)
+    compact = _compact_batch(args)
     _line("batched oblivious sorts (Mrec/s over padded records)", padded / tb / 1e6, "Mrec/s", args.steps,
           args.warmup, tb * 1e3, "u64",
           {"workload": "4096 sorts, sizes log-uniform in [2^10, 2^20] (numpy default_rng(0)), u64 keys + u32 "
@@ -717,8 +719,148 @@ def bench_batch(args):
            "instances": len(items), "raw_records": int(sizes.sum()), "padded_records": padded,
            "autotuned": bool(args.autotune)},
           per_instance_ms=round(tp * 1e3, 2), speedup_vs_per_instance=round(tp / tb, 2),
-          batched_equals_per_instance=same, roofline=roof)
+          batched_equals_per_instance=same, roofline=roof, resident_one_launch=resident, compact_batch=compact)
     ctx.close()
+
+
+def _batch_resident(args, ctx, keys, pay, total, items, kb, pb, tp):
+    """The same batch as one command of the resident queue kernel (one
+    persistent launch; its size classes share the grid between barriers):
+    submit -> completion, inputs generated before each run."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    import velodb_amd
+    from velodb_amd import _abi
+    rctx = velodb_amd.Context(flags=_abi.CFG_PERSISTENT, queue_capacity=64)
+    arr = (_abi.BatchItem * len(items))(*[_abi.BatchItem(o, n, p) for (o, n, p) in items])
+    cmd = _abi.Command()
+    cmd.opcode = _abi.OP_SORT_BATCH
+    cmd.args.sort_batch = _abi.SortBatchArgs(keys.data_ptr(), pay.data_ptr(), C.cast(arr, C.c_void_p), len(items))
+    ts = []
+    for _ in range(max(1, args.warmup) + args.steps):
+        ctx.gen_sort_input(7, keys, pay, total)
+        torch.cuda.synchronize()  # before the resident kernel holds the CUs
+        rctx.queue_start()
+        t0 = time.perf_counter()
+        rctx.wait(rctx.submit(cmd), 120000)
+        ts.append(time.perf_counter() - t0)
+        rctx.queue_stop()
+    torch.cuda.synchronize()
+    tr = float(np.median(ts[max(1, args.warmup):]))
+    same = bool(torch.equal(kb, keys) and torch.equal(pb, pay))
+    rctx.close()
+    return {"ms": round(tr * 1e3, 2), "Mrec_s": round(sum(p for _, _, p in items) / tr / 1e6, 1),
+            "speedup_vs_per_instance": round(tp / tr, 2), "equals_launch_mode_batch": same,
+            "note": "one OP_SORT_BATCH command of the resident queue (512 blocks of 256 threads, 2^12-record "
+                    "tiles); submit to completion, including the host recording of its program"}
+
+
+def _compact_batch(args):
+    """1024 independent ORCompact compactions, sizes log-uniform in
+    [2^12, 2^20] (numpy default_rng(1)), two columns (u64, u32) and a u8 mask
+    (30% selected, torch generator seed 5): one OP_COMPACT_BATCH call vs one
+    vo_compact per item, and the same batch as one resident command."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    import velodb_amd
+    from velodb_amd import _abi
+    rng = np.random.default_rng(1)
+    sizes = np.exp(rng.uniform(np.log(2 ** 12), np.log(2 ** 20), 1024)).astype(np.int64)
+    items, off = [], 0
+    for n in sizes:
+        p = 1 << int(np.ceil(np.log2(max(n, 4096))))
+        off = (off + p - 1) // p * p
+        items.append((off, int(n), p))
+        off += p
+    total = off
+    padded = sum(p for _, _, p in items)
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(5)
+    c0_in = torch.randint(-2**62, 2**62, (total,), device=dev, generator=g)
+    c1_in = torch.randint(-2**31, 2**31 - 1, (total,), device=dev, dtype=torch.int32, generator=g)
+    mask = (torch.rand(total, device=dev, generator=g) < 0.3).to(torch.uint8)
+    c0, c1 = c0_in.clone(), c1_in.clone()
+    counts = torch.zeros(len(items), dtype=torch.int64, device=dev)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream)
+
+    def reset():
+        c0.copy_(c0_in)
+        c1.copy_(c1_in)
+        torch.cuda.synchronize()
+
+    def timed(fn, reps):
+        ts = []
+        for _ in range(reps):
+            reset()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    a, arr = velodb_amd.Context.compact_batch_args(mask, [c0, c1], [8, 4], items, counts)
+    batched = lambda: ctx.check(ctx.lib.vo_compact_batch(ctx.h, C.byref(a)))  # noqa: E731
+    timed(batched, max(1, args.warmup))
+    tb = timed(batched, args.steps)
+    r0, r1, rc = c0.clone(), c1.clone(), counts.clone()
+    one = torch.zeros(len(items), dtype=torch.int64, device=dev)
+    cmds = []
+    for k, (o, n, p) in enumerate(items):
+        cmd = _abi.Command()
+        cmd.opcode = _abi.OP_COMPACT
+        ca = cmd.args.compact
+        ca.mask = mask.data_ptr() + o
+        ca.cols[0], ca.cols[1] = c0.data_ptr() + 8 * o, c1.data_ptr() + 4 * o
+        ca.col_bytes[0], ca.col_bytes[1] = 8, 4
+        ca.n_cols, ca.n, ca.n_pad = 2, n, p
+        ca.out_count = one.data_ptr() + 8 * k
+        cmds.append(cmd)
+
+    def per_item():
+        for cmd in cmds:
+            ctx.submit(cmd)
+        ctx.sync()
+
+    tp = timed(per_item, max(1, min(args.steps, 3)))
+    same = bool(torch.equal(r0, c0) and torch.equal(r1, c1) and torch.equal(rc, one))
+    # the same batch as one resident command
+    rctx = velodb_amd.Context(flags=_abi.CFG_PERSISTENT, queue_capacity=64)
+    rcmd = _abi.Command()
+    rcmd.opcode = _abi.OP_COMPACT_BATCH
+    rcmd.args.compact_batch = a
+    ts = []
+    for _ in range(max(1, args.warmup) + args.steps):
+        reset()
+        rctx.queue_start()
+        t0 = time.perf_counter()
+        rctx.wait(rctx.submit(rcmd), 120000)
+        ts.append(time.perf_counter() - t0)
+        rctx.queue_stop()
+    torch.cuda.synchronize()
+    tr = float(np.median(ts[max(1, args.warmup):]))
+    same_r = bool(torch.equal(r0, c0) and torch.equal(r1, c1) and torch.equal(rc, counts))
+    rctx.close()
+    ctx.close()
+    # algorithmic bytes: the count pass reads the mask; the local pass reads
+    # the mask and reads + writes both columns; every further pass of the
+    # network (levels 12 .. log2 p - 1 in groups of at most 8) reads + writes
+    # both columns: 12 B per row each way
+    def passes(p):
+        lv = int(np.log2(p)) - 12
+        return 1 + (lv + 7) // 8
+    alg = sum(p * (1 + 1 + 24 * passes(p)) for _, _, p in items)
+    return {"workload": "1024 compactions, sizes log-uniform in [2^12, 2^20] (numpy default_rng(1)), columns "
+                        "u64 + u32, u8 mask with 30% selected",
+            "padded_rows": padded, "ms": round(tb * 1e3, 2), "Mrows_s": round(padded / tb / 1e6, 1),
+            "alg_GBps": round(alg / tb / 1e9, 1), "per_item_ms": round(tp * 1e3, 2),
+            "speedup_vs_per_item": round(tp / tb, 2), "batched_equals_per_item": same,
+            "resident_one_launch_ms": round(tr * 1e3, 2), "resident_equals_launch_mode": same_r}
 
 
 def bench_queue(args):

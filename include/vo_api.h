@@ -93,6 +93,7 @@ typedef enum vo_opcode {
     VO_OP_SORT_BATCH = 0x105,           /* many independent (u64,u32) sorts */
     VO_OP_GATHER_OBLIVIOUS = 0x106,     /* table values by record id, no data-dependent address */
     VO_OP_PREDICATE_MASK = 0x107,       /* WHERE terms (column vs constant, AND / OR) -> u8 mask */
+    VO_OP_COMPACT_BATCH = 0x108,        /* many independent ORCompact compactions */
     VO_OP_TERMINATE = 0xFFFFFFFFu
 } vo_opcode;
 
@@ -389,6 +390,24 @@ typedef struct vo_sort_batch_args {
     uint64_t n_items;
 } vo_sort_batch_args;
 
+/* OP_COMPACT_BATCH: independent compactions (OP_COMPACT's network) in one
+ * schedule.  Item i compacts rows [offset, offset + n_pad) of every column
+ * by the same rows of `mask`: the rows r < n with mask[offset + r] != 0 move,
+ * in order, to the front of the item's rows, as vo_compact on that item
+ * alone would leave them, and out_counts[i] receives their count.  n_pad is
+ * a power of two in [4096, 2^32] and offset a multiple of n_pad.  Items are
+ * bucketed by n_pad: one launch per pass serves every item of a class.  The
+ * virtual rows of one class (items x n_pad) are at most 2^32. */
+typedef struct vo_compact_batch_args {
+    const uint8_t *mask;                      /* device */
+    void *cols[VO_MAX_SORT_COLUMNS];          /* device */
+    uint32_t col_bytes[VO_MAX_SORT_COLUMNS];  /* 1, 2, 4 or 8 */
+    uint32_t n_cols;
+    const vo_batch_item *items;               /* host array */
+    uint64_t n_items;
+    uint64_t *out_counts;                     /* device, n_items (may be NULL) */
+} vo_compact_batch_args;
+
 typedef union vo_command_args {
     vo_sort_args sort;
     vo_sort_kp_args sort_kp;
@@ -403,6 +422,7 @@ typedef union vo_command_args {
     vo_filter_args filter;
     vo_join_args join;
     vo_sort_batch_args sort_batch;
+    vo_compact_batch_args compact_batch;
     vo_join_pairs_args join_pairs; /* OP_SORT_MERGE_JOIN_COUNT / _WRITE */
     vo_gather_oblivious_args gather_oblivious;
     vo_predicate_args predicate;   /* OP_PREDICATE_MASK */
@@ -489,6 +509,10 @@ VO_EXPORT int vo_shuffle(vo_ctx *ctx, const vo_shuffle_args *args);
 VO_EXPORT int vo_filter_sum(vo_ctx *ctx, const vo_filter_args *args);
 VO_EXPORT int vo_pkfk_join_groupsum(vo_ctx *ctx, const vo_join_args *args);
 VO_EXPORT int vo_sort_batch(vo_ctx *ctx, const vo_sort_batch_args *args);
+/* OP_COMPACT_BATCH, synchronous.  No reference counterpart: the batched
+ * form of ORCompact (obliviator/join/enclave/oblivious_compact.c:59-338)
+ * for thousands of compactions of uneven public sizes. */
+VO_EXPORT int vo_compact_batch(vo_ctx *ctx, const vo_compact_batch_args *args);
 /* OP_SORT_MERGE_JOIN_COUNT then OP_SORT_MERGE_JOIN_WRITE; *n_out (host, may
  * be NULL) receives the pair count after COUNT. */
 VO_EXPORT int vo_join_pairs_count(vo_ctx *ctx, const vo_join_pairs_args *args, uint64_t *n_out);

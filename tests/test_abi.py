@@ -42,7 +42,8 @@ STRUCTS = ["vo_sort_args", "vo_sort_kp_args", "vo_permute_args", "vo_scatter_arg
            "vo_compare_eq_args", "vo_compose_args", "vo_enrich_args", "vo_compact_args", "vo_shuffle_args",
            "vo_filter_args", "vo_join_args", "vo_batch_item", "vo_sort_batch_args", "vo_command", "vo_config",
            "vo_trace", "vo_predicate_term", "vo_predicate_args", "vo_server_config", "vo_query",
-           "vo_query_result", "vo_run_stats"]
+           "vo_query_result", "vo_run_stats", "vo_compact_batch_args", "vo_join_pairs_args",
+           "vo_gather_oblivious_args"]
 PY = {"vo_sort_args": _abi.SortArgs, "vo_sort_kp_args": _abi.SortKpArgs, "vo_permute_args": _abi.PermuteArgs,
       "vo_scatter_args": _abi.ScatterArgs, "vo_gather_args": _abi.GatherArgs,
       "vo_compare_eq_args": _abi.CompareEqArgs, "vo_compose_args": _abi.ComposeArgs,
@@ -51,7 +52,8 @@ PY = {"vo_sort_args": _abi.SortArgs, "vo_sort_kp_args": _abi.SortKpArgs, "vo_per
       "vo_sort_batch_args": _abi.SortBatchArgs, "vo_command": _abi.Command, "vo_config": _abi.Config,
       "vo_trace": _abi.Trace, "vo_predicate_term": _abi.PredicateTerm, "vo_predicate_args": _abi.PredicateArgs,
       "vo_server_config": _abi.ServerConfig, "vo_query": _abi.Query, "vo_query_result": _abi.QueryResult,
-      "vo_run_stats": _abi.RunStats}
+      "vo_run_stats": _abi.RunStats, "vo_compact_batch_args": _abi.CompactBatchArgs,
+      "vo_join_pairs_args": _abi.JoinPairsArgs, "vo_gather_oblivious_args": _abi.GatherObliviousArgs}
 
 
 def test_struct_layout_matches_c():

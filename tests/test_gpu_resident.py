@@ -132,6 +132,44 @@ def test_resident_sort_batch_equals_launch_mode(rctx, vctx):
     assert torch.equal(lk, rk) and torch.equal(lp, rp)
 
 
+def test_resident_compact_batch_equals_launch_mode(rctx, vctx):
+    """OP_COMPACT_BATCH as one resident program (its size classes share the
+    lane between barriers) gives the launch-mode bytes and counts."""
+    from velodb_amd import Context, _abi
+    rng = np.random.default_rng(13)
+    sizes = [0, 4096, 5000, 30000, 70000, 1 << 17] + list(rng.integers(10, 40000, 40))
+    items, off = [], 0
+    for n in sizes:
+        p = 4096
+        while p < n:
+            p <<= 1
+        off = (off + p - 1) // p * p
+        items.append((off, int(n), p))
+        off += p
+    c0 = rng.integers(0, 2**63, off, dtype=np.uint64)
+    c1 = rng.integers(0, 2**16, off, dtype=np.uint64).astype(np.uint16)
+    mask = (rng.random(off) < 0.3).astype(np.uint8)
+
+    def make():
+        return {"c0": to_dev(c0), "c1": to_dev(c1), "m": to_dev(mask),
+                "cnt": to_dev(np.zeros(len(items), np.uint64))}
+
+    def run(ctx, b):
+        if ctx is vctx:
+            ctx.compact_batch(b["m"], [b["c0"], b["c1"]], [8, 2], items, b["cnt"])
+            return
+        a, _arr = Context.compact_batch_args(b["m"], [b["c0"], b["c1"]], [8, 2], items, b["cnt"])
+        cmd = _abi.Command()
+        cmd.opcode = _abi.OP_COMPACT_BATCH
+        cmd.args.compact_batch = a
+        _run(ctx, cmd)
+
+    d = _both(vctx, rctx, make, run)
+    cnt = _host(d["cnt"], np.uint64)
+    for k, (o, n, p) in enumerate(items):
+        assert int(cnt[k]) == int(mask[o:o + min(n, p)].sum()), k
+
+
 # -------------------------------------------------------------------- lanes
 
 def test_lanes_run_commands_side_by_side(torch_dev):

@@ -319,6 +319,26 @@ class Context:
         a = _abi.SortBatchArgs(_ptr(keys), _ptr(payload), C.cast(arr, C.c_void_p), len(items))
         return self.check(self.lib.vo_sort_batch(self.h, C.byref(a)))
 
+    @staticmethod
+    def compact_batch_args(mask, cols, col_bytes, items, out_counts=None):
+        """vo_compact_batch_args for items [(offset, n, n_pad), ...]; the
+        returned tuple keeps the host item array alive with the struct."""
+        arr = (_abi.BatchItem * len(items))(*[_abi.BatchItem(o, n, p) for (o, n, p) in items])
+        a = _abi.CompactBatchArgs()
+        a.mask = _ptr(mask)
+        for i, c in enumerate(cols):
+            a.cols[i] = _ptr(c)
+            a.col_bytes[i] = col_bytes[i]
+        a.n_cols = len(cols)
+        a.items = C.cast(arr, C.c_void_p)
+        a.n_items = len(items)
+        a.out_counts = _ptr(out_counts)
+        return a, arr
+
+    def compact_batch(self, mask, cols, col_bytes, items, out_counts=None):
+        a, _arr = self.compact_batch_args(mask, cols, col_bytes, items, out_counts)
+        return self.check(self.lib.vo_compact_batch(self.h, C.byref(a)))
+
     def queue_start(self):
         return self.check(self.lib.vo_queue_start(self.h))
 

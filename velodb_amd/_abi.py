@@ -40,6 +40,7 @@ OP_PKFK_JOIN_GROUPSUM = 0x104
 OP_SORT_BATCH = 0x105
 OP_GATHER_OBLIVIOUS = 0x106
 OP_PREDICATE_MASK = 0x107
+OP_COMPACT_BATCH = 0x108
 OP_TERMINATE = 0xFFFFFFFF
 
 CFG_PERSISTENT = 1 << 0
@@ -113,6 +114,11 @@ class SortBatchArgs(C.Structure):
     _fields_ = [("keys", P), ("payload", P), ("items", P), ("n_items", u64)]
 
 
+class CompactBatchArgs(C.Structure):
+    _fields_ = [("mask", P), ("cols", P * 8), ("col_bytes", u32 * 8), ("n_cols", u32), ("items", P),
+                ("n_items", u64), ("out_counts", P)]
+
+
 class JoinPairsArgs(C.Structure):
     _fields_ = [("left_keys", P), ("n_left", u64), ("right_keys", P), ("n_right", u64), ("type_id", u32),
                 ("reserved", u32), ("out_count", P), ("out_left", P), ("out_right", P), ("out_cap", u64),
@@ -156,7 +162,7 @@ class CommandArgs(C.Union):
                 ("enrich_positions", EnrichArgs), ("compact", CompactArgs), ("shuffle", ShuffleArgs),
                 ("filter", FilterArgs), ("join", JoinArgs), ("sort_batch", SortBatchArgs),
                 ("join_pairs", JoinPairsArgs), ("gather_oblivious", GatherObliviousArgs),
-                ("predicate", PredicateArgs)]
+                ("predicate", PredicateArgs), ("compact_batch", CompactBatchArgs)]
 
 
 class Command(C.Structure):
@@ -221,6 +227,7 @@ PROTOTYPES = [
     ("vo_filter_sum", C.c_int, [P, C.POINTER(FilterArgs)]),
     ("vo_pkfk_join_groupsum", C.c_int, [P, C.POINTER(JoinArgs)]),
     ("vo_sort_batch", C.c_int, [P, C.POINTER(SortBatchArgs)]),
+    ("vo_compact_batch", C.c_int, [P, C.POINTER(CompactBatchArgs)]),
     ("vo_join_pairs_count", C.c_int, [P, C.POINTER(JoinPairsArgs), C.POINTER(u64)]),
     ("vo_transfer_to_device", C.c_int, [P, P, P, u64, u64, C.POINTER(u64)]),
     ("vo_gather_oblivious", C.c_int, [P, C.POINTER(GatherObliviousArgs)]),

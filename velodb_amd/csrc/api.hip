@@ -159,6 +159,62 @@ void dispatch(vo_ctx *ctx, const vo_command &cmd);
 
 namespace vo {
 
+void run_tracks(vo_ctx *ctx, size_t n, const std::function<void(size_t)> &f)
+{
+    if (!ctx->rec) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    Recorder *const parent = ctx->rec;
+    std::vector<Recorder> tr(n);
+    try {
+        for (size_t i = 0; i < n; ++i) {
+            tr[i].lane = parent->lane;
+            ctx->rec = &tr[i];
+            f(i);
+        }
+    } catch (...) {
+        ctx->rec = parent;
+        throw;
+    }
+    ctx->rec = parent;
+    // each track's steps (a step and the slots its arguments run on over)
+    std::vector<std::vector<size_t>> starts(n);
+    size_t phases = 0;
+    for (size_t i = 0; i < n; ++i) {
+        for (size_t s = 0; s < tr[i].steps.size(); s += std::max<uint16_t>(1, tr[i].steps[s].slots))
+            starts[i].push_back(s);
+        phases = std::max(phases, starts[i].size());
+    }
+    // data: each track's bytes appended, its references moved with them
+    std::vector<size_t> data_at(n);
+    for (size_t i = 0; i < n; ++i) {
+        const size_t off = (parent->data.size() + 15) & ~(size_t)15;
+        parent->data.resize(off + tr[i].data.size());
+        if (!tr[i].data.empty()) std::memcpy(parent->data.data() + off, tr[i].data.data(), tr[i].data.size());
+        data_at[i] = off;
+    }
+    std::vector<std::vector<size_t>> new_index(n);
+    for (size_t i = 0; i < n; ++i) new_index[i].assign(tr[i].steps.size(), 0);
+    for (size_t k = 0; k < phases; ++k) {
+        size_t last = SIZE_MAX;
+        for (size_t i = 0; i < n; ++i) {
+            if (k >= starts[i].size()) continue;
+            const size_t s0 = starts[i][k];
+            const size_t s1 = k + 1 < starts[i].size() ? starts[i][k + 1] : tr[i].steps.size();
+            last = parent->steps.size();
+            for (size_t s = s0; s < s1; ++s) {
+                new_index[i][s] = parent->steps.size();
+                parent->steps.push_back(tr[i].steps[s]);
+            }
+            parent->steps[last].aux |= kStepNoBarrier;
+        }
+        if (last != SIZE_MAX) parent->steps[last].aux &= (uint16_t)~kStepNoBarrier; // barrier after the phase
+    }
+    for (size_t i = 0; i < n; ++i)
+        for (const auto &ref : tr[i].data_refs) parent->data_refs.emplace_back(new_index[i][ref.first], data_at[i] + ref.second);
+}
+
 void record_command(vo_ctx *ctx, const vo_command &cmd, Recorder &rec)
 {
     VO_CHECK(!ctx->tracing, VO_ERR_UNSUPPORTED, "tracing runs in launch mode only");
@@ -273,6 +329,7 @@ void dispatch(vo_ctx *ctx, const vo_command &cmd)
     case VO_OP_FILTER_SUM: op_filter_sum(ctx, a.filter); break;
     case VO_OP_PKFK_JOIN_GROUPSUM: op_join_groupsum(ctx, a.join); break;
     case VO_OP_SORT_BATCH: op_sort_batch(ctx, a.sort_batch); break;
+    case VO_OP_COMPACT_BATCH: op_compact_batch(ctx, a.compact_batch); break;
     case VO_OP_SORT_MERGE_JOIN_COUNT: (void)op_join_pairs_count(ctx, a.join_pairs); break;
     case VO_OP_SORT_MERGE_JOIN_WRITE: op_join_pairs_write(ctx, a.join_pairs); break;
     case VO_OP_GATHER_OBLIVIOUS: op_gather_oblivious(ctx, a.gather_oblivious); break;
@@ -564,6 +621,7 @@ VO_SYNC_OP(vo_shuffle, VO_OP_SHUFFLE, shuffle, vo_shuffle_args)
 VO_SYNC_OP(vo_filter_sum, VO_OP_FILTER_SUM, filter, vo_filter_args)
 VO_SYNC_OP(vo_pkfk_join_groupsum, VO_OP_PKFK_JOIN_GROUPSUM, join, vo_join_args)
 VO_SYNC_OP(vo_sort_batch, VO_OP_SORT_BATCH, sort_batch, vo_sort_batch_args)
+VO_SYNC_OP(vo_compact_batch, VO_OP_COMPACT_BATCH, compact_batch, vo_compact_batch_args)
 VO_SYNC_OP(vo_join_pairs_write, VO_OP_SORT_MERGE_JOIN_WRITE, join_pairs, vo_join_pairs_args)
 VO_SYNC_OP(vo_gather_oblivious, VO_OP_GATHER_OBLIVIOUS, gather_oblivious, vo_gather_oblivious_args)
 VO_SYNC_OP(vo_predicate_mask, VO_OP_PREDICATE_MASK, predicate, vo_predicate_args)

@@ -63,12 +63,16 @@ Workspace workspace(vo_ctx *ctx, uint64_t n_pad, bool want_mask)
     return ws;
 }
 
+// The network's passes.  One compaction: n_pad rows, levels below
+// log2(n_pad).  A batch class (in.off set): `rows` virtual rows of instances
+// of 2^in.c rows each, levels below in.c, every pass over all of them.
 template <int NW>
 void run_network(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, uint64_t n, uint64_t n_pad,
-                 const uint64_t *tile_pre)
+                 const uint64_t *tile_pre, Inst in = Inst{nullptr, nullptr, 0}, uint64_t rows = 0)
 {
-    const int n_log = ilog2(n_pad);
-    const uint32_t nt = (uint32_t)(n_pad / CT);
+    const int n_log = in.off ? in.c : ilog2(n_pad);
+    if (!in.off) rows = n_pad;
+    const uint32_t nt = (uint32_t)(rows / CT);
     const size_t lds = (size_t)CTPAD * 8 + (CT + 1) * 2;
     bool w8 = true;
     for (int w = 0; w < NW; ++w) w8 = w8 && src.c[w].w == 8 && cols.c[w].w == 8;
@@ -77,7 +81,7 @@ void run_network(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, 
                  : (w8 ? k_compact_local<NW, true, false> : k_compact_local<NW, false, false>);
     VO_HIP(hipFuncSetAttribute((const void *)kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     launch(ctx, LaunchInfo{"compact_local", dim3(nt), dim3(CBLOCK), (uint32_t)lds, hash_mix(NW, n_log)}, kl, src,
-           cols, mask, n, tile_pre, trace_slot(ctx));
+           cols, mask, n, tile_pre, in, trace_slot(ctx));
     // levels CT_LOG .. n_log-1: passes of at most 8 levels, as even as
     // possible; 5..8 levels run LDS-staged, fewer in registers
     const int count = n_log - CT_LOG;
@@ -91,7 +95,7 @@ void run_network(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, 
             auto pick = [&](auto k5, auto k6, auto k7, auto k8) {
                 auto k = s == 5 ? k5 : s == 6 ? k6 : s == 7 ? k7 : k8;
                 VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hl));
-                launch(ctx, li, k, cols, tile_pre, q, trace_slot(ctx));
+                launch(ctx, li, k, cols, tile_pre, q, in, trace_slot(ctx));
             };
             if (tr) {
                 if (w8)
@@ -111,21 +115,21 @@ void run_network(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, 
             q += s;
             continue;
         }
-        const uint32_t blocks = (uint32_t)((n_pad >> s) / 256);
+        const uint32_t blocks = (uint32_t)((rows >> s) / 256);
         LaunchInfo li{"compact_global", dim3(blocks), dim3(256), 0, hash_mix(hash_mix(NW, q), s)};
         switch (s) {
         case 1: launch(ctx, li, tr ? k_compact_global<NW, 1, true> : k_compact_global<NW, 1, false>, cols, tile_pre, q,
-                          trace_slot(ctx)); break;
+                          in, trace_slot(ctx)); break;
         case 2: launch(ctx, li, tr ? k_compact_global<NW, 2, true> : k_compact_global<NW, 2, false>, cols, tile_pre, q,
-                          trace_slot(ctx)); break;
+                          in, trace_slot(ctx)); break;
         case 3: launch(ctx, li, tr ? k_compact_global<NW, 3, true> : k_compact_global<NW, 3, false>, cols, tile_pre, q,
-                          trace_slot(ctx)); break;
+                          in, trace_slot(ctx)); break;
         case 4: launch(ctx, li, tr ? k_compact_global<NW, 4, true> : k_compact_global<NW, 4, false>, cols, tile_pre, q,
-                          trace_slot(ctx)); break;
+                          in, trace_slot(ctx)); break;
         default:
             if constexpr (NW == 1) {
                 launch(ctx, li, tr ? k_compact_global<NW, 5, true> : k_compact_global<NW, 5, false>, cols, tile_pre, q,
-                       trace_slot(ctx));
+                       in, trace_slot(ctx));
                 break;
             }
             throw Error{VO_ERR_INVALID, "compaction pass width"};
@@ -140,7 +144,7 @@ void count_and_scan(vo_ctx *ctx, const uint8_t *mask, uint64_t n, uint64_t n_pad
 {
     const uint32_t nt = (uint32_t)(n_pad / CT);
     launch(ctx, LaunchInfo{"compact_count", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_count_mask, mask, n, ws.tile_cnt,
-           trace_slot(ctx));
+           Inst{nullptr, nullptr, 0}, trace_slot(ctx));
     launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)ws.tile_cnt,
            nt, ws.tile_pre, out_count ? out_count : ws.count, trace_slot(ctx));
 }
@@ -165,7 +169,7 @@ void compact_rows2(vo_ctx *ctx, uint64_t *c0, uint64_t *c1, const uint8_t *mask,
     uint32_t *tile_cnt = (uint32_t *)(count_dev + 1);
     uint64_t *tile_pre = (uint64_t *)(((uintptr_t)(tile_cnt + nt) + 255) & ~(uintptr_t)255);
     launch(ctx, LaunchInfo{"compact_count", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_count_mask, mask, n_pad, tile_cnt,
-           trace_slot(ctx));
+           Inst{nullptr, nullptr, 0}, trace_slot(ctx));
     launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)tile_cnt, nt,
            tile_pre, count_dev, trace_slot(ctx));
     Cols<2> cs{{{c0, 8}, {c1, 8}}};
@@ -180,7 +184,7 @@ void compact_rows1(vo_ctx *ctx, uint64_t *c0, const uint8_t *mask, uint64_t n_pa
     uint32_t *tile_cnt = (uint32_t *)(count_dev + 1);
     uint64_t *tile_pre = (uint64_t *)(((uintptr_t)(tile_cnt + nt) + 255) & ~(uintptr_t)255);
     launch(ctx, LaunchInfo{"compact_count", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_count_mask, mask, n_pad, tile_cnt,
-           trace_slot(ctx));
+           Inst{nullptr, nullptr, 0}, trace_slot(ctx));
     launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)tile_cnt, nt,
            tile_pre, count_dev, trace_slot(ctx));
     Cols<1> cs{{{c0, 8}}};
@@ -227,6 +231,92 @@ void op_compact(vo_ctx *ctx, const vo_compact_args &a)
 {
     check_pad(a.n, a.n_pad);
     compact_with(ctx, a, workspace(ctx, a.n_pad, false));
+}
+
+// Independent compactions bucketed by power-of-two size class.  A class's
+// items run as one compaction over its virtual rows (item k of the class at
+// virtual rows [k 2^c, (k+1) 2^c), physical rows from its offset): one count
+// pass, one scan block per item (each item's prefix counts restart at zero),
+// then the network's passes for levels below c.  A level below c never pairs
+// rows of two items, and each item sees the prefix counts it would see
+// alone, so its rows end where vo_compact on it alone leaves them.  Classes
+// are independent pieces of the command (in the resident queue they share
+// the lane between barriers).
+void op_compact_batch(vo_ctx *ctx, const vo_compact_batch_args &a)
+{
+    VO_CHECK(a.mask && (a.items || a.n_items == 0), VO_ERR_INVALID, "null buffer");
+    VO_CHECK(a.n_cols >= 1 && a.n_cols <= VO_MAX_SORT_COLUMNS, VO_ERR_INVALID, "bad compaction columns");
+    for (uint32_t c = 0; c < a.n_cols; ++c) {
+        const uint32_t w = a.col_bytes[c];
+        VO_CHECK(a.cols[c] && (w == 1 || w == 2 || w == 4 || w == 8), VO_ERR_INVALID, "bad column");
+    }
+    std::vector<std::vector<uint64_t>> off(33), ns(33), idx(33);
+    for (uint64_t i = 0; i < a.n_items; ++i) {
+        const vo_batch_item &it = a.items[i];
+        VO_CHECK(is_pow2(it.n_pad) && it.n_pad >= (uint64_t)CT && it.n_pad <= (1ull << 32) && it.n <= it.n_pad &&
+                     it.offset % it.n_pad == 0,
+                 VO_ERR_INVALID,
+                 "compaction batch item: n_pad must be a power of two in [4096, 2^32], >= n, dividing offset");
+        const int c = ilog2(it.n_pad);
+        off[c].push_back(it.offset);
+        ns[c].push_back(it.n);
+        idx[c].push_back(i);
+    }
+    std::vector<int> classes;
+    std::vector<uint64_t> host;
+    std::vector<size_t> at(33, 0), cnt_at(33, 0), pre_at(33, 0);
+    for (int c = 32; c >= CT_LOG; --c) {
+        if (off[c].empty()) continue;
+        VO_CHECK(off[c].size() <= ((1ull << 32) >> c), VO_ERR_INVALID,
+                 "compaction batch: the items of one size class exceed 2^32 rows");
+        classes.push_back(c);
+        at[c] = host.size();
+        host.insert(host.end(), off[c].begin(), off[c].end());
+        host.insert(host.end(), ns[c].begin(), ns[c].end());
+        host.insert(host.end(), idx[c].begin(), idx[c].end());
+    }
+    if (classes.empty()) return;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t bytes = al(host.size() * 8);
+    for (int c : classes) {
+        const uint64_t tiles = (uint64_t)off[c].size() << (c - CT_LOG);
+        cnt_at[c] = bytes;
+        bytes += al(tiles * 4);
+        pre_at[c] = bytes;
+        bytes += al((tiles + 1) * 8); // the hybrid pass reads one prefix past its group
+    }
+    char *base = (char *)scratch(ctx, bytes);
+    uint64_t *dtab = (uint64_t *)base;
+    upload_async(ctx, dtab, host.data(), host.size() * 8);
+    run_tracks(ctx, classes.size(), [&](size_t t) {
+        const int c = classes[t];
+        const uint64_t k = off[c].size();
+        const Inst in{dtab + at[c], dtab + at[c] + k, c};
+        const uint64_t *ix = dtab + at[c] + 2 * k;
+        const uint32_t nts = 1u << (c - CT_LOG);
+        const uint64_t tiles = k * nts;
+        uint32_t *tile_cnt = (uint32_t *)(base + cnt_at[c]);
+        uint64_t *tile_pre = (uint64_t *)(base + pre_at[c]);
+        launch(ctx, LaunchInfo{"compact_count", dim3((uint32_t)tiles), dim3(CBLOCK), 0, hash_mix(k, c)}, k_count_mask,
+               a.mask, (uint64_t)0, tile_cnt, in, trace_slot(ctx));
+        launch(ctx, LaunchInfo{"compact_scan_seg", dim3((uint32_t)k), dim3(CBLOCK), 0, hash_mix(k, c)}, k_scan_seg,
+               (const uint32_t *)tile_cnt, nts, tile_pre, a.out_counts, ix, trace_slot(ctx));
+        const uint64_t rows = k << c;
+        uint32_t j = 0;
+        while (j < a.n_cols) {
+            const uint32_t take = std::min<uint32_t>(2, a.n_cols - j);
+            if (take == 2) {
+                Cols<2> cs{{{a.cols[j], a.col_bytes[j]}, {a.cols[j + 1], a.col_bytes[j + 1]}}};
+                run_network<2>(ctx, cs, cs, a.mask, 0, 0, tile_pre, in, rows);
+            } else {
+                Cols<1> cs{{{a.cols[j], a.col_bytes[j]}}};
+                run_network<1>(ctx, cs, cs, a.mask, 0, 0, tile_pre, in, rows);
+            }
+            j += take;
+        }
+    });
+    // the host table must outlive its upload (recorded: it travels with the program)
+    if (!ctx->rec) VO_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 namespace {

@@ -93,21 +93,63 @@ __device__ __forceinline__ bool level_flag32(uint32_t cb, uint32_t cbh, int q, u
     return base ^ (i >= om1);
 }
 
+// Instances of one size class 2^c of a batch: virtual row v of the class is
+// row v mod 2^c of instance v >> c, stored at physical row
+// off[v >> c] + (v mod 2^c), whose live rows are n[v >> c].  off == nullptr:
+// a single compaction, virtual rows are physical rows and the caller's n
+// applies.  Every caller passes a v whose instance is the same for the whole
+// wavefront, so the table reads are scalar.
+struct Inst {
+    const uint64_t *off;
+    const uint64_t *n;
+    int c;
+};
+
+__device__ __forceinline__ uint64_t uni64(uint64_t v)
+{
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
+__device__ __forceinline__ uint32_t inst_of(const Inst &in, uint64_t v)
+{
+    return __builtin_amdgcn_readfirstlane((uint32_t)(v >> in.c));
+}
+
+// physical row of virtual row v
+__device__ __forceinline__ uint64_t inst_row(const Inst &in, uint64_t v)
+{
+    if (in.off == nullptr) return v;
+    return uni64(in.off[inst_of(in, v)]) + (v & ((1ull << in.c) - 1));
+}
+
+// row of v inside its instance, and that instance's live rows
+__device__ __forceinline__ uint64_t inst_local(const Inst &in, uint64_t v)
+{
+    return in.off == nullptr ? v : (v & ((1ull << in.c) - 1));
+}
+
+__device__ __forceinline__ uint64_t inst_live(const Inst &in, uint64_t v, uint64_t n)
+{
+    return in.off == nullptr ? n : uni64(in.n[inst_of(in, v)]);
+}
+
 // ---------------------------------------------------------------- counting
 
 // Selected rows per tile: sel[i] = mask[i] != 0 && i < n.
 __device__ __forceinline__ void b_count_mask(const uint8_t *__restrict__ mask, uint64_t n,
-                                             uint32_t *__restrict__ tile_cnt, TraceSlot *slot, uint32_t vb)
+                                             uint32_t *__restrict__ tile_cnt, Inst in, TraceSlot *slot, uint32_t vb)
 {
     __shared__ uint32_t red[CBLOCK / 64];
     dev::Atr tr(slot);
-    const uint64_t t0 = (uint64_t)vb * CT;
+    const uint64_t tv = (uint64_t)vb * CT;
+    const uint64_t t0 = inst_row(in, tv), l0 = inst_local(in, tv), nn = inst_live(in, tv, n);
     uint32_t c = 0;
 #pragma unroll 4
     for (int k = 0; k < CT / CBLOCK; ++k) {
-        const uint64_t i = t0 + k * CBLOCK + threadIdx.x;
-        c += (mask[i] != 0) & (i < n);
-        tr.at(1, i);
+        const uint32_t i = k * CBLOCK + threadIdx.x;
+        c += (mask[t0 + i] != 0) & (l0 + i < nn);
+        tr.at(1, t0 + i);
     }
     for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
@@ -122,9 +164,9 @@ __device__ __forceinline__ void b_count_mask(const uint8_t *__restrict__ mask, u
 }
 
 __global__ __launch_bounds__(CBLOCK) void k_count_mask(const uint8_t *__restrict__ mask, uint64_t n,
-                                                       uint32_t *__restrict__ tile_cnt, TraceSlot *slot)
+                                                       uint32_t *__restrict__ tile_cnt, Inst in, TraceSlot *slot)
 {
-    b_count_mask(mask, n, tile_cnt, slot, blockIdx.x);
+    b_count_mask(mask, n, tile_cnt, in, slot, blockIdx.x);
 }
 
 // Q6-style predicate: writes mask[i] and per-tile counts in one pass.
@@ -176,7 +218,7 @@ __global__ __launch_bounds__(CBLOCK) void k_filter_count(const int32_t *__restri
 
 // Exclusive scan of the tile counts by one block of NT threads:
 // pre[0..nt], pre[nt] = total; part holds NT words of LDS.
-template <uint32_t NT>
+template <uint32_t NT, bool TOTAL = true>
 __device__ __forceinline__ void b_scan_tiles(const uint32_t *__restrict__ cnt, uint32_t nt, uint64_t *__restrict__ pre,
                                              uint64_t *__restrict__ out_count, TraceSlot *slot, uint64_t *part)
 {
@@ -209,7 +251,7 @@ __device__ __forceinline__ void b_scan_tiles(const uint32_t *__restrict__ cnt, u
         }
     }
     if (threadIdx.x == NT - 1) {
-        pre[nt] = part[NT - 1];
+        if (TOTAL) pre[nt] = part[NT - 1];
         if (out_count) *out_count = part[NT - 1];
         tr.at(6, nt);
     }
@@ -222,6 +264,27 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict_
 {
     __shared__ uint64_t part[1024];
     b_scan_tiles<1024>(cnt, nt, pre, out_count, slot, part);
+}
+
+// The tile scans of every instance of a batch class, instance vb by block vb:
+// its nts tiles' exclusive prefix counts go to pre[vb * nts ..] (each
+// instance restarts at zero, so each compacts to its own front) and its
+// selected count to out[idx[vb]] when out is given.
+__device__ __forceinline__ void b_scan_seg(const uint32_t *__restrict__ cnt, uint32_t nts, uint64_t *__restrict__ pre,
+                                           uint64_t *__restrict__ out, const uint64_t *__restrict__ idx,
+                                           TraceSlot *slot, uint64_t *part, uint32_t vb)
+{
+    const uint64_t o = (uint64_t)vb * nts;
+    uint64_t *oc = out ? out + uni64(idx[vb]) : nullptr;
+    b_scan_tiles<CBLOCK, false>(cnt + o, nts, pre + o, oc, slot, part);
+}
+
+__global__ __launch_bounds__(CBLOCK) void k_scan_seg(const uint32_t *__restrict__ cnt, uint32_t nts,
+                                                     uint64_t *__restrict__ pre, uint64_t *__restrict__ out,
+                                                     const uint64_t *__restrict__ idx, TraceSlot *slot)
+{
+    __shared__ uint64_t part[CBLOCK];
+    b_scan_seg(cnt, nts, pre, out, idx, slot, part, blockIdx.x);
 }
 
 // ---------------------------------------------------------- local levels
@@ -291,15 +354,16 @@ __device__ __forceinline__ void clevels(uint64_t (&r)[16][NW], const uint16_t *c
 // W8: every column is 8 bytes wide (no per-load width dispatch)
 template <int NW, bool W8, bool TRACE>
 __device__ __forceinline__ void b_compact_local(Cols<NW> src, Cols<NW> dst, const uint8_t *__restrict__ mask,
-                                                uint64_t n, const uint64_t *__restrict__ tile_pre, TraceSlot *slot,
-                                                char *lds_raw, uint32_t vb)
+                                                uint64_t n, const uint64_t *__restrict__ tile_pre, Inst in,
+                                                TraceSlot *slot, char *lds_raw, uint32_t vb)
 {
     static_assert(CT == 16 * CBLOCK, "16 rows per thread");
     uint64_t *buf = reinterpret_cast<uint64_t *>(lds_raw);          // one column, CTPAD words
     uint16_t *cpre = reinterpret_cast<uint16_t *>(buf + CTPAD);      // CT + 1 counts <= 4096
     __shared__ uint32_t wsum[CBLOCK / 64];
     const uint32_t tid = threadIdx.x;
-    const uint64_t t0 = (uint64_t)vb * CT;
+    const uint64_t tv = (uint64_t)vb * CT;
+    const uint64_t t0 = inst_row(in, tv), l0 = inst_local(in, tv), nn = inst_live(in, tv, n);
     const uint64_t tbase = tile_pre[vb];
     constexpr int PER = CT / CBLOCK; // 16 consecutive rows per thread for the scan
     dev::AtrT<TRACE> tr(slot);
@@ -320,10 +384,10 @@ __device__ __forceinline__ void b_compact_local(Cols<NW> src, Cols<NW> dst, cons
     uint32_t run = 0;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const uint64_t i = t0 + tid * PER + k;
-        sel[k] = (mask[i] != 0) & (i < n);
+        const uint32_t i = tid * PER + k;
+        sel[k] = (mask[t0 + i] != 0) & (l0 + i < nn);
         run += sel[k];
-        tr.at(1, i);
+        tr.at(1, t0 + i);
     }
     uint32_t incl = run;
     const uint32_t lane = tid & 63;
@@ -366,11 +430,11 @@ __device__ __forceinline__ void b_compact_local(Cols<NW> src, Cols<NW> dst, cons
 
 template <int NW, bool W8, bool TRACE>
 __global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> src, Cols<NW> dst, const uint8_t *__restrict__ mask,
-                                                          uint64_t n, const uint64_t *__restrict__ tile_pre,
+                                                          uint64_t n, const uint64_t *__restrict__ tile_pre, Inst in,
                                                           TraceSlot *slot)
 {
     extern __shared__ __align__(16) char lds_raw[];
-    b_compact_local<NW, W8, TRACE>(src, dst, mask, n, tile_pre, slot, lds_raw, blockIdx.x);
+    b_compact_local<NW, W8, TRACE>(src, dst, mask, n, tile_pre, in, slot, lds_raw, blockIdx.x);
 }
 
 // ---------------------------------------------------------- global levels
@@ -381,18 +445,20 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> src, Cols<NW>
 // wavefront and come from the tile prefix array.
 template <int NW, int S, bool TRACE>
 __device__ __forceinline__ void b_compact_global(Cols<NW> cols, const uint64_t *__restrict__ tile_pre, int q_lo,
-                                                 TraceSlot *slot, uint32_t vb)
+                                                 Inst in, TraceSlot *slot, uint32_t vb)
 {
     dev::AtrT<TRACE> tr(slot);
     const uint64_t g = (uint64_t)vb * 256 + threadIdx.x;
     const uint64_t base = (g & ((1ull << q_lo) - 1)) | ((g >> q_lo) << (q_lo + S));
+    // the 2^S rows lie in one instance (q_lo + S <= its size class)
+    const uint64_t pb = inst_row(in, base);
     uint64_t r[1 << S][NW];
 #pragma unroll
     for (int m = 0; m < (1 << S); ++m)
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            r[m][w] = ld_col(cols.c[w], base + ((uint64_t)m << q_lo));
-            tr.at(8 + w, base + ((uint64_t)m << q_lo));
+            r[m][w] = ld_col(cols.c[w], pb + ((uint64_t)m << q_lo));
+            tr.at(8 + w, pb + ((uint64_t)m << q_lo));
         }
     // uniform part of base (bits >= 8 are shared by the wavefront)
     const uint64_t ubase = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(base >> 32)) << 32) |
@@ -425,17 +491,17 @@ __device__ __forceinline__ void b_compact_global(Cols<NW> cols, const uint64_t *
     for (int m = 0; m < (1 << S); ++m)
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            st_col(cols.c[w], base + ((uint64_t)m << q_lo), r[m][w]);
-            tr.at(16 + w, base + ((uint64_t)m << q_lo));
+            st_col(cols.c[w], pb + ((uint64_t)m << q_lo), r[m][w]);
+            tr.at(16 + w, pb + ((uint64_t)m << q_lo));
         }
     tr.done(dev::gtid());
 }
 
 template <int NW, int S, bool TRACE>
 __global__ __launch_bounds__(256) void k_compact_global(Cols<NW> cols, const uint64_t *__restrict__ tile_pre, int q_lo,
-                                                        TraceSlot *slot)
+                                                        Inst in, TraceSlot *slot)
 {
-    b_compact_global<NW, S, TRACE>(cols, tile_pre, q_lo, slot, blockIdx.x);
+    b_compact_global<NW, S, TRACE>(cols, tile_pre, q_lo, in, slot, blockIdx.x);
 }
 
 // ------------------------------------------------- LDS-staged global levels
@@ -476,7 +542,7 @@ __device__ __forceinline__ void hlevels(uint64_t (&r)[16][NW], const uint32_t *P
 
 template <int NW, int S, bool W8, bool TRACE>
 __device__ __forceinline__ void b_compact_hybrid(Cols<NW> cols, const uint64_t *__restrict__ tile_pre, int q_lo,
-                                                 TraceSlot *slot, char *lds_raw, uint32_t vb)
+                                                 Inst in, TraceSlot *slot, char *lds_raw, uint32_t vb)
 {
     dev::AtrT<TRACE> tr(slot);
     static_assert(S >= 5 && S <= 8, "hybrid widths 5..8");
@@ -488,6 +554,8 @@ __device__ __forceinline__ void b_compact_hybrid(Cols<NW> cols, const uint64_t *
     const uint32_t runs = (1u << q_lo) >> CB; // blocks per group of 2^(q_lo+S) rows
     const uint32_t hibase = (vb / runs) << (q_lo + S);
     const uint32_t j0 = (vb % runs) << CB;
+    // the group's 2^(q_lo+S) rows lie in one instance: physical row of its first
+    const uint64_t ph = inst_row(in, hibase);
     for (uint32_t k = tid; k <= (1u << S); k += CBLOCK) {
         P[k] = (uint32_t)tile_pre[((uint64_t)hibase + ((uint64_t)k << q_lo)) >> CT_LOG];
         tr.at(7, ((uint64_t)hibase + ((uint64_t)k << q_lo)) >> CT_LOG);
@@ -496,11 +564,11 @@ __device__ __forceinline__ void b_compact_hybrid(Cols<NW> cols, const uint64_t *
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const uint32_t l = tid | ((uint32_t)m << 8);
-        const uint32_t g = hibase + ((l >> CB) << q_lo) + j0 + (l & ((1u << CB) - 1));
+        const uint32_t g = ((l >> CB) << q_lo) + j0 + (l & ((1u << CB) - 1)); // row in the group
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            r[m][w] = W8 ? ((const uint64_t *)cols.c[w].p)[g] : ld_col(cols.c[w], g);
-            tr.at(8 + w, g);
+            r[m][w] = W8 ? ((const uint64_t *)cols.c[w].p + ph)[g] : ld_col(cols.c[w], ph + g);
+            tr.at(8 + w, ph + g);
         }
     }
     ctranspose<NW>(r, buf, tid, 8, 4); // its barriers also publish P
@@ -514,14 +582,14 @@ __device__ __forceinline__ void b_compact_hybrid(Cols<NW> cols, const uint64_t *
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const uint32_t l = tid_s | ((uint32_t)m << 8);
-        const uint32_t g = hibase + ((l >> CB) << q_lo) + j0 + (l & ((1u << CB) - 1));
+        const uint32_t g = ((l >> CB) << q_lo) + j0 + (l & ((1u << CB) - 1));
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
             if (W8)
-                ((uint64_t *)cols.c[w].p)[g] = r[m][w];
+                ((uint64_t *)cols.c[w].p + ph)[g] = r[m][w];
             else
-                st_col(cols.c[w], g, r[m][w]);
-            tr.at(16 + w, g);
+                st_col(cols.c[w], ph + g, r[m][w]);
+            tr.at(16 + w, ph + g);
         }
     }
     tr.done(dev::gtid());
@@ -529,10 +597,10 @@ __device__ __forceinline__ void b_compact_hybrid(Cols<NW> cols, const uint64_t *
 
 template <int NW, int S, bool W8, bool TRACE>
 __global__ __launch_bounds__(CBLOCK) void k_compact_hybrid(Cols<NW> cols, const uint64_t *__restrict__ tile_pre,
-                                                           int q_lo, TraceSlot *slot)
+                                                           int q_lo, Inst in, TraceSlot *slot)
 {
     extern __shared__ __align__(16) char lds_raw[];
-    b_compact_hybrid<NW, S, W8, TRACE>(cols, tile_pre, q_lo, slot, lds_raw, blockIdx.x);
+    b_compact_hybrid<NW, S, W8, TRACE>(cols, tile_pre, q_lo, in, slot, lds_raw, blockIdx.x);
 }
 
 // ------------------------------------------------------- int128 SUM pass

@@ -490,8 +490,10 @@ __global__ __launch_bounds__(RB, 2) void k_queue(QArgs qa)
         for (uint32_t s = 0; s < ns && ok;) {
             run_step(prog + s, lp.rank, lp.nbl);
             const uint32_t sl = __builtin_amdgcn_readfirstlane((uint32_t)prog[s].slots);
+            const uint32_t fl = __builtin_amdgcn_readfirstlane((uint32_t)prog[s].aux);
             s += sl ? sl : 1;
-            if (s < ns) ok = lbarrier(qd, lp, epoch, qa.bar_ticks, NoHook{});
+            // the next step reads nothing this one wrote: run on into it
+            if (s < ns && !(fl & kStepNoBarrier)) ok = lbarrier(qd, lp, epoch, qa.bar_ticks, NoHook{});
         }
         if (!ok) {
             reason = 4;

@@ -15,6 +15,7 @@
 #pragma once
 
 // Included at the end of vo_internal.hpp (needs vo_ctx and LaunchInfo).
+#include <functional>
 #include <tuple>
 #include <type_traits>
 #include <utility>
@@ -34,11 +35,15 @@ struct alignas(16) Step {
     uint32_t id;    // StepId or a body id from the tables
     uint32_t grid;  // virtual blocks
     uint32_t lds;   // dynamic LDS bytes the body reads from its `lds` argument
-    uint16_t aux;   // fill: byte value
+    uint16_t aux;   // fill: byte value (low 8 bits); kStepNoBarrier
     uint16_t slots; // program slots this step takes (>= 1)
     char blob[kStepBlob];
 };
 static_assert(sizeof(Step) == 256, "one program slot is 256 bytes");
+
+// Step::aux flag: the next step does not read what this one writes, so the
+// lane runs on into it without a barrier.
+constexpr uint16_t kStepNoBarrier = 0x8000;
 
 // Built-in steps (ids below kFirstBody).
 enum StepId : uint32_t {
@@ -187,6 +192,14 @@ void copy_async(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes); // dev
 void upload_async(vo_ctx *ctx, void *dst, const void *host_src, uint64_t bytes); // host to device
 // device to pinned (host-mapped) memory; while recording a copy step writes it
 void to_host_async(vo_ctx *ctx, void *pinned_dst, const void *src, uint64_t bytes);
+
+// Independent pieces of one command: f(i) for i < n, where no piece reads
+// what another writes.  In launch mode they run in order on the context
+// stream.  While recording, each piece's steps go to a recorder of their own
+// and are merged phase by phase into the command's program: phase k holds
+// step k of every piece, with no barrier between the steps of one phase and a
+// barrier after it, so the pieces share the lane's blocks.
+void run_tracks(vo_ctx *ctx, size_t n, const std::function<void(size_t)> &f);
 
 // The recorded program of one command (persistent mode): runs the command's
 // host driver with a recorder attached.

@@ -113,6 +113,7 @@
     X(kB_count_mask, (compact_k::k_count_mask), (compact_k::b_count_mask(a..., vb)), 0)                           \
     X(kB_filter_count, (compact_k::k_filter_count), (compact_k::b_filter_count(a..., vb)), 0)                     \
     X(kB_scan_tiles, (compact_k::k_scan_tiles), (compact_k::b_scan_tiles<256>(a..., (uint64_t *)lds)), 2048)      \
+    X(kB_scan_seg, (compact_k::k_scan_seg), (compact_k::b_scan_seg(a..., (uint64_t *)lds, vb)), 2048)             \
     X(kB_sum_prod, (compact_k::k_sum_prod_i128), (compact_k::b_sum_prod_i128(a..., (__int128 *)lds, vb, nvb)),    \
       4096)                                                                                                        \
     X(kB_sum_final, (compact_k::k_sum_final), (compact_k::b_sum_final(a..., (__int128 *)lds)), 4096)              \

@@ -115,16 +115,21 @@ void op_sort_batch(vo_ctx *ctx, const vo_sort_batch_args &a)
     Bufs<RecKP> b;
     b.k[0] = a.keys;
     b.p = a.payload;
-    for (int c = 32; c >= 0; --c) {
+    // size classes are independent pieces (in the resident queue they share
+    // the lane between barriers)
+    std::vector<int> classes;
+    for (int c = 32; c >= 0; --c)
+        if (!off[c].empty()) classes.push_back(c);
+    run_tracks(ctx, classes.size(), [&](size_t t) {
+        const int c = classes[t];
         const uint64_t cnt = off[c].size();
-        if (cnt == 0) continue;
         if (c < 4) {
             // instances of at most 8 rows: one small launch each
             for (uint64_t i = 0; i < cnt; ++i) op_sort_kp(ctx, a.keys + off[c][i], a.payload + off[c][i], ns[c][i], 1ull << c);
-            continue;
+            return;
         }
         run_class(ctx, b, c, cnt, dtab + at[c], dtab + at[c] + cnt);
-    }
+    });
     // the host table must outlive its upload (recorded: it travels with the program)
     if (!ctx->rec) VO_HIP(hipStreamSynchronize(ctx->stream));
 }

@@ -181,6 +181,7 @@ inline SortTuning apply_tuning_word(SortTuning t, uint32_t w)
 void op_sort_kp(vo_ctx *ctx, uint64_t *keys, uint32_t *payload, uint64_t n, uint64_t n_pad);
 void op_sort_multi(vo_ctx *ctx, const vo_sort_args &a);
 void op_sort_batch(vo_ctx *ctx, const vo_sort_batch_args &a);
+void op_compact_batch(vo_ctx *ctx, const vo_compact_batch_args &a);
 void op_permute(vo_ctx *ctx, const vo_permute_args &a);
 void op_scatter(vo_ctx *ctx, const vo_scatter_args &a);
 void op_gather(vo_ctx *ctx, const vo_gather_args &a);
