@@ -86,6 +86,15 @@ const BodyInfo *find_body(const void *kernel)
     return nullptr;
 }
 
+namespace {
+// Blocks a recorded fill or copy uses: one per 64 KiB, at most 4096 (the lane
+// strides over the bytes with at most all of its blocks).
+uint32_t byte_step_blocks(uint64_t bytes)
+{
+    return (uint32_t)std::min<uint64_t>(4096, std::max<uint64_t>(1, (bytes + 65535) >> 16));
+}
+} // namespace
+
 void fill_async(vo_ctx *ctx, void *dst, int value, uint64_t bytes)
 {
     if (bytes == 0) return;
@@ -95,7 +104,7 @@ void fill_async(vo_ctx *ctx, void *dst, int value, uint64_t bytes)
     }
     Step s{};
     s.id = kStepFill;
-    s.grid = 1;
+    s.grid = byte_step_blocks(bytes);
     s.slots = 1;
     s.aux = (uint16_t)(value & 0xFF);
     const FillArgs a{dst, bytes};
@@ -112,7 +121,7 @@ void copy_async(vo_ctx *ctx, void *dst, const void *src, uint64_t bytes)
     }
     Step s{};
     s.id = kStepCopy;
-    s.grid = 1;
+    s.grid = byte_step_blocks(bytes);
     s.slots = 1;
     const CopyArgs a{dst, src, bytes};
     std::memcpy(s.blob, &a, sizeof a);
@@ -143,7 +152,7 @@ void upload_async(vo_ctx *ctx, void *dst, const void *host_src, uint64_t bytes)
     std::memcpy(r.data.data() + off, host_src, bytes);
     Step s{};
     s.id = kStepCopy;
-    s.grid = 1;
+    s.grid = byte_step_blocks(bytes);
     s.slots = 1;
     const CopyArgs a{dst, nullptr, bytes};
     std::memcpy(s.blob, &a, sizeof a);

@@ -58,7 +58,9 @@ struct QEntry {
     uint32_t opcode;
     uint32_t n_steps;  // program length (0 for NOP / TERMINATE)
     const Step *prog;  // host-mapped program
-    uint64_t pad_[5];
+    uint32_t act;      // blocks of the lane the program uses (its largest step grid)
+    uint32_t n_bar;    // barriers between its steps
+    uint64_t pad_[4];
 };
 static_assert(sizeof(QEntry) == 64, "ring entries are one line");
 
@@ -87,7 +89,9 @@ struct QLaneDev {
     uint64_t seq;     // sequence id of the current entry
     uint64_t prog;    // host address of its program
     uint32_t opcode;
-    uint32_t pad_[5];
+    uint32_t act;     // blocks of the lane the current entry uses
+    uint32_t n_bar;   // barriers between its steps
+    uint32_t pad_[3];
 };
 static_assert(sizeof(QLaneDev) == 64, "lane state is one line");
 
@@ -114,11 +118,6 @@ struct QArgs {
     uint32_t pad_;
     uint64_t idle_ticks, bar_ticks;
 };
-
-__device__ __forceinline__ uint64_t ld_sys(const uint64_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
 
 __device__ __forceinline__ uint32_t ld_sys32(const uint32_t *p)
 {
@@ -162,10 +161,15 @@ struct LanePos {
 // Barrier number ++epoch of one lane.  The lane leader (rank 0) waits until
 // every other block of the lane has arrived, runs hook() with all its
 // threads (the hook sees every block's writes), then releases the lane.
+// `pub`: this block wrote data that others read after the barrier (its
+// arrival is a release); `sub`: it reads data others wrote before it (an
+// acquire follows the release).  A block without either touches no cache
+// (flags are read and written by device-coherent atomics, polled relaxed).
 // Every wait is bounded: a barrier that does not complete within
 // limit_ticks sets `abort` and returns false in every block.
 template <class Hook>
-__device__ bool lbarrier(QDev *qd, const LanePos &lp, uint32_t &epoch, uint64_t limit_ticks, Hook hook)
+__device__ bool lbarrier(QDev *qd, const LanePos &lp, uint32_t &epoch, uint64_t limit_ticks, Hook hook, bool pub,
+                         bool sub)
 {
     __shared__ int ok;
     QLaneDev *ld = &qd->lane[lp.lane];
@@ -173,18 +177,21 @@ __device__ bool lbarrier(QDev *qd, const LanePos &lp, uint32_t &epoch, uint64_t 
     __syncthreads();
     if (lp.rank != 0) {
         if (threadIdx.x == 0) {
-            __threadfence();
-            __hip_atomic_store(&qd->arrive[blockIdx.x * kFlagStride], e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t *a = &qd->arrive[blockIdx.x * kFlagStride];
+            if (pub)
+                __hip_atomic_store(a, e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            else
+                __hip_atomic_store(a, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             int good = 1;
             const uint64_t t0 = wall_clock64();
-            while ((int32_t)(__hip_atomic_load(&ld->release, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - e) < 0) {
+            while ((int32_t)(__hip_atomic_load(&ld->release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - e) < 0) {
                 if (timed_out(qd, t0, limit_ticks)) {
                     good = 0;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            __threadfence();
+            if (sub) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             ok = good;
         }
         __syncthreads();
@@ -214,10 +221,8 @@ __device__ bool lbarrier(QDev *qd, const LanePos &lp, uint32_t &epoch, uint64_t 
     if (!ok) return false;
     hook();
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        __hip_atomic_store(&ld->release, e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    // the leader always releases: the hook wrote the lane state and program
+    if (threadIdx.x == 0) __hip_atomic_store(&ld->release, e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
 
@@ -329,8 +334,14 @@ __device__ void run_step(const Step *st, uint32_t rank, uint32_t nbl)
     const uint32_t id = __builtin_amdgcn_readfirstlane(st->id);
     const uint32_t grid = __builtin_amdgcn_readfirstlane(st->grid);
     switch (id) {
-    case kStepFill: step_fill(uread<FillArgs>(st->blob), __builtin_amdgcn_readfirstlane((uint32_t)st->aux), rank, nbl); break;
-    case kStepCopy: step_copy(uread<CopyArgs>(st->blob), rank, nbl); break;
+    // fill and copy: the first min(grid, nbl) blocks stride over the bytes
+    case kStepFill:
+        if (rank < grid) step_fill(uread<FillArgs>(st->blob), __builtin_amdgcn_readfirstlane((uint32_t)st->aux), rank,
+                                   grid < nbl ? grid : nbl);
+        break;
+    case kStepCopy:
+        if (rank < grid) step_copy(uread<CopyArgs>(st->blob), rank, grid < nbl ? grid : nbl);
+        break;
 #define VO_CASE(ID, K, CALL, L)                                                                                    \
     case ID: run_##ID(st->blob, grid, rank, nbl); break;
         VO_ALL_BODIES(VO_CASE)
@@ -353,6 +364,7 @@ struct FetchHook {
 
     __device__ void operator()() const
     {
+        __shared__ uint32_t shared_ns, shared_ex;
         QDev *qd = qa->qd;
         QLaneDev *ld = &qd->lane[lane];
         QCtrl *ctrl = qa->ctrl;
@@ -369,7 +381,8 @@ struct FetchHook {
                     ex = 4;
                     break;
                 }
-                if (ld_sys(&ctrl->lane[lane].head) > consumed) {
+                if (__hip_atomic_load(&ctrl->lane[lane].head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) >
+                    consumed) {
                     if (voted && threadIdx.x == 0)
                         __hip_atomic_fetch_sub(&qd->idle_votes, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     voted = false;
@@ -401,25 +414,35 @@ struct FetchHook {
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
+            // the entry (published after its program) and the program: one
+            // acquire for the poll that saw the head move
+            if (ex == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
             if (threadIdx.x == 0) {
+                uint32_t ns = 0;
                 if (ex == 0) {
                     const QEntry *en = &qa->ring[lane * qa->cap_lane + consumed % qa->cap_lane];
                     const uint64_t *w = reinterpret_cast<const uint64_t *>(en);
                     const uint64_t w0 = __hip_atomic_load(w + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                     const uint64_t w1 = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                     const uint64_t w2 = __hip_atomic_load(w + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    const uint64_t w3 = __hip_atomic_load(w + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    ns = (uint32_t)(w1 >> 32);
+                    ns = ns <= kMaxSteps ? ns : 0;
                     ld->seq = w0;
                     ld->opcode = (uint32_t)w1;
-                    const uint32_t ns = (uint32_t)(w1 >> 32);
-                    ld->n_steps = ns <= kMaxSteps ? ns : 0;
                     ld->prog = w2;
+                    ld->act = ns ? (uint32_t)w3 : 0;
+                    ld->n_bar = ns ? (uint32_t)(w3 >> 32) : 0;
                 }
+                ld->n_steps = ns;
                 ld->exit = ex;
+                shared_ns = ns;
+                shared_ex = ex;
             }
         }
         __syncthreads();
-        if (ld->exit != 0) return;
-        const uint32_t ns = ld->n_steps;
+        if (shared_ex != 0) return;
+        const uint32_t ns = shared_ns;
         if (ns == 0) return;
         const u32x4 *src = (const u32x4 *)ld->prog;
         u32x4 *dst = (u32x4 *)(qa->prog + (uint64_t)lane * kMaxSteps);
@@ -458,23 +481,27 @@ __global__ __launch_bounds__(RB, 2) void k_queue(QArgs qa)
     for (;;) {
         // the other blocks wait here while the leader polls the ring: allow the idle time
         const FetchHook hook{&qa, lp.lane, consumed, done_seq};
-        if (!lbarrier(qd, lp, epoch, qa.idle_ticks + qa.bar_ticks, hook)) {
+        // the previous command's results were written back by the release
+        // below; the lane state is read with device-coherent atomics
+        if (!lbarrier(qd, lp, epoch, qa.idle_ticks + qa.bar_ticks, hook, false, false)) {
             reason = 4;
             break;
         }
-        const uint32_t ex = ld->exit;
+        const uint32_t ex = ld_ag(&ld->exit);
         if (ex) {
             reason = ex;
             break;
         }
-        const uint32_t ns = __builtin_amdgcn_readfirstlane(ld->n_steps);
+        const uint32_t ns = __builtin_amdgcn_readfirstlane(ld_ag(&ld->n_steps));
+        // blocks of rank >= act have no virtual block in any step: they skip
+        // the program and only keep the lane's barrier count
+        const bool active = lp.rank < __builtin_amdgcn_readfirstlane(ld_ag(&ld->act));
         // inputs may have been written by the host or a copy engine since
-        // this CU last looked: drop cached lines (every XCD has blocks here);
-        // a command without steps reads nothing
-        if (ns && threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+        // this CU last looked, and the program is new: drop cached lines
+        if (active && threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
         __syncthreads();
-        const uint32_t opcode = __builtin_amdgcn_readfirstlane(ld->opcode);
-        const uint64_t seq = ld->seq;
+        const uint32_t opcode = __builtin_amdgcn_readfirstlane(ld_ag(&ld->opcode));
+        const uint64_t seq = __hip_atomic_load(&ld->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ++consumed;
         done_seq = seq;
         if (opcode == VO_OP_TERMINATE) {
@@ -487,21 +514,31 @@ __global__ __launch_bounds__(RB, 2) void k_queue(QArgs qa)
             break;
         }
         bool ok = true;
-        for (uint32_t s = 0; s < ns && ok;) {
-            run_step(prog + s, lp.rank, lp.nbl);
-            const uint32_t sl = __builtin_amdgcn_readfirstlane((uint32_t)prog[s].slots);
-            const uint32_t fl = __builtin_amdgcn_readfirstlane((uint32_t)prog[s].aux);
-            s += sl ? sl : 1;
-            // the next step reads nothing this one wrote: run on into it
-            if (s < ns && !(fl & kStepNoBarrier)) ok = lbarrier(qd, lp, epoch, qa.bar_ticks, NoHook{});
+        if (active) {
+            bool wrote = false; // a virtual block of this block since the last barrier
+            for (uint32_t s = 0; s < ns && ok;) {
+                wrote |= lp.rank < __builtin_amdgcn_readfirstlane(prog[s].grid);
+                run_step(prog + s, lp.rank, lp.nbl);
+                const uint32_t sl = __builtin_amdgcn_readfirstlane((uint32_t)prog[s].slots);
+                const uint32_t fl = __builtin_amdgcn_readfirstlane((uint32_t)prog[s].aux);
+                s += sl ? sl : 1;
+                // the next step reads nothing this one wrote: run on into it
+                if (s < ns && !(fl & kStepNoBarrier)) {
+                    ok = lbarrier(qd, lp, epoch, qa.bar_ticks, NoHook{}, wrote, true);
+                    wrote = false;
+                }
+            }
+            // results out of every XCD's L2 before the leader tells the host
+            __syncthreads();
+            if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        } else {
+            const uint32_t nb = __builtin_amdgcn_readfirstlane(ld_ag(&ld->n_bar));
+            for (uint32_t k = 0; k < nb && ok; ++k) ok = lbarrier(qd, lp, epoch, qa.bar_ticks, NoHook{}, false, false);
         }
         if (!ok) {
             reason = 4;
             break;
         }
-        // results out of every XCD's L2 before the leader tells the host
-        __syncthreads();
-        if (ns && threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
     }
     if (lp.rank == 0 && threadIdx.x == 0) {
         ld->consumed = consumed;
@@ -673,6 +710,21 @@ const Step *place_program(vo_ctx *ctx, QueueState *q, Recorder &rec, uint64_t se
     return dst;
 }
 
+// The blocks a program needs (its largest step grid: blocks of higher rank
+// have no virtual block in any step) and the barriers between its steps, the
+// same count the resident kernel's step loop takes.
+void program_shape(const std::vector<Step> &st, uint32_t &act, uint32_t &n_bar)
+{
+    act = 0;
+    n_bar = 0;
+    for (size_t s = 0; s < st.size();) {
+        act = std::max(act, st[s].grid);
+        const size_t next = s + std::max<uint16_t>(1, st[s].slots);
+        if (next < st.size() && !(st[s].aux & kStepNoBarrier)) ++n_bar;
+        s = next;
+    }
+}
+
 // Publishes one entry on `lane`.  Caller holds ctx->mu.
 uint64_t publish_locked(vo_ctx *ctx, QueueState *q, uint32_t lane, uint32_t opcode, Recorder *rec)
 {
@@ -683,16 +735,19 @@ uint64_t publish_locked(vo_ctx *ctx, QueueState *q, uint32_t lane, uint32_t opco
                 "command ring full");
     if (seq > q->cap) wait_locked(ctx, q, [&] { return seq_done(q, seq - q->cap); }, "completion flags full");
     const Step *prog = nullptr;
-    uint32_t ns = 0;
+    uint32_t ns = 0, act = 0, n_bar = 0;
     if (rec && !rec->steps.empty()) {
         prog = place_program(ctx, q, *rec, seq);
         ns = (uint32_t)rec->steps.size();
+        program_shape(rec->steps, act, n_bar);
     }
     QEntry *en = &q->ring[lane * q->cap_lane + q->lane_submitted[lane] % q->cap_lane];
     en->seq = seq;
     en->opcode = opcode;
     en->n_steps = ns;
     en->prog = prog;
+    en->act = act;
+    en->n_bar = n_bar;
     q->submitted = seq;
     q->lane_submitted[lane] += 1;
     __atomic_store_n(&q->ctrl->lane[lane].head, q->lane_submitted[lane], __ATOMIC_RELEASE);
