@@ -370,8 +370,9 @@ struct FetchHook {
         QCtrl *ctrl = qa->ctrl;
         if (threadIdx.x < 64) {
             if (done_seq && threadIdx.x == 0) {
+                // the flag's release covers the lane count stored after it
                 st_sys(&qa->flag[done_seq % qa->cap], done_seq);
-                st_sys(&ctrl->lane[lane].done, consumed);
+                __hip_atomic_store(&ctrl->lane[lane].done, consumed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             }
             const uint64_t t0 = wall_clock64();
             uint32_t ex = 0;
@@ -417,15 +418,17 @@ struct FetchHook {
             // the entry (published after its program) and the program: one
             // acquire for the poll that saw the head move
             if (ex == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+            // the entry's first four words, one per lane: one PCIe round trip
+            uint64_t wv = 0;
+            if (ex == 0 && threadIdx.x < 4) {
+                const QEntry *en = &qa->ring[lane * qa->cap_lane + consumed % qa->cap_lane];
+                wv = __hip_atomic_load(reinterpret_cast<const uint64_t *>(en) + threadIdx.x, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            const uint64_t w0 = __shfl(wv, 0), w1 = __shfl(wv, 1), w2 = __shfl(wv, 2), w3 = __shfl(wv, 3);
             if (threadIdx.x == 0) {
                 uint32_t ns = 0;
                 if (ex == 0) {
-                    const QEntry *en = &qa->ring[lane * qa->cap_lane + consumed % qa->cap_lane];
-                    const uint64_t *w = reinterpret_cast<const uint64_t *>(en);
-                    const uint64_t w0 = __hip_atomic_load(w + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                    const uint64_t w1 = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                    const uint64_t w2 = __hip_atomic_load(w + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                    const uint64_t w3 = __hip_atomic_load(w + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                     ns = (uint32_t)(w1 >> 32);
                     ns = ns <= kMaxSteps ? ns : 0;
                     ld->seq = w0;
