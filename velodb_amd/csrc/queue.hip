@@ -201,11 +201,20 @@ __device__ bool lbarrier(QDev *qd, const LanePos &lp, uint32_t &epoch, uint64_t 
         int good = 1;
         const uint64_t t0 = wall_clock64();
         for (;;) {
+            // up to 8 flags per lane per round, loaded before any is tested:
+            // one memory round trip per 512 blocks
             int all = 1;
-            for (uint32_t k = 1 + threadIdx.x; k < lp.nbl; k += 64) {
-                const uint32_t blk = lp.lane + k * lp.L;
-                all &= (int32_t)(__hip_atomic_load(&qd->arrive[blk * kFlagStride], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT) - e) >= 0;
+            for (uint32_t k0 = 1 + threadIdx.x; k0 < lp.nbl; k0 += 512) {
+                uint32_t v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t k = k0 + 64u * j;
+                    v[j] = k < lp.nbl ? __hip_atomic_load(&qd->arrive[(lp.lane + k * lp.L) * kFlagStride],
+                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                      : e;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) all &= (int32_t)(v[j] - e) >= 0;
             }
             if (__all(all)) break;
             if (timed_out(qd, t0, limit_ticks)) { // uniform: scalar clock, same flag for every lane
@@ -378,25 +387,30 @@ struct FetchHook {
             uint32_t ex = 0;
             bool voted = false;
             for (;;) { // every exit condition reads one address for all lanes: uniform
-                if (ld_ag(&qd->abort)) {
+                // the four words of one round are loaded before any is tested
+                const uint32_t abort = ld_ag(&qd->abort);
+                const uint64_t head =
+                    __hip_atomic_load(&ctrl->lane[lane].head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                const uint32_t exit_all = ld_ag(&qd->exit_all);
+                const uint32_t stop = ld_sys32(&ctrl->stop);
+                if (abort) {
                     ex = 4;
                     break;
                 }
-                if (__hip_atomic_load(&ctrl->lane[lane].head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) >
-                    consumed) {
+                if (head > consumed) {
                     if (voted && threadIdx.x == 0)
                         __hip_atomic_fetch_sub(&qd->idle_votes, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     voted = false;
                     // the others may have decided to leave meanwhile: leave too
                     // (the host starts the kernel again for this entry)
-                    if (ld_ag(&qd->exit_all)) ex = 2;
+                    if (exit_all) ex = 2;
                     break;
                 }
-                if (ld_ag(&qd->exit_all)) {
+                if (exit_all) {
                     ex = 2;
                     break;
                 }
-                if (ld_sys32(&ctrl->stop)) {
+                if (stop) {
                     ex = 1;
                     break;
                 }
