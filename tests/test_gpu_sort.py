@@ -163,15 +163,17 @@ def test_sort_multi_bit_exact_with_ties(vctx, types, asc, n, npad, distinct):
         assert np.array_equal(from_dev(dc, TYPES[t]).view(np.uint8), ec.view(np.uint8))
 
 
-@pytest.mark.parametrize("tuning", [0x100, 0x5 | (1 << 4), 0x3 | (7 << 4), 0x1 | (1 << 4), 0x4 | (8 << 4)])
+@pytest.mark.parametrize("tuning", [0x100, 0x5 | (1 << 4), 0x3 | (7 << 4), 0x1 | (1 << 4), 0x4 | (8 << 4),
+                                    0x200, 0x200 | 0x3 | (7 << 4), 0x200 | 0x100])
 def test_sort_schedule_variants_bit_exact(tuning):
     """Every schedule variant (fused-stage widths, hybrid on/off, resident
-    merge blocks) applies the same network: bit-exact at 2^20 and 2^17+3."""
+    merge blocks, 1024-thread blocks) applies the same network: bit-exact at
+    2^20, 2^17+3, one full tile and a partial tile."""
     import torch
     import velodb_amd
     print(f"schedule tuning={tuning:#x}", flush=True)
     with velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, tuning=tuning) as ctx:
-        for n, npad in [(1 << 20, 1 << 20), ((1 << 17) + 3, 1 << 18)]:
+        for n, npad in [(1 << 20, 1 << 20), ((1 << 17) + 3, 1 << 18), (1 << 13, 1 << 13), (5000, 8192)]:
             k, p = oracle.gen_sort_input(77 + n, n)
             gk, gp = _run_sort(ctx, k, p, n, npad)
             ek, ep = oracle.bitonic_sort_kp(k, p, n, npad)

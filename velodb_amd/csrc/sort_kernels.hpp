@@ -265,9 +265,12 @@ __device__ __forceinline__ void body_local_ct(Bufs<R> b, IMap map, int p_top, ui
     if constexpr (TRACE) tcommit(slot, (uint64_t)blk * BLOCK + tau, h, nops);
 }
 
+// 1024-thread blocks: two per CU, 8 waves per SIMD (at most 64 VGPRs; the
+// second bound is the minimum waves per execution unit)
 template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
-__global__ __launch_bounds__(BLOCK) void k_sort_local_ct(Bufs<R> b, IMap map, int p_top, uint64_t n_valid, int fill,
-                                                         TraceSlot *slot)
+__global__ __launch_bounds__(BLOCK, BLOCK >= 1024 ? 8 : 1) void k_sort_local_ct(Bufs<R> b, IMap map, int p_top,
+                                                                                uint64_t n_valid, int fill,
+                                                                                TraceSlot *slot)
 {
     extern __shared__ __align__(16) char lds[];
     body_local_ct<R, T_LOG, R_LOG, BLOCK, TRACE>(b, map, p_top, n_valid, fill, slot, lds, blockIdx.x);

@@ -23,11 +23,16 @@ using SortKV = SortImpl<RecKV, 12, 4, 256, 5, 7, 3, 27>;
 // The resident queue kernel runs 256-thread blocks: the 12-byte sort runs
 // there on 2^12-record tiles (the same network, so the same output).
 using SortKP_R = SortImpl<RecKP, 12, 4, 256, 5, 7, 3, 27>;
+// The same tiles with 1024-thread blocks of 8 records per thread (tuning
+// bit 0x200): the same passes at up to twice the waves per SIMD.
+using SortKP_W = SortImpl<RecKP, 13, 3, 1024, 5, 8, 3, 27>;
 
 void run_kp(vo_ctx *ctx, Bufs<RecKP> b, int n_log, uint64_t n_valid, int fill)
 {
     if (ctx->rec)
         SortKP_R::run(ctx, b, n_log, n_valid, fill);
+    else if (ctx->tuning.wide_block)
+        SortKP_W::run(ctx, b, n_log, n_valid, fill);
     else
         SortKP::run(ctx, b, n_log, n_valid, fill);
 }

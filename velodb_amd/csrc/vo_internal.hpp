@@ -66,6 +66,7 @@ struct SortTuning {
     int max_fused_stages = 5;  // stages per register-only global pass (1..5)
     int max_hybrid_stages = 8; // stages per LDS-staged global pass (0 = off, 4..8; used above max_fused_stages)
     int pipe_merge = 0;        // resident merge blocks with next-tile prefetch
+    int wide_block = 0;        // 12-byte sort: 1024-thread blocks, 8 records per thread
 };
 
 struct Recorder;
@@ -174,6 +175,7 @@ inline SortTuning apply_tuning_word(SortTuning t, uint32_t w)
     if (w & 0xF) t.max_fused_stages = (int)(w & 0xF);
     if (w & 0xF0) t.max_hybrid_stages = (int)((w >> 4) & 0xF) - 1;
     if (w & 0x100) t.pipe_merge = !t.pipe_merge;
+    if (w & 0x200) t.wide_block = !t.wide_block;
     return t;
 }
 
