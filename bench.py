@@ -909,7 +909,7 @@ def bench_queue(args):
     print(json.dumps(d), flush=True)
 
 
-def served_workload(slots=16, rates=(0, 800), n_tables=12, check_queries=200, log=None):
+def served_workload(slots=16, rates=(0, 800), n_tables=12, check_queries=200, log=None, queue_lanes=0):
     """The served mix (50% filter+SUM, 30% join+group-by, 20% sort) over
     n_tables tables of each kind with row counts log-uniform in [1e4, 1e7],
     200 GB admission budget.  For each offered load (0 = every query arrives
@@ -917,8 +917,8 @@ def served_workload(slots=16, rates=(0, 800), n_tables=12, check_queries=200, lo
     slots over the resident tables), plain (one query at a time, inputs
     shipped from pinned host memory per query: the reference engine's path)
     plain_resident (one at a time over the resident tables) and served_queue
-    (every query one program in the resident command queue, a lane per slot
-    up to 8 lanes).  The first
+    (every query one program in the resident command queue, slot i on lane
+    i mod lanes, `queue_lanes` lanes or min(slots, 8)).  The first
     `check_queries` queries of the first load then run served again with
     their whole results copied out, each checked against the oracle's closed
     form of its table."""
@@ -927,7 +927,7 @@ def served_workload(slots=16, rates=(0, 800), n_tables=12, check_queries=200, lo
     import oracle
     import velodb_amd
     S = velodb_amd.Server
-    srv = S(n_slots=slots, hbm_budget_bytes=200 << 30)
+    srv = S(n_slots=slots, hbm_budget_bytes=200 << 30, queue_lanes=queue_lanes)
     rng = np.random.default_rng(0)
     tables = {"f": [], "j": [], "s": []}
     for i in range(n_tables):
@@ -1042,7 +1042,7 @@ def bench_serve(args):
                "offered_qps": ld["offered_qps"], "queries": ld["queries"]},
               **{k: v for k, v in ld.items() if k not in ("offered_qps", "queries")})
 
-    res = served_workload(args.slots, rates, log=log)
+    res = served_workload(args.slots, rates, log=log, queue_lanes=args.queue_lanes)
     print(json.dumps({"served_oracle_check": {k: res[k] for k in ("oracle_checked_queries",
                                                                   "results_equal_oracle", "results_equal")}}),
           flush=True)
@@ -1065,6 +1065,8 @@ def main():
                     help="sort = the headline line; the others print their own line")
     ap.add_argument("--autotune", action="store_true", help="batch: run the autotuner first")
     ap.add_argument("--slots", type=int, default=16, help="serve: concurrent query slots")
+    ap.add_argument("--queue-lanes", type=int, default=0,
+                    help="serve: lanes of the resident queue in queue mode (0: min(slots, 8))")
     ap.add_argument("--rates", type=str, default="0,50,200,800",
                     help="serve: offered loads in QPS (0 = every query arrives at once)")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -622,6 +622,9 @@ typedef struct vo_server_config {
     uint32_t n_slots;          /* concurrent queries (1 = the plain loop) */
     uint32_t device;
     uint64_t hbm_budget_bytes; /* admission budget for query intermediates */
+    uint32_t queue_lanes;      /* VO_RUN_QUEUE: lanes of the resident queue, 1..8 (0: min(slots, 8));
+                                  slot i runs on lane i mod lanes */
+    uint32_t reserved;         /* 0 */
 } vo_server_config;
 
 typedef struct vo_query {

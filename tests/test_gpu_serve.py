@@ -8,9 +8,9 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _server(n_slots=8, budget=200 << 30):
+def _server(n_slots=8, budget=200 << 30, queue_lanes=0):
     import velodb_amd
-    s = velodb_amd.Server(n_slots=n_slots, hbm_budget_bytes=budget)
+    s = velodb_amd.Server(n_slots=n_slots, hbm_budget_bytes=budget, queue_lanes=queue_lanes)
     rng = np.random.default_rng(0)
     tables = {"f": [], "j": [], "s": []}
     for i in range(4):
@@ -133,3 +133,21 @@ def test_queue_mode_matches_the_graph_loop():
         assert [list(r.r) for r in again] == [list(r.r) for r in want]
     finally:
         s.close()
+
+
+def test_queue_mode_with_fewer_lanes_than_slots():
+    """queue_lanes = 2 with 8 slots: slots share lanes (slot i on lane i mod
+    2), the commands of one lane run in order, and the results are the graph
+    loop's; more than 8 lanes is rejected."""
+    import velodb_amd
+    s, tables = _server(n_slots=8, queue_lanes=2)
+    try:
+        qs = _mix(tables, 40, 0, 11)
+        want, _ = s.run(qs)
+        got, _ = s.run(qs, queue=True)
+        assert all(r.status == 0 for r in got)
+        assert [list(r.r) for r in got] == [list(r.r) for r in want]
+    finally:
+        s.close()
+    with pytest.raises(velodb_amd.VoError):
+        velodb_amd.Server(n_slots=2, queue_lanes=9)

@@ -64,9 +64,9 @@ class Server:
 
     FILTER_SUM, JOIN_GROUPSUM, SORT = _abi.Q_FILTER_SUM, _abi.Q_JOIN_GROUPSUM, _abi.Q_SORT
 
-    def __init__(self, n_slots=16, hbm_budget_bytes=200 << 30, device=0):
+    def __init__(self, n_slots=16, hbm_budget_bytes=200 << 30, device=0, queue_lanes=0):
         self.lib = load_library()
-        cfg = _abi.ServerConfig(n_slots, device, hbm_budget_bytes)
+        cfg = _abi.ServerConfig(n_slots, device, hbm_budget_bytes, queue_lanes, 0)
         h = C.c_void_p()
         st = self.lib.vo_server_create(C.byref(cfg), C.byref(h))
         if st != 0:

@@ -180,7 +180,8 @@ RUN_RESIDENT, RUN_TRANSFER, RUN_QUEUE = 0, 1, 2
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("n_slots", u32), ("device", u32), ("hbm_budget_bytes", u64)]
+    _fields_ = [("n_slots", u32), ("device", u32), ("hbm_budget_bytes", u64), ("queue_lanes", u32),
+                ("reserved", u32)]
 
 
 class Query(C.Structure):

@@ -135,6 +135,7 @@ using namespace vo;
 struct vo_server {
     int device = 0;
     uint32_t n_slots = 1;
+    uint32_t queue_lanes = 0; // VO_RUN_QUEUE lanes (0: min(slots, 8))
     uint64_t budget = 0;
     std::string last_error;
     std::vector<vo_ctx *> slots;
@@ -572,11 +573,12 @@ extern "C" {
 
 int vo_server_create(const vo_server_config *cfg, vo_server **out)
 {
-    if (!cfg || !out) return VO_ERR_INVALID;
+    if (!cfg || !out || cfg->queue_lanes > 8) return VO_ERR_INVALID;
     *out = nullptr;
     vo_server *s = new vo_server();
     s->device = (int)cfg->device;
     s->n_slots = std::max<uint32_t>(1, cfg->n_slots);
+    s->queue_lanes = cfg->queue_lanes;
     s->budget = cfg->hbm_budget_bytes ? cfg->hbm_budget_bytes : (200ull << 30);
     int st = sguard(s, [&] {
         for (uint32_t i = 0; i < s->n_slots; ++i) slot_ctx(s, i);
@@ -752,7 +754,7 @@ int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_s
         const bool transfer = mode == VO_RUN_TRANSFER;
         const bool queued = mode == VO_RUN_QUEUE;
         const uint32_t slots = n_slots_override ? n_slots_override : s->n_slots;
-        const uint32_t lanes = std::min<uint32_t>(slots, 8);
+        const uint32_t lanes = std::min<uint32_t>(slots, s->queue_lanes ? s->queue_lanes : 8);
         // size every slot for the largest query it may take (no allocation while timing)
         std::vector<uint64_t> fp(n);
         uint64_t max_buf = 64, max_scr = 1 << 20, max_hbuf = 0;
