@@ -514,7 +514,11 @@ inline std::vector<Pass> plan_passes(int n_log, int t_log, int s_max, int r_log,
 
 // S_LARGE / LARGE_LOG: from n_pad = 2^LARGE_LOG up, register passes take at
 // most S_LARGE stages (wider groups go to the hybrid pass); below, S_MAX.
-template <class R, int T_LOG, int R_LOG, int BLOCK, int S_MAX, int H_MAX = 0, int S_LARGE = S_MAX, int LARGE_LOG = 64>
+// Hybrid passes of at most HW_MAX stages run with twice the threads per block
+// and half the records per thread (the same tiles, so the same records per
+// block): more waves per SIMD where that pass is faster.
+template <class R, int T_LOG, int R_LOG, int BLOCK, int S_MAX, int H_MAX = 0, int S_LARGE = S_MAX, int LARGE_LOG = 64,
+          int HW_MAX = 0>
 struct SortImpl {
     static_assert(T_LOG >= 8, "global passes need q_lo >= 8 (256-thread blocks)");
     static_assert(H_MAX == 0 || (T_LOG - H_MAX >= 5 && BLOCK << R_LOG == 1 << T_LOG), "hybrid tile shape");
@@ -531,14 +535,17 @@ struct SortImpl {
     static void launch_hybrid(vo_ctx *ctx, Bufs<R> b, IMap map, int p, int q_lo, int p_top, uint64_t n_virtual)
     {
         if constexpr (S <= H_MAX && T_LOG - S >= 5) {
-            using TL = dev::TileCT<R, T_LOG, R_LOG, TRACE>;
+            constexpr bool wide = S <= HW_MAX;
+            constexpr int RL = wide ? R_LOG - 1 : R_LOG;
+            constexpr int BL = wide ? 2 * BLOCK : BLOCK;
+            using TL = dev::TileCT<R, T_LOG, RL, TRACE>;
             const size_t lds = TL::lds_bytes();
             static_assert(S >= 4 && S <= 8, "hybrid pass widths 4..8");
             static const char *const names[] = {"sort_hybrid_s4", "sort_hybrid_s5", "sort_hybrid_s6", "sort_hybrid_s7",
                                                 "sort_hybrid_s8"};
-            LaunchInfo li{names[S - 4], dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BLOCK), (uint32_t)lds,
+            LaunchInfo li{names[S - 4], dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BL), (uint32_t)lds,
                           hash_mix(hash_mix(p, q_lo), hash_mix(S + 100, p_top))};
-            auto k = dev::k_sort_hybrid<R, T_LOG, R_LOG, BLOCK, S, TRACE>;
+            auto k = dev::k_sort_hybrid<R, T_LOG, RL, BL, S, TRACE>;
             VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             launch(ctx, li, k, b, map, p, q_lo, p_top, trace_slot(ctx));
         } else {

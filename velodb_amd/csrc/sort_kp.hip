@@ -16,7 +16,9 @@ const bool kRegistered = (register_bodies(kBodies, sizeof kBodies / sizeof kBodi
 // to 3 from 2^27 on, where groups of 4 and 5 stages run faster as LDS-staged
 // passes (2^28: 5455 vs 5162 and 5165 vs 4955 GB/s); below 2^27 the register
 // passes win (2^25: 5.06 vs 5.13 ms per sort, 2^23: 1.107 vs 1.128 ms).
-using SortKP = SortImpl<RecKP, 13, 4, 512, 5, 8, 3, 27>;
+// Hybrid passes of 4-6 stages with 1024-thread blocks: at 2^28, 5494 -> 5611,
+// 5147 -> 5216 and 5319 -> 5483 GB/s (7 and 8 stages are faster at 512).
+using SortKP = SortImpl<RecKP, 13, 4, 512, 5, 8, 3, 27, 6>;
 // the 16-byte sort likewise: at 2^28 its 4-5 stage register passes ran at
 // 4.80-4.99 TB/s against 5.11-5.36 for its LDS-staged passes
 using SortKV = SortImpl<RecKV, 12, 4, 256, 5, 7, 3, 27>;
