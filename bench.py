@@ -169,7 +169,8 @@ def bench_filter(args):
     day = torch.empty(n_pad, dtype=torch.int32, device=dev)
     qty, price, disc, op, od = (torch.empty(n_pad, dtype=torch.int64, device=dev) for _ in range(5))
     out = torch.empty(4, dtype=torch.int64, device=dev)
-    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING,
+                             tuning=args.tuning)
     ctx.gen_fact(1, day, qty, price, disc, n)
 
     def step():
@@ -266,7 +267,8 @@ def bench_join(args):
     og = torch.empty(64, dtype=torch.int32, device=dev)
     osum = torch.empty(64, dtype=torch.int64, device=dev)
     on = torch.empty(1, dtype=torch.int64, device=dev)
-    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING,
+                             tuning=args.tuning)
 
     def step(dense):
         # the group attribute's public domain is 0..24: dense GROUP BY over 25 ids
@@ -363,7 +365,8 @@ def bench_join_pairs(args):
     cap = 1 << 23
     ol = torch.empty(cap, dtype=torch.int64, device=dev)
     orr = torch.empty(cap, dtype=torch.int64, device=dev)
-    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING,
+                             tuning=args.tuning)
 
     def step():
         n = ctx.join_pairs_count(d_dk, n_dim, d_fk, n_fact, _abi.T_INTEGER)
@@ -481,7 +484,8 @@ def bench_gather(args):
     ids = torch.randint(0, nt, (n,), device="cuda:0", generator=g, dtype=torch.int64)
     ids[: n // 8] = 7
     out = torch.empty(n, dtype=torch.int64, device="cuda:0")
-    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING,
+                             tuning=args.tuning)
 
     def step():
         ctx.gather_oblivious(table, nt, ids, n, out, 8)
@@ -538,7 +542,8 @@ def bench_order_by(args):
     a, b = a0.clone(), b0.clone()
     idx = torch.empty(n_pad, dtype=torch.int64, device="cuda:0")
     o1, o2 = torch.empty_like(p1), torch.empty_like(p2)
-    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING,
+                             tuning=args.tuning)
 
     def step():
         a.copy_(a0)
@@ -592,7 +597,8 @@ def bench_shuffle(args):
     ident = torch.arange(n_pad, device=dev, dtype=torch.int64).to(torch.int32)
     pay = torch.empty_like(ident)
     perm = torch.empty(n_pad, dtype=torch.int32, device=dev)
-    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING,
+                             tuning=args.tuning)
 
     def timed(seed0, with_perm):
         t_ms = 0.0

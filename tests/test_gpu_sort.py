@@ -164,7 +164,7 @@ def test_sort_multi_bit_exact_with_ties(vctx, types, asc, n, npad, distinct):
 
 
 @pytest.mark.parametrize("tuning", [0x100, 0x5 | (1 << 4), 0x3 | (7 << 4), 0x1 | (1 << 4), 0x4 | (8 << 4),
-                                    0x200, 0x200 | 0x3 | (7 << 4), 0x200 | 0x100])
+                                    0x200, 0x200 | 0x3 | (7 << 4), 0x200 | 0x100, 0x400])
 def test_sort_schedule_variants_bit_exact(tuning):
     """Every schedule variant (fused-stage widths, hybrid on/off, resident
     merge blocks, 1024-thread blocks) applies the same network: bit-exact at

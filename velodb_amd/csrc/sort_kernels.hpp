@@ -531,23 +531,32 @@ struct SortImpl {
     }
     static int hmax(vo_ctx *ctx) { return std::min(H_MAX, ctx->tuning.max_hybrid_stages); }
 
+    template <bool TRACE, int S, int RL, int BL>
+    static void hybrid_shape(vo_ctx *ctx, Bufs<R> b, IMap map, int p, int q_lo, int p_top, uint64_t n_virtual)
+    {
+        using TL = dev::TileCT<R, T_LOG, RL, TRACE>;
+        const size_t lds = TL::lds_bytes();
+        static const char *const names[] = {"sort_hybrid_s4", "sort_hybrid_s5", "sort_hybrid_s6", "sort_hybrid_s7",
+                                            "sort_hybrid_s8"};
+        LaunchInfo li{names[S - 4], dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BL), (uint32_t)lds,
+                      hash_mix(hash_mix(p, q_lo), hash_mix(S + 100, p_top))};
+        auto k = dev::k_sort_hybrid<R, T_LOG, RL, BL, S, TRACE>;
+        VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        launch(ctx, li, k, b, map, p, q_lo, p_top, trace_slot(ctx));
+    }
+
     template <bool TRACE, int S>
     static void launch_hybrid(vo_ctx *ctx, Bufs<R> b, IMap map, int p, int q_lo, int p_top, uint64_t n_virtual)
     {
         if constexpr (S <= H_MAX && T_LOG - S >= 5) {
-            constexpr bool wide = S <= HW_MAX;
-            constexpr int RL = wide ? R_LOG - 1 : R_LOG;
-            constexpr int BL = wide ? 2 * BLOCK : BLOCK;
-            using TL = dev::TileCT<R, T_LOG, RL, TRACE>;
-            const size_t lds = TL::lds_bytes();
             static_assert(S >= 4 && S <= 8, "hybrid pass widths 4..8");
-            static const char *const names[] = {"sort_hybrid_s4", "sort_hybrid_s5", "sort_hybrid_s6", "sort_hybrid_s7",
-                                                "sort_hybrid_s8"};
-            LaunchInfo li{names[S - 4], dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BL), (uint32_t)lds,
-                          hash_mix(hash_mix(p, q_lo), hash_mix(S + 100, p_top))};
-            auto k = dev::k_sort_hybrid<R, T_LOG, RL, BL, S, TRACE>;
-            VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            launch(ctx, li, k, b, map, p, q_lo, p_top, trace_slot(ctx));
+            // tuning may flip the shape (both are built when the wide one is)
+            const bool wide = (S <= HW_MAX) != (HW_MAX > 0 && ctx->tuning.flip_hybrid_block);
+            if (wide) {
+                if constexpr (HW_MAX > 0) hybrid_shape<TRACE, S, R_LOG - 1, 2 * BLOCK>(ctx, b, map, p, q_lo, p_top, n_virtual);
+            } else {
+                hybrid_shape<TRACE, S, R_LOG, BLOCK>(ctx, b, map, p, q_lo, p_top, n_virtual);
+            }
         } else {
             throw Error{VO_ERR_INVALID, "hybrid pass width not built"};
         }

@@ -21,6 +21,8 @@ const bool kRegistered = (register_bodies(kBodies, sizeof kBodies / sizeof kBodi
 using SortKP = SortImpl<RecKP, 13, 4, 512, 5, 8, 3, 27, 6>;
 // the 16-byte sort likewise: at 2^28 its 4-5 stage register passes ran at
 // 4.80-4.99 TB/s against 5.11-5.36 for its LDS-staged passes
+// (512-thread hybrid passes measured no faster for it: 78.04-78.05 vs 78.06-78.14 ms per
+// 2^28-row shuffle with permutation output)
 using SortKV = SortImpl<RecKV, 12, 4, 256, 5, 7, 3, 27>;
 // The resident queue kernel runs 256-thread blocks: the 12-byte sort runs
 // there on 2^12-record tiles (the same network, so the same output).

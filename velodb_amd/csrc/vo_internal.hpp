@@ -67,6 +67,7 @@ struct SortTuning {
     int max_hybrid_stages = 8; // stages per LDS-staged global pass (0 = off, 4..8; used above max_fused_stages)
     int pipe_merge = 0;        // resident merge blocks with next-tile prefetch
     int wide_block = 0;        // 12-byte sort: 1024-thread blocks, 8 records per thread
+    int flip_hybrid_block = 0; // 12-byte sort's hybrid passes: the other block shape (twice / half the threads)
 };
 
 struct Recorder;
@@ -176,6 +177,7 @@ inline SortTuning apply_tuning_word(SortTuning t, uint32_t w)
     if (w & 0xF0) t.max_hybrid_stages = (int)((w >> 4) & 0xF) - 1;
     if (w & 0x100) t.pipe_merge = !t.pipe_merge;
     if (w & 0x200) t.wide_block = !t.wide_block;
+    if (w & 0x400) t.flip_hybrid_block = !t.flip_hybrid_block;
     return t;
 }
 
