@@ -145,12 +145,20 @@ __device__ __forceinline__ void b_og_carry(const Last *__restrict__ in, uint32_t
     dev::AtrT<TRACE> tr(slot);
     const uint32_t per = (nt + NT - 1) / NT;
     const uint32_t lo = threadIdx.x * per;
+    // a thread's chunk in groups of 8, loaded before any is combined
+    constexpr int G = 8;
     Last s{NONE, 0};
-    for (uint32_t k = 0; k < per; ++k)
-        if (lo + k < nt) { // public size
-            s = LastOp{}(s, in[lo + k]);
-            tr.at(1, lo + k);
-        }
+    for (uint32_t k0 = 0; k0 < per; k0 += G) {
+        Last v[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) v[j] = (k0 + j < per && lo + k0 + j < nt) ? in[lo + k0 + j] : Last{NONE, 0};
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (k0 + j < per && lo + k0 + j < nt) { // public size
+                s = LastOp{}(s, v[j]);
+                tr.at(1, lo + k0 + j);
+            }
+    }
     sh[threadIdx.x] = s;
     __syncthreads();
     for (uint32_t off = 1; off < NT; off <<= 1) {
@@ -160,12 +168,18 @@ __device__ __forceinline__ void b_og_carry(const Last *__restrict__ in, uint32_t
         __syncthreads();
     }
     Last run = threadIdx.x ? sh[threadIdx.x - 1] : Last{NONE, 0};
-    for (uint32_t k = 0; k < per; ++k)
-        if (lo + k < nt) {
-            out[lo + k] = run;
-            run = LastOp{}(run, in[lo + k]);
-            tr.at(2, lo + k);
-        }
+    for (uint32_t k0 = 0; k0 < per; k0 += G) {
+        Last v[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) v[j] = (k0 + j < per && lo + k0 + j < nt) ? in[lo + k0 + j] : Last{NONE, 0};
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (k0 + j < per && lo + k0 + j < nt) {
+                out[lo + k0 + j] = run;
+                run = LastOp{}(run, v[j]);
+                tr.at(2, lo + k0 + j);
+            }
+    }
     tr.done(dev::gtid());
 }
 

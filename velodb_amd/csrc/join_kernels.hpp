@@ -196,12 +196,20 @@ __device__ void block_exclusive_scan(const T *__restrict__ in, uint32_t nt, T *_
 {
     const uint32_t per = (nt + NT - 1) / NT;
     const uint32_t lo = threadIdx.x * per;
+    // a thread's chunk in groups of 8, loaded before any is combined
+    constexpr int G = 8;
     T s = id;
-    for (uint32_t k = 0; k < per; ++k)
-        if (lo + k < nt) { // public size
-            s = op(s, in[lo + k]);
-            tr.at(1, lo + k);
-        }
+    for (uint32_t k0 = 0; k0 < per; k0 += G) {
+        T v[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) v[j] = (k0 + j < per && lo + k0 + j < nt) ? in[lo + k0 + j] : id;
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (k0 + j < per && lo + k0 + j < nt) { // public size
+                s = op(s, v[j]);
+                tr.at(1, lo + k0 + j);
+            }
+    }
     sh[threadIdx.x] = s;
     __syncthreads();
     for (uint32_t off = 1; off < NT; off <<= 1) {
@@ -211,12 +219,18 @@ __device__ void block_exclusive_scan(const T *__restrict__ in, uint32_t nt, T *_
         __syncthreads();
     }
     T run = threadIdx.x ? sh[threadIdx.x - 1] : id; // exclusive prefix of this chunk
-    for (uint32_t k = 0; k < per; ++k)
-        if (lo + k < nt) {
-            out[lo + k] = run;
-            run = op(run, in[lo + k]);
-            tr.at(2, lo + k);
-        }
+    for (uint32_t k0 = 0; k0 < per; k0 += G) {
+        T v[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) v[j] = (k0 + j < per && lo + k0 + j < nt) ? in[lo + k0 + j] : id;
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (k0 + j < per && lo + k0 + j < nt) {
+                out[lo + k0 + j] = run;
+                run = op(run, v[j]);
+                tr.at(2, lo + k0 + j);
+            }
+    }
 }
 
 struct FillOp {

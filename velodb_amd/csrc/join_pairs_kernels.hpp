@@ -162,12 +162,20 @@ __device__ void tile_scan(const T *__restrict__ in, uint32_t nt, T *__restrict__
     const uint32_t per = (nt + NT - 1) / NT;
     const uint32_t lo = threadIdx.x * per;
     auto at = [&](uint32_t x) { return REV ? nt - 1 - x : x; };
+    // a thread's chunk in groups of 8, loaded before any is combined
+    constexpr int G = 8;
     T s = id;
-    for (uint32_t k = 0; k < per; ++k)
-        if (lo + k < nt) { // public size
-            s = op(s, in[at(lo + k)]);
-            tr.at(1, at(lo + k));
-        }
+    for (uint32_t k0 = 0; k0 < per; k0 += G) {
+        T v[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) v[j] = (k0 + j < per && lo + k0 + j < nt) ? in[at(lo + k0 + j)] : id;
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (k0 + j < per && lo + k0 + j < nt) { // public size
+                s = op(s, v[j]);
+                tr.at(1, at(lo + k0 + j));
+            }
+    }
     sh[threadIdx.x] = s;
     __syncthreads();
     for (uint32_t off = 1; off < NT; off <<= 1) {
@@ -177,12 +185,18 @@ __device__ void tile_scan(const T *__restrict__ in, uint32_t nt, T *__restrict__
         __syncthreads();
     }
     T run = threadIdx.x ? sh[threadIdx.x - 1] : id;
-    for (uint32_t k = 0; k < per; ++k)
-        if (lo + k < nt) {
-            out[at(lo + k)] = run;
-            run = op(run, in[at(lo + k)]);
-            tr.at(2, at(lo + k));
-        }
+    for (uint32_t k0 = 0; k0 < per; k0 += G) {
+        T v[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) v[j] = (k0 + j < per && lo + k0 + j < nt) ? in[at(lo + k0 + j)] : id;
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (k0 + j < per && lo + k0 + j < nt) {
+                out[at(lo + k0 + j)] = run;
+                run = op(run, v[j]);
+                tr.at(2, at(lo + k0 + j));
+            }
+    }
 }
 
 struct Min2 {
