@@ -1069,9 +1069,18 @@ static void test_task_manager_queue(ExecutionContext &)
     CHECK(!tm.isRunning());
 }
 
-int main()
+// --persistent: the shared TaskManager runs every operator's commands in the
+// resident queue kernel (TaskManager::start with persistent = true), as a
+// reference caller with a persistent TaskManager would.
+int main(int argc, char **argv)
 {
-    TaskManager tm;
+    const bool persistent = argc > 1 && std::string(argv[1]) == "--persistent";
+    TaskManager tm(1024, persistent);
+    if (persistent && !tm.start()) {
+        std::printf("FAILED: the resident queue did not start\n");
+        return 1;
+    }
+    std::printf("mode: %s\n", persistent ? "persistent (resident queue)" : "launch");
     ExecutionContext ctx(tm);
     const std::vector<std::pair<const char *, std::function<void()>>> tests = {
         {"order_by_integer", [&] { test_order_by_integer(ctx); }},
@@ -1102,14 +1111,24 @@ int main()
         const int before = g_fail;
         std::printf("RUN  %s\n", t.first);
         std::fflush(stdout);
+        // these two start a resident queue of their own: the shared one is
+        // stopped around them, so one resident kernel holds the GPU at a time
+        const bool own_queue = std::string(t.first) == "operators_over_resident_queue" ||
+                               std::string(t.first) == "task_manager_queue";
         try {
+            if (persistent && own_queue) CHECK(tm.stop());
             t.second();
+            if (persistent && own_queue) CHECK(tm.start());
         } catch (const std::exception &e) {
             std::printf("  exception: %s\n", e.what());
             ++g_fail;
         }
         std::printf("%s %s\n", g_fail == before ? "PASS" : "FAIL", t.first);
         std::fflush(stdout);
+    }
+    if (persistent && !tm.stop()) {
+        std::printf("  the resident queue did not stop\n");
+        ++g_fail;
     }
     std::printf("%s: %d failed checks\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;

@@ -12,9 +12,12 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_operators")
 
 
-def test_cpp_operator_suite():
+@pytest.mark.parametrize("mode", [[], ["--persistent"]], ids=["launch", "persistent"])
+def test_cpp_operator_suite(mode):
+    """The whole suite with a launch-mode TaskManager, then with a persistent
+    one whose resident queue kernel runs every operator's commands."""
     assert os.path.exists(BIN), "tests/cpp/test_operators is not built (__graft_entry__.build())"
-    r = subprocess.run([BIN], capture_output=True, text=True, timeout=110)
+    r = subprocess.run([BIN] + mode, capture_output=True, text=True, timeout=110)
     print(r.stdout)
     print(r.stderr)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
