@@ -41,6 +41,15 @@ static int g_fail = 0;
         }                                                                                                              \
     } while (0)
 
+// TaskManager::start with the library's message on failure.
+#define CHECK_START(tm)                                                                                                \
+    do {                                                                                                               \
+        if (!(tm).start()) {                                                                                           \
+            std::printf("  CHECK failed: start (%s:%d): %s\n", __FILE__, __LINE__, vo_last_error((tm).native()));      \
+            ++g_fail;                                                                                                  \
+        }                                                                                                              \
+    } while (0)
+
 // employees (test_order_by.cpp:35-42)
 static const std::vector<int32_t> kId = {3, 1, 5, 2, 4, 6, 7, 8};
 static const std::vector<const char *> kName = {"Charlie", "Alice", "Eve", "Bob", "David", "Frank", "Grace", "Helen"};
@@ -922,7 +931,7 @@ static void test_oblivious_table_manager(TaskManager &tm)
 static void test_operators_over_resident_queue()
 {
     TaskManager tm(64, true);
-    CHECK(tm.start());
+    CHECK_START(tm);
     ExecutionContext ctx(tm);
     std::vector<int32_t> id = {0, 1, 2, 3, 4}, x = {0, 10, 20, 30, 40}, y = {100, 101, 102, 103, 104};
     SortMergeJoinOperator j(ctx, std::make_unique<BatchSourceOperator>(ctx, id_value(id, x)),
@@ -1046,7 +1055,7 @@ static void test_task_manager_queue(ExecutionContext &)
 {
     // TaskManager::start / submitCommand / waitCommand / stop with the resident kernel
     TaskManager tm(64, true);
-    CHECK(tm.start());
+    CHECK_START(tm);
     CHECK(tm.isRunning());
     const size_t n = 5000, n_pad = 8192;
     std::vector<uint64_t> keys(n);
@@ -1077,7 +1086,7 @@ int main(int argc, char **argv)
     const bool persistent = argc > 1 && std::string(argv[1]) == "--persistent";
     TaskManager tm(1024, persistent);
     if (persistent && !tm.start()) {
-        std::printf("FAILED: the resident queue did not start\n");
+        std::printf("FAILED: the resident queue did not start: %s\n", vo_last_error(tm.native()));
         return 1;
     }
     std::printf("mode: %s\n", persistent ? "persistent (resident queue)" : "launch");
@@ -1118,7 +1127,7 @@ int main(int argc, char **argv)
         try {
             if (persistent && own_queue) CHECK(tm.stop());
             t.second();
-            if (persistent && own_queue) CHECK(tm.start());
+            if (persistent && own_queue) CHECK_START(tm);
         } catch (const std::exception &e) {
             std::printf("  exception: %s\n", e.what());
             ++g_fail;

@@ -1,7 +1,13 @@
 """C++ operator layer (include/velodb_amd/operators.hpp: TaskManager,
 SortOperator, FilterCompactionOperator, PKFKJoinGroupSumOperator) on the
 GPU: runs tests/cpp/test_operators, the ORDER BY cases of the reference's
-tests/query/test_order_by.cpp plus join and queue checks."""
+tests/query/test_order_by.cpp plus join and queue checks.
+
+The binary starts resident queues (cooperative launches).  On this runtime a
+process cannot make a cooperative launch on a GPU once another process has
+made one there, so this module must run before any test in the same pytest
+process that starts a queue; file order (test_gpu_cpp... before
+test_gpu_queue/test_gpu_resident/test_gpu_serve) gives that."""
 import os
 import subprocess
 
