@@ -285,3 +285,62 @@ def test_queue_parks_for_launch_only_call(qctx):
 def test_queue_start_without_persistent_flag_is_launch_mode(vctx):
     vctx.queue_start()  # no-op: commands keep launching on the context stream
     assert not vctx.queue_running()
+
+
+_SECOND_PROCESS = r'''
+import sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import torch
+import oracle
+import velodb_amd
+from velodb_amd import _abi
+from tests.gpu_util import to_dev
+
+ctx = velodb_amd.Context(flags=_abi.CFG_PERSISTENT)
+try:
+    ctx.queue_start()
+    print("started", flush=True)
+    ctx.queue_stop()
+except velodb_amd.VoError as e:
+    print("refused:", e, flush=True)
+n, npad = 3000, 4096
+k, p = oracle.gen_sort_input(9, n)
+dk, dp = to_dev(k, npad), to_dev(p, npad)
+torch.cuda.current_stream().synchronize()
+ctx.sort_kp(dk, dp, n, npad)  # launch mode when the queue did not start
+ek, ep = oracle.bitonic_sort_kp(k, p, n, npad)
+ok = np.array_equal(dk.cpu().numpy().view(np.uint64), ek) and np.array_equal(dp.cpu().numpy().view(np.uint32), ep)
+print("sort ok" if ok else "sort differs", flush=True)
+ctx.close()
+'''
+
+
+def test_queue_start_in_a_second_process(torch_dev):
+    """After this process has made a cooperative launch, a child process's
+    vo_queue_start either starts its own queue or (on this runtime, where
+    cooperative launches are one process per GPU) is refused with an error
+    that names the likely cause.  Either way the child's context then runs
+    launch-mode work with the oracle's bytes: the refused launch leaves no
+    stale HIP error behind for the next launch to report."""
+    import os
+    import subprocess
+    import sys
+
+    import velodb_amd
+    from velodb_amd import _abi
+    ctx = velodb_amd.Context(flags=_abi.CFG_PERSISTENT)
+    ctx.queue_start()
+    cmd = _abi.Command()
+    cmd.opcode = _abi.OP_NOP
+    ctx.wait(ctx.submit(cmd), 20000)
+    ctx.queue_stop()
+    ctx.close()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print("second process: queue_start, then a launch-mode sort", flush=True)
+    r = subprocess.run([sys.executable, "-c", _SECOND_PROCESS, root], capture_output=True, text=True, timeout=150)
+    print(r.stdout, r.stderr[-2000:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "sort ok" in r.stdout
+    if "refused" in r.stdout:
+        assert "another process" in r.stdout

@@ -631,8 +631,17 @@ void launch_queue_kernel(vo_ctx *ctx, QueueState *q)
     QArgs qa{q->ring, q->ctrl, q->qd, q->dprog, q->flag, q->cap, q->cap_lane, q->lanes, 0, q->idle_ticks,
              q->bar_ticks};
     void *args[] = {&qa};
-    VO_HIP(hipLaunchCooperativeKernel((const void *)k_queue, dim3(q->grid), dim3(RB), args, (unsigned)q->lds,
-                                      q->stream));
+    const hipError_t le =
+        hipLaunchCooperativeKernel((const void *)k_queue, dim3(q->grid), dim3(RB), args, (unsigned)q->lds, q->stream);
+    if (le != hipSuccess) {
+        (void)hipGetLastError();
+        std::string m = std::string("queue kernel launch: ") + hipGetErrorString(le);
+        // seen when another process has made a cooperative launch on this GPU
+        if (le == hipErrorIllegalState)
+            m += " (is another process running resident queues on this GPU? cooperative launches are one process "
+                 "per GPU on this runtime)";
+        throw Error{VO_ERR_HIP, m};
+    }
     // wait until the kernel reports that it runs
     const auto t0 = std::chrono::steady_clock::now();
     while (vctrl(q)->alive == 0) {

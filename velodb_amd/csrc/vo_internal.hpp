@@ -22,11 +22,16 @@ struct Error {
     std::string msg;
 };
 
+// A failed call's error stays the thread's last HIP error until read: it is
+// read here, so the next launch's hipGetLastError check does not report it
+// again as its own.
 #define VO_HIP(expr)                                                                              \
     do {                                                                                          \
         hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
+        if (e_ != hipSuccess) {                                                                   \
+            (void)hipGetLastError();                                                              \
             throw ::vo::Error{VO_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)};     \
+        }                                                                                         \
     } while (0)
 
 #define VO_CHECK(cond, status, msg)                                                               \
