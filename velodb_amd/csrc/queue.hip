@@ -814,29 +814,34 @@ uint64_t queue_publish_locked(vo_ctx *ctx, uint32_t lane, Recorder &rec)
     return publish_locked(ctx, q, lane, kOpProgram, &rec);
 }
 
-int queue_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms)
+void queue_wait_throw(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms)
 {
     QueueState *q = (QueueState *)ctx->queue;
-    try {
-        if (timeout_ms == 0) timeout_ms = ctx->wait_timeout_ms;
-        VO_CHECK(seq >= 1 && seq <= q->submitted, VO_ERR_INVALID, "wait on a sequence id never submitted");
-        // Spin, then yield, then sleep: a sleep lasts the timer slack (tens of
-        // microseconds) at least, longer than most resident commands.
-        const auto t0 = std::chrono::steady_clock::now();
-        for (uint64_t spins = 0; !seq_done(q, seq); ++spins) {
-            if (vctrl(q)->alive == 0 || vctrl(q)->error) {
-                std::lock_guard<std::mutex> lk(ctx->mu);
-                ensure_running_locked(ctx, q);
-            }
-            if ((spins & 63) != 63) continue;
-            const auto el = std::chrono::steady_clock::now() - t0;
-            if (el > std::chrono::milliseconds(timeout_ms))
-                throw Error{VO_ERR_TIMEOUT, "vo_wait: command " + std::to_string(seq) + " not finished"};
-            if (el > std::chrono::milliseconds(2))
-                std::this_thread::sleep_for(std::chrono::microseconds(20));
-            else if (el > std::chrono::microseconds(50))
-                std::this_thread::yield();
+    if (timeout_ms == 0) timeout_ms = ctx->wait_timeout_ms;
+    VO_CHECK(seq >= 1 && seq <= q->submitted, VO_ERR_INVALID, "wait on a sequence id never submitted");
+    // Spin, then yield, then sleep: a sleep lasts the timer slack (tens of
+    // microseconds) at least, longer than most resident commands.
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t spins = 0; !seq_done(q, seq); ++spins) {
+        if (vctrl(q)->alive == 0 || vctrl(q)->error) {
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            ensure_running_locked(ctx, q);
         }
+        if ((spins & 63) != 63) continue;
+        const auto el = std::chrono::steady_clock::now() - t0;
+        if (el > std::chrono::milliseconds(timeout_ms))
+            throw Error{VO_ERR_TIMEOUT, "vo_wait: command " + std::to_string(seq) + " not finished"};
+        if (el > std::chrono::milliseconds(2))
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        else if (el > std::chrono::microseconds(50))
+            std::this_thread::yield();
+    }
+}
+
+int queue_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms)
+{
+    try {
+        queue_wait_throw(ctx, seq, timeout_ms);
         return VO_OK;
     } catch (const Error &e) {
         return fail_q(ctx, e);

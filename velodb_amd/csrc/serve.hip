@@ -18,6 +18,7 @@
 #include "serve_kernels.hpp"
 
 #include <map>
+#include <new>
 
 #include <algorithm>
 #include <chrono>
@@ -496,8 +497,7 @@ void run_query_queue(vo_server *s, uint32_t slot, uint32_t lane, const vo_query 
         ctx->rec = nullptr;
         seq = queue_publish_locked(ctx, lane, rec);
     }
-    const int st = queue_wait(ctx, seq, 0);
-    if (st != VO_OK) throw Error{st, ctx->last_error};
+    queue_wait_throw(ctx, seq, 0); // several slot threads wait at once: no shared last_error
     finish_query(s, q, res, r);
     copy_out(s, ctx, q, res, loc);
 }
@@ -850,6 +850,16 @@ int vo_server_run_mode(vo_server *s, const vo_query *q, uint64_t n, uint32_t n_s
                     if (first_error == VO_OK) {
                         first_error = e.status;
                         err_msg = e.msg;
+                    }
+                } catch (const std::exception &e) {
+                    // nothing may leave a worker thread (std::terminate); the
+                    // status follows sguard's
+                    const int st = dynamic_cast<const std::bad_alloc *>(&e) ? VO_ERR_NOMEM : VO_ERR_INVALID;
+                    r.status = st;
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_error == VO_OK) {
+                        first_error = st;
+                        err_msg = std::string("serving worker: ") + e.what();
                     }
                 }
                 r.end_s = now_s();

@@ -227,6 +227,9 @@ void autotune_sort(vo_ctx *ctx, int log_min, int log_max, uint64_t rows_per_clas
 // ---- resident command queue (queue.hip); used when ctx->queue is set
 uint64_t queue_submit(vo_ctx *ctx, vo_command *cmd);
 int queue_wait(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms);
+// The same wait, throwing instead of setting the context's last error (for
+// callers that wait from several threads on one context).
+void queue_wait_throw(vo_ctx *ctx, uint64_t seq, uint32_t timeout_ms);
 int queue_poll(vo_ctx *ctx, uint64_t seq);
 int queue_sync(vo_ctx *ctx);
 int queue_park(vo_ctx *ctx);          // finish submitted commands, let the resident kernel leave
