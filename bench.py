@@ -663,7 +663,7 @@ def bench_batch(args):
     dev = torch.device("cuda:0")
     keys = torch.empty(total, dtype=torch.int64, device=dev)
     pay = torch.empty(total, dtype=torch.int32, device=dev)
-    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream)
+    ctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, tuning=args.tuning)
     if args.autotune:
         ctx.autotune_sort(10, 20, rows_per_class=1 << 24)
 
@@ -696,7 +696,8 @@ def bench_batch(args):
     # of padded size p moves 24 B x p per pass, and a class runs the pass plan
     # of its size, so the call's algorithmic bytes are sum(24 p passes(p))
     from velodb_amd import _abi
-    tctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING)
+    tctx = velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, flags=_abi.CFG_KERNEL_TIMING,
+                              tuning=args.tuning)
     plan = {}
     for _, _, p in items:
         if p not in plan:

@@ -129,12 +129,13 @@ def test_payload_only_shuffle_trace_value_independent(vctx):
     assert t1.launch_digest == t2.launch_digest and t1.address_digest == t2.address_digest
 
 
-@pytest.mark.parametrize("tuning", [0x3 | (8 << 4), 0x5 | (8 << 4), 0x2 | (6 << 4)])
+@pytest.mark.parametrize("tuning", [0x3 | (8 << 4), 0x5 | (8 << 4), 0x2 | (6 << 4), 0x800])
 def test_kv_sort_schedules_give_the_same_shuffle(vctx, tuning):
     """The 16-byte sort behind perm_out with register passes capped at 3
-    (groups of 4-7 stages as LDS-staged passes), 5, or 2 (hybrid up to 5):
-    every schedule applies the same network, so payload and permutation
-    equal the default context's at 2^20 and 2^17."""
+    (groups of 4-7 stages as LDS-staged passes), 5, or 2 (hybrid up to 5),
+    or with streaming loads and stores (0x800): every schedule applies the
+    same network, so payload and permutation equal the default context's at
+    2^20 and 2^17."""
     import torch
     import velodb_amd
     print(f"kv schedule tuning={tuning:#x}", flush=True)

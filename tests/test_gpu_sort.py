@@ -163,12 +163,37 @@ def test_sort_multi_bit_exact_with_ties(vctx, types, asc, n, npad, distinct):
         assert np.array_equal(from_dev(dc, TYPES[t]).view(np.uint8), ec.view(np.uint8))
 
 
+@pytest.mark.parametrize("types,asc,n,npad,distinct", [
+    ([5, 4], [True, False], 100000, 1 << 17, 10),
+    ([6, 10, 5, 4], [False, True, True, False], 70000, 1 << 17, 4),
+    ([4, 4, 4, 4, 4, 4, 4, 4], [True, False] * 4, 20000, 1 << 15, 2),
+])
+def test_sort_multi_streaming_bit_exact(types, asc, n, npad, distinct):
+    """OP_SORT with the sort passes' streaming loads and stores (tuning 0x800
+    picks them below the size that would): indices and every column equal
+    the oracle's, ties included."""
+    import torch
+    import velodb_amd
+    print(f"sort_multi streaming types={types} n={n}", flush=True)
+    rng = np.random.default_rng(len(types) * 1000 + n + 7)
+    cols = [_rand_col(rng, t, npad, distinct) for t in types]
+    ecols, eidx = oracle.bitonic_sort_multi(cols, types, asc, n, npad)
+    dcols = [to_dev(c) for c in cols]
+    didx = empty_dev(npad, np.int64)
+    with velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, tuning=0x800) as ctx:
+        ctx.sort(dcols, types, asc, didx, n, npad)
+    assert np.array_equal(from_dev(didx, np.int64), eidx)
+    for dc, ec, t in zip(dcols, ecols, types):
+        assert np.array_equal(from_dev(dc, TYPES[t]).view(np.uint8), ec.view(np.uint8))
+
+
 @pytest.mark.parametrize("tuning", [0x100, 0x5 | (1 << 4), 0x3 | (7 << 4), 0x1 | (1 << 4), 0x4 | (8 << 4),
-                                    0x200, 0x200 | 0x3 | (7 << 4), 0x200 | 0x100, 0x400])
+                                    0x200, 0x200 | 0x3 | (7 << 4), 0x200 | 0x100, 0x400, 0x800, 0x800 | 0x200])
 def test_sort_schedule_variants_bit_exact(tuning):
     """Every schedule variant (fused-stage widths, hybrid on/off, resident
-    merge blocks, 1024-thread blocks) applies the same network: bit-exact at
-    2^20, 2^17+3, one full tile and a partial tile."""
+    merge blocks, 1024-thread blocks, streaming loads and stores below the
+    size that picks them) applies the same network: bit-exact at 2^20,
+    2^17+3, one full tile and a partial tile."""
     import torch
     import velodb_amd
     print(f"schedule tuning={tuning:#x}", flush=True)

@@ -38,6 +38,12 @@ __device__ __forceinline__ uint64_t map_uniform(const IMap &m, uint64_t v)
     return ou + (v & ((1ull << m.c) - 1));
 }
 
+// Cache policy of the passes' buffer loads and stores (the instructions'
+// aux field): 0 for arrays that fit the 256 MB MALL, where the next pass
+// re-reads them from it; kCpStream (non-temporal) for larger ones, which
+// every pass streams through once.
+constexpr int kCpStream = 2;
+
 // Raw buffer access: the uniform base goes into a buffer resource (scalar
 // registers), the per-thread part is a 32-bit byte offset.  Dword 3 of the
 // resource selects the 32-bit data format (0x00020000) used for untyped
@@ -49,50 +55,50 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *ubase)
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
-template <typename T>
+template <int CP = 0, typename T>
 __device__ __forceinline__ T ld_u(const T *ubase, uint32_t vi)
 {
     const __amdgpu_buffer_rsrc_t r = buf_rsrc(ubase);
     if constexpr (sizeof(T) == 8) {
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vi * 8u, 0, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vi * 8u, 0, CP));
     } else {
         static_assert(sizeof(T) == 4, "4- or 8-byte columns");
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vi * 4u, 0, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vi * 4u, 0, CP));
     }
 }
 
-template <typename T>
+template <int CP = 0, typename T>
 __device__ __forceinline__ void st_u(T *ubase, uint32_t vi, T v)
 {
     const __amdgpu_buffer_rsrc_t r = buf_rsrc(ubase);
     if constexpr (sizeof(T) == 8) {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vi * 8u, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vi * 8u, 0, CP);
     } else {
         static_assert(sizeof(T) == 4, "4- or 8-byte columns");
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vi * 4u, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vi * 4u, 0, CP);
     }
 }
 
 // Record at uniform index u plus per-thread offset vi.
-template <class R, bool TRACE>
+template <class R, bool TRACE, int CP = 0>
 __device__ __forceinline__ void gload_u(R &r, const Bufs<R> &b, uint64_t u, uint32_t vi, uint64_t &h, uint32_t &n)
 {
 #pragma unroll
-    for (int w = 0; w < R::KW; ++w) r.k[w] = ld_u(b.k[w] + u, vi);
-    r.p = ld_u(b.p + u, vi);
+    for (int w = 0; w < R::KW; ++w) r.k[w] = ld_u<CP>(b.k[w] + u, vi);
+    r.p = ld_u<CP>(b.p + u, vi);
     if constexpr (TRACE) {
         h = tfold(h, 1, u + vi);
         ++n;
     }
 }
 
-template <class R, bool TRACE>
+template <class R, bool TRACE, int CP = 0>
 __device__ __forceinline__ void gstore_u(const R &r, const Bufs<R> &b, uint64_t u, uint32_t vi, uint64_t &h,
                                          uint32_t &n)
 {
 #pragma unroll
-    for (int w = 0; w < R::KW; ++w) st_u(b.k[w] + u, vi, r.k[w]);
-    st_u(b.p + u, vi, r.p);
+    for (int w = 0; w < R::KW; ++w) st_u<CP>(b.k[w] + u, vi, r.k[w]);
+    st_u<CP>(b.p + u, vi, r.p);
     if constexpr (TRACE) {
         h = tfold(h, 2, u + vi);
         ++n;
@@ -103,44 +109,44 @@ __device__ __forceinline__ void gstore_u(const R &r, const Bufs<R> &b, uint64_t 
 // Tile form: one buffer resource per column for the tile's uniform base u,
 // the per-register offset so (elements, uniform, < 2^16) in the scalar
 // offset field and the per-thread offset vi in the vector offset.
-template <typename T>
+template <typename T, int CP = 0>
 __device__ __forceinline__ T ld_t(__amdgpu_buffer_rsrc_t r, uint32_t vi, uint32_t so)
 {
     if constexpr (sizeof(T) == 8)
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vi * 8u, so * 8u, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vi * 8u, so * 8u, CP));
     else
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vi * 4u, so * 4u, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vi * 4u, so * 4u, CP));
 }
 
-template <typename T>
+template <typename T, int CP = 0>
 __device__ __forceinline__ void st_t(__amdgpu_buffer_rsrc_t r, uint32_t vi, uint32_t so, T v)
 {
     if constexpr (sizeof(T) == 8)
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vi * 8u, so * 8u, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vi * 8u, so * 8u, CP);
     else
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vi * 4u, so * 4u, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vi * 4u, so * 4u, CP);
 }
 
-template <class R, bool TRACE>
+template <class R, bool TRACE, int CP = 0>
 __device__ __forceinline__ void gload_t(R &r, const Bufs<R> &b, uint64_t u, uint32_t so, uint32_t vi, uint64_t &h,
                                         uint32_t &n)
 {
 #pragma unroll
-    for (int w = 0; w < R::KW; ++w) r.k[w] = ld_t<uint64_t>(buf_rsrc(b.k[w] + u), vi, so);
-    r.p = ld_t<typename R::P>(buf_rsrc(b.p + u), vi, so);
+    for (int w = 0; w < R::KW; ++w) r.k[w] = ld_t<uint64_t, CP>(buf_rsrc(b.k[w] + u), vi, so);
+    r.p = ld_t<typename R::P, CP>(buf_rsrc(b.p + u), vi, so);
     if constexpr (TRACE) {
         h = tfold(h, 1, u + so + vi);
         ++n;
     }
 }
 
-template <class R, bool TRACE>
+template <class R, bool TRACE, int CP = 0>
 __device__ __forceinline__ void gstore_t(const R &r, const Bufs<R> &b, uint64_t u, uint32_t so, uint32_t vi,
                                          uint64_t &h, uint32_t &n)
 {
 #pragma unroll
-    for (int w = 0; w < R::KW; ++w) st_t<uint64_t>(buf_rsrc(b.k[w] + u), vi, so, r.k[w]);
-    st_t<typename R::P>(buf_rsrc(b.p + u), vi, so, r.p);
+    for (int w = 0; w < R::KW; ++w) st_t<uint64_t, CP>(buf_rsrc(b.k[w] + u), vi, so, r.k[w]);
+    st_t<typename R::P, CP>(buf_rsrc(b.p + u), vi, so, r.p);
     if constexpr (TRACE) {
         h = tfold(h, 2, u + so + vi);
         ++n;
@@ -236,7 +242,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_local(Bufs<R> b, IMap map, int t
 }
 
 // Full 2^T_LOG tiles: all levels p = 1 .. T_LOG with a compile-time schedule.
-template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
+template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE, int CP = 0>
 __device__ __forceinline__ void body_local_ct(Bufs<R> b, IMap map, int p_top, uint64_t n_valid, int fill,
                                               TraceSlot *slot, char *lds, uint32_t blk)
 {
@@ -251,7 +257,7 @@ __device__ __forceinline__ void body_local_ct(Bufs<R> b, IMap map, int p_top, ui
     const uint32_t base = lay_base<R_LOG>(tau, A0);
 #pragma unroll
     for (int m = 0; m < (1 << R_LOG); ++m) {
-        gload_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
+        gload_t<R, TRACE, CP>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
         if (fill) {
             const uint64_t v = vbase + (base | ((uint32_t)m << A0));
             pad_select(r[m], local_row(map, v), live_rows(map, vbase, n_valid));
@@ -261,23 +267,23 @@ __device__ __forceinline__ void body_local_ct(Bufs<R> b, IMap map, int p_top, ui
     TL::template levels<1, A0>(r, lds, tau, asc_hi, p_top, h, nops);
     TL::template relayout<TL::end_all(1, A0), A0>(r, lds, tau, h, nops);
 #pragma unroll
-    for (int m = 0; m < (1 << R_LOG); ++m) gstore_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
+    for (int m = 0; m < (1 << R_LOG); ++m) gstore_t<R, TRACE, CP>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
     if constexpr (TRACE) tcommit(slot, (uint64_t)blk * BLOCK + tau, h, nops);
 }
 
 // 1024-thread blocks: two per CU, 8 waves per SIMD (at most 64 VGPRs; the
 // second bound is the minimum waves per execution unit)
-template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
+template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE, int CP = 0>
 __global__ __launch_bounds__(BLOCK, BLOCK >= 1024 ? 8 : 1) void k_sort_local_ct(Bufs<R> b, IMap map, int p_top,
                                                                                 uint64_t n_valid, int fill,
                                                                                 TraceSlot *slot)
 {
     extern __shared__ __align__(16) char lds[];
-    body_local_ct<R, T_LOG, R_LOG, BLOCK, TRACE>(b, map, p_top, n_valid, fill, slot, lds, blockIdx.x);
+    body_local_ct<R, T_LOG, R_LOG, BLOCK, TRACE, CP>(b, map, p_top, n_valid, fill, slot, lds, blockIdx.x);
 }
 
 // Stages (p, T_LOG-1 .. 0) of one level p > T_LOG on tiles of 2^T_LOG.
-template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
+template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE, int CP = 0>
 __device__ __forceinline__ void body_merge(Bufs<R> b, IMap map, int p, int p_top, TraceSlot *slot, char *lds,
                                            uint32_t blk)
 {
@@ -291,20 +297,20 @@ __device__ __forceinline__ void body_merge(Bufs<R> b, IMap map, int p, int p_top
     R r[1 << R_LOG];
     const uint32_t base = lay_base<R_LOG>(tau, A0);
 #pragma unroll
-    for (int m = 0; m < (1 << R_LOG); ++m) gload_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
+    for (int m = 0; m < (1 << R_LOG); ++m) gload_t<R, TRACE, CP>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
     const bool asc = dir_asc(vbase, p, p_top);
     TL::template level<-1, T_LOG - 1, A0>(r, lds, tau, asc, p_top, h, nops);
     TL::template relayout<TL::end_layout(T_LOG - 1, A0), A0>(r, lds, tau, h, nops);
 #pragma unroll
-    for (int m = 0; m < (1 << R_LOG); ++m) gstore_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
+    for (int m = 0; m < (1 << R_LOG); ++m) gstore_t<R, TRACE, CP>(r[m], b, pbase, (uint32_t)m << A0, base, h, nops);
     if constexpr (TRACE) tcommit(slot, (uint64_t)blk * BLOCK + tau, h, nops);
 }
 
-template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE>
+template <class R, int T_LOG, int R_LOG, int BLOCK, bool TRACE, int CP = 0>
 __global__ __launch_bounds__(BLOCK) void k_sort_merge(Bufs<R> b, IMap map, int p, int p_top, TraceSlot *slot)
 {
     extern __shared__ __align__(16) char lds[];
-    body_merge<R, T_LOG, R_LOG, BLOCK, TRACE>(b, map, p, p_top, slot, lds, blockIdx.x);
+    body_merge<R, T_LOG, R_LOG, BLOCK, TRACE, CP>(b, map, p, p_top, slot, lds, blockIdx.x);
 }
 
 // The same merge with resident blocks: each block walks tiles
@@ -359,7 +365,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_merge_pipe(Bufs<R> b, IMap map, 
 // bits), so tile bits [C, T_LOG) are the stage bits and the stages run as a
 // tile schedule on them.  With C >= 6 a wavefront's 64 lanes always read
 // and write 64 consecutive records.
-template <class R, int T_LOG, int R_LOG, int BLOCK, int S, bool TRACE>
+template <class R, int T_LOG, int R_LOG, int BLOCK, int S, bool TRACE, int CP = 0>
 __device__ __forceinline__ void body_hybrid(Bufs<R> b, IMap map, int p, int q_lo, int p_top, TraceSlot *slot,
                                             char *lds, uint32_t blk32)
 {
@@ -386,21 +392,21 @@ __device__ __forceinline__ void body_hybrid(Bufs<R> b, IMap map, int p, int q_lo
     uint32_t nops = 0;
     R r[1 << R_LOG];
 #pragma unroll
-    for (int m = 0; m < (1 << R_LOG); ++m) gload_u<R, TRACE>(r[m], b, pw + eoff((uint64_t)m << A0), vl, h, nops);
+    for (int m = 0; m < (1 << R_LOG); ++m) gload_u<R, TRACE, CP>(r[m], b, pw + eoff((uint64_t)m << A0), vl, h, nops);
     const bool asc = dir_asc(ublk, p, p_top);
     TL::template level<-1, T_LOG - 1, A0, C>(r, lds, tau, asc, p_top, h, nops);
     TL::template relayout<TL::end_layout(T_LOG - 1, A0, C), A0>(r, lds, tau, h, nops);
 #pragma unroll
-    for (int m = 0; m < (1 << R_LOG); ++m) gstore_u<R, TRACE>(r[m], b, pw + eoff((uint64_t)m << A0), vl, h, nops);
+    for (int m = 0; m < (1 << R_LOG); ++m) gstore_u<R, TRACE, CP>(r[m], b, pw + eoff((uint64_t)m << A0), vl, h, nops);
     if constexpr (TRACE) tcommit(slot, blk * BLOCK + tau, h, nops);
 }
 
-template <class R, int T_LOG, int R_LOG, int BLOCK, int S, bool TRACE>
+template <class R, int T_LOG, int R_LOG, int BLOCK, int S, bool TRACE, int CP = 0>
 __global__ __launch_bounds__(BLOCK) void k_sort_hybrid(Bufs<R> b, IMap map, int p, int q_lo, int p_top,
                                                        TraceSlot *slot)
 {
     extern __shared__ __align__(16) char lds[];
-    body_hybrid<R, T_LOG, R_LOG, BLOCK, S, TRACE>(b, map, p, q_lo, p_top, slot, lds, blockIdx.x);
+    body_hybrid<R, T_LOG, R_LOG, BLOCK, S, TRACE, CP>(b, map, p, q_lo, p_top, slot, lds, blockIdx.x);
 }
 
 // ------------------------------------------------------------ global pass
@@ -411,7 +417,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_hybrid(Bufs<R> b, IMap map, int 
 // so base(g) = ubase + tid and consecutive lanes touch consecutive records:
 // each wavefront load or store is one contiguous 512-byte (keys) or 256-byte
 // (payload) run.
-template <class R, int S, bool TRACE>
+template <class R, int S, bool TRACE, int CP = 0>
 __device__ __forceinline__ void body_global(Bufs<R> b, IMap map, int p, int q_lo, int p_top, TraceSlot *slot,
                                             uint32_t blk, uint32_t tid)
 {
@@ -422,7 +428,7 @@ __device__ __forceinline__ void body_global(Bufs<R> b, IMap map, int p, int q_lo
     uint32_t nops = 0;
     R r[1 << S];
 #pragma unroll
-    for (int m = 0; m < (1 << S); ++m) gload_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << q_lo, tid, h, nops);
+    for (int m = 0; m < (1 << S); ++m) gload_t<R, TRACE, CP>(r[m], b, pbase, (uint32_t)m << q_lo, tid, h, nops);
     const bool asc = dir_asc(ubase, p, p_top);
 #pragma unroll
     for (int qq = S - 1; qq >= 0; --qq) {
@@ -433,15 +439,15 @@ __device__ __forceinline__ void body_global(Bufs<R> b, IMap map, int p, int q_lo
         }
     }
 #pragma unroll
-    for (int m = 0; m < (1 << S); ++m) gstore_t<R, TRACE>(r[m], b, pbase, (uint32_t)m << q_lo, tid, h, nops);
+    for (int m = 0; m < (1 << S); ++m) gstore_t<R, TRACE, CP>(r[m], b, pbase, (uint32_t)m << q_lo, tid, h, nops);
     if constexpr (TRACE) tcommit(slot, g0 + tid, h, nops);
 }
 
-template <class R, int S, bool TRACE>
+template <class R, int S, bool TRACE, int CP = 0>
 __global__ __launch_bounds__(256) void k_sort_global(Bufs<R> b, IMap map, int p, int q_lo, int p_top,
                                                      TraceSlot *slot)
 {
-    body_global<R, S, TRACE>(b, map, p, q_lo, p_top, slot, blockIdx.x, threadIdx.x);
+    body_global<R, S, TRACE, CP>(b, map, p, q_lo, p_top, slot, blockIdx.x, threadIdx.x);
 }
 
 } // namespace dev
@@ -531,7 +537,7 @@ struct SortImpl {
     }
     static int hmax(vo_ctx *ctx) { return std::min(H_MAX, ctx->tuning.max_hybrid_stages); }
 
-    template <bool TRACE, int S, int RL, int BL>
+    template <bool TRACE, int S, int RL, int BL, int CP>
     static void hybrid_shape(vo_ctx *ctx, Bufs<R> b, IMap map, int p, int q_lo, int p_top, uint64_t n_virtual)
     {
         using TL = dev::TileCT<R, T_LOG, RL, TRACE>;
@@ -540,12 +546,12 @@ struct SortImpl {
                                             "sort_hybrid_s8"};
         LaunchInfo li{names[S - 4], dim3((uint32_t)(n_virtual >> T_LOG)), dim3(BL), (uint32_t)lds,
                       hash_mix(hash_mix(p, q_lo), hash_mix(S + 100, p_top))};
-        auto k = dev::k_sort_hybrid<R, T_LOG, RL, BL, S, TRACE>;
+        auto k = dev::k_sort_hybrid<R, T_LOG, RL, BL, S, TRACE, CP>;
         VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         launch(ctx, li, k, b, map, p, q_lo, p_top, trace_slot(ctx));
     }
 
-    template <bool TRACE, int S>
+    template <bool TRACE, int S, int CP>
     static void launch_hybrid(vo_ctx *ctx, Bufs<R> b, IMap map, int p, int q_lo, int p_top, uint64_t n_virtual)
     {
         if constexpr (S <= H_MAX && T_LOG - S >= 5) {
@@ -553,22 +559,24 @@ struct SortImpl {
             // tuning may flip the shape (both are built when the wide one is)
             const bool wide = (S <= HW_MAX) != (HW_MAX > 0 && ctx->tuning.flip_hybrid_block);
             if (wide) {
-                if constexpr (HW_MAX > 0) hybrid_shape<TRACE, S, R_LOG - 1, 2 * BLOCK>(ctx, b, map, p, q_lo, p_top, n_virtual);
+                if constexpr (HW_MAX > 0)
+                    hybrid_shape<TRACE, S, R_LOG - 1, 2 * BLOCK, CP>(ctx, b, map, p, q_lo, p_top, n_virtual);
             } else {
-                hybrid_shape<TRACE, S, R_LOG, BLOCK>(ctx, b, map, p, q_lo, p_top, n_virtual);
+                hybrid_shape<TRACE, S, R_LOG, BLOCK, CP>(ctx, b, map, p, q_lo, p_top, n_virtual);
             }
         } else {
             throw Error{VO_ERR_INVALID, "hybrid pass width not built"};
         }
     }
 
-    template <bool TRACE, int S>
+    template <bool TRACE, int S, int CP>
     static void launch_global(vo_ctx *ctx, const LaunchInfo &li, Bufs<R> b, IMap map, int p, int q_lo, int p_top)
     {
-        if constexpr (S <= S_MAX) launch(ctx, li, dev::k_sort_global<R, S, TRACE>, b, map, p, q_lo, p_top, trace_slot(ctx));
+        if constexpr (S <= S_MAX)
+            launch(ctx, li, dev::k_sort_global<R, S, TRACE, CP>, b, map, p, q_lo, p_top, trace_slot(ctx));
     }
 
-    template <bool TRACE>
+    template <bool TRACE, int CP>
     static void global_pass(vo_ctx *ctx, Bufs<R> b, IMap map, int p, int q_lo, int s, int p_top, uint64_t n_virtual)
     {
         const uint64_t threads = n_virtual >> s;
@@ -579,16 +587,16 @@ struct SortImpl {
         LaunchInfo li{names[s], dim3((uint32_t)(threads / 256)), dim3(256), 0,
                       hash_mix(hash_mix(p, q_lo), hash_mix(s, p_top))};
         switch (s) {
-        case 1: launch_global<TRACE, 1>(ctx, li, b, map, p, q_lo, p_top); break;
-        case 2: launch_global<TRACE, 2>(ctx, li, b, map, p, q_lo, p_top); break;
-        case 3: launch_global<TRACE, 3>(ctx, li, b, map, p, q_lo, p_top); break;
-        case 4: launch_global<TRACE, 4>(ctx, li, b, map, p, q_lo, p_top); break;
-        case 5: launch_global<TRACE, 5>(ctx, li, b, map, p, q_lo, p_top); break;
+        case 1: launch_global<TRACE, 1, CP>(ctx, li, b, map, p, q_lo, p_top); break;
+        case 2: launch_global<TRACE, 2, CP>(ctx, li, b, map, p, q_lo, p_top); break;
+        case 3: launch_global<TRACE, 3, CP>(ctx, li, b, map, p, q_lo, p_top); break;
+        case 4: launch_global<TRACE, 4, CP>(ctx, li, b, map, p, q_lo, p_top); break;
+        case 5: launch_global<TRACE, 5, CP>(ctx, li, b, map, p, q_lo, p_top); break;
         default: throw Error{VO_ERR_INVALID, "bad fused stage count"};
         }
     }
 
-    template <bool TRACE>
+    template <bool TRACE, int CP>
     static void run_t(vo_ctx *ctx, Bufs<R> b, int n_log, uint64_t n_valid, int fill, IMap map, int p_top,
                       uint64_t n_virtual)
     {
@@ -606,7 +614,7 @@ struct SortImpl {
                 LaunchInfo li{"sort_local", dim3((uint32_t)(n_virtual >> t)), dim3(BLOCK), (uint32_t)lds,
                               hash_mix(hash_mix(t, p_top), fill)};
                 if (ct) {
-                    auto k = dev::k_sort_local_ct<R, T_LOG, R_LOG, BLOCK, TRACE>;
+                    auto k = dev::k_sort_local_ct<R, T_LOG, R_LOG, BLOCK, TRACE, CP>;
                     VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                     launch(ctx, li, k, b, map, p_top, n_valid, fill, trace_slot(ctx));
                 } else {
@@ -616,14 +624,14 @@ struct SortImpl {
                 }
                 break;
             }
-            case 2: global_pass<TRACE>(ctx, b, map, ps.p, ps.q_lo, ps.s, p_top, n_virtual); break;
+            case 2: global_pass<TRACE, CP>(ctx, b, map, ps.p, ps.q_lo, ps.s, p_top, n_virtual); break;
             case 4:
                 switch (ps.s) {
-                case 4: launch_hybrid<TRACE, 4>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
-                case 5: launch_hybrid<TRACE, 5>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
-                case 6: launch_hybrid<TRACE, 6>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
-                case 7: launch_hybrid<TRACE, 7>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
-                case 8: launch_hybrid<TRACE, 8>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
+                case 4: launch_hybrid<TRACE, 4, CP>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
+                case 5: launch_hybrid<TRACE, 5, CP>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
+                case 6: launch_hybrid<TRACE, 6, CP>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
+                case 7: launch_hybrid<TRACE, 7, CP>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
+                case 8: launch_hybrid<TRACE, 8, CP>(ctx, b, map, ps.p, ps.q_lo, p_top, n_virtual); break;
                 default: throw Error{VO_ERR_INVALID, "bad hybrid stage count"};
                 }
                 break;
@@ -640,7 +648,7 @@ struct SortImpl {
                     launch(ctx, li, k, b, map, ps.p, p_top, n_tiles, trace_slot(ctx));
                 } else {
                     LaunchInfo li{"sort_merge", dim3(n_tiles), dim3(BLOCK), (uint32_t)lds, hash_mix(ps.p, p_top)};
-                    auto k = dev::k_sort_merge<R, T_LOG, R_LOG, BLOCK, TRACE>;
+                    auto k = dev::k_sort_merge<R, T_LOG, R_LOG, BLOCK, TRACE, CP>;
                     VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                     launch(ctx, li, k, b, map, ps.p, p_top, trace_slot(ctx));
                 }
@@ -650,15 +658,34 @@ struct SortImpl {
         }
     }
 
+    // Streaming loads and stores (dev::kCpStream) when the arrays a launch
+    // covers exceed the 256 MB MALL, so that no pass re-reads them from it.
+    // Measured on the 12-byte sort: +4% at 2^26 and +2.7% at 2^28, but -8% at
+    // 2^24, which the MALL holds.  Resident programs keep the default (their
+    // bodies are built with it); tuning bit 0x800 flips the choice.
+    static bool streaming(vo_ctx *ctx, uint64_t n_virtual)
+    {
+        constexpr uint64_t kMallBytes = 256ull << 20;
+        const bool big = n_virtual * (8 * R::KW + sizeof(typename R::P)) > kMallBytes;
+        return !ctx->rec && (big != (ctx->tuning.flip_streaming != 0));
+    }
+
+    static void dispatch(vo_ctx *ctx, Bufs<R> b, int n_log, uint64_t n_valid, int fill, IMap map, int p_top,
+                         uint64_t n_virtual)
+    {
+        if (ctx->tracing)
+            run_t<true, 0>(ctx, b, n_log, n_valid, fill, map, p_top, n_virtual);
+        else if (streaming(ctx, n_virtual))
+            run_t<false, dev::kCpStream>(ctx, b, n_log, n_valid, fill, map, p_top, n_virtual);
+        else
+            run_t<false, 0>(ctx, b, n_log, n_valid, fill, map, p_top, n_virtual);
+    }
+
     static void run(vo_ctx *ctx, Bufs<R> b, int n_log, uint64_t n_valid, int fill)
     {
         VO_CHECK((1ull << n_log) <= kMaxSortRows, VO_ERR_INVALID,
                  "n_pad above 2^29 (32-bit buffer offsets of 8-byte columns)");
-        IMap map{nullptr, 0, nullptr};
-        if (ctx->tracing)
-            run_t<true>(ctx, b, n_log, n_valid, fill, map, n_log, 1ull << n_log);
-        else
-            run_t<false>(ctx, b, n_log, n_valid, fill, map, n_log, 1ull << n_log);
+        dispatch(ctx, b, n_log, n_valid, fill, IMap{nullptr, 0, nullptr}, n_log, 1ull << n_log);
     }
 
     // `count` independent instances of 2^c rows, instance i at element
@@ -667,11 +694,7 @@ struct SortImpl {
     static void run_mapped(vo_ctx *ctx, Bufs<R> b, int c, uint64_t count, const uint64_t *offsets, const uint64_t *ns)
     {
         VO_CHECK(c >= R_LOG && (1ull << c) <= kMaxSortRows, VO_ERR_INVALID, "batched instance size out of range");
-        IMap map{offsets, c, ns};
-        if (ctx->tracing)
-            run_t<true>(ctx, b, c, 0, 1, map, c, count << c);
-        else
-            run_t<false>(ctx, b, c, 0, 1, map, c, count << c);
+        dispatch(ctx, b, c, 0, 1, IMap{offsets, c, ns}, c, count << c);
     }
 
     static int passes(vo_ctx *ctx, int n_log) { return (int)plan_passes(n_log, T_LOG, smax(ctx, n_log), R_LOG, hmax(ctx)).size(); }

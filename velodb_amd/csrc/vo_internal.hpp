@@ -73,6 +73,7 @@ struct SortTuning {
     int pipe_merge = 0;        // resident merge blocks with next-tile prefetch
     int wide_block = 0;        // 12-byte sort: 1024-thread blocks, 8 records per thread
     int flip_hybrid_block = 0; // 12-byte sort's hybrid passes: the other block shape (twice / half the threads)
+    int flip_streaming = 0;    // sort passes: the other cache policy than the size rule picks
 };
 
 struct Recorder;
@@ -174,8 +175,9 @@ inline int ilog2(uint64_t v)
 inline bool is_pow2(uint64_t v) { return v && !(v & (v - 1)); }
 
 // Tuning word (vo_config.tuning): bits 0-3 register-pass stages, bits 4-7
-// hybrid-pass stages + 1, bit 8 toggles resident merge blocks.  0 keeps the
-// defaults.
+// hybrid-pass stages + 1; bits 8-11 toggle the prefetching merge, the
+// 1024-thread 12-byte sort, its hybrid block shape and the sort passes'
+// cache policy.  0 keeps the defaults.
 inline SortTuning apply_tuning_word(SortTuning t, uint32_t w)
 {
     if (w & 0xF) t.max_fused_stages = (int)(w & 0xF);
@@ -183,6 +185,7 @@ inline SortTuning apply_tuning_word(SortTuning t, uint32_t w)
     if (w & 0x100) t.pipe_merge = !t.pipe_merge;
     if (w & 0x200) t.wide_block = !t.wide_block;
     if (w & 0x400) t.flip_hybrid_block = !t.flip_hybrid_block;
+    if (w & 0x800) t.flip_streaming = !t.flip_streaming;
     return t;
 }
 
