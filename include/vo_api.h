@@ -455,9 +455,10 @@ typedef struct vo_config {
                                 LDS-staged pass + 1, 0x100 merge blocks that prefetch their
                                 next tile, 0x200 the (u64, u32) sort with 1024-thread blocks,
                                 0x400 its LDS-staged passes with the other block shape, 0x800
-                                the other cache policy of the sort passes (streaming loads and
-                                stores are chosen above 256 MB of sorted arrays).  Every word
-                                applies the same network: the output does not change */
+                                the other cache policy of the sort and compaction passes
+                                (streaming loads and stores are chosen above 256 MB of rows).
+                                Every word applies the same network: the output does not
+                                change */
     uint32_t queue_idle_us;  /* the resident queue kernel leaves after this long without a
                                 command (0 -> 20000); the next vo_submit starts it again */
     uint32_t queue_lanes;    /* lanes of the resident queue kernel, 1..8 (0 -> 1): the resident

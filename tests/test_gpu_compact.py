@@ -42,6 +42,34 @@ def test_compact_bit_exact(vctx, widths, n, n_pad, p):
     assert np.array_equal(got, expect)
 
 
+@pytest.mark.parametrize("widths,n,n_pad,p", [
+    ([8, 8], (1 << 20) - 5, 1 << 20, 0.0181),
+    ([8, 8, 4, 2, 1], 1 << 20, 1 << 21, 0.6),
+    ([8], 70000, 1 << 17, 0.5),
+])
+def test_compact_streaming_bit_exact(widths, n, n_pad, p):
+    """The network passes with streaming (non-temporal) row accesses, which
+    tuning 0x800 picks below the size that would: same rows and count as
+    the oracle (every pass width: local, LDS-staged and register passes)."""
+    import torch
+    import velodb_amd
+    print(f"compact streaming widths={widths} n={n} n_pad={n_pad}", flush=True)
+    rng = np.random.default_rng(n + len(widths) + 11)
+    cols = [rng.integers(0, np.iinfo(WIDTH_DT[w]).max, n_pad, dtype=WIDTH_DT[w], endpoint=True) for w in widths]
+    mask = (rng.random(n_pad) < p).astype(np.uint8)
+    sel = mask.copy()
+    sel[n:] = 0
+    expect = oracle.compact(_rows(cols), sel)
+    dcols = [to_dev(c) for c in cols]
+    dmask = to_dev(mask)
+    dcount = empty_dev(1, np.uint64)
+    with velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, tuning=0x800) as ctx:
+        ctx.compact(dmask, dcols, widths, n, n_pad, out_count=dcount)
+    got = _rows([from_dev(d, WIDTH_DT[w]) for d, w in zip(dcols, widths)])
+    assert int(from_dev(dcount, np.uint64)[0]) == int(sel.sum())
+    assert np.array_equal(got, expect)
+
+
 @pytest.mark.parametrize("mask_kind", ["all", "none", "alternate", "last"])
 def test_compact_edge_masks(vctx, mask_kind):
     n_pad = 1 << 16

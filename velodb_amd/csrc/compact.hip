@@ -63,22 +63,14 @@ Workspace workspace(vo_ctx *ctx, uint64_t n_pad, bool want_mask)
     return ws;
 }
 
-// The network's passes.  One compaction: n_pad rows, levels below
-// log2(n_pad).  A batch class (in.off set): `rows` virtual rows of instances
-// of 2^in.c rows each, levels below in.c, every pass over all of them.
-template <int NW>
-void run_network(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, uint64_t n, uint64_t n_pad,
-                 const uint64_t *tile_pre, Inst in = Inst{nullptr, nullptr, 0}, uint64_t rows = 0)
+// The network's passes with one tracing mode (TR) and cache policy (NT).
+template <int NW, bool TR, bool NT>
+void run_network_t(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, uint64_t n,
+                   const uint64_t *tile_pre, Inst in, uint64_t rows, int n_log, bool w8)
 {
-    const int n_log = in.off ? in.c : ilog2(n_pad);
-    if (!in.off) rows = n_pad;
     const uint32_t nt = (uint32_t)(rows / CT);
     const size_t lds = (size_t)CTPAD * 8 + (CT + 1) * 2;
-    bool w8 = true;
-    for (int w = 0; w < NW; ++w) w8 = w8 && src.c[w].w == 8 && cols.c[w].w == 8;
-    const bool tr = ctx->tracing;
-    auto kl = tr ? (w8 ? k_compact_local<NW, true, true> : k_compact_local<NW, false, true>)
-                 : (w8 ? k_compact_local<NW, true, false> : k_compact_local<NW, false, false>);
+    auto kl = w8 ? k_compact_local<NW, true, TR, NT> : k_compact_local<NW, false, TR, NT>;
     VO_HIP(hipFuncSetAttribute((const void *)kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     launch(ctx, LaunchInfo{"compact_local", dim3(nt), dim3(CBLOCK), (uint32_t)lds, hash_mix(NW, n_log)}, kl, src,
            cols, mask, n, tile_pre, in, trace_slot(ctx));
@@ -97,45 +89,60 @@ void run_network(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, 
                 VO_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hl));
                 launch(ctx, li, k, cols, tile_pre, q, in, trace_slot(ctx));
             };
-            if (tr) {
-                if (w8)
-                    pick(k_compact_hybrid<NW, 5, true, true>, k_compact_hybrid<NW, 6, true, true>,
-                         k_compact_hybrid<NW, 7, true, true>, k_compact_hybrid<NW, 8, true, true>);
-                else
-                    pick(k_compact_hybrid<NW, 5, false, true>, k_compact_hybrid<NW, 6, false, true>,
-                         k_compact_hybrid<NW, 7, false, true>, k_compact_hybrid<NW, 8, false, true>);
-            } else {
-                if (w8)
-                    pick(k_compact_hybrid<NW, 5, true, false>, k_compact_hybrid<NW, 6, true, false>,
-                         k_compact_hybrid<NW, 7, true, false>, k_compact_hybrid<NW, 8, true, false>);
-                else
-                    pick(k_compact_hybrid<NW, 5, false, false>, k_compact_hybrid<NW, 6, false, false>,
-                         k_compact_hybrid<NW, 7, false, false>, k_compact_hybrid<NW, 8, false, false>);
-            }
+            if (w8)
+                pick(k_compact_hybrid<NW, 5, true, TR, NT>, k_compact_hybrid<NW, 6, true, TR, NT>,
+                     k_compact_hybrid<NW, 7, true, TR, NT>, k_compact_hybrid<NW, 8, true, TR, NT>);
+            else
+                pick(k_compact_hybrid<NW, 5, false, TR, NT>, k_compact_hybrid<NW, 6, false, TR, NT>,
+                     k_compact_hybrid<NW, 7, false, TR, NT>, k_compact_hybrid<NW, 8, false, TR, NT>);
             q += s;
             continue;
         }
         const uint32_t blocks = (uint32_t)((rows >> s) / 256);
         LaunchInfo li{"compact_global", dim3(blocks), dim3(256), 0, hash_mix(hash_mix(NW, q), s)};
         switch (s) {
-        case 1: launch(ctx, li, tr ? k_compact_global<NW, 1, true> : k_compact_global<NW, 1, false>, cols, tile_pre, q,
-                          in, trace_slot(ctx)); break;
-        case 2: launch(ctx, li, tr ? k_compact_global<NW, 2, true> : k_compact_global<NW, 2, false>, cols, tile_pre, q,
-                          in, trace_slot(ctx)); break;
-        case 3: launch(ctx, li, tr ? k_compact_global<NW, 3, true> : k_compact_global<NW, 3, false>, cols, tile_pre, q,
-                          in, trace_slot(ctx)); break;
-        case 4: launch(ctx, li, tr ? k_compact_global<NW, 4, true> : k_compact_global<NW, 4, false>, cols, tile_pre, q,
-                          in, trace_slot(ctx)); break;
+        case 1: launch(ctx, li, k_compact_global<NW, 1, TR, NT>, cols, tile_pre, q, in, trace_slot(ctx)); break;
+        case 2: launch(ctx, li, k_compact_global<NW, 2, TR, NT>, cols, tile_pre, q, in, trace_slot(ctx)); break;
+        case 3: launch(ctx, li, k_compact_global<NW, 3, TR, NT>, cols, tile_pre, q, in, trace_slot(ctx)); break;
+        case 4: launch(ctx, li, k_compact_global<NW, 4, TR, NT>, cols, tile_pre, q, in, trace_slot(ctx)); break;
         default:
             if constexpr (NW == 1) {
-                launch(ctx, li, tr ? k_compact_global<NW, 5, true> : k_compact_global<NW, 5, false>, cols, tile_pre, q,
-                       in, trace_slot(ctx));
+                launch(ctx, li, k_compact_global<NW, 5, TR, NT>, cols, tile_pre, q, in, trace_slot(ctx));
                 break;
             }
             throw Error{VO_ERR_INVALID, "compaction pass width"};
         }
         q += s;
     }
+}
+
+// The network's passes.  One compaction: n_pad rows, levels below
+// log2(n_pad).  A batch class (in.off set): `rows` virtual rows of instances
+// of 2^in.c rows each, levels below in.c, every pass over all of them.
+// Streaming (non-temporal) row accesses when the rows a pass moves exceed
+// the 256 MB MALL: filter + SUM at 3e8 rows / 2^29 took 16.2 ms against
+// 17.0; at 2^23 (134 MB, held by the MALL) they cost 5%.  Resident programs
+// keep the default (their bodies are built with it); tuning bit 0x800 flips
+// the choice.
+template <int NW>
+void run_network(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, uint64_t n, uint64_t n_pad,
+                 const uint64_t *tile_pre, Inst in = Inst{nullptr, nullptr, 0}, uint64_t rows = 0)
+{
+    const int n_log = in.off ? in.c : ilog2(n_pad);
+    if (!in.off) rows = n_pad;
+    bool w8 = true;
+    uint64_t row_bytes = 0;
+    for (int w = 0; w < NW; ++w) {
+        w8 = w8 && src.c[w].w == 8 && cols.c[w].w == 8;
+        row_bytes += cols.c[w].w;
+    }
+    constexpr uint64_t kMallBytes = 256ull << 20;
+    if (ctx->tracing)
+        run_network_t<NW, true, false>(ctx, src, cols, mask, n, tile_pre, in, rows, n_log, w8);
+    else if (!ctx->rec && ((rows * row_bytes > kMallBytes) != (ctx->tuning.flip_streaming != 0)))
+        run_network_t<NW, false, true>(ctx, src, cols, mask, n, tile_pre, in, rows, n_log, w8);
+    else
+        run_network_t<NW, false, false>(ctx, src, cols, mask, n, tile_pre, in, rows, n_log, w8);
 }
 
 // Count + scan for a mask already in device memory.

@@ -45,23 +45,46 @@ struct Cols {
     Col c[NW];
 };
 
+// Row loads and stores of the network passes.  NT: non-temporal, for rows
+// beyond the 256 MB MALL, which every pass streams through once (below it
+// the next pass re-reads them from the MALL).
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_row(const T *p)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ void st_row(T *p, T v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
+template <bool NT = false>
 __device__ __forceinline__ uint64_t ld_col(const Col &c, uint64_t i)
 {
     switch (c.w) {
-    case 8: return ((const uint64_t *)c.p)[i];
-    case 4: return ((const uint32_t *)c.p)[i];
-    case 2: return ((const uint16_t *)c.p)[i];
-    default: return ((const uint8_t *)c.p)[i];
+    case 8: return ld_row<NT>((const uint64_t *)c.p + i);
+    case 4: return ld_row<NT>((const uint32_t *)c.p + i);
+    case 2: return ld_row<NT>((const uint16_t *)c.p + i);
+    default: return ld_row<NT>((const uint8_t *)c.p + i);
     }
 }
 
+template <bool NT = false>
 __device__ __forceinline__ void st_col(const Col &c, uint64_t i, uint64_t v)
 {
     switch (c.w) {
-    case 8: ((uint64_t *)c.p)[i] = v; break;
-    case 4: ((uint32_t *)c.p)[i] = (uint32_t)v; break;
-    case 2: ((uint16_t *)c.p)[i] = (uint16_t)v; break;
-    default: ((uint8_t *)c.p)[i] = (uint8_t)v; break;
+    case 8: st_row<NT>((uint64_t *)c.p + i, v); break;
+    case 4: st_row<NT>((uint32_t *)c.p + i, (uint32_t)v); break;
+    case 2: st_row<NT>((uint16_t *)c.p + i, (uint16_t)v); break;
+    default: st_row<NT>((uint8_t *)c.p + i, (uint8_t)v); break;
     }
 }
 
@@ -371,7 +394,7 @@ __device__ __forceinline__ void clevels(uint64_t (&r)[16][NW], const uint16_t *c
 }
 
 // W8: every column is 8 bytes wide (no per-load width dispatch)
-template <int NW, bool W8, bool TRACE>
+template <int NW, bool W8, bool TRACE, bool NT = false>
 __device__ __forceinline__ void b_compact_local(Cols<NW> src, Cols<NW> dst, const uint8_t *__restrict__ mask,
                                                 uint64_t n, const uint64_t *__restrict__ tile_pre, Inst in,
                                                 TraceSlot *slot, char *lds_raw, uint32_t vb)
@@ -394,7 +417,8 @@ __device__ __forceinline__ void b_compact_local(Cols<NW> src, Cols<NW> dst, cons
     for (int m = 0; m < 16; ++m)
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            r[m][w] = W8 ? ((const uint64_t *)src.c[w].p)[t0 + tid + 256u * m] : ld_col(src.c[w], t0 + tid + 256u * m);
+            r[m][w] = W8 ? ld_row<NT>((const uint64_t *)src.c[w].p + t0 + tid + 256u * m)
+                         : ld_col<NT>(src.c[w], t0 + tid + 256u * m);
             tr.at(8 + w, t0 + tid + 256u * m);
         }
 
@@ -439,21 +463,21 @@ __device__ __forceinline__ void b_compact_local(Cols<NW> src, Cols<NW> dst, cons
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
             if (W8)
-                ((uint64_t *)dst.c[w].p)[t0 + tid_s + 256u * m] = r[m][w];
+                st_row<NT>((uint64_t *)dst.c[w].p + t0 + tid_s + 256u * m, r[m][w]);
             else
-                st_col(dst.c[w], t0 + tid_s + 256u * m, r[m][w]);
+                st_col<NT>(dst.c[w], t0 + tid_s + 256u * m, r[m][w]);
             tr.at(16 + w, t0 + tid_s + 256u * m);
         }
     tr.done(dev::gtid());
 }
 
-template <int NW, bool W8, bool TRACE>
+template <int NW, bool W8, bool TRACE, bool NT = false>
 __global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> src, Cols<NW> dst, const uint8_t *__restrict__ mask,
                                                           uint64_t n, const uint64_t *__restrict__ tile_pre, Inst in,
                                                           TraceSlot *slot)
 {
     extern __shared__ __align__(16) char lds_raw[];
-    b_compact_local<NW, W8, TRACE>(src, dst, mask, n, tile_pre, in, slot, lds_raw, blockIdx.x);
+    b_compact_local<NW, W8, TRACE, NT>(src, dst, mask, n, tile_pre, in, slot, lds_raw, blockIdx.x);
 }
 
 // ---------------------------------------------------------- global levels
@@ -462,7 +486,7 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_local(Cols<NW> src, Cols<NW>
 // rows base + m * 2^q_lo.  The block start of any pair has every bit below
 // q_lo + 1 clear, so its prefix counts are the same for all lanes of a
 // wavefront and come from the tile prefix array.
-template <int NW, int S, bool TRACE>
+template <int NW, int S, bool TRACE, bool NT = false>
 __device__ __forceinline__ void b_compact_global(Cols<NW> cols, const uint64_t *__restrict__ tile_pre, int q_lo,
                                                  Inst in, TraceSlot *slot, uint32_t vb)
 {
@@ -476,7 +500,7 @@ __device__ __forceinline__ void b_compact_global(Cols<NW> cols, const uint64_t *
     for (int m = 0; m < (1 << S); ++m)
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            r[m][w] = ld_col(cols.c[w], pb + ((uint64_t)m << q_lo));
+            r[m][w] = ld_col<NT>(cols.c[w], pb + ((uint64_t)m << q_lo));
             tr.at(8 + w, pb + ((uint64_t)m << q_lo));
         }
     // uniform part of base (bits >= 8 are shared by the wavefront)
@@ -510,17 +534,17 @@ __device__ __forceinline__ void b_compact_global(Cols<NW> cols, const uint64_t *
     for (int m = 0; m < (1 << S); ++m)
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            st_col(cols.c[w], pb + ((uint64_t)m << q_lo), r[m][w]);
+            st_col<NT>(cols.c[w], pb + ((uint64_t)m << q_lo), r[m][w]);
             tr.at(16 + w, pb + ((uint64_t)m << q_lo));
         }
     tr.done(dev::gtid());
 }
 
-template <int NW, int S, bool TRACE>
+template <int NW, int S, bool TRACE, bool NT = false>
 __global__ __launch_bounds__(256) void k_compact_global(Cols<NW> cols, const uint64_t *__restrict__ tile_pre, int q_lo,
                                                         Inst in, TraceSlot *slot)
 {
-    b_compact_global<NW, S, TRACE>(cols, tile_pre, q_lo, in, slot, blockIdx.x);
+    b_compact_global<NW, S, TRACE, NT>(cols, tile_pre, q_lo, in, slot, blockIdx.x);
 }
 
 // ------------------------------------------------- LDS-staged global levels
@@ -559,7 +583,7 @@ __device__ __forceinline__ void hlevels(uint64_t (&r)[16][NW], const uint32_t *P
     }
 }
 
-template <int NW, int S, bool W8, bool TRACE>
+template <int NW, int S, bool W8, bool TRACE, bool NT = false>
 __device__ __forceinline__ void b_compact_hybrid(Cols<NW> cols, const uint64_t *__restrict__ tile_pre, int q_lo,
                                                  Inst in, TraceSlot *slot, char *lds_raw, uint32_t vb)
 {
@@ -586,7 +610,7 @@ __device__ __forceinline__ void b_compact_hybrid(Cols<NW> cols, const uint64_t *
         const uint32_t g = ((l >> CB) << q_lo) + j0 + (l & ((1u << CB) - 1)); // row in the group
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            r[m][w] = W8 ? ((const uint64_t *)cols.c[w].p + ph)[g] : ld_col(cols.c[w], ph + g);
+            r[m][w] = W8 ? ld_row<NT>((const uint64_t *)cols.c[w].p + ph + g) : ld_col<NT>(cols.c[w], ph + g);
             tr.at(8 + w, ph + g);
         }
     }
@@ -605,21 +629,21 @@ __device__ __forceinline__ void b_compact_hybrid(Cols<NW> cols, const uint64_t *
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
             if (W8)
-                ((uint64_t *)cols.c[w].p + ph)[g] = r[m][w];
+                st_row<NT>((uint64_t *)cols.c[w].p + ph + g, r[m][w]);
             else
-                st_col(cols.c[w], ph + g, r[m][w]);
+                st_col<NT>(cols.c[w], ph + g, r[m][w]);
             tr.at(16 + w, ph + g);
         }
     }
     tr.done(dev::gtid());
 }
 
-template <int NW, int S, bool W8, bool TRACE>
+template <int NW, int S, bool W8, bool TRACE, bool NT = false>
 __global__ __launch_bounds__(CBLOCK) void k_compact_hybrid(Cols<NW> cols, const uint64_t *__restrict__ tile_pre,
                                                            int q_lo, Inst in, TraceSlot *slot)
 {
     extern __shared__ __align__(16) char lds_raw[];
-    b_compact_hybrid<NW, S, W8, TRACE>(cols, tile_pre, q_lo, in, slot, lds_raw, blockIdx.x);
+    b_compact_hybrid<NW, S, W8, TRACE, NT>(cols, tile_pre, q_lo, in, slot, lds_raw, blockIdx.x);
 }
 
 // ------------------------------------------------------- int128 SUM pass

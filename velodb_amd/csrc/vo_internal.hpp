@@ -73,7 +73,7 @@ struct SortTuning {
     int pipe_merge = 0;        // resident merge blocks with next-tile prefetch
     int wide_block = 0;        // 12-byte sort: 1024-thread blocks, 8 records per thread
     int flip_hybrid_block = 0; // 12-byte sort's hybrid passes: the other block shape (twice / half the threads)
-    int flip_streaming = 0;    // sort passes: the other cache policy than the size rule picks
+    int flip_streaming = 0;    // sort and compaction passes: the other cache policy than the size rule picks
 };
 
 struct Recorder;
