@@ -110,6 +110,21 @@ def test_filter_sum_bit_exact(vctx, n, n_pad):
     assert np.array_equal(op, eop) and np.array_equal(od, eod)
 
 
+@pytest.mark.parametrize("n,n_pad", [(100000, 131072), (1 << 20, 1 << 20)])
+def test_filter_sum_streaming_bit_exact(n, n_pad):
+    """Filter + compaction + int128 SUM with streaming loads and stores in
+    every pass (tuning 0x800 picks them below the size that would): count,
+    sum and rows equal the oracle's."""
+    import torch
+    import velodb_amd
+    print(f"filter streaming n={n}", flush=True)
+    with velodb_amd.Context(stream=torch.cuda.current_stream().cuda_stream, tuning=0x800) as ctx:
+        (day, qty, price, disc), m, s, op, od = _filter_case(ctx, n, n_pad, 9)
+    em, es, eop, eod = oracle.filter_sum(day, qty, price, disc, n, n_pad, want_rows=True)
+    assert (m, s) == (em, es)
+    assert np.array_equal(op, eop) and np.array_equal(od, eod)
+
+
 def test_filter_sum_config2_6M_rows(vctx):
     """Config: 6,000,000 fact rows padded to 2^23, Q6-style predicate,
     stable oblivious compaction, exact int128 SUM(price*disc)."""

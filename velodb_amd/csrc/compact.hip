@@ -397,8 +397,13 @@ void filter_sum_with(vo_ctx *ctx, const vo_filter_args &a, const Workspace &ws)
     const bool in_place_copy = a.out_price != nullptr;
     const uint32_t nt = (uint32_t)(a.n_pad / CT);
     Pred pr{a.day_lo, a.day_hi, a.disc_lo, a.disc_hi, a.qty_lt};
-    launch(ctx, LaunchInfo{"filter_count", dim3(nt), dim3(CBLOCK), 0, a.n_pad}, k_filter_count, a.day, a.qty, a.disc,
-           a.n, pr, ws.mask, ws.tile_cnt, trace_slot(ctx));
+    // the count pass reads 20 B per row, the sum pass 16: streaming loads
+    // beyond the MALL, as in the network passes
+    const bool stream = !ctx->rec && !ctx->tracing &&
+                        ((a.n_pad * 16 > (256ull << 20)) != (ctx->tuning.flip_streaming != 0));
+    launch(ctx, LaunchInfo{"filter_count", dim3(nt), dim3(CBLOCK), 0, a.n_pad},
+           stream ? k_filter_count<true> : k_filter_count<false>, a.day, a.qty, a.disc, a.n, pr, ws.mask,
+           ws.tile_cnt, trace_slot(ctx));
     launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)ws.tile_cnt, nt,
            ws.tile_pre, ws.count, trace_slot(ctx));
     // the network's first pass reads the caller's price/disc and writes the
@@ -410,7 +415,8 @@ void filter_sum_with(vo_ctx *ctx, const vo_filter_args &a, const Workspace &ws)
     run_network<2>(ctx, in, cs, ws.mask, a.n, a.n_pad, ws.tile_pre);
     // at least 8 rows per thread, at most kSumBlocks partial sums
     const uint32_t sb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kSumBlocks, a.n_pad / 2048));
-    launch(ctx, LaunchInfo{"filter_sum", dim3(sb), dim3(256), 0, a.n_pad}, k_sum_prod_i128, (const int64_t *)op,
+    launch(ctx, LaunchInfo{"filter_sum", dim3(sb), dim3(256), 0, a.n_pad},
+           stream ? k_sum_prod_i128<true> : k_sum_prod_i128<false>, (const int64_t *)op,
            (const int64_t *)od, a.n_pad, (const uint64_t *)ws.count, ws.partial, trace_slot(ctx));
     launch(ctx, LaunchInfo{"filter_sum_final", dim3(1), dim3(256), 0, 0}, k_sum_final, (const __int128 *)ws.partial,
            sb, (const uint64_t *)ws.count, a.out, trace_slot(ctx));

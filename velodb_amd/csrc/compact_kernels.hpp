@@ -198,6 +198,7 @@ struct Pred {
     int64_t disc_lo, disc_hi, qty_lt;
 };
 
+template <bool NT = false>
 __device__ __forceinline__ void b_filter_count(const int32_t *__restrict__ day, const int64_t *__restrict__ qty,
                                                const int64_t *__restrict__ disc, uint64_t n, Pred pr,
                                                uint8_t *__restrict__ mask, uint32_t *__restrict__ tile_cnt,
@@ -210,11 +211,11 @@ __device__ __forceinline__ void b_filter_count(const int32_t *__restrict__ day, 
 #pragma unroll 4
     for (int k = 0; k < CT / CBLOCK; ++k) {
         const uint64_t i = t0 + k * CBLOCK + threadIdx.x;
-        const int32_t d = day[i];
-        const int64_t q = qty[i], ds = disc[i];
+        const int32_t d = ld_row<NT>(day + i);
+        const int64_t q = ld_row<NT>(qty + i), ds = ld_row<NT>(disc + i);
         const uint32_t m = (d >= pr.day_lo) & (d < pr.day_hi) & (ds >= pr.disc_lo) & (ds <= pr.disc_hi) &
                            (q < pr.qty_lt) & (i < n);
-        mask[i] = (uint8_t)m;
+        st_row<NT>(mask + i, (uint8_t)m);
         c += m;
         tr.at(3, i); // day, qty, disc read and mask written at row i
     }
@@ -230,13 +231,14 @@ __device__ __forceinline__ void b_filter_count(const int32_t *__restrict__ day, 
     tr.done(dev::gtid());
 }
 
+template <bool NT = false>
 __global__ __launch_bounds__(CBLOCK) void k_filter_count(const int32_t *__restrict__ day,
                                                          const int64_t *__restrict__ qty,
                                                          const int64_t *__restrict__ disc, uint64_t n, Pred pr,
                                                          uint8_t *__restrict__ mask, uint32_t *__restrict__ tile_cnt,
                                                          TraceSlot *slot)
 {
-    b_filter_count(day, qty, disc, n, pr, mask, tile_cnt, slot, blockIdx.x);
+    b_filter_count<NT>(day, qty, disc, n, pr, mask, tile_cnt, slot, blockIdx.x);
 }
 
 // Exclusive scan of the tile counts by one block of NT threads:
@@ -651,6 +653,7 @@ __global__ __launch_bounds__(CBLOCK) void k_compact_hybrid(Cols<NW> cols, const 
 // Exact SUM(a[i] * b[i]) over rows i < count (count read on the device);
 // every row is read, rows >= count contribute a multiplied-out zero.
 // red: 256 int128 of LDS.
+template <bool NT = false>
 __device__ __forceinline__ void b_sum_prod_i128(const int64_t *__restrict__ a, const int64_t *__restrict__ b,
                                                 uint64_t n_pad, const uint64_t *__restrict__ count,
                                                 __int128 *__restrict__ partial, TraceSlot *slot, __int128 *red,
@@ -660,7 +663,7 @@ __device__ __forceinline__ void b_sum_prod_i128(const int64_t *__restrict__ a, c
     const uint64_t m = *count;
     __int128 acc = 0;
     for (uint64_t i = (uint64_t)vb * 256 + threadIdx.x; i < n_pad; i += (uint64_t)nvb * 256) {
-        const __int128 prod = (__int128)a[i] * (__int128)b[i];
+        const __int128 prod = (__int128)ld_row<NT>(a + i) * (__int128)ld_row<NT>(b + i);
         acc += prod * (__int128)(i < m);
         tr.at(8, i);
     }
@@ -677,12 +680,13 @@ __device__ __forceinline__ void b_sum_prod_i128(const int64_t *__restrict__ a, c
     tr.done(dev::gtid());
 }
 
+template <bool NT = false>
 __global__ __launch_bounds__(256) void k_sum_prod_i128(const int64_t *__restrict__ a, const int64_t *__restrict__ b,
                                                        uint64_t n_pad, const uint64_t *__restrict__ count,
                                                        __int128 *__restrict__ partial, TraceSlot *slot)
 {
     __shared__ __int128 red[256];
-    b_sum_prod_i128(a, b, n_pad, count, partial, slot, red, blockIdx.x, gridDim.x);
+    b_sum_prod_i128<NT>(a, b, n_pad, count, partial, slot, red, blockIdx.x, gridDim.x);
 }
 
 // One 256-thread block: strided partial sums, then a tree in LDS (exact:

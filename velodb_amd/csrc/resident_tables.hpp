@@ -111,10 +111,11 @@
 #define VO_COMPACT_BODIES(X)                                                                                       \
     VO_COMPACT_NW(X, 1) VO_COMPACT_NW(X, 2) VO_COMPACT_G(X, 1, 5)                                                  \
     X(kB_count_mask, (compact_k::k_count_mask), (compact_k::b_count_mask(a..., vb)), 0)                           \
-    X(kB_filter_count, (compact_k::k_filter_count), (compact_k::b_filter_count(a..., vb)), 0)                     \
+    X(kB_filter_count, (compact_k::k_filter_count<false>), (compact_k::b_filter_count<false>(a..., vb)), 0)       \
     X(kB_scan_tiles, (compact_k::k_scan_tiles), (compact_k::b_scan_tiles<256>(a..., (uint64_t *)lds)), 2048)      \
     X(kB_scan_seg, (compact_k::k_scan_seg), (compact_k::b_scan_seg(a..., (uint64_t *)lds, vb)), 2048)             \
-    X(kB_sum_prod, (compact_k::k_sum_prod_i128), (compact_k::b_sum_prod_i128(a..., (__int128 *)lds, vb, nvb)),    \
+    X(kB_sum_prod, (compact_k::k_sum_prod_i128<false>),                                                            \
+      (compact_k::b_sum_prod_i128<false>(a..., (__int128 *)lds, vb, nvb)),                                         \
       4096)                                                                                                        \
     X(kB_sum_final, (compact_k::k_sum_final), (compact_k::b_sum_final(a..., (__int128 *)lds)), 4096)              \
     X(kB_scatter_positions, (compact_k::k_scatter_positions), (compact_k::b_scatter_positions(a..., vb)), 0)
