@@ -70,6 +70,28 @@ def test_compact_streaming_bit_exact(widths, n, n_pad, p):
     assert np.array_equal(got, expect)
 
 
+@pytest.mark.parametrize("n_pad,p", [(1 << 26, 0.3), (1 << 27, 0.02)])
+def test_compact_many_tiles_scan(vctx, n_pad, p):
+    """Above 8192 tiles (n_pad > 2^25) the tile counts are scanned in three
+    passes (segment sums, their scan, segment prefixes) instead of one
+    block: the count and the compacted column equal the mask-ordered
+    selection (a stable compaction's closed form), with a live-row bound
+    that cuts a segment."""
+    print(f"compact many tiles n_pad={n_pad}", flush=True)
+    rng = np.random.default_rng(n_pad + 3)
+    n = n_pad - 12345
+    vals = rng.integers(0, 2 ** 63, n_pad, dtype=np.uint64)
+    mask = (rng.random(n_pad) < p).astype(np.uint8)
+    sel = mask.astype(bool)
+    sel[n:] = False
+    dv, dm = to_dev(vals), to_dev(mask)
+    dcount = empty_dev(1, np.uint64)
+    vctx.compact(dm, [dv], [8], n, n_pad, out_count=dcount)
+    m = int(sel.sum())
+    assert int(from_dev(dcount, np.uint64)[0]) == m
+    assert np.array_equal(from_dev(dv, np.uint64)[:m], vals[sel])
+
+
 @pytest.mark.parametrize("mask_kind", ["all", "none", "alternate", "last"])
 def test_compact_edge_masks(vctx, mask_kind):
     n_pad = 1 << 16

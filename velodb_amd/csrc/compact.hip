@@ -63,6 +63,30 @@ Workspace workspace(vo_ctx *ctx, uint64_t n_pad, bool want_mask)
     return ws;
 }
 
+// Exclusive scan of nt tile counts into pre[0..nt] (pre[nt] and *out_count:
+// the total).  One 1024-thread block up to kScanSeg tiles and in resident
+// programs; above, in launch mode, the three-pass scan (one block's
+// latency-bound walk took 201 us over the 131072 tiles of 2^29 rows).  tmp:
+// 2 KB of device memory for the segment sums and bases.
+void scan_tiles(vo_ctx *ctx, const uint32_t *cnt, uint32_t nt, uint64_t *pre, uint64_t *out_count, void *tmp)
+{
+    if (nt <= kScanSeg || ctx->rec) {
+        launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, cnt, nt, pre, out_count,
+               trace_slot(ctx));
+        return;
+    }
+    const uint32_t nb = (nt + kScanSeg - 1) / kScanSeg;
+    VO_CHECK(nb <= 128, VO_ERR_INVALID, "tile scan above 2^32 rows");
+    uint32_t *seg_sum = (uint32_t *)tmp;
+    uint64_t *seg_base = (uint64_t *)((char *)tmp + 1024);
+    launch(ctx, LaunchInfo{"compact_scan_part", dim3(nb), dim3(1024), 0, nt}, k_scan_part, cnt, nt, seg_sum,
+           trace_slot(ctx));
+    launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nb}, k_scan_tiles, (const uint32_t *)seg_sum, nb,
+           seg_base, out_count, trace_slot(ctx));
+    launch(ctx, LaunchInfo{"compact_scan_fin", dim3(nb), dim3(1024), 0, nt}, k_scan_fin, cnt, nt,
+           (const uint64_t *)seg_base, pre, trace_slot(ctx));
+}
+
 // The network's passes with one tracing mode (TR) and cache policy (NT).
 template <int NW, bool TR, bool NT>
 void run_network_t(vo_ctx *ctx, Cols<NW> src, Cols<NW> cols, const uint8_t *mask, uint64_t n,
@@ -152,8 +176,8 @@ void count_and_scan(vo_ctx *ctx, const uint8_t *mask, uint64_t n, uint64_t n_pad
     const uint32_t nt = (uint32_t)(n_pad / CT);
     launch(ctx, LaunchInfo{"compact_count", dim3(nt), dim3(CBLOCK), 0, n_pad}, k_count_mask, mask, n, ws.tile_cnt,
            Inst{nullptr, nullptr, 0}, trace_slot(ctx));
-    launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)ws.tile_cnt,
-           nt, ws.tile_pre, out_count ? out_count : ws.count, trace_slot(ctx));
+    // the sum partials' region is free until a sum pass: the scan's temporaries
+    scan_tiles(ctx, ws.tile_cnt, nt, ws.tile_pre, out_count ? out_count : ws.count, ws.partial);
 }
 
 void check_pad(uint64_t n, uint64_t n_pad)
@@ -404,8 +428,7 @@ void filter_sum_with(vo_ctx *ctx, const vo_filter_args &a, const Workspace &ws)
     launch(ctx, LaunchInfo{"filter_count", dim3(nt), dim3(CBLOCK), 0, a.n_pad},
            stream ? k_filter_count<true> : k_filter_count<false>, a.day, a.qty, a.disc, a.n, pr, ws.mask,
            ws.tile_cnt, trace_slot(ctx));
-    launch(ctx, LaunchInfo{"compact_scan", dim3(1), dim3(1024), 0, nt}, k_scan_tiles, (const uint32_t *)ws.tile_cnt, nt,
-           ws.tile_pre, ws.count, trace_slot(ctx));
+    scan_tiles(ctx, ws.tile_cnt, nt, ws.tile_pre, ws.count, ws.partial); // partials are written after the scan
     // the network's first pass reads the caller's price/disc and writes the
     // output columns (or the inputs in place when no outputs are given)
     int64_t *op = in_place_copy ? a.out_price : const_cast<int64_t *>(a.price);

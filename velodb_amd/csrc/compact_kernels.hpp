@@ -310,6 +310,95 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *__restrict_
     b_scan_tiles<1024>(cnt, nt, pre, out_count, slot, part);
 }
 
+// Tile scans of many tiles (launch mode): segments of kScanSeg counts, one
+// 1024-thread block each, kScanPer contiguous counts per thread.
+// k_scan_part writes each segment's sum (<= 2^25, so 32 bits), k_scan_tiles
+// scans those sums into segment bases, and k_scan_fin writes every tile's
+// exclusive prefix from its segment's base.  The sums and the prefixes are
+// those of the one-block scan: integer addition in any grouping.
+constexpr uint32_t kScanPer = 8;
+constexpr uint32_t kScanSeg = 1024 * kScanPer;
+
+// Exclusive prefix of v over a 1024-thread block and the block's total;
+// sh: 16 words of LDS.
+__device__ __forceinline__ uint64_t block_scan1024(uint64_t v, uint64_t *sh, uint64_t &total)
+{
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t u = __shfl_up(inc, off, 64);
+        inc += lane >= (uint32_t)off ? u : 0;
+    }
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    uint64_t before = 0, tot = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) {
+        const uint64_t x = sh[k];
+        before += k < w ? x : 0;
+        tot += x;
+    }
+    total = tot;
+    return before + inc - v;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_part(const uint32_t *__restrict__ cnt, uint32_t nt,
+                                                    uint32_t *__restrict__ seg_sum, TraceSlot *slot)
+{
+    __shared__ uint64_t sh[16];
+    dev::Atr tr(slot);
+    const uint64_t lo = (uint64_t)blockIdx.x * kScanSeg + (uint64_t)threadIdx.x * kScanPer;
+    uint32_t v[kScanPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPer; ++j) v[j] = lo + j < nt ? cnt[lo + j] : 0u;
+    uint64_t s = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPer; ++j) {
+        s += v[j];
+        if (lo + j < nt) tr.at(4, lo + j);
+    }
+    uint64_t total;
+    (void)block_scan1024(s, sh, total);
+    if (threadIdx.x == 0) {
+        seg_sum[blockIdx.x] = (uint32_t)total;
+        tr.at(7, blockIdx.x);
+    }
+    tr.done(dev::gtid());
+}
+
+__global__ __launch_bounds__(1024) void k_scan_fin(const uint32_t *__restrict__ cnt, uint32_t nt,
+                                                   const uint64_t *__restrict__ seg_base, uint64_t *__restrict__ pre,
+                                                   TraceSlot *slot)
+{
+    __shared__ uint64_t sh[16];
+    dev::Atr tr(slot);
+    const uint64_t lo = (uint64_t)blockIdx.x * kScanSeg + (uint64_t)threadIdx.x * kScanPer;
+    uint32_t v[kScanPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPer; ++j) v[j] = lo + j < nt ? cnt[lo + j] : 0u;
+    uint64_t s = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPer; ++j) s += v[j];
+    uint64_t total;
+    uint64_t run = seg_base[blockIdx.x] + block_scan1024(s, sh, total);
+    tr.at(7, blockIdx.x);
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPer; ++j) {
+        if (lo + j < nt) {
+            pre[lo + j] = run;
+            tr.at(4, lo + j);
+            tr.at(5, lo + j);
+        }
+        run += v[j];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pre[nt] = seg_base[gridDim.x]; // the total
+        tr.at(6, nt);
+    }
+    tr.done(dev::gtid());
+}
+
 // The tile scans of every instance of a batch class, instance vb by block vb:
 // its nts tiles' exclusive prefix counts go to pre[vb * nts ..] (each
 // instance restarts at zero, so each compacts to its own front) and its
