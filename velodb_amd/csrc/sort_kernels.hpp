@@ -559,8 +559,11 @@ struct SortImpl {
             // tuning may flip the shape (both are built when the wide one is)
             const bool wide = (S <= HW_MAX) != (HW_MAX > 0 && ctx->tuning.flip_hybrid_block);
             if (wide) {
+                // the 6-stage pass on 1024-thread blocks measured faster with
+                // the default policy even at 2^28 (5.53 vs 5.13 TB/s)
+                constexpr int CPS = S == 6 ? 0 : CP;
                 if constexpr (HW_MAX > 0)
-                    hybrid_shape<TRACE, S, R_LOG - 1, 2 * BLOCK, CP>(ctx, b, map, p, q_lo, p_top, n_virtual);
+                    hybrid_shape<TRACE, S, R_LOG - 1, 2 * BLOCK, CPS>(ctx, b, map, p, q_lo, p_top, n_virtual);
             } else {
                 hybrid_shape<TRACE, S, R_LOG, BLOCK, CP>(ctx, b, map, p, q_lo, p_top, n_virtual);
             }
