@@ -46,7 +46,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * and key (seed_lo, seed_hi). */
 uint64_t orc_philox_tag(uint64_t seed, uint64_t i);
 
-/* Config-1 sort input: key[i] = orc_rand64(seed, 0, i), payload[i] = i. */
+/* Sort input (u64 keys, u32 row-index payloads): key[i] = orc_rand64(seed, 0, i), payload[i] = i. */
 void orc_gen_sort_input(uint64_t seed, size_t n, uint64_t *keys, uint32_t *payload);
 
 /* ------------------------------------------------------------------------ */
@@ -130,7 +130,7 @@ void orc_compact(void *buf, size_t elem_size, const uint8_t *selected, size_t n)
 void orc_compact_levels(void *buf, size_t elem_size, const uint8_t *selected, size_t n);
 
 /* ------------------------------------------------------------------------ */
-/* Filter + aggregate (config: Q6-style predicate, exact int128 SUM)         */
+/* Filter + aggregate (Q6-style predicate, exact int128 SUM)                 */
 /* ------------------------------------------------------------------------ */
 
 typedef struct orc_filter_spec {

@@ -87,8 +87,8 @@ def test_autotuner_choices_are_valid_and_used(vctx):
             assert np.array_equal(gk[off:off + n_pad], ek) and np.array_equal(gp[off:off + n_pad], ep)
 
 
-def test_batch_config4_full_size_equals_per_instance(vctx):
-    """Config 4 at full size: 4096 sorts, sizes log-uniform in [2^10, 2^20]
+def test_batch_4096_sorts_2p10_to_2p20_equals_per_instance(vctx):
+    """At full size: 4096 sorts, sizes log-uniform in [2^10, 2^20]
     (numpy default_rng(0)), 1.27e9 buffer rows: the batched schedule gives
     the same bytes as one sort per instance (both against device-generated
     inputs), and every instance comes out sorted."""

@@ -147,7 +147,7 @@ def test_filter_sum_streaming_bit_exact(n, n_pad):
     assert np.array_equal(op, eop) and np.array_equal(od, eod)
 
 
-def test_filter_sum_config2_6M_rows(vctx):
+def test_filter_sum_6M_rows_2p23(vctx):
     """Config: 6,000,000 fact rows padded to 2^23, Q6-style predicate,
     stable oblivious compaction, exact int128 SUM(price*disc)."""
     n, n_pad = 6_000_000, 1 << 23
@@ -200,7 +200,7 @@ def test_scatter_positions_stable(vctx):
     del ctypes
 
 
-def test_filter_sum_config2_3e8_rows_2p29(vctx):
+def test_filter_sum_3e8_rows_2p29(vctx):
     """Config: 3.0e8 fact rows padded to 2^29 (the roofline size), generated
     on the device by the generator the oracle pins (orc_gen_fact).  The
     selected count, the exact int128 SUM(price*disc) and the compacted
@@ -210,7 +210,7 @@ def test_filter_sum_config2_3e8_rows_2p29(vctx):
     1.05e9 and the total below 2^63, so int64 torch arithmetic is exact."""
     import torch
     n, n_pad = 300_000_000, 1 << 29
-    print(f"filter_sum config2 n={n} n_pad={n_pad}", flush=True)
+    print(f"filter_sum n={n} n_pad={n_pad}", flush=True)
     day = empty_dev(n_pad, np.int32)
     qty, price, disc = (empty_dev(n_pad, np.int64) for _ in range(3))
     vctx.gen_fact(11, day, qty, price, disc, n)

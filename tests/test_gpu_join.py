@@ -37,7 +37,7 @@ def test_join_groupsum_bit_exact(vctx, n_dim, n_fact, n_pad, dense):
 
 
 @pytest.mark.parametrize("dense", [False, True])
-def test_join_config3_full_size(vctx, dense):
+def test_join_1p5M_dim_6M_fact(vctx, dense):
     """Config: 1.5M dimension rows (unique keys 1..1.5M, 25 groups), 6M fact
     rows, union padded to 2^23; 25 exact int64 sums."""
     n_dim, n_fact, n_pad = 1_500_000, 6_000_000, 1 << 23
@@ -127,7 +127,7 @@ def test_join_compact_union_bit_exact(vctx, n_dim, n_fact, n_pad):
     assert np.array_equal(g, eg) and np.array_equal(s, es)
 
 
-def test_join_compact_union_config3_full_size(vctx):
+def test_join_compact_union_1p5M_dim_6M_fact(vctx):
     """1.5M dimension rows, 6M fact rows, union padded to 2^23, 25 groups:
     the closed-form sums."""
     n_dim, n_fact, n_pad = 1_500_000, 6_000_000, 1 << 23

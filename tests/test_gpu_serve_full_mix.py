@@ -80,13 +80,13 @@ def _run_checked(srv, S, qs, want, rows, **kw):
     return res, st
 
 
-def test_served_mix_at_config5_shape_matches_the_oracle():
+def test_served_mix_tables_to_1e7_at_800qps_matches_the_oracle():
     import velodb_amd
     S = velodb_amd.Server
     srv = S(n_slots=16, hbm_budget_bytes=200 << 30)
     try:
         tables, sizes = _tables(srv, S)
-        print(f"config5 tables {sizes}", flush=True)
+        print(f"served mix tables {sizes}", flush=True)
         qs = _mix(S, tables, 11)
         used = {(k, t) for k, t, _ in qs}
         want = {kt: _expected(srv, S, *kt) for kt in used}
@@ -94,12 +94,12 @@ def test_served_mix_at_config5_shape_matches_the_oracle():
         for k, t in used:
             col = 2 if k == S.JOIN_GROUPSUM else 0
             rows[(k, t)] = len(srv.table_column(k, t, col))
-        print("config5 served mix", flush=True)
+        print("served mix: 16 slots", flush=True)
         res, st = _run_checked(srv, S, qs, want, rows)
         assert st.max_concurrent > 1 and st.peak_bytes_in_flight <= 200 << 30
-        print("config5 plain loop", flush=True)
+        print("served mix: plain loop", flush=True)
         _run_checked(srv, S, qs, want, rows, n_slots=1, transfer=True)
-        print("config5 through the resident command queue", flush=True)
+        print("served mix: resident command queue", flush=True)
         _, stq = _run_checked(srv, S, qs, want, rows, queue=True)
         assert stq.max_concurrent > 1
         # the timed form (no output copies) returns the same result words

@@ -30,11 +30,11 @@ def test_sort_kp_bit_exact(vctx, n, npad):
     assert np.array_equal(gp, ep)
 
 
-def test_sort_config1_2p20_two_seeds(vctx):
+def test_sort_2p20_two_seeds(vctx):
     """Config: 2^20 records, u64 uniform keys, payload = row index, seeds 1 and 2."""
     n = 1 << 20
     for seed in (1, 2):
-        print(f"config1 seed={seed}", flush=True)
+        print(f"sort 2^20 seed={seed}", flush=True)
         k, p = oracle.gen_sort_input(seed, n)
         gk, gp = _run_sort(vctx, k, p, n, n)
         ek, ep = oracle.bitonic_sort_kp(k, p, n, n)
