@@ -696,13 +696,14 @@ __device__ __forceinline__ void b_fill_agg(const uint64_t *__restrict__ keys, co
         c.key = d ? k32 : c.key;
         c.group = d ? v : c.group;
         const bool matched = fact & (c.key == k32);
-        const uint64_t grp = c.group & 0xFFFFFFFFull;
+        const uint32_t grp = (uint32_t)c.group;
+        // the row's value where it counts, then one 32-bit compare and a
+        // select per group; the presence bit by one shift (no address
+        // depends on the group)
+        const int64_t vm = matched ? (int64_t)v : 0;
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const bool hit = matched & (grp == (uint64_t)g);
-            acc[g] += hit ? (int64_t)v : 0;
-            bits |= hit ? (1ull << g) : 0ull;
-        }
+        for (int g = 0; g < G; ++g) acc[g] += grp == (uint32_t)g ? vm : 0;
+        bits |= (matched & (grp < (uint32_t)G)) ? (1ull << (grp & 63u)) : 0ull;
         tr.at(5, i);
     }
 #pragma unroll
