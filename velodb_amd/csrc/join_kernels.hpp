@@ -779,11 +779,13 @@ __device__ __forceinline__ void b_dense_out(const int64_t *__restrict__ part_sum
     }
     __syncthreads();
     uint64_t present = 0;
+#pragma unroll 8
     for (int g = 0; g < G; ++g) present += tc[g] > 0;
     for (uint64_t j = t; j < max_groups; j += NT) {
         int32_t og = -1;
         int64_t os = 0;
         uint64_t rank = 0;
+#pragma unroll 8
         for (int g = 0; g < G; ++g) {
             const bool here = (tc[g] > 0) & (rank == j);
             og = here ? g : og;
