@@ -6,8 +6,8 @@ tests/query/test_order_by.cpp plus join and queue checks.
 The binary starts resident queues (cooperative launches).  On this runtime a
 process cannot make a cooperative launch on a GPU once another process has
 made one there, so this module must run before any test in the same pytest
-process that starts a queue; file order (test_gpu_cpp... before
-test_gpu_queue/test_gpu_resident/test_gpu_serve) gives that."""
+process that starts a queue: tests/conftest.py puts its tests first in every
+selection."""
 import os
 import subprocess
 

@@ -147,7 +147,7 @@ template <bool TRACE, bool KP>
 __device__ __forceinline__ void b_fill_tiles(const uint64_t *__restrict__ keys, const void *__restrict__ vals,
                                                    Fill *__restrict__ tile_last, TraceSlot *slot, uint32_t vb, uint32_t nvb)
 {
-    __shared__ uint64_t wk[JB], wg[JB];
+    __shared__ uint64_t wk[JB / 64], wg[JB / 64];
     dev::AtrT<TRACE> tr(slot);
     const uint64_t t0 = (uint64_t)vb * JT + (uint64_t)threadIdx.x * JPER;
     uint64_t lk = PAD_KEY, lg = 0;
@@ -160,15 +160,25 @@ __device__ __forceinline__ void b_fill_tiles(const uint64_t *__restrict__ keys, 
         lk = d ? k32 : lk;
         lg = d ? v : lg;
     }
-    wk[threadIdx.x] = lk;
-    wg[threadIdx.x] = lg;
+    // each wavefront's last lane that holds a dimension row (a register
+    // read, no address depends on it), then the block's last wavefront
+    const uint64_t has = __builtin_amdgcn_ballot_w64(lk != PAD_KEY);
+    const int src = has ? 63 - __builtin_clzll(has) : 0;
+    const uint64_t rk = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(lk >> 32), src) << 32) |
+                        (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)lk, src);
+    const uint64_t rg = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(lg >> 32), src) << 32) |
+                        (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)lg, src);
+    if ((threadIdx.x & 63) == 0) {
+        wk[threadIdx.x >> 6] = has ? rk : PAD_KEY;
+        wg[threadIdx.x >> 6] = has ? rg : 0;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t k = PAD_KEY, g = 0;
-        for (int t = 0; t < JB; ++t) {
-            const bool has = wk[t] != PAD_KEY;
-            k = has ? wk[t] : k;
-            g = has ? wg[t] : g;
+        for (int w = 0; w < JB / 64; ++w) {
+            const bool h = wk[w] != PAD_KEY;
+            k = h ? wk[w] : k;
+            g = h ? wg[w] : g;
         }
         tile_last[vb] = Fill{k, g};
         tr.at(2, vb);
