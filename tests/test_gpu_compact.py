@@ -148,7 +148,7 @@ def test_filter_sum_streaming_bit_exact(n, n_pad):
 
 
 def test_filter_sum_6M_rows_2p23(vctx):
-    """Config: 6,000,000 fact rows padded to 2^23, Q6-style predicate,
+    """6,000,000 fact rows padded to 2^23, Q6-style predicate,
     stable oblivious compaction, exact int128 SUM(price*disc)."""
     n, n_pad = 6_000_000, 1 << 23
     (day, qty, price, disc), m, s, op, od = _filter_case(vctx, n, n_pad, 1)
@@ -201,7 +201,7 @@ def test_scatter_positions_stable(vctx):
 
 
 def test_filter_sum_3e8_rows_2p29(vctx):
-    """Config: 3.0e8 fact rows padded to 2^29 (the roofline size), generated
+    """3.0e8 fact rows padded to 2^29 (the roofline size), generated
     on the device by the generator the oracle pins (orc_gen_fact).  The
     selected count, the exact int128 SUM(price*disc) and the compacted
     (price, disc) prefix equal their closed forms computed from the same

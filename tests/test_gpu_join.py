@@ -38,7 +38,7 @@ def test_join_groupsum_bit_exact(vctx, n_dim, n_fact, n_pad, dense):
 
 @pytest.mark.parametrize("dense", [False, True])
 def test_join_1p5M_dim_6M_fact(vctx, dense):
-    """Config: 1.5M dimension rows (unique keys 1..1.5M, 25 groups), 6M fact
+    """1.5M dimension rows (unique keys 1..1.5M, 25 groups), 6M fact
     rows, union padded to 2^23; 25 exact int64 sums."""
     n_dim, n_fact, n_pad = 1_500_000, 6_000_000, 1 << 23
     dk, dg, fk, fv = oracle.gen_join(1, n_dim, n_fact)

@@ -31,7 +31,7 @@ def test_sort_kp_bit_exact(vctx, n, npad):
 
 
 def test_sort_2p20_two_seeds(vctx):
-    """Config: 2^20 records, u64 uniform keys, payload = row index, seeds 1 and 2."""
+    """2^20 records, u64 uniform keys, payload = row index, seeds 1 and 2."""
     n = 1 << 20
     for seed in (1, 2):
         print(f"sort 2^20 seed={seed}", flush=True)
