@@ -168,10 +168,49 @@ void dispatch(vo_ctx *ctx, const vo_command &cmd);
 
 namespace vo {
 
+// Launch mode: piece i runs on the context stream (i mod 4 == 0) or on
+// helper stream i mod 4 - 1, forked from the context stream before the first
+// piece and joined back into it after the last, so that one piece's small or
+// latency-bound launches overlap another's.  The pieces share no memory they
+// write, and none of them allocates (the caller's scratch is in place).
+static void run_tracks_launch(vo_ctx *ctx, size_t n, const std::function<void(size_t)> &f)
+{
+    constexpr size_t kHelpers = 3;
+    const size_t used = std::min(kHelpers, n - 1);
+    for (size_t h = 0; h < used; ++h)
+        if (!ctx->track_streams[h]) VO_HIP(hipStreamCreateWithFlags(&ctx->track_streams[h], hipStreamNonBlocking));
+    for (size_t e = 0; e <= used; ++e)
+        if (!ctx->track_events[e]) VO_HIP(hipEventCreateWithFlags(&ctx->track_events[e], hipEventDisableTiming));
+    hipStream_t const main = ctx->stream;
+    VO_HIP(hipEventRecord(ctx->track_events[0], main));
+    for (size_t h = 0; h < used; ++h) VO_HIP(hipStreamWaitEvent(ctx->track_streams[h], ctx->track_events[0], 0));
+    auto join = [&] {
+        ctx->stream = main;
+        for (size_t h = 0; h < used; ++h) {
+            (void)hipEventRecord(ctx->track_events[h + 1], ctx->track_streams[h]);
+            (void)hipStreamWaitEvent(main, ctx->track_events[h + 1], 0);
+        }
+    };
+    try {
+        for (size_t i = 0; i < n; ++i) {
+            ctx->stream = i % (kHelpers + 1) == 0 ? main : ctx->track_streams[i % (kHelpers + 1) - 1];
+            f(i);
+        }
+    } catch (...) {
+        join();
+        throw;
+    }
+    join();
+}
+
 void run_tracks(vo_ctx *ctx, size_t n, const std::function<void(size_t)> &f)
 {
     if (!ctx->rec) {
-        for (size_t i = 0; i < n; ++i) f(i);
+        if (n < 2) {
+            for (size_t i = 0; i < n; ++i) f(i);
+            return;
+        }
+        run_tracks_launch(ctx, n, f);
         return;
     }
     Recorder *const parent = ctx->rec;
@@ -482,6 +521,10 @@ int vo_ctx_destroy(vo_ctx *ctx)
     join_pairs_state_free(ctx);
     staging_free(ctx);
     if (ctx->trace_slots) (void)hipFree(ctx->trace_slots);
+    for (auto &s : ctx->track_streams)
+        if (s) (void)hipStreamDestroy(s);
+    for (auto &e : ctx->track_events)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return VO_OK;

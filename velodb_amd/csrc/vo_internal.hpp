@@ -130,6 +130,11 @@ struct vo_ctx {
 
     // pinned staging ring and copy threads of the host transfers (transfer.hip)
     void *staging = nullptr;
+
+    // helper streams (default priority) and their fork/join events for the
+    // independent pieces of one command in launch mode (run_tracks)
+    hipStream_t track_streams[3] = {};
+    hipEvent_t track_events[4] = {};
 };
 
 namespace vo {
